@@ -1,0 +1,218 @@
+"""Benchmark: assembled NNZ/s of the 3-D Q1 Poisson numeric assembly on one MI355X.
+
+Workload (the single-GPU configuration of the metric): (0,1)^3, 128^3 HEX cells,
+scalar Q1, quadrature degree 2, κ_c = exp(z_c), z ~ N(0,1) i.i.d. (seeded),
+f = 1, homogeneous Dirichlet on "boundary"; element math FP32, nzval/b
+accumulated FP64.  One step = one numeric assembly of A (CSC nzval, 54,439,939
+entries) and b, inputs resident in HBM (assemble_matrix_and_vector! on the fixed
+pattern).  The pattern is built once beforehand (reported separately as the
+full-assembly time).
+
+Prints one JSON line.  `roofline` uses the algorithmic bytes per launch
+8*NNZ + 32*C + 8*C + 8*N and the kernel's average duration from HIP events on the
+library's stream.  `cpu_baseline` times the CPU oracle's numeric re-assembly of
+the same system (1 thread).
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "gridap.jl_amd"))
+
+HBM_PEAK = 8.0e12  # MI355X HBM3E, B/s
+METRIC = "assembled NNZ/s & %HBM roofline; PCG s-to-1e-8 learned-SPAI vs Jacobi, 1 GPU"
+
+
+class HipEvents:
+    """hipEvent timing on an existing stream (the library's)."""
+
+    def __init__(self):
+        self.L = C.CDLL("libamdhip64.so.7")
+        self.L.hipEventCreate.argtypes = [C.POINTER(C.c_void_p)]
+        self.L.hipEventRecord.argtypes = [C.c_void_p, C.c_void_p]
+        self.L.hipEventSynchronize.argtypes = [C.c_void_p]
+        self.L.hipEventElapsedTime.argtypes = [C.POINTER(C.c_float), C.c_void_p, C.c_void_p]
+        self.L.hipEventDestroy.argtypes = [C.c_void_p]
+
+    def create(self, n):
+        evs = []
+        for _ in range(n):
+            e = C.c_void_p()
+            assert self.L.hipEventCreate(C.byref(e)) == 0
+            evs.append(e)
+        return evs
+
+    def record(self, e, stream):
+        assert self.L.hipEventRecord(e, C.c_void_p(stream)) == 0
+
+    def elapsed_ms(self, e0, e1):
+        assert self.L.hipEventSynchronize(e1) == 0
+        ms = C.c_float()
+        assert self.L.hipEventElapsedTime(C.byref(ms), e0, e1) == 0
+        return ms.value
+
+    def destroy(self, evs):
+        for e in evs:
+            self.L.hipEventDestroy(e)
+
+
+def load_traffic(name):
+    p = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(p):
+        return None
+    try:
+        with open(p) as f:
+            return json.load(f)
+    except Exception:
+        return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--n", type=int, default=128)
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
+    ap.add_argument("--path", default="auto", choices=["auto", "general", "structured"])
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-pcg", action="store_true", help="skip the Jacobi-PCG solve")
+    ap.add_argument("--seed", type=int, default=2024)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: the engine runs on a single MI355X and no "
+              "multi-GPU path is built (asked for --gpus %d)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import gridap_hip as G
+    from gridap_hip import _lib
+
+    n = args.n
+    D = 3
+    t_host = time.time()
+    model = G.CartesianDiscreteModel((0, 1, 0, 1, 0, 1), (n, n, n))
+    reffe = G.ReferenceFE(G.lagrangian, G.Float64, 1)
+    V = G.TestFESpace(model, reffe, dirichlet_tags="boundary")
+    ncells = model.num_cells()
+    N = V.num_free_dofs()
+    rng = np.random.default_rng(args.seed)
+    kappa = np.exp(rng.standard_normal(ncells))
+    q = G.Quadrature(D, 2)
+    shp, grad = reffe.tabulate(D, q.coords)
+    cd = V.get_cell_dof_ids()
+    ptrs = (np.arange(ncells + 1, dtype=np.int64) * cd.shape[1] + 1).astype(np.int32)
+    host_setup_s = time.time() - t_host
+
+    ctx = _lib.Context(0)
+    ctx.mesh_cartesian(model.origin, model.sizes, model.partition)
+    ctx.space_upload(_lib.GH_REFFE_QK, 1, 1, cd.ravel(), ptrs, N, V.num_dirichlet_dofs())
+    ctx.reffe_tabulation(q.weights, shp, grad)
+    t0 = time.time()
+    nnz = ctx.symbolic()
+    symbolic_s = time.time() - t0
+    ctx.coef_upload(kappa, None, None)
+    prec = {"fp32": _lib.GH_PREC_FP32_ELEM, "fp64": _lib.GH_PREC_FP64}[args.precision]
+    path = {"auto": _lib.GH_PATH_AUTO, "general": _lib.GH_PATH_GENERAL,
+            "structured": _lib.GH_PATH_STRUCTURED}[args.path]
+    stream = ctx.stream()
+    ev = HipEvents()
+
+    for _ in range(args.warmup):
+        ctx.assemble(_lib.GH_FORM_DIFFUSION, prec, path)
+    ctx.values_get(0, 0)  # waits for the stream
+
+    # Timed region: K back-to-back launches on the library stream, an event before
+    # each, so consecutive event pairs bracket one launch each.
+    evs = ev.create(args.steps + 1)
+    t_wall = time.time()
+    ev.record(evs[0], stream)
+    for k in range(args.steps):
+        ctx.assemble(_lib.GH_FORM_DIFFUSION, prec, path)
+        ev.record(evs[k + 1], stream)
+    total_ms = ev.elapsed_ms(evs[0], evs[-1])
+    wall_s = time.time() - t_wall
+    per_launch = [ev.elapsed_ms(evs[k], evs[k + 1]) for k in range(args.steps)]
+    ev.destroy(evs)
+    ms_per_step = total_ms / args.steps
+    kernel_ms = float(np.mean(per_launch))
+
+    nzval, b = ctx.values_get(nnz, N)
+    assert np.all(np.isfinite(nzval)) and np.all(np.isfinite(b))
+    colptr, rowval = ctx.pattern_get(N, nnz)
+
+    algo_bytes = 8 * nnz + 4 * 8 * ncells + 8 * ncells + 8 * N
+    compulsory_bytes = 8 * nnz + 4 * model.num_nodes() + 8 * ncells + 8 * N + 4 * (N + 1)
+    achieved = algo_bytes / (kernel_ms * 1e-3)
+    traffic = load_traffic("q1_lattice_128cube_traffic.json")
+    out = {
+        "metric": METRIC,
+        "value": nnz / (ms_per_step * 1e-3),
+        "unit": "NNZ/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 element math, f64 nzval/b" if args.precision == "fp32" else "f64",
+        "data": "synthetic (κ=exp(N(0,1)) per cell, seed %d)" % args.seed,
+        "config": {
+            "workload": "3D Q1 Poisson numeric assembly (A and b), %d^3 HEX, degree 2, "
+                        "Dirichlet boundary" % n,
+            "cells": ncells, "free_dofs": N, "nnz": nnz,
+            "path": "lattice owner-computes" if path != _lib.GH_PATH_GENERAL else "general",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved / 1e9,
+            "peak": HBM_PEAK / 1e9,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK,
+            "traffic": None if traffic is None else traffic.get("bytes_per_launch"),
+            "algorithmic_bytes_per_launch": algo_bytes,
+            "kernel_ms": kernel_ms,
+            "kernel_ms_min": float(np.min(per_launch)),
+            "compulsory_bytes_per_launch": compulsory_bytes,
+            "frac_on_compulsory_bytes": compulsory_bytes / (kernel_ms * 1e-3) / HBM_PEAK,
+        },
+        "full_assembly_s": symbolic_s + ms_per_step * 1e-3,
+        "symbolic_s": symbolic_s,
+        "host_setup_s": host_setup_s,
+        "timed_wall_s": wall_s,
+    }
+
+    if not args.no_pcg:
+        x, st = ctx.pcg(b=None, n=N, precond=_lib.GH_PRECOND_JACOBI, rtol=1e-8, maxit=20000,
+                        raise_on_noconv=False)
+        out["pcg_jacobi"] = {"iterations": int(st.iterations), "converged": int(st.converged),
+                             "rel_residual": float(st.rel_residual),
+                             "seconds": float(st.device_ms) * 1e-3}
+
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        t = time.time()
+        nz_cpu, b_cpu = O.numeric_refill_cartesian(D, model.sizes, cd, q.weights, shp, grad,
+                                                   kappa, None, colptr, rowval)
+        cpu_s = time.time() - t
+        out["cpu_baseline"] = {
+            "value": nnz / cpu_s, "unit": "NNZ/s", "cores": 1, "kind": "port",
+            "sample": "the full %d^3 workload once: CPU restatement of Gridap's "
+                      "assemble_matrix_and_vector! numeric loop (FP64 element integration + "
+                      "nz_index insertion), 1 thread, %.1f s" % (n, cpu_s),
+        }
+        out["gpu_vs_cpu_rel_frobenius"] = float(np.linalg.norm(nzval - nz_cpu) /
+                                                np.linalg.norm(nz_cpu))
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,467 @@
+// Numeric phase: element integration + scatter into the CSC nzval and the RHS b,
+// for a(u,v)=∫κ∇v⋅∇u dΩ and l(v)=∫f v dΩ with Dirichlet lifting.
+//
+// Two device paths produce the same matrix:
+//  * lattice ("owner computes"): Cartesian mesh + scalar Q1 with CLagrangian
+//    numbering (checked at symbolic time).  One thread owns one CSC column (one
+//    free node): it gathers the element contributions of its 2^D incident cells
+//    into its 3^D stencil entries in registers, in ascending cell order (the
+//    order of Gridap's cell loop), stages the column in LDS and the workgroup
+//    writes its contiguous nzval range once, coalesced.  No atomics, no
+//    read-modify-write of nzval, deterministic.
+//  * general: one thread per cell computes the element matrix with the
+//    isoparametric (or Cartesian) Jacobian at every quadrature point and adds it
+//    into nzval through a binary search of the column (nz_index,
+//    src/Algebra/SparseMatrixCSC.jl:14-22).  Cells are launched colour by colour
+//    (no two cells of a colour share a free dof), so the scatter is race-free and
+//    deterministic.
+//
+// Element math (src/CellData/CellQuadratures.jl:138-160,
+// src/Fields/FieldsInterfaces.jl:737-760): K_ij = Σ_q κ (J⁻ᵀ∇̂φ_i)⋅(J⁻ᵀ∇̂φ_j) |det J| w_q,
+// b_i = Σ_q f φ_i |det J| w_q, then b -= K u_d on cells with Dirichlet dofs
+// (src/CellData/AttachDirichlet.jl:59-84).
+#include <algorithm>
+#include <climits>
+#include <cmath>
+
+#include "gh_internal.h"
+
+namespace gh {
+namespace {
+
+// ---------------------------------------------------------------------------
+// Lattice (owner-computes) kernel
+// ---------------------------------------------------------------------------
+
+struct LatticeArgs {
+  int nn[3];   // lattice nodes per axis
+  int nc[3];   // cells per axis
+  int64_t nnodes;
+  const int32_t *node_dof;
+  const int32_t *colptr;
+  const double *kappa;  // per cell or nullptr (κ = 1)
+  const double *fsrc;   // per cell or nullptr (f = 1)
+  const double *dvals;  // Dirichlet values or nullptr (homogeneous)
+  double *nzval;
+  double *b;
+  double Kd[64];  // reference-cell stiffness for κ=1, [a*8+l] = K(row a, col l)
+  float Kf[64];
+  double bd[8];   // reference-cell load for f=1
+  float bf[8];
+};
+
+template <int D>
+struct Pow3 {
+  static constexpr int v = 3 * Pow3<D - 1>::v;
+};
+template <>
+struct Pow3<0> {
+  static constexpr int v = 1;
+};
+
+template <int D, int BLOCK, bool F32>
+__global__ __launch_bounds__(BLOCK) void lattice_q1_kernel(const LatticeArgs P) {
+  constexpr int NB = Pow3<D>::v;  // stencil entries per column
+  constexpr int NV = 1 << D;      // nodes per cell = cells per node
+  extern __shared__ double stage[];
+  __shared__ int jmin_s, jmax_s;
+
+  const int tid = threadIdx.x;
+  if (tid == 0) {
+    jmin_s = INT_MAX;
+    jmax_s = -1;
+  }
+  __syncthreads();
+
+  const int64_t n = (int64_t)blockIdx.x * BLOCK + tid;
+  int xi[3] = {0, 0, 0};
+  int d0 = 0;
+  if (n < P.nnodes) {
+    int64_t r = n;
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      xi[k] = (int)(r % P.nn[k]);
+      r /= P.nn[k];
+    }
+    d0 = P.node_dof[n];
+  }
+  if (d0 > 0) {
+    atomicMin(&jmin_s, d0 - 1);
+    atomicMax(&jmax_s, d0 - 1);
+  }
+
+  // Stencil dofs (free > 0, Dirichlet < 0, outside the lattice 0).
+  int nb[NB];
+  unsigned mask = 0;
+#pragma unroll
+  for (int o = 0; o < NB; ++o) {
+    int oo = o;
+    bool in = d0 > 0;
+    int64_t node = 0, stride = 1;
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      int v = xi[k] + (oo % 3) - 1;
+      oo /= 3;
+      in = in && v >= 0 && v < P.nn[k];
+      node += (int64_t)v * stride;
+      stride *= P.nn[k];
+    }
+    nb[o] = in ? P.node_dof[node] : 0;
+    mask |= (unsigned)(nb[o] > 0) << o;
+  }
+
+  double acc[NB];
+#pragma unroll
+  for (int o = 0; o < NB; ++o) acc[o] = 0.0;
+  double bacc = 0.0;
+
+  // Incident cells in ascending cell id: bit d of e selects cell x_d-1+e_d.
+#pragma unroll
+  for (int e = 0; e < NV; ++e) {
+    bool cin = d0 > 0;
+    int64_t cid = 0, cstride = 1;
+    int lb = 0;  // local index of this node in the cell
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      int ek = (e >> k) & 1;
+      int ck = xi[k] - 1 + ek;
+      cin = cin && ck >= 0 && ck < P.nc[k];
+      cid += (int64_t)ck * cstride;
+      cstride *= P.nc[k];
+      lb |= (1 - ek) << k;
+    }
+    if (cin) {
+      const double kap = P.kappa ? P.kappa[cid] : 1.0;
+      const double fc = P.fsrc ? P.fsrc[cid] : 1.0;
+      const float kapf = (float)kap;
+#pragma unroll
+      for (int a = 0; a < NV; ++a) {
+        int o = 0, p3 = 1;
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+          o += (((e >> k) & 1) + ((a >> k) & 1)) * p3;
+          p3 *= 3;
+        }
+        if (F32)
+          acc[o] += (double)(kapf * P.Kf[a * NV + lb]);
+        else
+          acc[o] += kap * P.Kd[a * NV + lb];
+      }
+      double t = F32 ? (double)((float)fc * P.bf[lb]) : fc * P.bd[lb];
+      if (P.dvals) {
+        double lift = 0.0;
+#pragma unroll
+        for (int a = 0; a < NV; ++a) {
+          int o = 0, p3 = 1;
+#pragma unroll
+          for (int k = 0; k < D; ++k) {
+            o += (((e >> k) & 1) + ((a >> k) & 1)) * p3;
+            p3 *= 3;
+          }
+          if (nb[o] < 0) {
+            double kv = F32 ? (double)(kapf * P.Kf[lb * NV + a]) : kap * P.Kd[lb * NV + a];
+            lift += kv * P.dvals[-nb[o] - 1];
+          }
+        }
+        t -= lift;
+      }
+      bacc += t;
+    }
+  }
+
+  __syncthreads();
+  const int jmin = jmin_s, jmax = jmax_s;
+  if (jmax < 0) return;  // block without free nodes (uniform: whole block exits)
+  const int P0 = P.colptr[jmin];
+  const int P1 = P.colptr[jmax + 1];
+  if (d0 > 0) {
+    int w = P.colptr[d0 - 1] - P0;
+#pragma unroll
+    for (int o = 0; o < NB; ++o)
+      if ((mask >> o) & 1u) stage[w++] = acc[o];
+    P.b[d0 - 1] = bacc;
+  }
+  __syncthreads();
+  double *dst = P.nzval + P0;
+  for (int k = tid; k < P1 - P0; k += BLOCK) dst[k] = stage[k];
+}
+
+// Reference-cell stiffness and load on the Cartesian cell (Jacobian diag(h)).
+void lattice_reference(const Ctx &c, double *K, double *bv) {
+  const int D = c.dim, nv = 1 << D;
+  double det = 1.0;
+  for (int d = 0; d < D; ++d) det *= c.sizes[d];
+  for (int i = 0; i < nv * nv; ++i) K[i] = 0.0;
+  for (int i = 0; i < nv; ++i) bv[i] = 0.0;
+  for (int q = 0; q < c.nq; ++q) {
+    double dV = std::fabs(det) * c.qw[q];
+    for (int j = 0; j < nv; ++j)
+      for (int i = 0; i < nv; ++i) {
+        double dot = 0.0;
+        for (int d = 0; d < D; ++d)
+          dot += (c.grad[(q * nv + i) * D + d] / c.sizes[d]) *
+                 (c.grad[(q * nv + j) * D + d] / c.sizes[d]);
+        K[i * nv + j] += dot * dV;
+      }
+    for (int i = 0; i < nv; ++i) bv[i] += c.shp[q * nv + i] * dV;
+  }
+}
+
+template <int D, int BLOCK>
+void launch_lattice(Ctx &c, int precision) {
+  LatticeArgs P{};
+  for (int k = 0; k < 3; ++k) {
+    P.nn[k] = (int)(c.part[k] + 1);
+    P.nc[k] = (int)c.part[k];
+  }
+  P.nnodes = c.nnodes;
+  P.node_dof = c.node_dof.get();
+  P.colptr = c.colptr.get();
+  P.kappa = c.have_kappa ? c.kappa.get() : nullptr;
+  P.fsrc = c.have_fsrc ? c.fsrc.get() : nullptr;
+  P.dvals = c.have_dvals ? c.dvals.get() : nullptr;
+  P.nzval = c.nzval.get();
+  P.b = c.b.get();
+  lattice_reference(c, P.Kd, P.bd);
+  for (int i = 0; i < 64; ++i) P.Kf[i] = (float)P.Kd[i];
+  for (int i = 0; i < 8; ++i) P.bf[i] = (float)P.bd[i];
+  constexpr int NB = Pow3<D>::v;
+  size_t lds = sizeof(double) * BLOCK * NB;
+  unsigned grid = (unsigned)((c.nnodes + BLOCK - 1) / BLOCK);
+  if (precision == GH_PREC_FP32_ELEM)
+    hipLaunchKernelGGL((lattice_q1_kernel<D, BLOCK, true>), dim3(grid), dim3(BLOCK), lds,
+                       c.stream, P);
+  else
+    hipLaunchKernelGGL((lattice_q1_kernel<D, BLOCK, false>), dim3(grid), dim3(BLOCK), lds,
+                       c.stream, P);
+  GH_CHECK_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// General per-cell kernel (Q1 on quads/hexes, any geometry), coloured scatter
+// ---------------------------------------------------------------------------
+
+struct CellArgs {
+  const int32_t *cells;  // cells of this colour
+  int64_t ncells;
+  const int32_t *cell_dofs;
+  const int32_t *cell_nodes;  // nullptr: Cartesian analytic
+  const double *xyz;
+  double h[3];
+  int nq;
+  const double *tab;  // per q: w, shp[NLOC], grad[NLOC*D]
+  const double *kappa, *fsrc, *dvals;
+  const int32_t *colptr, *rowval;
+  double *nzval, *b;
+};
+
+template <int D>
+__device__ __forceinline__ double det_inv(const double *J, double *Ji) {
+  if (D == 2) {
+    double det = J[0] * J[3] - J[1] * J[2];
+    double r = 1.0 / det;
+    Ji[0] = J[3] * r; Ji[1] = -J[1] * r; Ji[2] = -J[2] * r; Ji[3] = J[0] * r;
+    return det;
+  } else {
+    double a = J[0], b = J[1], cc = J[2], d = J[3], e = J[4], f = J[5], g = J[6], h = J[7],
+           i = J[8];
+    double c0 = e * i - f * h, c1 = f * g - d * i, c2 = d * h - e * g;
+    double det = a * c0 + b * c1 + cc * c2;
+    double r = 1.0 / det;
+    Ji[0] = c0 * r; Ji[1] = (cc * h - b * i) * r; Ji[2] = (b * f - cc * e) * r;
+    Ji[3] = c1 * r; Ji[4] = (a * i - cc * g) * r; Ji[5] = (cc * d - a * f) * r;
+    Ji[6] = c2 * r; Ji[7] = (b * g - a * h) * r; Ji[8] = (a * e - b * d) * r;
+    return det;
+  }
+}
+
+template <int D, bool F32>
+__global__ __launch_bounds__(128) void cell_q1_kernel(const CellArgs A) {
+  constexpr int NL = 1 << D;
+  using T = typename std::conditional<F32, float, double>::type;
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= A.ncells) return;
+  const int64_t cell = A.cells[t];
+  int32_t dofs[NL];
+#pragma unroll
+  for (int a = 0; a < NL; ++a) dofs[a] = A.cell_dofs[cell * NL + a];
+  double X[NL][D];
+  if (A.cell_nodes) {
+#pragma unroll
+    for (int a = 0; a < NL; ++a) {
+      int64_t nd = A.cell_nodes[cell * NL + a];
+#pragma unroll
+      for (int k = 0; k < D; ++k) X[a][k] = A.xyz[nd * D + k];
+    }
+  }
+  const double kap = A.kappa ? A.kappa[cell] : 1.0;
+  const double fc = A.fsrc ? A.fsrc[cell] : 1.0;
+  T K[NL][NL];
+  T bv[NL];
+#pragma unroll
+  for (int i = 0; i < NL; ++i) {
+    bv[i] = 0;
+#pragma unroll
+    for (int j = 0; j < NL; ++j) K[i][j] = 0;
+  }
+  const int stride = 1 + NL + NL * D;
+  for (int q = 0; q < A.nq; ++q) {
+    const double *tq = A.tab + q * stride;
+    double J[D * D], Ji[D * D];
+    if (A.cell_nodes) {
+#pragma unroll
+      for (int i = 0; i < D; ++i)
+#pragma unroll
+        for (int j = 0; j < D; ++j) {
+          double s = 0.0;
+#pragma unroll
+          for (int a = 0; a < NL; ++a) s += tq[1 + NL + a * D + i] * X[a][j];
+          J[i * D + j] = s;
+        }
+    } else {
+#pragma unroll
+      for (int i = 0; i < D * D; ++i) J[i] = 0.0;
+#pragma unroll
+      for (int k = 0; k < D; ++k) J[k * D + k] = A.h[k];
+    }
+    const double det = det_inv<D>(J, Ji);
+    const T dV = (T)(fabs(det) * tq[0]);
+    T g[NL][D];
+#pragma unroll
+    for (int a = 0; a < NL; ++a)
+#pragma unroll
+      for (int r = 0; r < D; ++r) {
+        double s = 0.0;
+#pragma unroll
+        for (int k = 0; k < D; ++k) s += Ji[r * D + k] * tq[1 + NL + a * D + k];
+        g[a][r] = (T)s;
+      }
+    const T kd = (T)kap * dV;
+#pragma unroll
+    for (int i = 0; i < NL; ++i) {
+#pragma unroll
+      for (int j = 0; j < NL; ++j) {
+        T dot = 0;
+#pragma unroll
+        for (int r = 0; r < D; ++r) dot += g[i][r] * g[j][r];
+        K[i][j] += dot * kd;
+      }
+      bv[i] += (T)tq[1 + i] * ((T)fc * dV);
+    }
+  }
+  // Dirichlet lifting b -= K u_d on the cell.
+  if (A.dvals) {
+#pragma unroll
+    for (int i = 0; i < NL; ++i) {
+      double s = 0.0;
+#pragma unroll
+      for (int j = 0; j < NL; ++j)
+        if (dofs[j] < 0) s += (double)K[i][j] * A.dvals[-dofs[j] - 1];
+      bv[i] = (T)((double)bv[i] - s);
+    }
+  }
+  // Scatter: columns lj, rows li, Dirichlet ids skipped.
+#pragma unroll
+  for (int lj = 0; lj < NL; ++lj) {
+    const int32_t j = dofs[lj];
+    if (j <= 0) continue;
+    const int32_t p0 = A.colptr[j - 1], p1 = A.colptr[j];
+#pragma unroll
+    for (int li = 0; li < NL; ++li) {
+      const int32_t i = dofs[li];
+      if (i <= 0) continue;
+      int lo = p0, hi = p1;
+      while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (A.rowval[mid] < i - 1) lo = mid + 1;
+        else hi = mid;
+      }
+      if (lo < p1 && A.rowval[lo] == i - 1) A.nzval[lo] += (double)K[li][lj];
+    }
+  }
+#pragma unroll
+  for (int li = 0; li < NL; ++li)
+    if (dofs[li] > 0) A.b[dofs[li] - 1] += (double)bv[li];
+}
+
+void launch_general(Ctx &c, int precision) {
+  GH_REQUIRE(c.reffe_kind == GH_REFFE_QK && c.order == 1 && c.ncomp == 1 &&
+                 c.nloc == (1 << c.dim),
+             GH_ERR_UNSUPPORTED, "general path supports scalar Q1 on quads/hexes");
+  GH_REQUIRE(c.cartesian || c.nv == c.nloc, GH_ERR_UNSUPPORTED,
+             "geometry must be Q1 (2^D nodes per cell)");
+  GH_REQUIRE((int)c.qw.size() == c.nq && (int)c.shp.size() == c.nq * c.nloc, GH_ERR_STATE,
+             "tabulation does not match the element (dofs per cell)");
+  build_coloring(c);
+  const int NL = c.nloc, D = c.dim;
+  const int stride = 1 + NL + NL * D;
+  std::vector<double> tab(size_t(c.nq) * stride);
+  for (int q = 0; q < c.nq; ++q) {
+    tab[q * stride] = c.qw[q];
+    for (int a = 0; a < NL; ++a) tab[q * stride + 1 + a] = c.shp[q * NL + a];
+    for (int a = 0; a < NL * D; ++a) tab[q * stride + 1 + NL + a] = c.grad[q * NL * D + a];
+  }
+  DBuf<double> dtab;
+  dtab.resize(tab.size());
+  GH_CHECK_HIP(hipMemcpyAsync(dtab.get(), tab.data(), sizeof(double) * tab.size(),
+                              hipMemcpyHostToDevice, c.stream));
+  GH_CHECK_HIP(hipMemsetAsync(c.nzval.get(), 0, sizeof(double) * std::max<int64_t>(c.nnz, 1),
+                              c.stream));
+  GH_CHECK_HIP(hipMemsetAsync(c.b.get(), 0, sizeof(double) * std::max<int64_t>(c.nfree, 1),
+                              c.stream));
+  CellArgs A{};
+  A.cell_dofs = c.cell_dofs.get();
+  A.cell_nodes = c.cartesian ? nullptr : c.cell_nodes.get();
+  A.xyz = c.cartesian ? nullptr : c.xyz.get();
+  for (int k = 0; k < 3; ++k) A.h[k] = c.sizes[k];
+  A.nq = c.nq;
+  A.tab = dtab.get();
+  A.kappa = c.have_kappa ? c.kappa.get() : nullptr;
+  A.fsrc = c.have_fsrc ? c.fsrc.get() : nullptr;
+  A.dvals = c.have_dvals ? c.dvals.get() : nullptr;
+  A.colptr = c.colptr.get();
+  A.rowval = c.rowval.get();
+  A.nzval = c.nzval.get();
+  A.b = c.b.get();
+  for (int col = 0; col < c.ncolors; ++col) {
+    A.cells = c.color_cells.get() + c.color_ptr[col];
+    A.ncells = c.color_ptr[col + 1] - c.color_ptr[col];
+    if (A.ncells == 0) continue;
+    unsigned grid = (unsigned)((A.ncells + 127) / 128);
+    bool f32 = precision == GH_PREC_FP32_ELEM;
+    if (D == 2) {
+      if (f32) hipLaunchKernelGGL((cell_q1_kernel<2, true>), dim3(grid), dim3(128), 0, c.stream, A);
+      else hipLaunchKernelGGL((cell_q1_kernel<2, false>), dim3(grid), dim3(128), 0, c.stream, A);
+    } else {
+      if (f32) hipLaunchKernelGGL((cell_q1_kernel<3, true>), dim3(grid), dim3(128), 0, c.stream, A);
+      else hipLaunchKernelGGL((cell_q1_kernel<3, false>), dim3(grid), dim3(128), 0, c.stream, A);
+    }
+    GH_CHECK_HIP(hipGetLastError());
+  }
+  GH_CHECK_HIP(hipStreamSynchronize(c.stream));
+}
+
+}  // namespace
+
+void assemble_diffusion(Ctx &c, int precision, int path) {
+  GH_REQUIRE(!c.have_kappa || c.kappa.n == (size_t)c.ncells, GH_ERR_STATE,
+             "kappa was uploaded for a different mesh");
+  GH_REQUIRE(!c.have_dvals || c.dvals.n == (size_t)c.ndiri, GH_ERR_STATE,
+             "Dirichlet values were uploaded for a different space");
+  bool lattice = c.structured && path != GH_PATH_GENERAL;
+  GH_REQUIRE(path != GH_PATH_STRUCTURED || c.structured, GH_ERR_UNSUPPORTED,
+             "the lattice path needs a Cartesian mesh with a scalar Q1 CLagrangian space");
+  if (lattice) {
+    GH_REQUIRE((int)c.shp.size() == c.nq * c.nloc, GH_ERR_STATE,
+               "tabulation does not match the element (dofs per cell)");
+    // Enqueued only: gh_values_get (or a wait on gh_stream) orders the results.
+    if (c.dim == 3)
+      launch_lattice<3, 128>(c, precision);
+    else
+      launch_lattice<2, 256>(c, precision);
+  } else {
+    launch_general(c, precision);
+  }
+}
+
+}  // namespace gh

@@ -1,0 +1,333 @@
+// C ABI of the assembly engine: argument checks, host<->device copies and the
+// status-code boundary.  Compute lives in pattern.hip, assemble.hip, solve.hip.
+#include <algorithm>
+#include <cstring>
+
+#include "gh_internal.h"
+
+struct gh_ctx {
+  gh::Ctx c;
+};
+
+namespace {
+
+template <class F>
+int guarded(gh_ctx *ctx, F &&f) {
+  if (!ctx) return GH_ERR_ARG;
+  try {
+    GH_CHECK_HIP(hipSetDevice(ctx->c.device));
+    f(ctx->c);
+    ctx->c.err.clear();
+    return GH_OK;
+  } catch (const gh::Error &e) {
+    ctx->c.err = e.what();
+    return e.code;
+  } catch (const std::bad_alloc &) {
+    ctx->c.err = "host allocation failed";
+    return GH_ERR_OOM;
+  } catch (const std::exception &e) {
+    ctx->c.err = e.what();
+    return GH_ERR_HIP;
+  }
+}
+
+void invalidate_pattern(gh::Ctx &c) {
+  c.have_pattern = false;
+  c.have_values = false;
+  c.structured = false;
+  c.ncolors = 0;
+  c.color_ptr.clear();
+}
+
+}  // namespace
+
+extern "C" {
+
+int gh_create(gh_ctx **out, int device) {
+  if (!out) return GH_ERR_ARG;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return GH_ERR_HIP;
+  if (device < 0 || device >= ndev) return GH_ERR_ARG;
+  gh_ctx *ctx = new (std::nothrow) gh_ctx();
+  if (!ctx) return GH_ERR_OOM;
+  ctx->c.device = device;
+  if (hipSetDevice(device) != hipSuccess ||
+      hipStreamCreateWithFlags(&ctx->c.stream, hipStreamNonBlocking) != hipSuccess) {
+    delete ctx;
+    return GH_ERR_HIP;
+  }
+  *out = ctx;
+  return GH_OK;
+}
+
+int gh_destroy(gh_ctx *ctx) {
+  if (!ctx) return GH_OK;
+  (void)hipSetDevice(ctx->c.device);
+  (void)hipStreamSynchronize(ctx->c.stream);
+  hipStream_t s = ctx->c.stream;
+  delete ctx;  // frees device buffers
+  (void)hipStreamDestroy(s);
+  return GH_OK;
+}
+
+const char *gh_last_error(const gh_ctx *ctx) {
+  return ctx ? ctx->c.err.c_str() : "null context";
+}
+
+int gh_stream(gh_ctx *ctx, void **stream) {
+  return guarded(ctx, [&](gh::Ctx &c) {
+    GH_REQUIRE(stream, GH_ERR_ARG, "stream pointer is NULL");
+    *stream = (void *)c.stream;
+  });
+}
+
+int gh_mesh_cartesian(gh_ctx *ctx, int dim, const double *origin, const double *sizes,
+                      const int64_t *partition) {
+  return guarded(ctx, [&](gh::Ctx &c) {
+    GH_REQUIRE(dim == 2 || dim == 3, GH_ERR_ARG, "Cartesian mesh dimension must be 2 or 3");
+    GH_REQUIRE(origin && sizes && partition, GH_ERR_ARG, "NULL descriptor array");
+    int64_t nn = 1, nc = 1;
+    for (int d = 0; d < dim; ++d) {
+      GH_REQUIRE(partition[d] >= 1, GH_ERR_ARG, "partition entries must be >= 1");
+      GH_REQUIRE(sizes[d] > 0, GH_ERR_ARG, "cell sizes must be positive");
+      nn *= partition[d] + 1;
+      nc *= partition[d];
+    }
+    GH_REQUIRE(nn < (int64_t(1) << 31), GH_ERR_ARG, "too many nodes for 32-bit ids");
+    c.dim = dim;
+    c.cartesian = true;
+    for (int d = 0; d < 3; ++d) {
+      c.origin[d] = d < dim ? origin[d] : 0.0;
+      c.sizes[d] = d < dim ? sizes[d] : 1.0;
+      c.part[d] = d < dim ? partition[d] : 1;
+    }
+    c.nnodes = nn;
+    c.ncells = nc;
+    c.nv = 1 << dim;
+    c.xyz.release();
+    c.cell_nodes.release();
+    invalidate_pattern(c);
+  });
+}
+
+int gh_mesh_upload(gh_ctx *ctx, int dim, int64_t nnodes, const double *xyz, int64_t ncells,
+                   const int32_t *data, const int32_t *ptrs) {
+  return guarded(ctx, [&](gh::Ctx &c) {
+    GH_REQUIRE(dim == 2 || dim == 3, GH_ERR_ARG, "mesh dimension must be 2 or 3");
+    GH_REQUIRE(nnodes > 0 && ncells > 0 && xyz && data && ptrs, GH_ERR_ARG,
+               "empty mesh or NULL array");
+    GH_REQUIRE(ptrs[0] == 1, GH_ERR_ARG, "cell_node_ids ptrs must start at 1");
+    int nv = ptrs[1] - ptrs[0];
+    GH_REQUIRE(nv > 0 && nv <= 32, GH_ERR_ARG, "unsupported nodes per cell");
+    std::vector<int32_t> cn(size_t(ncells) * nv);
+    for (int64_t k = 0; k < ncells; ++k) {
+      GH_REQUIRE(ptrs[k + 1] - ptrs[k] == nv, GH_ERR_UNSUPPORTED,
+                 "mixed cell types are not supported");
+      for (int a = 0; a < nv; ++a) {
+        int32_t n = data[ptrs[k] - 1 + a];
+        GH_REQUIRE(n >= 1 && n <= nnodes, GH_ERR_RANGE, "cell node id out of range");
+        cn[size_t(k) * nv + a] = n - 1;
+      }
+    }
+    c.dim = dim;
+    c.cartesian = false;
+    c.nnodes = nnodes;
+    c.ncells = ncells;
+    c.nv = nv;
+    c.xyz.resize(size_t(nnodes) * dim);
+    c.cell_nodes.resize(cn.size());
+    GH_CHECK_HIP(hipMemcpy(c.xyz.get(), xyz, sizeof(double) * nnodes * dim,
+                           hipMemcpyHostToDevice));
+    GH_CHECK_HIP(hipMemcpy(c.cell_nodes.get(), cn.data(), sizeof(int32_t) * cn.size(),
+                           hipMemcpyHostToDevice));
+    invalidate_pattern(c);
+  });
+}
+
+int gh_space_upload(gh_ctx *ctx, int reffe_kind, int order, int ncomp, const int32_t *data,
+                    const int32_t *ptrs, int64_t ncells, int64_t nfree, int64_t ndiri) {
+  return guarded(ctx, [&](gh::Ctx &c) {
+    GH_REQUIRE(c.dim > 0, GH_ERR_STATE, "upload a mesh before the FE space");
+    GH_REQUIRE(ncells == c.ncells, GH_ERR_ARG, "cell count differs from the mesh");
+    GH_REQUIRE(data && ptrs, GH_ERR_ARG, "NULL cell_dof_ids array");
+    GH_REQUIRE(reffe_kind == GH_REFFE_QK || reffe_kind == GH_REFFE_PK, GH_ERR_UNSUPPORTED,
+               "unknown reference element kind");
+    GH_REQUIRE(order >= 1 && ncomp >= 1, GH_ERR_ARG, "order and ncomp must be >= 1");
+    GH_REQUIRE(nfree >= 0 && ndiri >= 0 && nfree < (int64_t(1) << 31) &&
+                   ndiri < (int64_t(1) << 31),
+               GH_ERR_ARG, "dof counts out of range");
+    GH_REQUIRE(ptrs[0] == 1, GH_ERR_ARG, "cell_dof_ids ptrs must start at 1");
+    int nloc = ptrs[1] - ptrs[0];
+    GH_REQUIRE(nloc >= 1 && nloc <= 256, GH_ERR_UNSUPPORTED, "unsupported dofs per cell");
+    std::vector<int32_t> cd(size_t(ncells) * nloc);
+    for (int64_t k = 0; k < ncells; ++k) {
+      GH_REQUIRE(ptrs[k + 1] - ptrs[k] == nloc, GH_ERR_UNSUPPORTED,
+                 "cells with different numbers of dofs are not supported");
+      const int32_t *src = data + (ptrs[k] - 1);
+      for (int a = 0; a < nloc; ++a) {
+        int32_t d = src[a];
+        GH_REQUIRE(d != 0 && d <= nfree && -d <= ndiri, GH_ERR_RANGE,
+                   "cell dof id out of range");
+      }
+      std::memcpy(&cd[size_t(k) * nloc], src, sizeof(int32_t) * nloc);
+    }
+    c.reffe_kind = reffe_kind;
+    c.order = order;
+    c.ncomp = ncomp;
+    c.nloc = nloc;
+    c.nfree = nfree;
+    c.ndiri = ndiri;
+    c.cell_dofs.resize(cd.size());
+    GH_CHECK_HIP(hipMemcpy(c.cell_dofs.get(), cd.data(), sizeof(int32_t) * cd.size(),
+                           hipMemcpyHostToDevice));
+    c.h_cell_dofs.swap(cd);
+    c.have_space = true;
+    invalidate_pattern(c);
+    c.have_dvals = false;
+  });
+}
+
+int gh_reffe_tabulation(gh_ctx *ctx, int nq, int nloc, const double *qw, const double *shp,
+                        const double *grad) {
+  return guarded(ctx, [&](gh::Ctx &c) {
+    GH_REQUIRE(c.dim > 0, GH_ERR_STATE, "upload a mesh before the tabulation");
+    GH_REQUIRE(nq >= 1 && nq <= 512 && nloc >= 1 && nloc <= 256, GH_ERR_ARG,
+               "tabulation sizes out of range");
+    GH_REQUIRE(qw && shp && grad, GH_ERR_ARG, "NULL tabulation array");
+    c.nq = nq;
+    c.qw.assign(qw, qw + nq);
+    c.shp.assign(shp, shp + size_t(nq) * nloc);
+    c.grad.assign(grad, grad + size_t(nq) * nloc * c.dim);
+    c.have_values = false;
+  });
+}
+
+int gh_symbolic(gh_ctx *ctx, int64_t *nnz_out) {
+  return guarded(ctx, [&](gh::Ctx &c) {
+    GH_REQUIRE(c.have_space, GH_ERR_STATE, "upload the FE space before gh_symbolic");
+    invalidate_pattern(c);
+    gh::symbolic_general(c);
+    gh::check_structured(c);
+    if (nnz_out) *nnz_out = c.nnz;
+  });
+}
+
+int gh_pattern_get(gh_ctx *ctx, int64_t *colptr, int64_t *rowval) {
+  return guarded(ctx, [&](gh::Ctx &c) {
+    GH_REQUIRE(c.have_pattern, GH_ERR_STATE, "no pattern: call gh_symbolic first");
+    std::vector<int32_t> cp(c.nfree + 1), rv(c.nnz);
+    GH_CHECK_HIP(hipMemcpyAsync(cp.data(), c.colptr.get(), sizeof(int32_t) * cp.size(),
+                                hipMemcpyDeviceToHost, c.stream));
+    if (c.nnz)
+      GH_CHECK_HIP(hipMemcpyAsync(rv.data(), c.rowval.get(), sizeof(int32_t) * rv.size(),
+                                  hipMemcpyDeviceToHost, c.stream));
+    GH_CHECK_HIP(hipStreamSynchronize(c.stream));
+    if (colptr)
+      for (size_t k = 0; k < cp.size(); ++k) colptr[k] = int64_t(cp[k]) + 1;
+    if (rowval)
+      for (size_t k = 0; k < rv.size(); ++k) rowval[k] = int64_t(rv[k]) + 1;
+  });
+}
+
+int gh_coef_upload(gh_ctx *ctx, const double *kappa, const double *fsrc,
+                   const double *dirichlet_vals) {
+  return guarded(ctx, [&](gh::Ctx &c) {
+    GH_REQUIRE(c.have_space, GH_ERR_STATE, "upload the FE space before coefficients");
+    auto up = [&](gh::DBuf<double> &buf, bool &flag, const double *src, int64_t n) {
+      flag = src != nullptr && n > 0;
+      if (!flag) return;
+      buf.resize(n);
+      GH_CHECK_HIP(hipMemcpyAsync(buf.get(), src, sizeof(double) * n, hipMemcpyHostToDevice,
+                                  c.stream));
+    };
+    up(c.kappa, c.have_kappa, kappa, c.ncells);
+    up(c.fsrc, c.have_fsrc, fsrc, c.ncells);
+    up(c.dvals, c.have_dvals, dirichlet_vals, c.ndiri);
+    GH_CHECK_HIP(hipStreamSynchronize(c.stream));
+  });
+}
+
+int gh_assemble(gh_ctx *ctx, int form_kind, int precision, int path) {
+  return guarded(ctx, [&](gh::Ctx &c) {
+    GH_REQUIRE(form_kind == GH_FORM_DIFFUSION, GH_ERR_UNSUPPORTED, "unknown form kind");
+    GH_REQUIRE(precision == GH_PREC_FP64 || precision == GH_PREC_FP32_ELEM, GH_ERR_ARG,
+               "unknown precision mode");
+    GH_REQUIRE(path == GH_PATH_AUTO || path == GH_PATH_GENERAL || path == GH_PATH_STRUCTURED,
+               GH_ERR_ARG, "unknown numeric path");
+    GH_REQUIRE(c.have_pattern, GH_ERR_STATE, "no pattern: call gh_symbolic first");
+    GH_REQUIRE(c.nq > 0, GH_ERR_STATE, "no reference tabulation uploaded");
+    gh::assemble_diffusion(c, precision, path);
+    c.have_values = true;
+  });
+}
+
+int gh_values_get(gh_ctx *ctx, double *nzval, double *b) {
+  return guarded(ctx, [&](gh::Ctx &c) {
+    GH_REQUIRE(c.have_values, GH_ERR_STATE, "no assembled values: call gh_assemble first");
+    if (nzval && c.nnz)
+      GH_CHECK_HIP(hipMemcpyAsync(nzval, c.nzval.get(), sizeof(double) * c.nnz,
+                                  hipMemcpyDeviceToHost, c.stream));
+    if (b && c.nfree)
+      GH_CHECK_HIP(hipMemcpyAsync(b, c.b.get(), sizeof(double) * c.nfree,
+                                  hipMemcpyDeviceToHost, c.stream));
+    GH_CHECK_HIP(hipStreamSynchronize(c.stream));
+  });
+}
+
+int gh_numeric(gh_ctx *ctx, int form_kind, const double *cell_coef, const double *cell_source,
+               int precision, double *nzval, double *b, const double *dirichlet_vals) {
+  int st = gh_coef_upload(ctx, cell_coef, cell_source, dirichlet_vals);
+  if (st) return st;
+  st = gh_assemble(ctx, form_kind, precision, GH_PATH_AUTO);
+  if (st) return st;
+  return gh_values_get(ctx, nzval, b);
+}
+
+int gh_spmv(gh_ctx *ctx, const double *x, double *y) {
+  return guarded(ctx, [&](gh::Ctx &c) {
+    GH_REQUIRE(c.have_values, GH_ERR_STATE, "no assembled matrix");
+    GH_REQUIRE(x && y, GH_ERR_ARG, "NULL vector");
+    c.s_x.resize(c.nfree);
+    c.s_q.resize(c.nfree);
+    GH_CHECK_HIP(hipMemcpyAsync(c.s_x.get(), x, sizeof(double) * c.nfree,
+                                hipMemcpyHostToDevice, c.stream));
+    gh::spmv_device(c, c.s_x.get(), c.s_q.get());
+    GH_CHECK_HIP(hipMemcpyAsync(y, c.s_q.get(), sizeof(double) * c.nfree,
+                                hipMemcpyDeviceToHost, c.stream));
+    GH_CHECK_HIP(hipStreamSynchronize(c.stream));
+  });
+}
+
+int gh_pcg(gh_ctx *ctx, int precond, const double *b, double *x, double rtol, int maxit,
+           gh_solve_stats *stats) {
+  int st = guarded(ctx, [&](gh::Ctx &c) {
+    GH_REQUIRE(c.have_values, GH_ERR_STATE, "no assembled matrix");
+    GH_REQUIRE(x, GH_ERR_ARG, "NULL solution vector");
+    GH_REQUIRE(precond == GH_PRECOND_NONE || precond == GH_PRECOND_JACOBI,
+               GH_ERR_UNSUPPORTED, "unknown preconditioner");
+    GH_REQUIRE(rtol > 0 && maxit >= 0, GH_ERR_ARG, "bad tolerance or iteration cap");
+    size_t n = c.nfree;
+    c.s_x.resize(n);
+    c.s_b.resize(n);
+    GH_CHECK_HIP(hipMemcpyAsync(c.s_x.get(), x, sizeof(double) * n, hipMemcpyHostToDevice,
+                                c.stream));
+    if (b)
+      GH_CHECK_HIP(hipMemcpyAsync(c.s_b.get(), b, sizeof(double) * n, hipMemcpyHostToDevice,
+                                  c.stream));
+    else
+      GH_CHECK_HIP(hipMemcpyAsync(c.s_b.get(), c.b.get(), sizeof(double) * n,
+                                  hipMemcpyDeviceToDevice, c.stream));
+    gh_solve_stats local{};
+    gh::pcg_device(c, precond, rtol, maxit, &local);
+    GH_CHECK_HIP(hipMemcpyAsync(x, c.s_x.get(), sizeof(double) * n, hipMemcpyDeviceToHost,
+                                c.stream));
+    GH_CHECK_HIP(hipStreamSynchronize(c.stream));
+    if (stats) *stats = local;
+    if (!local.converged)
+      throw gh::Error(GH_ERR_NOCONV, "PCG did not reach the requested tolerance");
+  });
+  return st;
+}
+
+}  // extern "C"

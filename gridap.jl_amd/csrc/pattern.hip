@@ -1,0 +1,329 @@
+// Symbolic phase on the device: the CSC pattern of create_from_nz.
+//
+// Gridap (src/Algebra/SparseMatrixCSC.jl:72-283) counts, allocates, inserts each
+// (i,j) touched by a cell into column j by binary search and compacts: the result
+// is, per column, the sorted unique rows of every (i>0, j>0) pair of every cell,
+// explicit zeros kept.  Here every cell emits its pairs as packed (col,row) keys,
+// one radix sort orders them, and a unique-flag scan writes rowval/colptr.  The
+// order is total, so the pattern is bit-identical whatever the launch order.
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <climits>
+
+#include "gh_internal.h"
+
+namespace gh {
+namespace {
+
+constexpr int kBlock = 256;
+
+inline unsigned grid_for(int64_t n, int block = kBlock) {
+  int64_t g = (n + block - 1) / block;
+  return (unsigned)std::max<int64_t>(1, g);
+}
+
+int bits_for(int64_t v) {  // bits needed to hold values 0..v
+  int b = 1;
+  while ((int64_t(1) << b) <= v) ++b;
+  return b;
+}
+
+// One thread per (cell, lj, li): the pairs of _add_entries!
+// (src/Algebra/AlgebraInterfaces.jl:175-187), Dirichlet ids skipped.
+__global__ void emit_pairs(int64_t ncells, int nloc, const int32_t *__restrict__ cell_dofs,
+                           int rb, uint64_t invalid, uint64_t *__restrict__ keys) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t per = (int64_t)nloc * nloc;
+  if (t >= ncells * per) return;
+  int64_t c = t / per;
+  int r = (int)(t - c * per);
+  int lj = r / nloc, li = r - lj * nloc;
+  int32_t j = cell_dofs[c * nloc + lj];
+  int32_t i = cell_dofs[c * nloc + li];
+  keys[t] = (i > 0 && j > 0) ? ((uint64_t)(j - 1) << rb) | (uint64_t)(i - 1) : invalid;
+}
+
+__global__ void unique_flags(int64_t n, const uint64_t *__restrict__ keys, uint64_t invalid,
+                             int rb, int32_t *__restrict__ flags, int32_t *__restrict__ colcnt) {
+  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  uint64_t key = keys[k];
+  int f = key < invalid && (k == 0 || keys[k - 1] != key);
+  flags[k] = f;
+  if (f) atomicAdd(&colcnt[key >> rb], 1);
+}
+
+__global__ void scatter_rows(int64_t n, const uint64_t *__restrict__ keys,
+                             const int32_t *__restrict__ flags, const int32_t *__restrict__ pos,
+                             uint64_t rowmask, int32_t *__restrict__ rowval) {
+  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n || !flags[k]) return;
+  rowval[pos[k]] = (int32_t)(keys[k] & rowmask);
+}
+
+// ---- lattice eligibility (Cartesian mesh, scalar Q1) ----
+
+__device__ __forceinline__ int64_t cart_node(int D, const int64_t *np, int64_t cell,
+                                             const int64_t *nc, int a) {
+  int64_t node = 0, stride = 1, r = cell;
+  for (int d = 0; d < D; ++d) {
+    int64_t ci = r % nc[d];
+    r /= nc[d];
+    node += (ci + ((a >> d) & 1)) * stride;
+    stride *= np[d];
+  }
+  return node;
+}
+
+struct Lattice {
+  int D;
+  int64_t nc[3], np[3];
+};
+
+__global__ void lattice_scatter(Lattice L, int64_t ncells, int nloc,
+                                const int32_t *__restrict__ cell_dofs,
+                                int32_t *__restrict__ node_dof) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ncells * nloc) return;
+  int64_t c = t / nloc;
+  int a = (int)(t - c * nloc);
+  node_dof[cart_node(L.D, L.np, c, L.nc, a)] = cell_dofs[t];
+}
+
+__global__ void lattice_verify(Lattice L, int64_t ncells, int nloc,
+                               const int32_t *__restrict__ cell_dofs,
+                               const int32_t *__restrict__ node_dof, int32_t *bad) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ncells * nloc) return;
+  int64_t c = t / nloc;
+  int a = (int)(t - c * nloc);
+  if (node_dof[cart_node(L.D, L.np, c, L.nc, a)] != cell_dofs[t]) atomicOr(bad, 1);
+}
+
+__global__ void free_flags(int64_t nnodes, const int32_t *__restrict__ node_dof,
+                           int32_t *__restrict__ f, int32_t *bad) {
+  int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= nnodes) return;
+  int32_t d = node_dof[n];
+  if (d == 0) atomicOr(bad, 2);
+  f[n] = d > 0;
+}
+
+// Free dofs must be numbered in node order (CLagrangian numbering) and the
+// pattern column of node n must be the sorted free lattice neighbours of n.
+__global__ void lattice_check_pattern(Lattice L, int64_t nnodes,
+                                      const int32_t *__restrict__ node_dof,
+                                      const int32_t *__restrict__ scan,
+                                      const int32_t *__restrict__ colptr,
+                                      const int32_t *__restrict__ rowval, int32_t *bad) {
+  int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= nnodes) return;
+  int32_t d = node_dof[n];
+  if (d <= 0) return;
+  if (d != scan[n] + 1) {
+    atomicOr(bad, 4);
+    return;
+  }
+  int64_t xi[3] = {0, 0, 0}, r = n;
+  for (int k = 0; k < L.D; ++k) {
+    xi[k] = r % L.np[k];
+    r /= L.np[k];
+  }
+  int nb = 1;
+  for (int k = 0; k < L.D; ++k) nb *= 3;
+  int32_t p = colptr[d - 1], pend = colptr[d];
+  for (int o = 0; o < nb; ++o) {
+    int oo = o;
+    int64_t node = 0, stride = 1;
+    bool in = true;
+    for (int k = 0; k < L.D; ++k) {
+      int64_t v = xi[k] + (oo % 3) - 1;
+      oo /= 3;
+      in = in && v >= 0 && v < L.np[k];
+      node += v * stride;
+      stride *= L.np[k];
+    }
+    if (!in) continue;
+    int32_t nd = node_dof[node];
+    if (nd <= 0) continue;
+    if (p >= pend || rowval[p] != nd - 1) {
+      atomicOr(bad, 8);
+      return;
+    }
+    ++p;
+  }
+  if (p != pend) atomicOr(bad, 8);
+}
+
+template <class T>
+void exclusive_sum(const T *in, T *out, int64_t n, hipStream_t s) {
+  size_t bytes = 0;
+  GH_CHECK_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, in, out, (int)n, s));
+  DBuf<unsigned char> tmp;
+  tmp.resize(std::max<size_t>(bytes, 1));
+  GH_CHECK_HIP(hipcub::DeviceScan::ExclusiveSum(tmp.get(), bytes, in, out, (int)n, s));
+  GH_CHECK_HIP(hipStreamSynchronize(s));
+}
+
+}  // namespace
+
+void symbolic_general(Ctx &c) {
+  hipStream_t s = c.stream;
+  const int64_t per = (int64_t)c.nloc * c.nloc;
+  const int64_t total = c.ncells * per;
+  GH_REQUIRE(total < (int64_t(1) << 31), GH_ERR_UNSUPPORTED,
+             "cell pair count exceeds 2^31 in one symbolic pass");
+  const int rb = bits_for(std::max<int64_t>(c.nfree, 1));
+  GH_REQUIRE(2 * rb <= 64, GH_ERR_UNSUPPORTED, "dof count too large for packed keys");
+  const uint64_t invalid = (uint64_t)c.nfree << rb;
+
+  DBuf<uint64_t> keys, sorted;
+  keys.resize(std::max<int64_t>(total, 1));
+  sorted.resize(std::max<int64_t>(total, 1));
+  if (total)
+    hipLaunchKernelGGL(emit_pairs, dim3(grid_for(total)), dim3(kBlock), 0, s, c.ncells,
+                       c.nloc, c.cell_dofs.get(), rb, invalid, keys.get());
+  GH_CHECK_HIP(hipGetLastError());
+  {
+    size_t bytes = 0;
+    GH_CHECK_HIP(hipcub::DeviceRadixSort::SortKeys(nullptr, bytes, keys.get(), sorted.get(),
+                                                   (int)total, 0, 2 * rb, s));
+    DBuf<unsigned char> tmp;
+    tmp.resize(std::max<size_t>(bytes, 1));
+    GH_CHECK_HIP(hipcub::DeviceRadixSort::SortKeys(tmp.get(), bytes, keys.get(), sorted.get(),
+                                                   (int)total, 0, 2 * rb, s));
+    GH_CHECK_HIP(hipStreamSynchronize(s));
+  }
+  keys.release();
+
+  DBuf<int32_t> flags, pos, colcnt;
+  flags.resize(std::max<int64_t>(total, 1));
+  pos.resize(std::max<int64_t>(total, 1));
+  colcnt.resize(c.nfree + 1);
+  GH_CHECK_HIP(hipMemsetAsync(colcnt.get(), 0, sizeof(int32_t) * (c.nfree + 1), s));
+  if (total)
+    hipLaunchKernelGGL(unique_flags, dim3(grid_for(total)), dim3(kBlock), 0, s, total,
+                       sorted.get(), invalid, rb, flags.get(), colcnt.get());
+  GH_CHECK_HIP(hipGetLastError());
+  int64_t nnz = 0;
+  if (total) {
+    exclusive_sum(flags.get(), pos.get(), total, s);
+    int32_t last_pos = 0, last_flag = 0;
+    GH_CHECK_HIP(hipMemcpy(&last_pos, pos.get() + total - 1, 4, hipMemcpyDeviceToHost));
+    GH_CHECK_HIP(hipMemcpy(&last_flag, flags.get() + total - 1, 4, hipMemcpyDeviceToHost));
+    nnz = (int64_t)last_pos + last_flag;
+  }
+  c.nnz = nnz;
+  c.colptr.resize(c.nfree + 1);
+  exclusive_sum(colcnt.get(), c.colptr.get(), c.nfree + 1, s);
+  c.rowval.resize(std::max<int64_t>(nnz, 1));
+  if (total)
+    hipLaunchKernelGGL(scatter_rows, dim3(grid_for(total)), dim3(kBlock), 0, s, total,
+                       sorted.get(), flags.get(), pos.get(), (uint64_t(1) << rb) - 1,
+                       c.rowval.get());
+  GH_CHECK_HIP(hipGetLastError());
+  GH_CHECK_HIP(hipStreamSynchronize(s));
+  c.have_pattern = true;
+  c.nzval.resize(std::max<int64_t>(nnz, 1));
+  c.b.resize(std::max<int64_t>(c.nfree, 1));
+}
+
+void check_structured(Ctx &c) {
+  c.structured = false;
+  c.node_dof.release();
+  if (!(c.cartesian && c.reffe_kind == GH_REFFE_QK && c.order == 1 && c.ncomp == 1 &&
+        c.nloc == (1 << c.dim) && c.nfree > 0))
+    return;
+  hipStream_t s = c.stream;
+  Lattice L;
+  L.D = c.dim;
+  for (int d = 0; d < 3; ++d) {
+    L.nc[d] = c.part[d];
+    L.np[d] = c.part[d] + 1;
+  }
+  c.node_dof.resize(c.nnodes);
+  DBuf<int32_t> bad, f, scan;
+  bad.resize(1);
+  f.resize(c.nnodes);
+  scan.resize(c.nnodes);
+  GH_CHECK_HIP(hipMemsetAsync(c.node_dof.get(), 0, sizeof(int32_t) * c.nnodes, s));
+  GH_CHECK_HIP(hipMemsetAsync(bad.get(), 0, sizeof(int32_t), s));
+  int64_t t = c.ncells * c.nloc;
+  hipLaunchKernelGGL(lattice_scatter, dim3(grid_for(t)), dim3(kBlock), 0, s, L, c.ncells,
+                     c.nloc, c.cell_dofs.get(), c.node_dof.get());
+  hipLaunchKernelGGL(lattice_verify, dim3(grid_for(t)), dim3(kBlock), 0, s, L, c.ncells,
+                     c.nloc, c.cell_dofs.get(), c.node_dof.get(), bad.get());
+  hipLaunchKernelGGL(free_flags, dim3(grid_for(c.nnodes)), dim3(kBlock), 0, s, c.nnodes,
+                     c.node_dof.get(), f.get(), bad.get());
+  GH_CHECK_HIP(hipGetLastError());
+  exclusive_sum(f.get(), scan.get(), c.nnodes, s);
+  hipLaunchKernelGGL(lattice_check_pattern, dim3(grid_for(c.nnodes)), dim3(kBlock), 0, s, L,
+                     c.nnodes, c.node_dof.get(), scan.get(), c.colptr.get(), c.rowval.get(),
+                     bad.get());
+  GH_CHECK_HIP(hipGetLastError());
+  int32_t hbad = 0;
+  GH_CHECK_HIP(hipMemcpyAsync(&hbad, bad.get(), 4, hipMemcpyDeviceToHost, s));
+  GH_CHECK_HIP(hipStreamSynchronize(s));
+  c.structured = (hbad == 0);
+  if (!c.structured) c.node_dof.release();
+}
+
+// Greedy colouring of cells so that no two cells of one colour share a free dof:
+// each colour's scatter is race-free and the colour sequence fixes the summation
+// order (deterministic).  Cells keep ascending order inside a colour.
+void build_coloring(Ctx &c) {
+  if (c.ncolors > 0) return;
+  const int64_t C = c.ncells;
+  const int nloc = c.nloc;
+  const std::vector<int32_t> &cd = c.h_cell_dofs;
+  std::vector<int64_t> dptr(c.nfree + 1, 0);
+  for (int64_t k = 0; k < C * nloc; ++k)
+    if (cd[k] > 0) dptr[cd[k]]++;
+  for (int64_t j = 0; j < c.nfree; ++j) dptr[j + 1] += dptr[j];
+  std::vector<int32_t> dcells(dptr[c.nfree]);
+  {
+    std::vector<int64_t> cur(dptr.begin(), dptr.end() - 1);
+    for (int64_t cell = 0; cell < C; ++cell)
+      for (int a = 0; a < nloc; ++a) {
+        int32_t d = cd[cell * nloc + a];
+        if (d > 0) dcells[cur[d - 1]++] = (int32_t)cell;
+      }
+  }
+  std::vector<int32_t> color(C, -1);
+  std::vector<int64_t> stamp(64, -1);
+  int ncol = 0;
+  for (int64_t cell = 0; cell < C; ++cell) {
+    for (int a = 0; a < nloc; ++a) {
+      int32_t d = cd[cell * nloc + a];
+      if (d <= 0) continue;
+      for (int64_t k = dptr[d - 1]; k < dptr[d]; ++k) {
+        int32_t other = dcells[k];
+        int32_t col = color[other];
+        if (col >= 0) {
+          if ((size_t)col >= stamp.size()) stamp.resize(col * 2 + 1, -1);
+          stamp[col] = cell;
+        }
+      }
+    }
+    int chosen = 0;
+    while ((size_t)chosen < stamp.size() && stamp[chosen] == cell) ++chosen;
+    if ((size_t)chosen >= stamp.size()) stamp.resize(chosen * 2 + 1, -1);
+    color[cell] = chosen;
+    ncol = std::max(ncol, chosen + 1);
+  }
+  c.color_ptr.assign(ncol + 1, 0);
+  for (int64_t cell = 0; cell < C; ++cell) c.color_ptr[color[cell] + 1]++;
+  for (int k = 0; k < ncol; ++k) c.color_ptr[k + 1] += c.color_ptr[k];
+  std::vector<int32_t> cells(C);
+  {
+    std::vector<int64_t> cur(c.color_ptr.begin(), c.color_ptr.end() - 1);
+    for (int64_t cell = 0; cell < C; ++cell) cells[cur[color[cell]]++] = (int32_t)cell;
+  }
+  c.color_cells.resize(std::max<int64_t>(C, 1));
+  GH_CHECK_HIP(hipMemcpy(c.color_cells.get(), cells.data(), sizeof(int32_t) * C,
+                         hipMemcpyHostToDevice));
+  c.ncolors = ncol;
+}
+
+}  // namespace gh

@@ -1,0 +1,202 @@
+"""ctypes binding of libgridap_hip.so (the C ABI in include/gridap_hip.h).
+
+The HIP library is the only compute path: importing this module without the
+built library raises, and a failed device call raises GridapHipError with the
+library's message.  There is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libgridap_hip.so")
+
+GH_OK = 0
+GH_ERR_ARG = 1
+GH_ERR_RANGE = 2
+GH_ERR_HIP = 3
+GH_ERR_OOM = 4
+GH_ERR_STATE = 5
+GH_ERR_NOCONV = 6
+GH_ERR_UNSUPPORTED = 7
+
+GH_REFFE_QK = 1
+GH_REFFE_PK = 2
+GH_FORM_DIFFUSION = 1
+GH_PREC_FP64 = 0
+GH_PREC_FP32_ELEM = 1
+GH_PATH_AUTO = 0
+GH_PATH_GENERAL = 1
+GH_PATH_STRUCTURED = 2
+GH_PRECOND_NONE = 0
+GH_PRECOND_JACOBI = 1
+
+# Every symbol the header declares, with (restype, argtypes).
+_i32, _i64, _dbl, _vp = C.c_int32, C.c_int64, C.c_double, C.c_void_p
+_pi64 = C.POINTER(C.c_int64)
+
+
+class SolveStats(C.Structure):
+    _fields_ = [("iterations", C.c_int32), ("converged", C.c_int32),
+                ("rel_residual", C.c_double), ("device_ms", C.c_double)]
+
+
+SIGNATURES = {
+    "gh_create": (_i32, [C.POINTER(_vp), _i32]),
+    "gh_destroy": (_i32, [_vp]),
+    "gh_last_error": (C.c_char_p, [_vp]),
+    "gh_stream": (_i32, [_vp, C.POINTER(_vp)]),
+    "gh_mesh_cartesian": (_i32, [_vp, _i32, _vp, _vp, _vp]),
+    "gh_mesh_upload": (_i32, [_vp, _i32, _i64, _vp, _i64, _vp, _vp]),
+    "gh_space_upload": (_i32, [_vp, _i32, _i32, _i32, _vp, _vp, _i64, _i64, _i64]),
+    "gh_reffe_tabulation": (_i32, [_vp, _i32, _i32, _vp, _vp, _vp]),
+    "gh_symbolic": (_i32, [_vp, _pi64]),
+    "gh_pattern_get": (_i32, [_vp, _vp, _vp]),
+    "gh_coef_upload": (_i32, [_vp, _vp, _vp, _vp]),
+    "gh_assemble": (_i32, [_vp, _i32, _i32, _i32]),
+    "gh_values_get": (_i32, [_vp, _vp, _vp]),
+    "gh_numeric": (_i32, [_vp, _i32, _vp, _vp, _i32, _vp, _vp, _vp]),
+    "gh_spmv": (_i32, [_vp, _vp, _vp]),
+    "gh_pcg": (_i32, [_vp, _i32, _vp, _vp, _dbl, _i32, C.POINTER(SolveStats)]),
+}
+
+_lib = None
+
+
+class GridapHipError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("gridap_hip error %d: %s" % (code, msg))
+        self.code = code
+
+
+def lib():
+    """Load libgridap_hip.so; raise if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(
+                "libgridap_hip.so is not built (expected at %s); run "
+                "`python gridap.jl_amd/build.py` or __graft_entry__.build()" % LIB_PATH)
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def ptr(a):
+    """Raw pointer of a contiguous numpy array (None -> NULL)."""
+    if a is None:
+        return None
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class Context:
+    """One device context (gh_ctx): mesh, space, pattern and values in HBM."""
+
+    def __init__(self, device=0):
+        L = lib()
+        h = C.c_void_p()
+        st = L.gh_create(C.byref(h), int(device))
+        if st != GH_OK:
+            raise GridapHipError(st, "gh_create failed (no usable HIP device %d)" % device)
+        self.h = h
+        self.device = device
+
+    def check(self, st):
+        if st != GH_OK:
+            msg = lib().gh_last_error(self.h)
+            raise GridapHipError(st, msg.decode() if msg else "")
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().gh_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def stream(self):
+        s = C.c_void_p()
+        self.check(lib().gh_stream(self.h, C.byref(s)))
+        return s.value
+
+    # thin wrappers ---------------------------------------------------------
+    def mesh_cartesian(self, origin, sizes, partition):
+        D = len(partition)
+        o = np.ascontiguousarray(origin, dtype=np.float64)
+        s = np.ascontiguousarray(sizes, dtype=np.float64)
+        p = np.ascontiguousarray(partition, dtype=np.int64)
+        self.check(lib().gh_mesh_cartesian(self.h, D, ptr(o), ptr(s), ptr(p)))
+
+    def mesh_upload(self, xyz, cell_nodes_data, cell_nodes_ptrs):
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+        d = np.ascontiguousarray(cell_nodes_data, dtype=np.int32)
+        p = np.ascontiguousarray(cell_nodes_ptrs, dtype=np.int32)
+        self.check(lib().gh_mesh_upload(self.h, xyz.shape[1], xyz.shape[0], ptr(xyz),
+                                         len(p) - 1, ptr(d), ptr(p)))
+
+    def space_upload(self, reffe_kind, order, ncomp, data, ptrs, nfree, ndiri):
+        d = np.ascontiguousarray(data, dtype=np.int32)
+        p = np.ascontiguousarray(ptrs, dtype=np.int32)
+        self.check(lib().gh_space_upload(self.h, reffe_kind, order, ncomp, ptr(d), ptr(p),
+                                         len(p) - 1, int(nfree), int(ndiri)))
+
+    def reffe_tabulation(self, qw, shp, grad):
+        qw = np.ascontiguousarray(qw, dtype=np.float64)
+        shp = np.ascontiguousarray(shp, dtype=np.float64)
+        grad = np.ascontiguousarray(grad, dtype=np.float64)
+        self.check(lib().gh_reffe_tabulation(self.h, len(qw), shp.shape[1], ptr(qw),
+                                              ptr(shp), ptr(grad)))
+
+    def symbolic(self):
+        nnz = C.c_int64()
+        self.check(lib().gh_symbolic(self.h, C.byref(nnz)))
+        return nnz.value
+
+    def pattern_get(self, nfree, nnz):
+        colptr = np.zeros(nfree + 1, dtype=np.int64)
+        rowval = np.zeros(max(nnz, 1), dtype=np.int64)
+        self.check(lib().gh_pattern_get(self.h, ptr(colptr), ptr(rowval)))
+        return colptr, rowval[:nnz]
+
+    def coef_upload(self, kappa=None, fsrc=None, dirichlet_vals=None):
+        k = None if kappa is None else np.ascontiguousarray(kappa, dtype=np.float64)
+        f = None if fsrc is None else np.ascontiguousarray(fsrc, dtype=np.float64)
+        g = None if dirichlet_vals is None else np.ascontiguousarray(dirichlet_vals,
+                                                                     dtype=np.float64)
+        self.check(lib().gh_coef_upload(self.h, ptr(k), ptr(f), ptr(g)))
+
+    def assemble(self, form=GH_FORM_DIFFUSION, precision=GH_PREC_FP64, path=GH_PATH_AUTO):
+        self.check(lib().gh_assemble(self.h, form, precision, path))
+
+    def values_get(self, nnz, nfree):
+        nz = np.zeros(max(nnz, 1))
+        b = np.zeros(max(nfree, 1))
+        self.check(lib().gh_values_get(self.h, ptr(nz), ptr(b)))
+        return nz[:nnz], b[:nfree]
+
+    def spmv(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.zeros_like(x)
+        self.check(lib().gh_spmv(self.h, ptr(x), ptr(y)))
+        return y
+
+    def pcg(self, b=None, x0=None, n=None, precond=GH_PRECOND_JACOBI, rtol=1e-8, maxit=10000,
+            raise_on_noconv=True):
+        bb = None if b is None else np.ascontiguousarray(b, dtype=np.float64)
+        n = len(bb) if bb is not None else n
+        x = np.zeros(n) if x0 is None else np.ascontiguousarray(x0, dtype=np.float64).copy()
+        st = SolveStats()
+        code = lib().gh_pcg(self.h, precond, ptr(bb), ptr(x), float(rtol), int(maxit),
+                            C.byref(st))
+        if code != GH_OK and (code != GH_ERR_NOCONV or raise_on_noconv):
+            self.check(code)
+        return x, st

@@ -1,0 +1,262 @@
+"""SparseMatrixAssembler, assemble_matrix/vector, AffineFEOperator, solve.
+
+The Gridap entry points keep their names and meaning
+(src/FESpaces/Assemblers.jl:302-400, src/FESpaces/SparseMatrixAssemblers.jl:
+42-170, src/FESpaces/AffineFEOperators.jl:8-98, src/FESpaces/FESolvers.jl:65):
+the result is a CSC matrix with 1-based Int64 colptr/rowval (Gridap's
+SparseMatrixCSC{Float64,Int}) and a dense Float64 vector.  All numerics run on
+the device through libgridap_hip; this module only moves Gridap's data across
+the C ABI.
+"""
+import numpy as np
+
+from . import _lib
+from ._lib import Context
+from .celldata import (DiffusionTerm, DomainContribution, SourceTerm, get_fe_basis,
+                       get_trial_fe_basis)
+from .fespaces import FEFunction, TrialFESpace
+from .reference_fes import Quadrature
+
+
+class SparseMatrixCSC:
+    """Julia SparseMatrixCSC layout: m x n, 1-based colptr[n+1], rowval[nnz], nzval."""
+
+    def __init__(self, m, n, colptr, rowval, nzval):
+        self.m, self.n = m, n
+        self.colptr, self.rowval, self.nzval = colptr, rowval, nzval
+
+    @property
+    def shape(self):
+        return (self.m, self.n)
+
+    def nnz(self):
+        return len(self.rowval)
+
+    def to_scipy(self):
+        import scipy.sparse as sp
+        return sp.csc_matrix((self.nzval, self.rowval - 1, self.colptr - 1),
+                             shape=(self.m, self.n))
+
+    def __getitem__(self, ij):
+        i, j = ij  # 1-based, as in Julia
+        lo, hi = self.colptr[j - 1] - 1, self.colptr[j] - 1
+        k = lo + np.searchsorted(self.rowval[lo:hi], i)
+        return self.nzval[k] if k < hi and self.rowval[k] == i else 0.0
+
+
+def _space(U):
+    return U.space if isinstance(U, TrialFESpace) else U
+
+
+class SparseMatrixAssembler:
+    """SparseMatrixAssembler(U, V) on one device.
+
+    precision: "fp64" (element math in FP64) or "fp32" (element math in FP32,
+    nzval/b accumulated in FP64); path: "auto", "general" or "structured"."""
+
+    _PREC = {"fp64": _lib.GH_PREC_FP64, "fp32": _lib.GH_PREC_FP32_ELEM}
+    _PATH = {"auto": _lib.GH_PATH_AUTO, "general": _lib.GH_PATH_GENERAL,
+             "structured": _lib.GH_PATH_STRUCTURED}
+
+    def __init__(self, U, V, device=0, precision="fp64", path="auto"):
+        Us, Vs = _space(U), _space(V)
+        if Us is not Vs:
+            raise NotImplementedError("trial and test spaces must share one numbering")
+        self.U, self.V = U, V
+        self.space = Vs
+        self.precision = self._PREC[precision]
+        self.path = self._PATH[path]
+        self.device = device
+        self.ctx = None
+        self.nnz = None
+        self.colptr = self.rowval = None
+        self._degree = None
+
+    def get_rows(self):
+        return self.space.get_free_dof_ids()
+
+    get_cols = get_rows
+
+    # device set-up ---------------------------------------------------------
+    def _setup(self, degree):
+        V = self.space
+        model = V.model
+        if self.ctx is None:
+            ctx = Context(self.device)
+            D = model.D
+            if model.is_affine_cartesian():
+                ctx.mesh_cartesian(model.origin, model.sizes, model.partition)
+            else:
+                cn = model.cell_node_ids()
+                ptrs = (np.arange(cn.shape[0] + 1, dtype=np.int64) * cn.shape[1] + 1)
+                ctx.mesh_upload(model.node_coordinates(), cn.ravel(), ptrs.astype(np.int32))
+            cd = V.cell_dof_ids
+            ptrs = (np.arange(cd.shape[0] + 1, dtype=np.int64) * cd.shape[1] + 1)
+            ctx.space_upload(_lib.GH_REFFE_QK, V.order, V.ncomp, cd.ravel(),
+                             ptrs.astype(np.int32), V.nfree, V.ndirichlet)
+            self.nnz = ctx.symbolic()
+            self.colptr, self.rowval = ctx.pattern_get(V.nfree, self.nnz)
+            self.ctx = ctx
+            self._D = D
+        if degree != self._degree:
+            q = Quadrature(model.D, degree)
+            shp, grad = V.reffe.tabulate(model.D, q.coords)
+            self.ctx.reffe_tabulation(q.weights, shp, grad)
+            self._degree = degree
+
+    def _lower(self, matcontrib, veccontrib):
+        ncells = self.space.model.num_cells()
+        kappa, fsrc, degs = None, None, set()
+        have_mat = have_vec = False
+        if matcontrib is not None:
+            for term, meas in matcontrib.terms:
+                if not isinstance(term, DiffusionTerm):
+                    raise NotImplementedError("bilinear forms: κ ∇v⋅∇u only")
+                k = term.coef.per_cell(ncells) if term.coef is not None else None
+                k = np.ones(ncells) if k is None else k
+                kappa = k if kappa is None else kappa + k
+                degs.add(meas.degree)
+                have_mat = True
+        if veccontrib is not None:
+            for term, meas in veccontrib.terms:
+                if not isinstance(term, SourceTerm):
+                    raise NotImplementedError("linear forms: f v only")
+                f = term.coef.per_cell(ncells) if term.coef is not None else None
+                f = np.ones(ncells) if f is None else f
+                fsrc = f if fsrc is None else fsrc + f
+                degs.add(meas.degree)
+                have_vec = True
+        if len(degs) > 1:
+            raise NotImplementedError("matrix and vector terms must share one Measure")
+        if not degs:
+            raise ValueError("empty weak form")
+        if have_mat and kappa is not None and np.all(kappa == 1.0):
+            kappa = None
+        if have_vec and fsrc is not None and np.all(fsrc == 1.0):
+            fsrc = None
+        if not have_vec:
+            fsrc = np.zeros(ncells)
+        if not have_mat:
+            kappa = np.zeros(ncells)
+        return kappa, fsrc, degs.pop()
+
+    def assemble(self, matcontrib, veccontrib, lift):
+        """Device numeric phase; returns (SparseMatrixCSC, b)."""
+        kappa, fsrc, degree = self._lower(matcontrib, veccontrib)
+        self._setup(degree)
+        dvals = None
+        if lift:
+            dv = self.U.dirichlet_values() if isinstance(self.U, TrialFESpace) else None
+            if dv is not None and dv.size and np.any(dv != 0):
+                dvals = dv
+        self.ctx.coef_upload(kappa, fsrc, dvals)
+        self.ctx.assemble(_lib.GH_FORM_DIFFUSION, self.precision, self.path)
+        nz, b = self.ctx.values_get(self.nnz, self.space.nfree)
+        n = self.space.nfree
+        A = SparseMatrixCSC(n, n, self.colptr.copy(), self.rowval.copy(), nz)
+        return A, b
+
+
+def _contrib(form, *args):
+    out = form(*args) if callable(form) else form
+    if not isinstance(out, DomainContribution):
+        raise TypeError("weak form must return integrate(...) contributions")
+    return out
+
+
+def assemble_matrix(a, U, V=None, assem=None):
+    """assemble_matrix(a, U, V) or assemble_matrix(a, assem, U, V)
+    (src/FESpaces/Assemblers.jl:372-375)."""
+    if isinstance(U, SparseMatrixAssembler):
+        assem, U, V = U, V, assem
+    V = U if V is None else V
+    assem = assem or SparseMatrixAssembler(U, V)
+    m = _contrib(a, get_trial_fe_basis(U), get_fe_basis(V))
+    return assem.assemble(m, None, lift=False)[0]
+
+
+def assemble_vector(l, V, assem=None):
+    """assemble_vector(l, V) (src/FESpaces/Assemblers.jl:382-385)."""
+    if isinstance(V, SparseMatrixAssembler):
+        assem, V = V, assem
+    assem = assem or SparseMatrixAssembler(V, V)
+    vcon = _contrib(l, get_fe_basis(V))
+    return assem.assemble(None, vcon, lift=False)[1]
+
+
+def assemble_matrix_and_vector(a, l, U, V, assem=None):
+    """assemble_matrix_and_vector(a, l, U, V) with Dirichlet lifting
+    (src/FESpaces/Assemblers.jl:392-400,496-541)."""
+    assem = assem or SparseMatrixAssembler(U, V)
+    m = _contrib(a, get_trial_fe_basis(U), get_fe_basis(V))
+    vcon = _contrib(l, get_fe_basis(V))
+    return assem.assemble(m, vcon, lift=True)
+
+
+class AffineFEOperator:
+    """AffineFEOperator(a, l, U, V[, assem]) or AffineFEOperator(weakform, U, V[, assem])
+    with weakform(u, v) -> (a_contribution, l_contribution)
+    (src/FESpaces/AffineFEOperators.jl:23-54)."""
+
+    def __init__(self, *args, assem=None):
+        if len(args) >= 4 and callable(args[0]) and callable(args[1]):
+            a, l, U, V = args[:4]
+            if len(args) > 4:
+                assem = args[4]
+            weakform = lambda u, v: (a(u, v), l(v))
+        else:
+            weakform, U, V = args[:3]
+            if len(args) > 3:
+                assem = args[3]
+        self.trial, self.test = U, V
+        self.assem = assem or SparseMatrixAssembler(U, V)
+        u, v = get_trial_fe_basis(U), get_fe_basis(V)
+        mcon, vcon = weakform(u, v)
+        self.matrix, self.vector = self.assem.assemble(mcon, vcon, lift=True)
+
+    def get_matrix(self):
+        return self.matrix
+
+    def get_vector(self):
+        return self.vector
+
+    def get_trial(self):
+        return self.trial
+
+    def get_test(self):
+        return self.test
+
+
+def get_matrix(op):
+    return op.get_matrix()
+
+
+def get_vector(op):
+    return op.get_vector()
+
+
+class PCGSolver:
+    """FP64 preconditioned CG on the device (Jacobi by default); stands in for
+    Gridap's LUSolver (src/Algebra/LinearSolvers.jl:249-274)."""
+
+    def __init__(self, rtol=1e-12, maxit=100000, precond="jacobi"):
+        self.rtol = rtol
+        self.maxit = maxit
+        self.precond = {"none": _lib.GH_PRECOND_NONE,
+                        "jacobi": _lib.GH_PRECOND_JACOBI}[precond]
+        self.stats = None
+
+    def solve(self, assem, b):
+        x, st = assem.ctx.pcg(b=b, precond=self.precond, rtol=self.rtol, maxit=self.maxit)
+        self.stats = st
+        return x
+
+
+LinearFESolver = PCGSolver
+
+
+def solve(op, solver=None):
+    """uh = solve(op) (src/FESpaces/FESolvers.jl:65-69)."""
+    solver = solver or PCGSolver()
+    x = solver.solve(op.assem, op.vector)
+    return FEFunction(op.trial, x)

@@ -1,0 +1,199 @@
+"""Weak-form expressions: CellField, gradient, inner, Measure, integrate.
+
+Gridap builds lazy cell arrays from `∫(κ*∇(v)⋅∇(u))dΩ`
+(src/CellData/DomainContributions.jl:188-200, src/CellData/CellQuadratures.jl:
+138-170).  Here the same expressions are recorded symbolically and lowered to a
+form kind plus per-cell coefficients, which is what the device kernels consume:
+
+    a(u,v) = integrate(κ * inner(gradient(v), gradient(u)), dΩ)   -> diffusion, κ
+    l(v)   = integrate(f * v, dΩ)                                 -> source, f
+
+κ and f are numbers or per-cell arrays (CellField(values, Ω)).
+"""
+import numbers
+
+import numpy as np
+
+from .geometry import Triangulation
+
+
+class CellField:
+    """A field constant on each cell: CellField(values_per_cell, Ω) or a number."""
+
+    def __init__(self, values, trian=None):
+        self.values = values
+        self.trian = trian
+
+    def per_cell(self, ncells):
+        v = self.values
+        if isinstance(v, numbers.Number):
+            return None if v == 1 else np.full(ncells, float(v))
+        a = np.ascontiguousarray(v, dtype=np.float64).reshape(-1)
+        if a.shape[0] != ncells:
+            raise ValueError("CellField has %d values for %d cells" % (a.shape[0], ncells))
+        return a
+
+    def __mul__(self, other):
+        return _scale(other, self)
+
+    __rmul__ = __mul__
+
+
+def _coef_product(a, b):
+    if a is None:
+        return b
+    if b is None:
+        return a
+    va = a.values if isinstance(a, CellField) else a
+    vb = b.values if isinstance(b, CellField) else b
+    return CellField(np.asarray(va, dtype=np.float64) * np.asarray(vb, dtype=np.float64))
+
+
+class _Expr:
+    coef = None
+
+    def __mul__(self, other):
+        return _scale(other, self)
+
+    __rmul__ = __mul__
+
+    def __add__(self, other):
+        return _Sum([self, other])
+
+    __radd__ = __add__
+
+
+class FEBasis(_Expr):
+    """get_fe_basis(V) (test, v) or get_trial_fe_basis(U) (trial, u)."""
+
+    def __init__(self, space, trial):
+        self.space = space
+        self.trial = trial
+
+
+class _Grad(_Expr):
+    def __init__(self, basis):
+        self.basis = basis
+
+
+class DiffusionTerm(_Expr):
+    """coef * ∇v⋅∇u."""
+
+    def __init__(self, coef=None):
+        self.coef = coef
+
+
+class SourceTerm(_Expr):
+    """coef * v."""
+
+    def __init__(self, coef=None):
+        self.coef = coef
+
+
+class _Sum(_Expr):
+    def __init__(self, terms):
+        self.terms = terms
+
+
+def _scale(factor, expr):
+    if isinstance(factor, _Expr) and not isinstance(expr, _Expr):
+        factor, expr = expr, factor
+    if isinstance(expr, FEBasis):
+        if expr.trial:
+            raise NotImplementedError("only ∫ f v source terms are provided")
+        return SourceTerm(_as_coef(factor))
+    if isinstance(expr, (DiffusionTerm, SourceTerm)):
+        return type(expr)(_coef_product(expr.coef, _as_coef(factor)))
+    raise NotImplementedError("unsupported weak-form product")
+
+
+def _as_coef(x):
+    if isinstance(x, CellField):
+        return x
+    if isinstance(x, numbers.Number):
+        return CellField(float(x))
+    if isinstance(x, np.ndarray):
+        return CellField(x)
+    if callable(x):
+        raise NotImplementedError("coefficients must be constant per cell "
+                                  "(a number or a per-cell array)")
+    raise TypeError("unsupported coefficient %r" % (x,))
+
+
+def gradient(x):
+    if not isinstance(x, FEBasis):
+        raise NotImplementedError("gradient of an FE basis only")
+    return _Grad(x)
+
+
+grad = gradient
+
+
+def inner(a, b):
+    if isinstance(a, _Grad) and isinstance(b, _Grad) and a.basis.trial != b.basis.trial:
+        return DiffusionTerm()
+    if isinstance(a, FEBasis) and not isinstance(b, _Expr):
+        return _scale(b, a)
+    if isinstance(b, FEBasis) and not isinstance(a, _Expr):
+        return _scale(a, b)
+    raise NotImplementedError("inner/dot of these operands is not provided")
+
+
+dot = inner
+
+
+class Measure:
+    """Measure(Ω, degree): the tensor-product Gauss rule of that degree."""
+
+    def __init__(self, trian, degree):
+        if not isinstance(trian, Triangulation):
+            raise TypeError("Measure(Ω, degree) needs a Triangulation")
+        self.trian = trian
+        self.degree = int(degree)
+
+    def __rmul__(self, integrand):
+        if isinstance(integrand, Integrand):
+            return integrate(integrand.expr, self)
+        return NotImplemented
+
+
+class DomainContribution:
+    """Terms integrated over measures (src/CellData/DomainContributions.jl:7-56)."""
+
+    def __init__(self, terms):
+        self.terms = terms  # list of (term, measure)
+
+    def __add__(self, other):
+        return DomainContribution(self.terms + other.terms)
+
+    def __radd__(self, other):
+        if other == 0:
+            return self
+        return NotImplemented
+
+
+def integrate(expr, measure):
+    terms = expr.terms if isinstance(expr, _Sum) else [expr]
+    for t in terms:
+        if not isinstance(t, (DiffusionTerm, SourceTerm)):
+            raise NotImplementedError("unsupported integrand %r" % (t,))
+    return DomainContribution([(t, measure) for t in terms])
+
+
+class Integrand:
+    """integral(expr) * dΩ — the `∫(expr)dΩ` spelling."""
+
+    def __init__(self, expr):
+        self.expr = expr
+
+
+def integral(expr):
+    return Integrand(expr)
+
+
+def get_fe_basis(V):
+    return FEBasis(V, trial=False)
+
+
+def get_trial_fe_basis(U):
+    return FEBasis(U, trial=True)

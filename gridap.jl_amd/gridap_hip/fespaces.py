@@ -1,0 +1,142 @@
+"""FE spaces on the host: the DOF numbering whose cell_dof_ids the engine consumes.
+
+Q1 spaces on grids whose nodes are the vertices use Gridap's CLagrangianFESpace
+numbering (src/FESpaces/FESpaceFactories.jl:56-85 -> src/FESpaces/
+CLagrangianFESpaces.jl:155-379): node-major, components interleaved per node,
+free ids 1.. and Dirichlet ids -1.. in node order; local dofs are
+component-blocked (ldof = lnode + nnodes_cell*(comp-1),
+src/ReferenceFEs/LagrangianDofBases.jl:77-96).
+"""
+import numpy as np
+
+from .geometry import CartesianDiscreteModel, Triangulation
+from .reference_fes import LagrangianRefFE
+
+
+class FESpace:
+    def __init__(self, model, reffe, conformity="H1", dirichlet_tags=(), dirichlet_masks=None,
+                 trian=None):
+        if isinstance(model, Triangulation):
+            model = model.model
+        if not isinstance(model, CartesianDiscreteModel):
+            raise TypeError("FESpace needs a CartesianDiscreteModel")
+        if not isinstance(reffe, LagrangianRefFE):
+            raise TypeError("FESpace needs a ReferenceFE(lagrangian, T, order)")
+        if conformity not in ("H1", ":H1"):
+            raise NotImplementedError("only H1-conforming spaces are provided")
+        if reffe.order != 1:
+            raise NotImplementedError("conforming spaces of order > 1 are not provided yet")
+        self.model = model
+        self.reffe = reffe
+        self.order = reffe.order
+        self.ncomp = reffe.ncomp
+        if isinstance(dirichlet_tags, (str, int, np.integer)):
+            dirichlet_tags = [dirichlet_tags]
+        self.dirichlet_tags = list(dirichlet_tags)
+        labels = model.get_face_labeling()
+        node_to_tag = labels.face_tag_index(self.dirichlet_tags, 0)
+        ntags = len(self.dirichlet_tags)
+        if dirichlet_masks is None:
+            masks = np.ones((max(ntags, 1), self.ncomp), dtype=bool)
+        else:
+            masks = np.array(dirichlet_masks, dtype=bool).reshape(ntags, self.ncomp)
+        self._number(node_to_tag, masks)
+
+    def _number(self, node_to_tag, masks):
+        nn, nc = len(node_to_tag), self.ncomp
+        tag = node_to_tag.astype(np.int64)
+        is_d = np.zeros((nn, nc), dtype=bool)
+        has = tag > 0
+        is_d[has] = masks[tag[has] - 1]
+        flat = is_d.ravel()  # node-major, component-minor
+        ids = np.empty(flat.shape, dtype=np.int64)
+        ids[~flat] = np.arange(1, np.count_nonzero(~flat) + 1)
+        ids[flat] = -np.arange(1, np.count_nonzero(flat) + 1)
+        self.node_and_comp_to_dof = ids.reshape(nn, nc).astype(np.int32)
+        self.nfree = int(np.count_nonzero(~flat))
+        self.ndirichlet = int(np.count_nonzero(flat))
+        node_of = np.repeat(np.arange(1, nn + 1, dtype=np.int32), nc)
+        comp_of = np.tile(np.arange(1, nc + 1, dtype=np.int16), nn)
+        self.free_dof_to_node = node_of[~flat]
+        self.free_dof_to_comp = comp_of[~flat]
+        self.dirichlet_dof_to_node = node_of[flat]
+        self.dirichlet_dof_to_comp = comp_of[flat]
+        cn = self.model.cell_node_ids() - 1
+        n2d = self.node_and_comp_to_dof
+        if nc == 1:
+            cd = n2d[cn, 0]
+        else:
+            cd = np.concatenate([n2d[cn, k] for k in range(nc)], axis=1)
+        self.cell_dof_ids = np.ascontiguousarray(cd, dtype=np.int32)
+
+    # Gridap accessors --------------------------------------------------------
+    def get_cell_dof_ids(self):
+        return self.cell_dof_ids
+
+    def num_free_dofs(self):
+        return self.nfree
+
+    def num_dirichlet_dofs(self):
+        return self.ndirichlet
+
+    def get_free_dof_ids(self):
+        return range(1, self.nfree + 1)
+
+    def get_triangulation(self):
+        return Triangulation(self.model)
+
+    def dirichlet_values(self):
+        return np.zeros(self.ndirichlet)
+
+
+def TestFESpace(model, reffe, **kwargs):
+    return FESpace(model, reffe, **kwargs)
+
+
+class TrialFESpace:
+    """TrialFESpace(V, g): V with Dirichlet values g evaluated at the Dirichlet
+    nodes (Lagrangian dofs are point evaluations)."""
+
+    def __init__(self, V, g=None):
+        self.space = V
+        self.g = g
+        vals = np.zeros(V.ndirichlet)
+        if g is not None and V.ndirichlet:
+            xyz = V.model.node_coordinates()
+            nodes = V.dirichlet_dof_to_node - 1
+            if callable(g):
+                gv = np.asarray(g(xyz[nodes]), dtype=np.float64)
+                if gv.ndim > 1:
+                    gv = gv[np.arange(len(nodes)), V.dirichlet_dof_to_comp - 1]
+                vals = gv.reshape(-1)
+            else:
+                vals[:] = float(g)
+        self._dvals = vals
+
+    def __getattr__(self, name):
+        return getattr(self.space, name)
+
+    def dirichlet_values(self):
+        return self._dvals
+
+
+class FEFunction:
+    """Free and Dirichlet dof values of a function of a (trial) space."""
+
+    def __init__(self, space, free_values, dirichlet_values=None):
+        self.space = space
+        self.free_values = np.asarray(free_values)
+        self.dirichlet_values = (space.dirichlet_values() if dirichlet_values is None
+                                 else np.asarray(dirichlet_values))
+
+    def get_free_dof_values(self):
+        return self.free_values
+
+    def nodal_values(self):
+        """Values at the nodes (scalar Q1): free dofs and Dirichlet data merged."""
+        V = self.space
+        n2d = V.node_and_comp_to_dof[:, 0].astype(np.int64)
+        out = np.where(n2d > 0, self.free_values[np.maximum(n2d, 1) - 1],
+                       self.dirichlet_values[np.maximum(-n2d, 1) - 1] if V.ndirichlet
+                       else 0.0)
+        return out

@@ -1,0 +1,168 @@
+"""Discrete models on the host: Gridap's CartesianDiscreteModel and its labels.
+
+Mirrors src/Geometry/CartesianDiscreteModels.jl and src/Geometry/CartesianGrids.jl:
+node ids over LinearIndices(partition .+ 1) (x fastest), cell nodes over
+CartesianIndices((2,...,2)) from the cell's lower corner, and the face labeling
+whose tags are "tag_XX" (one per reference n-cube face), "interior" and
+"boundary".  Only what the assembly path needs is kept on the host; the device
+evaluates the CartesianMap analytically.
+"""
+import numpy as np
+
+
+def ncube_num_faces(D):
+    return 3 ** D
+
+
+def ncube_face_id(D, free_mask, anchor_bits):
+    """1-based id of the reference n-cube face with free axes `free_mask` and
+    fixed coordinates `anchor_bits`, in the face order of
+    src/ReferenceFEs/ExtrusionPolytopes.jl:460-478 (dimension, then extrusion, then
+    anchor, the last axis most significant)."""
+    from math import comb
+    d = bin(free_mask).count("1")
+    offset = sum(comb(D, k) << (D - k) for k in range(d))
+    rank = sum(1 for m in range(free_mask) if bin(m).count("1") == d)
+    a, bit = 0, 0
+    for ax in range(D):
+        if free_mask >> ax & 1:
+            continue
+        if anchor_bits >> ax & 1:
+            a |= 1 << bit
+        bit += 1
+    return offset + rank * (1 << (D - d)) + a + 1
+
+
+class FaceLabeling:
+    """Vertex entities and tags of a Cartesian model (_fill_cartesian_face_labeling!,
+    src/Geometry/CartesianDiscreteModels.jl:137-170)."""
+
+    def __init__(self, D, vertex_entity):
+        self.D = D
+        self.vertex_entity = vertex_entity  # int32 per vertex
+        nf = ncube_num_faces(D)
+        width = len(str(nf))
+        self.tag_names = ["tag_%s" % str(i).zfill(width) for i in range(1, nf)]
+        self.tag_names += ["interior", "boundary"]
+        self.tag_entities = [[i] for i in range(1, nf)] + [[nf], list(range(1, nf))]
+
+    def tag_to_entities(self, tag):
+        if isinstance(tag, str):
+            if tag not in self.tag_names:
+                raise KeyError("unknown tag %r" % tag)
+            return self.tag_entities[self.tag_names.index(tag)]
+        t = int(tag)
+        if not 1 <= t <= len(self.tag_entities):
+            raise KeyError("unknown tag id %r" % tag)
+        return self.tag_entities[t - 1]
+
+    def face_tag_index(self, tags, d=0):
+        """get_face_tag_index (src/Geometry/FaceLabelings.jl:411-427) for vertices:
+        position (1-based) of the first tag holding the vertex's entity, 0 if none."""
+        if d != 0:
+            raise NotImplementedError("only vertex tags are needed by Q1 spaces")
+        if isinstance(tags, (str, int, np.integer)):
+            tags = [tags]
+        out = np.zeros(len(self.vertex_entity), dtype=np.int8)
+        for i, tag in enumerate(tags):
+            ents = np.asarray(self.tag_to_entities(tag), dtype=np.int32)
+            hit = np.isin(self.vertex_entity, ents) & (out == 0)
+            out[hit] = i + 1
+        return out
+
+
+class CartesianDiscreteModel:
+    """CartesianDiscreteModel(domain, partition; map=identity).
+
+    domain = (x0, x1, y0, y1[, z0, z1]); partition = (nx, ny[, nz]).  A non-identity
+    `map` moves the node coordinates (src/Geometry/CartesianGrids.jl:107-124); the
+    cell map is then isoparametric instead of the analytic CartesianMap."""
+
+    def __init__(self, domain, partition, map=None):
+        D = len(partition)
+        if D not in (2, 3) or len(domain) != 2 * D:
+            raise ValueError("domain/partition must describe a 2-D or 3-D box")
+        self.D = D
+        self.partition = tuple(int(n) for n in partition)
+        if min(self.partition) < 1:
+            raise ValueError("partition entries must be >= 1")
+        self.origin = np.array([float(domain[2 * d]) for d in range(D)])
+        self.sizes = np.array([(float(domain[2 * d + 1]) - float(domain[2 * d])) / partition[d]
+                               for d in range(D)])
+        self.map = map
+        self._xyz = None
+        self._cell_nodes = None
+        self._labels = None
+
+    # sizes ----------------------------------------------------------------
+    def num_cells(self):
+        return int(np.prod(self.partition))
+
+    def num_nodes(self):
+        return int(np.prod([n + 1 for n in self.partition]))
+
+    num_vertices = num_nodes
+
+    def is_affine_cartesian(self):
+        return self.map is None
+
+    # geometry ---------------------------------------------------------------
+    def node_coordinates(self):
+        if self._xyz is None:
+            axes = [self.origin[d] + np.arange(self.partition[d] + 1) * self.sizes[d]
+                    for d in range(self.D)]
+            grids = np.meshgrid(*axes, indexing="ij")
+            xyz = np.stack([g.ravel(order="F") for g in grids], axis=1)
+            if self.map is not None:
+                xyz = np.ascontiguousarray(np.asarray(self.map(xyz), dtype=np.float64))
+            self._xyz = xyz
+        return self._xyz
+
+    def cell_node_ids(self):
+        """1-based Int32 cell_node_ids, cells x 2^D (CartesianCellNodes)."""
+        if self._cell_nodes is None:
+            D, part = self.D, self.partition
+            ci = np.indices(part).reshape(D, -1, order="F")  # cell index per axis
+            nodes = np.zeros((ci.shape[1], 1 << D), dtype=np.int64)
+            for a in range(1 << D):
+                node = np.zeros(ci.shape[1], dtype=np.int64)
+                stride = 1
+                for d in range(D):
+                    node += (ci[d] + ((a >> d) & 1)) * stride
+                    stride *= part[d] + 1
+                nodes[:, a] = node + 1
+            self._cell_nodes = nodes.astype(np.int32)
+        return self._cell_nodes
+
+    # labels -------------------------------------------------------------------
+    def get_face_labeling(self):
+        if self._labels is None:
+            D, part = self.D, self.partition
+            idx = np.indices([n + 1 for n in part]).reshape(D, -1, order="F")
+            free = np.zeros(idx.shape[1], dtype=np.int64)
+            anchor = np.zeros(idx.shape[1], dtype=np.int64)
+            for d in range(D):
+                free |= ((idx[d] > 0) & (idx[d] < part[d])).astype(np.int64) << d
+                anchor |= (idx[d] == part[d]).astype(np.int64) << d
+            table = np.zeros((1 << D, 1 << D), dtype=np.int32)
+            for m in range(1 << D):
+                for a in range(1 << D):
+                    table[m, a] = ncube_face_id(D, m, a & ~m)
+            ent = table[free, anchor & ~free]
+            ent[free == (1 << D) - 1] = ncube_num_faces(D)
+            self._labels = FaceLabeling(D, ent.astype(np.int32))
+        return self._labels
+
+
+class Triangulation:
+    """Triangulation(model): the cells of the model (volume integrals only)."""
+
+    def __init__(self, model):
+        self.model = model
+
+    def num_cells(self):
+        return self.model.num_cells()
+
+
+def get_triangulation(model):
+    return Triangulation(model)

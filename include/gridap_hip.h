@@ -1,0 +1,145 @@
+/*
+ * gridap_hip.h -- C ABI of the MI355X finite-element assembly engine.
+ *
+ * One opaque context per device holds the mesh, the FE space (cell dof ids), the
+ * CSC pattern and the assembled values in HBM.  The host (Gridap's Julia side
+ * through `ccall`, or any FFI) hands over Gridap's own data and gets back a
+ * SparseMatrixCSC{Float64,Int64} (1-based colptr/rowval) and a Vector{Float64}.
+ *
+ * Conventions (Gridap's): cell tables are Table{Int32} (data + 1-based ptrs,
+ * src/Arrays/Tables.jl:23-30); dof ids are 1-based, free > 0, Dirichlet < 0
+ * (src/FESpaces/CLagrangianFESpaces.jl:155-207); matrices are CSC with 1-based
+ * Int64 colptr/rowval (src/FESpaces/SparseMatrixAssemblers.jl:155-160).
+ *
+ * Every entry returns GH_OK (0) or an error code; gh_last_error() gives the
+ * message.  No C++ exception crosses the ABI.  Host pointers are read or written
+ * only during the call.  Calls on one context are serialised by the caller
+ * (Gridap assembles from one task); the library uses one HIP stream per context.
+ */
+#ifndef GRIDAP_HIP_H
+#define GRIDAP_HIP_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---- */
+#define GH_OK 0
+#define GH_ERR_ARG 1         /* bad argument / size mismatch (the @assert checks of
+                                src/FESpaces/SparseMatrixAssemblers.jl:219-220)       */
+#define GH_ERR_RANGE 2       /* dof id out of range                                   */
+#define GH_ERR_HIP 3         /* HIP runtime error                                     */
+#define GH_ERR_OOM 4         /* device allocation failed                              */
+#define GH_ERR_STATE 5       /* call out of order (e.g. numeric before symbolic)      */
+#define GH_ERR_NOCONV 6      /* iterative solver did not reach the tolerance          */
+#define GH_ERR_UNSUPPORTED 7 /* reference element / form not provided by the engine  */
+
+/* ---- reference elements (Lagrangian, src/ReferenceFEs/CLagrangianRefFEs.jl) ---- */
+#define GH_REFFE_QK 1 /* n-cube Q_k (QUAD/HEX), k = order */
+#define GH_REFFE_PK 2 /* simplex P_k (TRI/TET)             */
+
+/* ---- weak forms ---- */
+#define GH_FORM_DIFFUSION 1 /* a(u,v)=∫ κ ∇v⋅∇u dΩ,  l(v)=∫ f v dΩ, κ and f constant per cell */
+
+/* ---- element precision ---- */
+#define GH_PREC_FP64 0      /* element arithmetic and accumulation in FP64           */
+#define GH_PREC_FP32_ELEM 1 /* element arithmetic in FP32, nzval/b accumulated FP64  */
+
+/* ---- numeric kernel choice ---- */
+#define GH_PATH_AUTO 0       /* owner-computes lattice kernel when the space allows it */
+#define GH_PATH_GENERAL 1    /* per-cell element kernel + coloured deterministic scatter */
+#define GH_PATH_STRUCTURED 2 /* owner-computes lattice kernel (error if not eligible)  */
+
+/* ---- preconditioners ---- */
+#define GH_PRECOND_NONE 0
+#define GH_PRECOND_JACOBI 1
+
+typedef struct gh_ctx gh_ctx;
+
+typedef struct gh_solve_stats {
+  int32_t iterations;
+  int32_t converged;   /* 1 when ||b - A x|| / ||b|| <= rtol (true residual, FP64) */
+  double rel_residual; /* true residual at exit                                  */
+  double device_ms;    /* solver wall time on the device stream                  */
+} gh_solve_stats;
+
+/* Context.  Replaces nothing in Gridap (it is the state a GPU assembler holds). */
+int gh_create(gh_ctx **ctx, int device);
+int gh_destroy(gh_ctx *ctx);
+const char *gh_last_error(const gh_ctx *ctx);
+/* The HIP stream (hipStream_t) every kernel of this context runs on. */
+int gh_stream(gh_ctx *ctx, void **stream);
+
+/* Mesh from a CartesianDescriptor (origin, sizes, partition): analytic node
+ * coordinates, CartesianCellNodes and CartesianMap Jacobian.
+ * Replaces CartesianGrid/CartesianMap evaluation, src/Geometry/CartesianGrids.jl:107-283. */
+int gh_mesh_cartesian(gh_ctx *ctx, int dim, const double *origin, const double *sizes,
+                      const int64_t *partition);
+
+/* Unstructured mesh: node coordinates (dim x nnodes, Julia Point layout, i.e. the
+ * coordinates of node n at xyz[dim*n ..]) and cell_node_ids Table (1-based).
+ * Replaces UnstructuredGrid's cell map, src/Geometry/UnstructuredGrids.jl:52-100. */
+int gh_mesh_upload(gh_ctx *ctx, int dim, int64_t nnodes, const double *xyz, int64_t ncells,
+                   const int32_t *cell_nodes_data, const int32_t *cell_nodes_ptrs);
+
+/* FE space: get_cell_dof_ids(V) as a Table{Int32} plus num_free_dofs and
+ * num_dirichlet_dofs.  Replaces the rows/cols inputs of the cell loops,
+ * src/FESpaces/SparseMatrixAssemblers.jl:174-248 (cell_rows = cell_cols, U = V). */
+int gh_space_upload(gh_ctx *ctx, int reffe_kind, int order, int ncomp,
+                    const int32_t *cell_dofs_data, const int32_t *cell_dofs_ptrs,
+                    int64_t ncells, int64_t nfree, int64_t ndiri);
+
+/* Reference tabulation, computed once per reference cell by the host:
+ * quadrature weights qw[nq], shape values shp[nq*nloc] and reference gradients
+ * grad[nq*nloc*dim] (point-major).  Replaces the per-cell-type evaluation of
+ * src/CellData/CellQuadratures.jl:138-160 with src/Arrays/CompressedArrays.jl:66-94. */
+int gh_reffe_tabulation(gh_ctx *ctx, int nq, int nloc, const double *qw, const double *shp,
+                        const double *grad);
+
+/* Symbolic phase: counter -> allocation -> sorted unique rows per column, built on
+ * the device.  nnz_out = stored entries of create_from_nz.
+ * Replaces symbolic_loop_matrix! + nz_allocation + create_from_nz pattern,
+ * src/FESpaces/SparseMatrixAssemblers.jl:174-210, src/Algebra/SparseMatrixCSC.jl:214-283. */
+int gh_symbolic(gh_ctx *ctx, int64_t *nnz_out);
+
+/* 1-based Int64 colptr[nfree+1] and rowval[nnz], bit-identical to the
+ * SparseMatrixCSC that create_from_nz returns. */
+int gh_pattern_get(gh_ctx *ctx, int64_t *colptr, int64_t *rowval);
+
+/* Per-cell coefficients resident on the device: kappa[ncells] (NULL = 1) and
+ * f[ncells] (NULL = 1) for GH_FORM_DIFFUSION; Dirichlet values[ndiri] of uhd
+ * (NULL = homogeneous).  These are the CellField inputs of the weak form. */
+int gh_coef_upload(gh_ctx *ctx, const double *kappa, const double *fsrc,
+                   const double *dirichlet_vals);
+
+/* Numeric phase on device-resident inputs: nzval and b in HBM, the matrix part of
+ * assemble_matrix_and_vector! with attach_dirichlet lifting.  The work is enqueued
+ * on the context stream; gh_values_get / gh_spmv / gh_pcg wait for it.
+ * Replaces numeric_loop_matrix_and_vector! + InserterCSC.add_entry!,
+ * src/FESpaces/SparseMatrixAssemblers.jl:365-405, src/Algebra/SparseMatrixCSC.jl:124-150,
+ * src/CellData/AttachDirichlet.jl:59-84. */
+int gh_assemble(gh_ctx *ctx, int form_kind, int precision, int path);
+
+/* Copy the assembled nzval[nnz] and b[nfree] to the host (either may be NULL). */
+int gh_values_get(gh_ctx *ctx, double *nzval, double *b);
+
+/* One-call numeric phase from host arrays: gh_coef_upload + gh_assemble(AUTO) +
+ * gh_values_get.  qdeg must match the uploaded tabulation's rule (checked by nq). */
+int gh_numeric(gh_ctx *ctx, int form_kind, const double *cell_coef, const double *cell_source,
+               int precision, double *nzval, double *b, const double *dirichlet_vals);
+
+/* y = A x with the assembled matrix (host vectors of length nfree).
+ * Replaces the SpMV of residual!, src/Algebra/LinearSolvers.jl:36-58. */
+int gh_spmv(gh_ctx *ctx, const double *x, double *y);
+
+/* Preconditioned CG in FP64 on the assembled (SPD) matrix; b NULL = the assembled
+ * b.  x holds the initial guess on entry and the solution on exit.
+ * Stands in for solve!(x, LUSolver(), op), src/Algebra/LinearSolvers.jl:202-274. */
+int gh_pcg(gh_ctx *ctx, int precond, const double *b, double *x, double rtol, int maxit,
+           gh_solve_stats *stats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,108 @@
+"""Host side of the drop-in (no GPU): the Gridap-API mirror produces the same
+model, labels, numbering and tabulation as the oracle, and the C-ABI library
+loads and exports every entry point of include/gridap_hip.h."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import gridap_hip as G
+from gridap_hip import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+@pytest.mark.parametrize("part", [(2, 2), (5, 3), (3, 4, 2), (6, 5, 7)])
+@pytest.mark.parametrize("tags", [["boundary"], [], [1, 2, 4, 5, 8], ["tag_25", "tag_21"]])
+def test_numbering_matches_oracle(oracle, part, tags):
+    D = len(part)
+    if D == 2 and any(isinstance(t, str) and t.startswith("tag_2") for t in tags):
+        pytest.skip("3-D tag names")
+    if D == 3 and tags == [1, 2, 4, 5, 8]:
+        tags = [1, 2, 4, 5, 8, 13, 22]
+    dom = tuple(v for d in range(D) for v in (0.0, 1.0 + d))
+    model = G.CartesianDiscreteModel(dom, part)
+    om = oracle.cartesian_model(dom, part)
+    np.testing.assert_array_equal(model.cell_node_ids(), om["cell_nodes"])
+    np.testing.assert_allclose(model.node_coordinates(), om["xyz"], rtol=0, atol=1e-15)
+    np.testing.assert_array_equal(model.get_face_labeling().vertex_entity, om["vertex_labels"])
+    V = G.TestFESpace(model, G.ReferenceFE(G.lagrangian, G.Float64, 1), dirichlet_tags=tags)
+    ents = [oracle.cartesian_tag_entities(D, t) for t in tags]
+    n2d, nf, nd = oracle.clagrangian(oracle.face_tag_index(om["vertex_labels"], ents))
+    cd = oracle.clagrangian_cell_dofs(om["cell_nodes"], n2d)
+    np.testing.assert_array_equal(V.get_cell_dof_ids(), cd)
+    assert (V.num_free_dofs(), V.num_dirichlet_dofs()) == (nf, nd)
+
+
+def test_vector_numbering_fixture():
+    """test/FESpacesTests/CLagrangianFESpacesTests.jl:103-110 through the host API."""
+    model = G.CartesianDiscreteModel((0, 1, 0, 1), (2, 2))
+    reffe = G.ReferenceFE(G.lagrangian, G.VectorValue[2, G.Float64], 1)
+    V = G.FESpace(model, reffe, dirichlet_tags=[1, 2, 4, 5, 8],
+                  dirichlet_masks=[(True, True), (True, False), (False, False), (False, True),
+                                   (True, True)])
+    assert [tuple(r) for r in V.node_and_comp_to_dof] == [
+        (-1, -2), (1, -3), (-4, 2), (3, 4), (5, 6), (-5, -6), (7, 8), (9, 10), (11, 12)]
+    assert V.free_dof_to_node.tolist() == [2, 3, 4, 4, 5, 5, 7, 7, 8, 8, 9, 9]
+    assert V.dirichlet_dof_to_comp.tolist() == [1, 2, 2, 1, 1, 2]
+
+
+@pytest.mark.parametrize("D,degree", [(2, 2), (3, 2), (3, 4), (2, 5)])
+def test_tabulation_matches_oracle(oracle, D, degree):
+    q = G.Quadrature(D, degree)
+    x, w = oracle.tensor_quadrature(D, degree)
+    np.testing.assert_allclose(q.coords, x, rtol=0, atol=2e-16)
+    np.testing.assert_allclose(q.weights, w, rtol=0, atol=4e-15)
+    for order in (1, 2):
+        reffe = G.ReferenceFE(G.lagrangian, G.Float64, order)
+        np.testing.assert_array_equal(reffe.nodes(D), oracle.qk_nodes(D, order))
+        shp, grad = reffe.tabulate(D, q.coords)
+        oshp, ograd = oracle.qk_tabulate(D, order, q.coords)
+        np.testing.assert_allclose(shp, oshp, rtol=0, atol=1e-15)
+        np.testing.assert_allclose(grad, ograd, rtol=0, atol=1e-14)
+
+
+def test_weak_form_lowering():
+    model = G.CartesianDiscreteModel((0, 1, 0, 1), (2, 3))
+    Ω = G.Triangulation(model)
+    dΩ = G.Measure(Ω, 2)
+    V = G.TestFESpace(model, G.ReferenceFE(G.lagrangian, G.Float64, 1),
+                      dirichlet_tags="boundary")
+    u, v = G.get_trial_fe_basis(V), G.get_fe_basis(V)
+    k = np.arange(1.0, 7.0)
+    c = G.integrate(G.CellField(k, Ω) * G.inner(G.gradient(v), G.gradient(u)), dΩ)
+    assert len(c.terms) == 1 and np.array_equal(c.terms[0][0].coef.values, k)
+    c2 = G.integral(G.dot(G.gradient(v), G.gradient(u)) * 2.0) * dΩ
+    assert c2.terms[0][0].coef.values == 2.0
+    s = G.integrate(v * 3.0, dΩ) + G.integrate(1.0 * v, dΩ)
+    assert len(s.terms) == 2
+    with pytest.raises(NotImplementedError):
+        G.integrate(v * (lambda x: x[0]), dΩ)
+    with pytest.raises(NotImplementedError):
+        G.FESpace(model, G.ReferenceFE(G.lagrangian, G.Float64, 2))
+
+
+def _header_symbols():
+    text = open(os.path.join(ROOT, "include", "gridap_hip.h")).read()
+    return sorted(set(re.findall(r"^(?:int|const char \*)\s*\*?\s*(gh_\w+)\(", text, re.M)))
+
+
+def test_abi_library_exports_every_header_symbol():
+    syms = _header_symbols()
+    assert len(syms) >= 16
+    L = ctypes.CDLL(_lib.LIB_PATH)
+    for s in syms:
+        assert hasattr(L, s), "missing export %s" % s
+    # the Python binding declares exactly the header's entry points
+    assert sorted(_lib.SIGNATURES) == syms
+
+
+def test_product_does_not_reach_the_oracle():
+    pkg = os.path.join(ROOT, "gridap.jl_amd")
+    for dirpath, _, files in os.walk(pkg):
+        for f in files:
+            if f.endswith((".py", ".hip", ".h", ".cpp")):
+                text = open(os.path.join(dirpath, f)).read()
+                assert "gridap_oracle" not in text and "import oracle" not in text, f
