@@ -39,6 +39,10 @@ class HipEvents:
         self.L.hipEventSynchronize.argtypes = [C.c_void_p]
         self.L.hipEventElapsedTime.argtypes = [C.POINTER(C.c_float), C.c_void_p, C.c_void_p]
         self.L.hipEventDestroy.argtypes = [C.c_void_p]
+        self.L.hipStreamSynchronize.argtypes = [C.c_void_p]
+
+    def sync(self, stream):
+        assert self.L.hipStreamSynchronize(C.c_void_p(stream)) == 0
 
     def create(self, n):
         evs = []
@@ -125,7 +129,7 @@ def main():
 
     for _ in range(args.warmup):
         ctx.assemble(_lib.GH_FORM_DIFFUSION, prec, path)
-    ctx.values_get(0, 0)  # waits for the stream
+    ev.sync(stream)
 
     # Timed region: K back-to-back launches on the library stream, an event before
     # each, so consecutive event pairs bracket one launch each.
@@ -142,9 +146,9 @@ def main():
     ms_per_step = total_ms / args.steps
     kernel_ms = float(np.mean(per_launch))
 
-    nzval, b = ctx.values_get(nnz, N)
+    nzval, b = ctx.values_get()
     assert np.all(np.isfinite(nzval)) and np.all(np.isfinite(b))
-    colptr, rowval = ctx.pattern_get(N, nnz)
+    colptr, rowval = ctx.pattern_get()
 
     algo_bytes = 8 * nnz + 4 * 8 * ncells + 8 * ncells + 8 * N
     compulsory_bytes = 8 * nnz + 4 * model.num_nodes() + 8 * ncells + 8 * N + 4 * (N + 1)
@@ -189,7 +193,7 @@ def main():
     }
 
     if not args.no_pcg:
-        x, st = ctx.pcg(b=None, n=N, precond=_lib.GH_PRECOND_JACOBI, rtol=1e-8, maxit=20000,
+        x, st = ctx.pcg(b=None, precond=_lib.GH_PRECOND_JACOBI, rtol=1e-8, maxit=20000,
                         raise_on_noconv=False)
         out["pcg_jacobi"] = {"iterations": int(st.iterations), "converged": int(st.converged),
                              "rel_residual": float(st.rel_residual),

@@ -106,6 +106,8 @@ class Context:
             raise GridapHipError(st, "gh_create failed (no usable HIP device %d)" % device)
         self.h = h
         self.device = device
+        self.nfree = 0  # sizes of the arrays the library returns, tracked here so the
+        self.nnz = 0    # host buffers always match what the library writes
 
     def check(self, st):
         if st != GH_OK:
@@ -148,6 +150,8 @@ class Context:
         p = np.ascontiguousarray(ptrs, dtype=np.int32)
         self.check(lib().gh_space_upload(self.h, reffe_kind, order, ncomp, ptr(d), ptr(p),
                                          len(p) - 1, int(nfree), int(ndiri)))
+        self.nfree = int(nfree)
+        self.nnz = 0
 
     def reffe_tabulation(self, qw, shp, grad):
         qw = np.ascontiguousarray(qw, dtype=np.float64)
@@ -159,13 +163,14 @@ class Context:
     def symbolic(self):
         nnz = C.c_int64()
         self.check(lib().gh_symbolic(self.h, C.byref(nnz)))
+        self.nnz = nnz.value
         return nnz.value
 
-    def pattern_get(self, nfree, nnz):
-        colptr = np.zeros(nfree + 1, dtype=np.int64)
-        rowval = np.zeros(max(nnz, 1), dtype=np.int64)
+    def pattern_get(self):
+        colptr = np.zeros(self.nfree + 1, dtype=np.int64)
+        rowval = np.zeros(max(self.nnz, 1), dtype=np.int64)
         self.check(lib().gh_pattern_get(self.h, ptr(colptr), ptr(rowval)))
-        return colptr, rowval[:nnz]
+        return colptr, rowval[:self.nnz]
 
     def coef_upload(self, kappa=None, fsrc=None, dirichlet_vals=None):
         k = None if kappa is None else np.ascontiguousarray(kappa, dtype=np.float64)
@@ -177,23 +182,29 @@ class Context:
     def assemble(self, form=GH_FORM_DIFFUSION, precision=GH_PREC_FP64, path=GH_PATH_AUTO):
         self.check(lib().gh_assemble(self.h, form, precision, path))
 
-    def values_get(self, nnz, nfree):
-        nz = np.zeros(max(nnz, 1))
-        b = np.zeros(max(nfree, 1))
+    def values_get(self):
+        nz = np.zeros(max(self.nnz, 1))
+        b = np.zeros(max(self.nfree, 1))
         self.check(lib().gh_values_get(self.h, ptr(nz), ptr(b)))
-        return nz[:nnz], b[:nfree]
+        return nz[:self.nnz], b[:self.nfree]
 
     def spmv(self, x):
         x = np.ascontiguousarray(x, dtype=np.float64)
+        if x.shape != (self.nfree,):
+            raise ValueError("x must have num_free_dofs entries")
         y = np.zeros_like(x)
         self.check(lib().gh_spmv(self.h, ptr(x), ptr(y)))
         return y
 
-    def pcg(self, b=None, x0=None, n=None, precond=GH_PRECOND_JACOBI, rtol=1e-8, maxit=10000,
+    def pcg(self, b=None, x0=None, precond=GH_PRECOND_JACOBI, rtol=1e-8, maxit=10000,
             raise_on_noconv=True):
         bb = None if b is None else np.ascontiguousarray(b, dtype=np.float64)
-        n = len(bb) if bb is not None else n
+        n = self.nfree
+        if bb is not None and bb.shape != (n,):
+            raise ValueError("b must have num_free_dofs entries")
         x = np.zeros(n) if x0 is None else np.ascontiguousarray(x0, dtype=np.float64).copy()
+        if x.shape != (n,):
+            raise ValueError("x0 must have num_free_dofs entries")
         st = SolveStats()
         code = lib().gh_pcg(self.h, precond, ptr(bb), ptr(x), float(rtol), int(maxit),
                             C.byref(st))

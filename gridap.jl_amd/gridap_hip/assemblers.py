@@ -95,7 +95,7 @@ class SparseMatrixAssembler:
             ctx.space_upload(_lib.GH_REFFE_QK, V.order, V.ncomp, cd.ravel(),
                              ptrs.astype(np.int32), V.nfree, V.ndirichlet)
             self.nnz = ctx.symbolic()
-            self.colptr, self.rowval = ctx.pattern_get(V.nfree, self.nnz)
+            self.colptr, self.rowval = ctx.pattern_get()
             self.ctx = ctx
             self._D = D
         if degree != self._degree:
@@ -151,7 +151,7 @@ class SparseMatrixAssembler:
                 dvals = dv
         self.ctx.coef_upload(kappa, fsrc, dvals)
         self.ctx.assemble(_lib.GH_FORM_DIFFUSION, self.precision, self.path)
-        nz, b = self.ctx.values_get(self.nnz, self.space.nfree)
+        nz, b = self.ctx.values_get()
         n = self.space.nfree
         A = SparseMatrixCSC(n, n, self.colptr.copy(), self.rowval.copy(), nz)
         return A, b
