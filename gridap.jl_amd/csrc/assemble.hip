@@ -34,20 +34,20 @@ namespace {
 // ---------------------------------------------------------------------------
 
 struct LatticeArgs {
-  int nn[3];   // lattice nodes per axis
-  int nc[3];   // cells per axis
-  int64_t nnodes;
+  int nn[3];       // lattice nodes per axis
+  int nc[3];       // cells per axis
+  int nnodes;
+  double inv_nn0, inv_nn1;  // reciprocals for the node-index split
   const int32_t *node_dof;
+  const uint32_t *fmask;    // free stencil neighbours per node (symbolic phase)
   const int32_t *colptr;
-  const double *kappa;  // per cell or nullptr (κ = 1)
-  const double *fsrc;   // per cell or nullptr (f = 1)
-  const double *dvals;  // Dirichlet values or nullptr (homogeneous)
+  const double *kappa;      // per cell or nullptr (κ = 1)
+  const double *fsrc;       // per cell or nullptr (f = 1)
+  const double *dvals;      // Dirichlet values or nullptr (homogeneous)
   double *nzval;
   double *b;
-  double Kd[64];  // reference-cell stiffness for κ=1, [a*8+l] = K(row a, col l)
-  float Kf[64];
-  double bd[8];   // reference-cell load for f=1
-  float bf[8];
+  double Kc[8];  // reference stiffness (κ=1) by class a⊕b of the node pair
+  double b0;     // reference load ∫φ_a (f=1), equal for every a on a box cell
 };
 
 template <int D>
@@ -59,131 +59,161 @@ struct Pow3<0> {
   static constexpr int v = 1;
 };
 
-template <int D, int BLOCK, bool F32>
-__global__ __launch_bounds__(BLOCK) void lattice_q1_kernel(const LatticeArgs P) {
-  constexpr int NB = Pow3<D>::v;  // stencil entries per column
-  constexpr int NV = 1 << D;      // nodes per cell = cells per node
-  extern __shared__ double stage[];
-  __shared__ int jmin_s, jmax_s;
+// Workgroups are dealt round-robin to the 8 XCDs; remap so XCD x sweeps the
+// contiguous x-th slice of the lattice and the neighbouring lines/planes its
+// nodes need are in that XCD's L2.  Bijective on [0, nblocks).
+__device__ __forceinline__ int xcd_remap(int b, int nblocks) {
+  constexpr int kXcds = 8;
+  const int q = nblocks / kXcds, r = nblocks % kXcds, x = b % kXcds;
+  return x * q + min(x, r) + b / kXcds;
+}
 
-  const int tid = threadIdx.x;
-  if (tid == 0) {
-    jmin_s = INT_MAX;
-    jmax_s = -1;
+// floor(n / d) for 0 <= n < 2^31 from a double reciprocal, corrected exactly.
+__device__ __forceinline__ int div_by(int n, int d, double inv) {
+  int q = (int)((double)n * inv);
+  const int r = n - q * d;
+  q += (r >= d) - (r < 0);
+  return q;
+}
+
+// Wave-wide exclusive prefix sum of v (64 lanes).
+__device__ __forceinline__ int wave_exclusive_sum(int v, int lane) {
+  int incl = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int t = __shfl_up(incl, o);
+    if (lane >= o) incl += t;
   }
-  __syncthreads();
+  return incl - v;
+}
 
-  const int64_t n = (int64_t)blockIdx.x * BLOCK + tid;
+// One wave per workgroup; the wave's 64 consecutive lattice nodes own a
+// contiguous range of CSC columns, hence of nzval.  On a box cell with a
+// tensor-product Q1 basis and a symmetric rule, K̂[a][b] depends only on a⊕b,
+// so the entry of column n at stencil offset δ is K̂[class(δ)] times the sum of
+// κ over the cells that hold both nodes (the cells of the column whose index is
+// compatible with δ).  Those sums and products are FP64 (the arithmetic is a
+// few dozen flops per column; the kernel is bound by the nzval stores).  The
+// column offsets come from the free-neighbour masks (one load per node instead
+// of 3^D); the wave stages its columns in LDS and stores them in 16-byte chunks.
+template <int D>
+__global__ __launch_bounds__(64) void lattice_q1_kernel(const LatticeArgs P) {
+  constexpr int NB = Pow3<D>::v;  // stencil entries per column
+  constexpr int NV = 1 << D;      // incident cells per node
+  __shared__ __align__(16) double stage[64 * NB + 2];
+
+  const int lane = threadIdx.x;
+  const int n = xcd_remap(blockIdx.x, gridDim.x) * 64 + lane;
   int xi[3] = {0, 0, 0};
   int d0 = 0;
   if (n < P.nnodes) {
-    int64_t r = n;
-#pragma unroll
-    for (int k = 0; k < D; ++k) {
-      xi[k] = (int)(r % P.nn[k]);
-      r /= P.nn[k];
+    const int t = div_by(n, P.nn[0], P.inv_nn0);
+    xi[0] = n - t * P.nn[0];
+    if (D == 3) {
+      xi[2] = div_by(t, P.nn[1], P.inv_nn1);
+      xi[1] = t - xi[2] * P.nn[1];
+    } else {
+      xi[1] = t;
     }
     d0 = P.node_dof[n];
   }
-  if (d0 > 0) {
-    atomicMin(&jmin_s, d0 - 1);
-    atomicMax(&jmax_s, d0 - 1);
-  }
+  const unsigned long long freem = __ballot(d0 > 0);
+  if (freem == 0ull) return;  // wave-uniform: no free node, nothing to write
+  const uint32_t fm = d0 > 0 ? P.fmask[n] : 0u;
 
-  // Stencil dofs (free > 0, Dirichlet < 0, outside the lattice 0).
-  int nb[NB];
-  unsigned mask = 0;
-#pragma unroll
-  for (int o = 0; o < NB; ++o) {
-    int oo = o;
-    bool in = d0 > 0;
-    int64_t node = 0, stride = 1;
-#pragma unroll
-    for (int k = 0; k < D; ++k) {
-      int v = xi[k] + (oo % 3) - 1;
-      oo /= 3;
-      in = in && v >= 0 && v < P.nn[k];
-      node += (int64_t)v * stride;
-      stride *= P.nn[k];
-    }
-    nb[o] = in ? P.node_dof[node] : 0;
-    mask |= (unsigned)(nb[o] > 0) << o;
-  }
-
-  double acc[NB];
-#pragma unroll
-  for (int o = 0; o < NB; ++o) acc[o] = 0.0;
-  double bacc = 0.0;
-
-  // Incident cells in ascending cell id: bit d of e selects cell x_d-1+e_d.
+  // κ and f of the incident cells, in ascending cell id (bit k of e: cell
+  // x_k - 1 + e_k); cells outside the lattice contribute nothing.
+  double kc[NV];
+  double fsum = 0.0;
 #pragma unroll
   for (int e = 0; e < NV; ++e) {
-    bool cin = d0 > 0;
-    int64_t cid = 0, cstride = 1;
-    int lb = 0;  // local index of this node in the cell
+    bool in = d0 > 0;
+    int cid = 0, cstride = 1;
 #pragma unroll
     for (int k = 0; k < D; ++k) {
-      int ek = (e >> k) & 1;
-      int ck = xi[k] - 1 + ek;
-      cin = cin && ck >= 0 && ck < P.nc[k];
-      cid += (int64_t)ck * cstride;
+      const int ck = xi[k] - 1 + ((e >> k) & 1);
+      in = in && ck >= 0 && ck < P.nc[k];
+      cid += ck * cstride;
       cstride *= P.nc[k];
-      lb |= (1 - ek) << k;
     }
-    if (cin) {
-      const double kap = P.kappa ? P.kappa[cid] : 1.0;
-      const double fc = P.fsrc ? P.fsrc[cid] : 1.0;
-      const float kapf = (float)kap;
+    kc[e] = in ? (P.kappa ? P.kappa[cid] : 1.0) : 0.0;
+    fsum += in ? (P.fsrc ? P.fsrc[cid] : 1.0) : 0.0;
+  }
+
+  // acc[o] = K̂[class(o)] Σ_{cells holding node and neighbour o} κ_c
+  double acc[NB];
 #pragma unroll
-      for (int a = 0; a < NV; ++a) {
-        int o = 0, p3 = 1;
+  for (int o = 0; o < NB; ++o) {
+    double sum = 0.0;
+    int cls = 0;
 #pragma unroll
-        for (int k = 0; k < D; ++k) {
-          o += (((e >> k) & 1) + ((a >> k) & 1)) * p3;
-          p3 *= 3;
-        }
-        if (F32)
-          acc[o] += (double)(kapf * P.Kf[a * NV + lb]);
-        else
-          acc[o] += kap * P.Kd[a * NV + lb];
+    for (int e = 0; e < NV; ++e) {
+      bool ok = true;
+      int oo = o;
+#pragma unroll
+      for (int k = 0; k < D; ++k) {
+        const int dk = oo % 3 - 1;  // neighbour displacement on axis k
+        oo /= 3;
+        const int ek = (e >> k) & 1;
+        ok = ok && !(dk == -1 && ek == 1) && !(dk == 1 && ek == 0);
       }
-      double t = F32 ? (double)((float)fc * P.bf[lb]) : fc * P.bd[lb];
-      if (P.dvals) {
-        double lift = 0.0;
+      if (ok) sum += kc[e];
+    }
+    {
+      int oo = o;
 #pragma unroll
-        for (int a = 0; a < NV; ++a) {
-          int o = 0, p3 = 1;
-#pragma unroll
-          for (int k = 0; k < D; ++k) {
-            o += (((e >> k) & 1) + ((a >> k) & 1)) * p3;
-            p3 *= 3;
-          }
-          if (nb[o] < 0) {
-            double kv = F32 ? (double)(kapf * P.Kf[lb * NV + a]) : kap * P.Kd[lb * NV + a];
-            lift += kv * P.dvals[-nb[o] - 1];
-          }
-        }
-        t -= lift;
+      for (int k = 0; k < D; ++k) {
+        cls |= (oo % 3 != 1) << k;
+        oo /= 3;
       }
-      bacc += t;
+    }
+    acc[o] = P.Kc[cls] * sum;
+  }
+
+  double bacc = P.b0 * fsum;
+  if (P.dvals && d0 > 0) {
+    // Dirichlet lifting b -= Σ_δ A(n, n+δ) u_d(n+δ) over Dirichlet neighbours.
+#pragma unroll
+    for (int o = 0; o < NB; ++o) {
+      if ((fm >> o) & 1u) continue;
+      int oo = o, node = 0, stride = 1;
+      bool in = true;
+#pragma unroll
+      for (int k = 0; k < D; ++k) {
+        const int v = xi[k] + oo % 3 - 1;
+        oo /= 3;
+        in = in && v >= 0 && v < P.nn[k];
+        node += v * stride;
+        stride *= P.nn[k];
+      }
+      if (!in) continue;
+      const int nd = P.node_dof[node];
+      if (nd < 0) bacc -= acc[o] * P.dvals[-nd - 1];
     }
   }
 
-  __syncthreads();
-  const int jmin = jmin_s, jmax = jmax_s;
-  if (jmax < 0) return;  // block without free nodes (uniform: whole block exits)
+  // Column offsets inside the wave's nzval range from the masks.
+  const int cnt = __popc(fm);
+  const int off = wave_exclusive_sum(cnt, lane);
+  const int total = __shfl(off + cnt, 63);
+  const int first = __ffsll((long long)freem) - 1;
+  const int jmin = __shfl(d0, first) - 1;
   const int P0 = P.colptr[jmin];
-  const int P1 = P.colptr[jmax + 1];
+  const int sh = P0 & 1;  // stage[sh + k] <-> nzval[P0 + k]; 16-byte aligned chunks
   if (d0 > 0) {
-    int w = P.colptr[d0 - 1] - P0;
+    int w = sh + off;
 #pragma unroll
     for (int o = 0; o < NB; ++o)
-      if ((mask >> o) & 1u) stage[w++] = acc[o];
+      if ((fm >> o) & 1u) stage[w++] = acc[o];
     P.b[d0 - 1] = bacc;
   }
   __syncthreads();
-  double *dst = P.nzval + P0;
-  for (int k = tid; k < P1 - P0; k += BLOCK) dst[k] = stage[k];
+  const int lo = sh, hi = sh + total;
+  double2 *dst2 = reinterpret_cast<double2 *>(P.nzval + (P0 - sh));
+  const double2 *st2 = reinterpret_cast<const double2 *>(stage);
+  for (int c = (lo + 1) / 2 + lane; c < hi / 2; c += 64) dst2[c] = st2[c];
+  if (lane == 0 && (lo & 1)) P.nzval[P0] = stage[lo];
+  if (lane == 1 && (hi & 1)) P.nzval[P0 + total - 1] = stage[hi - 1];
 }
 
 // Reference-cell stiffness and load on the Cartesian cell (Jacobian diag(h)).
@@ -207,33 +237,47 @@ void lattice_reference(const Ctx &c, double *K, double *bv) {
   }
 }
 
-template <int D, int BLOCK>
-void launch_lattice(Ctx &c, int precision) {
+// The lattice kernel needs K̂[a][b] = K̂[a⊕b][0] and equal load entries (true for
+// a tensor-product basis with a symmetric rule on a box); checked to 1e-13.
+bool lattice_reference_symmetric(const Ctx &c, double *Kc, double *b0) {
+  const int nv = 1 << c.dim;
+  double K[64], bv[8];
+  lattice_reference(c, K, bv);
+  double kmax = 0.0, bmax = 0.0;
+  for (int i = 0; i < nv * nv; ++i) kmax = std::max(kmax, std::fabs(K[i]));
+  for (int i = 0; i < nv; ++i) bmax = std::max(bmax, std::fabs(bv[i]));
+  for (int a = 0; a < nv; ++a)
+    for (int b = 0; b < nv; ++b)
+      if (std::fabs(K[a * nv + b] - K[(a ^ b) * nv]) > 1e-13 * kmax) return false;
+  for (int a = 0; a < nv; ++a)
+    if (std::fabs(bv[a] - bv[0]) > 1e-13 * bmax) return false;
+  for (int k = 0; k < 8; ++k) Kc[k] = k < nv ? K[k * nv] : 0.0;
+  *b0 = bv[0];
+  return true;
+}
+
+template <int D>
+void launch_lattice(Ctx &c, const double *Kc, double b0) {
   LatticeArgs P{};
   for (int k = 0; k < 3; ++k) {
     P.nn[k] = (int)(c.part[k] + 1);
     P.nc[k] = (int)c.part[k];
   }
-  P.nnodes = c.nnodes;
+  P.nnodes = (int)c.nnodes;
+  P.inv_nn0 = 1.0 / P.nn[0];
+  P.inv_nn1 = 1.0 / P.nn[1];
   P.node_dof = c.node_dof.get();
+  P.fmask = c.node_fmask.get();
   P.colptr = c.colptr.get();
   P.kappa = c.have_kappa ? c.kappa.get() : nullptr;
   P.fsrc = c.have_fsrc ? c.fsrc.get() : nullptr;
   P.dvals = c.have_dvals ? c.dvals.get() : nullptr;
   P.nzval = c.nzval.get();
   P.b = c.b.get();
-  lattice_reference(c, P.Kd, P.bd);
-  for (int i = 0; i < 64; ++i) P.Kf[i] = (float)P.Kd[i];
-  for (int i = 0; i < 8; ++i) P.bf[i] = (float)P.bd[i];
-  constexpr int NB = Pow3<D>::v;
-  size_t lds = sizeof(double) * BLOCK * NB;
-  unsigned grid = (unsigned)((c.nnodes + BLOCK - 1) / BLOCK);
-  if (precision == GH_PREC_FP32_ELEM)
-    hipLaunchKernelGGL((lattice_q1_kernel<D, BLOCK, true>), dim3(grid), dim3(BLOCK), lds,
-                       c.stream, P);
-  else
-    hipLaunchKernelGGL((lattice_q1_kernel<D, BLOCK, false>), dim3(grid), dim3(BLOCK), lds,
-                       c.stream, P);
+  for (int k = 0; k < 8; ++k) P.Kc[k] = Kc[k];
+  P.b0 = b0;
+  const unsigned grid = (unsigned)((c.nnodes + 63) / 64);
+  hipLaunchKernelGGL((lattice_q1_kernel<D>), dim3(grid), dim3(64), 0, c.stream, P);
   GH_CHECK_HIP(hipGetLastError());
 }
 
@@ -448,17 +492,23 @@ void assemble_diffusion(Ctx &c, int precision, int path) {
              "kappa was uploaded for a different mesh");
   GH_REQUIRE(!c.have_dvals || c.dvals.n == (size_t)c.ndiri, GH_ERR_STATE,
              "Dirichlet values were uploaded for a different space");
+  double Kc[8] = {0}, b0 = 0.0;
   bool lattice = c.structured && path != GH_PATH_GENERAL;
-  GH_REQUIRE(path != GH_PATH_STRUCTURED || c.structured, GH_ERR_UNSUPPORTED,
-             "the lattice path needs a Cartesian mesh with a scalar Q1 CLagrangian space");
   if (lattice) {
     GH_REQUIRE((int)c.shp.size() == c.nq * c.nloc, GH_ERR_STATE,
                "tabulation does not match the element (dofs per cell)");
+    lattice = lattice_reference_symmetric(c, Kc, &b0);
+  }
+  GH_REQUIRE(path != GH_PATH_STRUCTURED || lattice, GH_ERR_UNSUPPORTED,
+             "the lattice path needs a Cartesian mesh, a scalar Q1 CLagrangian space and a "
+             "symmetric quadrature rule");
+  if (lattice) {
     // Enqueued only: gh_values_get (or a wait on gh_stream) orders the results.
+    // The lattice arithmetic is FP64 in both precision modes (see the kernel).
     if (c.dim == 3)
-      launch_lattice<3, 128>(c, precision);
+      launch_lattice<3>(c, Kc, b0);
     else
-      launch_lattice<2, 256>(c, precision);
+      launch_lattice<2>(c, Kc, b0);
   } else {
     launch_general(c, precision);
   }

@@ -91,6 +91,7 @@ struct Ctx {
   // ---- structured (lattice) eligibility ----
   bool structured = false;
   DBuf<int32_t> node_dof;           // nnodes: Gridap dof of each lattice node
+  DBuf<uint32_t> node_fmask;        // nnodes: bit o set when stencil neighbour o is free
 
   // ---- colouring for the general path ----
   int ncolors = 0;

@@ -112,13 +112,18 @@ __global__ void free_flags(int64_t nnodes, const int32_t *__restrict__ node_dof,
 
 // Free dofs must be numbered in node order (CLagrangian numbering) and the
 // pattern column of node n must be the sorted free lattice neighbours of n.
+// Also records, per free node, the bit mask of free stencil neighbours (bit o for
+// offset o = Σ_k (δ_k+1) 3^k) that the numeric lattice kernel uses in place of
+// the 3^D neighbour loads.
 __global__ void lattice_check_pattern(Lattice L, int64_t nnodes,
                                       const int32_t *__restrict__ node_dof,
                                       const int32_t *__restrict__ scan,
                                       const int32_t *__restrict__ colptr,
-                                      const int32_t *__restrict__ rowval, int32_t *bad) {
+                                      const int32_t *__restrict__ rowval,
+                                      uint32_t *__restrict__ fmask, int32_t *bad) {
   int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= nnodes) return;
+  fmask[n] = 0u;
   int32_t d = node_dof[n];
   if (d <= 0) return;
   if (d != scan[n] + 1) {
@@ -133,6 +138,7 @@ __global__ void lattice_check_pattern(Lattice L, int64_t nnodes,
   int nb = 1;
   for (int k = 0; k < L.D; ++k) nb *= 3;
   int32_t p = colptr[d - 1], pend = colptr[d];
+  uint32_t m = 0u;
   for (int o = 0; o < nb; ++o) {
     int oo = o;
     int64_t node = 0, stride = 1;
@@ -152,8 +158,10 @@ __global__ void lattice_check_pattern(Lattice L, int64_t nnodes,
       return;
     }
     ++p;
+    m |= 1u << o;
   }
   if (p != pend) atomicOr(bad, 8);
+  fmask[n] = m;
 }
 
 template <class T>
@@ -232,6 +240,7 @@ void symbolic_general(Ctx &c) {
 void check_structured(Ctx &c) {
   c.structured = false;
   c.node_dof.release();
+  c.node_fmask.release();
   if (!(c.cartesian && c.reffe_kind == GH_REFFE_QK && c.order == 1 && c.ncomp == 1 &&
         c.nloc == (1 << c.dim) && c.nfree > 0))
     return;
@@ -243,6 +252,7 @@ void check_structured(Ctx &c) {
     L.np[d] = c.part[d] + 1;
   }
   c.node_dof.resize(c.nnodes);
+  c.node_fmask.resize(c.nnodes);
   DBuf<int32_t> bad, f, scan;
   bad.resize(1);
   f.resize(c.nnodes);
@@ -260,13 +270,16 @@ void check_structured(Ctx &c) {
   exclusive_sum(f.get(), scan.get(), c.nnodes, s);
   hipLaunchKernelGGL(lattice_check_pattern, dim3(grid_for(c.nnodes)), dim3(kBlock), 0, s, L,
                      c.nnodes, c.node_dof.get(), scan.get(), c.colptr.get(), c.rowval.get(),
-                     bad.get());
+                     c.node_fmask.get(), bad.get());
   GH_CHECK_HIP(hipGetLastError());
   int32_t hbad = 0;
   GH_CHECK_HIP(hipMemcpyAsync(&hbad, bad.get(), 4, hipMemcpyDeviceToHost, s));
   GH_CHECK_HIP(hipStreamSynchronize(s));
   c.structured = (hbad == 0);
-  if (!c.structured) c.node_dof.release();
+  if (!c.structured) {
+    c.node_dof.release();
+    c.node_fmask.release();
+  }
 }
 
 // Greedy colouring of cells so that no two cells of one colour share a free dof:
