@@ -40,6 +40,7 @@ struct LatticeArgs {
   double inv_nn0, inv_nn1;  // reciprocals for the node-index split
   const int32_t *node_dof;
   const uint32_t *fmask;    // free stencil neighbours per node (symbolic phase)
+  const int32_t *tile_p0;   // nzval start of each 64-node tile (symbolic phase)
   const int32_t *colptr;
   const double *kappa;      // per cell or nullptr (κ = 1)
   const double *fsrc;       // per cell or nullptr (f = 1)
@@ -58,6 +59,10 @@ template <>
 struct Pow3<0> {
   static constexpr int v = 1;
 };
+
+// Staging rounds of the lattice kernel's tile: LDS per wave is 64/R columns, so
+// more waves fit a CU (measured on MI355X, 128^3 Q1).
+constexpr int kLatticeStageRounds = 2;
 
 // Workgroups are dealt round-robin to the 8 XCDs; remap so XCD x sweeps the
 // contiguous x-th slice of the lattice and the neighbouring lines/planes its
@@ -87,58 +92,81 @@ __device__ __forceinline__ int wave_exclusive_sum(int v, int lane) {
   return incl - v;
 }
 
-// One wave per workgroup; the wave's 64 consecutive lattice nodes own a
-// contiguous range of CSC columns, hence of nzval.  On a box cell with a
-// tensor-product Q1 basis and a symmetric rule, K̂[a][b] depends only on a⊕b,
-// so the entry of column n at stencil offset δ is K̂[class(δ)] times the sum of
-// κ over the cells that hold both nodes (the cells of the column whose index is
-// compatible with δ).  Those sums and products are FP64 (the arithmetic is a
-// few dozen flops per column; the kernel is bound by the nzval stores).  The
-// column offsets come from the free-neighbour masks (one load per node instead
-// of 3^D); the wave stages its columns in LDS and stores them in 16-byte chunks.
+// Inputs of one 64-node tile for one lane, loaded ahead of their use.
 template <int D>
-__global__ __launch_bounds__(64) void lattice_q1_kernel(const LatticeArgs P) {
-  constexpr int NB = Pow3<D>::v;  // stencil entries per column
-  constexpr int NV = 1 << D;      // incident cells per node
-  __shared__ __align__(16) double stage[64 * NB + 2];
+struct TileIn {
+  int xi[3];
+  int d0;
+  uint32_t fm;
+  double kc[1 << D];
+  double fsum;
+};
 
-  const int lane = threadIdx.x;
-  const int n = xcd_remap(blockIdx.x, gridDim.x) * 64 + lane;
-  int xi[3] = {0, 0, 0};
-  int d0 = 0;
+template <int D>
+__device__ __forceinline__ void load_tile(const LatticeArgs &P, int tile, int lane,
+                                          TileIn<D> &T) {
+  constexpr int NV = 1 << D;
+  const int n = tile * kTileNodes + lane;
+  T.xi[0] = T.xi[1] = T.xi[2] = 0;
+  T.d0 = 0;
   if (n < P.nnodes) {
     const int t = div_by(n, P.nn[0], P.inv_nn0);
-    xi[0] = n - t * P.nn[0];
+    T.xi[0] = n - t * P.nn[0];
     if (D == 3) {
-      xi[2] = div_by(t, P.nn[1], P.inv_nn1);
-      xi[1] = t - xi[2] * P.nn[1];
+      T.xi[2] = div_by(t, P.nn[1], P.inv_nn1);
+      T.xi[1] = t - T.xi[2] * P.nn[1];
     } else {
-      xi[1] = t;
+      T.xi[1] = t;
     }
-    d0 = P.node_dof[n];
+    T.d0 = P.node_dof[n];
+    T.fm = P.fmask[n];  // 0 for nodes that own no column
+  } else {
+    T.fm = 0u;
   }
-  const unsigned long long freem = __ballot(d0 > 0);
-  if (freem == 0ull) return;  // wave-uniform: no free node, nothing to write
-  const uint32_t fm = d0 > 0 ? P.fmask[n] : 0u;
-
   // κ and f of the incident cells, in ascending cell id (bit k of e: cell
   // x_k - 1 + e_k); cells outside the lattice contribute nothing.
-  double kc[NV];
-  double fsum = 0.0;
+  T.fsum = 0.0;
+  const bool valid = n < P.nnodes;
 #pragma unroll
   for (int e = 0; e < NV; ++e) {
-    bool in = d0 > 0;
+    bool in = valid;
     int cid = 0, cstride = 1;
 #pragma unroll
     for (int k = 0; k < D; ++k) {
-      const int ck = xi[k] - 1 + ((e >> k) & 1);
+      const int ck = T.xi[k] - 1 + ((e >> k) & 1);
       in = in && ck >= 0 && ck < P.nc[k];
       cid += ck * cstride;
       cstride *= P.nc[k];
     }
-    kc[e] = in ? (P.kappa ? P.kappa[cid] : 1.0) : 0.0;
-    fsum += in ? (P.fsrc ? P.fsrc[cid] : 1.0) : 0.0;
+    T.kc[e] = in ? (P.kappa ? P.kappa[cid] : 1.0) : 0.0;
+    T.fsum += in ? (P.fsrc ? P.fsrc[cid] : 1.0) : 0.0;
   }
+}
+
+// One wave per workgroup owns one tile of 64 consecutive lattice nodes; the
+// tile's nodes own a contiguous range of CSC columns, hence of nzval.  On a box
+// cell with a tensor-product Q1 basis and a symmetric rule, K̂[a][b] depends only
+// on a⊕b, so the entry of column n at stencil offset δ is K̂[class(δ)] times the
+// sum of κ over the cells that hold both nodes.  Those sums and products are FP64
+// (a few dozen flops per column; the kernel is bound by the nzval stores).
+// Column offsets come from the free-neighbour masks (one load per node instead
+// of 3^D); the tile is staged in LDS and stored in 16-byte chunks.
+template <int D>
+__global__ __launch_bounds__(64) void lattice_q1_kernel(const LatticeArgs P) {
+  constexpr int NB = Pow3<D>::v;  // stencil entries per column
+  constexpr int NV = 1 << D;      // incident cells per node
+  constexpr int R = kLatticeStageRounds;
+  constexpr int LPR = 64 / R;     // lanes staged per round
+  __shared__ __align__(16) double stage[LPR * NB + 2];
+
+  const int lane = threadIdx.x;
+  const int tile = xcd_remap(blockIdx.x, gridDim.x);
+  const int P0 = P.tile_p0[tile];  // uniform: every load below is independent of it
+  TileIn<D> cur;
+  load_tile<D>(P, tile, lane, cur);
+  if (P0 < 0) return;  // wave-uniform: no free node in the tile
+  const int d0 = cur.d0;
+  const uint32_t fm = cur.fm;
 
   // acc[o] = K̂[class(o)] Σ_{cells holding node and neighbour o} κ_c
   double acc[NB];
@@ -157,7 +185,7 @@ __global__ __launch_bounds__(64) void lattice_q1_kernel(const LatticeArgs P) {
         const int ek = (e >> k) & 1;
         ok = ok && !(dk == -1 && ek == 1) && !(dk == 1 && ek == 0);
       }
-      if (ok) sum += kc[e];
+      if (ok) sum += cur.kc[e];
     }
     {
       int oo = o;
@@ -170,7 +198,7 @@ __global__ __launch_bounds__(64) void lattice_q1_kernel(const LatticeArgs P) {
     acc[o] = P.Kc[cls] * sum;
   }
 
-  double bacc = P.b0 * fsum;
+  double bacc = P.b0 * cur.fsum;
   if (P.dvals && d0 > 0) {
     // Dirichlet lifting b -= Σ_δ A(n, n+δ) u_d(n+δ) over Dirichlet neighbours.
 #pragma unroll
@@ -180,7 +208,7 @@ __global__ __launch_bounds__(64) void lattice_q1_kernel(const LatticeArgs P) {
       bool in = true;
 #pragma unroll
       for (int k = 0; k < D; ++k) {
-        const int v = xi[k] + oo % 3 - 1;
+        const int v = cur.xi[k] + oo % 3 - 1;
         oo /= 3;
         in = in && v >= 0 && v < P.nn[k];
         node += v * stride;
@@ -192,28 +220,35 @@ __global__ __launch_bounds__(64) void lattice_q1_kernel(const LatticeArgs P) {
     }
   }
 
-  // Column offsets inside the wave's nzval range from the masks.
+  // Column offsets inside the tile's nzval range from the masks.
   const int cnt = __popc(fm);
   const int off = wave_exclusive_sum(cnt, lane);
   const int total = __shfl(off + cnt, 63);
-  const int first = __ffsll((long long)freem) - 1;
-  const int jmin = __shfl(d0, first) - 1;
-  const int P0 = P.colptr[jmin];
-  const int sh = P0 & 1;  // stage[sh + k] <-> nzval[P0 + k]; 16-byte aligned chunks
-  if (d0 > 0) {
-    int w = sh + off;
+  if (d0 > 0) P.b[d0 - 1] = bacc;
+
+  // R rounds of LPR lanes each: stage that lane group's columns, store them.
 #pragma unroll
-    for (int o = 0; o < NB; ++o)
-      if ((fm >> o) & 1u) stage[w++] = acc[o];
-    P.b[d0 - 1] = bacc;
+  for (int r = 0; r < R; ++r) {
+    const int beg = r == 0 ? 0 : __shfl(off, r * LPR);
+    const int end = r == R - 1 ? total : __shfl(off, (r + 1) * LPR);
+    if (end == beg) continue;  // wave-uniform
+    const int Pr = P0 + beg;
+    const int sh = Pr & 1;  // stage[sh + k] <-> nzval[Pr + k]; 16-byte aligned chunks
+    if (r > 0) __syncthreads();  // the previous round's stage has been read out
+    if (d0 > 0 && lane / LPR == r) {
+      int w = sh + off - beg;
+#pragma unroll
+      for (int o = 0; o < NB; ++o)
+        if ((fm >> o) & 1u) stage[w++] = acc[o];
+    }
+    __syncthreads();
+    const int lo = sh, hi = sh + (end - beg);
+    double2 *dst2 = reinterpret_cast<double2 *>(P.nzval + (Pr - sh));
+    const double2 *st2 = reinterpret_cast<const double2 *>(stage);
+    for (int c = (lo + 1) / 2 + lane; c < hi / 2; c += 64) dst2[c] = st2[c];
+    if (lane == 0 && (lo & 1)) P.nzval[Pr] = stage[lo];
+    if (lane == 1 && (hi & 1)) P.nzval[Pr + (end - beg) - 1] = stage[hi - 1];
   }
-  __syncthreads();
-  const int lo = sh, hi = sh + total;
-  double2 *dst2 = reinterpret_cast<double2 *>(P.nzval + (P0 - sh));
-  const double2 *st2 = reinterpret_cast<const double2 *>(stage);
-  for (int c = (lo + 1) / 2 + lane; c < hi / 2; c += 64) dst2[c] = st2[c];
-  if (lane == 0 && (lo & 1)) P.nzval[P0] = stage[lo];
-  if (lane == 1 && (hi & 1)) P.nzval[P0 + total - 1] = stage[hi - 1];
 }
 
 // Reference-cell stiffness and load on the Cartesian cell (Jacobian diag(h)).
@@ -268,6 +303,7 @@ void launch_lattice(Ctx &c, const double *Kc, double b0) {
   P.inv_nn1 = 1.0 / P.nn[1];
   P.node_dof = c.node_dof.get();
   P.fmask = c.node_fmask.get();
+  P.tile_p0 = c.tile_p0.get();
   P.colptr = c.colptr.get();
   P.kappa = c.have_kappa ? c.kappa.get() : nullptr;
   P.fsrc = c.have_fsrc ? c.fsrc.get() : nullptr;
@@ -276,8 +312,8 @@ void launch_lattice(Ctx &c, const double *Kc, double b0) {
   P.b = c.b.get();
   for (int k = 0; k < 8; ++k) P.Kc[k] = Kc[k];
   P.b0 = b0;
-  const unsigned grid = (unsigned)((c.nnodes + 63) / 64);
-  hipLaunchKernelGGL((lattice_q1_kernel<D>), dim3(grid), dim3(64), 0, c.stream, P);
+  const unsigned grid = (unsigned)((c.nnodes + kTileNodes - 1) / kTileNodes);
+  hipLaunchKernelGGL((lattice_q1_kernel<D>), dim3(grid), dim3(kTileNodes), 0, c.stream, P);
   GH_CHECK_HIP(hipGetLastError());
 }
 

@@ -11,6 +11,9 @@
 
 namespace gh {
 
+// Lattice nodes per tile of the lattice kernel (one wavefront).
+constexpr int kTileNodes = 64;
+
 // Error carried from the point of failure to the ABI boundary, where it becomes a
 // status code plus gh_last_error() text.
 struct Error : std::runtime_error {
@@ -92,6 +95,7 @@ struct Ctx {
   bool structured = false;
   DBuf<int32_t> node_dof;           // nnodes: Gridap dof of each lattice node
   DBuf<uint32_t> node_fmask;        // nnodes: bit o set when stencil neighbour o is free
+  DBuf<int32_t> tile_p0;            // per 64-node tile: nzval start of its columns, or -1
 
   // ---- colouring for the general path ----
   int ncolors = 0;
