@@ -41,6 +41,7 @@ struct LatticeArgs {
   const int32_t *node_dof;
   const uint32_t *fmask;    // free stencil neighbours per node (symbolic phase)
   const int32_t *tile_p0;   // nzval start of each 64-node tile (symbolic phase)
+  const int32_t *tile_d0;   // dof of each tile's first free node (symbolic phase)
   const int32_t *colptr;
   const double *kappa;      // per cell or nullptr (κ = 1)
   const double *fsrc;       // per cell or nullptr (f = 1)
@@ -96,7 +97,6 @@ __device__ __forceinline__ int wave_exclusive_sum(int v, int lane) {
 template <int D>
 struct TileIn {
   int xi[3];
-  int d0;
   uint32_t fm;
   double kc[1 << D];
   double fsum;
@@ -108,7 +108,6 @@ __device__ __forceinline__ void load_tile(const LatticeArgs &P, int tile, int la
   constexpr int NV = 1 << D;
   const int n = tile * kTileNodes + lane;
   T.xi[0] = T.xi[1] = T.xi[2] = 0;
-  T.d0 = 0;
   if (n < P.nnodes) {
     const int t = div_by(n, P.nn[0], P.inv_nn0);
     T.xi[0] = n - t * P.nn[0];
@@ -118,8 +117,7 @@ __device__ __forceinline__ void load_tile(const LatticeArgs &P, int tile, int la
     } else {
       T.xi[1] = t;
     }
-    T.d0 = P.node_dof[n];
-    T.fm = P.fmask[n];  // 0 for nodes that own no column
+    T.fm = P.fmask[n];  // 0 exactly for nodes that own no column (Dirichlet)
   } else {
     T.fm = 0u;
   }
@@ -162,11 +160,15 @@ __global__ __launch_bounds__(64) void lattice_q1_kernel(const LatticeArgs P) {
   const int lane = threadIdx.x;
   const int tile = xcd_remap(blockIdx.x, gridDim.x);
   const int P0 = P.tile_p0[tile];  // uniform: every load below is independent of it
+  const int dbase = P.tile_d0[tile];
   TileIn<D> cur;
   load_tile<D>(P, tile, lane, cur);
   if (P0 < 0) return;  // wave-uniform: no free node in the tile
-  const int d0 = cur.d0;
   const uint32_t fm = cur.fm;
+  // Free dofs are numbered in node order (checked in the symbolic phase): the
+  // lane's dof is the tile's first free dof plus the free lanes below it.
+  const unsigned long long freem = __ballot(fm != 0u);
+  const int d0 = fm != 0u ? dbase + __popcll(freem & ((1ull << lane) - 1ull)) : 0;
 
   // acc[o] = K̂[class(o)] Σ_{cells holding node and neighbour o} κ_c
   double acc[NB];
@@ -304,6 +306,7 @@ void launch_lattice(Ctx &c, const double *Kc, double b0) {
   P.node_dof = c.node_dof.get();
   P.fmask = c.node_fmask.get();
   P.tile_p0 = c.tile_p0.get();
+  P.tile_d0 = c.tile_d0.get();
   P.colptr = c.colptr.get();
   P.kappa = c.have_kappa ? c.kappa.get() : nullptr;
   P.fsrc = c.have_fsrc ? c.fsrc.get() : nullptr;

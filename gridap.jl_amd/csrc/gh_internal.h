@@ -96,6 +96,7 @@ struct Ctx {
   DBuf<int32_t> node_dof;           // nnodes: Gridap dof of each lattice node
   DBuf<uint32_t> node_fmask;        // nnodes: bit o set when stencil neighbour o is free
   DBuf<int32_t> tile_p0;            // per 64-node tile: nzval start of its columns, or -1
+  DBuf<int32_t> tile_d0;            // per 64-node tile: dof of its first free node, or -1
 
   // ---- colouring for the general path ----
   int ncolors = 0;

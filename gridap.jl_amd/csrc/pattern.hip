@@ -170,18 +170,20 @@ __global__ void lattice_check_pattern(Lattice L, int64_t nnodes,
 __global__ void lattice_tile_p0(int64_t nnodes, int64_t ntiles,
                                 const int32_t *__restrict__ node_dof,
                                 const int32_t *__restrict__ colptr,
-                                int32_t *__restrict__ tile_p0) {
+                                int32_t *__restrict__ tile_p0, int32_t *__restrict__ tile_d0) {
   int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= ntiles) return;
-  int32_t p0 = -1;
+  int32_t p0 = -1, d0 = -1;
   for (int64_t n = t * kTileNodes; n < std::min<int64_t>((t + 1) * kTileNodes, nnodes); ++n) {
     const int32_t d = node_dof[n];
     if (d > 0) {
       p0 = colptr[d - 1];
+      d0 = d;
       break;
     }
   }
   tile_p0[t] = p0;
+  tile_d0[t] = d0;
 }
 
 template <class T>
@@ -262,6 +264,7 @@ void check_structured(Ctx &c) {
   c.node_dof.release();
   c.node_fmask.release();
   c.tile_p0.release();
+  c.tile_d0.release();
   if (!(c.cartesian && c.reffe_kind == GH_REFFE_QK && c.order == 1 && c.ncomp == 1 &&
         c.nloc == (1 << c.dim) && c.nfree > 0))
     return;
@@ -294,8 +297,10 @@ void check_structured(Ctx &c) {
                      c.node_fmask.get(), bad.get());
   const int64_t ntiles = (c.nnodes + kTileNodes - 1) / kTileNodes;
   c.tile_p0.resize(ntiles);
+  c.tile_d0.resize(ntiles);
   hipLaunchKernelGGL(lattice_tile_p0, dim3(grid_for(ntiles)), dim3(kBlock), 0, s, c.nnodes,
-                     ntiles, c.node_dof.get(), c.colptr.get(), c.tile_p0.get());
+                     ntiles, c.node_dof.get(), c.colptr.get(), c.tile_p0.get(),
+                     c.tile_d0.get());
   GH_CHECK_HIP(hipGetLastError());
   int32_t hbad = 0;
   GH_CHECK_HIP(hipMemcpyAsync(&hbad, bad.get(), 4, hipMemcpyDeviceToHost, s));
@@ -305,6 +310,7 @@ void check_structured(Ctx &c) {
     c.node_dof.release();
     c.node_fmask.release();
     c.tile_p0.release();
+    c.tile_d0.release();
   }
 }
 
