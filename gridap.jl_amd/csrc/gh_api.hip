@@ -1,6 +1,7 @@
 // C ABI of the assembly engine: argument checks, host<->device copies and the
 // status-code boundary.  Compute lives in pattern.hip, assemble.hip, solve.hip.
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 
 #include "gh_internal.h"
@@ -37,6 +38,8 @@ void invalidate_pattern(gh::Ctx &c) {
   c.structured = false;
   c.ncolors = 0;
   c.color_ptr.clear();
+  c.fsai_ready = false;
+  c.fsai_built = false;
 }
 
 }  // namespace
@@ -259,6 +262,7 @@ int gh_assemble(gh_ctx *ctx, int form_kind, int precision, int path) {
     GH_REQUIRE(c.nq > 0, GH_ERR_STATE, "no reference tabulation uploaded");
     gh::assemble_diffusion(c, precision, path);
     c.have_values = true;
+    c.fsai_built = false;
   });
 }
 
@@ -304,12 +308,13 @@ int gh_pcg(gh_ctx *ctx, int precond, const double *b, double *x, double rtol, in
   int st = guarded(ctx, [&](gh::Ctx &c) {
     GH_REQUIRE(c.have_values, GH_ERR_STATE, "no assembled matrix");
     GH_REQUIRE(x, GH_ERR_ARG, "NULL solution vector");
-    GH_REQUIRE(precond == GH_PRECOND_NONE || precond == GH_PRECOND_JACOBI,
+    GH_REQUIRE(precond == GH_PRECOND_NONE || precond == GH_PRECOND_JACOBI ||
+                   precond == GH_PRECOND_FSAI,
                GH_ERR_UNSUPPORTED, "unknown preconditioner");
     GH_REQUIRE(rtol > 0 && maxit >= 0, GH_ERR_ARG, "bad tolerance or iteration cap");
     size_t n = c.nfree;
-    c.s_x.resize(n);
-    c.s_b.resize(n);
+    c.s_x.resize(std::max<size_t>(n, 1));
+    c.s_b.resize(std::max<size_t>(n, 1));
     GH_CHECK_HIP(hipMemcpyAsync(c.s_x.get(), x, sizeof(double) * n, hipMemcpyHostToDevice,
                                 c.stream));
     if (b)
@@ -318,8 +323,22 @@ int gh_pcg(gh_ctx *ctx, int precond, const double *b, double *x, double rtol, in
     else
       GH_CHECK_HIP(hipMemcpyAsync(c.s_b.get(), c.b.get(), sizeof(double) * n,
                                   hipMemcpyDeviceToDevice, c.stream));
+    if (precond == GH_PRECOND_FSAI && !c.fsai_built) gh::fsai_forward_build(c);
     gh_solve_stats local{};
     gh::pcg_device(c, precond, rtol, maxit, &local);
+    local.precond_used = precond;
+    if (!local.converged && precond == GH_PRECOND_FSAI) {
+      // Jacobi-PCG from the caller's initial guess is the fallback of the learned
+      // preconditioner; the reported time covers both attempts.
+      const double first_ms = local.device_ms;
+      const int first_it = local.iterations;
+      GH_CHECK_HIP(hipMemcpyAsync(c.s_x.get(), x, sizeof(double) * n, hipMemcpyHostToDevice,
+                                  c.stream));
+      gh::pcg_device(c, GH_PRECOND_JACOBI, rtol, maxit, &local);
+      local.precond_used = GH_PRECOND_JACOBI;
+      local.device_ms += first_ms;
+      local.iterations += first_it;
+    }
     GH_CHECK_HIP(hipMemcpyAsync(x, c.s_x.get(), sizeof(double) * n, hipMemcpyDeviceToHost,
                                 c.stream));
     GH_CHECK_HIP(hipStreamSynchronize(c.stream));
@@ -328,6 +347,126 @@ int gh_pcg(gh_ctx *ctx, int precond, const double *b, double *x, double rtol, in
       throw gh::Error(GH_ERR_NOCONV, "PCG did not reach the requested tolerance");
   });
   return st;
+}
+
+int gh_mlp_nparams(int64_t *n) {
+  if (!n) return GH_ERR_ARG;
+  *n = gh::kMlpParams;
+  return GH_OK;
+}
+
+namespace {
+void mlp_alloc(gh::Ctx &c) {
+  c.params.resize(gh::kMlpParams);
+  c.grads.resize(gh::kMlpParams);
+  c.adam_m.resize(gh::kMlpParams);
+  c.adam_v.resize(gh::kMlpParams);
+  c.params_bf16.resize(gh::kMlpParams);
+}
+void mlp_reset_optimizer(gh::Ctx &c) {
+  GH_CHECK_HIP(hipMemsetAsync(c.adam_m.get(), 0, sizeof(float) * gh::kMlpParams, c.stream));
+  GH_CHECK_HIP(hipMemsetAsync(c.adam_v.get(), 0, sizeof(float) * gh::kMlpParams, c.stream));
+  GH_CHECK_HIP(hipMemsetAsync(c.grads.get(), 0, sizeof(float) * gh::kMlpParams, c.stream));
+  c.adam_t = 0;
+}
+}  // namespace
+
+int gh_mlp_init(gh_ctx *ctx, uint64_t seed) {
+  return guarded(ctx, [&](gh::Ctx &c) {
+    std::vector<float> w(gh::kMlpParams, 0.0f);
+    uint64_t state = seed;
+    auto uniform = [&]() {  // splitmix64 -> [-1, 1)
+      uint64_t x = (state += 0x9E3779B97F4A7C15ull);
+      x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+      x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+      x ^= x >> 31;
+      return (float)((double)(x >> 11) * (2.0 / 9007199254740992.0) - 1.0);
+    };
+    auto xavier = [&](int off, int nout, int nin) {
+      const float a = std::sqrt(6.0f / (float)(nin + nout));
+      for (int k = 0; k < nout * nin; ++k) w[off + k] = a * uniform();
+    };
+    xavier(gh::kMlpW1, gh::kMlpHidden, gh::kMlpIn);
+    xavier(gh::kMlpW2, gh::kMlpHidden, gh::kMlpHidden);
+    xavier(gh::kMlpW3, gh::kMlpHidden, gh::kMlpHidden);
+    // last layer zero: Ĝ = I, the preconditioner starts as Jacobi
+    mlp_alloc(c);
+    GH_CHECK_HIP(hipMemcpyAsync(c.params.get(), w.data(), sizeof(float) * w.size(),
+                                hipMemcpyHostToDevice, c.stream));
+    mlp_reset_optimizer(c);
+    gh::mlp_sync_bf16(c);
+    GH_CHECK_HIP(hipStreamSynchronize(c.stream));
+    c.have_mlp = true;
+    c.fsai_built = false;
+  });
+}
+
+int gh_mlp_set(gh_ctx *ctx, const float *params, int64_t n) {
+  return guarded(ctx, [&](gh::Ctx &c) {
+    GH_REQUIRE(params && n == gh::kMlpParams, GH_ERR_ARG, "expected gh_mlp_nparams() weights");
+    for (int64_t k = 0; k < n; ++k)
+      GH_REQUIRE(std::isfinite(params[k]), GH_ERR_ARG, "non-finite network weight");
+    mlp_alloc(c);
+    GH_CHECK_HIP(hipMemcpyAsync(c.params.get(), params, sizeof(float) * n, hipMemcpyHostToDevice,
+                                c.stream));
+    mlp_reset_optimizer(c);
+    gh::mlp_sync_bf16(c);
+    GH_CHECK_HIP(hipStreamSynchronize(c.stream));
+    c.have_mlp = true;
+    c.fsai_built = false;
+  });
+}
+
+int gh_mlp_get(gh_ctx *ctx, float *params, int64_t n) {
+  return guarded(ctx, [&](gh::Ctx &c) {
+    GH_REQUIRE(c.have_mlp, GH_ERR_STATE, "no network weights");
+    GH_REQUIRE(params && n == gh::kMlpParams, GH_ERR_ARG, "expected gh_mlp_nparams() weights");
+    GH_CHECK_HIP(hipMemcpyAsync(params, c.params.get(), sizeof(float) * n, hipMemcpyDeviceToHost,
+                                c.stream));
+    GH_CHECK_HIP(hipStreamSynchronize(c.stream));
+  });
+}
+
+int gh_grf_kappa(gh_ctx *ctx, int nmodes, const double *omega, const double *phase,
+                 double amp) {
+  return guarded(ctx, [&](gh::Ctx &c) {
+    GH_REQUIRE(c.dim > 0, GH_ERR_STATE, "upload a mesh first");
+    gh::grf_kappa(c, nmodes, omega, phase, amp);
+    c.have_values = false;
+    c.fsai_built = false;
+  });
+}
+
+int gh_fsai_build(gh_ctx *ctx) {
+  return guarded(ctx, [&](gh::Ctx &c) {
+    GH_REQUIRE(c.have_values, GH_ERR_STATE, "assemble the matrix first");
+    gh::fsai_forward_build(c);
+    GH_CHECK_HIP(hipStreamSynchronize(c.stream));
+  });
+}
+
+int gh_fsai_train_step(gh_ctx *ctx, int nprobes, uint64_t seed, float lr, double *loss) {
+  return guarded(ctx, [&](gh::Ctx &c) {
+    GH_REQUIRE(c.have_values, GH_ERR_STATE, "assemble the matrix first");
+    GH_REQUIRE(lr >= 0.0f && std::isfinite(lr), GH_ERR_ARG, "bad learning rate");
+    const double l = gh::fsai_train_step(c, nprobes, seed, lr);
+    if (loss) *loss = l;
+  });
+}
+
+int gh_fsai_get(gh_ctx *ctx, float *feat, float *yout, float *gl, float *gu, float *grad) {
+  return guarded(ctx, [&](gh::Ctx &c) {
+    GH_REQUIRE(c.fsai_ready && c.have_mlp, GH_ERR_STATE, "no FSAI state: build or train first");
+    auto cp = [&](void *dst, const void *src, size_t bytes) {
+      if (dst) GH_CHECK_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c.stream));
+    };
+    cp(feat, c.feat.get(), sizeof(float) * c.ncells * gh::kMlpInPad);
+    cp(yout, c.yo.get(), sizeof(float) * c.ncells * gh::kMlpOutPad);
+    cp(gl, c.gl.get(), sizeof(float) * c.nnz);
+    cp(gu, c.gu.get(), sizeof(float) * c.nnz);
+    cp(grad, c.grads.get(), sizeof(float) * gh::kMlpParams);
+    GH_CHECK_HIP(hipStreamSynchronize(c.stream));
+  });
 }
 
 }  // extern "C"

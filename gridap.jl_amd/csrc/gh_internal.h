@@ -110,7 +110,25 @@ struct Ctx {
   DBuf<double> nzval, b;
 
   // ---- solver scratch ----
-  DBuf<double> s_r, s_z, s_p, s_q, s_x, s_dinv, s_b, s_part, s_scal;
+  DBuf<double> s_r, s_z, s_p, s_q, s_x, s_dinv, s_b, s_part, s_scal, s_y;
+
+  // ---- learned FSAI (lattice Q1) ----
+  bool fsai_ready = false;   // structures allocated for the current pattern
+  bool fsai_built = false;   // G holds the factor of the current matrix
+  DBuf<int32_t> diagpos;     // N: nzval position of each column's diagonal
+  DBuf<double> fs;           // N: diag(A)^(-1/2)
+  DBuf<float> gl, gu;        // NNZ: Ĝ in column-prefix (rows of G) and suffix (rows of Gᵀ) layout
+  DBuf<float> gG;            // NNZ: loss gradient w.r.t. Ĝ, prefix layout
+  DBuf<float> feat;          // C x 16 (10 used)
+  DBuf<float> z1, h1, z2, h2, z3, h3;  // C x 64 pre-/post-activations
+  DBuf<float> yo, dy;        // C x 40 (36 used): network output and its gradient
+  DBuf<float> dz1, dz2, dz3; // C x 64
+  DBuf<float> params, grads, adam_m, adam_v;  // NPARAM, FP32 master copy and Adam state
+  DBuf<uint16_t> params_bf16;                 // the weights the kernels use
+  DBuf<float> wpart;         // per-block weight-gradient partials
+  DBuf<double> pz, pw, pv, pu, pr, pt, pq, lpart;  // probe vectors, loss partials
+  int adam_t = 0;
+  bool have_mlp = false;
 };
 
 // pattern.hip
@@ -122,5 +140,16 @@ void assemble_diffusion(Ctx &c, int precision, int path);
 // solve.hip
 void spmv_device(Ctx &c, const double *x, double *y);
 void pcg_device(Ctx &c, int precond, double rtol, int maxit, gh_solve_stats *st);
+// fsai.hip
+constexpr int kMlpIn = 10, kMlpInPad = 16, kMlpHidden = 64, kMlpOut = 36, kMlpOutPad = 40;
+constexpr int kMlpW1 = 0, kMlpB1 = 640, kMlpW2 = 704, kMlpB2 = 4800, kMlpW3 = 4864,
+              kMlpB3 = 8960, kMlpW4 = 9024, kMlpB4 = 11328, kMlpParams = 11364;
+void fsai_prepare(Ctx &c);                       // diag positions, D^-1/2, buffers
+void fsai_forward_build(Ctx &c);                 // features -> MLP -> Ĝ
+double fsai_train_step(Ctx &c, int nprobes, uint64_t seed, float lr);
+void fsai_apply_lower(Ctx &c, const double *r, double *y);   // y = Ĝ (D^-1/2 r)
+void fsai_apply_upper(Ctx &c, const double *y, double *z, const double *rdot, double *part);
+void mlp_sync_bf16(Ctx &c);
+void grf_kappa(Ctx &c, int nmodes, const double *omega, const double *phase, double amp);
 
 }  // namespace gh

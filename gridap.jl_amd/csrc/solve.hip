@@ -79,22 +79,26 @@ __global__ void diag_inv_kernel(int64_t n, const int32_t *__restrict__ cp,
   dinv[j] = d != 0.0 ? 1.0 / d : 1.0;
 }
 
-// r = b - q, z = M r, p = z; partials of r.z, r.r, b.b.
+// r = b - q; with a diagonal (or no) preconditioner also z = M r, p = z.  With a
+// factored preconditioner (`factored`) z, p and the r.z partials come from the
+// apply kernels instead.  Partials of r.z, r.r, b.b.
 __global__ __launch_bounds__(kBlock) void init_kernel(int64_t n, const double *__restrict__ b,
                                                       const double *__restrict__ q,
                                                       const double *__restrict__ dinv,
-                                                      double *r, double *z, double *p,
-                                                      double *part) {
+                                                      int factored, double *r, double *z,
+                                                      double *p, double *part) {
   __shared__ double sh[kBlock / 64];
   double rz = 0.0, rr = 0.0, bb = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * kBlock) {
     double ri = b[i] - q[i];
-    double zi = dinv ? ri * dinv[i] : ri;
     r[i] = ri;
-    z[i] = zi;
-    p[i] = zi;
-    rz += ri * zi;
+    if (!factored) {
+      double zi = dinv ? ri * dinv[i] : ri;
+      z[i] = zi;
+      p[i] = zi;
+      rz += ri * zi;
+    }
     rr += ri * ri;
     bb += b[i] * b[i];
   }
@@ -102,10 +106,15 @@ __global__ __launch_bounds__(kBlock) void init_kernel(int64_t n, const double *_
   rr = block_sum(rr, sh);
   bb = block_sum(bb, sh);
   if (threadIdx.x == 0) {
-    part[blockIdx.x] = rz;
+    if (!factored) part[blockIdx.x] = rz;
     part[kParts + blockIdx.x] = rr;
     part[2 * kParts + blockIdx.x] = bb;
   }
+}
+
+__global__ void copy_kernel(int64_t n, const double *__restrict__ a, double *__restrict__ b) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = a[i];
 }
 
 // scal layout: [0..1] rz (double-buffered by iteration parity), [2] rr, [3] bb.
@@ -122,15 +131,16 @@ __global__ __launch_bounds__(kBlock) void init_scalars(const double *part, doubl
   }
 }
 
-// alpha = rz / (p.q); x += alpha p; r -= alpha q; z = M r; partials of r.z, r.r.
+// alpha = rz / (p.q); x += alpha p; r -= alpha q; z = M r (diagonal M; with a
+// factored M the apply kernels follow); partials of r.z, r.r.
 __global__ __launch_bounds__(kBlock) void update_kernel(int64_t n, int parity,
                                                         const double *__restrict__ pq_part,
                                                         const double *__restrict__ scal,
                                                         const double *__restrict__ p,
                                                         const double *__restrict__ q,
                                                         const double *__restrict__ dinv,
-                                                        double *x, double *r, double *z,
-                                                        double *part) {
+                                                        int factored, double *x, double *r,
+                                                        double *z, double *part) {
   __shared__ double sh[kBlock / 64];
   const double pq = sum_parts(pq_part, sh);
   const double alpha = scal[parity] / pq;
@@ -139,16 +149,18 @@ __global__ __launch_bounds__(kBlock) void update_kernel(int64_t n, int parity,
        i += (int64_t)gridDim.x * kBlock) {
     x[i] += alpha * p[i];
     double ri = r[i] - alpha * q[i];
-    double zi = dinv ? ri * dinv[i] : ri;
     r[i] = ri;
-    z[i] = zi;
-    rz += ri * zi;
+    if (!factored) {
+      double zi = dinv ? ri * dinv[i] : ri;
+      z[i] = zi;
+      rz += ri * zi;
+    }
     rr += ri * ri;
   }
   rz = block_sum(rz, sh);
   rr = block_sum(rr, sh);
   if (threadIdx.x == 0) {
-    part[blockIdx.x] = rz;
+    if (!factored) part[blockIdx.x] = rz;
     part[kParts + blockIdx.x] = rr;
   }
 }
@@ -196,11 +208,21 @@ void pcg_device(Ctx &c, int precond, double rtol, int maxit, gh_solve_stats *st)
   // Partials must be zero where a grid has fewer than kParts blocks (the SpMV).
   GH_CHECK_HIP(hipMemsetAsync(c.s_part.get(), 0, sizeof(double) * 4 * kParts, s));
   double *dinv = nullptr;
+  const int factored = precond == GH_PRECOND_FSAI;
   if (precond == GH_PRECOND_JACOBI) {
     hipLaunchKernelGGL(diag_inv_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n,
                        c.colptr.get(), c.rowval.get(), c.nzval.get(), c.s_dinv.get());
     dinv = c.s_dinv.get();
   }
+  if (factored) {
+    GH_REQUIRE(c.fsai_built, GH_ERR_STATE, "no FSAI factor for this matrix: call gh_fsai_build");
+    c.s_y.resize(std::max<int64_t>(n, 1));
+  }
+  // z = Gᵀ (G r) with the r.z partials in part[0, kParts)
+  auto apply_factored = [&]() {
+    fsai_apply_lower(c, c.s_r.get(), c.s_y.get());
+    fsai_apply_upper(c, c.s_y.get(), c.s_z.get(), c.s_r.get(), c.s_part.get());
+  };
   hipEvent_t e0, e1;
   GH_CHECK_HIP(hipEventCreate(&e0));
   GH_CHECK_HIP(hipEventCreate(&e1));
@@ -217,8 +239,13 @@ void pcg_device(Ctx &c, int precond, double rtol, int maxit, gh_solve_stats *st)
     // (re)start from the true residual r = b - A x
     spmv_device(c, c.s_x.get(), c.s_q.get());
     hipLaunchKernelGGL(init_kernel, dim3(vgrid), dim3(kBlock), 0, s, n, c.s_b.get(),
-                       c.s_q.get(), dinv, c.s_r.get(), c.s_z.get(), c.s_p.get(),
+                       c.s_q.get(), dinv, factored, c.s_r.get(), c.s_z.get(), c.s_p.get(),
                        c.s_part.get());
+    if (factored) {
+      apply_factored();
+      hipLaunchKernelGGL(copy_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n,
+                         c.s_z.get(), c.s_p.get());
+    }
     hipLaunchKernelGGL(init_scalars, dim3(1), dim3(kBlock), 0, s, c.s_part.get(),
                        c.s_scal.get());
     GH_CHECK_HIP(hipGetLastError());
@@ -239,8 +266,9 @@ void pcg_device(Ctx &c, int precond, double rtol, int maxit, gh_solve_stats *st)
         hipLaunchKernelGGL(spmv_kernel, dim3(mgrid), dim3(kBlock), 0, s, n, c.colptr.get(),
                            c.rowval.get(), c.nzval.get(), c.s_p.get(), c.s_q.get(), pq_part);
         hipLaunchKernelGGL(update_kernel, dim3(vgrid), dim3(kBlock), 0, s, n, parity, pq_part,
-                           c.s_scal.get(), c.s_p.get(), c.s_q.get(), dinv, c.s_x.get(),
-                           c.s_r.get(), c.s_z.get(), c.s_part.get());
+                           c.s_scal.get(), c.s_p.get(), c.s_q.get(), dinv, factored,
+                           c.s_x.get(), c.s_r.get(), c.s_z.get(), c.s_part.get());
+        if (factored) apply_factored();
         hipLaunchKernelGGL(direction_kernel, dim3(vgrid), dim3(kBlock), 0, s, n, parity,
                            c.s_part.get(), c.s_scal.get(), c.s_z.get(), c.s_p.get());
         parity ^= 1;

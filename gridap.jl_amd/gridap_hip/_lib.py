@@ -31,6 +31,7 @@ GH_PATH_GENERAL = 1
 GH_PATH_STRUCTURED = 2
 GH_PRECOND_NONE = 0
 GH_PRECOND_JACOBI = 1
+GH_PRECOND_FSAI = 2
 
 # Every symbol the header declares, with (restype, argtypes).
 _i32, _i64, _dbl, _vp = C.c_int32, C.c_int64, C.c_double, C.c_void_p
@@ -39,7 +40,8 @@ _pi64 = C.POINTER(C.c_int64)
 
 class SolveStats(C.Structure):
     _fields_ = [("iterations", C.c_int32), ("converged", C.c_int32),
-                ("rel_residual", C.c_double), ("device_ms", C.c_double)]
+                ("rel_residual", C.c_double), ("device_ms", C.c_double),
+                ("precond_used", C.c_int32), ("reserved", C.c_int32)]
 
 
 SIGNATURES = {
@@ -59,6 +61,14 @@ SIGNATURES = {
     "gh_numeric": (_i32, [_vp, _i32, _vp, _vp, _i32, _vp, _vp, _vp]),
     "gh_spmv": (_i32, [_vp, _vp, _vp]),
     "gh_pcg": (_i32, [_vp, _i32, _vp, _vp, _dbl, _i32, C.POINTER(SolveStats)]),
+    "gh_mlp_nparams": (_i32, [_pi64]),
+    "gh_mlp_init": (_i32, [_vp, C.c_uint64]),
+    "gh_mlp_set": (_i32, [_vp, _vp, _i64]),
+    "gh_mlp_get": (_i32, [_vp, _vp, _i64]),
+    "gh_grf_kappa": (_i32, [_vp, _i32, _vp, _vp, _dbl]),
+    "gh_fsai_build": (_i32, [_vp]),
+    "gh_fsai_train_step": (_i32, [_vp, _i32, C.c_uint64, C.c_float, C.POINTER(_dbl)]),
+    "gh_fsai_get": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp]),
 }
 
 _lib = None
@@ -108,6 +118,7 @@ class Context:
         self.device = device
         self.nfree = 0  # sizes of the arrays the library returns, tracked here so the
         self.nnz = 0    # host buffers always match what the library writes
+        self.ncells = 0
 
     def check(self, st):
         if st != GH_OK:
@@ -137,6 +148,7 @@ class Context:
         s = np.ascontiguousarray(sizes, dtype=np.float64)
         p = np.ascontiguousarray(partition, dtype=np.int64)
         self.check(lib().gh_mesh_cartesian(self.h, D, ptr(o), ptr(s), ptr(p)))
+        self.ncells = int(np.prod(p))
 
     def mesh_upload(self, xyz, cell_nodes_data, cell_nodes_ptrs):
         xyz = np.ascontiguousarray(xyz, dtype=np.float64)
@@ -144,6 +156,7 @@ class Context:
         p = np.ascontiguousarray(cell_nodes_ptrs, dtype=np.int32)
         self.check(lib().gh_mesh_upload(self.h, xyz.shape[1], xyz.shape[0], ptr(xyz),
                                          len(p) - 1, ptr(d), ptr(p)))
+        self.ncells = len(p) - 1
 
     def space_upload(self, reffe_kind, order, ncomp, data, ptrs, nfree, ndiri):
         d = np.ascontiguousarray(data, dtype=np.int32)
@@ -211,3 +224,47 @@ class Context:
         if code != GH_OK and (code != GH_ERR_NOCONV or raise_on_noconv):
             self.check(code)
         return x, st
+
+    # learned FSAI ------------------------------------------------------------
+    @staticmethod
+    def mlp_nparams():
+        n = C.c_int64()
+        st = lib().gh_mlp_nparams(C.byref(n))
+        if st != GH_OK:
+            raise GridapHipError(st, "gh_mlp_nparams")
+        return n.value
+
+    def mlp_init(self, seed):
+        self.check(lib().gh_mlp_init(self.h, C.c_uint64(seed)))
+
+    def mlp_set(self, params):
+        w = np.ascontiguousarray(params, dtype=np.float32)
+        self.check(lib().gh_mlp_set(self.h, ptr(w), w.size))
+
+    def mlp_get(self):
+        w = np.zeros(self.mlp_nparams(), dtype=np.float32)
+        self.check(lib().gh_mlp_get(self.h, ptr(w), w.size))
+        return w
+
+    def grf_kappa(self, omega, phase, amp):
+        om = np.ascontiguousarray(omega, dtype=np.float64).reshape(-1)
+        ph = np.ascontiguousarray(phase, dtype=np.float64)
+        self.check(lib().gh_grf_kappa(self.h, len(ph), ptr(om), ptr(ph), float(amp)))
+
+    def fsai_build(self):
+        self.check(lib().gh_fsai_build(self.h))
+
+    def fsai_train_step(self, nprobes, seed, lr):
+        loss = C.c_double()
+        self.check(lib().gh_fsai_train_step(self.h, int(nprobes), C.c_uint64(seed),
+                                            C.c_float(lr), C.byref(loss)))
+        return loss.value
+
+    def fsai_get(self):
+        feat = np.zeros((self.ncells, 16), dtype=np.float32)
+        yo = np.zeros((self.ncells, 40), dtype=np.float32)
+        gl = np.zeros(max(self.nnz, 1), dtype=np.float32)
+        gu = np.zeros(max(self.nnz, 1), dtype=np.float32)
+        gr = np.zeros(self.mlp_nparams(), dtype=np.float32)
+        self.check(lib().gh_fsai_get(self.h, ptr(feat), ptr(yo), ptr(gl), ptr(gu), ptr(gr)))
+        return dict(feat=feat, yout=yo, gl=gl[:self.nnz], gu=gu[:self.nnz], grad=gr)

@@ -243,7 +243,8 @@ class PCGSolver:
         self.rtol = rtol
         self.maxit = maxit
         self.precond = {"none": _lib.GH_PRECOND_NONE,
-                        "jacobi": _lib.GH_PRECOND_JACOBI}[precond]
+                        "jacobi": _lib.GH_PRECOND_JACOBI,
+                        "fsai": _lib.GH_PRECOND_FSAI}[precond]
         self.stats = None
 
     def solve(self, assem, b):

@@ -54,6 +54,8 @@ extern "C" {
 /* ---- preconditioners ---- */
 #define GH_PRECOND_NONE 0
 #define GH_PRECOND_JACOBI 1
+#define GH_PRECOND_FSAI 2 /* learned factorized sparse approximate inverse, M = GᵀG;
+                             Jacobi-PCG is rerun from x0 if it fails to converge  */
 
 typedef struct gh_ctx gh_ctx;
 
@@ -62,6 +64,8 @@ typedef struct gh_solve_stats {
   int32_t converged;   /* 1 when ||b - A x|| / ||b|| <= rtol (true residual, FP64) */
   double rel_residual; /* true residual at exit                                  */
   double device_ms;    /* solver wall time on the device stream                  */
+  int32_t precond_used; /* GH_PRECOND_* that produced x (FSAI falls back to JACOBI) */
+  int32_t reserved;
 } gh_solve_stats;
 
 /* Context.  Replaces nothing in Gridap (it is the state a GPU assembler holds). */
@@ -138,6 +142,32 @@ int gh_spmv(gh_ctx *ctx, const double *x, double *y);
  * Stands in for solve!(x, LUSolver(), op), src/Algebra/LinearSolvers.jl:202-274. */
 int gh_pcg(gh_ctx *ctx, int precond, const double *b, double *x, double rtol, int maxit,
            gh_solve_stats *stats);
+
+/* ---- learned FSAI preconditioner (3-D Cartesian mesh, scalar Q1) ----
+ * No counterpart in Gridap (it ships LU only, src/Algebra/LinearSolvers.jl:249-303).
+ * G = Ĝ D^(-1/2) on the pattern of tril(A); Ĝ is assembled from the outputs of a
+ * per-cell MLP (10 -> 64 -> 64 -> 64 -> 36, GELU, BF16 weights, FP32 accumulate). */
+
+/* Number of network parameters (flat FP32 layout W1,b1,W2,b2,W3,b3,W4,b4). */
+int gh_mlp_nparams(int64_t *n);
+/* Seeded initialisation (Xavier-uniform hidden layers, zero last layer: G = Jacobi). */
+int gh_mlp_init(gh_ctx *ctx, uint64_t seed);
+/* Set / get the FP32 master weights; setting resets the optimiser state. */
+int gh_mlp_set(gh_ctx *ctx, const float *params, int64_t n);
+int gh_mlp_get(gh_ctx *ctx, float *params, int64_t n);
+/* Per-cell κ = exp(amp Σ_k cos(ω_k·x_c + φ_k)) at the cell centres x_c, written to the
+ * device coefficient (a Gaussian random field by random Fourier features). */
+int gh_grf_kappa(gh_ctx *ctx, int nmodes, const double *omega, const double *phase,
+                 double amp);
+/* Features -> network -> Ĝ for the currently assembled matrix. */
+int gh_fsai_build(gh_ctx *ctx);
+/* One training step on the currently assembled matrix: nprobes Rademacher probes of
+ * ||I - G A Gᵀ||_F^2 / N, back-propagation and one Adam step (lr = 0: gradient only).
+ * loss receives the probe estimate before the step. */
+int gh_fsai_train_step(gh_ctx *ctx, int nprobes, uint64_t seed, float lr, double *loss);
+/* Diagnostic copies: features [C x 16], network output [C x 40], Ĝ in the column-prefix
+ * and column-suffix layouts [nnz], last parameter gradient [nparams] (any may be NULL). */
+int gh_fsai_get(gh_ctx *ctx, float *feat, float *yout, float *gl, float *gu, float *grad);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,215 @@
+"""Parity of the learned-FSAI kernels with the FP64 oracle (needs an MI355X).
+
+Bar (the network runs with BF16 weights and FP32 activations/accumulation, the
+oracle in FP64 with the same BF16-rounded weights): features within 1e-6
+relative, network outputs and Ĝ within 1e-4 of max |value|, probe loss within
+1e-5 relative, weight gradient within 1e-3 relative L2 per layer.  Probes are the
+same splitmix64 Rademacher vectors on both sides.
+"""
+import sys
+
+import numpy as np
+import pytest
+import scipy.sparse.linalg as spla
+
+import gridap_hip as G
+from gridap_hip import _lib
+import fsai_oracle as F
+
+pytestmark = pytest.mark.gpu
+
+PART = (6, 5, 4)
+
+
+def announce(msg):
+    print("[gpu-case] " + msg, flush=True)
+    sys.stdout.flush()
+
+
+def make_context(part, kappa):
+    model = G.CartesianDiscreteModel((0, 1, 0, 1, 0, 1), part)
+    reffe = G.ReferenceFE(G.lagrangian, G.Float64, 1)
+    V = G.TestFESpace(model, reffe, dirichlet_tags="boundary")
+    q = G.Quadrature(3, 2)
+    shp, grad = reffe.tabulate(3, q.coords)
+    cd = V.get_cell_dof_ids()
+    ptrs = (np.arange(cd.shape[0] + 1, dtype=np.int64) * cd.shape[1] + 1).astype(np.int32)
+    ctx = _lib.Context(0)
+    ctx.mesh_cartesian(model.origin, model.sizes, model.partition)
+    ctx.space_upload(_lib.GH_REFFE_QK, 1, 1, cd.ravel(), ptrs, V.nfree, V.ndirichlet)
+    ctx.reffe_tabulation(q.weights, shp, grad)
+    ctx.symbolic()
+    ctx.coef_upload(kappa=np.asarray(kappa, dtype=np.float64), fsrc=np.ones(len(kappa)))
+    ctx.assemble(_lib.GH_FORM_DIFFUSION, _lib.GH_PREC_FP64, _lib.GH_PATH_STRUCTURED)
+    return ctx
+
+
+def random_params(seed, scale4=0.05):
+    p = F.init_params(seed).astype(np.float64)
+    rng = np.random.default_rng(seed)
+    p[F.OFF["W4"]:] = rng.normal(0, scale4, F.NPARAMS - F.OFF["W4"])
+    p[F.OFF["B1"]:F.OFF["W2"]] = rng.normal(0, 0.1, 64)
+    return p.astype(np.float32)
+
+
+@pytest.fixture(scope="module")
+def case(oracle):
+    announce("fsai fixture: %s cells, lognormal kappa" % (PART,))
+    rng = np.random.default_rng(11)
+    kappa = np.exp(rng.normal(0, 1, int(np.prod(PART))))
+    prob = F.Problem(oracle, PART, kappa)
+    ctx = make_context(PART, kappa)
+    colptr, rowval = ctx.pattern_get()
+    return dict(prob=prob, ctx=ctx, kappa=kappa, colptr=colptr, rowval=rowval)
+
+
+def split_factor(c, gl, gu):
+    """Dense Ĝ from the device's column-prefix (gl) and column-suffix (gu) arrays."""
+    N = c["prob"].N
+    lo = np.zeros((N, N))
+    up = np.zeros((N, N))
+    cp, rv = c["colptr"], c["rowval"]
+    for j in range(N):
+        for p in range(cp[j] - 1, cp[j + 1] - 1):
+            i = rv[p] - 1
+            if i <= j:
+                lo[j, i] = gl[p]          # row j of Ĝ, column i <= j
+            if i >= j:
+                up[i, j] = gu[p]          # Ĝ(i, j), i >= j
+    return lo, up
+
+
+def test_mlp_init_matches_oracle(case):
+    announce("fsai: mlp_init weights")
+    ctx = case["ctx"]
+    for seed in (0, 7, 2**63 + 5):
+        ctx.mlp_init(seed)
+        np.testing.assert_array_equal(ctx.mlp_get(), F.init_params(seed))
+
+
+def test_forward_matches_oracle(case):
+    announce("fsai: features, network, factor")
+    ctx, prob = case["ctx"], case["prob"]
+    p = random_params(3)
+    ctx.mlp_set(p)
+    ctx.fsai_build()
+    got = ctx.fsai_get()
+    X = prob.features()
+    feat = got["feat"].astype(np.float64)
+    np.testing.assert_allclose(feat[:, :10], X, rtol=1e-6, atol=1e-7)
+    assert np.all(feat[:, 10:] == 0)
+    y, _ = F.mlp_forward(p, X)
+    yd = got["yout"][:, :36].astype(np.float64)
+    assert np.max(np.abs(yd - y)) <= 1e-4 * np.max(np.abs(y))
+    Gh = prob.ghat(y)
+    lo, up = split_factor(case, got["gl"], got["gu"])
+    assert np.max(np.abs(lo - Gh)) <= 1e-4 * np.max(np.abs(Gh))
+    assert np.max(np.abs(up - Gh)) <= 1e-4 * np.max(np.abs(Gh))
+
+
+def test_train_step_matches_oracle(case):
+    announce("fsai: loss and weight gradient")
+    ctx, prob = case["ctx"], case["prob"]
+    p = random_params(5, scale4=0.1)
+    ctx.mlp_set(p)
+    seed, m = 1234, 3
+    loss = ctx.fsai_train_step(m, seed, 0.0)          # lr 0: gradient only
+    np.testing.assert_array_equal(ctx.mlp_get(), p)   # weights untouched
+    L, g, _ = prob.loss_and_grad(p, seed, m)
+    assert abs(loss - L) <= 1e-5 * L
+    gd = ctx.fsai_get()["grad"].astype(np.float64)
+    for name, lo, hi in (("W1", 0, 704), ("W2", 704, 4864), ("W3", 4864, 9024),
+                         ("W4", 9024, F.NPARAMS)):
+        err = np.linalg.norm(gd[lo:hi] - g[lo:hi]) / np.linalg.norm(g[lo:hi])
+        assert err <= 1e-3, (name, err)
+
+
+def test_factor_is_deterministic(case):
+    announce("fsai: determinism")
+    ctx = case["ctx"]
+    ctx.mlp_set(random_params(6))
+    ctx.fsai_build()
+    a = ctx.fsai_get()
+    l1 = ctx.fsai_train_step(2, 99, 0.0)
+    g1 = ctx.fsai_get()["grad"]
+    ctx.fsai_build()
+    b = ctx.fsai_get()
+    l2 = ctx.fsai_train_step(2, 99, 0.0)
+    g2 = ctx.fsai_get()["grad"]
+    np.testing.assert_array_equal(a["gl"], b["gl"])
+    np.testing.assert_array_equal(a["gu"], b["gu"])
+    assert l1 == l2
+    np.testing.assert_array_equal(g1, g2)
+
+
+def test_untrained_network_is_jacobi(case):
+    announce("fsai: zero last layer vs Jacobi-PCG")
+    ctx, prob = case["ctx"], case["prob"]
+    ctx.mlp_init(1)
+    ctx.fsai_build()
+    got = ctx.fsai_get()
+    lo, up = split_factor(case, got["gl"], got["gu"])
+    np.testing.assert_array_equal(lo, np.eye(prob.N))
+    np.testing.assert_array_equal(up, np.eye(prob.N))
+    xj, sj = ctx.pcg(precond=_lib.GH_PRECOND_JACOBI, rtol=1e-10)
+    xf, sf = ctx.pcg(precond=_lib.GH_PRECOND_FSAI, rtol=1e-10)
+    assert sf.precond_used == _lib.GH_PRECOND_FSAI
+    assert abs(sf.iterations - sj.iterations) <= 1
+    np.testing.assert_allclose(xf, xj, rtol=1e-8, atol=1e-12 * np.max(np.abs(xj)))
+
+
+def test_fsai_pcg_solves_the_system(case, oracle):
+    announce("fsai: FSAI-PCG solution vs scipy")
+    ctx, prob = case["ctx"], case["prob"]
+    ctx.mlp_set(random_params(8, scale4=0.05))
+    ctx.fsai_build()
+    x, st = ctx.pcg(precond=_lib.GH_PRECOND_FSAI, rtol=1e-10)
+    assert st.converged == 1 and st.precond_used == _lib.GH_PRECOND_FSAI
+    nz, b = ctx.values_get()
+    A = oracle.to_scipy(case["colptr"], case["rowval"], nz, prob.N).tocsc()
+    xs = spla.spsolve(A, b)
+    assert np.max(np.abs(x - xs)) <= 1e-8 * np.max(np.abs(xs))
+
+
+def test_diverging_factor_falls_back_to_jacobi(case):
+    announce("fsai: overflowing factor -> Jacobi fallback")
+    ctx = case["ctx"]
+    p = F.init_params(2)
+    p[F.OFF["B4"]:F.OFF["B4"] + 8] = 100.0            # Ĝ_ii = exp(800) = inf in FP32
+    ctx.mlp_set(p)
+    x, st = ctx.pcg(precond=_lib.GH_PRECOND_FSAI, rtol=1e-10)
+    assert st.converged == 1 and st.precond_used == _lib.GH_PRECOND_JACOBI
+    xj, _ = ctx.pcg(precond=_lib.GH_PRECOND_JACOBI, rtol=1e-10)
+    np.testing.assert_allclose(x, xj, rtol=1e-9, atol=1e-14)
+
+
+def test_training_lowers_the_loss(case):
+    announce("fsai: 30 Adam steps")
+    ctx = case["ctx"]
+    ctx.mlp_init(4)
+    first = ctx.fsai_train_step(4, 77, 0.0)
+    for k in range(30):
+        ctx.fsai_train_step(4, 1000 + k, 3e-3)
+    last = ctx.fsai_train_step(4, 77, 0.0)
+    assert last < 0.9 * first, (first, last)
+    ctx.fsai_build()
+    x, st = ctx.pcg(precond=_lib.GH_PRECOND_FSAI, rtol=1e-10)
+    assert st.converged == 1 and st.precond_used == _lib.GH_PRECOND_FSAI
+
+
+def test_grf_kappa_matches_formula(case):
+    announce("fsai: random-field coefficient")
+    from gridap_hip.fsai import grf_modes
+    ctx = make_context(PART, np.ones(int(np.prod(PART))))
+    omega, phase, amp = grf_modes(3, nmodes=16, corr_len=0.3)
+    ctx.grf_kappa(omega, phase, amp)
+    ctx.assemble(_lib.GH_FORM_DIFFUSION, _lib.GH_PREC_FP64, _lib.GH_PATH_STRUCTURED)
+    ctx.mlp_init(0)
+    ctx.fsai_build()
+    logk = ctx.fsai_get()["feat"][:, 8].astype(np.float64)
+    h = 1.0 / np.array(PART)
+    c = np.arange(int(np.prod(PART)))
+    xc = np.stack([(c % PART[0] + 0.5) * h[0], ((c // PART[0]) % PART[1] + 0.5) * h[1],
+                   (c // (PART[0] * PART[1]) + 0.5) * h[2]], axis=1)
+    want = amp * np.cos(xc @ omega.T + phase).sum(axis=1)
+    np.testing.assert_allclose(logk, want, rtol=1e-6, atol=1e-6)
