@@ -87,6 +87,10 @@ def main():
     ap.add_argument("--path", default="auto", choices=["auto", "general", "structured"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-pcg", action="store_true", help="skip the Jacobi-PCG solve")
+    ap.add_argument("--no-fsai", action="store_true",
+                    help="skip the learned-FSAI vs Jacobi PCG comparison (64^3 GRF instances)")
+    ap.add_argument("--fsai-instances", type=int, default=16,
+                    help="held-out instances in the learned-FSAI comparison")
     ap.add_argument("--seed", type=int, default=2024)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -169,7 +173,9 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 element math, f64 nzval/b" if args.precision == "fp32" else "f64",
+        # the lattice kernel computes in FP64 in both precision modes
+        "dtype": "f32 element math, f64 nzval/b"
+                 if args.precision == "fp32" and path == _lib.GH_PATH_GENERAL else "f64",
         "data": "synthetic (κ=exp(N(0,1)) per cell, seed %d)" % args.seed,
         "config": {
             "workload": "3D Q1 Poisson numeric assembly (A and b), %d^3 HEX, degree 2, "
@@ -218,7 +224,38 @@ def main():
         out["gpu_vs_cpu_rel_frobenius"] = float(np.linalg.norm(nzval - nz_cpu) /
                                                 np.linalg.norm(nz_cpu))
     ctx.close()
+
+    if not args.no_fsai:
+        out["pcg_learned_fsai"] = fsai_leg(args.fsai_instances)
     print(json.dumps(out), flush=True)
+
+
+def fsai_leg(k):
+    """Learned-FSAI-PCG vs Jacobi-PCG to relative residual 1e-8 on held-out 64^3 Q1
+    Poisson instances with κ = exp(GRF), using the committed trained weights."""
+    from gridap_hip import fsai as FS
+    if not os.path.exists(FS.DEFAULT_WEIGHTS):
+        return {"skipped": "no trained weights at %s" % FS.DEFAULT_WEIGHTS}
+    inst = FS.PoissonInstances(64)
+    net = FS.LearnedFSAI.load(FS.DEFAULT_WEIGHTS)
+    net.attach(inst.ctx)
+    rows = FS.evaluate(inst, list(FS.HELDOUT_SEEDS)[:k])
+    sm = FS.summarize(rows)
+    inst.ctx.close()
+    return {
+        "workload": "64^3 HEX Q1 Poisson, κ=exp(GRF: 64 Fourier modes, ℓ=0.1), held-out "
+                    "seeds %d..%d, FP64 PCG from x0=0 to ‖r‖/‖b‖<=1e-8" %
+                    (FS.HELDOUT_SEEDS[0], FS.HELDOUT_SEEDS[0] + k - 1),
+        "free_dofs": int(inst.nfree), "nnz": int(inst.nnz), "instances": k,
+        "jacobi": {"iterations": sm["jacobi_iters"], "seconds": sm["jacobi_ms"] * 1e-3},
+        "learned_fsai": {"iterations": sm["fsai_iters"], "seconds": sm["fsai_ms"] * 1e-3,
+                         "setup_seconds": sm["fsai_setup_ms"] * 1e-3,
+                         "seconds_with_setup": (sm["fsai_ms"] + sm["fsai_setup_ms"]) * 1e-3,
+                         "fallbacks_to_jacobi": sm["fallbacks"]},
+        "speedup_solve": sm["time_ratio_solve"],
+        "speedup_with_setup": sm["time_ratio_with_setup"],
+        "iteration_ratio": sm["iteration_ratio"],
+    }
 
 
 if __name__ == "__main__":

@@ -1,0 +1,380 @@
+"""Plain-Python restatement of Gridap's simplex/conforming path (TEST INFRASTRUCTURE
+ONLY): extrusion-polytope face order, first-touch grid topology, the conforming
+dof numbering, hex -> tet simplexification and the P2 tetrahedral Poisson
+assembly.  Loops are written for clarity on small meshes; each function cites the
+Gridap lines it follows (Gridap.jl v0.20.8).
+"""
+import math
+
+import numpy as np
+
+HEX_AXIS, TET_AXIS = 1, 2
+
+
+# ---- reference polytopes -------------------------------------------------------
+
+def _nface_boundary(anchor, extrusion, extend, isanchor, out):
+    """src/ReferenceFEs/ExtrusionPolytopes.jl:_nfaceboundary! — recursive
+    enumeration of the n-faces (anchor, extrusion) of an extrusion polytope."""
+    D = len(extrusion)
+    out.append((tuple(anchor), tuple(extrusion)))
+    ext = list(extend)
+    for i in range(D):
+        cur = ext[i]
+        if cur <= 0:
+            continue
+        ext = [0 if j == i else ext[j] for j in range(D)]
+        unit = [1 if j == i else 0 for j in range(D)]
+        if cur == HEX_AXIS:
+            _nface_boundary([a + u for a, u in zip(anchor, unit)], extrusion, ext, False, out)
+        elif isanchor:
+            base = [anchor[j] if j >= i else 0 for j in range(D)]
+            _nface_boundary([a + u for a, u in zip(base, unit)], extrusion, ext, False, out)
+        _nface_boundary(anchor, [x + u * cur for x, u in zip(extrusion, unit)], ext, False, out)
+    return out
+
+
+def _nface_dim(extrusion):
+    return sum(1 for e in extrusion if e > 0)
+
+
+def polytope_nfaces(anchor, extrusion):
+    """The n-faces of the face (anchor, extrusion), in Gridap's order
+    (src/ReferenceFEs/ExtrusionPolytopes.jl:_polytopenfaces): by dimension, then by
+    the extrusion (last axis most significant), then by the anchor (last axis most
+    significant); ties keep the recursion order."""
+    D = len(extrusion)
+    faces = _nface_boundary(list(anchor), [0] * D, list(extrusion), True, [])
+    keyed = sorted(range(len(faces)), key=lambda k: (
+        _nface_dim(faces[k][1]),
+        tuple(faces[k][1][i] for i in reversed(range(D))),
+        tuple(faces[k][0][i] for i in reversed(range(D))), k))
+    return [faces[k] for k in keyed]
+
+
+class Polytope:
+    """Face lists of an extrusion polytope: faces[d] = list of vertex tuples
+    (1-based local vertex ids), vertex coordinates."""
+
+    def __init__(self, extrusion):
+        self.extrusion = tuple(extrusion)
+        self.D = len(extrusion)
+        nf = polytope_nfaces([0] * self.D, extrusion)
+        ids = {f: k for k, f in enumerate(nf)}
+        self.nfaces = nf
+        self.faces = [[] for _ in range(self.D + 1)]
+        for f in nf:
+            sub = polytope_nfaces(f[0], f[1])
+            verts = tuple(ids[s] + 1 for s in sub if _nface_dim(s[1]) == 0)
+            self.faces[_nface_dim(f[1])].append(verts)
+        self.vertex_coords = np.array([f[0] for f in nf if _nface_dim(f[1]) == 0], dtype=float)
+
+
+TET = Polytope((TET_AXIS,) * 3)
+TRI = Polytope((TET_AXIS,) * 2)
+HEX = Polytope((HEX_AXIS,) * 3)
+QUAD = Polytope((HEX_AXIS,) * 2)
+
+
+def oriented_simplexify_hex():
+    """The six tetrahedra of a HEX, local vertex lists (1-based), from
+    src/ReferenceFEs/ExtrusionPolytopes.jl:290-299 (oriented variant, the default of
+    simplexify)."""
+    return [(1, 2, 3, 7), (1, 2, 5, 7), (2, 3, 4, 7), (2, 4, 7, 8), (2, 5, 6, 7), (2, 6, 7, 8)]
+
+
+# ---- grid topology ------------------------------------------------------------------
+
+def first_touch_faces(cell_vertices, lface_vertices):
+    """cell -> d-face ids (1-based) by first touch: cells in order, local faces in
+    order; an unnumbered face gets the next id in every cell that holds it
+    (src/Geometry/GridTopologies.jl:1184-1251, _cell_to_faces_fill!)."""
+    ncells = len(cell_vertices)
+    out = np.zeros((ncells, len(lface_vertices)), dtype=np.int64)
+    seen = {}
+    nxt = 1
+    for c in range(ncells):
+        for lf, lv in enumerate(lface_vertices):
+            key = frozenset(int(cell_vertices[c][v - 1]) for v in lv)
+            if key not in seen:
+                seen[key] = nxt
+                nxt += 1
+            out[c, lf] = seen[key]
+    return out, nxt - 1
+
+
+# ---- conforming numbering -------------------------------------------------------------
+
+def conforming_cell_dofs(d_cell_faces, d_face_tags, d_lface_own, nloc):
+    """Gridap's generic conforming numbering (src/FESpaces/ConformingFESpaces.jl:
+    367-575): every d-face owns the dofs its reference face owns; faces are visited
+    dimension by dimension in face-id order and their own dofs get the next free
+    (tag unset) or Dirichlet (tag set) id; cell dofs are gathered face by face.
+
+    d_cell_faces[d]: cells x nlfaces_d array of 1-based d-face ids.
+    d_face_tags[d]: per d-face, 0 = free, else the Dirichlet tag (1-based).
+    d_lface_own[d][lf]: local dof ids (0-based) owned by local d-face lf, in order.
+    Returns (cell_dofs [cells x nloc], nfree, ndiri, dirichlet_dof_tag)."""
+    D = len(d_cell_faces) - 1
+    ncells = d_cell_faces[0].shape[0]
+    face_dofs = []
+    nfree = ndiri = 0
+    diri_tag = []
+    for d in range(D + 1):
+        nface = len(d_face_tags[d])
+        # own dof count of each d-face: from any cell holding it (one cell type)
+        nown = np.zeros(nface, dtype=np.int64)
+        for c in range(ncells):
+            for lf, f in enumerate(d_cell_faces[d][c]):
+                nown[f - 1] = len(d_lface_own[d][lf])
+        ids = []
+        for f in range(nface):
+            tag = d_face_tags[d][f]
+            mine = []
+            for _ in range(nown[f]):
+                if tag == 0:
+                    nfree += 1
+                    mine.append(nfree)
+                else:
+                    ndiri += 1
+                    mine.append(-ndiri)
+                    diri_tag.append(tag)
+            ids.append(mine)
+        face_dofs.append(ids)
+    cd = np.zeros((ncells, nloc), dtype=np.int64)
+    for c in range(ncells):
+        for d in range(D + 1):
+            for lf, f in enumerate(d_cell_faces[d][c]):
+                for k, ldof in enumerate(d_lface_own[d][lf]):
+                    cd[c, ldof] = face_dofs[d][f - 1][k]
+    return cd, nfree, ndiri, diri_tag
+
+
+def lagrangian_own_dofs(poly, order):
+    """Local dofs owned by each face for scalar Lagrangian order 1 or 2 with Gridap's
+    node order (vertices, then one node per edge, then per quad face / hex interior
+    for Q2), src/ReferenceFEs/LagrangianRefFEs.jl own-nodes ordering."""
+    D = poly.D
+    out = []
+    k = 0
+    for d in range(D + 1):
+        per = []
+        for lf, verts in enumerate(poly.faces[d]):
+            if d == 0:
+                per.append([k])
+                k += 1
+            elif order == 2 and (d == 1 or len(verts) == 2 ** d):
+                per.append([k])   # edge midpoint; face/cell centre of an n-cube face
+                k += 1
+            else:
+                per.append([])
+        out.append(per)
+    return out, k
+
+
+# ---- Cartesian box entities (labels) ---------------------------------------------------
+
+def box_entity(D, fixed):
+    """1-based id of the face of the D-cube [0,1]^D where axis d is fixed at
+    fixed[d] in {0, 1} or free (None): Gridap's n-cube face order, which is also the
+    CartesianDiscreteModel entity numbering (src/Geometry/CartesianDiscreteModels.jl)."""
+    poly = HEX if D == 3 else QUAD
+    ext = tuple(0 if f is not None else HEX_AXIS for f in fixed)
+    anc = tuple(f if f is not None else 0 for f in fixed)
+    for k, f in enumerate(poly.nfaces):
+        if f == (anc, ext):
+            return k + 1
+    raise ValueError(fixed)
+
+
+def simplex_face_entity(D, partition, node_ijk_list):
+    """Entity label of a face of the simplexified Cartesian model whose vertices
+    have lattice coordinates node_ijk_list: the smallest box face containing them
+    (the label transfer of src/Geometry/UnstructuredDiscreteModels.jl:85-311 for
+    vertices, edges on hex edges/faces, face diagonals and body diagonals)."""
+    fixed = []
+    for d in range(D):
+        vals = {int(ijk[d]) for ijk in node_ijk_list}
+        if vals == {0}:
+            fixed.append(0)
+        elif vals == {partition[d]}:
+            fixed.append(1)
+        else:
+            fixed.append(None)
+    return box_entity(D, fixed)
+
+
+def boundary_entities(D):
+    return set(range(1, 3 ** D))        # every entity but the interior (the last)
+
+
+# ---- the simplexified Cartesian model ------------------------------------------------
+
+def simplexify_cartesian(partition):
+    """Tet cells (0-based node ids, hex-major, the six local tets in order) of the
+    simplexified CartesianDiscreteModel (src/Geometry/Grids.jl:487-568)."""
+    n0, n1, n2 = partition
+    tets = oriented_simplexify_hex()
+    cells = []
+    for k in range(n2):
+        for j in range(n1):
+            for i in range(n0):
+                hexn = [(i + a) + (n0 + 1) * ((j + b) + (n1 + 1) * (k + c))
+                        for c in (0, 1) for b in (0, 1) for a in (0, 1)]
+                for t in tets:
+                    cells.append([hexn[v - 1] for v in t])
+    return np.array(cells, dtype=np.int64)
+
+
+def node_ijk(partition, n):
+    n0, n1 = partition[0] + 1, partition[1] + 1
+    return (n % n0, (n // n0) % n1, n // (n0 * n1))
+
+
+def p2_tet_numbering(partition, dirichlet_entities):
+    """P2 conforming cell dofs on the simplexified Cartesian grid."""
+    cells = simplexify_cartesian(partition)
+    D = 3
+    nnodes = int(np.prod([p + 1 for p in partition]))
+    cell_vertices = cells + 1
+    vfaces = cell_vertices.copy()                           # d = 0: vertices
+    edges, nedges = first_touch_faces(cells, TET.faces[1])
+    faces2, nf2 = first_touch_faces(cells, TET.faces[2])
+    cellsd = np.arange(1, len(cells) + 1).reshape(-1, 1)
+    # tags
+    vtag = [1 if simplex_face_entity(D, partition, [node_ijk(partition, n)]) in dirichlet_entities
+            else 0 for n in range(nnodes)]
+    etag = [0] * nedges
+    for c in range(len(cells)):
+        for le, lv in enumerate(TET.faces[1]):
+            e = edges[c, le] - 1
+            ent = simplex_face_entity(D, partition, [node_ijk(partition, cells[c][v - 1]) for v in lv])
+            etag[e] = 1 if ent in dirichlet_entities else 0
+    ftag = [0] * nf2
+    ctag = [0] * len(cells)
+    own, nloc = lagrangian_own_dofs(TET, 2)
+    cd, nfree, ndiri, _ = conforming_cell_dofs([vfaces, edges, faces2, cellsd],
+                                               [vtag, etag, ftag, ctag], own, nloc)
+    return cells, cd, nfree, ndiri, edges
+
+
+# ---- P2 on the tetrahedron ---------------------------------------------------------------
+
+def p2_tet_nodes():
+    """Gridap's P2 TET nodes: the 4 vertices, then the 6 edge midpoints in TET edge
+    order."""
+    V = TET.vertex_coords
+    mids = [0.5 * (V[a - 1] + V[b - 1]) for a, b in TET.faces[1]]
+    return np.vstack([V, np.array(mids)])
+
+
+def p2_tet_tabulate(x):
+    """Nodal P2 basis on the reference tet at points x [nq x 3]: values [nq x 10],
+    gradients [nq x 10 x 3].  λ = (1-x-y-z, x, y, z) are the vertex barycentrics;
+    vertex a: λ_a(2λ_a - 1); edge (a,b): 4 λ_a λ_b."""
+    x = np.atleast_2d(x)
+    lam = np.stack([1 - x.sum(1), x[:, 0], x[:, 1], x[:, 2]], 1)
+    dlam = np.array([[-1, -1, -1], [1, 0, 0], [0, 1, 0], [0, 0, 1]], dtype=float)
+    nq = len(x)
+    shp = np.zeros((nq, 10))
+    grad = np.zeros((nq, 10, 3))
+    for a in range(4):
+        shp[:, a] = lam[:, a] * (2 * lam[:, a] - 1)
+        grad[:, a, :] = (4 * lam[:, a] - 1)[:, None] * dlam[a]
+    for k, (a, b) in enumerate(TET.faces[1]):
+        a -= 1
+        b -= 1
+        shp[:, 4 + k] = 4 * lam[:, a] * lam[:, b]
+        grad[:, 4 + k, :] = 4 * (lam[:, a][:, None] * dlam[b] + lam[:, b][:, None] * dlam[a])
+    return shp, grad
+
+
+def tet_monomial_integral(a, b, c):
+    """∫ x^a y^b z^c over the reference tet = a! b! c! / (a+b+c+3)!"""
+    return math.factorial(a) * math.factorial(b) * math.factorial(c) / math.factorial(a + b + c + 3)
+
+
+def p2_tet_stiffness_exact():
+    """Reference-tet P2 stiffness ∫∇φ_a·∇φ_b by exact polynomial integration (no
+    quadrature): the gradients are linear in (x, y, z)."""
+    # each gradient component = c0 + c1 x + c2 y + c3 z; coefficients from the
+    # tabulation at 4 points (exact for linear functions)
+    pts = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]], dtype=float)
+    _, g = p2_tet_tabulate(pts)                     # [4 x 10 x 3]
+    # linear function values at the vertices -> monomial coefficients
+    M = np.array([[1, 0, 0, 0], [1, 1, 0, 0], [1, 0, 1, 0], [1, 0, 0, 1]], dtype=float)
+    coef = np.einsum("pv,vad->pad", np.linalg.inv(M), g)  # [4 coefs x 10 x 3]
+    I = np.zeros((4, 4))
+    mono = [(0, 0, 0), (1, 0, 0), (0, 1, 0), (0, 0, 1)]
+    for i, mi in enumerate(mono):
+        for j, mj in enumerate(mono):
+            I[i, j] = tet_monomial_integral(*(np.add(mi, mj)))
+    return np.einsum("iad,ij,jbd->ab", coef, I, coef)
+
+
+def p2_tet_poisson(partition, xyz, kappa=None, fsrc=None, qx=None, qw=None,
+                   dirichlet_entities=None):
+    """Cell matrices and vectors of ∫κ∇v·∇u, ∫f v with P2 on the affine tets
+    (J from the 4 vertices, |det J| measure), FP64, and the global assembly via the
+    given assembler callable later.  Returns a dict like the Q1 driver."""
+    D = 3
+    if dirichlet_entities is None:
+        dirichlet_entities = boundary_entities(D)
+    cells, cd, nfree, ndiri, edges = p2_tet_numbering(partition, dirichlet_entities)
+    C = len(cells)
+    kappa = np.ones(C) if kappa is None else np.asarray(kappa, dtype=float)
+    fsrc = np.ones(C) if fsrc is None else np.asarray(fsrc, dtype=float)
+    shp, grad = p2_tet_tabulate(qx)
+    mats = np.zeros((C, 10, 10))
+    vecs = np.zeros((C, 10))
+    for c in range(C):
+        X = xyz[cells[c]]                              # 4 x 3
+        J = (X[1:] - X[0]).T                           # J_ij = ∂x_i/∂ξ_j
+        det = np.linalg.det(J)
+        Jinv = np.linalg.inv(J)
+        for q in range(len(qw)):
+            G = grad[q] @ Jinv                         # rows: J⁻ᵀ ∇̂φ_a
+            dV = qw[q] * abs(det)
+            mats[c] += kappa[c] * dV * (G @ G.T)
+            vecs[c] += fsrc[c] * dV * shp[q]
+    return dict(cells=cells, cell_dofs=cd, nfree=nfree, ndiri=ndiri, cellmats=mats,
+                cellvecs=vecs, edges=edges)
+
+
+# ---- quadrature -----------------------------------------------------------------------
+
+def _tet_orbit_points(w1, a1, w2, a2, w3, b):
+    pts, wts = [], []
+    for w, a in ((w1, a1), (w2, a2)):                   # S31 orbits: (a, a, a, 1-3a)
+        for k in range(4):
+            lam = [a] * 4
+            lam[k] = 1 - 3 * a
+            pts.append(lam[1:])
+            wts.append(w)
+    for i in range(4):                                   # S22 orbit: (b, b, ½-b, ½-b)
+        for j in range(i + 1, 4):
+            lam = [0.5 - b] * 4
+            lam[i] = lam[j] = b
+            pts.append(lam[1:])
+            wts.append(w3)
+    return np.array(pts), np.array(wts)
+
+
+def tet_degree5_rule():
+    """The symmetric 14-point rule on the reference tet exact to degree 5 (two S31
+    orbits and one S22 orbit, interior points, positive weights): Gridap's
+    Quadrature(TET, 4) and (TET, 5) use a rule of this structure
+    (src/ReferenceFEs/Quadratures.jl:157-175, Witherden & Vincent 2015).  Derived
+    here from the moment equations by Newton's method; the interior,
+    positive-weight root is unique for this orbit structure."""
+    from scipy.optimize import least_squares
+    exps = [(a, b, c) for a in range(6) for b in range(6) for c in range(6) if a + b + c <= 5]
+
+    def res(p):
+        P, W = _tet_orbit_points(*p)
+        return [np.sum(W * P[:, 0] ** a * P[:, 1] ** b * P[:, 2] ** c) - tet_monomial_integral(a, b, c)
+                for a, b, c in exps]
+
+    r = least_squares(res, [0.012, 0.09, 0.019, 0.31, 0.007, 0.045], xtol=3e-16, ftol=3e-16,
+                      gtol=3e-16)
+    return _tet_orbit_points(*r.x)
