@@ -531,6 +531,13 @@ void assemble_diffusion(Ctx &c, int precision, int path) {
              "kappa was uploaded for a different mesh");
   GH_REQUIRE(!c.have_dvals || c.dvals.n == (size_t)c.ndiri, GH_ERR_STATE,
              "Dirichlet values were uploaded for a different space");
+  if (c.simplex) {
+    GH_REQUIRE(path != GH_PATH_STRUCTURED, GH_ERR_UNSUPPORTED,
+               "the lattice path needs a Cartesian mesh, a scalar Q1 CLagrangian space and a "
+               "symmetric quadrature rule");
+    assemble_simplex(c, precision);
+    return;
+  }
   double Kc[8] = {0}, b0 = 0.0;
   bool lattice = c.structured && path != GH_PATH_GENERAL;
   if (lattice) {

@@ -40,6 +40,7 @@ void invalidate_pattern(gh::Ctx &c) {
   c.color_ptr.clear();
   c.fsai_ready = false;
   c.fsai_built = false;
+  c.simplex = false;
 }
 
 }  // namespace
@@ -212,6 +213,7 @@ int gh_symbolic(gh_ctx *ctx, int64_t *nnz_out) {
     invalidate_pattern(c);
     gh::symbolic_general(c);
     gh::check_structured(c);
+    gh::simplex_symbolic(c);
     if (nnz_out) *nnz_out = c.nnz;
   });
 }

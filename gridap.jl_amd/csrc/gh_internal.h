@@ -98,6 +98,14 @@ struct Ctx {
   DBuf<int32_t> tile_p0;            // per 64-node tile: nzval start of its columns, or -1
   DBuf<int32_t> tile_d0;            // per 64-node tile: dof of its first free node, or -1
 
+  // ---- affine simplices (P1/P2 tetrahedra) ----
+  bool simplex = false;
+  int max_col = 0;                  // longest CSC column
+  DBuf<int32_t> adjptr;             // nfree+1: cells of each free dof
+  DBuf<int32_t> adj;                // (cell << 4 | local index), cells ascending per dof
+  DBuf<double> geo;                 // ncells x 8: κ|det| DF⁻¹DF⁻ᵀ (6), f|det|, pad
+  DBuf<double> sref;                // reference tensor Ĝ (6 x nloc x nloc) and m (nloc)
+
   // ---- colouring for the general path ----
   int ncolors = 0;
   std::vector<int64_t> color_ptr;   // ncolors+1
@@ -137,6 +145,10 @@ void check_structured(Ctx &c);
 void build_coloring(Ctx &c);
 // assemble.hip
 void assemble_diffusion(Ctx &c, int precision, int path);
+// simplex.hip
+bool simplex_eligible(const Ctx &c);
+void simplex_symbolic(Ctx &c);
+void assemble_simplex(Ctx &c, int precision);
 // solve.hip
 void spmv_device(Ctx &c, const double *x, double *y);
 void pcg_device(Ctx &c, int precond, double rtol, int maxit, gh_solve_stats *st);

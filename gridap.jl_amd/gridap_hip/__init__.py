@@ -22,14 +22,16 @@ from .assemblers import (AffineFEOperator, LinearFESolver, PCGSolver, SparseMatr
 from .celldata import (CellField, DomainContribution, Measure, dot, get_fe_basis,
                        get_trial_fe_basis, grad, gradient, inner, integral, integrate)
 from .fespaces import FEFunction, FESpace, TestFESpace, TrialFESpace
-from .geometry import CartesianDiscreteModel, FaceLabeling, Triangulation, get_triangulation
+from .geometry import (CartesianDiscreteModel, FaceLabeling, Triangulation, UnstructuredDiscreteModel,
+                       get_triangulation, simplexify)
 from .reference_fes import Float64, Quadrature, ReferenceFE, VectorValue, lagrangian
 
 __all__ = [
     "AffineFEOperator", "CartesianDiscreteModel", "CellField", "DomainContribution",
     "FEFunction", "FESpace", "FaceLabeling", "Float64", "GridapHipError", "LinearFESolver",
     "Measure", "PCGSolver", "Quadrature", "ReferenceFE", "SparseMatrixAssembler",
-    "SparseMatrixCSC", "TestFESpace", "TrialFESpace", "Triangulation", "VectorValue",
+    "SparseMatrixCSC", "TestFESpace", "TrialFESpace", "Triangulation",
+    "UnstructuredDiscreteModel", "VectorValue", "simplexify",
     "assemble_matrix", "assemble_matrix_and_vector", "assemble_vector", "dot",
     "get_fe_basis", "get_matrix", "get_triangulation", "get_trial_fe_basis", "get_vector",
     "grad", "gradient", "inner", "integral", "integrate", "lagrangian", "solve",

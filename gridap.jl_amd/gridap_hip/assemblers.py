@@ -81,6 +81,7 @@ class SparseMatrixAssembler:
     def _setup(self, degree):
         V = self.space
         model = V.model
+        self._simplex = bool(getattr(V, "simplex", False))
         if self.ctx is None:
             ctx = Context(self.device)
             D = model.D
@@ -92,15 +93,16 @@ class SparseMatrixAssembler:
                 ctx.mesh_upload(model.node_coordinates(), cn.ravel(), ptrs.astype(np.int32))
             cd = V.cell_dof_ids
             ptrs = (np.arange(cd.shape[0] + 1, dtype=np.int64) * cd.shape[1] + 1)
-            ctx.space_upload(_lib.GH_REFFE_QK, V.order, V.ncomp, cd.ravel(),
+            kind = _lib.GH_REFFE_PK if self._simplex else _lib.GH_REFFE_QK
+            ctx.space_upload(kind, V.order, V.ncomp, cd.ravel(),
                              ptrs.astype(np.int32), V.nfree, V.ndirichlet)
             self.nnz = ctx.symbolic()
             self.colptr, self.rowval = ctx.pattern_get()
             self.ctx = ctx
             self._D = D
         if degree != self._degree:
-            q = Quadrature(model.D, degree)
-            shp, grad = V.reffe.tabulate(model.D, q.coords)
+            q = Quadrature(model.D, degree, simplex=self._simplex)
+            shp, grad = V.reffe.tabulate(model.D, q.coords, simplex=self._simplex)
             self.ctx.reffe_tabulation(q.weights, shp, grad)
             self._degree = degree
 

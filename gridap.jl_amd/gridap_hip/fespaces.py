@@ -9,7 +9,7 @@ src/ReferenceFEs/LagrangianDofBases.jl:77-96).
 """
 import numpy as np
 
-from .geometry import CartesianDiscreteModel, Triangulation
+from .geometry import CartesianDiscreteModel, Triangulation, UnstructuredDiscreteModel
 from .reference_fes import LagrangianRefFE
 
 
@@ -18,14 +18,18 @@ class FESpace:
                  trian=None):
         if isinstance(model, Triangulation):
             model = model.model
-        if not isinstance(model, CartesianDiscreteModel):
-            raise TypeError("FESpace needs a CartesianDiscreteModel")
+        if not isinstance(model, (CartesianDiscreteModel, UnstructuredDiscreteModel)):
+            raise TypeError("FESpace needs a CartesianDiscreteModel or a simplexified one")
         if not isinstance(reffe, LagrangianRefFE):
             raise TypeError("FESpace needs a ReferenceFE(lagrangian, T, order)")
         if conformity not in ("H1", ":H1"):
             raise NotImplementedError("only H1-conforming spaces are provided")
-        if reffe.order != 1:
-            raise NotImplementedError("conforming spaces of order > 1 are not provided yet")
+        self.simplex = isinstance(model, UnstructuredDiscreteModel)
+        if self.simplex:
+            if reffe.order not in (1, 2) or reffe.ncomp != 1:
+                raise NotImplementedError("simplicial spaces: scalar P1 and P2")
+        elif reffe.order != 1:
+            raise NotImplementedError("n-cube spaces of order > 1 are not provided yet")
         self.model = model
         self.reffe = reffe
         self.order = reffe.order
@@ -34,6 +38,9 @@ class FESpace:
             dirichlet_tags = [dirichlet_tags]
         self.dirichlet_tags = list(dirichlet_tags)
         labels = model.get_face_labeling()
+        if self.simplex:
+            self._number_simplex(labels)
+            return
         node_to_tag = labels.face_tag_index(self.dirichlet_tags, 0)
         ntags = len(self.dirichlet_tags)
         if dirichlet_masks is None:
@@ -69,6 +76,42 @@ class FESpace:
             cd = np.concatenate([n2d[cn, k] for k in range(nc)], axis=1)
         self.cell_dof_ids = np.ascontiguousarray(cd, dtype=np.int32)
 
+    def _number_simplex(self, labels):
+        """Conforming numbering on the simplicial model (src/FESpaces/
+        ConformingFESpaces.jl:367-575, _split_face_own_dofs_into_free_and_dirichlet_
+        generic!): faces dimension by dimension in face-id order -- vertices, then
+        (P2) edges -- each owning one dof; free ids 1.., Dirichlet ids -1.. in that
+        order.  Cell dofs: the 4 vertex dofs, then the 6 edge dofs in TET edge order."""
+        m = self.model
+        ents = set()
+        for tag in self.dirichlet_tags:
+            ents.update(labels.tag_to_entities(tag))
+        ents = np.array(sorted(ents), dtype=np.int32)
+        vdir = np.isin(m.vertex_entity(), ents)
+        face_dir = [vdir]
+        if self.order == 2:
+            face_dir.append(np.isin(m.edge_entity(), ents))
+        isd = np.concatenate(face_dir)
+        ids = np.empty(len(isd), dtype=np.int64)
+        ids[~isd] = np.arange(1, np.count_nonzero(~isd) + 1)
+        ids[isd] = -np.arange(1, np.count_nonzero(isd) + 1)
+        nv = m.num_vertices()
+        cells = m.cell_node_ids().astype(np.int64) - 1
+        cd = ids[cells]
+        if self.order == 2:
+            cd = np.concatenate([cd, ids[nv + m.cell_edges()]], axis=1)
+        self.cell_dof_ids = np.ascontiguousarray(cd, dtype=np.int32)
+        self.nfree = int(np.count_nonzero(~isd))
+        self.ndirichlet = int(np.count_nonzero(isd))
+        # coordinates of every dof node: vertices, then edge midpoints (affine cells)
+        xyz = m.node_coordinates()
+        if self.order == 2:
+            ev = m.edge_vertices()
+            xyz = np.concatenate([xyz, 0.5 * (xyz[ev[:, 0]] + xyz[ev[:, 1]])])
+        self.free_dof_coords = xyz[~isd]
+        self.dirichlet_dof_coords = xyz[isd]
+        self.node_and_comp_to_dof = None
+
     # Gridap accessors --------------------------------------------------------
     def get_cell_dof_ids(self):
         return self.cell_dof_ids
@@ -101,7 +144,10 @@ class TrialFESpace:
         self.space = V
         self.g = g
         vals = np.zeros(V.ndirichlet)
-        if g is not None and V.ndirichlet:
+        if g is not None and V.ndirichlet and getattr(V, "simplex", False):
+            vals = (np.asarray(g(V.dirichlet_dof_coords), dtype=np.float64).reshape(-1)
+                    if callable(g) else np.full(V.ndirichlet, float(g)))
+        elif g is not None and V.ndirichlet:
             xyz = V.model.node_coordinates()
             nodes = V.dirichlet_dof_to_node - 1
             if callable(g):

@@ -166,3 +166,151 @@ class Triangulation:
 
 def get_triangulation(model):
     return Triangulation(model)
+
+
+# ---- simplicial models ------------------------------------------------------------
+
+# Local vertex lists of the six tetrahedra of a HEX (x-fastest local node order),
+# the oriented split that simplexify uses by default
+# (src/ReferenceFEs/ExtrusionPolytopes.jl:290-299).
+HEX_TO_TETS = ((0, 1, 2, 6), (0, 1, 4, 6), (1, 2, 3, 6), (1, 3, 6, 7), (1, 4, 5, 6), (1, 5, 6, 7))
+# Local edges of TET as vertex pairs, in Gridap's face order
+# (src/ReferenceFEs/ExtrusionPolytopes.jl:_polytopenfaces on (TET_AXIS,)*3).
+TET_EDGES = ((0, 1), (0, 2), (1, 2), (0, 3), (1, 3), (2, 3))
+
+
+def _first_touch(keys):
+    """Face ids (0-based) of a flat array of face keys visited in (cell, local face)
+    order: the k-th distinct key met gets id k (the first-touch numbering of
+    src/Geometry/GridTopologies.jl:1184-1251)."""
+    uniq, first, inverse = np.unique(keys, return_index=True, return_inverse=True)
+    rank = np.empty(len(uniq), dtype=np.int64)
+    rank[np.argsort(first, kind="stable")] = np.arange(len(uniq))
+    return rank[inverse.reshape(-1)], len(uniq)
+
+
+class UnstructuredDiscreteModel:
+    """simplexify(CartesianDiscreteModel): tetrahedra (or triangles) on the
+    Cartesian nodes with their grid topology (vertices = nodes, edges numbered by
+    first touch) and the labels transferred from the Cartesian entities
+    (src/Geometry/UnstructuredDiscreteModels.jl:85-311).  Node coordinates may be
+    replaced (jittered) without changing the topology."""
+
+    def __init__(self, cart, xyz=None):
+        if cart.D != 3:
+            raise NotImplementedError("simplexify is provided for 3-D Cartesian models")
+        self.D = 3
+        self.cart = cart
+        self.partition = cart.partition
+        self._xyz = cart.node_coordinates() if xyz is None else np.ascontiguousarray(xyz)
+        hexn = cart.cell_node_ids().astype(np.int64) - 1           # cells x 8
+        tets = hexn[:, np.array(HEX_TO_TETS)]                        # cells x 6 x 4
+        self._cells = np.ascontiguousarray(tets.reshape(-1, 4))      # 0-based, hex-major
+        self._edges = None
+        self._edge_entity = None
+
+    # sizes / geometry -----------------------------------------------------------
+    def num_cells(self):
+        return self._cells.shape[0]
+
+    def num_nodes(self):
+        return self._xyz.shape[0]
+
+    num_vertices = num_nodes
+
+    def is_affine_cartesian(self):
+        return False
+
+    def is_simplex(self):
+        return True
+
+    def node_coordinates(self):
+        return self._xyz
+
+    def cell_node_ids(self):
+        return (self._cells + 1).astype(np.int32)
+
+    def with_node_coordinates(self, xyz):
+        m = UnstructuredDiscreteModel(self.cart, xyz)
+        m._edges, m._edge_entity = self._edges, self._edge_entity
+        return m
+
+    def jittered(self, seed, amplitude=0.2):
+        """Interior vertices moved by independent U(-a h_d, a h_d) per axis from a seeded
+        generator (boundary vertices stay, so the domain is unchanged)."""
+        rng = np.random.default_rng(seed)
+        xyz = self.cart.node_coordinates().copy()
+        ijk = self._node_ijk()
+        interior = np.all([(ijk[d] > 0) & (ijk[d] < self.partition[d]) for d in range(3)], axis=0)
+        h = self.cart.sizes
+        shift = rng.uniform(-amplitude, amplitude, size=xyz.shape) * h[None, :]
+        xyz[interior] += shift[interior]
+        return self.with_node_coordinates(xyz)
+
+    # topology --------------------------------------------------------------------
+    def _node_ijk(self):
+        n = np.arange(self.num_nodes(), dtype=np.int64)
+        n0, n1 = self.partition[0] + 1, self.partition[1] + 1
+        return np.stack([n % n0, (n // n0) % n1, n // (n0 * n1)])
+
+    def cell_edges(self):
+        """cells x 6 edge ids (0-based), first-touch order."""
+        if self._edges is None:
+            c = self._cells
+            nn = self.num_nodes()
+            ends = [(np.minimum(c[:, a], c[:, b]), np.maximum(c[:, a], c[:, b])) for a, b in TET_EDGES]
+            keys = np.stack([lo * nn + hi for lo, hi in ends], axis=1).reshape(-1)
+            ids, ne = _first_touch(keys)
+            self._edges = ids.reshape(-1, 6)
+            self._nedges = ne
+            # the two end vertices of every edge
+            ev = np.empty((ne, 2), dtype=np.int64)
+            flat = keys
+            ev[ids, 0] = flat // nn
+            ev[ids, 1] = flat % nn
+            self._edge_vertices = ev
+        return self._edges
+
+    def num_edges(self):
+        self.cell_edges()
+        return self._nedges
+
+    def edge_vertices(self):
+        self.cell_edges()
+        return self._edge_vertices
+
+    # labels ------------------------------------------------------------------------
+    def vertex_entity(self):
+        return self.cart.get_face_labeling().vertex_entity
+
+    def edge_entity(self):
+        """Entity of each edge: the smallest Cartesian box face holding both ends (a
+        hex edge keeps its label, a face diagonal gets the face's, the body diagonal
+        the interior)."""
+        if self._edge_entity is None:
+            ev = self.edge_vertices()
+            ijk = self._node_ijk()
+            a, b = ijk[:, ev[:, 0]], ijk[:, ev[:, 1]]
+            free = np.zeros(len(ev), dtype=np.int64)
+            anchor = np.zeros(len(ev), dtype=np.int64)
+            for d in range(3):
+                lo = (a[d] == 0) & (b[d] == 0)
+                hi = (a[d] == self.partition[d]) & (b[d] == self.partition[d])
+                free |= (~(lo | hi)).astype(np.int64) << d
+                anchor |= hi.astype(np.int64) << d
+            table = np.zeros((8, 8), dtype=np.int32)
+            for m in range(8):
+                for an in range(8):
+                    table[m, an] = ncube_face_id(3, m, an & ~m)
+            self._edge_entity = table[free, anchor & ~free]
+        return self._edge_entity
+
+    def get_face_labeling(self):
+        return self.cart.get_face_labeling()
+
+
+def simplexify(model):
+    """simplexify(model) (src/Geometry/DiscreteModels.jl:374-379) for Cartesian models."""
+    if isinstance(model, UnstructuredDiscreteModel):
+        return model
+    return UnstructuredDiscreteModel(model)

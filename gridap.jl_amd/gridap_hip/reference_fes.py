@@ -70,8 +70,10 @@ class LagrangianRefFE:
                         out.append(x)
         return np.array(out)
 
-    def tabulate(self, D, pts):
+    def tabulate(self, D, pts, simplex=False):
         """Shape values [npts, nnodes] and reference gradients [npts, nnodes, D]."""
+        if simplex:
+            return tabulate_simplex(D, self.order, pts)
         k = self.order
         nodes = self.nodes(D)
         pts = np.atleast_2d(np.asarray(pts, dtype=np.float64))
@@ -103,6 +105,33 @@ class LagrangianRefFE:
         return shp, grad
 
 
+# Local edges of TET in Gridap's face order (vertex pairs, 0-based).
+_TET_EDGES = ((0, 1), (0, 2), (1, 2), (0, 3), (1, 3), (2, 3))
+
+
+def tabulate_simplex(D, order, pts):
+    """Nodal Lagrangian basis of order 1 or 2 on the reference TET (vertices (0,0,0),
+    (1,0,0), (0,1,0), (0,0,1)); nodes: the vertices, then one per edge in TET edge
+    order (src/ReferenceFEs/LagrangianRefFEs.jl).  With barycentrics λ = (1-x-y-z,
+    x, y, z): P1 φ_a = λ_a; P2 φ_a = λ_a (2 λ_a - 1) at vertices, 4 λ_a λ_b on edges."""
+    if D != 3 or order not in (1, 2):
+        raise NotImplementedError("simplicial elements: P1 and P2 on TET")
+    x = np.atleast_2d(np.asarray(pts, dtype=np.float64))
+    lam = np.stack([1.0 - x.sum(1), x[:, 0], x[:, 1], x[:, 2]], axis=1)
+    dlam = np.array([[-1.0, -1.0, -1.0], [1.0, 0.0, 0.0], [0.0, 1.0, 0.0], [0.0, 0.0, 1.0]])
+    npts = x.shape[0]
+    if order == 1:
+        return lam.copy(), np.broadcast_to(dlam, (npts, 4, 3)).copy()
+    shp = np.zeros((npts, 10))
+    grad = np.zeros((npts, 10, 3))
+    shp[:, :4] = lam * (2.0 * lam - 1.0)
+    grad[:, :4, :] = (4.0 * lam - 1.0)[:, :, None] * dlam[None, :, :]
+    for k, (a, b) in enumerate(_TET_EDGES):
+        shp[:, 4 + k] = 4.0 * lam[:, a] * lam[:, b]
+        grad[:, 4 + k, :] = 4.0 * (lam[:, a][:, None] * dlam[b] + lam[:, b][:, None] * dlam[a])
+    return shp, grad
+
+
 def ReferenceFE(name, value_type=Float64, order=1):
     if name is not lagrangian:
         raise NotImplementedError("only lagrangian reference elements are provided")
@@ -116,11 +145,58 @@ def gauss_legendre01(n):
     return (x + 1.0) / 2.0, w / 2.0
 
 
-class Quadrature:
-    """Tensor-product Gauss-Legendre rule of a degree on the n-cube: div(degree,2)+1
-    points per axis, x fastest (src/ReferenceFEs/Quadratures.jl:191,203-222)."""
+# Symmetric TET rules (barycentric orbits; weights sum to 1/6).  Degree 2: the
+# 4-point S31 rule with a = (5 - sqrt 5)/20.  Degrees 4-5: the 14-point rule of two
+# S31 orbits and one S22 orbit exact to degree 5 -- the structure of the
+# Witherden-Vincent rule Gridap selects for TET at degrees 4 and 5
+# (src/ReferenceFEs/Quadratures.jl:157-175); the values below are the interior,
+# positive-weight root of its moment equations.
+_TET5_S31 = ((0.0122488405193937, 0.09273525031089135),
+             (0.01878132095300283, 0.3108859192633008))
+_TET5_S22 = (0.00709100346284676, 0.04550370412564866)
 
-    def __init__(self, D, degree):
+
+def _tet_rule(degree):
+    if degree <= 1:
+        return np.array([[0.25, 0.25, 0.25]]), np.array([1.0 / 6.0])
+    orbits31, orbits22 = [], []
+    if degree == 2:
+        orbits31 = [(1.0 / 24.0, (5.0 - 5.0 ** 0.5) / 20.0)]
+    elif degree in (4, 5):
+        orbits31, orbits22 = list(_TET5_S31), [_TET5_S22]
+    else:
+        raise NotImplementedError("TET quadrature is provided for degrees 1, 2, 4 and 5")
+    pts, wts = [], []
+    for w, a in orbits31:
+        for k in range(4):
+            lam = [a] * 4
+            lam[k] = 1.0 - 3.0 * a
+            pts.append(lam[1:])
+            wts.append(w)
+    for w, b in orbits22:
+        for i in range(4):
+            for j in range(i + 1, 4):
+                lam = [0.5 - b] * 4
+                lam[i] = lam[j] = b
+                pts.append(lam[1:])
+                wts.append(w)
+    return np.array(pts), np.array(wts)
+
+
+class Quadrature:
+    """Quadrature(polytope, degree).  n-cube: tensor-product Gauss-Legendre rule,
+    div(degree,2)+1 points per axis, x fastest (src/ReferenceFEs/Quadratures.jl:191,
+    203-222).  simplex=True: the symmetric TET rules above."""
+
+    def __init__(self, D, degree, simplex=False):
+        self.degree = degree
+        self.D = D
+        self.simplex = simplex
+        if simplex:
+            if D != 3:
+                raise NotImplementedError("simplicial quadrature is provided on TET")
+            self.coords, self.weights = _tet_rule(degree)
+            return
         n = degree // 2 + 1
         x1, w1 = gauss_legendre01(n)
         idx = np.indices((n,) * D).reshape(D, -1, order="F")
@@ -129,5 +205,3 @@ class Quadrature:
         for d in range(D):
             w = w * w1[idx[d]]
         self.weights = w
-        self.degree = degree
-        self.D = D

@@ -231,17 +231,15 @@ def node_ijk(partition, n):
     return (n % n0, (n // n0) % n1, n // (n0 * n1))
 
 
-def p2_tet_numbering(partition, dirichlet_entities):
-    """P2 conforming cell dofs on the simplexified Cartesian grid."""
+def tet_numbering(partition, dirichlet_entities, order=2):
+    """P1/P2 conforming cell dofs on the simplexified Cartesian grid."""
     cells = simplexify_cartesian(partition)
     D = 3
     nnodes = int(np.prod([p + 1 for p in partition]))
     cell_vertices = cells + 1
-    vfaces = cell_vertices.copy()                           # d = 0: vertices
     edges, nedges = first_touch_faces(cells, TET.faces[1])
     faces2, nf2 = first_touch_faces(cells, TET.faces[2])
     cellsd = np.arange(1, len(cells) + 1).reshape(-1, 1)
-    # tags
     vtag = [1 if simplex_face_entity(D, partition, [node_ijk(partition, n)]) in dirichlet_entities
             else 0 for n in range(nnodes)]
     etag = [0] * nedges
@@ -250,12 +248,15 @@ def p2_tet_numbering(partition, dirichlet_entities):
             e = edges[c, le] - 1
             ent = simplex_face_entity(D, partition, [node_ijk(partition, cells[c][v - 1]) for v in lv])
             etag[e] = 1 if ent in dirichlet_entities else 0
-    ftag = [0] * nf2
-    ctag = [0] * len(cells)
-    own, nloc = lagrangian_own_dofs(TET, 2)
-    cd, nfree, ndiri, _ = conforming_cell_dofs([vfaces, edges, faces2, cellsd],
-                                               [vtag, etag, ftag, ctag], own, nloc)
+    own, nloc = lagrangian_own_dofs(TET, order)
+    cd, nfree, ndiri, _ = conforming_cell_dofs([cell_vertices, edges, faces2, cellsd],
+                                               [vtag, etag, [0] * nf2, [0] * len(cells)],
+                                               own, nloc)
     return cells, cd, nfree, ndiri, edges
+
+
+def p2_tet_numbering(partition, dirichlet_entities):
+    return tet_numbering(partition, dirichlet_entities, 2)
 
 
 # ---- P2 on the tetrahedron ---------------------------------------------------------------
@@ -312,33 +313,57 @@ def p2_tet_stiffness_exact():
     return np.einsum("iad,ij,jbd->ab", coef, I, coef)
 
 
-def p2_tet_poisson(partition, xyz, kappa=None, fsrc=None, qx=None, qw=None,
-                   dirichlet_entities=None):
-    """Cell matrices and vectors of ∫κ∇v·∇u, ∫f v with P2 on the affine tets
-    (J from the 4 vertices, |det J| measure), FP64, and the global assembly via the
-    given assembler callable later.  Returns a dict like the Q1 driver."""
+def p1_tet_tabulate(x):
+    x = np.atleast_2d(x)
+    lam = np.stack([1 - x.sum(1), x[:, 0], x[:, 1], x[:, 2]], 1)
+    dlam = np.array([[-1, -1, -1], [1, 0, 0], [0, 1, 0], [0, 0, 1]], dtype=float)
+    return lam, np.broadcast_to(dlam, (len(x), 4, 3)).copy()
+
+
+def tet_poisson(partition, xyz, order=2, kappa=None, fsrc=None, qx=None, qw=None,
+                dirichlet_entities=None, g=None):
+    """Cell matrices and vectors of ∫κ∇v·∇u, ∫f v for P1/P2 on the affine tets of the
+    simplexified Cartesian grid with node coordinates xyz: DF from the 4 vertices,
+    physical gradients DF⁻ᵀ∇̂φ, measure |det DF| (the affine cell map of
+    src/Geometry/UnstructuredGrids.jl with the integration of
+    src/CellData/CellQuadratures.jl:138-160), FP64.  g: callable for the Dirichlet
+    values at the dof nodes (vertices, edge midpoints); the cell vectors are then
+    lifted (attach_dirichlet)."""
     D = 3
     if dirichlet_entities is None:
         dirichlet_entities = boundary_entities(D)
-    cells, cd, nfree, ndiri, edges = p2_tet_numbering(partition, dirichlet_entities)
+    cells, cd, nfree, ndiri, edges = tet_numbering(partition, dirichlet_entities, order)
     C = len(cells)
     kappa = np.ones(C) if kappa is None else np.asarray(kappa, dtype=float)
     fsrc = np.ones(C) if fsrc is None else np.asarray(fsrc, dtype=float)
-    shp, grad = p2_tet_tabulate(qx)
-    mats = np.zeros((C, 10, 10))
-    vecs = np.zeros((C, 10))
-    for c in range(C):
-        X = xyz[cells[c]]                              # 4 x 3
-        J = (X[1:] - X[0]).T                           # J_ij = ∂x_i/∂ξ_j
-        det = np.linalg.det(J)
-        Jinv = np.linalg.inv(J)
-        for q in range(len(qw)):
-            G = grad[q] @ Jinv                         # rows: J⁻ᵀ ∇̂φ_a
-            dV = qw[q] * abs(det)
-            mats[c] += kappa[c] * dV * (G @ G.T)
-            vecs[c] += fsrc[c] * dV * shp[q]
+    shp, grad = (p2_tet_tabulate if order == 2 else p1_tet_tabulate)(qx)
+    X = xyz[cells]                                       # C x 4 x 3
+    J = np.transpose(X[:, 1:, :] - X[:, :1, :], (0, 2, 1))   # J_ij = ∂x_i/∂ξ_j
+    det = np.linalg.det(J)
+    Jinv = np.linalg.inv(J)
+    G = np.einsum("qad,cdi->cqai", grad, Jinv)           # physical gradients
+    dV = np.abs(det)[:, None] * qw[None, :]              # C x nq
+    mats = np.einsum("c,cq,cqai,cqbi->cab", kappa, dV, G, G)
+    vecs = np.einsum("c,cq,qa->ca", fsrc, dV, shp)
+    dvals = np.zeros(ndiri)
+    if g is not None:
+        nloc = cd.shape[1]
+        nodes = np.concatenate([X, 0.5 * (X[:, [a - 1 for a, b in TET.faces[1]]] +
+                                          X[:, [b - 1 for a, b in TET.faces[1]]])], axis=1)
+        for c in range(C):
+            for a in range(nloc):
+                if cd[c, a] < 0:
+                    dvals[-cd[c, a] - 1] = g(nodes[c, a])
+        for c in range(C):
+            m = cd[c] < 0
+            if m.any():
+                vecs[c] -= mats[c][:, m] @ dvals[-cd[c, m] - 1]
     return dict(cells=cells, cell_dofs=cd, nfree=nfree, ndiri=ndiri, cellmats=mats,
-                cellvecs=vecs, edges=edges)
+                cellvecs=vecs, edges=edges, dirichlet_vals=dvals)
+
+
+def p2_tet_poisson(partition, xyz, **kw):
+    return tet_poisson(partition, xyz, 2, **kw)
 
 
 # ---- quadrature -----------------------------------------------------------------------

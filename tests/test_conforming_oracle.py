@@ -187,3 +187,50 @@ def test_p2_numbering_invariants():
         u, v = [O.node_ijk(part, n) for n in key]
         on_plane = any((u[d] == v[d] == 0) or (u[d] == v[d] == part[d]) for d in range(3))
         assert (dof < 0) == on_plane
+
+
+def jittered_xyz(part, seed, amp=0.2):
+    n0, n1, n2 = part
+    xyz = np.array([[i / n0, j / n1, k / n2] for k in range(n2 + 1) for j in range(n1 + 1)
+                    for i in range(n0 + 1)])
+    rng = np.random.default_rng(seed)
+    ijk = np.array([O.node_ijk(part, n) for n in range(len(xyz))])
+    inner = np.all((ijk > 0) & (ijk < np.array(part)), axis=1)
+    h = 1.0 / np.array(part)
+    xyz[inner] += rng.uniform(-amp, amp, (inner.sum(), 3)) * h
+    return xyz
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_tet_assembly_reproduces_polynomials(oracle, order):
+    # P_k holds every polynomial of degree k, so with f = -Δu and Dirichlet data from
+    # u the discrete system is solved exactly by the interpolant: A u_I = b.
+    part = (3, 3, 2)
+    xyz = jittered_xyz(part, 1)
+    if order == 2:
+        u = lambda p: (1 + p[0] - 2 * p[1] + 0.5 * p[2] + p[0] ** 2 + p[1] ** 2 +
+                       2 * p[1] * p[2] - 0.5 * p[2] ** 2)
+        lap = 2.0 + 2.0 - 1.0
+    else:
+        u = lambda p: 1 + p[0] - 2 * p[1] + 0.5 * p[2]
+        lap = 0.0
+    P, W = O.tet_degree5_rule()
+    r = O.tet_poisson(part, xyz, order, fsrc=np.full(6 * 18, -lap), qx=P, qw=W, g=u)
+    cp, rv, nz = oracle.assemble_csc(r["cell_dofs"], r["cellmats"], r["nfree"], r["nfree"])
+    b = oracle.assemble_vector(r["cell_dofs"], r["cellvecs"], r["nfree"])
+    # interpolant at the free dofs
+    cells, cd = r["cells"], r["cell_dofs"]
+    X = xyz[cells]
+    nodes = np.concatenate([X, 0.5 * (X[:, [a - 1 for a, _ in O.TET.faces[1]]] +
+                                      X[:, [b - 1 for _, b in O.TET.faces[1]]])], axis=1)
+    uI = np.zeros(r["nfree"])
+    for c in range(len(cells)):
+        for a in range(cd.shape[1]):
+            if cd[c, a] > 0:
+                uI[cd[c, a] - 1] = u(nodes[c, a])
+    A = oracle.to_scipy(cp, rv, nz, r["nfree"])
+    assert np.max(np.abs(A @ uI - b)) < 1e-12 * max(1.0, np.max(np.abs(b)))
+    # without the Dirichlet lifting the identity fails (the check has teeth)
+    r0 = O.tet_poisson(part, xyz, order, fsrc=np.full(6 * 18, -lap), qx=P, qw=W)
+    b0 = oracle.assemble_vector(r0["cell_dofs"], r0["cellvecs"], r0["nfree"])
+    assert np.max(np.abs(A @ uI - b0)) > 1e-3

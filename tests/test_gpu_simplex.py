@@ -1,0 +1,154 @@
+"""Parity of the simplex (P1/P2 tetrahedra) assembly path with the FP64 oracle
+(needs an MI355X).
+
+Bar: CSC pattern bit-exact with the oracle's create_from_nz output on the same
+cell_dof_ids; FP64 mode: nzval and b within 1e-12 relative (max-abs error / max-abs
+value); FP32-contraction mode: relative Frobenius error <= 1e-6.  At 48^3 the check
+is the size-independent polynomial identity A u_I = b (P2 holds quadratics).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import gridap_hip as G
+from gridap_hip import _lib
+import conforming_oracle as CO
+
+pytestmark = pytest.mark.gpu
+
+
+def announce(msg):
+    print("[gpu-case] " + msg, flush=True)
+    sys.stdout.flush()
+
+
+def relmax(a, b):
+    assert a.shape == b.shape
+    return np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300) if a.size else 0.0
+
+
+def relfrob(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def build(part, order, seed=5, kappa=None, fsrc=None, tags=("boundary",), g=None,
+          precision="fp64", jitter=0.2):
+    cart = G.CartesianDiscreteModel((0, 1, 0, 1, 0, 1), part)
+    model = G.simplexify(cart)
+    if jitter:
+        model = model.jittered(seed, jitter)
+    reffe = G.ReferenceFE(G.lagrangian, G.Float64, order)
+    V = G.TestFESpace(model, reffe, dirichlet_tags=list(tags))
+    U = G.TrialFESpace(V, g)
+    Ω = G.Triangulation(model)
+    dΩ = G.Measure(Ω, 2 * order)
+    k = 1.0 if kappa is None else G.CellField(kappa, Ω)
+    f = 1.0 if fsrc is None else G.CellField(fsrc, Ω)
+    a = lambda u, v: G.integrate(k * G.inner(G.gradient(v), G.gradient(u)), dΩ)
+    l = lambda v: G.integrate(f * v, dΩ)
+    assem = G.SparseMatrixAssembler(U, V, precision=precision)
+    op = G.AffineFEOperator(a, l, U, V, assem)
+    return op, V, model
+
+
+def oracle_system(oracle, part, order, xyz, kappa=None, fsrc=None, ents=None, g=None):
+    q = G.Quadrature(3, 2 * order, simplex=True)
+    r = CO.tet_poisson(part, xyz, order, kappa=kappa, fsrc=fsrc, qx=q.coords, qw=q.weights,
+                       dirichlet_entities=ents, g=g)
+    cp, rv, nz = oracle.assemble_csc(r["cell_dofs"], r["cellmats"], r["nfree"], r["nfree"])
+    b = oracle.assemble_vector(r["cell_dofs"], r["cellvecs"], r["nfree"])
+    return r, cp, rv, nz, b
+
+
+def check(op, V, r, cp, rv, nz, b, precision="fp64"):
+    np.testing.assert_array_equal(V.cell_dof_ids, r["cell_dofs"])
+    A = op.get_matrix()
+    np.testing.assert_array_equal(A.colptr, cp)
+    np.testing.assert_array_equal(A.rowval, rv)
+    if precision == "fp64":
+        assert relmax(A.nzval, nz) <= 1e-12
+        assert relmax(op.get_vector(), b) <= 1e-12
+    else:
+        assert relfrob(A.nzval, nz) <= 1e-6
+        assert relfrob(op.get_vector(), b) <= 1e-6
+
+
+@pytest.mark.parametrize("order,precision", [(2, "fp64"), (2, "fp32"), (1, "fp64")])
+def test_tet_matches_oracle(oracle, order, precision):
+    part = (4, 3, 3)
+    announce("simplex P%d %s %s jittered" % (order, precision, part))
+    rng = np.random.default_rng(order)
+    C = 6 * int(np.prod(part))
+    kappa = np.exp(rng.normal(0, 1, C))
+    fsrc = rng.uniform(-1, 1, C)
+    op, V, model = build(part, order, kappa=kappa, fsrc=fsrc, precision=precision)
+    r, cp, rv, nz, b = oracle_system(oracle, part, order, model.node_coordinates(), kappa, fsrc)
+    check(op, V, r, cp, rv, nz, b, precision)
+
+
+def test_tet_partial_tags_and_lifting(oracle):
+    part = (3, 4, 2)
+    announce("simplex P2 partial tags + lifting %s" % (part,))
+    g = lambda x: 1.0 + x[..., 0] - 2.0 * x[..., 1] * x[..., 2]
+    op, V, model = build(part, 2, tags=("tag_21", "tag_01", "tag_25"), g=g)
+    ents = {21, 1, 25}
+    r, cp, rv, nz, b = oracle_system(oracle, part, 2, model.node_coordinates(), ents=ents,
+                                     g=lambda p: 1.0 + p[0] - 2.0 * p[1] * p[2])
+    check(op, V, r, cp, rv, nz, b)
+
+
+def quadratic_case(part, seed):
+    u = lambda x: (1 + x[..., 0] - 2 * x[..., 1] + 0.5 * x[..., 2] + x[..., 0] ** 2 +
+                   x[..., 1] ** 2 + 2 * x[..., 1] * x[..., 2] - 0.5 * x[..., 2] ** 2)
+    C = 6 * int(np.prod(part))
+    op, V, model = build(part, 2, seed=seed, fsrc=np.full(C, -3.0), g=u)
+    return op, V, u(V.free_dof_coords)
+
+
+def test_tet_reproduces_quadratics_and_solves():
+    announce("simplex P2 quadratic reproduction + PCG (6x5x4)")
+    op, V, uI = quadratic_case((6, 5, 4), 2)
+    A, b = op.get_matrix().to_scipy(), op.get_vector()
+    assert np.max(np.abs(A @ uI - b)) <= 1e-12 * np.max(np.abs(b))
+    x = G.solve(op).get_free_dof_values()
+    assert np.max(np.abs(x - uI)) <= 1e-9
+
+
+def test_tet_16cube_matches_oracle(oracle):
+    part = (16, 16, 16)
+    announce("simplex P2 16^3 vs oracle")
+    rng = np.random.default_rng(16)
+    C = 6 * int(np.prod(part))
+    kappa = np.exp(rng.normal(0, 1, C))
+    op, V, model = build(part, 2, seed=16, kappa=kappa)
+    r, cp, rv, nz, b = oracle_system(oracle, part, 2, model.node_coordinates(), kappa)
+    check(op, V, r, cp, rv, nz, b)
+
+
+def test_tet_assembly_is_deterministic():
+    announce("simplex P2 repeat bit-identical (8^3)")
+    part = (8, 8, 8)
+    rng = np.random.default_rng(8)
+    kappa = np.exp(rng.normal(0, 1, 6 * 512))
+    op, V, _ = build(part, 2, kappa=kappa)
+    assem = op.assem
+    nz1, b1 = assem.ctx.values_get()
+    assem.ctx.assemble(_lib.GH_FORM_DIFFUSION, assem.precision, assem.path)
+    nz2, b2 = assem.ctx.values_get()
+    np.testing.assert_array_equal(nz1, nz2)
+    np.testing.assert_array_equal(b1, b2)
+
+
+def test_tet_48cube_polynomial_identity():
+    announce("simplex P2 48^3 A u_I = b")
+    op, V, uI = quadratic_case((48, 48, 48), 48)
+    A = op.get_matrix()
+    Ad = op.assem.ctx
+    r = Ad.spmv(uI) - op.get_vector()
+    assert np.max(np.abs(r)) <= 1e-11 * np.max(np.abs(op.get_vector()))
+    # pattern: symmetric, sorted, diagonal present
+    S = A.to_scipy()
+    assert (S != S.T).nnz == 0 or abs(S - S.T).max() <= 1e-12 * abs(S).max()
+    d = S.diagonal()
+    assert np.all(d > 0)
