@@ -26,7 +26,9 @@ namespace gh {
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int kMlpCellsPerBlock = 256;
+constexpr int kMlpCellsPerBlock = 256;  // backward / weight-gradient partials
+constexpr int kMlpFwdCells = 32;        // forward: small blocks, many waves in flight
+constexpr int kEllSlots = kFsaiEllSlots;
 
 inline unsigned grid_for(int64_t n, int block = kBlock) {
   return (unsigned)std::max<int64_t>(1, (n + block - 1) / block);
@@ -138,7 +140,7 @@ __global__ void features_kernel(Lat3 L, int64_t ncells, const double *__restrict
 
 // ---- MLP: lane = output unit, one cell row (uniform) at a time ---------------
 
-template <int NIN, int NINP, int NOUT, int NOUTP, bool GELU>
+template <int NIN, int NINP, int NOUT, int NOUTP, bool GELU, bool SOA = false>
 __global__ __launch_bounds__(64) void mlp_fwd(const float *__restrict__ in,
                                               const uint16_t *__restrict__ wq, int woff,
                                               int boff, float *__restrict__ zout,
@@ -155,8 +157,157 @@ __global__ __launch_bounds__(64) void mlp_fwd(const float *__restrict__ in,
 #pragma unroll
     for (int k = 0; k < NIN; ++k) acc = fmaf(w[k], row[k], acc);
     if (j < NOUT) {
-      zout[(size_t)c * NOUTP + j] = acc;
+      zout[SOA ? (size_t)j * C + c : (size_t)c * NOUTP + j] = acc;
       if (GELU) hout[(size_t)c * NOUTP + j] = gelu(acc);
+    }
+  }
+}
+
+// Inference forward of the whole network on the matrix cores
+// (v_mfma_f32_16x16x32_bf16): a wave owns 16-cell tiles; layer inputs are rounded
+// to BF16 (A operand: lane l holds cell l%16, inputs 8(l/16)..+7), weights are the
+// BF16 values (B operand: output unit l%16, inputs 8(l/16)..+7, kept in registers),
+// accumulation is FP32 (D: output unit l%16, cells 4(l/16)..+3).  Bias + GELU are
+// applied in FP32 and the activations go through LDS to the next layer's A layout.
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+constexpr int kFwdWaves = 4;
+
+__device__ __forceinline__ __bf16 to_bf(float f) {
+  const uint16_t h = f32_to_bf16(f);
+  __bf16 b;
+  memcpy(&b, &h, 2);
+  return b;
+}
+__device__ __forceinline__ __bf16 bf_bits(uint16_t h) {
+  __bf16 b;
+  memcpy(&b, &h, 2);
+  return b;
+}
+
+__global__ __launch_bounds__(kFwdWaves * 64) void mlp_fwd_mfma(const float *__restrict__ feat,
+                                                               const uint16_t *__restrict__ wq,
+                                                               int C, float *__restrict__ yo) {
+  __shared__ float act[kFwdWaves][16][kMlpHidden + 1];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int r16 = lane & 15, kg = lane >> 4;
+  const __bf16 zero = to_bf(0.0f);
+  v8bf w1[4], w2[4][2], w3[4][2], w4[3][2];
+  float b1[4], b2[4], b3[4], b4[3];
+  // 8 consecutive BF16 of a 64-wide weight row: one 16-byte load (row offsets are
+  // multiples of 8 elements)
+  auto row8 = [&](int off) {
+    const uint4 u = *reinterpret_cast<const uint4 *>(wq + off);
+    v8bf v;
+    const uint32_t d[4] = {u.x, u.y, u.z, u.w};
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      v[2 * i] = bf_bits((uint16_t)(d[i] & 0xffffu));
+      v[2 * i + 1] = bf_bits((uint16_t)(d[i] >> 16));
+    }
+    return v;
+  };
+  static_assert(kMlpW2 % 8 == 0 && kMlpW3 % 8 == 0 && kMlpW4 % 8 == 0, "aligned rows");
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    const int n = t * 16 + r16;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const int k = kg * 8 + i;
+      w1[t][i] = k < kMlpIn ? bf_bits(wq[kMlpW1 + n * kMlpIn + k]) : zero;
+    }
+#pragma unroll
+    for (int sk = 0; sk < 2; ++sk) {
+      const int kk = sk * 32 + kg * 8;
+      w2[t][sk] = row8(kMlpW2 + n * kMlpHidden + kk);
+      w3[t][sk] = row8(kMlpW3 + n * kMlpHidden + kk);
+      if (t < 3) {
+        if (n < kMlpOut) {
+          w4[t][sk] = row8(kMlpW4 + n * kMlpHidden + kk);
+        } else {
+#pragma unroll
+          for (int i = 0; i < 8; ++i) w4[t][sk][i] = zero;
+        }
+      }
+    }
+    b1[t] = bf16_to_f32(wq[kMlpB1 + n]);
+    b2[t] = bf16_to_f32(wq[kMlpB2 + n]);
+    b3[t] = bf16_to_f32(wq[kMlpB3 + n]);
+    if (t < 3) b4[t] = n < kMlpOut ? bf16_to_f32(wq[kMlpB4 + n]) : 0.0f;
+  }
+  const int ntiles = (C + 15) / 16;
+  for (int t0 = blockIdx.x * kFwdWaves; t0 < ntiles; t0 += gridDim.x * kFwdWaves) {
+    const int tile = t0 + w;
+    const int cell = tile * 16 + r16;
+    const bool valid = tile < ntiles && cell < C;
+    v8bf a;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const int k = kg * 8 + i;
+      a[i] = (valid && k < kMlpInPad) ? to_bf(feat[(size_t)cell * kMlpInPad + k]) : zero;
+    }
+    v4f acc[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      acc[t] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
+      acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, w1[t], acc[t], 0, 0, 0);
+    }
+    // hidden layers 2 and 3
+#pragma unroll
+    for (int layer = 0; layer < 2; ++layer) {
+      const float *bias = layer == 0 ? b1 : b2;
+      __syncthreads();
+#pragma unroll
+      for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) act[w][kg * 4 + j][t * 16 + r16] = gelu(acc[t][j] + bias[t]);
+      __syncthreads();
+#pragma unroll
+      for (int t = 0; t < 4; ++t) acc[t] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+      for (int sk = 0; sk < 2; ++sk) {
+        v8bf x;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) x[i] = to_bf(act[w][r16][sk * 32 + kg * 8 + i]);
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+          acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(x, layer == 0 ? w2[t][sk] : w3[t][sk],
+                                                          acc[t], 0, 0, 0);
+      }
+    }
+    // output layer
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) act[w][kg * 4 + j][t * 16 + r16] = gelu(acc[t][j] + b3[t]);
+    __syncthreads();
+    v4f out[3];
+#pragma unroll
+    for (int t = 0; t < 3; ++t) out[t] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int sk = 0; sk < 2; ++sk) {
+      v8bf x;
+#pragma unroll
+      for (int i = 0; i < 8; ++i) x[i] = to_bf(act[w][r16][sk * 32 + kg * 8 + i]);
+#pragma unroll
+      for (int t = 0; t < 3; ++t)
+        out[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(x, w4[t][sk], out[t], 0, 0, 0);
+    }
+    // output-major store yo[n][cell] through LDS: 16 consecutive cells per row
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < 3; ++t)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) act[w][kg * 4 + j][t * 16 + r16] = out[t][j] + b4[t];
+    __syncthreads();
+    if (tile < ntiles) {
+      const int c = tile * 16 + r16;
+#pragma unroll
+      for (int nb = 0; nb < kMlpOut; nb += 4) {
+        const int n = nb + kg;
+        if (n < kMlpOut && c < C) yo[(size_t)n * C + c] = act[w][r16][n];
+      }
     }
   }
 }
@@ -242,11 +393,51 @@ __device__ __forceinline__ int pair_index(int a, int b) {  // a > b, local numbe
   return 8 + a * (a - 1) / 2 + b;
 }
 
+// Ĝ entry of node n and its stencil neighbour m = n + (DX, DY, DZ) (the diagonal for
+// 0, 0, 0): Σ over the cells holding both, in ascending cell order, so that both
+// endpoints compute the same bits.  Along an axis with offset 0 the cell index is
+// x-1 or x, else min(x, x+D); cells outside the lattice are skipped.
+template <int DX, int DY, int DZ>
+__device__ __forceinline__ float pair_sum(const Lat3 &L, const int *xn,
+                                          const float *__restrict__ yo, size_t C) {
+  constexpr int NX = DX == 0 ? 2 : 1, NY = DY == 0 ? 2 : 1, NZ = DZ == 0 ? 2 : 1;
+  float s = 0.0f;
+#pragma unroll
+  for (int kz = 0; kz < NZ; ++kz)
+#pragma unroll
+    for (int ky = 0; ky < NY; ++ky)
+#pragma unroll
+      for (int kx = 0; kx < NX; ++kx) {
+        const int cx = DX == 0 ? xn[0] - 1 + kx : xn[0] + min(DX, 0);
+        const int cy = DY == 0 ? xn[1] - 1 + ky : xn[1] + min(DY, 0);
+        const int cz = DZ == 0 ? xn[2] - 1 + kz : xn[2] + min(DZ, 0);
+        if (cx < 0 || cx >= L.nc[0] || cy < 0 || cy >= L.nc[1] || cz < 0 || cz >= L.nc[2])
+          continue;
+        const size_t cell = (size_t)cx + (size_t)L.nc[0] * (cy + (size_t)L.nc[1] * cz);
+        const int ln = (xn[0] - cx) | ((xn[1] - cy) << 1) | ((xn[2] - cz) << 2);
+        const int lm = (xn[0] + DX - cx) | ((xn[1] + DY - cy) << 1) | ((xn[2] + DZ - cz) << 2);
+        const int idx = ln == lm ? ln : (ln > lm ? pair_index(ln, lm) : pair_index(lm, ln));
+        s += yo[(size_t)idx * C + cell];
+      }
+  return s;
+}
+
+template <int O>
+__device__ __forceinline__ float pair_sum_o(const Lat3 &L, const int *xn,
+                                            const float *__restrict__ yo, size_t C) {
+  return pair_sum<O % 3 - 1, (O / 3) % 3 - 1, O / 9 - 1>(L, xn, yo, C);
+}
+
+// One thread per lattice node n (row i of G): the column-i prefix (gl, rows of G)
+// and suffix (gu, rows of Gᵀ) of Ĝ in CSC positions, and the solver's copy of
+// G = Ĝ D^-1/2 in 16-slot rows: gle[i] = G(i, j ≤ i), gue[i] = G(j ≥ i, i).
+template <bool CSC>
 __global__ void g_assemble_kernel(Lat3 L, int nnodes, const int32_t *__restrict__ node_dof,
                                   const uint32_t *__restrict__ fmask,
                                   const int32_t *__restrict__ colptr,
-                                  const float *__restrict__ yo, float *__restrict__ gl,
-                                  float *__restrict__ gu) {
+                                  const float *__restrict__ yo, const double *__restrict__ fs,
+                                  float *__restrict__ gl, float *__restrict__ gu,
+                                  float *__restrict__ gle, float *__restrict__ gue) {
   const int n = blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= nnodes) return;
   const int d0 = node_dof[n];
@@ -255,37 +446,58 @@ __global__ void g_assemble_kernel(Lat3 L, int nnodes, const int32_t *__restrict_
   node_xyz(L, n, xi);
   const uint32_t fm = fmask[n];
   const int p0 = colptr[d0 - 1];
+  const double sn = fs[d0 - 1];
+  float *el = gle + (size_t)(d0 - 1) * kEllSlots, *eu = gue + (size_t)(d0 - 1) * kEllSlots;
+  int kl = 0, ku = 0;
+  float sums[27];
+  const size_t C = (size_t)L.nc[0] * L.nc[1] * L.nc[2];
+#define GH_PAIR(O) sums[O] = ((fm >> (O)) & 1u) ? pair_sum_o<O>(L, xi, yo, C) : 0.0f;
+  GH_PAIR(0) GH_PAIR(1) GH_PAIR(2) GH_PAIR(3) GH_PAIR(4) GH_PAIR(5) GH_PAIR(6) GH_PAIR(7)
+  GH_PAIR(8) GH_PAIR(9) GH_PAIR(10) GH_PAIR(11) GH_PAIR(12) GH_PAIR(13) GH_PAIR(14)
+  GH_PAIR(15) GH_PAIR(16) GH_PAIR(17) GH_PAIR(18) GH_PAIR(19) GH_PAIR(20) GH_PAIR(21)
+  GH_PAIR(22) GH_PAIR(23) GH_PAIR(24) GH_PAIR(25) GH_PAIR(26)
+#undef GH_PAIR
 #pragma unroll
   for (int o = 0; o < 27; ++o) {
     if (!((fm >> o) & 1u)) continue;
     const int pos = p0 + __popc(fm & ((1u << o) - 1u));
-    const int dx = o % 3 - 1, dy = (o / 3) % 3 - 1, dz = o / 9 - 1;
-    float s = 0.0f;
-#pragma unroll
-    for (int e = 0; e < 8; ++e) {
-      const int ex = e & 1, ey = (e >> 1) & 1, ez = e >> 2;
-      // local coordinates of this node and of the neighbour in cell (x - 1 + e)
-      const int lnx = 1 - ex, lny = 1 - ey, lnz = 1 - ez;
-      const int lmx = lnx + dx, lmy = lny + dy, lmz = lnz + dz;
-      if (lmx < 0 || lmx > 1 || lmy < 0 || lmy > 1 || lmz < 0 || lmz > 1) continue;
-      const int cx = xi[0] - 1 + ex, cy = xi[1] - 1 + ey, cz = xi[2] - 1 + ez;
-      if (cx < 0 || cx >= L.nc[0] || cy < 0 || cy >= L.nc[1] || cz < 0 || cz >= L.nc[2])
-        continue;
-      const size_t cell = (size_t)cx + (size_t)L.nc[0] * (cy + (size_t)L.nc[1] * cz);
-      const int ln = lnx | (lny << 1) | (lnz << 2);
-      const int lm = lmx | (lmy << 1) | (lmz << 2);
-      const int idx = o == 13 ? ln : (o < 13 ? pair_index(ln, lm) : pair_index(lm, ln));
-      s += yo[cell * kMlpOutPad + idx];
-    }
+    const float sum = sums[o];
     if (o == 13) {
-      const float g = expf(s);
-      gl[pos] = g;
-      gu[pos] = g;
+      const float g = expf(sum);
+      if (CSC) {
+        gl[pos] = g;
+        gu[pos] = g;
+      }
+      const float ge = (float)((double)g * sn);
+      el[kl++] = ge;
+      eu[ku++] = ge;
     } else if (o < 13) {
-      gl[pos] = s;  // Ĝ(n, m), m < n: row n of G, column-n prefix
+      if (CSC) gl[pos] = sum;  // Ĝ(n, m), m < n
+      const int m = (xi[0] + o % 3 - 1) + L.nn[0] * ((xi[1] + (o / 3) % 3 - 1) +
+                                                     L.nn[1] * (xi[2] + o / 9 - 1));
+      el[kl++] = (float)((double)sum * fs[node_dof[m] - 1]);
     } else {
-      gu[pos] = s;  // Ĝ(m, n), m > n: row n of Gᵀ, column-n suffix
+      if (CSC) gu[pos] = sum;  // Ĝ(m, n), m > n
+      eu[ku++] = (float)((double)sum * sn);
     }
+  }
+  for (; kl < kEllSlots; ++kl) el[kl] = 0.0f;
+  for (; ku < kEllSlots; ++ku) eu[ku] = 0.0f;
+}
+
+// Column indices of the 16-slot rows (from the pattern; padding points at the row
+// itself, with zero weight); flags rows whose halves exceed 16 entries.
+__global__ void ell_index_kernel(int64_t n, const int32_t *__restrict__ cp,
+                                 const int32_t *__restrict__ rv, const int32_t *__restrict__ dpos,
+                                 int32_t *__restrict__ gli, int32_t *__restrict__ gui,
+                                 int *__restrict__ overflow) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int lo = cp[i], d = dpos[i], hi = cp[i + 1];
+  if (d - lo + 1 > kEllSlots || hi - d > kEllSlots) atomicOr(overflow, 1);
+  for (int k = 0; k < kEllSlots; ++k) {
+    gli[i * kEllSlots + k] = lo + k <= d ? rv[lo + k] : (int32_t)i;
+    gui[i * kEllSlots + k] = d + k < hi ? rv[d + k] : (int32_t)i;
   }
 }
 
@@ -472,19 +684,34 @@ void fsai_prepare(Ctx &c) {
     c.feat.resize(C * kMlpInPad);
     for (auto *b : {&c.z1, &c.h1, &c.z2, &c.h2, &c.z3, &c.h3, &c.dz1, &c.dz2, &c.dz3})
       b->resize(C * kMlpHidden);
-    c.yo.resize(C * kMlpOutPad);
+    c.yo.resize(C * kMlpOut);  // output-major: yo[n * C + cell]
     c.dy.resize(C * kMlpOutPad);
     const int nb = (int)((C + kMlpCellsPerBlock - 1) / kMlpCellsPerBlock);
     c.wpart.resize((size_t)nb * (kMlpHidden * kMlpHidden + kMlpHidden));
     for (auto *b : {&c.pz, &c.pw, &c.pv, &c.pu, &c.pr, &c.pt, &c.pq})
       b->resize(std::max<int64_t>(n, 1));
     c.lpart.resize(kResidualBlocks);
-    c.fsai_ready = true;
+    for (auto *b : {&c.gle, &c.gue}) b->resize((size_t)std::max<int64_t>(n, 1) * kEllSlots);
+    for (auto *b : {&c.gli, &c.gui}) b->resize((size_t)std::max<int64_t>(n, 1) * kEllSlots);
   }
   hipLaunchKernelGGL(diag_setup_kernel, dim3(grid_for(n)), dim3(kBlock), 0, c.stream, n,
                      c.colptr.get(), c.rowval.get(), c.nzval.get(), c.diagpos.get(),
                      c.fs.get());
   GH_CHECK_HIP(hipGetLastError());
+  if (!c.fsai_ready) {
+    DBuf<int> ovf;
+    ovf.resize(1);
+    GH_CHECK_HIP(hipMemsetAsync(ovf.get(), 0, sizeof(int), c.stream));
+    hipLaunchKernelGGL(ell_index_kernel, dim3(grid_for(n)), dim3(kBlock), 0, c.stream, n,
+                       c.colptr.get(), c.rowval.get(), c.diagpos.get(), c.gli.get(), c.gui.get(),
+                       ovf.get());
+    GH_CHECK_HIP(hipGetLastError());
+    int h = 0;
+    GH_CHECK_HIP(hipMemcpyAsync(&h, ovf.get(), sizeof(int), hipMemcpyDeviceToHost, c.stream));
+    GH_CHECK_HIP(hipStreamSynchronize(c.stream));
+    GH_REQUIRE(h == 0, GH_ERR_UNSUPPORTED, "FSAI rows longer than the 16-slot layout");
+    c.fsai_ready = true;
+  }
 }
 
 void mlp_sync_bf16(Ctx &c) {
@@ -493,7 +720,7 @@ void mlp_sync_bf16(Ctx &c) {
   GH_CHECK_HIP(hipGetLastError());
 }
 
-void fsai_forward_build(Ctx &c) {
+void fsai_forward_build(Ctx &c, bool inference) {
   GH_REQUIRE(c.have_mlp, GH_ERR_STATE, "no network weights: call gh_mlp_init or gh_mlp_set");
   fsai_prepare(c);
   const Lat3 L = lattice_of(c);
@@ -516,23 +743,36 @@ void fsai_forward_build(Ctx &c) {
   hipLaunchKernelGGL(features_kernel, dim3(grid_for(C)), dim3(kBlock), 0, s, L, (int64_t)C,
                      c.have_kappa ? c.kappa.get() : nullptr, c.node_dof.get(), c.diagpos.get(),
                      c.nzval.get(), kdiag, logh, c.feat.get());
-  const int nb = (C + kMlpCellsPerBlock - 1) / kMlpCellsPerBlock;
-  const uint16_t *wq = c.params_bf16.get();
-  hipLaunchKernelGGL((mlp_fwd<kMlpIn, kMlpInPad, kMlpHidden, kMlpHidden, true>), dim3(nb),
-                     dim3(64), 0, s, c.feat.get(), wq, kMlpW1, kMlpB1, c.z1.get(), c.h1.get(), C,
-                     kMlpCellsPerBlock);
-  hipLaunchKernelGGL((mlp_fwd<kMlpHidden, kMlpHidden, kMlpHidden, kMlpHidden, true>), dim3(nb),
-                     dim3(64), 0, s, c.h1.get(), wq, kMlpW2, kMlpB2, c.z2.get(), c.h2.get(), C,
-                     kMlpCellsPerBlock);
-  hipLaunchKernelGGL((mlp_fwd<kMlpHidden, kMlpHidden, kMlpHidden, kMlpHidden, true>), dim3(nb),
-                     dim3(64), 0, s, c.h2.get(), wq, kMlpW3, kMlpB3, c.z3.get(), c.h3.get(), C,
-                     kMlpCellsPerBlock);
-  hipLaunchKernelGGL((mlp_fwd<kMlpHidden, kMlpHidden, kMlpOut, kMlpOutPad, false>), dim3(nb),
-                     dim3(64), 0, s, c.h3.get(), wq, kMlpW4, kMlpB4, c.yo.get(),
-                     (float *)nullptr, C, kMlpCellsPerBlock);
-  hipLaunchKernelGGL(g_assemble_kernel, dim3(grid_for(c.nnodes)), dim3(kBlock), 0, s, L,
-                     (int)c.nnodes, c.node_dof.get(), c.node_fmask.get(), c.colptr.get(),
-                     c.yo.get(), c.gl.get(), c.gu.get());
+  if (inference) {
+    const int ntiles = (C + 15) / 16;
+    const unsigned grid =
+        (unsigned)std::max(1, std::min((ntiles + kFwdWaves - 1) / kFwdWaves, 512));
+    hipLaunchKernelGGL(mlp_fwd_mfma, dim3(grid), dim3(kFwdWaves * 64), 0, s, c.feat.get(),
+                       c.params_bf16.get(), C, c.yo.get());
+  } else {
+    const int nb = (C + kMlpFwdCells - 1) / kMlpFwdCells;
+    const uint16_t *wq = c.params_bf16.get();
+    hipLaunchKernelGGL((mlp_fwd<kMlpIn, kMlpInPad, kMlpHidden, kMlpHidden, true>), dim3(nb),
+                       dim3(64), 0, s, c.feat.get(), wq, kMlpW1, kMlpB1, c.z1.get(), c.h1.get(), C,
+                       kMlpFwdCells);
+    hipLaunchKernelGGL((mlp_fwd<kMlpHidden, kMlpHidden, kMlpHidden, kMlpHidden, true>), dim3(nb),
+                       dim3(64), 0, s, c.h1.get(), wq, kMlpW2, kMlpB2, c.z2.get(), c.h2.get(), C,
+                       kMlpFwdCells);
+    hipLaunchKernelGGL((mlp_fwd<kMlpHidden, kMlpHidden, kMlpHidden, kMlpHidden, true>), dim3(nb),
+                       dim3(64), 0, s, c.h2.get(), wq, kMlpW3, kMlpB3, c.z3.get(), c.h3.get(), C,
+                       kMlpFwdCells);
+    hipLaunchKernelGGL((mlp_fwd<kMlpHidden, kMlpHidden, kMlpOut, kMlpOutPad, false, true>), dim3(nb),
+                       dim3(64), 0, s, c.h3.get(), wq, kMlpW4, kMlpB4, c.yo.get(),
+                       (float *)nullptr, C, kMlpFwdCells);
+  }
+  if (inference)
+    hipLaunchKernelGGL(g_assemble_kernel<false>, dim3(grid_for(c.nnodes)), dim3(kBlock), 0, s, L,
+                       (int)c.nnodes, c.node_dof.get(), c.node_fmask.get(), c.colptr.get(),
+                       c.yo.get(), c.fs.get(), c.gl.get(), c.gu.get(), c.gle.get(), c.gue.get());
+  else
+    hipLaunchKernelGGL(g_assemble_kernel<true>, dim3(grid_for(c.nnodes)), dim3(kBlock), 0, s, L,
+                       (int)c.nnodes, c.node_dof.get(), c.node_fmask.get(), c.colptr.get(),
+                       c.yo.get(), c.fs.get(), c.gl.get(), c.gu.get(), c.gle.get(), c.gue.get());
   GH_CHECK_HIP(hipGetLastError());
   c.fsai_built = true;
 }
@@ -558,13 +798,9 @@ void fsai_apply_upper_grid(Ctx &c, const double *y, double *z, const double *d, 
   GH_CHECK_HIP(hipGetLastError());
 }
 
-void fsai_apply_upper(Ctx &c, const double *y, double *z, const double *rdot, double *part) {
-  fsai_apply_upper_grid(c, y, z, rdot, part, 512);
-}
-
 double fsai_train_step(Ctx &c, int nprobes, uint64_t seed, float lr) {
   GH_REQUIRE(nprobes >= 1 && nprobes <= 1024, GH_ERR_ARG, "nprobes must be in 1..1024");
-  fsai_forward_build(c);
+  fsai_forward_build(c, false);
   hipStream_t s = c.stream;
   const int64_t n = c.nfree;
   const int C = (int)c.ncells;

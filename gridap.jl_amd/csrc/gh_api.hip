@@ -325,7 +325,7 @@ int gh_pcg(gh_ctx *ctx, int precond, const double *b, double *x, double rtol, in
     else
       GH_CHECK_HIP(hipMemcpyAsync(c.s_b.get(), c.b.get(), sizeof(double) * n,
                                   hipMemcpyDeviceToDevice, c.stream));
-    if (precond == GH_PRECOND_FSAI && !c.fsai_built) gh::fsai_forward_build(c);
+    if (precond == GH_PRECOND_FSAI && !c.fsai_built) gh::fsai_forward_build(c, true);
     gh_solve_stats local{};
     gh::pcg_device(c, precond, rtol, maxit, &local);
     local.precond_used = precond;
@@ -442,7 +442,7 @@ int gh_grf_kappa(gh_ctx *ctx, int nmodes, const double *omega, const double *pha
 int gh_fsai_build(gh_ctx *ctx) {
   return guarded(ctx, [&](gh::Ctx &c) {
     GH_REQUIRE(c.have_values, GH_ERR_STATE, "assemble the matrix first");
-    gh::fsai_forward_build(c);
+    gh::fsai_forward_build(c, true);
     GH_CHECK_HIP(hipStreamSynchronize(c.stream));
   });
 }
@@ -462,11 +462,32 @@ int gh_fsai_get(gh_ctx *ctx, float *feat, float *yout, float *gl, float *gu, flo
     auto cp = [&](void *dst, const void *src, size_t bytes) {
       if (dst) GH_CHECK_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c.stream));
     };
-    cp(feat, c.feat.get(), sizeof(float) * c.ncells * gh::kMlpInPad);
-    cp(yout, c.yo.get(), sizeof(float) * c.ncells * gh::kMlpOutPad);
+    const size_t C = (size_t)c.ncells;
+    std::vector<float> yt(yout ? C * gh::kMlpOut : 0);
+    cp(feat, c.feat.get(), sizeof(float) * C * gh::kMlpInPad);
+    cp(yout ? yt.data() : nullptr, c.yo.get(), sizeof(float) * C * gh::kMlpOut);
     cp(gl, c.gl.get(), sizeof(float) * c.nnz);
     cp(gu, c.gu.get(), sizeof(float) * c.nnz);
     cp(grad, c.grads.get(), sizeof(float) * gh::kMlpParams);
+    GH_CHECK_HIP(hipStreamSynchronize(c.stream));
+    if (yout)  // device layout is output-major; hand back cells x kMlpOutPad
+      for (size_t cell = 0; cell < C; ++cell)
+        for (int n = 0; n < gh::kMlpOutPad; ++n)
+          yout[cell * gh::kMlpOutPad + n] = n < gh::kMlpOut ? yt[(size_t)n * C + cell] : 0.0f;
+  });
+}
+
+int gh_fsai_factor_get(gh_ctx *ctx, float *gle, int32_t *gli, float *gue, int32_t *gui) {
+  return guarded(ctx, [&](gh::Ctx &c) {
+    GH_REQUIRE(c.fsai_built, GH_ERR_STATE, "no FSAI factor: call gh_fsai_build");
+    const size_t n = (size_t)c.nfree * gh::kFsaiEllSlots;
+    auto cp = [&](void *dst, const void *src, size_t bytes) {
+      if (dst) GH_CHECK_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c.stream));
+    };
+    cp(gle, c.gle.get(), sizeof(float) * n);
+    cp(gli, c.gli.get(), sizeof(int32_t) * n);
+    cp(gue, c.gue.get(), sizeof(float) * n);
+    cp(gui, c.gui.get(), sizeof(int32_t) * n);
     GH_CHECK_HIP(hipStreamSynchronize(c.stream));
   });
 }

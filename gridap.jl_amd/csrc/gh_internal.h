@@ -119,6 +119,7 @@ struct Ctx {
 
   // ---- solver scratch ----
   DBuf<double> s_r, s_z, s_p, s_q, s_x, s_dinv, s_b, s_part, s_scal, s_y;
+  DBuf<double> s_r2, s_p2;          // second buffers of r (factored M) and p
 
   // ---- learned FSAI (lattice Q1) ----
   bool fsai_ready = false;   // structures allocated for the current pattern
@@ -127,6 +128,8 @@ struct Ctx {
   DBuf<double> fs;           // N: diag(A)^(-1/2)
   DBuf<float> gl, gu;        // NNZ: Ĝ in column-prefix (rows of G) and suffix (rows of Gᵀ) layout
   DBuf<float> gG;            // NNZ: loss gradient w.r.t. Ĝ, prefix layout
+  DBuf<float> gle, gue;      // N x 16: rows of G = Ĝ D^-1/2 and of Gᵀ (solver copy)
+  DBuf<int32_t> gli, gui;    // N x 16: their column indices (padding: the row itself)
   DBuf<float> feat;          // C x 16 (10 used)
   DBuf<float> z1, h1, z2, h2, z3, h3;  // C x 64 pre-/post-activations
   DBuf<float> yo, dy;        // C x 40 (36 used): network output and its gradient
@@ -153,14 +156,16 @@ void assemble_simplex(Ctx &c, int precision);
 void spmv_device(Ctx &c, const double *x, double *y);
 void pcg_device(Ctx &c, int precond, double rtol, int maxit, gh_solve_stats *st);
 // fsai.hip
+constexpr int kFsaiEllSlots = 16;  // slots per row of the solver's G and Gᵀ
 constexpr int kMlpIn = 10, kMlpInPad = 16, kMlpHidden = 64, kMlpOut = 36, kMlpOutPad = 40;
 constexpr int kMlpW1 = 0, kMlpB1 = 640, kMlpW2 = 704, kMlpB2 = 4800, kMlpW3 = 4864,
               kMlpB3 = 8960, kMlpW4 = 9024, kMlpB4 = 11328, kMlpParams = 11364;
 void fsai_prepare(Ctx &c);                       // diag positions, D^-1/2, buffers
-void fsai_forward_build(Ctx &c);                 // features -> MLP -> Ĝ
+// features -> MLP -> Ĝ; inference: the BF16 matrix-core forward (no stored
+// activations), else the FP32 forward that keeps the activations for training
+void fsai_forward_build(Ctx &c, bool inference);
 double fsai_train_step(Ctx &c, int nprobes, uint64_t seed, float lr);
 void fsai_apply_lower(Ctx &c, const double *r, double *y);   // y = Ĝ (D^-1/2 r)
-void fsai_apply_upper(Ctx &c, const double *y, double *z, const double *rdot, double *part);
 void mlp_sync_bf16(Ctx &c);
 void grf_kappa(Ctx &c, int nmodes, const double *omega, const double *phase, double amp);
 

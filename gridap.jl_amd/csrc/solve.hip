@@ -3,8 +3,22 @@
 // Gridap solves AffineFEOperators with a direct LU (src/Algebra/LinearSolvers.jl:
 // 249-274); this engine keeps the matrix in HBM and runs CG instead.  The matrix
 // is symmetric (the assembled forms are), so the CSC arrays are read as CSR rows.
-// Reductions are two-level with a fixed order (block partials, then every
-// consumer sums the partials in index order), so runs are bit-reproducible.
+//
+// One iteration is two kernels with a diagonal (or no) preconditioner and three
+// with the factored FSAI preconditioner M = GᵀG:
+//   spmv_dir      p' = z + β p and q = A p' in one pass (p' of the neighbours is
+//                 formed on the fly from z and p), partials of p'·q;
+//   update        x += α p', r -= α q, z = M r, partials of r·z and r·r  (diagonal M);
+//   update_lower  x += α p', r' = r - α q, y = G r' (r' of the neighbours formed on
+//                 the fly from r and q; r double-buffered), partials of r'·r';
+//   upper_rz      z = Gᵀ y, partials of r'·z;
+// G and Gᵀ are read from the solver's 16-slot row copy (weights with D^-1/2
+// folded in, column indices), written with the factor.
+// Reductions are deterministic and need no grid-wide synchronisation: a kernel
+// writes one partial per block, and every block of the consuming kernel sums all
+// of them in index order at its start (the same value everywhere); block 0 of
+// spmv_dir publishes r·z for the next iteration's β.  The host sums the r·r
+// partials itself at the convergence checks.
 #include <algorithm>
 #include <cmath>
 
@@ -14,57 +28,70 @@ namespace gh {
 namespace {
 
 constexpr int kBlock = 512;
-constexpr int kParts = 512;  // blocks of the vector kernels = partials per reduction
-constexpr int kLanes = 8;    // lanes per matrix row in the SpMV
+constexpr int kMaxParts = 8192;  // partial slots per reduced quantity (max blocks per grid)
+constexpr int kSpmvLanes = 8;    // lanes per matrix row in the SpMV
+constexpr int kApplyLanes = 4;   // lanes per row in the FSAI applications
+
+// scalar slots
+constexpr int kRz = 0;  // [0..1]: r·z, double-buffered by iteration parity
+constexpr int kRr = 2, kBb = 3, kPq = 4;
 
 __device__ __forceinline__ double block_sum(double v, double *sh) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  __syncthreads();
   if (lane == 0) sh[w] = v;
   __syncthreads();
   double r = 0.0;
-  if (threadIdx.x == 0) {
-    for (int k = 0; k < (int)(blockDim.x >> 6); ++k) r += sh[k];
-    sh[0] = r;
-  }
-  __syncthreads();
-  r = sh[0];
-  __syncthreads();
+  for (int k = 0; k < (int)(blockDim.x >> 6); ++k) r += sh[k];
   return r;
 }
 
-// Sum of kParts partials in index order, identical in every block: one partial
-// per thread (kParts == kBlock), then the fixed shuffle/LDS tree of block_sum.
-static_assert(kParts == kBlock, "sum_parts assumes one partial per thread");
-__device__ __forceinline__ double sum_parts(const double *parts, double *sh) {
-  return block_sum(parts[threadIdx.x], sh);
+// One partial per block: part[blockIdx.x] = Σ_block v.
+__device__ __forceinline__ void put_partial(double v, double *part, double *sh) {
+  v = block_sum(v, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = v;
 }
 
-// y = A x (rows from the CSC of a symmetric A); optional partial of x.y.
+// Σ part[0..np) in a fixed order, the same in every block.
+__device__ __forceinline__ double sum_partials(const double *__restrict__ part, int np,
+                                               double *sh) {
+  constexpr int kUnroll = 8;
+  const int T = (int)blockDim.x;
+  double a = 0.0;
+  for (int b0 = threadIdx.x; b0 < np; b0 += T * kUnroll) {
+    double w[kUnroll];
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      const int b = b0 + u * T;
+      w[u] = b < np ? part[b] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) a += w[u];
+  }
+  return block_sum(a, sh);
+}
+
+inline unsigned grid_rows(int64_t rows, int rows_per_block) {
+  return (unsigned)std::min<int64_t>(std::max<int64_t>((rows + rows_per_block - 1) / rows_per_block, 1),
+                                     kMaxParts);
+}
+
+// y = A x (rows from the CSC of a symmetric A).
 __global__ __launch_bounds__(kBlock) void spmv_kernel(int64_t n, const int32_t *__restrict__ cp,
                                                       const int32_t *__restrict__ rv,
                                                       const double *__restrict__ val,
                                                       const double *__restrict__ x,
-                                                      double *__restrict__ y, double *part) {
-  __shared__ double sh[kBlock / 64];
-  const int sub = threadIdx.x % kLanes;
-  const int64_t rows_per_iter = (int64_t)gridDim.x * (kBlock / kLanes);
-  double dot = 0.0;
-  for (int64_t row = (int64_t)blockIdx.x * (kBlock / kLanes) + threadIdx.x / kLanes; row < n;
-       row += rows_per_iter) {
-    const int32_t p0 = cp[row], p1 = cp[row + 1];
+                                                      double *__restrict__ y) {
+  const int sub = threadIdx.x % kSpmvLanes;
+  const int64_t stride = (int64_t)gridDim.x * (kBlock / kSpmvLanes);
+  for (int64_t row = (int64_t)blockIdx.x * (kBlock / kSpmvLanes) + threadIdx.x / kSpmvLanes;
+       row < n; row += stride) {
     double s = 0.0;
-    for (int32_t p = p0 + sub; p < p1; p += kLanes) s += val[p] * x[rv[p]];
+    for (int32_t p = cp[row] + sub; p < cp[row + 1]; p += kSpmvLanes) s += val[p] * x[rv[p]];
 #pragma unroll
-    for (int o = kLanes / 2; o > 0; o >>= 1) s += __shfl_xor(s, o);
-    if (sub == 0) {
-      y[row] = s;
-      dot += s * x[row];
-    }
-  }
-  if (part) {
-    double t = block_sum(dot, sh);
-    if (threadIdx.x == 0) part[blockIdx.x] = t;
+    for (int o = kSpmvLanes / 2; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    if (sub == 0) y[row] = s;
   }
 }
 
@@ -79,205 +106,293 @@ __global__ void diag_inv_kernel(int64_t n, const int32_t *__restrict__ cp,
   dinv[j] = d != 0.0 ? 1.0 / d : 1.0;
 }
 
-// r = b - q; with a diagonal (or no) preconditioner also z = M r, p = z.  With a
-// factored preconditioner (`factored`) z, p and the r.z partials come from the
-// apply kernels instead.  Partials of r.z, r.r, b.b.
+// r = b - q; with a diagonal (or no) preconditioner also z = M r; partials of r·z,
+// r·r, b·b.
 __global__ __launch_bounds__(kBlock) void init_kernel(int64_t n, const double *__restrict__ b,
                                                       const double *__restrict__ q,
                                                       const double *__restrict__ dinv,
-                                                      int factored, double *r, double *z,
-                                                      double *p, double *part) {
+                                                      int factored, double *__restrict__ r,
+                                                      double *__restrict__ z,
+                                                      double *__restrict__ part_rz,
+                                                      double *__restrict__ part_rr,
+                                                      double *__restrict__ part_bb) {
   __shared__ double sh[kBlock / 64];
   double rz = 0.0, rr = 0.0, bb = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * kBlock) {
-    double ri = b[i] - q[i];
+    const double ri = b[i] - q[i];
     r[i] = ri;
     if (!factored) {
-      double zi = dinv ? ri * dinv[i] : ri;
+      const double zi = dinv ? ri * dinv[i] : ri;
       z[i] = zi;
-      p[i] = zi;
       rz += ri * zi;
     }
     rr += ri * ri;
     bb += b[i] * b[i];
   }
-  rz = block_sum(rz, sh);
-  rr = block_sum(rr, sh);
-  bb = block_sum(bb, sh);
-  if (threadIdx.x == 0) {
-    if (!factored) part[blockIdx.x] = rz;
-    part[kParts + blockIdx.x] = rr;
-    part[2 * kParts + blockIdx.x] = bb;
-  }
+  if (!factored) put_partial(rz, part_rz, sh);
+  put_partial(rr, part_rr, sh);
+  put_partial(bb, part_bb, sh);
 }
 
-__global__ void copy_kernel(int64_t n, const double *__restrict__ a, double *__restrict__ b) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) b[i] = a[i];
-}
-
-// scal layout: [0..1] rz (double-buffered by iteration parity), [2] rr, [3] bb.
-__global__ __launch_bounds__(kBlock) void init_scalars(const double *part, double *scal) {
+// r·z of this iteration from its partials (published to scal[rz slot] by block 0);
+// p' = z + β p, q = A p', partials of p'·q  (β = 0 on the first iteration of a start)
+__global__ __launch_bounds__(kBlock) void spmv_dir_kernel(
+    int64_t n, const int32_t *__restrict__ cp, const int32_t *__restrict__ rv,
+    const double *__restrict__ val, const double *__restrict__ z,
+    const double *__restrict__ pold, double *__restrict__ pnew, double *__restrict__ q,
+    double *scal, int par, int first, const double *__restrict__ part_rz, int np_rz,
+    double *__restrict__ part_pq) {
   __shared__ double sh[kBlock / 64];
-  double rz = sum_parts(part, sh);
-  double rr = sum_parts(part + kParts, sh);
-  double bb = sum_parts(part + 2 * kParts, sh);
-  if (threadIdx.x == 0) {
-    scal[0] = rz;
-    scal[1] = rz;
-    scal[2] = rr;
-    scal[3] = bb;
+  const double rz = sum_partials(part_rz, np_rz, sh);
+  const double beta = first ? 0.0 : rz / scal[kRz + (par ^ 1)];
+  if (blockIdx.x == 0 && threadIdx.x == 0) scal[kRz + par] = rz;
+  const int sub = threadIdx.x % kSpmvLanes;
+  const int64_t stride = (int64_t)gridDim.x * (kBlock / kSpmvLanes);
+  double dot = 0.0;
+  for (int64_t row = (int64_t)blockIdx.x * (kBlock / kSpmvLanes) + threadIdx.x / kSpmvLanes;
+       row < n; row += stride) {
+    double s = 0.0;
+    if (first) {
+      for (int32_t p = cp[row] + sub; p < cp[row + 1]; p += kSpmvLanes) s += val[p] * z[rv[p]];
+    } else {
+      for (int32_t p = cp[row] + sub; p < cp[row + 1]; p += kSpmvLanes) {
+        const int32_t j = rv[p];
+        s += val[p] * (z[j] + beta * pold[j]);
+      }
+    }
+#pragma unroll
+    for (int o = kSpmvLanes / 2; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    if (sub == 0) {
+      const double pn = first ? z[row] : z[row] + beta * pold[row];
+      pnew[row] = pn;
+      q[row] = s;
+      dot += pn * s;
+    }
   }
+  put_partial(dot, part_pq, sh);
 }
 
-// alpha = rz / (p.q); x += alpha p; r -= alpha q; z = M r (diagonal M; with a
-// factored M the apply kernels follow); partials of r.z, r.r.
-__global__ __launch_bounds__(kBlock) void update_kernel(int64_t n, int parity,
-                                                        const double *__restrict__ pq_part,
-                                                        const double *__restrict__ scal,
-                                                        const double *__restrict__ p,
-                                                        const double *__restrict__ q,
-                                                        const double *__restrict__ dinv,
-                                                        int factored, double *x, double *r,
-                                                        double *z, double *part) {
+// α = r·z / p'·q; x += α p'; r -= α q; z = M r (diagonal M or none); partials of
+// r·z and r·r
+__global__ __launch_bounds__(kBlock) void update_kernel(
+    int64_t n, int par, const double *__restrict__ dinv, double *__restrict__ x,
+    double *__restrict__ r, double *__restrict__ z, const double *__restrict__ p,
+    const double *__restrict__ q, const double *scal, const double *__restrict__ part_pq,
+    int np_pq, double *__restrict__ part_rz, double *__restrict__ part_rr) {
   __shared__ double sh[kBlock / 64];
-  const double pq = sum_parts(pq_part, sh);
-  const double alpha = scal[parity] / pq;
+  const double alpha = scal[kRz + par] / sum_partials(part_pq, np_pq, sh);
   double rz = 0.0, rr = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * kBlock) {
     x[i] += alpha * p[i];
-    double ri = r[i] - alpha * q[i];
+    const double ri = r[i] - alpha * q[i];
     r[i] = ri;
-    if (!factored) {
-      double zi = dinv ? ri * dinv[i] : ri;
-      z[i] = zi;
-      rz += ri * zi;
-    }
+    const double zi = dinv ? ri * dinv[i] : ri;
+    z[i] = zi;
+    rz += ri * zi;
     rr += ri * ri;
   }
-  rz = block_sum(rz, sh);
-  rr = block_sum(rr, sh);
-  if (threadIdx.x == 0) {
-    if (!factored) part[blockIdx.x] = rz;
-    part[kParts + blockIdx.x] = rr;
-  }
+  put_partial(rz, part_rz, sh);
+  put_partial(rr, part_rr, sh);
 }
 
-// beta = rz_new / rz_old; p = z + beta p; block 0 publishes rz_new and r.r.
-__global__ __launch_bounds__(kBlock) void direction_kernel(int64_t n, int parity,
-                                                           const double *__restrict__ part,
-                                                           double *scal,
-                                                           const double *__restrict__ z,
-                                                           double *p) {
+// G and Gᵀ are 16-slot rows (kFsaiEllSlots): 4 lanes per row, 4 slots per lane, one
+// 16-byte load of weights and one of column indices per lane.
+constexpr int kSlots = kFsaiEllSlots;
+static_assert(kSlots == 4 * kApplyLanes, "4 slots per lane");
+
+// α = r·z / p'·q; x += α p'; r' = r - α q; y = G r'; partials of r'·r'
+__global__ __launch_bounds__(kBlock) void update_lower_kernel(
+    int64_t n, int par, const float *__restrict__ ge, const int32_t *__restrict__ gi,
+    double *__restrict__ x, const double *__restrict__ rold, double *__restrict__ rnew,
+    const double *__restrict__ p, const double *__restrict__ q, double *__restrict__ y,
+    const double *scal, const double *__restrict__ part_pq, int np_pq,
+    double *__restrict__ part_rr) {
   __shared__ double sh[kBlock / 64];
-  const double rz_new = sum_parts(part, sh);
-  const double rr = sum_parts(part + kParts, sh);
-  const double beta = rz_new / scal[parity];
-  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * kBlock)
-    p[i] = z[i] + beta * p[i];
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    scal[parity ^ 1] = rz_new;
-    scal[2] = rr;
+  const double alpha = scal[kRz + par] / sum_partials(part_pq, np_pq, sh);
+  const int sub = threadIdx.x % kApplyLanes;
+  const int64_t stride = (int64_t)gridDim.x * (kBlock / kApplyLanes);
+  double rr = 0.0;
+  for (int64_t row = (int64_t)blockIdx.x * (kBlock / kApplyLanes) + threadIdx.x / kApplyLanes;
+       row < n; row += stride) {
+    const float4 g = reinterpret_cast<const float4 *>(ge + row * kSlots)[sub];
+    const int4 j = reinterpret_cast<const int4 *>(gi + row * kSlots)[sub];
+    double s = (double)g.x * (rold[j.x] - alpha * q[j.x]);
+    s += (double)g.y * (rold[j.y] - alpha * q[j.y]);
+    s += (double)g.z * (rold[j.z] - alpha * q[j.z]);
+    s += (double)g.w * (rold[j.w] - alpha * q[j.w]);
+#pragma unroll
+    for (int o = kApplyLanes / 2; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    if (sub == 0) {
+      y[row] = s;
+      const double ri = rold[row] - alpha * q[row];
+      rnew[row] = ri;
+      x[row] += alpha * p[row];
+      rr += ri * ri;
+    }
+  }
+  put_partial(rr, part_rr, sh);
+}
+
+// y = G r (start of a solve)
+__global__ __launch_bounds__(kBlock) void lower_kernel(int64_t n, const float *__restrict__ ge,
+                                                       const int32_t *__restrict__ gi,
+                                                       const double *__restrict__ r,
+                                                       double *__restrict__ y) {
+  const int sub = threadIdx.x % kApplyLanes;
+  const int64_t stride = (int64_t)gridDim.x * (kBlock / kApplyLanes);
+  for (int64_t row = (int64_t)blockIdx.x * (kBlock / kApplyLanes) + threadIdx.x / kApplyLanes;
+       row < n; row += stride) {
+    const float4 g = reinterpret_cast<const float4 *>(ge + row * kSlots)[sub];
+    const int4 j = reinterpret_cast<const int4 *>(gi + row * kSlots)[sub];
+    double s = (double)g.x * r[j.x] + (double)g.y * r[j.y] + (double)g.z * r[j.z] +
+               (double)g.w * r[j.w];
+#pragma unroll
+    for (int o = kApplyLanes / 2; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    if (sub == 0) y[row] = s;
   }
 }
 
-unsigned spmv_grid(int64_t n) {
-  int64_t rows_per_block = kBlock / kLanes;
-  int64_t g = (n + rows_per_block - 1) / rows_per_block;
-  return (unsigned)std::min<int64_t>(std::max<int64_t>(g, 1), kParts);
+// z = Gᵀ y; partials of r·z
+__global__ __launch_bounds__(kBlock) void upper_rz_kernel(
+    int64_t n, const float *__restrict__ ge, const int32_t *__restrict__ gi,
+    const double *__restrict__ y, const double *__restrict__ r, double *__restrict__ z,
+    double *__restrict__ part_rz) {
+  __shared__ double sh[kBlock / 64];
+  const int sub = threadIdx.x % kApplyLanes;
+  const int64_t stride = (int64_t)gridDim.x * (kBlock / kApplyLanes);
+  double rz = 0.0;
+  for (int64_t row = (int64_t)blockIdx.x * (kBlock / kApplyLanes) + threadIdx.x / kApplyLanes;
+       row < n; row += stride) {
+    const float4 g = reinterpret_cast<const float4 *>(ge + row * kSlots)[sub];
+    const int4 j = reinterpret_cast<const int4 *>(gi + row * kSlots)[sub];
+    double s = (double)g.x * y[j.x] + (double)g.y * y[j.y] + (double)g.z * y[j.z] +
+               (double)g.w * y[j.w];
+#pragma unroll
+    for (int o = kApplyLanes / 2; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    if (sub == 0) {
+      z[row] = s;
+      rz += s * r[row];
+    }
+  }
+  put_partial(rz, part_rz, sh);
 }
 
 }  // namespace
 
 void spmv_device(Ctx &c, const double *x, double *y) {
-  hipLaunchKernelGGL(spmv_kernel, dim3(spmv_grid(c.nfree)), dim3(kBlock), 0, c.stream,
-                     (int64_t)c.nfree, c.colptr.get(), c.rowval.get(), c.nzval.get(), x, y,
-                     (double *)nullptr);
+  hipLaunchKernelGGL(spmv_kernel, dim3(grid_rows(c.nfree, kBlock / kSpmvLanes)), dim3(kBlock), 0,
+                     c.stream, (int64_t)c.nfree, c.colptr.get(), c.rowval.get(), c.nzval.get(),
+                     x, y);
   GH_CHECK_HIP(hipGetLastError());
 }
+
+namespace {
+// Σ of np partials on the host (convergence checks only; fixed order).
+double host_sum(const std::vector<double> &v, int np) {
+  double a = 0.0;
+  for (int k = 0; k < np; ++k) a += v[k];
+  return a;
+}
+}  // namespace
 
 void pcg_device(Ctx &c, int precond, double rtol, int maxit, gh_solve_stats *st) {
   const int64_t n = c.nfree;
   hipStream_t s = c.stream;
-  for (auto *bf : {&c.s_r, &c.s_z, &c.s_p, &c.s_q, &c.s_dinv}) bf->resize(std::max<int64_t>(n, 1));
-  c.s_part.resize(4 * kParts);
-  c.s_scal.resize(4);
-  // Partials must be zero where a grid has fewer than kParts blocks (the SpMV).
-  GH_CHECK_HIP(hipMemsetAsync(c.s_part.get(), 0, sizeof(double) * 4 * kParts, s));
+  const size_t nn = (size_t)std::max<int64_t>(n, 1);
+  for (auto *bf : {&c.s_r, &c.s_z, &c.s_p, &c.s_p2, &c.s_q}) bf->resize(nn);
+  c.s_part.resize(4 * kMaxParts);
+  c.s_scal.resize(8);
   double *dinv = nullptr;
   const int factored = precond == GH_PRECOND_FSAI;
   if (precond == GH_PRECOND_JACOBI) {
+    c.s_dinv.resize(nn);
     hipLaunchKernelGGL(diag_inv_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n,
                        c.colptr.get(), c.rowval.get(), c.nzval.get(), c.s_dinv.get());
     dinv = c.s_dinv.get();
   }
   if (factored) {
     GH_REQUIRE(c.fsai_built, GH_ERR_STATE, "no FSAI factor for this matrix: call gh_fsai_build");
-    c.s_y.resize(std::max<int64_t>(n, 1));
+    c.s_y.resize(nn);
+    c.s_r2.resize(nn);
   }
-  // z = Gᵀ (G r) with the r.z partials in part[0, kParts)
-  auto apply_factored = [&]() {
-    fsai_apply_lower(c, c.s_r.get(), c.s_y.get());
-    fsai_apply_upper(c, c.s_y.get(), c.s_z.get(), c.s_r.get(), c.s_part.get());
-  };
+  double *scal = c.s_scal.get();
+  double *part_pq = c.s_part.get(), *part_rz = part_pq + kMaxParts;
+  double *part_rr = part_rz + kMaxParts, *part_bb = part_rr + kMaxParts;
+  double *rbuf[2] = {c.s_r.get(), factored ? c.s_r2.get() : c.s_r.get()};
+  double *pbuf[2] = {c.s_p.get(), c.s_p2.get()};
+  const unsigned g_vec = grid_rows(n, kBlock);
+  const unsigned g_spmv = grid_rows(n, kBlock / kSpmvLanes);
+  const unsigned g_apply = grid_rows(n, kBlock / kApplyLanes);
+  // partial counts of each quantity, by producer
+  const int np_rz = factored ? (int)g_apply : (int)g_vec;
+  const int np_rr_iter = factored ? (int)g_apply : (int)g_vec;
+  std::vector<double> hrr(kMaxParts), hbb(kMaxParts);
+
   hipEvent_t e0, e1;
   GH_CHECK_HIP(hipEventCreate(&e0));
   GH_CHECK_HIP(hipEventCreate(&e1));
   GH_CHECK_HIP(hipEventRecord(e0, s));
 
-  double *pq_part = c.s_part.get() + 3 * kParts;  // written only by the SpMV grid
-  const unsigned vgrid = kParts;
-  const unsigned mgrid = spmv_grid(n);
-  double h[4] = {0, 0, 0, 0};
   double bnorm = 1.0, relres = INFINITY;
   int it = 0, restarts = 0;
   bool converged = false;
   while (true) {
     // (re)start from the true residual r = b - A x
     spmv_device(c, c.s_x.get(), c.s_q.get());
-    hipLaunchKernelGGL(init_kernel, dim3(vgrid), dim3(kBlock), 0, s, n, c.s_b.get(),
-                       c.s_q.get(), dinv, factored, c.s_r.get(), c.s_z.get(), c.s_p.get(),
-                       c.s_part.get());
+    hipLaunchKernelGGL(init_kernel, dim3(g_vec), dim3(kBlock), 0, s, n, c.s_b.get(), c.s_q.get(),
+                       dinv, factored, rbuf[0], c.s_z.get(), part_rz, part_rr, part_bb);
     if (factored) {
-      apply_factored();
-      hipLaunchKernelGGL(copy_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n,
-                         c.s_z.get(), c.s_p.get());
+      hipLaunchKernelGGL(lower_kernel, dim3(g_apply), dim3(kBlock), 0, s, n, c.gle.get(),
+                         c.gli.get(), rbuf[0], c.s_y.get());
+      hipLaunchKernelGGL(upper_rz_kernel, dim3(g_apply), dim3(kBlock), 0, s, n, c.gue.get(),
+                         c.gui.get(), c.s_y.get(), rbuf[0], c.s_z.get(), part_rz);
     }
-    hipLaunchKernelGGL(init_scalars, dim3(1), dim3(kBlock), 0, s, c.s_part.get(),
-                       c.s_scal.get());
     GH_CHECK_HIP(hipGetLastError());
-    GH_CHECK_HIP(hipMemcpyAsync(h, c.s_scal.get(), sizeof(h), hipMemcpyDeviceToHost, s));
+    GH_CHECK_HIP(hipMemcpyAsync(hrr.data(), part_rr, sizeof(double) * g_vec,
+                                hipMemcpyDeviceToHost, s));
+    GH_CHECK_HIP(hipMemcpyAsync(hbb.data(), part_bb, sizeof(double) * g_vec,
+                                hipMemcpyDeviceToHost, s));
     GH_CHECK_HIP(hipStreamSynchronize(s));
-    bnorm = std::sqrt(h[3]) > 0 ? std::sqrt(h[3]) : 1.0;
-    relres = std::sqrt(h[2]) / bnorm;
+    const double bb = host_sum(hbb, (int)g_vec);
+    bnorm = std::sqrt(bb) > 0 ? std::sqrt(bb) : 1.0;
+    relres = std::sqrt(host_sum(hrr, (int)g_vec)) / bnorm;
     if (relres <= rtol) {
       converged = true;
       break;
     }
     if (it >= maxit || restarts > 3) break;
     ++restarts;
-    int parity = 0;  // scal[0] holds rz after init
     const int check = 8;
+    int k = 0;  // iteration since this start: selects the buffers and scalar slots
     while (it < maxit) {
-      for (int k = 0; k < check && it < maxit; ++k, ++it) {
-        hipLaunchKernelGGL(spmv_kernel, dim3(mgrid), dim3(kBlock), 0, s, n, c.colptr.get(),
-                           c.rowval.get(), c.nzval.get(), c.s_p.get(), c.s_q.get(), pq_part);
-        hipLaunchKernelGGL(update_kernel, dim3(vgrid), dim3(kBlock), 0, s, n, parity, pq_part,
-                           c.s_scal.get(), c.s_p.get(), c.s_q.get(), dinv, factored,
-                           c.s_x.get(), c.s_r.get(), c.s_z.get(), c.s_part.get());
-        if (factored) apply_factored();
-        hipLaunchKernelGGL(direction_kernel, dim3(vgrid), dim3(kBlock), 0, s, n, parity,
-                           c.s_part.get(), c.s_scal.get(), c.s_z.get(), c.s_p.get());
-        parity ^= 1;
+      for (int m = 0; m < check && it < maxit; ++m, ++it, ++k) {
+        const int par = k & 1;
+        double *pold = pbuf[par], *pnew = pbuf[par ^ 1];
+        hipLaunchKernelGGL(spmv_dir_kernel, dim3(g_spmv), dim3(kBlock), 0, s, n, c.colptr.get(),
+                           c.rowval.get(), c.nzval.get(), c.s_z.get(), pold, pnew, c.s_q.get(),
+                           scal, par, k == 0 ? 1 : 0, part_rz, np_rz, part_pq);
+        if (factored) {
+          double *rold = rbuf[par], *rnew = rbuf[par ^ 1];
+          hipLaunchKernelGGL(update_lower_kernel, dim3(g_apply), dim3(kBlock), 0, s, n, par,
+                             c.gle.get(), c.gli.get(), c.s_x.get(), rold, rnew, pnew, c.s_q.get(),
+                             c.s_y.get(), scal, part_pq, (int)g_spmv, part_rr);
+          hipLaunchKernelGGL(upper_rz_kernel, dim3(g_apply), dim3(kBlock), 0, s, n, c.gue.get(),
+                             c.gui.get(), c.s_y.get(), rnew, c.s_z.get(), part_rz);
+        } else {
+          hipLaunchKernelGGL(update_kernel, dim3(g_vec), dim3(kBlock), 0, s, n, par, dinv,
+                             c.s_x.get(), rbuf[0], c.s_z.get(), pnew, c.s_q.get(), scal, part_pq,
+                             (int)g_spmv, part_rz, part_rr);
+        }
       }
       GH_CHECK_HIP(hipGetLastError());
-      GH_CHECK_HIP(hipMemcpyAsync(h, c.s_scal.get(), sizeof(h), hipMemcpyDeviceToHost, s));
+      GH_CHECK_HIP(hipMemcpyAsync(hrr.data(), part_rr, sizeof(double) * np_rr_iter,
+                                  hipMemcpyDeviceToHost, s));
       GH_CHECK_HIP(hipStreamSynchronize(s));
-      if (!std::isfinite(h[2])) break;
-      if (std::sqrt(h[2]) / bnorm <= rtol) break;
+      const double rr = host_sum(hrr, np_rr_iter);
+      if (!std::isfinite(rr)) break;
+      if (std::sqrt(rr) / bnorm <= rtol) break;
     }
   }
   GH_CHECK_HIP(hipEventRecord(e1, s));

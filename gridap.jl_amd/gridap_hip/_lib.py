@@ -69,6 +69,7 @@ SIGNATURES = {
     "gh_fsai_build": (_i32, [_vp]),
     "gh_fsai_train_step": (_i32, [_vp, _i32, C.c_uint64, C.c_float, C.POINTER(_dbl)]),
     "gh_fsai_get": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp]),
+    "gh_fsai_factor_get": (_i32, [_vp, _vp, _vp, _vp, _vp]),
 }
 
 _lib = None
@@ -268,3 +269,14 @@ class Context:
         gr = np.zeros(self.mlp_nparams(), dtype=np.float32)
         self.check(lib().gh_fsai_get(self.h, ptr(feat), ptr(yo), ptr(gl), ptr(gu), ptr(gr)))
         return dict(feat=feat, yout=yo, gl=gl[:self.nnz], gu=gu[:self.nnz], grad=gr)
+
+    def fsai_factor_get(self):
+        """The solver's factor: rows of G and of Gᵀ, 16 slots each (weights, 0-based
+        column indices)."""
+        n = max(self.nfree, 1) * 16
+        gle, gue = np.zeros(n, dtype=np.float32), np.zeros(n, dtype=np.float32)
+        gli, gui = np.zeros(n, dtype=np.int32), np.zeros(n, dtype=np.int32)
+        self.check(lib().gh_fsai_factor_get(self.h, ptr(gle), ptr(gli), ptr(gue), ptr(gui)))
+        k = self.nfree * 16
+        return (gle[:k].reshape(-1, 16), gli[:k].reshape(-1, 16), gue[:k].reshape(-1, 16),
+                gui[:k].reshape(-1, 16))

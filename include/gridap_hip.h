@@ -166,8 +166,13 @@ int gh_fsai_build(gh_ctx *ctx);
  * loss receives the probe estimate before the step. */
 int gh_fsai_train_step(gh_ctx *ctx, int nprobes, uint64_t seed, float lr, double *loss);
 /* Diagnostic copies: features [C x 16], network output [C x 40], Ĝ in the column-prefix
- * and column-suffix layouts [nnz], last parameter gradient [nparams] (any may be NULL). */
+ * and column-suffix layouts [nnz] (written by gh_fsai_train_step), last parameter
+ * gradient [nparams] (any may be NULL). */
 int gh_fsai_get(gh_ctx *ctx, float *feat, float *yout, float *gl, float *gu, float *grad);
+/* The factor the solver applies, 16 slots per row [N x 16]: gle/gli = weights and
+ * 0-based column indices of row i of G = Ĝ D^(-1/2) (columns j <= i), gue/gui = row i
+ * of Gᵀ (entries G(j, i), j >= i); unused slots have weight 0 (any may be NULL). */
+int gh_fsai_factor_get(gh_ctx *ctx, float *gle, int32_t *gli, float *gue, int32_t *gui);
 
 #ifdef __cplusplus
 }

@@ -50,17 +50,20 @@ def gelu_d(x):
     return 0.5 * (1.0 + erf(x / math.sqrt(2.0))) + x * np.exp(-0.5 * x * x) / math.sqrt(2 * math.pi)
 
 
-def mlp_forward(params, X, bf16=True):
+def mlp_forward(params, X, bf16=True, act_bf16=False):
     """bf16=False evaluates the same network with unrounded weights (for finite
-    differences, which the rounding would make piecewise constant)."""
+    differences, which the rounding would make piecewise constant).  act_bf16=True
+    also rounds every layer input to BF16, as the matrix-core inference forward does
+    (its products are exact in FP32 and accumulated in FP32)."""
     P = unpack(bf16_round(params) if bf16 else params)
-    z1 = X @ P["W1"].T + P["b1"]
+    r = bf16_round if act_bf16 else (lambda a: a)
+    z1 = r(X) @ P["W1"].T + P["b1"]
     h1 = gelu(z1)
-    z2 = h1 @ P["W2"].T + P["b2"]
+    z2 = r(h1) @ P["W2"].T + P["b2"]
     h2 = gelu(z2)
-    z3 = h2 @ P["W3"].T + P["b3"]
+    z3 = r(h2) @ P["W3"].T + P["b3"]
     h3 = gelu(z3)
-    y = h3 @ P["W4"].T + P["b4"]
+    y = r(h3) @ P["W4"].T + P["b4"]
     return y, dict(X=X, z1=z1, h1=h1, z2=z2, h2=h2, z3=z3, h3=h3, P=P)
 
 

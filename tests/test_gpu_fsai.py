@@ -1,9 +1,12 @@
 """Parity of the learned-FSAI kernels with the FP64 oracle (needs an MI355X).
 
-Bar (the network runs with BF16 weights and FP32 activations/accumulation, the
-oracle in FP64 with the same BF16-rounded weights): features within 1e-6
-relative, network outputs and Ĝ within 1e-4 of max |value|, probe loss within
-1e-5 relative, weight gradient within 1e-3 relative L2 per layer.  Probes are the
+Bar (BF16 weights, FP32 accumulation; the oracle in FP64 with the same
+BF16-rounded weights): features within 1e-6 relative; training forward (FP32
+activations): probe loss within 1e-5 relative, weight gradient within 1e-3
+relative L2 per layer; inference forward (matrix cores, layer inputs rounded to
+BF16, mirrored by the oracle): network outputs and Ĝ within 1e-4 of max |value| for
+99.9% of the entries and within 1e-2 for all (an input that lands on a BF16
+rounding boundary may round the other way in FP32 than in FP64).  Probes are the
 same splitmix64 Rademacher vectors on both sides.
 """
 import sys
@@ -63,20 +66,16 @@ def case(oracle):
     return dict(prob=prob, ctx=ctx, kappa=kappa, colptr=colptr, rowval=rowval)
 
 
-def split_factor(c, gl, gu):
-    """Dense Ĝ from the device's column-prefix (gl) and column-suffix (gu) arrays."""
-    N = c["prob"].N
+def dense_factor(ctx, N):
+    """Dense G (from the rows of G) and Gᵀ (from the rows of Gᵀ) of the solver's copy."""
+    gle, gli, gue, gui = ctx.fsai_factor_get()
     lo = np.zeros((N, N))
-    up = np.zeros((N, N))
-    cp, rv = c["colptr"], c["rowval"]
-    for j in range(N):
-        for p in range(cp[j] - 1, cp[j + 1] - 1):
-            i = rv[p] - 1
-            if i <= j:
-                lo[j, i] = gl[p]          # row j of Ĝ, column i <= j
-            if i >= j:
-                up[i, j] = gu[p]          # Ĝ(i, j), i >= j
-    return lo, up
+    upT = np.zeros((N, N))
+    for i in range(N):
+        for k in range(16):
+            lo[i, gli[i, k]] += gle[i, k]
+            upT[i, gui[i, k]] += gue[i, k]
+    return lo, upT.T
 
 
 def test_mlp_init_matches_oracle(case):
@@ -98,13 +97,22 @@ def test_forward_matches_oracle(case):
     feat = got["feat"].astype(np.float64)
     np.testing.assert_allclose(feat[:, :10], X, rtol=1e-6, atol=1e-7)
     assert np.all(feat[:, 10:] == 0)
-    y, _ = F.mlp_forward(p, X)
+    # the oracle's input: the device's FP32 features (so BF16 rounding sees the same values)
+    y, _ = F.mlp_forward(p, feat[:, :10], act_bf16=True)
     yd = got["yout"][:, :36].astype(np.float64)
-    assert np.max(np.abs(yd - y)) <= 1e-4 * np.max(np.abs(y))
-    Gh = prob.ghat(y)
-    lo, up = split_factor(case, got["gl"], got["gu"])
-    assert np.max(np.abs(lo - Gh)) <= 1e-4 * np.max(np.abs(Gh))
-    assert np.max(np.abs(up - Gh)) <= 1e-4 * np.max(np.abs(Gh))
+
+    def close(a, b):
+        e = np.abs(a - b) / np.max(np.abs(b))
+        return np.quantile(e, 0.999) <= 1e-4 and e.max() <= 1e-2
+
+    assert close(yd, y)
+    S = 1.0 / np.sqrt(np.diag(prob.A))
+    Gref = prob.ghat(y) * S[None, :]                  # G = Ĝ D^-1/2
+    lo, up = dense_factor(ctx, prob.N)
+    mask = prob.lower
+    assert np.all(lo[~mask] == 0) and np.all(up[~mask] == 0)
+    assert close(lo[mask], Gref[mask])
+    np.testing.assert_array_equal(lo, up)             # G and Gᵀ copies are exact transposes
 
 
 def test_train_step_matches_oracle(case):
@@ -129,17 +137,18 @@ def test_factor_is_deterministic(case):
     ctx = case["ctx"]
     ctx.mlp_set(random_params(6))
     ctx.fsai_build()
-    a = ctx.fsai_get()
+    a = ctx.fsai_factor_get()
     l1 = ctx.fsai_train_step(2, 99, 0.0)
-    g1 = ctx.fsai_get()["grad"]
+    g1 = ctx.fsai_get()
     ctx.fsai_build()
-    b = ctx.fsai_get()
+    b = ctx.fsai_factor_get()
     l2 = ctx.fsai_train_step(2, 99, 0.0)
-    g2 = ctx.fsai_get()["grad"]
-    np.testing.assert_array_equal(a["gl"], b["gl"])
-    np.testing.assert_array_equal(a["gu"], b["gu"])
+    g2 = ctx.fsai_get()
+    for u, v in zip(a, b):
+        np.testing.assert_array_equal(u, v)
     assert l1 == l2
-    np.testing.assert_array_equal(g1, g2)
+    for k in ("grad", "gl", "gu", "yout"):
+        np.testing.assert_array_equal(g1[k], g2[k])
 
 
 def test_untrained_network_is_jacobi(case):
@@ -147,10 +156,10 @@ def test_untrained_network_is_jacobi(case):
     ctx, prob = case["ctx"], case["prob"]
     ctx.mlp_init(1)
     ctx.fsai_build()
-    got = ctx.fsai_get()
-    lo, up = split_factor(case, got["gl"], got["gu"])
-    np.testing.assert_array_equal(lo, np.eye(prob.N))
-    np.testing.assert_array_equal(up, np.eye(prob.N))
+    lo, up = dense_factor(ctx, prob.N)
+    S = (1.0 / np.sqrt(np.diag(prob.A))).astype(np.float32).astype(np.float64)
+    np.testing.assert_array_equal(lo, np.diag(S))     # Ĝ = I: G = D^-1/2 (in FP32)
+    np.testing.assert_array_equal(up, np.diag(S))
     xj, sj = ctx.pcg(precond=_lib.GH_PRECOND_JACOBI, rtol=1e-10)
     xf, sf = ctx.pcg(precond=_lib.GH_PRECOND_FSAI, rtol=1e-10)
     assert sf.precond_used == _lib.GH_PRECOND_FSAI
