@@ -41,6 +41,7 @@ void invalidate_pattern(gh::Ctx &c) {
   c.fsai_ready = false;
   c.fsai_built = false;
   c.simplex = false;
+  c.sref_ready = false;
 }
 
 }  // namespace
@@ -204,6 +205,7 @@ int gh_reffe_tabulation(gh_ctx *ctx, int nq, int nloc, const double *qw, const d
     c.shp.assign(shp, shp + size_t(nq) * nloc);
     c.grad.assign(grad, grad + size_t(nq) * nloc * c.dim);
     c.have_values = false;
+    c.sref_ready = false;
   });
 }
 
@@ -211,7 +213,7 @@ int gh_symbolic(gh_ctx *ctx, int64_t *nnz_out) {
   return guarded(ctx, [&](gh::Ctx &c) {
     GH_REQUIRE(c.have_space, GH_ERR_STATE, "upload the FE space before gh_symbolic");
     invalidate_pattern(c);
-    gh::symbolic_general(c);
+    gh::symbolic_general(c, gh::simplex_eligible(c));
     gh::check_structured(c);
     gh::simplex_symbolic(c);
     if (nnz_out) *nnz_out = c.nnz;

@@ -100,11 +100,15 @@ struct Ctx {
 
   // ---- affine simplices (P1/P2 tetrahedra) ----
   bool simplex = false;
-  int max_col = 0;                  // longest CSC column
   DBuf<int32_t> adjptr;             // nfree+1: cells of each free dof
   DBuf<int32_t> adj;                // (cell << 4 | local index), cells ascending per dof
-  DBuf<double> geo;                 // ncells x 8: κ|det| DF⁻¹DF⁻ᵀ (6), f|det|, pad
+  DBuf<int32_t> cstart;             // nnz+1: contribution range of each nonzero
+  DBuf<int32_t> perm;               // contributions: index into the per-cell symmetric entries
+  DBuf<uint16_t> dmask;             // per cell: bit b set when local dof b is Dirichlet
+  DBuf<unsigned char> elem;         // per cell: the nloc(nloc+1)/2 element entries (FP32/FP64)
+  DBuf<double> fdet;                // ncells: f |det DF|
   DBuf<double> sref;                // reference tensor Ĝ (6 x nloc x nloc) and m (nloc)
+  bool sref_ready = false;          // sref matches the uploaded tabulation
 
   // ---- colouring for the general path ----
   int ncolors = 0;
@@ -143,7 +147,9 @@ struct Ctx {
 };
 
 // pattern.hip
-void symbolic_general(Ctx &c);
+// contributions: also keep, per nonzero, the ascending list of the (cell, a, b)
+// element entries that sum into it (cstart, perm)
+void symbolic_general(Ctx &c, bool contributions);
 void check_structured(Ctx &c);
 void build_coloring(Ctx &c);
 // assemble.hip

@@ -54,6 +54,39 @@ __global__ void unique_flags(int64_t n, const uint64_t *__restrict__ keys, uint6
   if (f) atomicAdd(&colcnt[key >> rb], 1);
 }
 
+// Contribution ranges of the nonzeros (CSC order): cstart[pos[k]] = k for the first
+// entry of each nonzero; cstart[nnz] = number of entries with a free row and column.
+__global__ void contrib_ranges(int64_t n, const uint64_t *__restrict__ keys, uint64_t invalid,
+                               const int32_t *__restrict__ flags, const int32_t *__restrict__ pos,
+                               int64_t nnz, int32_t *__restrict__ cstart) {
+  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  if (flags[k]) cstart[pos[k]] = (int32_t)k;
+  if (keys[k] < invalid && (k + 1 == n || keys[k + 1] >= invalid)) cstart[nnz] = (int32_t)(k + 1);
+}
+
+// perm[k] = index of contribution k in the symmetric per-cell storage
+// (cell * nloc(nloc+1)/2 + packed (min, max) local pair).
+__global__ void contrib_perm(int64_t n, const uint64_t *__restrict__ keys, uint64_t invalid,
+                             const int32_t *__restrict__ vals, int nloc,
+                             int32_t *__restrict__ perm) {
+  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n || keys[k] >= invalid) return;
+  const int64_t t = vals[k];
+  const int64_t per = (int64_t)nloc * nloc;
+  const int64_t c = t / per;
+  const int r = (int)(t - c * per);
+  const int x = r / nloc, y = r - x * nloc;
+  const int lo = min(x, y), hi = max(x, y);
+  const int nsym = nloc * (nloc + 1) / 2;
+  perm[k] = (int32_t)(c * nsym + lo * nloc - lo * (lo - 1) / 2 + (hi - lo));
+}
+
+__global__ void iota_kernel(int64_t n, int32_t *__restrict__ v) {
+  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) v[k] = (int32_t)k;
+}
+
 __global__ void scatter_rows(int64_t n, const uint64_t *__restrict__ keys,
                              const int32_t *__restrict__ flags, const int32_t *__restrict__ pos,
                              uint64_t rowmask, int32_t *__restrict__ rowval) {
@@ -198,7 +231,7 @@ void exclusive_sum(const T *in, T *out, int64_t n, hipStream_t s) {
 
 }  // namespace
 
-void symbolic_general(Ctx &c) {
+void symbolic_general(Ctx &c, bool contributions) {
   hipStream_t s = c.stream;
   const int64_t per = (int64_t)c.nloc * c.nloc;
   const int64_t total = c.ncells * per;
@@ -209,13 +242,31 @@ void symbolic_general(Ctx &c) {
   const uint64_t invalid = (uint64_t)c.nfree << rb;
 
   DBuf<uint64_t> keys, sorted;
+  DBuf<int32_t> vals, svals;  // contribution ids (emit index), carried by a stable sort
   keys.resize(std::max<int64_t>(total, 1));
   sorted.resize(std::max<int64_t>(total, 1));
   if (total)
     hipLaunchKernelGGL(emit_pairs, dim3(grid_for(total)), dim3(kBlock), 0, s, c.ncells,
                        c.nloc, c.cell_dofs.get(), rb, invalid, keys.get());
   GH_CHECK_HIP(hipGetLastError());
-  {
+  if (contributions) {
+    vals.resize(std::max<int64_t>(total, 1));
+    svals.resize(std::max<int64_t>(total, 1));
+    if (total)
+      hipLaunchKernelGGL(iota_kernel, dim3(grid_for(total)), dim3(kBlock), 0, s, total,
+                         vals.get());
+    size_t bytes = 0;
+    GH_CHECK_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, keys.get(), sorted.get(),
+                                                    vals.get(), svals.get(), (int)total, 0,
+                                                    2 * rb, s));
+    DBuf<unsigned char> tmp;
+    tmp.resize(std::max<size_t>(bytes, 1));
+    GH_CHECK_HIP(hipcub::DeviceRadixSort::SortPairs(tmp.get(), bytes, keys.get(), sorted.get(),
+                                                    vals.get(), svals.get(), (int)total, 0,
+                                                    2 * rb, s));
+    GH_CHECK_HIP(hipStreamSynchronize(s));
+    vals.release();
+  } else {
     size_t bytes = 0;
     GH_CHECK_HIP(hipcub::DeviceRadixSort::SortKeys(nullptr, bytes, keys.get(), sorted.get(),
                                                    (int)total, 0, 2 * rb, s));
@@ -253,6 +304,20 @@ void symbolic_general(Ctx &c) {
                        sorted.get(), flags.get(), pos.get(), (uint64_t(1) << rb) - 1,
                        c.rowval.get());
   GH_CHECK_HIP(hipGetLastError());
+  c.cstart.release();
+  c.perm.release();
+  if (contributions) {
+    c.cstart.resize(nnz + 1);
+    GH_CHECK_HIP(hipMemsetAsync(c.cstart.get(), 0, sizeof(int32_t) * (nnz + 1), s));
+    c.perm.resize(std::max<int64_t>(total, 1));
+    if (total) {
+      hipLaunchKernelGGL(contrib_ranges, dim3(grid_for(total)), dim3(kBlock), 0, s, total,
+                         sorted.get(), invalid, flags.get(), pos.get(), nnz, c.cstart.get());
+      hipLaunchKernelGGL(contrib_perm, dim3(grid_for(total)), dim3(kBlock), 0, s, total,
+                         sorted.get(), invalid, svals.get(), c.nloc, c.perm.get());
+    }
+    GH_CHECK_HIP(hipGetLastError());
+  }
   GH_CHECK_HIP(hipStreamSynchronize(s));
   c.have_pattern = true;
   c.nzval.resize(std::max<int64_t>(nnz, 1));

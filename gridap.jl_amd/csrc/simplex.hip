@@ -12,12 +12,16 @@
 // (DF, its inverse and determinant) is FP64; the contraction is FP32 or FP64 by
 // precision mode; every sum into nzval and b is FP64.
 //
-// Each CSC column (free dof j) is owned by a group of 32 lanes.  The symbolic
-// phase lists the cells holding j in ascending cell order (the order of Gridap's
-// cell loop); for each, lane b computes K_ab (a = j's local index) and adds it
-// into the LDS slot of row dof_b, found by binary search in the column's rows.
-// The lanes of one step hit distinct slots and steps are ordered, so the sums are
-// deterministic; every nonzero is written to HBM once, with coalesced stores.
+// Three passes, each fully parallel and deterministic:
+//  * element: one thread per cell computes the metric (FP64); the block then writes
+//    the nloc(nloc+1)/2 distinct entries of every cell's symmetric element matrix
+//    with coalesced stores;
+//  * sum: one thread per nonzero adds its entries in ascending cell order (the
+//    order of Gridap's cell loop), from the contribution list that the symbolic
+//    phase keeps (a stable sort of the (column, row) keys carries the cell index),
+//    in FP64, and writes nzval once;
+//  * rhs: one thread per free dof walks its cells (dof -> cell adjacency, ascending)
+//    and adds f|det| m_a minus the lifting of the cell's Dirichlet dofs.
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
@@ -29,18 +33,9 @@ namespace gh {
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int kGroup = 32;                 // lanes per column
-constexpr int kGroups = kBlock / kGroup;   // columns per block
-constexpr int kGeoStride = 8;              // g[6], f|det|, pad
 
 inline unsigned grid_for(int64_t n, int block = kBlock) {
   return (unsigned)std::max<int64_t>(1, (n + block - 1) / block);
-}
-
-__device__ __forceinline__ int xcd_remap(int b, int nblocks) {
-  constexpr int kXcds = 8;
-  const int q = nblocks / kXcds, r = nblocks % kXcds, x = b % kXcds;
-  return x * q + min(x, r) + b / kXcds;
 }
 
 // ---- symbolic: dof -> (cell, local index), ascending cells ------------------------
@@ -74,125 +69,160 @@ __global__ void adjacency_fill(int64_t total, int64_t nfree, const uint64_t *__r
     for (int64_t x = d + 1; x <= nfree; ++x) adjptr[x] = (int32_t)total;
 }
 
-__global__ void max_column(int64_t n, const int32_t *__restrict__ colptr, int *__restrict__ out) {
-  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (j < n) atomicMax(out, colptr[j + 1] - colptr[j]);
+__global__ void dirichlet_mask_kernel(int64_t ncells, int nloc,
+                                      const int32_t *__restrict__ cell_dofs,
+                                      uint16_t *__restrict__ dmask) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncells) return;
+  uint32_t m = 0;
+  for (int a = 0; a < nloc; ++a)
+    if (cell_dofs[c * nloc + a] < 0) m |= 1u << a;
+  dmask[c] = (uint16_t)m;
 }
 
 // ---- numeric -----------------------------------------------------------------------
 
-__global__ void simplex_geom_kernel(int64_t ncells, const int32_t *__restrict__ cell_nodes,
-                                    const double *__restrict__ xyz,
-                                    const double *__restrict__ kappa,
-                                    const double *__restrict__ fsrc, double *__restrict__ geo) {
-  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= ncells) return;
-  const int32_t *cn = cell_nodes + c * 4;
-  double x[4][3];
-#pragma unroll
-  for (int v = 0; v < 4; ++v)
-#pragma unroll
-    for (int d = 0; d < 3; ++d) x[v][d] = xyz[(size_t)cn[v] * 3 + d];
-  double e[3][3];  // e[j] = column j of DF = x_{j+1} - x_0
-#pragma unroll
-  for (int j = 0; j < 3; ++j)
-#pragma unroll
-    for (int d = 0; d < 3; ++d) e[j][d] = x[j + 1][d] - x[0][d];
-  auto cross = [](const double *a, const double *b, double *o) {
-    o[0] = a[1] * b[2] - a[2] * b[1];
-    o[1] = a[2] * b[0] - a[0] * b[2];
-    o[2] = a[0] * b[1] - a[1] * b[0];
-  };
-  double r[3][3];  // rows of DF⁻¹ times det
-  cross(e[1], e[2], r[0]);
-  cross(e[2], e[0], r[1]);
-  cross(e[0], e[1], r[2]);
-  const double det = e[0][0] * r[0][0] + e[0][1] * r[0][1] + e[0][2] * r[0][2];
-  const double ad = fabs(det);
-  const double s = (kappa ? kappa[c] : 1.0) * ad / (det * det);
-  auto dot = [](const double *a, const double *b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; };
-  double *g = geo + c * kGeoStride;
-  g[0] = s * dot(r[0], r[0]);
-  g[1] = s * dot(r[1], r[1]);
-  g[2] = s * dot(r[2], r[2]);
-  g[3] = s * dot(r[0], r[1]);
-  g[4] = s * dot(r[0], r[2]);
-  g[5] = s * dot(r[1], r[2]);
-  g[6] = (fsrc ? fsrc[c] : 1.0) * ad;
-  g[7] = 0.0;
+__device__ __forceinline__ int sym_index(int nloc, int a, int b) {
+  const int lo = min(a, b), hi = max(a, b);
+  return lo * nloc - lo * (lo - 1) / 2 + (hi - lo);
 }
 
-struct GatherArgs {
-  int64_t nfree;
-  const int32_t *colptr, *rowval, *adjptr, *adj, *cell_dofs;
-  const double *geo, *gref, *mref, *dvals;
-  double *nzval, *b;
-};
-
-template <int NLOC, class T, int MAXNZ>
-__global__ __launch_bounds__(kBlock) void simplex_gather_kernel(GatherArgs A) {
-  __shared__ T ref[6 * NLOC * NLOC];
-  __shared__ double mref[NLOC];
-  __shared__ int rows[kGroups][MAXNZ];
-  __shared__ double acc[kGroups][MAXNZ];
-  for (int t = threadIdx.x; t < 6 * NLOC * NLOC; t += kBlock) ref[t] = (T)A.gref[t];
-  if (threadIdx.x < NLOC) mref[threadIdx.x] = A.mref[threadIdx.x];
-  const int g = threadIdx.x / kGroup, lane = threadIdx.x % kGroup;
-  const int64_t j = (int64_t)xcd_remap(blockIdx.x, gridDim.x) * kGroups + g;
-  const bool active = j < A.nfree;
-  int p0 = 0, n = 0;
-  if (active) {
-    p0 = A.colptr[j];
-    n = A.colptr[j + 1] - p0;  // 1 <= n <= MAXNZ (checked on the host)
-  }
-  for (int k = lane; k < n; k += kGroup) {
-    rows[g][k] = A.rowval[p0 + k];
-    acc[g][k] = 0.0;
-  }
-  __syncthreads();
-  double bl = 0.0;
-  if (active && lane < NLOC) {
-    const int q1 = A.adjptr[j + 1];
-    for (int q = A.adjptr[j]; q < q1; ++q) {
-      const int e = A.adj[q];
-      const int cell = e >> 4, a = e & 15;
-      const double *gc = A.geo + (size_t)cell * kGeoStride;
-      const int d = A.cell_dofs[(size_t)cell * NLOC + lane];
-      T kab = (T)0;
-#pragma unroll
-      for (int k = 0; k < 6; ++k) kab += (T)gc[k] * ref[(k * NLOC + a) * NLOC + lane];
-      if (d > 0) {
-        const int key = d - 1;
-        int lo = 0, hi = n - 1;
-        while (lo < hi) {
-          const int mid = (lo + hi) >> 1;
-          if (rows[g][mid] < key) lo = mid + 1;
-          else hi = mid;
-        }
-        atomicAdd(&acc[g][lo], (double)kab);
-      } else if (A.dvals) {
-        bl -= (double)kab * A.dvals[-d - 1];
-      }
-      if (lane == 0) bl += gc[6] * mref[a];
+// Element entries of kBlock cells: E[c][e] = Σ_k g_k(c) Ĝ^k_e, g = κ|det| DF⁻¹DF⁻ᵀ.
+template <int NLOC, class T>
+__global__ __launch_bounds__(kBlock) void simplex_element_kernel(
+    int64_t ncells, const int32_t *__restrict__ cell_nodes, const double *__restrict__ xyz,
+    const double *__restrict__ kappa, const double *__restrict__ fsrc,
+    const double *__restrict__ gref, T *__restrict__ E, double *__restrict__ fdet) {
+  constexpr int NS = NLOC * (NLOC + 1) / 2;
+  __shared__ T ref[6][NS];
+  __shared__ double sg[kBlock][6];
+  for (int t = threadIdx.x; t < 6 * NS; t += kBlock) {
+    const int k = t / NS, e = t - k * NS;
+    int a = 0, rem = e;
+    while (rem >= NLOC - a) {
+      rem -= NLOC - a;
+      ++a;
     }
+    ref[k][e] = (T)gref[(k * NLOC + a) * NLOC + a + rem];
   }
-  for (int o = kGroup / 2; o > 0; o >>= 1) bl += __shfl_xor(bl, o);
+  const int64_t c0 = (int64_t)blockIdx.x * kBlock;
+  const int64_t c = c0 + threadIdx.x;
+  if (c < ncells) {
+    const int32_t *cn = cell_nodes + c * 4;
+    double x[4][3];
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) x[v][d] = xyz[(size_t)cn[v] * 3 + d];
+    double e[3][3];  // e[j] = column j of DF = x_{j+1} - x_0
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) e[j][d] = x[j + 1][d] - x[0][d];
+    auto cross = [](const double *u, const double *w, double *o) {
+      o[0] = u[1] * w[2] - u[2] * w[1];
+      o[1] = u[2] * w[0] - u[0] * w[2];
+      o[2] = u[0] * w[1] - u[1] * w[0];
+    };
+    double r[3][3];  // rows of DF⁻¹ times det
+    cross(e[1], e[2], r[0]);
+    cross(e[2], e[0], r[1]);
+    cross(e[0], e[1], r[2]);
+    const double det = e[0][0] * r[0][0] + e[0][1] * r[0][1] + e[0][2] * r[0][2];
+    const double ad = fabs(det);
+    const double s = (kappa ? kappa[c] : 1.0) * ad / (det * det);
+    auto dot = [](const double *u, const double *w) {
+      return u[0] * w[0] + u[1] * w[1] + u[2] * w[2];
+    };
+    sg[threadIdx.x][0] = s * dot(r[0], r[0]);
+    sg[threadIdx.x][1] = s * dot(r[1], r[1]);
+    sg[threadIdx.x][2] = s * dot(r[2], r[2]);
+    sg[threadIdx.x][3] = s * dot(r[0], r[1]);
+    sg[threadIdx.x][4] = s * dot(r[0], r[2]);
+    sg[threadIdx.x][5] = s * dot(r[1], r[2]);
+    fdet[c] = (fsrc ? fsrc[c] : 1.0) * ad;
+  }
   __syncthreads();
-  for (int k = lane; k < n; k += kGroup) A.nzval[p0 + k] = acc[g][k];
-  if (active && lane == 0) A.b[j] = bl;
+  const int64_t ncb = min<int64_t>(kBlock, ncells - c0);
+  T *out = E + c0 * NS;
+  for (int idx = threadIdx.x; idx < ncb * NS; idx += kBlock) {
+    const int cl = idx / NS, ee = idx - cl * NS;
+    T v = (T)0;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) v += (T)sg[cl][k] * ref[k][ee];
+    out[idx] = v;
+  }
+}
+
+template <class T>
+__global__ void simplex_sum_kernel(int64_t nnz, const int32_t *__restrict__ cstart,
+                                   const int32_t *__restrict__ perm, const T *__restrict__ E,
+                                   double *__restrict__ nzval) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= nnz) return;
+  // batches of 8: all index loads, then all value loads, then the ordered sum
+  constexpr int U = 8;
+  double s = 0.0;
+  const int32_t e = cstart[p + 1];
+  for (int32_t i0 = cstart[p]; i0 < e; i0 += U) {
+    int32_t id[U];
+    T v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) id[u] = i0 + u < e ? perm[i0 + u] : -1;
+#pragma unroll
+    for (int u = 0; u < U; ++u) v[u] = id[u] >= 0 ? E[id[u]] : (T)0;
+#pragma unroll
+    for (int u = 0; u < U; ++u) s += (double)v[u];
+  }
+  nzval[p] = s;
 }
 
 template <int NLOC, class T>
-void launch_gather(Ctx &c, const GatherArgs &A, unsigned grid) {
-  if (c.max_col <= 64)
-    hipLaunchKernelGGL((simplex_gather_kernel<NLOC, T, 64>), dim3(grid), dim3(kBlock), 0,
-                       c.stream, A);
-  else if (c.max_col <= 128)
-    hipLaunchKernelGGL((simplex_gather_kernel<NLOC, T, 128>), dim3(grid), dim3(kBlock), 0,
-                       c.stream, A);
-  else
-    hipLaunchKernelGGL((simplex_gather_kernel<NLOC, T, 256>), dim3(grid), dim3(kBlock), 0,
-                       c.stream, A);
+__global__ void simplex_rhs_kernel(int64_t nfree, const int32_t *__restrict__ adjptr,
+                                   const int32_t *__restrict__ adj,
+                                   const double *__restrict__ fdet,
+                                   const double *__restrict__ mref,
+                                   const uint16_t *__restrict__ dmask,
+                                   const int32_t *__restrict__ cell_dofs,
+                                   const T *__restrict__ E, const double *__restrict__ dvals,
+                                   double *__restrict__ b) {
+  constexpr int NS = NLOC * (NLOC + 1) / 2;
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nfree) return;
+  double s = 0.0;
+  for (int q = adjptr[j]; q < adjptr[j + 1]; ++q) {
+    const int e = adj[q];
+    const int cell = e >> 4, a = e & 15;
+    double v = fdet[cell] * mref[a];
+    uint32_t m = dvals ? dmask[cell] : 0u;
+    while (m) {
+      const int bb = __ffs(m) - 1;
+      m &= m - 1;
+      const int d = cell_dofs[(size_t)cell * NLOC + bb];
+      v -= (double)E[(size_t)cell * NS + sym_index(NLOC, a, bb)] * dvals[-d - 1];
+    }
+    s += v;
+  }
+  b[j] = s;
+}
+
+template <int NLOC, class T>
+void launch_simplex(Ctx &c) {
+  constexpr int NS = NLOC * (NLOC + 1) / 2;
+  c.elem.resize((size_t)c.ncells * NS * sizeof(T));
+  T *E = reinterpret_cast<T *>(c.elem.get());
+  hipLaunchKernelGGL((simplex_element_kernel<NLOC, T>), dim3(grid_for(c.ncells)), dim3(kBlock),
+                     0, c.stream, c.ncells, c.cell_nodes.get(), c.xyz.get(),
+                     c.have_kappa ? c.kappa.get() : nullptr,
+                     c.have_fsrc ? c.fsrc.get() : nullptr, c.sref.get(), E, c.fdet.get());
+  if (c.nnz)
+    hipLaunchKernelGGL((simplex_sum_kernel<T>), dim3(grid_for(c.nnz)), dim3(kBlock), 0, c.stream,
+                       c.nnz, c.cstart.get(), c.perm.get(), (const T *)E, c.nzval.get());
+  hipLaunchKernelGGL((simplex_rhs_kernel<NLOC, T>), dim3(grid_for(c.nfree)), dim3(kBlock), 0,
+                     c.stream, c.nfree, c.adjptr.get(), c.adj.get(), c.fdet.get(),
+                     c.sref.get() + 6 * NLOC * NLOC, c.dmask.get(), c.cell_dofs.get(),
+                     (const T *)E, c.have_dvals ? c.dvals.get() : nullptr, c.b.get());
+  GH_CHECK_HIP(hipGetLastError());
 }
 
 }  // namespace
@@ -206,6 +236,8 @@ void simplex_symbolic(Ctx &c) {
   c.simplex = false;
   c.adjptr.release();
   c.adj.release();
+  c.dmask.release();
+  c.elem.release();
   if (!simplex_eligible(c) || c.nfree == 0) return;
   hipStream_t s = c.stream;
   const int64_t total = c.ncells * c.nloc;
@@ -235,18 +267,12 @@ void simplex_symbolic(Ctx &c) {
   c.adj.resize(total);
   hipLaunchKernelGGL(adjacency_fill, dim3(grid_for(total)), dim3(kBlock), 0, s, total, c.nfree,
                      sorted.get(), c.adjptr.get(), c.adj.get());
-  DBuf<int> mx;
-  mx.resize(1);
-  GH_CHECK_HIP(hipMemsetAsync(mx.get(), 0, sizeof(int), s));
-  hipLaunchKernelGGL(max_column, dim3(grid_for(c.nfree)), dim3(kBlock), 0, s, c.nfree,
-                     c.colptr.get(), mx.get());
+  c.dmask.resize(c.ncells);
+  hipLaunchKernelGGL(dirichlet_mask_kernel, dim3(grid_for(c.ncells)), dim3(kBlock), 0, s,
+                     c.ncells, c.nloc, c.cell_dofs.get(), c.dmask.get());
   GH_CHECK_HIP(hipGetLastError());
-  int m = 0;
-  GH_CHECK_HIP(hipMemcpyAsync(&m, mx.get(), sizeof(int), hipMemcpyDeviceToHost, s));
   GH_CHECK_HIP(hipStreamSynchronize(s));
-  c.max_col = m;
-  GH_REQUIRE(m <= 256, GH_ERR_UNSUPPORTED, "a column holds more than 256 entries");
-  c.geo.resize((size_t)c.ncells * kGeoStride);
+  c.fdet.resize(c.ncells);
   c.simplex = true;
 }
 
@@ -256,56 +282,37 @@ void assemble_simplex(Ctx &c, int precision) {
   GH_REQUIRE((int)c.qw.size() == c.nq && (int)c.shp.size() == c.nq * NL &&
                  (int)c.grad.size() == c.nq * NL * 3,
              GH_ERR_STATE, "tabulation does not match the element (dofs per cell)");
-  // reference tensor Ĝ^k and m from the uploaded tabulation (FP64)
-  static const int pd[6] = {0, 1, 2, 0, 0, 1}, pe[6] = {0, 1, 2, 1, 2, 2};
-  std::vector<double> gref(6 * NL * NL, 0.0), mref(NL, 0.0);
-  for (int q = 0; q < c.nq; ++q) {
-    const double w = c.qw[q];
-    const double *G = &c.grad[(size_t)q * NL * 3];
-    for (int k = 0; k < 6; ++k)
-      for (int a = 0; a < NL; ++a)
-        for (int b = 0; b < NL; ++b) {
-          double v = G[a * 3 + pd[k]] * G[b * 3 + pe[k]];
-          if (pd[k] != pe[k]) v += G[a * 3 + pe[k]] * G[b * 3 + pd[k]];
-          gref[(k * NL + a) * NL + b] += w * v;
-        }
-    for (int a = 0; a < NL; ++a) mref[a] += w * c.shp[(size_t)q * NL + a];
+  // reference tensor Ĝ^k and m from the uploaded tabulation (FP64), once per tabulation
+  const size_t ng = (size_t)6 * NL * NL;
+  if (!c.sref_ready) {
+    static const int pd[6] = {0, 1, 2, 0, 0, 1}, pe[6] = {0, 1, 2, 1, 2, 2};
+    std::vector<double> ref(ng + NL, 0.0);
+    for (int q = 0; q < c.nq; ++q) {
+      const double w = c.qw[q];
+      const double *G = &c.grad[(size_t)q * NL * 3];
+      for (int k = 0; k < 6; ++k)
+        for (int a = 0; a < NL; ++a)
+          for (int b = 0; b < NL; ++b) {
+            double v = G[a * 3 + pd[k]] * G[b * 3 + pe[k]];
+            if (pd[k] != pe[k]) v += G[a * 3 + pe[k]] * G[b * 3 + pd[k]];
+            ref[(k * NL + a) * NL + b] += w * v;
+          }
+      for (int a = 0; a < NL; ++a) ref[ng + a] += w * c.shp[(size_t)q * NL + a];
+    }
+    c.sref.resize(ref.size());
+    GH_CHECK_HIP(hipMemcpy(c.sref.get(), ref.data(), sizeof(double) * ref.size(),
+                           hipMemcpyHostToDevice));
+    c.sref_ready = true;
   }
-  c.sref.resize(gref.size() + mref.size());
-  GH_CHECK_HIP(hipMemcpyAsync(c.sref.get(), gref.data(), sizeof(double) * gref.size(),
-                              hipMemcpyHostToDevice, c.stream));
-  GH_CHECK_HIP(hipMemcpyAsync(c.sref.get() + gref.size(), mref.data(),
-                              sizeof(double) * mref.size(), hipMemcpyHostToDevice, c.stream));
-  hipLaunchKernelGGL(simplex_geom_kernel, dim3(grid_for(c.ncells)), dim3(kBlock), 0, c.stream,
-                     c.ncells, c.cell_nodes.get(), c.xyz.get(),
-                     c.have_kappa ? c.kappa.get() : nullptr, c.have_fsrc ? c.fsrc.get() : nullptr,
-                     c.geo.get());
-  GH_CHECK_HIP(hipGetLastError());
-  GatherArgs A{};
-  A.nfree = c.nfree;
-  A.colptr = c.colptr.get();
-  A.rowval = c.rowval.get();
-  A.adjptr = c.adjptr.get();
-  A.adj = c.adj.get();
-  A.cell_dofs = c.cell_dofs.get();
-  A.geo = c.geo.get();
-  A.gref = c.sref.get();
-  A.mref = c.sref.get() + gref.size();
-  A.dvals = c.have_dvals ? c.dvals.get() : nullptr;
-  A.nzval = c.nzval.get();
-  A.b = c.b.get();
-  const unsigned grid = (unsigned)((c.nfree + kGroups - 1) / kGroups);
   const bool f32 = precision == GH_PREC_FP32_ELEM;
   if (NL == 10) {
-    if (f32) launch_gather<10, float>(c, A, grid);
-    else launch_gather<10, double>(c, A, grid);
+    if (f32) launch_simplex<10, float>(c);
+    else launch_simplex<10, double>(c);
   } else {
-    if (f32) launch_gather<4, float>(c, A, grid);
-    else launch_gather<4, double>(c, A, grid);
+    if (f32) launch_simplex<4, float>(c);
+    else launch_simplex<4, double>(c);
   }
-  GH_CHECK_HIP(hipGetLastError());
-  // the host vectors above are read by the copies on the stream: wait for them
-  GH_CHECK_HIP(hipStreamSynchronize(c.stream));
+  // enqueued only: gh_values_get (or a wait on gh_stream) orders the results
 }
 
 }  // namespace gh

@@ -222,3 +222,27 @@ def test_grf_kappa_matches_formula(case):
                    (c // (PART[0] * PART[1]) + 0.5) * h[2]], axis=1)
     want = amp * np.cos(xc @ omega.T + phase).sum(axis=1)
     np.testing.assert_allclose(logk, want, rtol=1e-6, atol=1e-6)
+
+
+def test_trained_network_beats_jacobi_on_heldout():
+    # The committed weights (python -m gridap_hip.fsai: 512 training instances) against
+    # the same network untrained (zero last layer == Jacobi) on held-out instances of
+    # the 64^3 κ = exp(GRF) problem: fewer PCG iterations to 1e-8, same solution.
+    announce("fsai: committed weights vs Jacobi on held-out 64^3 instances")
+    from gridap_hip import fsai as FS
+    inst = FS.PoissonInstances(64)
+    net = FS.LearnedFSAI.load(FS.DEFAULT_WEIGHTS)
+    for seed in list(FS.HELDOUT_SEEDS)[:2]:
+        inst.load(seed)
+        xj, sj = inst.solve("jacobi")
+        net.attach(inst.ctx)
+        inst.ctx.fsai_build()
+        xf, sf = inst.solve("fsai")
+        assert sf.converged == 1 and sf.precond_used == _lib.GH_PRECOND_FSAI
+        assert sf.iterations <= 0.7 * sj.iterations, (sf.iterations, sj.iterations)
+        assert np.max(np.abs(xf - xj)) <= 1e-6 * np.max(np.abs(xj))
+        inst.ctx.mlp_init(0)                          # untrained: exactly Jacobi
+        inst.ctx.fsai_build()
+        _, su = inst.solve("fsai")
+        assert abs(su.iterations - sj.iterations) <= 8
+    inst.ctx.close()
