@@ -531,7 +531,7 @@ void assemble_diffusion(Ctx &c, int precision, int path) {
              "kappa was uploaded for a different mesh");
   GH_REQUIRE(!c.have_dvals || c.dvals.n == (size_t)c.ndiri, GH_ERR_STATE,
              "Dirichlet values were uploaded for a different space");
-  if (c.simplex) {
+  if (c.elemlist && simplex_eligible(c)) {
     GH_REQUIRE(path != GH_PATH_STRUCTURED, GH_ERR_UNSUPPORTED,
                "the lattice path needs a Cartesian mesh, a scalar Q1 CLagrangian space and a "
                "symmetric quadrature rule");

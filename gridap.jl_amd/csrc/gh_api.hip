@@ -40,7 +40,7 @@ void invalidate_pattern(gh::Ctx &c) {
   c.color_ptr.clear();
   c.fsai_ready = false;
   c.fsai_built = false;
-  c.simplex = false;
+  c.elemlist = false;
   c.sref_ready = false;
 }
 
@@ -213,9 +213,9 @@ int gh_symbolic(gh_ctx *ctx, int64_t *nnz_out) {
   return guarded(ctx, [&](gh::Ctx &c) {
     GH_REQUIRE(c.have_space, GH_ERR_STATE, "upload the FE space before gh_symbolic");
     invalidate_pattern(c);
-    gh::symbolic_general(c, gh::simplex_eligible(c));
+    gh::symbolic_general(c, gh::simplex_eligible(c) || gh::hexq2_eligible(c));
     gh::check_structured(c);
-    gh::simplex_symbolic(c);
+    gh::elemlist_symbolic(c);
     if (nnz_out) *nnz_out = c.nnz;
   });
 }
@@ -257,16 +257,34 @@ int gh_coef_upload(gh_ctx *ctx, const double *kappa, const double *fsrc,
 
 int gh_assemble(gh_ctx *ctx, int form_kind, int precision, int path) {
   return guarded(ctx, [&](gh::Ctx &c) {
-    GH_REQUIRE(form_kind == GH_FORM_DIFFUSION, GH_ERR_UNSUPPORTED, "unknown form kind");
+    GH_REQUIRE(form_kind == GH_FORM_DIFFUSION || form_kind == GH_FORM_ELASTICITY,
+               GH_ERR_UNSUPPORTED, "unknown form kind");
     GH_REQUIRE(precision == GH_PREC_FP64 || precision == GH_PREC_FP32_ELEM, GH_ERR_ARG,
                "unknown precision mode");
     GH_REQUIRE(path == GH_PATH_AUTO || path == GH_PATH_GENERAL || path == GH_PATH_STRUCTURED,
                GH_ERR_ARG, "unknown numeric path");
     GH_REQUIRE(c.have_pattern, GH_ERR_STATE, "no pattern: call gh_symbolic first");
     GH_REQUIRE(c.nq > 0, GH_ERR_STATE, "no reference tabulation uploaded");
-    gh::assemble_diffusion(c, precision, path);
+    if (form_kind == GH_FORM_ELASTICITY) {
+      GH_REQUIRE(path != GH_PATH_STRUCTURED, GH_ERR_UNSUPPORTED,
+                 "the lattice path assembles scalar diffusion only");
+      gh::assemble_elasticity(c, precision);
+    } else {
+      GH_REQUIRE(!gh::hexq2_eligible(c), GH_ERR_UNSUPPORTED,
+                 "vector Q2 spaces assemble the elasticity form");
+      gh::assemble_diffusion(c, precision, path);
+    }
     c.have_values = true;
     c.fsai_built = false;
+  });
+}
+
+int gh_elasticity_params(gh_ctx *ctx, double nu, const double *body_force) {
+  return guarded(ctx, [&](gh::Ctx &c) {
+    GH_REQUIRE(nu > -1.0 && nu < 0.5, GH_ERR_ARG, "Poisson ratio must lie in (-1, 0.5)");
+    c.nu = nu;
+    for (int d = 0; d < 3; ++d) c.body[d] = body_force && d < c.dim ? body_force[d] : 0.0;
+    c.have_values = false;
   });
 }
 

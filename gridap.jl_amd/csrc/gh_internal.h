@@ -98,15 +98,18 @@ struct Ctx {
   DBuf<int32_t> tile_p0;            // per 64-node tile: nzval start of its columns, or -1
   DBuf<int32_t> tile_d0;            // per 64-node tile: dof of its first free node, or -1
 
-  // ---- affine simplices (P1/P2 tetrahedra) ----
-  bool simplex = false;
+  // ---- element-list path (P1/P2 tetrahedra, vector Q2 hexahedra) ----
+  bool elemlist = false;            // adjacency, contribution lists and masks ready
   DBuf<int32_t> adjptr;             // nfree+1: cells of each free dof
   DBuf<int32_t> adj;                // (cell << 4 | local index), cells ascending per dof
   DBuf<int32_t> cstart;             // nnz+1: contribution range of each nonzero
   DBuf<int32_t> perm;               // contributions: index into the per-cell symmetric entries
   DBuf<uint16_t> dmask;             // per cell: bit b set when local dof b is Dirichlet
   DBuf<unsigned char> elem;         // per cell: the nloc(nloc+1)/2 element entries (FP32/FP64)
-  DBuf<double> fdet;                // ncells: f |det DF|
+  DBuf<double> fdet;                // simplices: f |det DF| per cell; hexes: Σ w|J| φ_a (C x 27)
+  DBuf<int16_t> hpairs, hsym;       // vector Q2: (p, q) of each symmetric entry; its index
+  double nu = 0.3;                  // elasticity: Poisson ratio
+  double body[3] = {0, 0, 0};       // elasticity: body force
   DBuf<double> sref;                // reference tensor Ĝ (6 x nloc x nloc) and m (nloc)
   bool sref_ready = false;          // sref matches the uploaded tabulation
 
@@ -156,8 +159,12 @@ void build_coloring(Ctx &c);
 void assemble_diffusion(Ctx &c, int precision, int path);
 // simplex.hip
 bool simplex_eligible(const Ctx &c);
-void simplex_symbolic(Ctx &c);
+void elemlist_symbolic(Ctx &c);         // dof -> cell adjacency, Dirichlet masks
+void elemlist_sum(Ctx &c, bool f32);     // nzval from the element entries
 void assemble_simplex(Ctx &c, int precision);
+// hexq2.hip
+bool hexq2_eligible(const Ctx &c);
+void assemble_elasticity(Ctx &c, int precision);
 // solve.hip
 void spmv_device(Ctx &c, const double *x, double *y);
 void pcg_device(Ctx &c, int precond, double rtol, int maxit, gh_solve_stats *st);

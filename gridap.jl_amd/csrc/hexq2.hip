@@ -1,0 +1,251 @@
+// Linear elasticity with vector Q2 on hexahedra: the cell loop of Gridap's
+// assemble_matrix_and_vector (src/FESpaces/SparseMatrixAssemblers.jl:214-248) for
+//     a(u,v) = ∫ ε(v) : σ(ε(u)),  σ = λ tr(ε) I + 2 μ ε,  λ, μ from E_c and ν,
+//     l(v)   = ∫ v · f  (f constant),
+// with Gridap's component-blocked local dofs (node + 27 comp).  For isotropic σ the
+// 81x81 element matrix is built from six 27x27 gradient products
+//     P_ij[a][b] = Σ_q w_q |J_q| g_ai(q) g_bj(q),   g = ∂φ/∂x = J⁻ᵀ ∇̂φ,
+//     K[(a,i),(b,j)] = E_c (λ̂ P_ij[a][b] + μ̂ P_ji[a][b] + μ̂ δ_ij Σ_k P_kk[a][b]),
+// λ̂ = ν/((1+ν)(1-2ν)), μ̂ = 1/(2(1+ν)).  One wave per cell: the Jacobian and its
+// inverse at the 27 quadrature points are FP64 (the CartesianMap; the kernel takes
+// the per-point Jacobian, so a trilinear geometry is a change of that block); the products
+// run on the matrix cores (v_mfma_f32_32x32x2f32, FP32) or in FP64 on the vector
+// units; the 3321 distinct entries of the symmetric matrix are written with
+// coalesced stores, and the sum/rhs passes of the element-list path assemble them.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "gh_internal.h"
+
+namespace gh {
+namespace {
+
+constexpr int kNn = 27;                      // Q2 nodes
+constexpr int kNq = 27;                      // degree-4 tensor rule
+constexpr int kNloc = 3 * kNn;               // 81
+constexpr int kNsym = kNloc * (kNloc + 1) / 2;  // 3321
+
+typedef float v16f __attribute__((ext_vector_type(16)));
+
+struct HexArgs {
+  int64_t ncells;
+  double h[3];
+  const double *E;            // per cell (NULL: 1)
+  const double *ref;          // ĝ[q][a][d] (27*27*3), w[q] (27), φ̂[q][a] (27*27)
+  const int16_t *pairs;       // kNsym x 2: (p, q) local dofs, p <= q
+  double lam, mu;             // λ̂, μ̂
+  void *E_out;                // ncells x kNsym (T)
+  double *mvec;               // ncells x 27: Σ_q w |J| φ_a
+};
+
+template <class T>
+__global__ __launch_bounds__(64) void hex_elasticity_element_kernel(HexArgs A) {
+  __shared__ T g[kNq][kNn][3];   // physical gradients
+  __shared__ double wj[kNq];     // w_q |J_q|
+  __shared__ T P[6][kNn][kNn + 1];
+  const int lane = threadIdx.x;
+  const int64_t cell = blockIdx.x;
+  const double *rg = A.ref, *rw = A.ref + kNq * kNn * 3, *rs = rw + kNq;
+  // geometry: one lane per quadrature point
+  if (lane < kNq) {
+    const int q = lane;
+    double J[3][3];  // CartesianMap: J = diag(h) (src/Geometry/CartesianGrids.jl:259-269)
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+#pragma unroll
+      for (int k = 0; k < 3; ++k) J[d][k] = d == k ? A.h[d] : 0.0;
+    const double det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) -
+                       J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+                       J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+    double inv[3][3];  // J⁻¹
+    inv[0][0] = (J[1][1] * J[2][2] - J[1][2] * J[2][1]) / det;
+    inv[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) / det;
+    inv[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) / det;
+    inv[1][0] = (J[1][2] * J[2][0] - J[1][0] * J[2][2]) / det;
+    inv[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) / det;
+    inv[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) / det;
+    inv[2][0] = (J[1][0] * J[2][1] - J[1][1] * J[2][0]) / det;
+    inv[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) / det;
+    inv[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) / det;
+    wj[q] = rw[q] * fabs(det);
+    for (int a = 0; a < kNn; ++a) {
+      const double *gh = rg + (q * kNn + a) * 3;
+#pragma unroll
+      for (int k = 0; k < 3; ++k)  // g_k = Σ_d (J⁻¹)_dk ĝ_d
+        g[q][a][k] = (T)(inv[0][k] * gh[0] + inv[1][k] * gh[1] + inv[2][k] * gh[2]);
+    }
+  }
+  __syncthreads();
+  // m_a = Σ_q w |J| φ̂_a (FP64)
+  if (lane < kNn) {
+    double m = 0.0;
+    for (int q = 0; q < kNq; ++q) m += wj[q] * rs[q * kNn + lane];
+    A.mvec[cell * kNn + lane] = m;
+  }
+  static const int pi[6] = {0, 1, 2, 0, 0, 1}, pj[6] = {0, 1, 2, 1, 2, 2};
+  if constexpr (sizeof(T) == 4) {
+    // P_ij = Aw_i G_jᵀ on the matrix cores: A[row a][k q] = g[q][a][i] w|J|_q,
+    // B[k q][col b] = g[q][b][j]; lane l: row/col l % 32, k = 2 s + l / 32
+    const int r = lane & 31, kh = lane >> 5;
+#pragma unroll
+    for (int pp = 0; pp < 6; ++pp) {
+      const int i = pi[pp], j = pj[pp];
+      v16f acc = {0};
+#pragma unroll
+      for (int s = 0; s < (kNq + 1) / 2; ++s) {
+        const int q = 2 * s + kh;
+        const bool ok = q < kNq && r < kNn;
+        const float av = ok ? (float)g[q][r][i] * (float)wj[q] : 0.0f;
+        const float bv = ok ? (float)g[q][r][j] : 0.0f;
+        acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av, bv, acc, 0, 0, 0);
+      }
+#pragma unroll
+      for (int t = 0; t < 16; ++t) {
+        const int row = (t / 4) * 8 + kh * 4 + (t % 4);
+        if (row < kNn && r < kNn) P[pp][row][r] = (T)acc[t];
+      }
+    }
+  } else {
+    for (int e = lane; e < 6 * kNn * kNn; e += 64) {
+      const int pp = e / (kNn * kNn), ab = e % (kNn * kNn), a = ab / kNn, b = ab % kNn;
+      const int i = pi[pp], j = pj[pp];
+      double s = 0.0;
+      for (int q = 0; q < kNq; ++q) s += wj[q] * (double)g[q][a][i] * (double)g[q][b][j];
+      P[pp][a][b] = (T)s;
+    }
+  }
+  __syncthreads();
+  // symmetric entries of K (p <= q local dofs)
+  const double Ec = A.E ? A.E[cell] : 1.0;
+  const T lam = (T)(Ec * A.lam), mu = (T)(Ec * A.mu);
+  T *out = reinterpret_cast<T *>(A.E_out) + cell * (int64_t)kNsym;
+  auto pidx = [](int i, int j) {  // slot of the product P_ij with i <= j
+    return i == j ? i : (i == 0 ? (j == 1 ? 3 : 4) : 5);
+  };
+  for (int e = lane; e < kNsym; e += 64) {
+    const int p = A.pairs[2 * e], qd = A.pairs[2 * e + 1];
+    const int a = p % kNn, i = p / kNn, b = qd % kNn, j = qd / kNn;
+    // P_ij[a][b] and P_ji[a][b] = P_ij[b][a]; stored products have i <= j
+    T pij, pji;
+    if (i <= j) {
+      pij = P[pidx(i, j)][a][b];
+      pji = P[pidx(i, j)][b][a];
+    } else {
+      pij = P[pidx(j, i)][b][a];
+      pji = P[pidx(j, i)][a][b];
+    }
+    T v = lam * pij + mu * pji;
+    if (i == j) v += mu * (P[0][a][b] + P[1][a][b] + P[2][a][b]);
+    out[e] = v;
+  }
+}
+
+template <class T>
+__global__ void hex_rhs_kernel(int64_t nfree, const int32_t *__restrict__ adjptr,
+                               const int32_t *__restrict__ adj, const double *__restrict__ mvec,
+                               double f0, double f1, double f2,
+                               const uint16_t *__restrict__ dflag,
+                               const int32_t *__restrict__ cell_dofs,
+                               const int16_t *__restrict__ symidx, const T *__restrict__ E,
+                               const double *__restrict__ dvals, double *__restrict__ b) {
+  const int64_t jd = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (jd >= nfree) return;
+  const double f[3] = {f0, f1, f2};
+  double s = 0.0;
+  for (int q = adjptr[jd]; q < adjptr[jd + 1]; ++q) {
+    const int e = adj[q];
+    const int cell = e >> 7, a = e & 127;
+    double v = f[a / kNn] * mvec[(size_t)cell * kNn + a % kNn];
+    if (dvals && dflag[cell]) {
+      for (int bb = 0; bb < kNloc; ++bb) {
+        const int d = cell_dofs[(size_t)cell * kNloc + bb];
+        if (d < 0) {
+          const int lo = min(a, bb), hi = max(a, bb);
+          v -= (double)E[(size_t)cell * kNsym + symidx[lo * kNloc + hi]] * dvals[-d - 1];
+        }
+      }
+    }
+    s += v;
+  }
+  b[jd] = s;
+}
+
+}  // namespace
+
+bool hexq2_eligible(const Ctx &c) {
+  return c.dim == 3 && c.reffe_kind == GH_REFFE_QK && c.order == 2 && c.ncomp == 3 &&
+         c.nloc == kNloc && c.cartesian;
+}
+
+void assemble_elasticity(Ctx &c, int precision) {
+  GH_REQUIRE(hexq2_eligible(c) && c.elemlist, GH_ERR_UNSUPPORTED,
+             "elasticity is provided for vector Q2 on Cartesian hexahedra");
+  GH_REQUIRE(c.nq == kNq && (int)c.shp.size() == kNq * kNn &&
+                 (int)c.grad.size() == kNq * kNn * 3,
+             GH_ERR_STATE, "the elasticity path needs the scalar Q2 tabulation at the 27-point rule");
+  if (!c.sref_ready) {
+    // ĝ[q][a][d], w[q], φ̂[q][a] of the uploaded (scalar Q2) tabulation
+    std::vector<double> ref((size_t)kNq * kNn * 3 + kNq + (size_t)kNq * kNn, 0.0);
+    std::copy(c.grad.begin(), c.grad.end(), ref.begin());
+    std::copy(c.qw.begin(), c.qw.end(), ref.begin() + kNq * kNn * 3);
+    std::copy(c.shp.begin(), c.shp.end(), ref.begin() + kNq * kNn * 3 + kNq);
+    c.sref.resize(ref.size());
+    GH_CHECK_HIP(hipMemcpy(c.sref.get(), ref.data(), sizeof(double) * ref.size(),
+                           hipMemcpyHostToDevice));
+    std::vector<int16_t> pairs(2 * kNsym), sym(kNloc * kNloc, -1);
+    int e = 0;
+    for (int p = 0; p < kNloc; ++p)
+      for (int q = p; q < kNloc; ++q, ++e) {
+        pairs[2 * e] = (int16_t)p;
+        pairs[2 * e + 1] = (int16_t)q;
+        sym[p * kNloc + q] = (int16_t)e;
+      }
+    c.hpairs.resize(pairs.size());
+    c.hsym.resize(sym.size());
+    GH_CHECK_HIP(hipMemcpy(c.hpairs.get(), pairs.data(), sizeof(int16_t) * pairs.size(),
+                           hipMemcpyHostToDevice));
+    GH_CHECK_HIP(hipMemcpy(c.hsym.get(), sym.data(), sizeof(int16_t) * sym.size(),
+                           hipMemcpyHostToDevice));
+    c.sref_ready = true;
+  }
+  const bool f32 = precision == GH_PREC_FP32_ELEM;
+  const size_t esz = f32 ? sizeof(float) : sizeof(double);
+  c.elem.resize((size_t)c.ncells * kNsym * esz);
+  c.fdet.resize((size_t)c.ncells * kNn);
+  HexArgs A{};
+  A.ncells = c.ncells;
+  for (int d = 0; d < 3; ++d) A.h[d] = c.sizes[d];
+  A.E = c.have_kappa ? c.kappa.get() : nullptr;
+  A.ref = c.sref.get();
+  A.pairs = c.hpairs.get();
+  A.lam = c.nu / ((1.0 + c.nu) * (1.0 - 2.0 * c.nu));
+  A.mu = 1.0 / (2.0 * (1.0 + c.nu));
+  A.E_out = c.elem.get();
+  A.mvec = c.fdet.get();
+  if (c.ncells) {
+    if (f32)
+      hipLaunchKernelGGL(hex_elasticity_element_kernel<float>, dim3((unsigned)c.ncells), dim3(64),
+                         0, c.stream, A);
+    else
+      hipLaunchKernelGGL(hex_elasticity_element_kernel<double>, dim3((unsigned)c.ncells),
+                         dim3(64), 0, c.stream, A);
+  }
+  GH_CHECK_HIP(hipGetLastError());
+  elemlist_sum(c, f32);
+  const unsigned grid = (unsigned)std::max<int64_t>(1, (c.nfree + 255) / 256);
+  const double *dv = c.have_dvals ? c.dvals.get() : nullptr;
+  if (f32)
+    hipLaunchKernelGGL(hex_rhs_kernel<float>, dim3(grid), dim3(256), 0, c.stream, c.nfree,
+                       c.adjptr.get(), c.adj.get(), c.fdet.get(), c.body[0], c.body[1],
+                       c.body[2], c.dmask.get(), c.cell_dofs.get(), c.hsym.get(),
+                       (const float *)c.elem.get(), dv, c.b.get());
+  else
+    hipLaunchKernelGGL(hex_rhs_kernel<double>, dim3(grid), dim3(256), 0, c.stream, c.nfree,
+                       c.adjptr.get(), c.adj.get(), c.fdet.get(), c.body[0], c.body[1],
+                       c.body[2], c.dmask.get(), c.cell_dofs.get(), c.hsym.get(),
+                       (const double *)c.elem.get(), dv, c.b.get());
+  GH_CHECK_HIP(hipGetLastError());
+}
+
+}  // namespace gh

@@ -40,14 +40,15 @@ inline unsigned grid_for(int64_t n, int block = kBlock) {
 
 // ---- symbolic: dof -> (cell, local index), ascending cells ------------------------
 
-__global__ void dof_cell_keys(int64_t ncells, int nloc, const int32_t *__restrict__ cell_dofs,
-                              uint64_t invalid, uint64_t *__restrict__ keys) {
+__global__ void dof_cell_keys(int64_t ncells, int nloc, int lbits,
+                              const int32_t *__restrict__ cell_dofs, uint64_t invalid,
+                              uint64_t *__restrict__ keys) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= ncells * nloc) return;
   const int64_t cell = t / nloc;
   const int a = (int)(t - cell * nloc);
   const int d = cell_dofs[t];
-  keys[t] = d > 0 ? ((uint64_t)(d - 1) << 32) | (uint64_t)((cell << 4) | a) : invalid;
+  keys[t] = d > 0 ? ((uint64_t)(d - 1) << 32) | (uint64_t)((cell << lbits) | a) : invalid;
 }
 
 // adj[i] = (cell << 4 | a) of sorted entry i; adjptr[x] = first sorted position of
@@ -74,9 +75,10 @@ __global__ void dirichlet_mask_kernel(int64_t ncells, int nloc,
                                       uint16_t *__restrict__ dmask) {
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= ncells) return;
+  // bit a: local dof a is Dirichlet (nloc <= 16); else 1 when any is
   uint32_t m = 0;
   for (int a = 0; a < nloc; ++a)
-    if (cell_dofs[c * nloc + a] < 0) m |= 1u << a;
+    if (cell_dofs[c * nloc + a] < 0) m |= nloc <= 16 ? 1u << a : 1u;
   dmask[c] = (uint16_t)m;
 }
 
@@ -215,9 +217,7 @@ void launch_simplex(Ctx &c) {
                      0, c.stream, c.ncells, c.cell_nodes.get(), c.xyz.get(),
                      c.have_kappa ? c.kappa.get() : nullptr,
                      c.have_fsrc ? c.fsrc.get() : nullptr, c.sref.get(), E, c.fdet.get());
-  if (c.nnz)
-    hipLaunchKernelGGL((simplex_sum_kernel<T>), dim3(grid_for(c.nnz)), dim3(kBlock), 0, c.stream,
-                       c.nnz, c.cstart.get(), c.perm.get(), (const T *)E, c.nzval.get());
+  elemlist_sum(c, sizeof(T) == 4);
   hipLaunchKernelGGL((simplex_rhs_kernel<NLOC, T>), dim3(grid_for(c.nfree)), dim3(kBlock), 0,
                      c.stream, c.nfree, c.adjptr.get(), c.adj.get(), c.fdet.get(),
                      c.sref.get() + 6 * NLOC * NLOC, c.dmask.get(), c.cell_dofs.get(),
@@ -232,17 +232,18 @@ bool simplex_eligible(const Ctx &c) {
          c.ncomp == 1 && (c.nloc == 4 || c.nloc == 10);
 }
 
-void simplex_symbolic(Ctx &c) {
-  c.simplex = false;
+void elemlist_symbolic(Ctx &c) {
+  c.elemlist = false;
   c.adjptr.release();
   c.adj.release();
   c.dmask.release();
   c.elem.release();
-  if (!simplex_eligible(c) || c.nfree == 0) return;
+  if (!(simplex_eligible(c) || hexq2_eligible(c)) || c.nfree == 0) return;
   hipStream_t s = c.stream;
   const int64_t total = c.ncells * c.nloc;
-  GH_REQUIRE(total < (int64_t(1) << 31) && c.ncells < (int64_t(1) << 27), GH_ERR_UNSUPPORTED,
-             "mesh too large for the simplex adjacency");
+  const int lbits = c.nloc <= 16 ? 4 : 7;
+  GH_REQUIRE(total < (int64_t(1) << 31) && c.ncells < (int64_t(1) << (31 - lbits)),
+             GH_ERR_UNSUPPORTED, "mesh too large for the element-list adjacency");
   int rb = 1;
   while ((int64_t(1) << rb) <= c.nfree) ++rb;
   const uint64_t invalid = (uint64_t)c.nfree << 32;
@@ -250,7 +251,7 @@ void simplex_symbolic(Ctx &c) {
   keys.resize(total);
   sorted.resize(total);
   hipLaunchKernelGGL(dof_cell_keys, dim3(grid_for(total)), dim3(kBlock), 0, s, c.ncells, c.nloc,
-                     c.cell_dofs.get(), invalid, keys.get());
+                     lbits, c.cell_dofs.get(), invalid, keys.get());
   GH_CHECK_HIP(hipGetLastError());
   {
     size_t bytes = 0;
@@ -273,11 +274,25 @@ void simplex_symbolic(Ctx &c) {
   GH_CHECK_HIP(hipGetLastError());
   GH_CHECK_HIP(hipStreamSynchronize(s));
   c.fdet.resize(c.ncells);
-  c.simplex = true;
+  c.elemlist = true;
+}
+
+void elemlist_sum(Ctx &c, bool f32) {
+  if (!c.nnz) return;
+  if (f32)
+    hipLaunchKernelGGL((simplex_sum_kernel<float>), dim3(grid_for(c.nnz)), dim3(kBlock), 0,
+                       c.stream, c.nnz, c.cstart.get(), c.perm.get(),
+                       (const float *)c.elem.get(), c.nzval.get());
+  else
+    hipLaunchKernelGGL((simplex_sum_kernel<double>), dim3(grid_for(c.nnz)), dim3(kBlock), 0,
+                       c.stream, c.nnz, c.cstart.get(), c.perm.get(),
+                       (const double *)c.elem.get(), c.nzval.get());
+  GH_CHECK_HIP(hipGetLastError());
 }
 
 void assemble_simplex(Ctx &c, int precision) {
-  GH_REQUIRE(c.simplex, GH_ERR_STATE, "no simplex adjacency: call gh_symbolic");
+  GH_REQUIRE(c.elemlist && simplex_eligible(c), GH_ERR_STATE,
+             "no simplex adjacency: call gh_symbolic");
   const int NL = c.nloc;
   GH_REQUIRE((int)c.qw.size() == c.nq && (int)c.shp.size() == c.nq * NL &&
                  (int)c.grad.size() == c.nq * NL * 3,

@@ -19,8 +19,9 @@ from ._lib import GridapHipError
 from .assemblers import (AffineFEOperator, LinearFESolver, PCGSolver, SparseMatrixAssembler,
                          SparseMatrixCSC, assemble_matrix, assemble_matrix_and_vector,
                          assemble_vector, get_matrix, get_vector, solve)
-from .celldata import (CellField, DomainContribution, Measure, dot, get_fe_basis,
-                       get_trial_fe_basis, grad, gradient, inner, integral, integrate)
+from .celldata import (CellField, DomainContribution, IsotropicStress, Measure, dot,
+                       get_fe_basis, get_trial_fe_basis, grad, gradient, inner, integral,
+                       integrate, symmetric_gradient, ε)
 from .fespaces import FEFunction, FESpace, TestFESpace, TrialFESpace
 from .geometry import (CartesianDiscreteModel, FaceLabeling, Triangulation, UnstructuredDiscreteModel,
                        get_triangulation, simplexify)
@@ -35,4 +36,5 @@ __all__ = [
     "assemble_matrix", "assemble_matrix_and_vector", "assemble_vector", "dot",
     "get_fe_basis", "get_matrix", "get_triangulation", "get_trial_fe_basis", "get_vector",
     "grad", "gradient", "inner", "integral", "integrate", "lagrangian", "solve",
+    "IsotropicStress", "symmetric_gradient", "ε",
 ]

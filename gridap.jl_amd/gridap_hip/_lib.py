@@ -26,6 +26,7 @@ GH_REFFE_PK = 2
 GH_FORM_DIFFUSION = 1
 GH_PREC_FP64 = 0
 GH_PREC_FP32_ELEM = 1
+GH_FORM_ELASTICITY = 2
 GH_PATH_AUTO = 0
 GH_PATH_GENERAL = 1
 GH_PATH_STRUCTURED = 2
@@ -70,6 +71,7 @@ SIGNATURES = {
     "gh_fsai_train_step": (_i32, [_vp, _i32, C.c_uint64, C.c_float, C.POINTER(_dbl)]),
     "gh_fsai_get": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp]),
     "gh_fsai_factor_get": (_i32, [_vp, _vp, _vp, _vp, _vp]),
+    "gh_elasticity_params": (_i32, [_vp, _dbl, _vp]),
 }
 
 _lib = None
@@ -192,6 +194,10 @@ class Context:
         g = None if dirichlet_vals is None else np.ascontiguousarray(dirichlet_vals,
                                                                      dtype=np.float64)
         self.check(lib().gh_coef_upload(self.h, ptr(k), ptr(f), ptr(g)))
+
+    def elasticity_params(self, nu, body_force=None):
+        f = None if body_force is None else np.ascontiguousarray(body_force, dtype=np.float64)
+        self.check(lib().gh_elasticity_params(self.h, float(nu), ptr(f)))
 
     def assemble(self, form=GH_FORM_DIFFUSION, precision=GH_PREC_FP64, path=GH_PATH_AUTO):
         self.check(lib().gh_assemble(self.h, form, precision, path))

@@ -12,8 +12,8 @@ import numpy as np
 
 from . import _lib
 from ._lib import Context
-from .celldata import (DiffusionTerm, DomainContribution, SourceTerm, get_fe_basis,
-                       get_trial_fe_basis)
+from .celldata import (BodyForceTerm, DiffusionTerm, DomainContribution, ElasticityTerm,
+                       SourceTerm, get_fe_basis, get_trial_fe_basis)
 from .fespaces import FEFunction, TrialFESpace
 from .reference_fes import Quadrature
 
@@ -107,13 +107,23 @@ class SparseMatrixAssembler:
             self._degree = degree
 
     def _lower(self, matcontrib, veccontrib):
+        """Weak form -> (form kind, per-cell coefficient, per-cell source, degree, extra)."""
         ncells = self.space.model.num_cells()
         kappa, fsrc, degs = None, None, set()
+        nu, force = None, None
         have_mat = have_vec = False
+        kinds = set()
         if matcontrib is not None:
             for term, meas in matcontrib.terms:
-                if not isinstance(term, DiffusionTerm):
-                    raise NotImplementedError("bilinear forms: κ ∇v⋅∇u only")
+                if isinstance(term, ElasticityTerm):
+                    kinds.add(_lib.GH_FORM_ELASTICITY)
+                    if nu is not None and nu != term.nu:
+                        raise NotImplementedError("one Poisson ratio per form")
+                    nu = term.nu
+                elif isinstance(term, DiffusionTerm):
+                    kinds.add(_lib.GH_FORM_DIFFUSION)
+                else:
+                    raise NotImplementedError("bilinear forms: κ ∇v⋅∇u or ε(v) : σ(ε(u))")
                 k = term.coef.per_cell(ncells) if term.coef is not None else None
                 k = np.ones(ncells) if k is None else k
                 kappa = k if kappa is None else kappa + k
@@ -121,30 +131,42 @@ class SparseMatrixAssembler:
                 have_mat = True
         if veccontrib is not None:
             for term, meas in veccontrib.terms:
-                if not isinstance(term, SourceTerm):
-                    raise NotImplementedError("linear forms: f v only")
-                f = term.coef.per_cell(ncells) if term.coef is not None else None
-                f = np.ones(ncells) if f is None else f
-                fsrc = f if fsrc is None else fsrc + f
+                if isinstance(term, BodyForceTerm):
+                    kinds.add(_lib.GH_FORM_ELASTICITY)
+                    f = np.asarray(term.force, dtype=np.float64)
+                    force = f if force is None else force + f
+                elif isinstance(term, SourceTerm):
+                    kinds.add(_lib.GH_FORM_DIFFUSION)
+                    f = term.coef.per_cell(ncells) if term.coef is not None else None
+                    f = np.ones(ncells) if f is None else f
+                    fsrc = f if fsrc is None else fsrc + f
+                else:
+                    raise NotImplementedError("linear forms: f v or v⋅f")
                 degs.add(meas.degree)
                 have_vec = True
+        if len(kinds) > 1:
+            raise NotImplementedError("one form kind per operator")
         if len(degs) > 1:
             raise NotImplementedError("matrix and vector terms must share one Measure")
         if not degs:
             raise ValueError("empty weak form")
+        form = kinds.pop()
         if have_mat and kappa is not None and np.all(kappa == 1.0):
             kappa = None
-        if have_vec and fsrc is not None and np.all(fsrc == 1.0):
-            fsrc = None
-        if not have_vec:
-            fsrc = np.zeros(ncells)
+        if form == _lib.GH_FORM_DIFFUSION:
+            if have_vec and fsrc is not None and np.all(fsrc == 1.0):
+                fsrc = None
+            if not have_vec:
+                fsrc = np.zeros(ncells)
         if not have_mat:
             kappa = np.zeros(ncells)
-        return kappa, fsrc, degs.pop()
+        extra = dict(nu=0.3 if nu is None else nu,
+                     force=np.zeros(3) if force is None else force)
+        return form, kappa, fsrc, degs.pop(), extra
 
     def assemble(self, matcontrib, veccontrib, lift):
         """Device numeric phase; returns (SparseMatrixCSC, b)."""
-        kappa, fsrc, degree = self._lower(matcontrib, veccontrib)
+        form, kappa, fsrc, degree, extra = self._lower(matcontrib, veccontrib)
         self._setup(degree)
         dvals = None
         if lift:
@@ -152,7 +174,9 @@ class SparseMatrixAssembler:
             if dv is not None and dv.size and np.any(dv != 0):
                 dvals = dv
         self.ctx.coef_upload(kappa, fsrc, dvals)
-        self.ctx.assemble(_lib.GH_FORM_DIFFUSION, self.precision, self.path)
+        if form == _lib.GH_FORM_ELASTICITY:
+            self.ctx.elasticity_params(extra["nu"], extra["force"])
+        self.ctx.assemble(form, self.precision, self.path)
         nz, b = self.ctx.values_get()
         n = self.space.nfree
         A = SparseMatrixCSC(n, n, self.colptr.copy(), self.rowval.copy(), nz)

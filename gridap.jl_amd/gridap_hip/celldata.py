@@ -7,8 +7,11 @@ form kind plus per-cell coefficients, which is what the device kernels consume:
 
     a(u,v) = integrate(κ * inner(gradient(v), gradient(u)), dΩ)   -> diffusion, κ
     l(v)   = integrate(f * v, dΩ)                                 -> source, f
+    a(u,v) = integrate(inner(ε(v), σ(ε(u))), dΩ), σ = IsotropicStress(E, ν)
+                                                                  -> elasticity, E, ν
+    l(v)   = integrate(dot(v, VectorValue(f1, f2, f3)), dΩ)       -> body force
 
-κ and f are numbers or per-cell arrays (CellField(values, Ω)).
+κ, f and E are numbers or per-cell arrays (CellField(values, Ω)).
 """
 import numbers
 
@@ -90,6 +93,49 @@ class SourceTerm(_Expr):
         self.coef = coef
 
 
+class _SymGrad(_Expr):
+    """ε(u) = (∇u + ∇uᵀ)/2 of an FE basis."""
+
+    def __init__(self, basis):
+        self.basis = basis
+
+
+class IsotropicStress:
+    """σ(ε) = λ tr(ε) I + 2 μ ε with λ = Eν/((1+ν)(1-2ν)), μ = E/(2(1+ν)), E a number or
+    per-cell CellField (test/GridapTests/IsotropicDamageTests.jl:13-17 spells the same
+    law); σ(ε(u)) is Gridap's σ∘ε(u)."""
+
+    def __init__(self, E, nu):
+        self.E = _as_coef(E)
+        self.nu = float(nu)
+
+    def __call__(self, eps):
+        if not isinstance(eps, _SymGrad):
+            raise NotImplementedError("σ applies to ε(u) of a trial basis")
+        return _Stress(eps, self)
+
+
+class _Stress(_Expr):
+    def __init__(self, eps, material):
+        self.eps = eps
+        self.material = material
+
+
+class ElasticityTerm(_Expr):
+    """ε(v) : σ(ε(u)) with per-cell E (coef) and ν."""
+
+    def __init__(self, coef=None, nu=0.3):
+        self.coef = coef
+        self.nu = nu
+
+
+class BodyForceTerm(_Expr):
+    """v · f with f constant."""
+
+    def __init__(self, force):
+        self.force = tuple(force)
+
+
 class _Sum(_Expr):
     def __init__(self, terms):
         self.terms = terms
@@ -129,9 +175,26 @@ def gradient(x):
 grad = gradient
 
 
+def symmetric_gradient(x):
+    if not isinstance(x, FEBasis):
+        raise NotImplementedError("ε of an FE basis only")
+    return _SymGrad(x)
+
+
+ε = symmetric_gradient
+
+
 def inner(a, b):
+    from .reference_fes import VectorValue
     if isinstance(a, _Grad) and isinstance(b, _Grad) and a.basis.trial != b.basis.trial:
         return DiffusionTerm()
+    if isinstance(a, _SymGrad) and isinstance(b, _Stress) and not a.basis.trial and \
+            b.eps.basis.trial:
+        return ElasticityTerm(b.material.E, b.material.nu)
+    if isinstance(a, FEBasis) and isinstance(b, VectorValue) and not a.trial:
+        return BodyForceTerm(b.comps)
+    if isinstance(b, FEBasis) and isinstance(a, VectorValue) and not b.trial:
+        return BodyForceTerm(a.comps)
     if isinstance(a, FEBasis) and not isinstance(b, _Expr):
         return _scale(b, a)
     if isinstance(b, FEBasis) and not isinstance(a, _Expr):
@@ -175,7 +238,7 @@ class DomainContribution:
 def integrate(expr, measure):
     terms = expr.terms if isinstance(expr, _Sum) else [expr]
     for t in terms:
-        if not isinstance(t, (DiffusionTerm, SourceTerm)):
+        if not isinstance(t, (DiffusionTerm, SourceTerm, ElasticityTerm, BodyForceTerm)):
             raise NotImplementedError("unsupported integrand %r" % (t,))
     return DomainContribution([(t, measure) for t in terms])
 

@@ -25,11 +25,15 @@ class FESpace:
         if conformity not in ("H1", ":H1"):
             raise NotImplementedError("only H1-conforming spaces are provided")
         self.simplex = isinstance(model, UnstructuredDiscreteModel)
+        self.conforming = self.simplex or reffe.order == 2
         if self.simplex:
             if reffe.order not in (1, 2) or reffe.ncomp != 1:
                 raise NotImplementedError("simplicial spaces: scalar P1 and P2")
+        elif reffe.order == 2:
+            if model.D != 3:
+                raise NotImplementedError("Q2 spaces are provided on 3-D Cartesian models")
         elif reffe.order != 1:
-            raise NotImplementedError("n-cube spaces of order > 1 are not provided yet")
+            raise NotImplementedError("n-cube spaces of order > 2 are not provided")
         self.model = model
         self.reffe = reffe
         self.order = reffe.order
@@ -40,6 +44,9 @@ class FESpace:
         labels = model.get_face_labeling()
         if self.simplex:
             self._number_simplex(labels)
+            return
+        if self.order == 2:
+            self._number_q2(labels, dirichlet_masks)
             return
         node_to_tag = labels.face_tag_index(self.dirichlet_tags, 0)
         ntags = len(self.dirichlet_tags)
@@ -112,6 +119,101 @@ class FESpace:
         self.dirichlet_dof_coords = xyz[isd]
         self.node_and_comp_to_dof = None
 
+    def _number_q2(self, labels, dirichlet_masks):
+        """Conforming (vector) Q2 numbering on a Cartesian hex grid
+        (src/FESpaces/ConformingFESpaces.jl:367-636): the grid topology numbers edges
+        and quad faces by first touch; faces are visited by dimension (vertices,
+        edges, quads, cells) in id order, each owning one node and its ncomp dofs
+        (node-major); free ids 1.., Dirichlet ids -1.. in that order, a component of a
+        tagged face being Dirichlet when its tag's mask says so.  Local dofs are
+        component-blocked: node + 27 * comp, nodes = 8 vertices, 12 edges, 6 faces, 1
+        interior in Gridap's face order."""
+        from .geometry import _first_touch, ncube_face_id
+        m = self.model
+        nc = self.ncomp
+        cells = m.cell_node_ids().astype(np.int64) - 1            # C x 8, x-fastest
+        nn = m.num_nodes()
+        C = cells.shape[0]
+        hex_edges = ((0, 1), (2, 3), (4, 5), (6, 7), (0, 2), (1, 3), (4, 6), (5, 7),
+                     (0, 4), (1, 5), (2, 6), (3, 7))
+        hex_faces = ((0, 1, 2, 3), (4, 5, 6, 7), (0, 1, 4, 5), (2, 3, 6, 7), (0, 2, 4, 6),
+                     (1, 3, 5, 7))
+        ekeys = np.stack([np.minimum(cells[:, a], cells[:, b]) * 3 + ax
+                          for (a, b), ax in zip(hex_edges, (0,) * 4 + (1,) * 4 + (2,) * 4)],
+                         axis=1).reshape(-1)
+        cell_edges, ne = _first_touch(ekeys)
+        cell_edges = cell_edges.reshape(C, 12)
+        fkeys = np.stack([cells[:, f[0]] * 3 + ax
+                          for f, ax in zip(hex_faces, (2, 2, 1, 1, 0, 0))], axis=1).reshape(-1)
+        cell_faces, nf = _first_touch(fkeys)
+        cell_faces = cell_faces.reshape(C, 6)
+        # entity of every face = the box face holding all its vertices
+        part = m.partition
+        n0, n1 = part[0] + 1, part[1] + 1
+        ijk = lambda v: np.stack([v % n0, (v // n0) % n1, v // (n0 * n1)])
+        table = np.zeros((8, 8), dtype=np.int32)
+        for mk in range(8):
+            for an in range(8):
+                table[mk, an] = ncube_face_id(3, mk, an & ~mk)
+
+        def entity(vs):                                   # vs: list of vertex-id arrays
+            L = [ijk(v) for v in vs]
+            free = np.zeros(len(vs[0]), dtype=np.int64)
+            anchor = np.zeros(len(vs[0]), dtype=np.int64)
+            for d in range(3):
+                lo = np.all([l[d] == 0 for l in L], axis=0)
+                hi = np.all([l[d] == part[d] for l in L], axis=0)
+                free |= (~(lo | hi)).astype(np.int64) << d
+                anchor |= hi.astype(np.int64) << d
+            return table[free, anchor & ~free]
+
+        first_e = np.zeros(ne, dtype=np.int64)
+        first_e[cell_edges.reshape(-1)] = np.arange(C * 12)
+        ce, le = first_e // 12, first_e % 12
+        edge_ent = entity([cells[ce, np.array(hex_edges)[le, 0]], cells[ce, np.array(hex_edges)[le, 1]]])
+        first_f = np.zeros(nf, dtype=np.int64)
+        first_f[cell_faces.reshape(-1)] = np.arange(C * 6)
+        cf, lf = first_f // 6, first_f % 6
+        hf = np.array(hex_faces)
+        face_ent = entity([cells[cf, hf[lf, k]] for k in range(4)])
+        ents = [m.get_face_labeling().vertex_entity.astype(np.int64), edge_ent, face_ent,
+                np.full(C, 27, dtype=np.int64)]
+        # tag index per face (first tag holding the entity), then per component
+        tags = self.dirichlet_tags
+        ntags = len(tags)
+        masks = (np.ones((max(ntags, 1), nc), dtype=bool) if dirichlet_masks is None else
+                 np.array(dirichlet_masks, dtype=bool).reshape(ntags, nc))
+        isd = []
+        for e in ents:
+            t = np.zeros(len(e), dtype=np.int64)
+            for k, tag in enumerate(tags):
+                hit = np.isin(e, labels.tag_to_entities(tag)) & (t == 0)
+                t[hit] = k + 1
+            dm = np.zeros((len(e), nc), dtype=bool)
+            dm[t > 0] = masks[t[t > 0] - 1]
+            isd.append(dm.reshape(-1))                    # node-major, comp-minor
+        isd = np.concatenate(isd)
+        ids = np.empty(len(isd), dtype=np.int64)
+        ids[~isd] = np.arange(1, np.count_nonzero(~isd) + 1)
+        ids[isd] = -np.arange(1, np.count_nonzero(isd) + 1)
+        off = np.cumsum([0, nn * nc, ne * nc, nf * nc])
+        node_face = np.concatenate([cells, off[1] // nc + cell_edges, off[2] // nc + cell_faces,
+                                    off[3] // nc + np.arange(C)[:, None]], axis=1)   # C x 27
+        cd = np.concatenate([ids[node_face * nc + k] for k in range(nc)], axis=1)
+        self.cell_dof_ids = np.ascontiguousarray(cd, dtype=np.int32)
+        self.nfree = int(np.count_nonzero(~isd))
+        self.ndirichlet = int(np.count_nonzero(isd))
+        # coordinates of every Q2 node (vertices, edge / face / cell midpoints), per dof
+        xyz = m.node_coordinates()
+        ev = np.stack([cells[ce, np.array(hex_edges)[le, 0]], cells[ce, np.array(hex_edges)[le, 1]]], 1)
+        fv = np.stack([cells[cf, hf[lf, k]] for k in range(4)], 1)
+        nodes = np.concatenate([xyz, xyz[ev].mean(1), xyz[fv].mean(1), xyz[cells].mean(1)])
+        comp = np.tile(np.arange(nc), len(nodes))
+        pts = np.repeat(nodes, nc, axis=0)
+        self.free_dof_coords, self.free_dof_comp = pts[~isd], comp[~isd]
+        self.dirichlet_dof_coords, self.dirichlet_dof_comp = pts[isd], comp[isd]
+        self.node_and_comp_to_dof = None
+
     # Gridap accessors --------------------------------------------------------
     def get_cell_dof_ids(self):
         return self.cell_dof_ids
@@ -144,9 +246,14 @@ class TrialFESpace:
         self.space = V
         self.g = g
         vals = np.zeros(V.ndirichlet)
-        if g is not None and V.ndirichlet and getattr(V, "simplex", False):
-            vals = (np.asarray(g(V.dirichlet_dof_coords), dtype=np.float64).reshape(-1)
-                    if callable(g) else np.full(V.ndirichlet, float(g)))
+        if g is not None and V.ndirichlet and getattr(V, "conforming", False):
+            if callable(g):
+                gv = np.asarray(g(V.dirichlet_dof_coords), dtype=np.float64)
+                if gv.ndim > 1:                           # vector data: pick each dof's comp
+                    gv = gv[np.arange(V.ndirichlet), V.dirichlet_dof_comp]
+                vals = gv.reshape(-1)
+            else:
+                vals = np.full(V.ndirichlet, float(g))
         elif g is not None and V.ndirichlet:
             xyz = V.model.node_coordinates()
             nodes = V.dirichlet_dof_to_node - 1

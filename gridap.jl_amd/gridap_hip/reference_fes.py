@@ -18,11 +18,21 @@ Float64 = float
 
 
 class VectorValue:
-    """Marker for vector-valued Lagrangian spaces: VectorValue[D, Float64]."""
+    """VectorValue[D, Float64] marks a vector-valued Lagrangian space;
+    VectorValue(0.0, 0.0, -1.0) is a constant vector (a body force)."""
 
     def __class_getitem__(cls, args):
         D = args[0] if isinstance(args, tuple) else args
         return ("VectorValue", int(D))
+
+    def __init__(self, *comps):
+        self.comps = tuple(float(c) for c in comps)
+
+    def __len__(self):
+        return len(self.comps)
+
+    def __iter__(self):
+        return iter(self.comps)
 
 
 class LagrangianRefFE:

@@ -41,6 +41,9 @@ extern "C" {
 
 /* ---- weak forms ---- */
 #define GH_FORM_DIFFUSION 1 /* a(u,v)=∫ κ ∇v⋅∇u dΩ,  l(v)=∫ f v dΩ, κ and f constant per cell */
+#define GH_FORM_ELASTICITY 2 /* a(u,v)=∫ ε(v) : σ(ε(u)) dΩ, σ = λ tr(ε) I + 2 μ ε with λ, μ from a
+                              * per-cell Young's modulus (the κ array) and ν; l(v)=∫ v⋅f dΩ with f
+                              * constant (gh_elasticity_params); vector Q2 on Cartesian hexes */
 
 /* ---- element precision ---- */
 #define GH_PREC_FP64 0      /* element arithmetic and accumulation in FP64           */
@@ -124,6 +127,11 @@ int gh_coef_upload(gh_ctx *ctx, const double *kappa, const double *fsrc,
  * src/FESpaces/SparseMatrixAssemblers.jl:365-405, src/Algebra/SparseMatrixCSC.jl:124-150,
  * src/CellData/AttachDirichlet.jl:59-84. */
 int gh_assemble(gh_ctx *ctx, int form_kind, int precision, int path);
+
+/* Material and load of GH_FORM_ELASTICITY: Poisson ratio ν in (-1, 0.5) and the body
+ * force f[dim] (NULL = 0).  The weak form of Gridap's elasticity drivers
+ * (ε(v) ⊙ (σ∘ε(u)), src/TensorValues/ and test/GridapTests/IsotropicDamageTests.jl:13-17). */
+int gh_elasticity_params(gh_ctx *ctx, double nu, const double *body_force);
 
 /* Copy the assembled nzval[nnz] and b[nfree] to the host (either may be NULL). */
 int gh_values_get(gh_ctx *ctx, double *nzval, double *b);

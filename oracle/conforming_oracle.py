@@ -105,7 +105,8 @@ def first_touch_faces(cell_vertices, lface_vertices):
 
 # ---- conforming numbering -------------------------------------------------------------
 
-def conforming_cell_dofs(d_cell_faces, d_face_tags, d_lface_own, nloc):
+def conforming_cell_dofs(d_cell_faces, d_face_tags, d_lface_own, nloc, ldof_comp=None,
+                         comp_masks=None):
     """Gridap's generic conforming numbering (src/FESpaces/ConformingFESpaces.jl:
     367-575): every d-face owns the dofs its reference face owns; faces are visited
     dimension by dimension in face-id order and their own dofs get the next free
@@ -114,6 +115,9 @@ def conforming_cell_dofs(d_cell_faces, d_face_tags, d_lface_own, nloc):
     d_cell_faces[d]: cells x nlfaces_d array of 1-based d-face ids.
     d_face_tags[d]: per d-face, 0 = free, else the Dirichlet tag (1-based).
     d_lface_own[d][lf]: local dof ids (0-based) owned by local d-face lf, in order.
+    comp_masks (optional, with ldof_comp): per tag, per component, whether that
+    component is Dirichlet on the tag's faces
+    (_split_face_own_dofs_into_free_and_dirichlet_with_components!, :577-636).
     Returns (cell_dofs [cells x nloc], nfree, ndiri, dirichlet_dof_tag)."""
     D = len(d_cell_faces) - 1
     ncells = d_cell_faces[0].shape[0]
@@ -124,15 +128,21 @@ def conforming_cell_dofs(d_cell_faces, d_face_tags, d_lface_own, nloc):
         nface = len(d_face_tags[d])
         # own dof count of each d-face: from any cell holding it (one cell type)
         nown = np.zeros(nface, dtype=np.int64)
+        owner_lf = np.zeros(nface, dtype=np.int64)
         for c in range(ncells):
             for lf, f in enumerate(d_cell_faces[d][c]):
                 nown[f - 1] = len(d_lface_own[d][lf])
+                owner_lf[f - 1] = lf
         ids = []
         for f in range(nface):
             tag = d_face_tags[d][f]
             mine = []
-            for _ in range(nown[f]):
-                if tag == 0:
+            for k in range(nown[f]):
+                diri = tag != 0
+                if diri and comp_masks is not None:
+                    ldof = d_lface_own[d][owner_lf[f]][k]
+                    diri = bool(comp_masks[tag - 1][ldof_comp[ldof]])
+                if not diri:
                     nfree += 1
                     mine.append(nfree)
                 else:
@@ -403,3 +413,137 @@ def tet_degree5_rule():
     r = least_squares(res, [0.012, 0.09, 0.019, 0.31, 0.007, 0.045], xtol=3e-16, ftol=3e-16,
                       gtol=3e-16)
     return _tet_orbit_points(*r.x)
+
+
+# ---- Cartesian conforming (vector) Lagrangian spaces -------------------------------------
+
+def cartesian_cells(partition):
+    """Cell node ids (0-based, local x fastest) of a CartesianDiscreteModel."""
+    D = len(partition)
+    n = [p + 1 for p in partition]
+    out = []
+    for idx in np.ndindex(*reversed(partition)):
+        ci = list(reversed(idx))
+        nodes = []
+        for a in range(2 ** D):
+            off = [(a >> d) & 1 for d in range(D)]
+            node, stride = 0, 1
+            for d in range(D):
+                node += (ci[d] + off[d]) * stride
+                stride *= n[d]
+            nodes.append(node)
+        out.append(nodes)
+    return np.array(out, dtype=np.int64)
+
+
+def lattice_ijk(partition, node):
+    n = [p + 1 for p in partition]
+    out, r = [], node
+    for d in range(len(partition)):
+        out.append(r % n[d])
+        r //= n[d]
+    return tuple(out)
+
+
+def qk_vector_own_dofs(poly, order, ncomp):
+    """Own local dofs per face of the vector Lagrangian Q_k element: each own node
+    contributes its ncomp dofs, node-major (local dof = node + nnodes * comp)."""
+    own, nnodes = lagrangian_own_dofs(poly, order)
+    vown = [[[n + nnodes * c for n in nodes for c in range(ncomp)] for nodes in per]
+            for per in own]
+    comp = [ld // nnodes for ld in range(nnodes * ncomp)]
+    return vown, nnodes * ncomp, comp
+
+
+def cartesian_numbering(partition, order, ncomp, tag_entities, comp_masks=None):
+    """Conforming numbering of a (vector) Q_k space on a Cartesian grid with the
+    first-touch grid topology; tag_entities: list of entity sets, one per Dirichlet
+    tag (a face takes the first tag holding its entity)."""
+    D = len(partition)
+    poly = HEX if D == 3 else QUAD
+    cells = cartesian_cells(partition)
+    d_faces = [cells + 1]
+    for d in range(1, D):
+        d_faces.append(first_touch_faces(cells, poly.faces[d])[0])
+    d_faces.append(np.arange(1, len(cells) + 1).reshape(-1, 1))
+    d_tags = []
+    for d in range(D + 1):
+        nf = int(d_faces[d].max())
+        tags = [0] * nf
+        for c in range(len(cells)):
+            for lf, lv in enumerate(poly.faces[d]):
+                ent = simplex_face_entity(D, partition,
+                                          [lattice_ijk(partition, cells[c][v - 1]) for v in lv])
+                t = 0
+                for k, ents in enumerate(tag_entities):
+                    if ent in ents:
+                        t = k + 1
+                        break
+                tags[d_faces[d][c, lf] - 1] = t
+        d_tags.append(tags)
+    own, nloc, comp = qk_vector_own_dofs(poly, order, ncomp)
+    cd, nfree, ndiri, dtag = conforming_cell_dofs(d_faces, d_tags, own, nloc, comp, comp_masks)
+    return cells, cd, nfree, ndiri, dtag
+
+
+# ---- isotropic linear elasticity with vector Q2 on boxes -----------------------------------
+
+def lame(E, nu):
+    """λ, μ of an isotropic material (test/GridapTests/IsotropicDamageTests.jl:13-17)."""
+    return E * nu / ((1 + nu) * (1 - 2 * nu)), E / (2 * (1 + nu))
+
+
+def elasticity_reference(oracle, h, nu, qdeg=4, order=2):
+    """Element matrix of ∫ ε(v) : σ(ε(u)) for E = 1 on a box of sizes h, vector Q_k
+    (component-blocked local dofs), and ∫ φ_a over the box: σ = λ tr(ε) I + 2 μ ε, so
+    K[(a,i),(b,j)] = Σ_q w|J| [λ g_ai g_bj + μ (g_aj g_bi + δ_ij g_a·g_b)], g = ∂φ/∂x."""
+    D = len(h)
+    qx, qw = oracle.tensor_quadrature(D, qdeg)
+    shp, grad = oracle.qk_tabulate(D, order, qx)
+    nn = shp.shape[1]
+    g = grad / np.asarray(h)[None, None, :]               # physical gradients (box)
+    det = float(np.prod(h))
+    lam, mu = lame(1.0, nu)
+    K = np.zeros((D * nn, D * nn))
+    for i in range(D):
+        for j in range(D):
+            blk = lam * np.einsum("q,qa,qb->ab", qw, g[:, :, i], g[:, :, j]) \
+                + mu * np.einsum("q,qa,qb->ab", qw, g[:, :, j], g[:, :, i])
+            if i == j:
+                blk = blk + mu * np.einsum("q,qak,qbk->ab", qw, g, g)
+            K[i * nn:(i + 1) * nn, j * nn:(j + 1) * nn] = det * blk
+    m = det * (qw @ shp)                                  # ∫ φ_a
+    return K, m
+
+
+def elasticity_q2_box(oracle, partition, E, nu, body_force, tag_entities, g=None,
+                      domain=None, qdeg=4):
+    """Cell matrices / vectors of the Cartesian vector-Q2 elasticity problem:
+    K_c = E_c K̂, F_c[(a,k)] = f_k ∫φ_a, Dirichlet data g(x) (callable -> D-vector) at
+    the Q2 nodes, lifted into F (attach_dirichlet)."""
+    D = len(partition)
+    domain = domain or tuple(v for _ in range(D) for v in (0.0, 1.0))
+    h = [(domain[2 * d + 1] - domain[2 * d]) / partition[d] for d in range(D)]
+    cells, cd, nfree, ndiri, _ = cartesian_numbering(partition, 2, D, tag_entities)
+    Khat, m = elasticity_reference(oracle, h, nu, qdeg)
+    E = np.asarray(E, dtype=float)
+    mats = E[:, None, None] * Khat[None]
+    nn = len(m)
+    F = np.concatenate([body_force[k] * m for k in range(D)])
+    vecs = np.repeat(F[None], len(cells), axis=0)
+    dvals = np.zeros(ndiri)
+    if g is not None:
+        nodes = oracle.qk_nodes(D, 2)                    # reference Q2 nodes
+        for c in range(len(cells)):
+            x0 = np.array([domain[2 * d] + lattice_ijk(partition, cells[c][0])[d] * h[d]
+                           for d in range(D)])
+            for a in range(D * nn):
+                if cd[c, a] < 0:
+                    x = x0 + nodes[a % nn] * np.asarray(h)
+                    dvals[-cd[c, a] - 1] = g(x)[a // nn]
+        for c in range(len(cells)):
+            msk = cd[c] < 0
+            if msk.any():
+                vecs[c] = vecs[c] - mats[c][:, msk] @ dvals[-cd[c, msk] - 1]
+    return dict(cells=cells, cell_dofs=cd, nfree=nfree, ndiri=ndiri, cellmats=mats,
+                cellvecs=vecs, dirichlet_vals=dvals, Khat=Khat, m=m, h=h)

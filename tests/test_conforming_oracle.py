@@ -234,3 +234,73 @@ def test_tet_assembly_reproduces_polynomials(oracle, order):
     r0 = O.tet_poisson(part, xyz, order, fsrc=np.full(6 * 18, -lap), qx=P, qw=W)
     b0 = oracle.assemble_vector(r0["cell_dofs"], r0["cellvecs"], r0["nfree"])
     assert np.max(np.abs(A @ uI - b0)) > 1e-3
+
+
+def test_vector_numbering_with_components_matches_reference_fixture():
+    # test/FESpacesTests/ConformingFESpacesTests.jl:48-65: 3x3 QUAD, vector order 1,
+    # tags 1 and 6 with masks (true, true) and (false, true)
+    cells, cd, nfree, ndiri, dtag = O.cartesian_numbering((3, 3), 1, 2, [{1}, {6}],
+                                                           comp_masks=[(1, 1), (0, 1)])
+    r = [[-1, 1, 7, 9, -2, 2, 8, 10], [1, 3, 9, 11, 2, 4, 10, 12], [3, 5, 11, 13, 4, 6, 12, 14],
+         [7, 9, 15, 17, 8, 10, 16, 18], [9, 11, 17, 19, 10, 12, 18, 20],
+         [11, 13, 19, 21, 12, 14, 20, 22], [15, 17, 23, 25, 16, 18, 24, -3],
+         [17, 19, 25, 26, 18, 20, -3, -4], [19, 21, 26, 27, 20, 22, -4, 28]]
+    np.testing.assert_array_equal(cd, r)
+    assert (nfree, ndiri) == (28, 4)
+    assert dtag == [1, 1, 2, 2]
+    # and the scalar Q2 fixture through the same Cartesian driver
+    _, cd2, nf2, nd2, _ = O.cartesian_numbering((3, 3), 2, 1, [{1}, {6}])
+    assert (nf2, nd2) == (43, 6) and cd2[0].tolist() == [-1, 1, 4, 5, 14, 15, 16, 17, 35]
+
+
+def rigid_modes(nodes):
+    nn = len(nodes)
+    modes = []
+    for k in range(3):
+        u = np.zeros((3, nn))
+        u[k] = 1.0
+        modes.append(u.ravel())
+    for a, b in ((0, 1), (1, 2), (2, 0)):            # infinitesimal rotations
+        u = np.zeros((3, nn))
+        u[a] = -nodes[:, b]
+        u[b] = nodes[:, a]
+        modes.append(u.ravel())
+    return np.array(modes)
+
+
+def test_elasticity_element_kernel_is_rigid_motions(oracle):
+    h = (0.5, 0.25, 0.2)
+    K, m = O.elasticity_reference(oracle, h, 0.3)
+    assert K.shape == (81, 81)
+    np.testing.assert_allclose(K, K.T, atol=1e-15 * np.abs(K).max())
+    nodes = oracle.qk_nodes(3, 2) * np.array(h)
+    R = rigid_modes(nodes)
+    assert np.abs(K @ R.T).max() < 1e-13 * np.abs(K).max()
+    ev = np.linalg.eigvalsh(K)
+    assert np.all(ev[:6] < 1e-12 * ev[-1]) and ev[6] > 1e-4 * ev[-1]
+    np.testing.assert_allclose(m.sum(), np.prod(h), rtol=1e-14)
+
+
+def test_elasticity_assembly_reproduces_quadratics(oracle):
+    # u = (xy, yz, zx + x^2) with E = 1: -div σ(u) = -((λ+μ)(1,1,1) + μ(0,0,2)), and the
+    # Q2 interpolant solves the discrete system exactly: A u_I = b
+    part = (3, 2, 2)
+    nu = 0.3
+    lam, mu = O.lame(1.0, nu)
+    f = -np.array([lam + mu, lam + mu, lam + 3 * mu])
+    u = lambda x: np.array([x[0] * x[1], x[1] * x[2], x[2] * x[0] + x[0] ** 2])
+    r = O.elasticity_q2_box(oracle, part, np.ones(12), nu, f, [O.boundary_entities(3)], g=u)
+    cp, rv, nz = oracle.assemble_csc(r["cell_dofs"], r["cellmats"], r["nfree"], r["nfree"])
+    b = oracle.assemble_vector(r["cell_dofs"], r["cellvecs"], r["nfree"])
+    nodes = oracle.qk_nodes(3, 2)
+    uI = np.zeros(r["nfree"])
+    for c, cell in enumerate(r["cells"]):
+        x0 = np.array([O.lattice_ijk(part, cell[0])[d] * r["h"][d] for d in range(3)])
+        for a in range(81):
+            if r["cell_dofs"][c, a] > 0:
+                uI[r["cell_dofs"][c, a] - 1] = u(x0 + nodes[a % 27] * np.array(r["h"]))[a // 27]
+    A = oracle.to_scipy(cp, rv, nz, r["nfree"])
+    assert np.max(np.abs(A @ uI - b)) < 1e-12 * max(1.0, np.abs(b).max())
+    r0 = O.elasticity_q2_box(oracle, part, np.ones(12), nu, f, [O.boundary_entities(3)])
+    b0 = oracle.assemble_vector(r0["cell_dofs"], r0["cellvecs"], r0["nfree"])
+    assert np.max(np.abs(A @ uI - b0)) > 1e-3           # the lifting matters

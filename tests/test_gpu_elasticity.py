@@ -1,0 +1,133 @@
+"""Parity of the vector-Q2 elasticity path with the FP64 oracle (needs an MI355X).
+
+Bar: CSC pattern bit-exact with the oracle's create_from_nz output on the same
+conforming cell_dof_ids; FP64 mode: nzval and b within 1e-12 relative (max-abs error /
+max-abs value); FP32 mode (element products on the matrix cores): relative Frobenius
+error <= 1e-6.  Size-independent checks: the quadratic patch identity A u_I = b,
+rigid-body modes in the kernel of the unconstrained matrix, FP32 vs FP64.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import gridap_hip as G
+from gridap_hip import _lib
+import conforming_oracle as CO
+
+pytestmark = pytest.mark.gpu
+
+CLAMP_X0 = ["tag_01", "tag_03", "tag_05", "tag_07", "tag_13", "tag_15", "tag_17", "tag_19",
+            "tag_25"]
+
+
+def announce(msg):
+    print("[gpu-case] " + msg, flush=True)
+    sys.stdout.flush()
+
+
+def relmax(a, b):
+    return np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300) if a.size else 0.0
+
+
+def relfrob(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def build(part, E, nu=0.3, force=(0.0, 0.0, -1.0), tags=CLAMP_X0, g=None, precision="fp64"):
+    model = G.CartesianDiscreteModel((0, 1, 0, 1, 0, 1), part)
+    reffe = G.ReferenceFE(G.lagrangian, G.VectorValue[3, G.Float64], 2)
+    V = G.TestFESpace(model, reffe, dirichlet_tags=list(tags))
+    U = G.TrialFESpace(V, g)
+    Ω = G.Triangulation(model)
+    dΩ = G.Measure(Ω, 4)
+    σ = G.IsotropicStress(G.CellField(E, Ω) if not np.isscalar(E) else E, nu)
+    a = lambda u, v: G.integrate(G.inner(G.ε(v), σ(G.ε(u))), dΩ)
+    l = lambda v: G.integrate(G.dot(v, G.VectorValue(*force)), dΩ)
+    assem = G.SparseMatrixAssembler(U, V, precision=precision)
+    return G.AffineFEOperator(a, l, U, V, assem), V
+
+
+def entities(tags):
+    if tags == ["boundary"]:
+        return CO.boundary_entities(3)
+    return {int(t.split("_")[1]) for t in tags}
+
+
+def oracle_system(oracle, part, E, nu, force, tags, g=None):
+    r = CO.elasticity_q2_box(oracle, part, E, nu, np.asarray(force), [entities(list(tags))], g=g)
+    cp, rv, nz = oracle.assemble_csc(r["cell_dofs"], r["cellmats"], r["nfree"], r["nfree"])
+    b = oracle.assemble_vector(r["cell_dofs"], r["cellvecs"], r["nfree"])
+    return r, cp, rv, nz, b
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_elasticity_matches_oracle(oracle, precision):
+    part = (3, 2, 2)
+    announce("elasticity Q2 %s %s clamped x=0" % (part, precision))
+    E = np.exp(np.random.default_rng(3).normal(0, 0.5, 12))
+    op, V = build(part, E, precision=precision)
+    r, cp, rv, nz, b = oracle_system(oracle, part, E, 0.3, (0, 0, -1), CLAMP_X0)
+    np.testing.assert_array_equal(V.cell_dof_ids, r["cell_dofs"])
+    A = op.get_matrix()
+    np.testing.assert_array_equal(A.colptr, cp)
+    np.testing.assert_array_equal(A.rowval, rv)
+    if precision == "fp64":
+        assert relmax(A.nzval, nz) <= 1e-12
+        assert relmax(op.get_vector(), b) <= 1e-12
+    else:
+        assert relfrob(A.nzval, nz) <= 1e-6
+        assert relfrob(op.get_vector(), b) <= 1e-6
+
+
+def test_elasticity_quadratic_patch_and_solve(oracle):
+    part = (3, 3, 2)
+    announce("elasticity Q2 quadratic patch + lifting + PCG %s" % (part,))
+    nu = 0.3
+    lam, mu = CO.lame(1.0, nu)
+    f = -np.array([lam + mu, lam + mu, lam + 3 * mu])
+    u = lambda x: np.stack([x[..., 0] * x[..., 1], x[..., 1] * x[..., 2],
+                            x[..., 2] * x[..., 0] + x[..., 0] ** 2], axis=-1)
+    op, V = build(part, 1.0, nu, f, ["boundary"], g=u)
+    uI = u(V.free_dof_coords)[np.arange(V.nfree), V.free_dof_comp]
+    A, b = op.get_matrix().to_scipy(), op.get_vector()
+    assert np.max(np.abs(A @ uI - b)) <= 1e-12 * np.max(np.abs(b))
+    r, cp, rv, nz, bo = oracle_system(oracle, part, np.ones(18), nu, f, ["boundary"],
+                                      g=lambda x: u(np.asarray(x)))
+    assert relmax(b, bo) <= 1e-12
+    x = G.solve(op).get_free_dof_values()
+    assert np.max(np.abs(x - uI)) <= 1e-8
+
+
+def test_elasticity_is_deterministic():
+    announce("elasticity Q2 repeat bit-identical (4^3)")
+    E = np.exp(np.random.default_rng(4).normal(0, 0.5, 64))
+    op, V = build((4, 4, 4), E, precision="fp32")
+    ctx = op.assem.ctx
+    nz1, b1 = ctx.values_get()
+    ctx.assemble(_lib.GH_FORM_ELASTICITY, op.assem.precision, op.assem.path)
+    nz2, b2 = ctx.values_get()
+    np.testing.assert_array_equal(nz1, nz2)
+    np.testing.assert_array_equal(b1, b2)
+
+
+def test_elasticity_fp32_vs_fp64_and_rigid_modes():
+    part = (12, 12, 12)
+    announce("elasticity Q2 %s FP32 vs FP64, rigid modes" % (part,))
+    E = np.exp(np.random.default_rng(12).normal(0, 0.5, 12 ** 3))
+    op64, V = build(part, E, precision="fp64")
+    op32, _ = build(part, E, precision="fp32")
+    assert relfrob(op32.get_matrix().nzval, op64.get_matrix().nzval) <= 1e-6
+    # without Dirichlet tags the six rigid motions span the kernel of A
+    opn, Vn = build(part, E, tags=[], precision="fp32")
+    xyz = Vn.free_dof_coords
+    comp = Vn.free_dof_comp
+    A = opn.get_matrix()
+    Ad = opn.assem.ctx
+    scale = np.abs(A.nzval).max()
+    for k in range(3):
+        t = (comp == k).astype(np.float64)
+        assert np.abs(Ad.spmv(t)).max() <= 1e-5 * scale
+    for a, b in ((0, 1), (1, 2), (2, 0)):
+        rot = np.where(comp == a, -xyz[:, b], 0.0) + np.where(comp == b, xyz[:, a], 0.0)
+        assert np.abs(Ad.spmv(rot)).max() <= 1e-5 * scale
