@@ -94,6 +94,9 @@ def main():
     ap.add_argument("--no-p2", action="store_true",
                     help="skip the P2 tetrahedral assembly leg (96^3 jittered, 6 tets/hex)")
     ap.add_argument("--p2-n", type=int, default=96)
+    ap.add_argument("--no-elasticity", action="store_true",
+                    help="skip the vector-Q2 elasticity leg (48^3 hexes)")
+    ap.add_argument("--elast-n", type=int, default=48)
     ap.add_argument("--seed", type=int, default=2024)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -232,7 +235,79 @@ def main():
         out["pcg_learned_fsai"] = fsai_leg(args.fsai_instances)
     if not args.no_p2:
         out["p2_tet_assembly"] = p2_tet_leg(args.p2_n, ev)
+    if not args.no_elasticity:
+        out["q2_elasticity_assembly"] = elasticity_leg(args.elast_n, ev)
     print(json.dumps(out), flush=True)
+
+
+MFMA_FP32_PEAK = 157.3e12  # MI355X dense FP32 matrix, FLOP/s
+
+
+def elasticity_leg(n, ev, steps=5, warmup=1, seed=11):
+    """Numeric assembly of 3-D linear elasticity with vector Q2 on n^3 hexes: E_c =
+    exp(N(0, 0.5^2)) per cell, ν = 0.3, f = (0, 0, -1), x = 0 face clamped, degree-4
+    rule (27 points).  FP32 mode runs the element gradient products on the matrix
+    cores; the FP64 mode is assembled once for the relative Frobenius error."""
+    import gridap_hip as G
+    from gridap_hip import _lib
+    t = time.time()
+    model = G.CartesianDiscreteModel((0, 1, 0, 1, 0, 1), (n, n, n))
+    reffe = G.ReferenceFE(G.lagrangian, G.VectorValue[3, G.Float64], 2)
+    clamp = ["tag_01", "tag_03", "tag_05", "tag_07", "tag_13", "tag_15", "tag_17", "tag_19",
+             "tag_25"]
+    V = G.TestFESpace(model, reffe, dirichlet_tags=clamp)
+    C = model.num_cells()
+    E = np.exp(np.random.default_rng(seed).normal(0.0, 0.5, C))
+    q = G.Quadrature(3, 4)
+    shp, grad = reffe.tabulate(3, q.coords)
+    host_s = time.time() - t
+    ctx = _lib.Context(0)
+    ctx.mesh_cartesian(model.origin, model.sizes, model.partition)
+    cd = V.cell_dof_ids
+    ptr = (np.arange(cd.shape[0] + 1, dtype=np.int64) * cd.shape[1] + 1).astype(np.int32)
+    ctx.space_upload(_lib.GH_REFFE_QK, 2, 3, cd.ravel(), ptr, V.nfree, V.ndirichlet)
+    ctx.reffe_tabulation(q.weights, shp, grad)
+    t = time.time()
+    nnz = ctx.symbolic()
+    symbolic_s = time.time() - t
+    ctx.coef_upload(E, None, None)
+    ctx.elasticity_params(0.3, [0.0, 0.0, -1.0])
+    ctx.assemble(_lib.GH_FORM_ELASTICITY, _lib.GH_PREC_FP64, _lib.GH_PATH_AUTO)
+    nz64, b64 = ctx.values_get()
+    stream = ctx.stream()
+    for _ in range(warmup):
+        ctx.assemble(_lib.GH_FORM_ELASTICITY, _lib.GH_PREC_FP32_ELEM, _lib.GH_PATH_AUTO)
+    ev.sync(stream)
+    evs = ev.create(2)
+    ev.record(evs[0], stream)
+    for _ in range(steps):
+        ctx.assemble(_lib.GH_FORM_ELASTICITY, _lib.GH_PREC_FP32_ELEM, _lib.GH_PATH_AUTO)
+    ev.record(evs[1], stream)
+    ms = ev.elapsed_ms(evs[0], evs[1]) / steps
+    ev.destroy(evs)
+    nz32, b32 = ctx.values_get()
+    N = V.nfree
+    ctx.close()
+    flops = C * 27 * 2 * (6 * 6 * 81 + 81 * 6 * 81)      # dense Voigt BᵀDB count
+    algo = 8 * nnz + 4 * 81 * C + 8 * C + 8 * N
+    return {
+        "workload": "3-D linear elasticity, vector Q2 on %d^3 HEX, degree 4, E=exp(N(0,0.5^2)) "
+                    "per cell, nu=0.3, f=(0,0,-1), x=0 face clamped; FP32 element products "
+                    "on MFMA, FP64 nzval/b" % n,
+        "cells": C, "free_dofs": N, "nnz": int(nnz),
+        "ms_per_assembly": ms, "value": nnz / (ms * 1e-3), "unit": "NNZ/s",
+        "roofline_mfma_dense_count": {"bound": "mfma", "achieved": flops / (ms * 1e-3) / 1e12,
+                                      "peak": MFMA_FP32_PEAK / 1e12, "unit": "TFLOP/s",
+                                      "frac": flops / (ms * 1e-3) / MFMA_FP32_PEAK,
+                                      "flops_per_launch": flops},
+        "roofline_hbm": {"bound": "hbm", "achieved": algo / (ms * 1e-3) / 1e9,
+                         "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                         "frac": algo / (ms * 1e-3) / HBM_PEAK,
+                         "algorithmic_bytes_per_launch": algo},
+        "fp32_vs_fp64_rel_frobenius": float(np.linalg.norm(nz32 - nz64) / np.linalg.norm(nz64)),
+        "rhs_rel_diff": float(np.linalg.norm(b32 - b64) / np.linalg.norm(b64)),
+        "symbolic_s": symbolic_s, "host_numbering_s": host_s,
+    }
 
 
 def p2_tet_leg(n, ev, steps=10, warmup=2, seed=7):

@@ -6,9 +6,9 @@
 // 81x81 element matrix is built from six 27x27 gradient products
 //     P_ij[a][b] = Σ_q w_q |J_q| g_ai(q) g_bj(q),   g = ∂φ/∂x = J⁻ᵀ ∇̂φ,
 //     K[(a,i),(b,j)] = E_c (λ̂ P_ij[a][b] + μ̂ P_ji[a][b] + μ̂ δ_ij Σ_k P_kk[a][b]),
-// λ̂ = ν/((1+ν)(1-2ν)), μ̂ = 1/(2(1+ν)).  One wave per cell: the Jacobian and its
-// inverse at the 27 quadrature points are FP64 (the CartesianMap; the kernel takes
-// the per-point Jacobian, so a trilinear geometry is a change of that block); the products
+// λ̂ = ν/((1+ν)(1-2ν)), μ̂ = 1/(2(1+ν)).  One wave per cell: the CartesianMap
+// Jacobian is diagonal, so g = ĝ/h and w|J| = w·h0·h1·h2 are formed in FP64 from
+// the reference tabulation as the operands are loaded; the products
 // run on the matrix cores (v_mfma_f32_32x32x2f32, FP32) or in FP64 on the vector
 // units; the 3321 distinct entries of the symmetric matrix are written with
 // coalesced stores, and the sum/rhs passes of the element-list path assemble them.
@@ -41,46 +41,18 @@ struct HexArgs {
 
 template <class T>
 __global__ __launch_bounds__(64) void hex_elasticity_element_kernel(HexArgs A) {
-  __shared__ T g[kNq][kNn][3];   // physical gradients
-  __shared__ double wj[kNq];     // w_q |J_q|
   __shared__ T P[6][kNn][kNn + 1];
   const int lane = threadIdx.x;
   const int64_t cell = blockIdx.x;
   const double *rg = A.ref, *rw = A.ref + kNq * kNn * 3, *rs = rw + kNq;
-  // geometry: one lane per quadrature point
-  if (lane < kNq) {
-    const int q = lane;
-    double J[3][3];  // CartesianMap: J = diag(h) (src/Geometry/CartesianGrids.jl:259-269)
-#pragma unroll
-    for (int d = 0; d < 3; ++d)
-#pragma unroll
-      for (int k = 0; k < 3; ++k) J[d][k] = d == k ? A.h[d] : 0.0;
-    const double det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) -
-                       J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
-                       J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
-    double inv[3][3];  // J⁻¹
-    inv[0][0] = (J[1][1] * J[2][2] - J[1][2] * J[2][1]) / det;
-    inv[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) / det;
-    inv[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) / det;
-    inv[1][0] = (J[1][2] * J[2][0] - J[1][0] * J[2][2]) / det;
-    inv[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) / det;
-    inv[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) / det;
-    inv[2][0] = (J[1][0] * J[2][1] - J[1][1] * J[2][0]) / det;
-    inv[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) / det;
-    inv[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) / det;
-    wj[q] = rw[q] * fabs(det);
-    for (int a = 0; a < kNn; ++a) {
-      const double *gh = rg + (q * kNn + a) * 3;
-#pragma unroll
-      for (int k = 0; k < 3; ++k)  // g_k = Σ_d (J⁻¹)_dk ĝ_d
-        g[q][a][k] = (T)(inv[0][k] * gh[0] + inv[1][k] * gh[1] + inv[2][k] * gh[2]);
-    }
-  }
-  __syncthreads();
+  // CartesianMap: J = diag(h) at every point (src/Geometry/CartesianGrids.jl:259-269),
+  // so g = ∂φ/∂x = ĝ / h and w|J| = w h0 h1 h2 (FP64)
+  const double det = A.h[0] * A.h[1] * A.h[2];
+  const double ih[3] = {1.0 / A.h[0], 1.0 / A.h[1], 1.0 / A.h[2]};
   // m_a = Σ_q w |J| φ̂_a (FP64)
   if (lane < kNn) {
     double m = 0.0;
-    for (int q = 0; q < kNq; ++q) m += wj[q] * rs[q * kNn + lane];
+    for (int q = 0; q < kNq; ++q) m += rw[q] * det * rs[q * kNn + lane];
     A.mvec[cell * kNn + lane] = m;
   }
   static const int pi[6] = {0, 1, 2, 0, 0, 1}, pj[6] = {0, 1, 2, 1, 2, 2};
@@ -96,8 +68,9 @@ __global__ __launch_bounds__(64) void hex_elasticity_element_kernel(HexArgs A) {
       for (int s = 0; s < (kNq + 1) / 2; ++s) {
         const int q = 2 * s + kh;
         const bool ok = q < kNq && r < kNn;
-        const float av = ok ? (float)g[q][r][i] * (float)wj[q] : 0.0f;
-        const float bv = ok ? (float)g[q][r][j] : 0.0f;
+        const double *gq = rg + (q * kNn + r) * 3;
+        const float av = ok ? (float)(gq[i] * ih[i] * rw[q] * det) : 0.0f;
+        const float bv = ok ? (float)(gq[j] * ih[j]) : 0.0f;
         acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av, bv, acc, 0, 0, 0);
       }
 #pragma unroll
@@ -111,8 +84,9 @@ __global__ __launch_bounds__(64) void hex_elasticity_element_kernel(HexArgs A) {
       const int pp = e / (kNn * kNn), ab = e % (kNn * kNn), a = ab / kNn, b = ab % kNn;
       const int i = pi[pp], j = pj[pp];
       double s = 0.0;
-      for (int q = 0; q < kNq; ++q) s += wj[q] * (double)g[q][a][i] * (double)g[q][b][j];
-      P[pp][a][b] = (T)s;
+      for (int q = 0; q < kNq; ++q)
+        s += rw[q] * rg[(q * kNn + a) * 3 + i] * rg[(q * kNn + b) * 3 + j];
+      P[pp][a][b] = (T)(s * det * ih[i] * ih[j]);
     }
   }
   __syncthreads();
