@@ -103,11 +103,10 @@ struct Ctx {
   DBuf<int32_t> adjptr;             // nfree+1: cells of each free dof
   DBuf<int32_t> adj;                // (cell << 4 | local index), cells ascending per dof
   DBuf<int32_t> cstart;             // nnz+1: contribution range of each nonzero
-  DBuf<int32_t> perm;               // contributions: index into the per-cell symmetric entries
+  DBuf<int32_t> perm;               // contributions: emit index cell·nloc² + lj·nloc + li
   DBuf<uint16_t> dmask;             // per cell: bit b set when local dof b is Dirichlet
-  DBuf<unsigned char> elem;         // per cell: the nloc(nloc+1)/2 element entries (FP32/FP64)
+  DBuf<unsigned char> elem;         // per cell: the nloc x nloc element matrix (FP32/FP64)
   DBuf<double> fdet;                // simplices: f |det DF| per cell; hexes: Σ w|J| φ_a (C x 27)
-  DBuf<int16_t> hpairs, hsym;       // vector Q2: (p, q) of each symmetric entry; its index
   double nu = 0.3;                  // elasticity: Poisson ratio
   double body[3] = {0, 0, 0};       // elasticity: body force
   DBuf<double> sref;                // reference tensor Ĝ (6 x nloc x nloc) and m (nloc)

@@ -10,8 +10,9 @@
 // Jacobian is diagonal, so g = ĝ/h and w|J| = w·h0·h1·h2 are formed in FP64 from
 // the reference tabulation as the operands are loaded; the products
 // run on the matrix cores (v_mfma_f32_32x32x2f32, FP32) or in FP64 on the vector
-// units; the 3321 distinct entries of the symmetric matrix are written with
-// coalesced stores, and the sum/rhs passes of the element-list path assemble them.
+// units; the 81x81 matrix is written row-major with coalesced stores (a column's
+// entries from one cell are a contiguous row), and the sum/rhs passes of the
+// element-list path assemble them.
 #include <algorithm>
 #include <cmath>
 #include <vector>
@@ -24,7 +25,6 @@ namespace {
 constexpr int kNn = 27;                      // Q2 nodes
 constexpr int kNq = 27;                      // degree-4 tensor rule
 constexpr int kNloc = 3 * kNn;               // 81
-constexpr int kNsym = kNloc * (kNloc + 1) / 2;  // 3321
 
 typedef float v16f __attribute__((ext_vector_type(16)));
 
@@ -33,9 +33,8 @@ struct HexArgs {
   double h[3];
   const double *E;            // per cell (NULL: 1)
   const double *ref;          // ĝ[q][a][d] (27*27*3), w[q] (27), φ̂[q][a] (27*27)
-  const int16_t *pairs;       // kNsym x 2: (p, q) local dofs, p <= q
   double lam, mu;             // λ̂, μ̂
-  void *E_out;                // ncells x kNsym (T)
+  void *E_out;                // ncells x 81 x 81 (T)
   double *mvec;               // ncells x 27: Σ_q w |J| φ_a
 };
 
@@ -90,15 +89,15 @@ __global__ __launch_bounds__(64) void hex_elasticity_element_kernel(HexArgs A) {
     }
   }
   __syncthreads();
-  // symmetric entries of K (p <= q local dofs)
+  // the element matrix, row-major 81 x 81
   const double Ec = A.E ? A.E[cell] : 1.0;
   const T lam = (T)(Ec * A.lam), mu = (T)(Ec * A.mu);
-  T *out = reinterpret_cast<T *>(A.E_out) + cell * (int64_t)kNsym;
+  T *out = reinterpret_cast<T *>(A.E_out) + cell * (int64_t)(kNloc * kNloc);
   auto pidx = [](int i, int j) {  // slot of the product P_ij with i <= j
     return i == j ? i : (i == 0 ? (j == 1 ? 3 : 4) : 5);
   };
-  for (int e = lane; e < kNsym; e += 64) {
-    const int p = A.pairs[2 * e], qd = A.pairs[2 * e + 1];
+  for (int e = lane; e < kNloc * kNloc; e += 64) {
+    const int p = e / kNloc, qd = e - p * kNloc;
     const int a = p % kNn, i = p / kNn, b = qd % kNn, j = qd / kNn;
     // P_ij[a][b] and P_ji[a][b] = P_ij[b][a]; stored products have i <= j
     T pij, pji;
@@ -121,7 +120,7 @@ __global__ void hex_rhs_kernel(int64_t nfree, const int32_t *__restrict__ adjptr
                                double f0, double f1, double f2,
                                const uint16_t *__restrict__ dflag,
                                const int32_t *__restrict__ cell_dofs,
-                               const int16_t *__restrict__ symidx, const T *__restrict__ E,
+                               const T *__restrict__ E,
                                const double *__restrict__ dvals, double *__restrict__ b) {
   const int64_t jd = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (jd >= nfree) return;
@@ -135,8 +134,7 @@ __global__ void hex_rhs_kernel(int64_t nfree, const int32_t *__restrict__ adjptr
       for (int bb = 0; bb < kNloc; ++bb) {
         const int d = cell_dofs[(size_t)cell * kNloc + bb];
         if (d < 0) {
-          const int lo = min(a, bb), hi = max(a, bb);
-          v -= (double)E[(size_t)cell * kNsym + symidx[lo * kNloc + hi]] * dvals[-d - 1];
+          v -= (double)E[((size_t)cell * kNloc + a) * kNloc + bb] * dvals[-d - 1];
         }
       }
     }
@@ -167,32 +165,17 @@ void assemble_elasticity(Ctx &c, int precision) {
     c.sref.resize(ref.size());
     GH_CHECK_HIP(hipMemcpy(c.sref.get(), ref.data(), sizeof(double) * ref.size(),
                            hipMemcpyHostToDevice));
-    std::vector<int16_t> pairs(2 * kNsym), sym(kNloc * kNloc, -1);
-    int e = 0;
-    for (int p = 0; p < kNloc; ++p)
-      for (int q = p; q < kNloc; ++q, ++e) {
-        pairs[2 * e] = (int16_t)p;
-        pairs[2 * e + 1] = (int16_t)q;
-        sym[p * kNloc + q] = (int16_t)e;
-      }
-    c.hpairs.resize(pairs.size());
-    c.hsym.resize(sym.size());
-    GH_CHECK_HIP(hipMemcpy(c.hpairs.get(), pairs.data(), sizeof(int16_t) * pairs.size(),
-                           hipMemcpyHostToDevice));
-    GH_CHECK_HIP(hipMemcpy(c.hsym.get(), sym.data(), sizeof(int16_t) * sym.size(),
-                           hipMemcpyHostToDevice));
     c.sref_ready = true;
   }
   const bool f32 = precision == GH_PREC_FP32_ELEM;
   const size_t esz = f32 ? sizeof(float) : sizeof(double);
-  c.elem.resize((size_t)c.ncells * kNsym * esz);
+  c.elem.resize((size_t)c.ncells * kNloc * kNloc * esz);
   c.fdet.resize((size_t)c.ncells * kNn);
   HexArgs A{};
   A.ncells = c.ncells;
   for (int d = 0; d < 3; ++d) A.h[d] = c.sizes[d];
   A.E = c.have_kappa ? c.kappa.get() : nullptr;
   A.ref = c.sref.get();
-  A.pairs = c.hpairs.get();
   A.lam = c.nu / ((1.0 + c.nu) * (1.0 - 2.0 * c.nu));
   A.mu = 1.0 / (2.0 * (1.0 + c.nu));
   A.E_out = c.elem.get();
@@ -212,12 +195,12 @@ void assemble_elasticity(Ctx &c, int precision) {
   if (f32)
     hipLaunchKernelGGL(hex_rhs_kernel<float>, dim3(grid), dim3(256), 0, c.stream, c.nfree,
                        c.adjptr.get(), c.adj.get(), c.fdet.get(), c.body[0], c.body[1],
-                       c.body[2], c.dmask.get(), c.cell_dofs.get(), c.hsym.get(),
+                       c.body[2], c.dmask.get(), c.cell_dofs.get(),
                        (const float *)c.elem.get(), dv, c.b.get());
   else
     hipLaunchKernelGGL(hex_rhs_kernel<double>, dim3(grid), dim3(256), 0, c.stream, c.nfree,
                        c.adjptr.get(), c.adj.get(), c.fdet.get(), c.body[0], c.body[1],
-                       c.body[2], c.dmask.get(), c.cell_dofs.get(), c.hsym.get(),
+                       c.body[2], c.dmask.get(), c.cell_dofs.get(),
                        (const double *)c.elem.get(), dv, c.b.get());
   GH_CHECK_HIP(hipGetLastError());
 }

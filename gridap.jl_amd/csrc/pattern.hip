@@ -65,23 +65,6 @@ __global__ void contrib_ranges(int64_t n, const uint64_t *__restrict__ keys, uin
   if (keys[k] < invalid && (k + 1 == n || keys[k + 1] >= invalid)) cstart[nnz] = (int32_t)(k + 1);
 }
 
-// perm[k] = index of contribution k in the symmetric per-cell storage
-// (cell * nloc(nloc+1)/2 + packed (min, max) local pair).
-__global__ void contrib_perm(int64_t n, const uint64_t *__restrict__ keys, uint64_t invalid,
-                             const int32_t *__restrict__ vals, int nloc,
-                             int32_t *__restrict__ perm) {
-  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= n || keys[k] >= invalid) return;
-  const int64_t t = vals[k];
-  const int64_t per = (int64_t)nloc * nloc;
-  const int64_t c = t / per;
-  const int r = (int)(t - c * per);
-  const int x = r / nloc, y = r - x * nloc;
-  const int lo = min(x, y), hi = max(x, y);
-  const int nsym = nloc * (nloc + 1) / 2;
-  perm[k] = (int32_t)(c * nsym + lo * nloc - lo * (lo - 1) / 2 + (hi - lo));
-}
-
 __global__ void iota_kernel(int64_t n, int32_t *__restrict__ v) {
   int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k < n) v[k] = (int32_t)k;
@@ -309,14 +292,14 @@ void symbolic_general(Ctx &c, bool contributions) {
   if (contributions) {
     c.cstart.resize(nnz + 1);
     GH_CHECK_HIP(hipMemsetAsync(c.cstart.get(), 0, sizeof(int32_t) * (nnz + 1), s));
-    c.perm.resize(std::max<int64_t>(total, 1));
-    if (total) {
+    if (total)
       hipLaunchKernelGGL(contrib_ranges, dim3(grid_for(total)), dim3(kBlock), 0, s, total,
                          sorted.get(), invalid, flags.get(), pos.get(), nnz, c.cstart.get());
-      hipLaunchKernelGGL(contrib_perm, dim3(grid_for(total)), dim3(kBlock), 0, s, total,
-                         sorted.get(), invalid, svals.get(), c.nloc, c.perm.get());
-    }
     GH_CHECK_HIP(hipGetLastError());
+    // the emit index cell * nloc^2 + (column local) * nloc + (row local) addresses the
+    // row-major element matrices directly
+    std::swap(c.perm.p, svals.p);
+    std::swap(c.perm.n, svals.n);
   }
   GH_CHECK_HIP(hipStreamSynchronize(s));
   c.have_pattern = true;

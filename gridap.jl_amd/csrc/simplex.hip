@@ -14,8 +14,8 @@
 //
 // Three passes, each fully parallel and deterministic:
 //  * element: one thread per cell computes the metric (FP64); the block then writes
-//    the nloc(nloc+1)/2 distinct entries of every cell's symmetric element matrix
-//    with coalesced stores;
+//    every cell's element matrix (row-major, nloc x nloc) with coalesced stores, so
+//    the entries a column needs from a cell form one contiguous row;
 //  * sum: one thread per nonzero adds its entries in ascending cell order (the
 //    order of Gridap's cell loop), from the contribution list that the symbolic
 //    phase keeps (a stable sort of the (column, row) keys carries the cell index),
@@ -89,7 +89,7 @@ __device__ __forceinline__ int sym_index(int nloc, int a, int b) {
   return lo * nloc - lo * (lo - 1) / 2 + (hi - lo);
 }
 
-// Element entries of kBlock cells: E[c][e] = Σ_k g_k(c) Ĝ^k_e, g = κ|det| DF⁻¹DF⁻ᵀ.
+// Element matrices of kBlock cells: E[c][a][b] = Σ_k g_k(c) Ĝ^k_ab, g = κ|det| DF⁻¹DF⁻ᵀ.
 template <int NLOC, class T>
 __global__ __launch_bounds__(kBlock) void simplex_element_kernel(
     int64_t ncells, const int32_t *__restrict__ cell_nodes, const double *__restrict__ xyz,
@@ -146,9 +146,11 @@ __global__ __launch_bounds__(kBlock) void simplex_element_kernel(
   }
   __syncthreads();
   const int64_t ncb = min<int64_t>(kBlock, ncells - c0);
-  T *out = E + c0 * NS;
-  for (int idx = threadIdx.x; idx < ncb * NS; idx += kBlock) {
-    const int cl = idx / NS, ee = idx - cl * NS;
+  constexpr int NN = NLOC * NLOC;
+  T *out = E + c0 * NN;
+  for (int idx = threadIdx.x; idx < ncb * NN; idx += kBlock) {
+    const int cl = idx / NN, ab = idx - cl * NN;
+    const int ee = sym_index(NLOC, ab / NLOC, ab % NLOC);
     T v = (T)0;
 #pragma unroll
     for (int k = 0; k < 6; ++k) v += (T)sg[cl][k] * ref[k][ee];
@@ -188,7 +190,6 @@ __global__ void simplex_rhs_kernel(int64_t nfree, const int32_t *__restrict__ ad
                                    const int32_t *__restrict__ cell_dofs,
                                    const T *__restrict__ E, const double *__restrict__ dvals,
                                    double *__restrict__ b) {
-  constexpr int NS = NLOC * (NLOC + 1) / 2;
   const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= nfree) return;
   double s = 0.0;
@@ -201,7 +202,7 @@ __global__ void simplex_rhs_kernel(int64_t nfree, const int32_t *__restrict__ ad
       const int bb = __ffs(m) - 1;
       m &= m - 1;
       const int d = cell_dofs[(size_t)cell * NLOC + bb];
-      v -= (double)E[(size_t)cell * NS + sym_index(NLOC, a, bb)] * dvals[-d - 1];
+      v -= (double)E[((size_t)cell * NLOC + a) * NLOC + bb] * dvals[-d - 1];
     }
     s += v;
   }
@@ -210,8 +211,7 @@ __global__ void simplex_rhs_kernel(int64_t nfree, const int32_t *__restrict__ ad
 
 template <int NLOC, class T>
 void launch_simplex(Ctx &c) {
-  constexpr int NS = NLOC * (NLOC + 1) / 2;
-  c.elem.resize((size_t)c.ncells * NS * sizeof(T));
+  c.elem.resize((size_t)c.ncells * NLOC * NLOC * sizeof(T));
   T *E = reinterpret_cast<T *>(c.elem.get());
   hipLaunchKernelGGL((simplex_element_kernel<NLOC, T>), dim3(grid_for(c.ncells)), dim3(kBlock),
                      0, c.stream, c.ncells, c.cell_nodes.get(), c.xyz.get(),
