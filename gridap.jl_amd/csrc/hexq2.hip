@@ -3,14 +3,15 @@
 //     a(u,v) = ∫ ε(v) : σ(ε(u)),  σ = λ tr(ε) I + 2 μ ε,  λ, μ from E_c and ν,
 //     l(v)   = ∫ v · f  (f constant),
 // with Gridap's component-blocked local dofs (node + 27 comp).  For isotropic σ the
-// 81x81 element matrix is built from six 27x27 gradient products
+// 81x81 element matrix is built from the 27x27 gradient products
 //     P_ij[a][b] = Σ_q w_q |J_q| g_ai(q) g_bj(q),   g = ∂φ/∂x = J⁻ᵀ ∇̂φ,
 //     K[(a,i),(b,j)] = E_c (λ̂ P_ij[a][b] + μ̂ P_ji[a][b] + μ̂ δ_ij Σ_k P_kk[a][b]),
 // λ̂ = ν/((1+ν)(1-2ν)), μ̂ = 1/(2(1+ν)).  One wave per cell: the CartesianMap
 // Jacobian is diagonal, so g = ĝ/h and w|J| = w·h0·h1·h2 are formed in FP64 from
-// the reference tabulation as the operands are loaded; the products
-// run on the matrix cores (v_mfma_f32_32x32x2f32, FP32) or in FP64 on the vector
-// units; the 81x81 matrix is written row-major with coalesced stores (a column's
+// the reference tabulation as the operands are loaded; the nine products run on
+// the matrix cores (v_mfma_f32_32x32x2f32, FP32) and the blocks are formed and
+// stored from the accumulators, or the six distinct ones in FP64 on the vector
+// units; the 81x81 matrix is written row-major (a column's
 // entries from one cell are a contiguous row), and the sum/rhs passes of the
 // element-list path assemble them.
 #include <algorithm>
@@ -40,7 +41,6 @@ struct HexArgs {
 
 template <class T>
 __global__ __launch_bounds__(64) void hex_elasticity_element_kernel(HexArgs A) {
-  __shared__ T P[6][kNn][kNn + 1];
   const int lane = threadIdx.x;
   const int64_t cell = blockIdx.x;
   const double *rg = A.ref, *rw = A.ref + kNq * kNn * 3, *rs = rw + kNq;
@@ -54,63 +54,94 @@ __global__ __launch_bounds__(64) void hex_elasticity_element_kernel(HexArgs A) {
     for (int q = 0; q < kNq; ++q) m += rw[q] * det * rs[q * kNn + lane];
     A.mvec[cell * kNn + lane] = m;
   }
-  static const int pi[6] = {0, 1, 2, 0, 0, 1}, pj[6] = {0, 1, 2, 1, 2, 2};
+  const double Ec = A.E ? A.E[cell] : 1.0;
+  T *out = reinterpret_cast<T *>(A.E_out) + cell * (int64_t)(kNloc * kNloc);
   if constexpr (sizeof(T) == 4) {
-    // P_ij = Aw_i G_jᵀ on the matrix cores: A[row a][k q] = g[q][a][i] w|J|_q,
-    // B[k q][col b] = g[q][b][j]; lane l: row/col l % 32, k = 2 s + l / 32
+    // All nine products P_ij = Aw_i G_jᵀ on the matrix cores (A[row a][k q] =
+    // g_i(q, a) w|J|_q, B[k q][col b] = g_j(q, b); lane l: row/col l % 32,
+    // k = 2 s + l / 32), so that P_ji comes out in the same lane layout as P_ij and
+    // each block K_ij = λ̂ P_ij + μ̂ P_ji (+ μ̂ Σ_k P_kk on the diagonal) is formed in
+    // registers and stored straight from the accumulator layout.
     const int r = lane & 31, kh = lane >> 5;
+    constexpr int S = (kNq + 1) / 2;
+    float av[S][3], bv[S][3];
 #pragma unroll
-    for (int pp = 0; pp < 6; ++pp) {
-      const int i = pi[pp], j = pj[pp];
-      v16f acc = {0};
+    for (int s = 0; s < S; ++s) {
+      const int q = 2 * s + kh;
+      const bool ok = q < kNq && r < kNn;
 #pragma unroll
-      for (int s = 0; s < (kNq + 1) / 2; ++s) {
-        const int q = 2 * s + kh;
-        const bool ok = q < kNq && r < kNn;
-        const double *gq = rg + (q * kNn + r) * 3;
-        const float av = ok ? (float)(gq[i] * ih[i] * rw[q] * det) : 0.0f;
-        const float bv = ok ? (float)(gq[j] * ih[j]) : 0.0f;
-        acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av, bv, acc, 0, 0, 0);
-      }
-#pragma unroll
-      for (int t = 0; t < 16; ++t) {
-        const int row = (t / 4) * 8 + kh * 4 + (t % 4);
-        if (row < kNn && r < kNn) P[pp][row][r] = (T)acc[t];
+      for (int k = 0; k < 3; ++k) {
+        const double gq = ok ? rg[(q * kNn + r) * 3 + k] * ih[k] : 0.0;
+        bv[s][k] = (float)gq;
+        av[s][k] = ok ? (float)(gq * rw[q] * det) : 0.0f;
       }
     }
+    auto prod = [&](int i, int j) {
+      v16f acc = v16f{0};
+#pragma unroll
+      for (int s = 0; s < S; ++s)
+        acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av[s][i], bv[s][j], acc, 0, 0, 0);
+      return acc;
+    };
+    const float lam = (float)(Ec * A.lam), mu = (float)(Ec * A.mu);
+    auto store = [&](int i, int j, const v16f &x, const v16f &y, const v16f *tr) {
+      if (r >= kNn) return;
+#pragma unroll
+      for (int t = 0; t < 16; ++t) {
+        const int a = (t / 4) * 8 + kh * 4 + (t % 4);
+        if (a >= kNn) continue;
+        float v = lam * x[t] + mu * y[t];
+        if (tr) v += mu * (*tr)[t];
+        out[(a + kNn * i) * kNloc + r + kNn * j] = v;
+      }
+    };
+    {  // diagonal blocks: λ̂ P_ii + μ̂ P_ii + μ̂ Σ_k P_kk
+      const v16f d0 = prod(0, 0), d1 = prod(1, 1), d2 = prod(2, 2);
+      v16f tr;
+#pragma unroll
+      for (int t = 0; t < 16; ++t) tr[t] = d0[t] + d1[t] + d2[t];
+      store(0, 0, d0, d0, &tr);
+      store(1, 1, d1, d1, &tr);
+      store(2, 2, d2, d2, &tr);
+    }
+#pragma unroll
+    for (int pr = 0; pr < 3; ++pr) {  // off-diagonal pairs (i, j), i < j
+      const int i = pr == 2 ? 1 : 0, j = pr == 0 ? 1 : 2;
+      const v16f x = prod(i, j), y = prod(j, i);
+      store(i, j, x, y, nullptr);
+      store(j, i, y, x, nullptr);
+    }
   } else {
+    __shared__ double P[6][kNn][kNn + 1];
+    static const int pi[6] = {0, 1, 2, 0, 0, 1}, pj[6] = {0, 1, 2, 1, 2, 2};
     for (int e = lane; e < 6 * kNn * kNn; e += 64) {
       const int pp = e / (kNn * kNn), ab = e % (kNn * kNn), a = ab / kNn, b = ab % kNn;
       const int i = pi[pp], j = pj[pp];
       double s = 0.0;
       for (int q = 0; q < kNq; ++q)
         s += rw[q] * rg[(q * kNn + a) * 3 + i] * rg[(q * kNn + b) * 3 + j];
-      P[pp][a][b] = (T)(s * det * ih[i] * ih[j]);
+      P[pp][a][b] = s * det * ih[i] * ih[j];
     }
-  }
-  __syncthreads();
-  // the element matrix, row-major 81 x 81
-  const double Ec = A.E ? A.E[cell] : 1.0;
-  const T lam = (T)(Ec * A.lam), mu = (T)(Ec * A.mu);
-  T *out = reinterpret_cast<T *>(A.E_out) + cell * (int64_t)(kNloc * kNloc);
-  auto pidx = [](int i, int j) {  // slot of the product P_ij with i <= j
-    return i == j ? i : (i == 0 ? (j == 1 ? 3 : 4) : 5);
-  };
-  for (int e = lane; e < kNloc * kNloc; e += 64) {
-    const int p = e / kNloc, qd = e - p * kNloc;
-    const int a = p % kNn, i = p / kNn, b = qd % kNn, j = qd / kNn;
-    // P_ij[a][b] and P_ji[a][b] = P_ij[b][a]; stored products have i <= j
-    T pij, pji;
-    if (i <= j) {
-      pij = P[pidx(i, j)][a][b];
-      pji = P[pidx(i, j)][b][a];
-    } else {
-      pij = P[pidx(j, i)][b][a];
-      pji = P[pidx(j, i)][a][b];
+    __syncthreads();
+    const double lam = Ec * A.lam, mu = Ec * A.mu;
+    auto pidx = [](int i, int j) {  // slot of the product P_ij with i <= j
+      return i == j ? i : (i == 0 ? (j == 1 ? 3 : 4) : 5);
+    };
+    for (int e = lane; e < kNloc * kNloc; e += 64) {
+      const int p = e / kNloc, qd = e - p * kNloc;
+      const int a = p % kNn, i = p / kNn, b = qd % kNn, j = qd / kNn;
+      double pij, pji;  // P_ji[a][b] = P_ij[b][a]; stored products have i <= j
+      if (i <= j) {
+        pij = P[pidx(i, j)][a][b];
+        pji = P[pidx(i, j)][b][a];
+      } else {
+        pij = P[pidx(j, i)][b][a];
+        pji = P[pidx(j, i)][a][b];
+      }
+      double v = lam * pij + mu * pji;
+      if (i == j) v += mu * (P[0][a][b] + P[1][a][b] + P[2][a][b]);
+      out[e] = (T)v;
     }
-    T v = lam * pij + mu * pji;
-    if (i == j) v += mu * (P[0][a][b] + P[1][a][b] + P[2][a][b]);
-    out[e] = v;
   }
 }
 

@@ -82,6 +82,95 @@ __global__ void dirichlet_mask_kernel(int64_t ncells, int nloc,
   dmask[c] = (uint16_t)m;
 }
 
+// ---- compact contribution lists ---------------------------------------------------
+// The column sum runs one block per range of NC consecutive columns.  A block's
+// nonzeros, its contributions (sorted by (column, row), so in nonzero order) and its
+// columns' adjacency entries are then three contiguous ranges, and 16 bits address
+// each of them relative to the range start:
+//   slot of a contribution  = (adjacency position − block's first) << obits | offset,
+//                             the entry being E[base(cell, lj) + offset];
+//   noff of a nonzero       = its first contribution − block's first.
+
+// out[0] = most adjacency entries of a block, out[1] = most contributions of a block
+__global__ void block_extent_kernel(int64_t nblocks, int64_t nfree, int nc,
+                                    const int32_t *__restrict__ adjptr,
+                                    const int32_t *__restrict__ colptr,
+                                    const int32_t *__restrict__ cstart,
+                                    int32_t *__restrict__ out) {
+  const int64_t bk = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (bk >= nblocks) return;
+  const int64_t j0 = bk * nc, j1 = min<int64_t>(j0 + nc, nfree);
+  atomicMax(out, adjptr[j1] - adjptr[j0]);
+  atomicMax(out + 1, cstart[colptr[j1]] - cstart[colptr[j0]]);
+}
+
+// Contribution q (emit index t = cell·nloc² + lj·nloc + li) of column J: the position
+// of (cell, lj) in the adjacency of J's block and the entry's offset from the
+// cell-column base, (cell·nloc + lj)·nloc (full rows, offset li) or cell·ns (packed
+// upper triangle, offset of (min, max) of li, lj).
+// With ipos (full rows): instead of the slot, the contribution's position in its
+// block's run, stored at the entry's place (adjacency position · nloc + li).
+__global__ void contrib_slot_kernel(int64_t ncontrib, int nc, int nloc, int lbits, int obits,
+                                    int ns, const int32_t *__restrict__ perm,
+                                    const int32_t *__restrict__ cell_dofs,
+                                    const int32_t *__restrict__ adjptr,
+                                    const int32_t *__restrict__ adj,
+                                    const int32_t *__restrict__ colptr,
+                                    const int32_t *__restrict__ cstart,
+                                    uint16_t *__restrict__ cslot, uint16_t *__restrict__ ipos,
+                                    int32_t *__restrict__ bad) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= ncontrib) return;
+  const int64_t t = perm[q];
+  const int64_t nn = (int64_t)nloc * nloc;
+  const int64_t cell = t / nn;
+  const int r = (int)(t - cell * nn);
+  const int lj = r / nloc, li = r - lj * nloc;
+  const int J = cell_dofs[cell * nloc + lj] - 1;
+  int lo = adjptr[J], hi = adjptr[J + 1] - 1;
+  const int32_t want = (int32_t)((cell << lbits) | lj);
+  while (lo < hi) {  // first entry >= want
+    const int mid = (lo + hi) >> 1;
+    if (adj[mid] < want) lo = mid + 1;
+    else hi = mid;
+  }
+  if (adj[lo] != want) {
+    atomicOr(bad, 1);
+    return;
+  }
+  int off = li;
+  if (ns) {
+    const int a = min(li, lj), b = max(li, lj);
+    off = a * nloc - a * (a - 1) / 2 + (b - a);
+  }
+  const int J0 = J - J % nc;
+  if (ipos) {
+    ipos[(int64_t)lo * nloc + li] = (uint16_t)(q - cstart[colptr[J0]]);
+    return;
+  }
+  const int k = lo - adjptr[J0];
+  if (((int64_t)k << obits) + off > 65535) atomicOr(bad, 2);
+  cslot[q] = (uint16_t)((k << obits) | off);
+}
+
+// One thread per column: noff of its nonzeros, relative to its block's first entry.
+__global__ void nonzero_offset_kernel(int64_t nfree, int nc, const int32_t *__restrict__ colptr,
+                                      const int32_t *__restrict__ cstart,
+                                      uint16_t *__restrict__ noff) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nfree) return;
+  const int32_t q0 = cstart[colptr[j - j % nc]];
+  for (int32_t p = colptr[j]; p < colptr[j + 1]; ++p) noff[p] = (uint16_t)(cstart[p] - q0);
+}
+
+__global__ void block_base_kernel(int64_t nblocks, int64_t nfree, int nc,
+                                  const int32_t *__restrict__ colptr,
+                                  const int32_t *__restrict__ cstart,
+                                  int32_t *__restrict__ bbase) {
+  const int64_t bk = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (bk <= nblocks) bbase[bk] = cstart[colptr[min<int64_t>(bk * nc, nfree)]];
+}
+
 // ---- numeric -----------------------------------------------------------------------
 
 __device__ __forceinline__ int sym_index(int nloc, int a, int b) {
@@ -89,8 +178,9 @@ __device__ __forceinline__ int sym_index(int nloc, int a, int b) {
   return lo * nloc - lo * (lo - 1) / 2 + (hi - lo);
 }
 
-// Element matrices of kBlock cells: E[c][a][b] = Σ_k g_k(c) Ĝ^k_ab, g = κ|det| DF⁻¹DF⁻ᵀ.
-template <int NLOC, class T>
+// Element matrices of kBlock cells: E[c][a][b] = Σ_k g_k(c) Ĝ^k_ab, g = κ|det| DF⁻¹DF⁻ᵀ,
+// as full rows or (PACKED) as the upper triangle, row by row.
+template <int NLOC, class T, bool PACKED>
 __global__ __launch_bounds__(kBlock) void simplex_element_kernel(
     int64_t ncells, const int32_t *__restrict__ cell_nodes, const double *__restrict__ xyz,
     const double *__restrict__ kappa, const double *__restrict__ fsrc,
@@ -146,11 +236,11 @@ __global__ __launch_bounds__(kBlock) void simplex_element_kernel(
   }
   __syncthreads();
   const int64_t ncb = min<int64_t>(kBlock, ncells - c0);
-  constexpr int NN = NLOC * NLOC;
+  constexpr int NN = PACKED ? NS : NLOC * NLOC;
   T *out = E + c0 * NN;
   for (int idx = threadIdx.x; idx < ncb * NN; idx += kBlock) {
     const int cl = idx / NN, ab = idx - cl * NN;
-    const int ee = sym_index(NLOC, ab / NLOC, ab % NLOC);
+    const int ee = PACKED ? ab : sym_index(NLOC, ab / NLOC, ab % NLOC);
     T v = (T)0;
 #pragma unroll
     for (int k = 0; k < 6; ++k) v += (T)sg[cl][k] * ref[k][ee];
@@ -181,7 +271,124 @@ __global__ void simplex_sum_kernel(int64_t nnz, const int32_t *__restrict__ csta
   nzval[p] = s;
 }
 
-template <int NLOC, class T>
+// One block per range of NC consecutive columns [J0, J1), in two phases:
+//  * stage: every contribution of the range (a contiguous run) is read from the
+//    element matrices into LDS at its run position;
+//  * sum: one thread per nonzero of the range adds its entries, noff[p] up to the
+//    next nonzero's, in ascending cell order in FP64 and writes nzval once.
+// Bit-identical to simplex_sum_kernel (same entries, same order).  The loads that
+// do not depend on each other (slots or entry positions, adjacency, the first R
+// nonzero offsets per thread) are all issued before the first wait, so a block
+// costs about three memory round trips.
+// Packed storage (FULL_NLOC = 0): the bases of the range's cell-columns go to LDS and
+// each contribution gathers its entry through its slot.  Full rows (FULL_NLOC = nloc):
+// the block walks its cell-columns entry by entry with contiguous reads and writes
+// each entry that is a contribution to its run position (cslot holds those positions,
+// 0xffff for rows that are not free).
+// LDS: maxbase bases, then maxq entries (the largest extents over all blocks).
+template <class T, int FULL_NLOC>
+__global__ __launch_bounds__(kBlock) void column_sum_kernel(
+    int64_t nfree, int nc, int nloc, int lbits, int obits, int ns, int maxbase,
+    const int32_t *__restrict__ colptr, const int32_t *__restrict__ bbase,
+    const int32_t *__restrict__ adjptr, const int32_t *__restrict__ adj,
+    const uint16_t *__restrict__ noff, const uint16_t *__restrict__ cslot,
+    const T *__restrict__ E, double *__restrict__ nzval) {
+  extern __shared__ __align__(16) uint32_t shm[];
+  constexpr int U = 12;  // staged entries per thread and batch
+  constexpr int R = 8;   // nonzeros per thread whose offsets are prefetched
+  uint32_t *bases = shm;
+  T *vals = reinterpret_cast<T *>(shm + ((maxbase + 3) & ~3));
+  const int64_t j0 = (int64_t)blockIdx.x * nc, j1 = min<int64_t>(j0 + nc, nfree);
+  const int a0 = adjptr[j0], na = adjptr[j1] - a0;
+  const int32_t q0 = bbase[blockIdx.x], nq = bbase[blockIdx.x + 1] - q0;
+  const int32_t p0 = colptr[j0], np = colptr[j1] - p0;
+  const int tid = threadIdx.x;
+  const int lmask = (1 << lbits) - 1;
+  // nonzero offsets of the first R rounds (start and end of each)
+  uint32_t ost[R], oen[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int i = tid + r * kBlock;
+    ost[r] = i < np ? noff[p0 + i] : 0u;
+    oen[r] = i + 1 < np ? noff[p0 + i + 1] : (uint32_t)nq;
+  }
+  if constexpr (FULL_NLOC > 0) {
+    const uint16_t *ip = cslot + (int64_t)a0 * FULL_NLOC;
+    const int ne = na * FULL_NLOC;
+    for (int eb = 0; eb < ne; eb += kBlock * U) {
+      uint32_t pos[U];
+      int32_t ae[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int e = eb + tid + u * kBlock;
+        pos[u] = e < ne ? ip[e] : 0xffffu;
+        ae[u] = e < ne ? adj[a0 + e / FULL_NLOC] : 0;
+      }
+      T v[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int e = eb + tid + u * kBlock;
+        const int li = e - (e / FULL_NLOC) * FULL_NLOC;
+        const uint32_t cell = (uint32_t)(ae[u] >> lbits), lj = (uint32_t)(ae[u] & lmask);
+        v[u] = pos[u] != 0xffffu
+                   ? E[(cell * (uint32_t)FULL_NLOC + lj) * (uint32_t)FULL_NLOC + li]
+                   : (T)0;
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (pos[u] != 0xffffu) vals[pos[u]] = v[u];
+    }
+  } else {
+    const int omask = (1 << obits) - 1;
+    uint32_t sl[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int q = tid + u * kBlock;
+      sl[u] = q < nq ? cslot[q0 + q] : 0xffffffffu;
+    }
+    for (int k = tid; k < na; k += kBlock) {
+      const int e = adj[a0 + k];
+      const uint32_t cell = (uint32_t)(e >> lbits), lj = (uint32_t)(e & lmask);
+      bases[k] = ns ? cell * (uint32_t)ns : (cell * (uint32_t)nloc + lj) * (uint32_t)nloc;
+    }
+    __syncthreads();
+    for (int qb = 0; qb < nq; qb += kBlock * U) {
+      if (qb) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int q = qb + tid + u * kBlock;
+          sl[u] = q < nq ? cslot[q0 + q] : 0xffffffffu;
+        }
+      }
+      T v[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        v[u] = sl[u] != 0xffffffffu ? E[bases[sl[u] >> obits] + (sl[u] & omask)] : (T)0;
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (sl[u] != 0xffffffffu) vals[qb + tid + u * kBlock] = v[u];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int i = tid + r * kBlock;
+    if (i < np) {
+      double s = 0.0;
+      for (uint32_t k = ost[r]; k < oen[r]; ++k) s += (double)vals[k];
+      nzval[p0 + i] = s;
+    }
+  }
+  for (int i = tid + R * kBlock; i < np; i += kBlock) {
+    const int st = noff[p0 + i];
+    const int en = i + 1 < np ? (int)noff[p0 + i + 1] : nq;
+    double s = 0.0;
+    for (int k = st; k < en; ++k) s += (double)vals[k];
+    nzval[p0 + i] = s;
+  }
+}
+
+template <int NLOC, class T, bool PACKED>
 __global__ void simplex_rhs_kernel(int64_t nfree, const int32_t *__restrict__ adjptr,
                                    const int32_t *__restrict__ adj,
                                    const double *__restrict__ fdet,
@@ -202,23 +409,26 @@ __global__ void simplex_rhs_kernel(int64_t nfree, const int32_t *__restrict__ ad
       const int bb = __ffs(m) - 1;
       m &= m - 1;
       const int d = cell_dofs[(size_t)cell * NLOC + bb];
-      v -= (double)E[((size_t)cell * NLOC + a) * NLOC + bb] * dvals[-d - 1];
+      const size_t ee = PACKED ? (size_t)cell * (NLOC * (NLOC + 1) / 2) + sym_index(NLOC, a, bb)
+                               : ((size_t)cell * NLOC + a) * NLOC + bb;
+      v -= (double)E[ee] * dvals[-d - 1];
     }
     s += v;
   }
   b[j] = s;
 }
 
-template <int NLOC, class T>
+template <int NLOC, class T, bool PACKED>
 void launch_simplex(Ctx &c) {
-  c.elem.resize((size_t)c.ncells * NLOC * NLOC * sizeof(T));
+  constexpr int NN = PACKED ? NLOC * (NLOC + 1) / 2 : NLOC * NLOC;
+  c.elem.resize((size_t)c.ncells * NN * sizeof(T));
   T *E = reinterpret_cast<T *>(c.elem.get());
-  hipLaunchKernelGGL((simplex_element_kernel<NLOC, T>), dim3(grid_for(c.ncells)), dim3(kBlock),
-                     0, c.stream, c.ncells, c.cell_nodes.get(), c.xyz.get(),
+  hipLaunchKernelGGL((simplex_element_kernel<NLOC, T, PACKED>), dim3(grid_for(c.ncells)),
+                     dim3(kBlock), 0, c.stream, c.ncells, c.cell_nodes.get(), c.xyz.get(),
                      c.have_kappa ? c.kappa.get() : nullptr,
                      c.have_fsrc ? c.fsrc.get() : nullptr, c.sref.get(), E, c.fdet.get());
   elemlist_sum(c, sizeof(T) == 4);
-  hipLaunchKernelGGL((simplex_rhs_kernel<NLOC, T>), dim3(grid_for(c.nfree)), dim3(kBlock), 0,
+  hipLaunchKernelGGL((simplex_rhs_kernel<NLOC, T, PACKED>), dim3(grid_for(c.nfree)), dim3(kBlock), 0,
                      c.stream, c.nfree, c.adjptr.get(), c.adj.get(), c.fdet.get(),
                      c.sref.get() + 6 * NLOC * NLOC, c.dmask.get(), c.cell_dofs.get(),
                      (const T *)E, c.have_dvals ? c.dvals.get() : nullptr, c.b.get());
@@ -230,6 +440,98 @@ void launch_simplex(Ctx &c) {
 bool simplex_eligible(const Ctx &c) {
   return c.dim == 3 && !c.cartesian && c.nv == 4 && c.reffe_kind == GH_REFFE_PK &&
          c.ncomp == 1 && (c.nloc == 4 || c.nloc == 10);
+}
+
+// Replace cstart/perm by the compact lists when they fit: 16-bit slots and offsets
+// within a block's ranges, and element storage addressable in 32 bits.  Simplices
+// then store packed upper triangles.  NC is the largest power of two whose blocks gather
+// at most about kBlock·12 entries on average; it halves until the block's extents fit 16 bits and the LDS.
+constexpr size_t kMaxColumnLds = 48 * 1024;
+
+size_t column_sum_lds(int maxbase, int maxq, size_t esz) {
+  return sizeof(uint32_t) * ((std::max(maxbase, 1) + 3) & ~3) + esz * std::max(maxq, 1);
+}
+
+void compact_contributions(Ctx &c, int lbits) {
+  c.slots = false;
+  c.packed = false;
+  c.cslot.release();
+  c.noff.release();
+  c.bbase.release();
+  if (!c.nnz || !c.cstart.get() || !c.perm.get()) return;
+  hipStream_t s = c.stream;
+  int32_t ncontrib = 0;
+  GH_CHECK_HIP(hipMemcpy(&ncontrib, c.cstart.get() + c.nnz, 4, hipMemcpyDeviceToHost));
+  const bool packed = c.nloc <= 16;
+  if (!packed && c.nloc != 81) return;  // full rows: the staging kernel is built for 81
+  const int ns = packed ? c.nloc * (c.nloc + 1) / 2 : 0;
+  int obits = 1;
+  while ((1 << obits) < (packed ? ns : c.nloc)) ++obits;
+  const int64_t per_cell = packed ? ns : (int64_t)c.nloc * c.nloc;
+  if (c.ncells * per_cell >= (int64_t(1) << 32)) return;
+  const double per_col = (double)ncontrib / (double)c.nfree;
+  int nc = 1;
+  while (nc < 256 && (2 * nc) * per_col <= kBlock * 12.0) nc *= 2;
+  DBuf<int32_t> dev;
+  dev.resize(2);
+  int32_t mx[2] = {0, 0};
+  for (;; nc /= 2) {
+    const int64_t nblocks = (c.nfree + nc - 1) / nc;
+    GH_CHECK_HIP(hipMemsetAsync(dev.get(), 0, 2 * sizeof(int32_t), s));
+    hipLaunchKernelGGL(block_extent_kernel, dim3(grid_for(nblocks)), dim3(kBlock), 0, s,
+                       nblocks, c.nfree, nc, c.adjptr.get(), c.colptr.get(), c.cstart.get(),
+                       dev.get());
+    GH_CHECK_HIP(hipMemcpyAsync(mx, dev.get(), 8, hipMemcpyDeviceToHost, s));
+    GH_CHECK_HIP(hipStreamSynchronize(s));
+    const bool fits = ((int64_t)mx[0] << obits) <= 65536 && mx[1] <= 65535 &&
+                      column_sum_lds(mx[0], mx[1], 8) <= kMaxColumnLds;
+    if (fits) break;
+    if (nc == 1) return;  // keep cstart/perm and full rows
+  }
+  const int64_t nblocks = (c.nfree + nc - 1) / nc;
+  DBuf<uint16_t> cslot, noff;
+  DBuf<int32_t> bbase;
+  // full rows: the inverse form (ipos), one entry per (free adjacency entry, row)
+  int32_t nadj = 0;
+  if (!packed)
+    GH_CHECK_HIP(hipMemcpy(&nadj, c.adjptr.get() + c.nfree, 4, hipMemcpyDeviceToHost));
+  const int64_t nslot = packed ? (int64_t)ncontrib : (int64_t)nadj * c.nloc;
+  cslot.resize(std::max<int64_t>(nslot, 1));
+  if (!packed)
+    GH_CHECK_HIP(hipMemsetAsync(cslot.get(), 0xff, sizeof(uint16_t) * cslot.n, s));
+  noff.resize(c.nnz);
+  bbase.resize(nblocks + 1);
+  GH_CHECK_HIP(hipMemsetAsync(dev.get(), 0, sizeof(int32_t), s));
+  if (ncontrib)
+    hipLaunchKernelGGL(contrib_slot_kernel, dim3(grid_for(ncontrib)), dim3(kBlock), 0, s,
+                       (int64_t)ncontrib, nc, c.nloc, lbits, obits, ns, c.perm.get(),
+                       c.cell_dofs.get(), c.adjptr.get(), c.adj.get(), c.colptr.get(),
+                       c.cstart.get(), packed ? cslot.get() : nullptr,
+                       packed ? nullptr : cslot.get(), dev.get());
+  hipLaunchKernelGGL(nonzero_offset_kernel, dim3(grid_for(c.nfree)), dim3(kBlock), 0, s,
+                     c.nfree, nc, c.colptr.get(), c.cstart.get(), noff.get());
+  hipLaunchKernelGGL(block_base_kernel, dim3(grid_for(nblocks + 1)), dim3(kBlock), 0, s,
+                     nblocks, c.nfree, nc, c.colptr.get(), c.cstart.get(), bbase.get());
+  GH_CHECK_HIP(hipGetLastError());
+  int32_t bad = 0;
+  GH_CHECK_HIP(hipMemcpyAsync(&bad, dev.get(), 4, hipMemcpyDeviceToHost, s));
+  GH_CHECK_HIP(hipStreamSynchronize(s));
+  if (bad) return;
+  std::swap(c.cslot.p, cslot.p);
+  std::swap(c.cslot.n, cslot.n);
+  std::swap(c.noff.p, noff.p);
+  std::swap(c.noff.n, noff.n);
+  std::swap(c.bbase.p, bbase.p);
+  std::swap(c.bbase.n, bbase.n);
+  c.cstart.release();
+  c.perm.release();
+  c.sum_nc = nc;
+  c.maxbase = mx[0];
+  c.maxq = mx[1];
+  c.obits = obits;
+  c.packed = packed;
+  c.slots = true;
+  c.elem.release();
 }
 
 void elemlist_symbolic(Ctx &c) {
@@ -275,11 +577,31 @@ void elemlist_symbolic(Ctx &c) {
   GH_CHECK_HIP(hipStreamSynchronize(s));
   c.fdet.resize(c.ncells);
   c.elemlist = true;
+  compact_contributions(c, lbits);
+}
+
+template <class T>
+void launch_column_sum(Ctx &c) {
+  const int lbits = c.nloc <= 16 ? 4 : 7;
+  const int ns = c.packed ? c.nloc * (c.nloc + 1) / 2 : 0;
+  const size_t shm = column_sum_lds(c.maxbase, c.maxq, sizeof(T));
+  const int64_t nblocks = (c.nfree + c.sum_nc - 1) / c.sum_nc;
+  auto go = [&](auto kern) {
+    hipLaunchKernelGGL(kern, dim3((unsigned)nblocks), dim3(kBlock), shm, c.stream, c.nfree,
+                       c.sum_nc, c.nloc, lbits, c.obits, ns, std::max(c.maxbase, 1),
+                       c.colptr.get(), c.bbase.get(), c.adjptr.get(), c.adj.get(),
+                       c.noff.get(), c.cslot.get(), (const T *)c.elem.get(), c.nzval.get());
+  };
+  if (c.packed) go(column_sum_kernel<T, 0>);
+  else go(column_sum_kernel<T, 81>);
 }
 
 void elemlist_sum(Ctx &c, bool f32) {
   if (!c.nnz) return;
-  if (f32)
+  if (c.slots) {
+    if (f32) launch_column_sum<float>(c);
+    else launch_column_sum<double>(c);
+  } else if (f32)
     hipLaunchKernelGGL((simplex_sum_kernel<float>), dim3(grid_for(c.nnz)), dim3(kBlock), 0,
                        c.stream, c.nnz, c.cstart.get(), c.perm.get(),
                        (const float *)c.elem.get(), c.nzval.get());
@@ -320,12 +642,21 @@ void assemble_simplex(Ctx &c, int precision) {
     c.sref_ready = true;
   }
   const bool f32 = precision == GH_PREC_FP32_ELEM;
-  if (NL == 10) {
-    if (f32) launch_simplex<10, float>(c);
-    else launch_simplex<10, double>(c);
+  if (c.packed) {
+    GH_REQUIRE(c.slots, GH_ERR_STATE, "packed element storage without compact lists");
+    if (NL == 10) {
+      if (f32) launch_simplex<10, float, true>(c);
+      else launch_simplex<10, double, true>(c);
+    } else {
+      if (f32) launch_simplex<4, float, true>(c);
+      else launch_simplex<4, double, true>(c);
+    }
+  } else if (NL == 10) {
+    if (f32) launch_simplex<10, float, false>(c);
+    else launch_simplex<10, double, false>(c);
   } else {
-    if (f32) launch_simplex<4, float>(c);
-    else launch_simplex<4, double>(c);
+    if (f32) launch_simplex<4, float, false>(c);
+    else launch_simplex<4, double, false>(c);
   }
   // enqueued only: gh_values_get (or a wait on gh_stream) orders the results
 }
