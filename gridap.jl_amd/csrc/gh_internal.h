@@ -59,6 +59,13 @@ struct DBuf {
   T *get() const { return p; }
 };
 
+// Element-matrix storage of the vector-Q2 hexahedron (81 local dofs, node + 27 comp):
+// column-major, one column of kHexColLd entries per local dof, the rows of component
+// i at 28 i + node, so that four consecutive nodes of one component are one aligned
+// 16-byte group.  Entry (row li, column lj) of cell c: (c·81 + lj)·kHexColLd + slot(li).
+constexpr int kHexColLd = 84;
+__host__ __device__ inline int hex_row_slot(int li) { return li + li / 27; }
+
 struct Ctx {
   int device = 0;
   hipStream_t stream = nullptr;

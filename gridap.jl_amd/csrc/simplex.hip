@@ -251,7 +251,7 @@ __global__ __launch_bounds__(kBlock) void simplex_element_kernel(
 template <class T>
 __global__ void simplex_sum_kernel(int64_t nnz, const int32_t *__restrict__ cstart,
                                    const int32_t *__restrict__ perm, const T *__restrict__ E,
-                                   double *__restrict__ nzval) {
+                                   bool hexcols, double *__restrict__ nzval) {
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= nnz) return;
   // batches of 8: all index loads, then all value loads, then the ordered sum
@@ -264,7 +264,13 @@ __global__ void simplex_sum_kernel(int64_t nnz, const int32_t *__restrict__ csta
 #pragma unroll
     for (int u = 0; u < U; ++u) id[u] = i0 + u < e ? perm[i0 + u] : -1;
 #pragma unroll
-    for (int u = 0; u < U; ++u) v[u] = id[u] >= 0 ? E[id[u]] : (T)0;
+    for (int u = 0; u < U; ++u) {
+      // emit index (cell·81 + lj)·81 + li → the hexahedron's padded column layout
+      const int64_t at = hexcols && id[u] >= 0
+                             ? (int64_t)(id[u] / 81) * kHexColLd + hex_row_slot(id[u] % 81)
+                             : id[u];
+      v[u] = id[u] >= 0 ? E[at] : (T)0;
+    }
 #pragma unroll
     for (int u = 0; u < U; ++u) s += (double)v[u];
   }
@@ -281,10 +287,11 @@ __global__ void simplex_sum_kernel(int64_t nnz, const int32_t *__restrict__ csta
 // nonzero offsets per thread) are all issued before the first wait, so a block
 // costs about three memory round trips.
 // Packed storage (FULL_NLOC = 0): the bases of the range's cell-columns go to LDS and
-// each contribution gathers its entry through its slot.  Full rows (FULL_NLOC = nloc):
-// the block walks its cell-columns entry by entry with contiguous reads and writes
-// each entry that is a contribution to its run position (cslot holds those positions,
-// 0xffff for rows that are not free).
+// each contribution gathers its entry through its slot.  Full columns (FULL_NLOC = 81,
+// the vector-Q2 hexahedron layout, kHexColLd entries per column): the block walks its
+// cell-columns entry by entry with contiguous reads and writes each entry that is a
+// contribution to its run position (cslot holds those positions, 0xffff for rows
+// that are not free).
 // LDS: maxbase bases, then maxq entries (the largest extents over all blocks).
 template <class T, int FULL_NLOC>
 __global__ __launch_bounds__(kBlock) void column_sum_kernel(
@@ -331,7 +338,8 @@ __global__ __launch_bounds__(kBlock) void column_sum_kernel(
         const int li = e - (e / FULL_NLOC) * FULL_NLOC;
         const uint32_t cell = (uint32_t)(ae[u] >> lbits), lj = (uint32_t)(ae[u] & lmask);
         v[u] = pos[u] != 0xffffu
-                   ? E[(cell * (uint32_t)FULL_NLOC + lj) * (uint32_t)FULL_NLOC + li]
+                   ? E[(cell * (uint32_t)FULL_NLOC + lj) * (uint32_t)kHexColLd +
+                       hex_row_slot(li)]
                    : (T)0;
       }
 #pragma unroll
@@ -467,7 +475,7 @@ void compact_contributions(Ctx &c, int lbits) {
   const int ns = packed ? c.nloc * (c.nloc + 1) / 2 : 0;
   int obits = 1;
   while ((1 << obits) < (packed ? ns : c.nloc)) ++obits;
-  const int64_t per_cell = packed ? ns : (int64_t)c.nloc * c.nloc;
+  const int64_t per_cell = packed ? ns : (int64_t)c.nloc * kHexColLd;
   if (c.ncells * per_cell >= (int64_t(1) << 32)) return;
   const double per_col = (double)ncontrib / (double)c.nfree;
   int nc = 1;
@@ -604,11 +612,11 @@ void elemlist_sum(Ctx &c, bool f32) {
   } else if (f32)
     hipLaunchKernelGGL((simplex_sum_kernel<float>), dim3(grid_for(c.nnz)), dim3(kBlock), 0,
                        c.stream, c.nnz, c.cstart.get(), c.perm.get(),
-                       (const float *)c.elem.get(), c.nzval.get());
+                       (const float *)c.elem.get(), c.nloc == 81, c.nzval.get());
   else
     hipLaunchKernelGGL((simplex_sum_kernel<double>), dim3(grid_for(c.nnz)), dim3(kBlock), 0,
                        c.stream, c.nnz, c.cstart.get(), c.perm.get(),
-                       (const double *)c.elem.get(), c.nzval.get());
+                       (const double *)c.elem.get(), c.nloc == 81, c.nzval.get());
   GH_CHECK_HIP(hipGetLastError());
 }
 
