@@ -277,6 +277,18 @@ __global__ void simplex_sum_kernel(int64_t nnz, const int32_t *__restrict__ csta
   nzval[p] = s;
 }
 
+// Right-hand side fused into the column sum of simplices (b = NULL: not fused): per
+// adjacency entry (cell, a) the term f|det| m_a minus the lifting of the cell's
+// Dirichlet dofs, and per column the sum of its terms in ascending cell order.
+struct FusedRhs {
+  const double *fdet;
+  const double *mref;
+  const uint16_t *dmask;
+  const int32_t *cell_dofs;
+  const double *dvals;
+  double *b;
+};
+
 // One block per range of NC consecutive columns [J0, J1), in two phases:
 //  * stage: every contribution of the range (a contiguous run) is read from the
 //    element matrices into LDS at its run position;
@@ -299,12 +311,13 @@ __global__ __launch_bounds__(kBlock) void column_sum_kernel(
     const int32_t *__restrict__ colptr, const int32_t *__restrict__ bbase,
     const int32_t *__restrict__ adjptr, const int32_t *__restrict__ adj,
     const uint16_t *__restrict__ noff, const uint16_t *__restrict__ cslot,
-    const T *__restrict__ E, double *__restrict__ nzval) {
+    const T *__restrict__ E, double *__restrict__ nzval, FusedRhs F) {
   extern __shared__ __align__(16) uint32_t shm[];
   constexpr int U = 12;  // staged entries per thread and batch
   constexpr int R = 8;   // nonzeros per thread whose offsets are prefetched
   uint32_t *bases = shm;
-  T *vals = reinterpret_cast<T *>(shm + ((maxbase + 3) & ~3));
+  double *rv = reinterpret_cast<double *>(shm + ((maxbase + 3) & ~3));
+  T *vals = reinterpret_cast<T *>(rv + (F.b ? maxbase : 0));
   const int64_t j0 = (int64_t)blockIdx.x * nc, j1 = min<int64_t>(j0 + nc, nfree);
   const int a0 = adjptr[j0], na = adjptr[j1] - a0;
   const int32_t q0 = bbase[blockIdx.x], nq = bbase[blockIdx.x + 1] - q0;
@@ -358,6 +371,19 @@ __global__ __launch_bounds__(kBlock) void column_sum_kernel(
       const int e = adj[a0 + k];
       const uint32_t cell = (uint32_t)(e >> lbits), lj = (uint32_t)(e & lmask);
       bases[k] = ns ? cell * (uint32_t)ns : (cell * (uint32_t)nloc + lj) * (uint32_t)nloc;
+      if (F.b) {
+        double v = F.fdet[cell] * F.mref[lj];
+        uint32_t m = F.dvals ? F.dmask[cell] : 0u;
+        while (m) {
+          const int bb = __ffs(m) - 1;
+          m &= m - 1;
+          const int d = F.cell_dofs[(size_t)cell * nloc + bb];
+          const int lo = min((int)lj, bb), hi = max((int)lj, bb);
+          v -= (double)E[(size_t)cell * ns + lo * nloc - lo * (lo - 1) / 2 + (hi - lo)] *
+               F.dvals[-d - 1];
+        }
+        rv[k] = v;
+      }
     }
     __syncthreads();
     for (int qb = 0; qb < nq; qb += kBlock * U) {
@@ -378,6 +404,12 @@ __global__ __launch_bounds__(kBlock) void column_sum_kernel(
     }
   }
   __syncthreads();
+  if (FULL_NLOC == 0 && F.b && tid < j1 - j0) {
+    const int64_t J = j0 + tid;
+    double bs = 0.0;
+    for (int k = adjptr[J] - a0, ke = adjptr[J + 1] - a0; k < ke; ++k) bs += rv[k];
+    F.b[J] = bs;
+  }
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const int i = tid + r * kBlock;
@@ -426,6 +458,12 @@ __global__ void simplex_rhs_kernel(int64_t nfree, const int32_t *__restrict__ ad
   b[j] = s;
 }
 
+}  // namespace
+
+bool elemlist_sum(Ctx &c, bool f32, const FusedRhs *rhs);
+
+namespace {
+
 template <int NLOC, class T, bool PACKED>
 void launch_simplex(Ctx &c) {
   constexpr int NN = PACKED ? NLOC * (NLOC + 1) / 2 : NLOC * NLOC;
@@ -435,11 +473,14 @@ void launch_simplex(Ctx &c) {
                      dim3(kBlock), 0, c.stream, c.ncells, c.cell_nodes.get(), c.xyz.get(),
                      c.have_kappa ? c.kappa.get() : nullptr,
                      c.have_fsrc ? c.fsrc.get() : nullptr, c.sref.get(), E, c.fdet.get());
-  elemlist_sum(c, sizeof(T) == 4);
-  hipLaunchKernelGGL((simplex_rhs_kernel<NLOC, T, PACKED>), dim3(grid_for(c.nfree)), dim3(kBlock), 0,
-                     c.stream, c.nfree, c.adjptr.get(), c.adj.get(), c.fdet.get(),
-                     c.sref.get() + 6 * NLOC * NLOC, c.dmask.get(), c.cell_dofs.get(),
-                     (const T *)E, c.have_dvals ? c.dvals.get() : nullptr, c.b.get());
+  const FusedRhs F{c.fdet.get(), c.sref.get() + 6 * NLOC * NLOC, c.dmask.get(),
+                   c.cell_dofs.get(), c.have_dvals ? c.dvals.get() : nullptr, c.b.get()};
+  if (!elemlist_sum(c, sizeof(T) == 4, &F))
+    hipLaunchKernelGGL((simplex_rhs_kernel<NLOC, T, PACKED>), dim3(grid_for(c.nfree)),
+                       dim3(kBlock), 0, c.stream, c.nfree, c.adjptr.get(), c.adj.get(),
+                       c.fdet.get(), c.sref.get() + 6 * NLOC * NLOC, c.dmask.get(),
+                       c.cell_dofs.get(), (const T *)E, c.have_dvals ? c.dvals.get() : nullptr,
+                       c.b.get());
   GH_CHECK_HIP(hipGetLastError());
 }
 
@@ -456,8 +497,9 @@ bool simplex_eligible(const Ctx &c) {
 // at most about kBlock·12 entries on average; it halves until the block's extents fit 16 bits and the LDS.
 constexpr size_t kMaxColumnLds = 48 * 1024;
 
-size_t column_sum_lds(int maxbase, int maxq, size_t esz) {
-  return sizeof(uint32_t) * ((std::max(maxbase, 1) + 3) & ~3) + esz * std::max(maxq, 1);
+size_t column_sum_lds(int maxbase, int maxq, size_t esz, bool rhs) {
+  return sizeof(uint32_t) * ((std::max(maxbase, 1) + 3) & ~3) +
+         (rhs ? sizeof(double) * std::max(maxbase, 1) : 0) + esz * std::max(maxq, 1);
 }
 
 void compact_contributions(Ctx &c, int lbits) {
@@ -492,7 +534,7 @@ void compact_contributions(Ctx &c, int lbits) {
     GH_CHECK_HIP(hipMemcpyAsync(mx, dev.get(), 8, hipMemcpyDeviceToHost, s));
     GH_CHECK_HIP(hipStreamSynchronize(s));
     const bool fits = ((int64_t)mx[0] << obits) <= 65536 && mx[1] <= 65535 &&
-                      column_sum_lds(mx[0], mx[1], 8) <= kMaxColumnLds;
+                      column_sum_lds(mx[0], mx[1], 8, packed) <= kMaxColumnLds;
     if (fits) break;
     if (nc == 1) return;  // keep cstart/perm and full rows
   }
@@ -589,26 +631,30 @@ void elemlist_symbolic(Ctx &c) {
 }
 
 template <class T>
-void launch_column_sum(Ctx &c) {
+void launch_column_sum(Ctx &c, const FusedRhs &F) {
   const int lbits = c.nloc <= 16 ? 4 : 7;
   const int ns = c.packed ? c.nloc * (c.nloc + 1) / 2 : 0;
-  const size_t shm = column_sum_lds(c.maxbase, c.maxq, sizeof(T));
+  const size_t shm = column_sum_lds(c.maxbase, c.maxq, sizeof(T), F.b != nullptr);
   const int64_t nblocks = (c.nfree + c.sum_nc - 1) / c.sum_nc;
   auto go = [&](auto kern) {
     hipLaunchKernelGGL(kern, dim3((unsigned)nblocks), dim3(kBlock), shm, c.stream, c.nfree,
                        c.sum_nc, c.nloc, lbits, c.obits, ns, std::max(c.maxbase, 1),
                        c.colptr.get(), c.bbase.get(), c.adjptr.get(), c.adj.get(),
-                       c.noff.get(), c.cslot.get(), (const T *)c.elem.get(), c.nzval.get());
+                       c.noff.get(), c.cslot.get(), (const T *)c.elem.get(), c.nzval.get(), F);
   };
   if (c.packed) go(column_sum_kernel<T, 0>);
   else go(column_sum_kernel<T, 81>);
 }
 
-void elemlist_sum(Ctx &c, bool f32) {
-  if (!c.nnz) return;
+// nzval from the element entries; with rhs (packed storage only) also b, fused
+bool elemlist_sum(Ctx &c, bool f32, const FusedRhs *rhs) {
+  if (!c.nnz) return false;
+  const bool fuse = rhs && c.slots && c.packed;
+  FusedRhs F{};
+  if (fuse) F = *rhs;
   if (c.slots) {
-    if (f32) launch_column_sum<float>(c);
-    else launch_column_sum<double>(c);
+    if (f32) launch_column_sum<float>(c, F);
+    else launch_column_sum<double>(c, F);
   } else if (f32)
     hipLaunchKernelGGL((simplex_sum_kernel<float>), dim3(grid_for(c.nnz)), dim3(kBlock), 0,
                        c.stream, c.nnz, c.cstart.get(), c.perm.get(),
@@ -618,7 +664,10 @@ void elemlist_sum(Ctx &c, bool f32) {
                        c.stream, c.nnz, c.cstart.get(), c.perm.get(),
                        (const double *)c.elem.get(), c.nloc == 81, c.nzval.get());
   GH_CHECK_HIP(hipGetLastError());
+  return fuse;
 }
+
+void elemlist_sum(Ctx &c, bool f32) { (void)elemlist_sum(c, f32, nullptr); }
 
 void assemble_simplex(Ctx &c, int precision) {
   GH_REQUIRE(c.elemlist && simplex_eligible(c), GH_ERR_STATE,
