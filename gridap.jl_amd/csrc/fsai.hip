@@ -437,7 +437,8 @@ __global__ void g_assemble_kernel(Lat3 L, int nnodes, const int32_t *__restrict_
                                   const int32_t *__restrict__ colptr,
                                   const float *__restrict__ yo, const double *__restrict__ fs,
                                   float *__restrict__ gl, float *__restrict__ gu,
-                                  float *__restrict__ gle, float *__restrict__ gue) {
+                                  float *__restrict__ gle, float *__restrict__ gue,
+                                  int64_t nfree, float *__restrict__ g14) {
   const int n = blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= nnodes) return;
   const int d0 = node_dof[n];
@@ -471,11 +472,13 @@ __global__ void g_assemble_kernel(Lat3 L, int nnodes, const int32_t *__restrict_
       const float ge = (float)((double)g * sn);
       el[kl++] = ge;
       eu[ku++] = ge;
+      if (g14) g14[13 * nfree + (d0 - 1)] = ge;
     } else if (o < 13) {
       if (CSC) gl[pos] = sum;  // Ĝ(n, m), m < n
       const int m = (xi[0] + o % 3 - 1) + L.nn[0] * ((xi[1] + (o / 3) % 3 - 1) +
                                                      L.nn[1] * (xi[2] + o / 9 - 1));
       el[kl++] = (float)((double)sum * fs[node_dof[m] - 1]);
+      if (g14) g14[o * nfree + (d0 - 1)] = el[kl - 1];
     } else {
       if (CSC) gu[pos] = sum;  // Ĝ(m, n), m > n
       eu[ku++] = (float)((double)sum * sn);
@@ -483,6 +486,9 @@ __global__ void g_assemble_kernel(Lat3 L, int nnodes, const int32_t *__restrict_
   }
   for (; kl < kEllSlots; ++kl) el[kl] = 0.0f;
   for (; ku < kEllSlots; ++ku) eu[ku] = 0.0f;
+  if (g14)  // lower offsets whose neighbour is not free (outside the box): zero
+    for (int o = 0; o < 13; ++o)
+      if (!((fm >> o) & 1u)) g14[o * nfree + (d0 - 1)] = 0.0f;
 }
 
 // Column indices of the 16-slot rows (from the pattern; padding points at the row
@@ -765,14 +771,22 @@ void fsai_forward_build(Ctx &c, bool inference) {
                        dim3(64), 0, s, c.h3.get(), wq, kMlpW4, kMlpB4, c.yo.get(),
                        (float *)nullptr, C, kMlpFwdCells);
   }
+  // box lattice: also the 14 lower-offset arrays of G for the stencil PCG kernels
+  float *g14 = nullptr;
+  if (c.box) {
+    c.g14.resize((size_t)14 * std::max<int64_t>(c.nfree, 1));
+    g14 = c.g14.get();
+  }
   if (inference)
     hipLaunchKernelGGL(g_assemble_kernel<false>, dim3(grid_for(c.nnodes)), dim3(kBlock), 0, s, L,
                        (int)c.nnodes, c.node_dof.get(), c.node_fmask.get(), c.colptr.get(),
-                       c.yo.get(), c.fs.get(), c.gl.get(), c.gu.get(), c.gle.get(), c.gue.get());
+                       c.yo.get(), c.fs.get(), c.gl.get(), c.gu.get(), c.gle.get(), c.gue.get(),
+                       c.nfree, g14);
   else
     hipLaunchKernelGGL(g_assemble_kernel<true>, dim3(grid_for(c.nnodes)), dim3(kBlock), 0, s, L,
                        (int)c.nnodes, c.node_dof.get(), c.node_fmask.get(), c.colptr.get(),
-                       c.yo.get(), c.fs.get(), c.gl.get(), c.gu.get(), c.gle.get(), c.gue.get());
+                       c.yo.get(), c.fs.get(), c.gl.get(), c.gu.get(), c.gle.get(), c.gue.get(),
+                       c.nfree, g14);
   GH_CHECK_HIP(hipGetLastError());
   c.fsai_built = true;
 }

@@ -146,6 +146,15 @@ struct Ctx {
   // ---- solver scratch ----
   DBuf<double> s_r, s_z, s_p, s_q, s_x, s_dinv, s_b, s_part, s_scal, s_y;
   DBuf<double> s_r2, s_p2;          // second buffers of r (factored M) and p
+  // box lattice: the free dofs are all nodes of a box of box_m[0] x box_m[1] x box_m[2]
+  // lattice nodes, numbered in node order (Q1, every boundary node outside the box
+  // Dirichlet), so A is a 27-point stencil in dof space and PCG runs the stencil
+  // kernels: A as 27 offset arrays of N values (a27, zero outside the box) and G as
+  // 14 offset arrays (g14, lower offsets in ascending order), no column indices
+  bool box = false;
+  int64_t box_m[3] = {0, 0, 0};
+  DBuf<double> a27;
+  DBuf<float> g14;
 
   // ---- learned FSAI (lattice Q1) ----
   bool fsai_ready = false;   // structures allocated for the current pattern

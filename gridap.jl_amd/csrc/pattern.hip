@@ -10,6 +10,7 @@
 
 #include <algorithm>
 #include <climits>
+#include <vector>
 
 #include "gh_internal.h"
 
@@ -307,6 +308,36 @@ void symbolic_general(Ctx &c, bool contributions) {
   c.b.resize(std::max<int64_t>(c.nfree, 1));
 }
 
+// Free dofs forming a full box of lattice nodes (numbered in node order, which the
+// structured check has verified): the dof offsets of the 27 neighbours are constant.
+void detect_box(Ctx &c) {
+  c.box = false;
+  c.a27.release();
+  c.g14.release();
+  if (!c.structured || c.dim != 3) return;
+  std::vector<int32_t> nd(c.nnodes);
+  GH_CHECK_HIP(hipMemcpy(nd.data(), c.node_dof.get(), sizeof(int32_t) * c.nnodes,
+                         hipMemcpyDeviceToHost));
+  const int64_t np0 = c.part[0] + 1, np1 = c.part[1] + 1;
+  int64_t lo[3] = {INT64_MAX, INT64_MAX, INT64_MAX}, hi[3] = {-1, -1, -1};
+  int64_t nfree = 0;
+  for (int64_t n = 0; n < c.nnodes; ++n) {
+    if (nd[n] <= 0) continue;
+    const int64_t x[3] = {n % np0, (n / np0) % np1, n / (np0 * np1)};
+    for (int d = 0; d < 3; ++d) {
+      lo[d] = std::min(lo[d], x[d]);
+      hi[d] = std::max(hi[d], x[d]);
+    }
+    ++nfree;
+  }
+  if (nfree != c.nfree || nfree == 0) return;
+  int64_t vol = 1;
+  for (int d = 0; d < 3; ++d) vol *= hi[d] - lo[d] + 1;
+  if (vol != nfree) return;
+  for (int d = 0; d < 3; ++d) c.box_m[d] = hi[d] - lo[d] + 1;
+  c.box = true;
+}
+
 void check_structured(Ctx &c) {
   c.structured = false;
   c.node_dof.release();
@@ -360,6 +391,7 @@ void check_structured(Ctx &c) {
     c.tile_p0.release();
     c.tile_d0.release();
   }
+  detect_box(c);
 }
 
 // Greedy colouring of cells so that no two cells of one colour share a free dof:

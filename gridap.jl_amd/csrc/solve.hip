@@ -13,7 +13,10 @@
 //                 the fly from r and q; r double-buffered), partials of r'·r';
 //   upper_rz      z = Gᵀ y, partials of r'·z;
 // G and Gᵀ are read from the solver's 16-slot row copy (weights with D^-1/2
-// folded in, column indices), written with the factor.
+// folded in, column indices), written with the factor.  When the free dofs form a
+// full box of Q1 lattice nodes, the same iteration runs on stencil kernels with no
+// column indices: A as 27 offset arrays, G as 14, Gᵀ read from G's arrays at the
+// mirrored offsets.
 // Reductions are deterministic and need no grid-wide synchronisation: a kernel
 // writes one partial per block, and every block of the consuming kernel sums all
 // of them in index order at its start (the same value everywhere); block 0 of
@@ -72,9 +75,15 @@ __device__ __forceinline__ double sum_partials(const double *__restrict__ part, 
   return block_sum(a, sh);
 }
 
+// Grids of the iteration kernels are capped at kPcgGrid blocks (grid-stride loops):
+// every block of a consuming kernel re-reads all partials of its producer, so the
+// partial count, not only the row count, sets the cost of a reduction.
+constexpr int kPcgGrid = 1024;
+static_assert(kPcgGrid <= kMaxParts, "partial slots");
+
 inline unsigned grid_rows(int64_t rows, int rows_per_block) {
-  return (unsigned)std::min<int64_t>(std::max<int64_t>((rows + rows_per_block - 1) / rows_per_block, 1),
-                                     kMaxParts);
+  return (unsigned)std::min<int64_t>(
+      std::max<int64_t>((rows + rows_per_block - 1) / rows_per_block, 1), kPcgGrid);
 }
 
 // y = A x (rows from the CSC of a symmetric A).
@@ -279,6 +288,181 @@ __global__ __launch_bounds__(kBlock) void upper_rz_kernel(
   put_partial(rz, part_rz, sh);
 }
 
+// ---- box lattice: stencil kernels ---------------------------------------------------
+// Row i ↔ box node (x, y, z); neighbour offset o = (dx+1) + 3 (dy+1) + 9 (dz+1) has dof
+// offset dx + m0 dy + m0 m1 dz, ascending in o.  A is held as a27[o·N + i] (zero where
+// the neighbour is outside the box), G as g14[o·N + i] for o ≤ 13 (lower offsets);
+// Gᵀ row i collects g14[o·N + i − off(o)] from the mirrored neighbours.
+
+struct Box {
+  int64_t n, m0, m1, m2;
+};
+
+__device__ __forceinline__ uint32_t box_valid(const Box &B, int64_t row) {
+  // bit o set when neighbour o of row i lies in the box (32-bit: N < 2^31)
+  const int i = (int)row, m0 = (int)B.m0, m1 = (int)B.m1;
+  const int x = i % m0, y = (i / m0) % m1, z = i / (m0 * m1);
+  const uint32_t vx = (x > 0 ? 1u : 0u) | 2u | (x < B.m0 - 1 ? 4u : 0u);
+  const uint32_t vy = (y > 0 ? 1u : 0u) | 2u | (y < B.m1 - 1 ? 4u : 0u);
+  const uint32_t vz = (z > 0 ? 1u : 0u) | 2u | (z < B.m2 - 1 ? 4u : 0u);
+  uint32_t m = 0;
+#pragma unroll
+  for (int o = 0; o < 27; ++o)
+    if (((vx >> (o % 3)) & 1u) && ((vy >> ((o / 3) % 3)) & 1u) && ((vz >> (o / 9)) & 1u))
+      m |= 1u << o;
+  return m;
+}
+
+__device__ __forceinline__ int64_t box_off(const Box &B, int o) {
+  return (int64_t)(o % 3 - 1) + B.m0 * ((o / 3) % 3 - 1) + B.m0 * B.m1 * (o / 9 - 1);
+}
+
+// a27 from the CSR rows of the box-lattice matrix.  A lattice row holds exactly the
+// neighbours that lie in the box, in ascending column order (the symbolic phase has
+// verified the pattern), which is ascending offset order: the row's k-th entry
+// belongs to the k-th in-box offset.  A block of kConvRows rows first stages its
+// contiguous run of values in LDS with coalesced loads; then all lanes write array o
+// together (coalesced, zeros outside the box written explicitly).  A row whose
+// length differs from its in-box neighbour count is flagged.  32-bit indices
+// (N < 2^31, checked by the caller).
+constexpr int kConvRows = 128;
+
+__global__ __launch_bounds__(kConvRows) void stencil_values_kernel(
+    Box B, const int32_t *__restrict__ cp, const double *__restrict__ val,
+    double *__restrict__ a27, int *__restrict__ bad) {
+  __shared__ double sv[kConvRows * 27];
+  const int n = (int)B.n;
+  const int i0 = blockIdx.x * kConvRows;
+  const int i1 = min(i0 + kConvRows, n);
+  const int q0 = cp[i0], nq = cp[i1] - q0;
+  if (nq > kConvRows * 27) {  // not a 27-point lattice matrix
+    if (threadIdx.x == 0) atomicOr(bad, 1);
+    return;
+  }
+  for (int k = threadIdx.x; k < nq; k += kConvRows) sv[k] = val[q0 + k];
+  __syncthreads();
+  const int i = i0 + (int)threadIdx.x;
+  if (i >= n) return;
+  const uint32_t vm = box_valid(B, i);
+  int p = cp[i] - q0;
+  const bool ok = cp[i + 1] - cp[i] == __popc(vm);
+#pragma unroll
+  for (int o = 0; o < 27; ++o) {
+    double v = 0.0;
+    if (((vm >> o) & 1u) && ok) v = sv[p++];
+    a27[(size_t)o * n + i] = v;
+  }
+  if (!ok) atomicOr(bad, 1);
+}
+
+// D^-1 of the box-lattice matrix from the centre array
+__global__ void diag_inv_box_kernel(int64_t n, const double *__restrict__ a27,
+                                    double *__restrict__ dinv) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double d = a27[13 * n + i];
+  dinv[i] = d != 0.0 ? 1.0 / d : 1.0;
+}
+
+// stencil form of spmv_dir_kernel
+__global__ __launch_bounds__(kBlock) void spmv_dir_box_kernel(
+    Box B, const double *__restrict__ a27, const double *__restrict__ z,
+    const double *__restrict__ pold, double *__restrict__ pnew, double *__restrict__ q,
+    double *scal, int par, int first, const double *__restrict__ part_rz, int np_rz,
+    double *__restrict__ part_pq) {
+  __shared__ double sh[kBlock / 64];
+  const double rz = sum_partials(part_rz, np_rz, sh);
+  const double beta = first ? 0.0 : rz / scal[kRz + (par ^ 1)];
+  if (blockIdx.x == 0 && threadIdx.x == 0) scal[kRz + par] = rz;
+  double dot = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < B.n;
+       i += (int64_t)gridDim.x * kBlock) {
+    const uint32_t vm = box_valid(B, i);
+    double s = 0.0;
+#pragma unroll
+    for (int o = 0; o < 27; ++o) {
+      if (!((vm >> o) & 1u)) continue;
+      const int64_t j = i + box_off(B, o);
+      const double pj = first ? z[j] : z[j] + beta * pold[j];
+      s += a27[(int64_t)o * B.n + i] * pj;
+    }
+    const double pn = first ? z[i] : z[i] + beta * pold[i];
+    pnew[i] = pn;
+    q[i] = s;
+    dot += pn * s;
+  }
+  put_partial(dot, part_pq, sh);
+}
+
+// stencil form of update_lower_kernel
+__global__ __launch_bounds__(kBlock) void update_lower_box_kernel(
+    Box B, int par, const float *__restrict__ g14, double *__restrict__ x,
+    const double *__restrict__ rold, double *__restrict__ rnew, const double *__restrict__ p,
+    const double *__restrict__ q, double *__restrict__ y, const double *scal,
+    const double *__restrict__ part_pq, int np_pq, double *__restrict__ part_rr) {
+  __shared__ double sh[kBlock / 64];
+  const double alpha = scal[kRz + par] / sum_partials(part_pq, np_pq, sh);
+  double rr = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < B.n;
+       i += (int64_t)gridDim.x * kBlock) {
+    const uint32_t vm = box_valid(B, i);
+    double s = 0.0;
+#pragma unroll
+    for (int o = 0; o < 14; ++o) {
+      if (!((vm >> o) & 1u)) continue;
+      const int64_t j = i + box_off(B, o);
+      s += (double)g14[(int64_t)o * B.n + i] * (rold[j] - alpha * q[j]);
+    }
+    y[i] = s;
+    const double ri = rold[i] - alpha * q[i];
+    rnew[i] = ri;
+    x[i] += alpha * p[i];
+    rr += ri * ri;
+  }
+  put_partial(rr, part_rr, sh);
+}
+
+// y = G r (start of a solve), stencil form
+__global__ __launch_bounds__(kBlock) void lower_box_kernel(Box B, const float *__restrict__ g14,
+                                                           const double *__restrict__ r,
+                                                           double *__restrict__ y) {
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < B.n;
+       i += (int64_t)gridDim.x * kBlock) {
+    const uint32_t vm = box_valid(B, i);
+    double s = 0.0;
+#pragma unroll
+    for (int o = 0; o < 14; ++o) {
+      if (!((vm >> o) & 1u)) continue;
+      s += (double)g14[(int64_t)o * B.n + i] * r[i + box_off(B, o)];
+    }
+    y[i] = s;
+  }
+}
+
+// z = Gᵀ y, partials of r·z; stencil form: entry (j, i) of G with j = i − off(o),
+// o ≤ 13, sits at g14[o·N + j], and j is in the box when the mirrored neighbour
+// 26 − o of i is
+__global__ __launch_bounds__(kBlock) void upper_rz_box_kernel(
+    Box B, const float *__restrict__ g14, const double *__restrict__ y,
+    const double *__restrict__ r, double *__restrict__ z, double *__restrict__ part_rz) {
+  __shared__ double sh[kBlock / 64];
+  double rz = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < B.n;
+       i += (int64_t)gridDim.x * kBlock) {
+    const uint32_t vm = box_valid(B, i);
+    double s = 0.0;
+#pragma unroll
+    for (int o = 13; o >= 0; --o) {
+      if (!((vm >> (26 - o)) & 1u)) continue;
+      const int64_t j = i - box_off(B, o);
+      s += (double)g14[(int64_t)o * B.n + j] * y[j];
+    }
+    z[i] = s;
+    rz += s * r[i];
+  }
+  put_partial(rz, part_rz, sh);
+}
+
 }  // namespace
 
 void spmv_device(Ctx &c, const double *x, double *y) {
@@ -304,12 +488,40 @@ void pcg_device(Ctx &c, int precond, double rtol, int maxit, gh_solve_stats *st)
   for (auto *bf : {&c.s_r, &c.s_z, &c.s_p, &c.s_p2, &c.s_q}) bf->resize(nn);
   c.s_part.resize(4 * kMaxParts);
   c.s_scal.resize(8);
+  // the solve is timed from here: per-matrix preparation (D^-1, the stencil arrays)
+  // included
+  hipEvent_t e0, e1;
+  GH_CHECK_HIP(hipEventCreate(&e0));
+  GH_CHECK_HIP(hipEventCreate(&e1));
+  GH_CHECK_HIP(hipEventRecord(e0, s));
   double *dinv = nullptr;
   const int factored = precond == GH_PRECOND_FSAI;
+  // box lattice: stencil kernels (A as 27 offset arrays, G as 14), one thread per row
+  bool box = c.box && n > 0 && n < (int64_t(1) << 31) &&
+             (!factored || c.g14.get() != nullptr);
+  const Box B{n, c.box_m[0], c.box_m[1], c.box_m[2]};
+  if (box) {
+    c.a27.resize((size_t)27 * nn);
+    DBuf<int> bad;
+    bad.resize(1);
+    GH_CHECK_HIP(hipMemsetAsync(bad.get(), 0, sizeof(int), s));
+    hipLaunchKernelGGL(stencil_values_kernel, dim3((unsigned)((n + kConvRows - 1) / kConvRows)),
+                       dim3(kConvRows), 0, s, B, c.colptr.get(), c.nzval.get(), c.a27.get(),
+                       bad.get());
+    GH_CHECK_HIP(hipGetLastError());
+    int hbad = 0;
+    GH_CHECK_HIP(hipMemcpyAsync(&hbad, bad.get(), sizeof(int), hipMemcpyDeviceToHost, s));
+    GH_CHECK_HIP(hipStreamSynchronize(s));
+    box = hbad == 0;
+  }
   if (precond == GH_PRECOND_JACOBI) {
     c.s_dinv.resize(nn);
-    hipLaunchKernelGGL(diag_inv_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n,
-                       c.colptr.get(), c.rowval.get(), c.nzval.get(), c.s_dinv.get());
+    if (box)
+      hipLaunchKernelGGL(diag_inv_box_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
+                         n, c.a27.get(), c.s_dinv.get());
+    else
+      hipLaunchKernelGGL(diag_inv_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n,
+                         c.colptr.get(), c.rowval.get(), c.nzval.get(), c.s_dinv.get());
     dinv = c.s_dinv.get();
   }
   if (factored) {
@@ -323,17 +535,12 @@ void pcg_device(Ctx &c, int precond, double rtol, int maxit, gh_solve_stats *st)
   double *rbuf[2] = {c.s_r.get(), factored ? c.s_r2.get() : c.s_r.get()};
   double *pbuf[2] = {c.s_p.get(), c.s_p2.get()};
   const unsigned g_vec = grid_rows(n, kBlock);
-  const unsigned g_spmv = grid_rows(n, kBlock / kSpmvLanes);
-  const unsigned g_apply = grid_rows(n, kBlock / kApplyLanes);
+  const unsigned g_spmv = box ? g_vec : grid_rows(n, kBlock / kSpmvLanes);
+  const unsigned g_apply = box ? g_vec : grid_rows(n, kBlock / kApplyLanes);
   // partial counts of each quantity, by producer
   const int np_rz = factored ? (int)g_apply : (int)g_vec;
   const int np_rr_iter = factored ? (int)g_apply : (int)g_vec;
   std::vector<double> hrr(kMaxParts), hbb(kMaxParts);
-
-  hipEvent_t e0, e1;
-  GH_CHECK_HIP(hipEventCreate(&e0));
-  GH_CHECK_HIP(hipEventCreate(&e1));
-  GH_CHECK_HIP(hipEventRecord(e0, s));
 
   double bnorm = 1.0, relres = INFINITY;
   int it = 0, restarts = 0;
@@ -343,7 +550,12 @@ void pcg_device(Ctx &c, int precond, double rtol, int maxit, gh_solve_stats *st)
     spmv_device(c, c.s_x.get(), c.s_q.get());
     hipLaunchKernelGGL(init_kernel, dim3(g_vec), dim3(kBlock), 0, s, n, c.s_b.get(), c.s_q.get(),
                        dinv, factored, rbuf[0], c.s_z.get(), part_rz, part_rr, part_bb);
-    if (factored) {
+    if (factored && box) {
+      hipLaunchKernelGGL(lower_box_kernel, dim3(g_apply), dim3(kBlock), 0, s, B, c.g14.get(),
+                         rbuf[0], c.s_y.get());
+      hipLaunchKernelGGL(upper_rz_box_kernel, dim3(g_apply), dim3(kBlock), 0, s, B,
+                         c.g14.get(), c.s_y.get(), rbuf[0], c.s_z.get(), part_rz);
+    } else if (factored) {
       hipLaunchKernelGGL(lower_kernel, dim3(g_apply), dim3(kBlock), 0, s, n, c.gle.get(),
                          c.gli.get(), rbuf[0], c.s_y.get());
       hipLaunchKernelGGL(upper_rz_kernel, dim3(g_apply), dim3(kBlock), 0, s, n, c.gue.get(),
@@ -370,10 +582,23 @@ void pcg_device(Ctx &c, int precond, double rtol, int maxit, gh_solve_stats *st)
       for (int m = 0; m < check && it < maxit; ++m, ++it, ++k) {
         const int par = k & 1;
         double *pold = pbuf[par], *pnew = pbuf[par ^ 1];
-        hipLaunchKernelGGL(spmv_dir_kernel, dim3(g_spmv), dim3(kBlock), 0, s, n, c.colptr.get(),
-                           c.rowval.get(), c.nzval.get(), c.s_z.get(), pold, pnew, c.s_q.get(),
-                           scal, par, k == 0 ? 1 : 0, part_rz, np_rz, part_pq);
-        if (factored) {
+        if (box)
+          hipLaunchKernelGGL(spmv_dir_box_kernel, dim3(g_spmv), dim3(kBlock), 0, s, B,
+                             c.a27.get(), c.s_z.get(), pold, pnew, c.s_q.get(), scal, par,
+                             k == 0 ? 1 : 0, part_rz, np_rz, part_pq);
+        else
+          hipLaunchKernelGGL(spmv_dir_kernel, dim3(g_spmv), dim3(kBlock), 0, s, n,
+                             c.colptr.get(), c.rowval.get(), c.nzval.get(), c.s_z.get(), pold,
+                             pnew, c.s_q.get(), scal, par, k == 0 ? 1 : 0, part_rz, np_rz,
+                             part_pq);
+        if (factored && box) {
+          double *rold = rbuf[par], *rnew = rbuf[par ^ 1];
+          hipLaunchKernelGGL(update_lower_box_kernel, dim3(g_apply), dim3(kBlock), 0, s, B, par,
+                             c.g14.get(), c.s_x.get(), rold, rnew, pnew, c.s_q.get(),
+                             c.s_y.get(), scal, part_pq, (int)g_spmv, part_rr);
+          hipLaunchKernelGGL(upper_rz_box_kernel, dim3(g_apply), dim3(kBlock), 0, s, B,
+                             c.g14.get(), c.s_y.get(), rnew, c.s_z.get(), part_rz);
+        } else if (factored) {
           double *rold = rbuf[par], *rnew = rbuf[par ^ 1];
           hipLaunchKernelGGL(update_lower_kernel, dim3(g_apply), dim3(kBlock), 0, s, n, par,
                              c.gle.get(), c.gli.get(), c.s_x.get(), rold, rnew, pnew, c.s_q.get(),
