@@ -64,6 +64,16 @@ struct Pow3<0> {
 // Staging rounds of the lattice kernel's tile: LDS per wave is 64/R columns, so
 // more waves fit a CU (measured on MI355X, 128^3 Q1).
 constexpr int kLatticeStageRounds = 2;
+// Tiles (waves) per workgroup of the lattice kernel; each wave stages through its
+// own LDS and synchronises only with itself.
+constexpr int kLatticeWaves = 4;
+
+// LDS writes of this wave visible to its other lanes (and its reads finished)
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 
 // Workgroups are dealt round-robin to the 8 XCDs; remap so XCD x sweeps the
 // contiguous x-th slice of the lattice and the neighbouring lines/planes its
@@ -141,8 +151,8 @@ __device__ __forceinline__ void load_tile(const LatticeArgs &P, int tile, int la
   }
 }
 
-// One wave per workgroup owns one tile of 64 consecutive lattice nodes; the
-// tile's nodes own a contiguous range of CSC columns, hence of nzval.  On a box
+// One wave owns one tile of 64 consecutive lattice nodes (kLatticeWaves tiles per
+// workgroup); the tile's nodes own a contiguous range of CSC columns, hence of nzval.  On a box
 // cell with a tensor-product Q1 basis and a symmetric rule, K̂[a][b] depends only
 // on a⊕b, so the entry of column n at stencil offset δ is K̂[class(δ)] times the
 // sum of κ over the cells that hold both nodes.  Those sums and products are FP64
@@ -150,15 +160,19 @@ __device__ __forceinline__ void load_tile(const LatticeArgs &P, int tile, int la
 // Column offsets come from the free-neighbour masks (one load per node instead
 // of 3^D); the tile is staged in LDS and stored in 16-byte chunks.
 template <int D>
-__global__ __launch_bounds__(64) void lattice_q1_kernel(const LatticeArgs P) {
+__global__ __launch_bounds__(64 * kLatticeWaves) void lattice_q1_kernel(const LatticeArgs P,
+                                                                        int ntiles) {
   constexpr int NB = Pow3<D>::v;  // stencil entries per column
   constexpr int NV = 1 << D;      // incident cells per node
   constexpr int R = kLatticeStageRounds;
   constexpr int LPR = 64 / R;     // lanes staged per round
-  __shared__ __align__(16) double stage[LPR * NB + 2];
+  __shared__ __align__(16) double stages[kLatticeWaves][LPR * NB + 2];
 
-  const int lane = threadIdx.x;
-  const int tile = xcd_remap(blockIdx.x, gridDim.x);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  double *stage = stages[wave];
+  // XCD-aware order at workgroup granularity (workgroups are dealt round-robin)
+  const int tile = xcd_remap(blockIdx.x, gridDim.x) * kLatticeWaves + wave;
+  if (tile >= ntiles) return;
   const int P0 = P.tile_p0[tile];  // uniform: every load below is independent of it
   const int dbase = P.tile_d0[tile];
   TileIn<D> cur;
@@ -236,14 +250,14 @@ __global__ __launch_bounds__(64) void lattice_q1_kernel(const LatticeArgs P) {
     if (end == beg) continue;  // wave-uniform
     const int Pr = P0 + beg;
     const int sh = Pr & 1;  // stage[sh + k] <-> nzval[Pr + k]; 16-byte aligned chunks
-    if (r > 0) __syncthreads();  // the previous round's stage has been read out
+    if (r > 0) wave_lds_sync();  // the previous round's stage has been read out
     if (d0 > 0 && lane / LPR == r) {
       int w = sh + off - beg;
 #pragma unroll
       for (int o = 0; o < NB; ++o)
         if ((fm >> o) & 1u) stage[w++] = acc[o];
     }
-    __syncthreads();
+    wave_lds_sync();
     const int lo = sh, hi = sh + (end - beg);
     double2 *dst2 = reinterpret_cast<double2 *>(P.nzval + (Pr - sh));
     const double2 *st2 = reinterpret_cast<const double2 *>(stage);
@@ -315,8 +329,10 @@ void launch_lattice(Ctx &c, const double *Kc, double b0) {
   P.b = c.b.get();
   for (int k = 0; k < 8; ++k) P.Kc[k] = Kc[k];
   P.b0 = b0;
-  const unsigned grid = (unsigned)((c.nnodes + kTileNodes - 1) / kTileNodes);
-  hipLaunchKernelGGL((lattice_q1_kernel<D>), dim3(grid), dim3(kTileNodes), 0, c.stream, P);
+  const int ntiles = (int)((c.nnodes + kTileNodes - 1) / kTileNodes);
+  const unsigned grid = (unsigned)((ntiles + kLatticeWaves - 1) / kLatticeWaves);
+  hipLaunchKernelGGL((lattice_q1_kernel<D>), dim3(grid), dim3(kTileNodes * kLatticeWaves), 0,
+                     c.stream, P, ntiles);
   GH_CHECK_HIP(hipGetLastError());
 }
 
