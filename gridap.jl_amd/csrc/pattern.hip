@@ -46,13 +46,34 @@ __global__ void emit_pairs(int64_t ncells, int nloc, const int32_t *__restrict__
 }
 
 __global__ void unique_flags(int64_t n, const uint64_t *__restrict__ keys, uint64_t invalid,
-                             int rb, int32_t *__restrict__ flags, int32_t *__restrict__ colcnt) {
+                             int32_t *__restrict__ flags) {
   int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
   uint64_t key = keys[k];
-  int f = key < invalid && (k == 0 || keys[k - 1] != key);
-  flags[k] = f;
-  if (f) atomicAdd(&colcnt[key >> rb], 1);
+  flags[k] = key < invalid && (k == 0 || keys[k - 1] != key);
+}
+
+// colptr from the sorted unique keys: column j starts at the unique position of its
+// first key; colptr[ncols] = nnz.  A column with no entry is flagged (every free
+// dof holds at least its diagonal entry).
+__global__ void column_starts(int64_t n, const uint64_t *__restrict__ keys, uint64_t invalid,
+                              int rb, const int32_t *__restrict__ flags,
+                              const int32_t *__restrict__ pos, int32_t *__restrict__ colptr) {
+  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n || !flags[k]) return;
+  const uint64_t col = keys[k] >> rb;
+  if (k == 0 || (keys[k - 1] >> rb) != col) colptr[col] = pos[k];
+}
+
+__global__ void column_check(int64_t ncols, int64_t nnz, int32_t *__restrict__ colptr,
+                             int32_t *__restrict__ bad) {
+  int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j > ncols) return;
+  if (j == ncols) {
+    colptr[j] = (int32_t)nnz;
+    return;
+  }
+  if (colptr[j] < 0) atomicOr(bad, 1);
 }
 
 // Contribution ranges of the nonzeros (CSC order): cstart[pos[k]] = k for the first
@@ -262,14 +283,12 @@ void symbolic_general(Ctx &c, bool contributions) {
   }
   keys.release();
 
-  DBuf<int32_t> flags, pos, colcnt;
+  DBuf<int32_t> flags, pos;
   flags.resize(std::max<int64_t>(total, 1));
   pos.resize(std::max<int64_t>(total, 1));
-  colcnt.resize(c.nfree + 1);
-  GH_CHECK_HIP(hipMemsetAsync(colcnt.get(), 0, sizeof(int32_t) * (c.nfree + 1), s));
   if (total)
     hipLaunchKernelGGL(unique_flags, dim3(grid_for(total)), dim3(kBlock), 0, s, total,
-                       sorted.get(), invalid, rb, flags.get(), colcnt.get());
+                       sorted.get(), invalid, flags.get());
   GH_CHECK_HIP(hipGetLastError());
   int64_t nnz = 0;
   if (total) {
@@ -281,7 +300,22 @@ void symbolic_general(Ctx &c, bool contributions) {
   }
   c.nnz = nnz;
   c.colptr.resize(c.nfree + 1);
-  exclusive_sum(colcnt.get(), c.colptr.get(), c.nfree + 1, s);
+  {
+    GH_CHECK_HIP(hipMemsetAsync(c.colptr.get(), 0xff, sizeof(int32_t) * (c.nfree + 1), s));
+    DBuf<int32_t> bad;
+    bad.resize(1);
+    GH_CHECK_HIP(hipMemsetAsync(bad.get(), 0, sizeof(int32_t), s));
+    if (total)
+      hipLaunchKernelGGL(column_starts, dim3(grid_for(total)), dim3(kBlock), 0, s, total,
+                         sorted.get(), invalid, rb, flags.get(), pos.get(), c.colptr.get());
+    hipLaunchKernelGGL(column_check, dim3(grid_for(c.nfree + 1)), dim3(kBlock), 0, s, c.nfree,
+                       nnz, c.colptr.get(), bad.get());
+    GH_CHECK_HIP(hipGetLastError());
+    int32_t hbad = 0;
+    GH_CHECK_HIP(hipMemcpyAsync(&hbad, bad.get(), 4, hipMemcpyDeviceToHost, s));
+    GH_CHECK_HIP(hipStreamSynchronize(s));
+    GH_REQUIRE(hbad == 0, GH_ERR_UNSUPPORTED, "a free dof with no matrix entry");
+  }
   c.rowval.resize(std::max<int64_t>(nnz, 1));
   if (total)
     hipLaunchKernelGGL(scatter_rows, dim3(grid_for(total)), dim3(kBlock), 0, s, total,

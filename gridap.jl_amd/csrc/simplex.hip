@@ -153,14 +153,16 @@ __global__ void contrib_slot_kernel(int64_t ncontrib, int nc, int nloc, int lbit
   cslot[q] = (uint16_t)((k << obits) | off);
 }
 
-// One thread per column: noff of its nonzeros, relative to its block's first entry.
+// One wave per column: noff of its nonzeros, relative to its block's first entry.
 __global__ void nonzero_offset_kernel(int64_t nfree, int nc, const int32_t *__restrict__ colptr,
                                       const int32_t *__restrict__ cstart,
                                       uint16_t *__restrict__ noff) {
-  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t j = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / 64;
+  const int lane = threadIdx.x & 63;
   if (j >= nfree) return;
   const int32_t q0 = cstart[colptr[j - j % nc]];
-  for (int32_t p = colptr[j]; p < colptr[j + 1]; ++p) noff[p] = (uint16_t)(cstart[p] - q0);
+  for (int32_t p = colptr[j] + lane; p < colptr[j + 1]; p += 64)
+    noff[p] = (uint16_t)(cstart[p] - q0);
 }
 
 __global__ void block_base_kernel(int64_t nblocks, int64_t nfree, int nc,
@@ -558,7 +560,7 @@ void compact_contributions(Ctx &c, int lbits) {
                        c.cell_dofs.get(), c.adjptr.get(), c.adj.get(), c.colptr.get(),
                        c.cstart.get(), packed ? cslot.get() : nullptr,
                        packed ? nullptr : cslot.get(), dev.get());
-  hipLaunchKernelGGL(nonzero_offset_kernel, dim3(grid_for(c.nfree)), dim3(kBlock), 0, s,
+  hipLaunchKernelGGL(nonzero_offset_kernel, dim3(grid_for(c.nfree * 64)), dim3(kBlock), 0, s,
                      c.nfree, nc, c.colptr.get(), c.cstart.get(), noff.get());
   hipLaunchKernelGGL(block_base_kernel, dim3(grid_for(nblocks + 1)), dim3(kBlock), 0, s,
                      nblocks, c.nfree, nc, c.colptr.get(), c.cstart.get(), bbase.get());
