@@ -117,6 +117,7 @@ struct Ctx {
   // relative to its block (noff), per block its first contribution (bbase)
   bool slots = false;
   bool packed = false;              // element matrices stored as packed upper triangles
+  bool inverse = false;             // cslot holds run positions per (adjacency entry, row)
   int obits = 0;                    // offset bits of a slot
   int sum_nc = 1;                   // columns per block of the column sum
   int maxbase = 0;                  // most adjacency entries of one block

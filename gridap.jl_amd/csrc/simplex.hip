@@ -307,6 +307,20 @@ struct FusedRhs {
 // contribution to its run position (cslot holds those positions, 0xffff for rows
 // that are not free).
 // LDS: maxbase bases, then maxq entries (the largest extents over all blocks).
+// Entry (row li, column lj) of a cell's element matrix in the layout of an nloc of
+// N: 81 = the vector-Q2 hexahedron's padded columns, otherwise the packed upper
+// triangle of N(N+1)/2 entries.
+template <int N>
+__device__ __forceinline__ uint32_t elem_addr(uint32_t cell, uint32_t lj, int li) {
+  if constexpr (N == 81) {
+    return (cell * 81u + lj) * (uint32_t)kHexColLd + hex_row_slot(li);
+  } else {
+    constexpr uint32_t NS = N * (N + 1) / 2;
+    const int a = min(li, (int)lj), b = max(li, (int)lj);
+    return cell * NS + (uint32_t)(a * N - a * (a - 1) / 2 + (b - a));
+  }
+}
+
 template <class T, int FULL_NLOC>
 __global__ __launch_bounds__(kBlock) void column_sum_kernel(
     int64_t nfree, int nc, int nloc, int lbits, int obits, int ns, int maxbase,
@@ -335,6 +349,21 @@ __global__ __launch_bounds__(kBlock) void column_sum_kernel(
     oen[r] = i + 1 < np ? noff[p0 + i + 1] : (uint32_t)nq;
   }
   if constexpr (FULL_NLOC > 0) {
+    if (F.b) {  // right-hand side terms of the range's adjacency entries (packed layout)
+      for (int k = tid; k < na; k += kBlock) {
+        const int e = adj[a0 + k];
+        const uint32_t cell = (uint32_t)(e >> lbits), lj = (uint32_t)(e & lmask);
+        double v = F.fdet[cell] * F.mref[lj];
+        uint32_t m = F.dvals ? F.dmask[cell] : 0u;
+        while (m) {
+          const int bb = __ffs(m) - 1;
+          m &= m - 1;
+          const int d = F.cell_dofs[(size_t)cell * nloc + bb];
+          v -= (double)E[elem_addr<FULL_NLOC>(cell, lj, bb)] * F.dvals[-d - 1];
+        }
+        rv[k] = v;
+      }
+    }
     const uint16_t *ip = cslot + (int64_t)a0 * FULL_NLOC;
     const int ne = na * FULL_NLOC;
     for (int eb = 0; eb < ne; eb += kBlock * U) {
@@ -352,10 +381,7 @@ __global__ __launch_bounds__(kBlock) void column_sum_kernel(
         const int e = eb + tid + u * kBlock;
         const int li = e - (e / FULL_NLOC) * FULL_NLOC;
         const uint32_t cell = (uint32_t)(ae[u] >> lbits), lj = (uint32_t)(ae[u] & lmask);
-        v[u] = pos[u] != 0xffffu
-                   ? E[(cell * (uint32_t)FULL_NLOC + lj) * (uint32_t)kHexColLd +
-                       hex_row_slot(li)]
-                   : (T)0;
+        v[u] = pos[u] != 0xffffu ? E[elem_addr<FULL_NLOC>(cell, lj, li)] : (T)0;
       }
 #pragma unroll
       for (int u = 0; u < U; ++u)
@@ -406,7 +432,7 @@ __global__ __launch_bounds__(kBlock) void column_sum_kernel(
     }
   }
   __syncthreads();
-  if (FULL_NLOC == 0 && F.b && tid < j1 - j0) {
+  if (F.b && tid < j1 - j0) {
     const int64_t J = j0 + tid;
     double bs = 0.0;
     for (int k = adjptr[J] - a0, ke = adjptr[J + 1] - a0; k < ke; ++k) bs += rv[k];
@@ -507,6 +533,7 @@ size_t column_sum_lds(int maxbase, int maxq, size_t esz, bool rhs) {
 void compact_contributions(Ctx &c, int lbits) {
   c.slots = false;
   c.packed = false;
+  c.inverse = false;
   c.cslot.release();
   c.noff.release();
   c.bbase.release();
@@ -543,13 +570,16 @@ void compact_contributions(Ctx &c, int lbits) {
   const int64_t nblocks = (c.nfree + nc - 1) / nc;
   DBuf<uint16_t> cslot, noff;
   DBuf<int32_t> bbase;
-  // full rows: the inverse form (ipos), one entry per (free adjacency entry, row)
+  // full columns: the inverse form (ipos: one run position per (free adjacency
+  // entry, row)); packed triangles: per-contribution slots (measured at 96³×6 P2:
+  // 2.31 ms per assembly with slots, 2.43 ms with the inverse form)
+  const bool inverse = !packed;
   int32_t nadj = 0;
-  if (!packed)
+  if (inverse)
     GH_CHECK_HIP(hipMemcpy(&nadj, c.adjptr.get() + c.nfree, 4, hipMemcpyDeviceToHost));
-  const int64_t nslot = packed ? (int64_t)ncontrib : (int64_t)nadj * c.nloc;
+  const int64_t nslot = inverse ? (int64_t)nadj * c.nloc : (int64_t)ncontrib;
   cslot.resize(std::max<int64_t>(nslot, 1));
-  if (!packed)
+  if (inverse)
     GH_CHECK_HIP(hipMemsetAsync(cslot.get(), 0xff, sizeof(uint16_t) * cslot.n, s));
   noff.resize(c.nnz);
   bbase.resize(nblocks + 1);
@@ -558,8 +588,8 @@ void compact_contributions(Ctx &c, int lbits) {
     hipLaunchKernelGGL(contrib_slot_kernel, dim3(grid_for(ncontrib)), dim3(kBlock), 0, s,
                        (int64_t)ncontrib, nc, c.nloc, lbits, obits, ns, c.perm.get(),
                        c.cell_dofs.get(), c.adjptr.get(), c.adj.get(), c.colptr.get(),
-                       c.cstart.get(), packed ? cslot.get() : nullptr,
-                       packed ? nullptr : cslot.get(), dev.get());
+                       c.cstart.get(), inverse ? nullptr : cslot.get(),
+                       inverse ? cslot.get() : nullptr, dev.get());
   hipLaunchKernelGGL(nonzero_offset_kernel, dim3(grid_for(c.nfree * 64)), dim3(kBlock), 0, s,
                      c.nfree, nc, c.colptr.get(), c.cstart.get(), noff.get());
   hipLaunchKernelGGL(block_base_kernel, dim3(grid_for(nblocks + 1)), dim3(kBlock), 0, s,
@@ -582,6 +612,7 @@ void compact_contributions(Ctx &c, int lbits) {
   c.maxq = mx[1];
   c.obits = obits;
   c.packed = packed;
+  c.inverse = inverse;
   c.slots = true;
   c.elem.release();
 }
@@ -644,7 +675,7 @@ void launch_column_sum(Ctx &c, const FusedRhs &F) {
                        c.colptr.get(), c.bbase.get(), c.adjptr.get(), c.adj.get(),
                        c.noff.get(), c.cslot.get(), (const T *)c.elem.get(), c.nzval.get(), F);
   };
-  if (c.packed) go(column_sum_kernel<T, 0>);
+  if (!c.inverse) go(column_sum_kernel<T, 0>);
   else go(column_sum_kernel<T, 81>);
 }
 
