@@ -38,6 +38,9 @@ constexpr int kApplyLanes = 4;   // lanes per row in the FSAI applications
 // scalar slots
 constexpr int kRz = 0;  // [0..1]: r·z, double-buffered by iteration parity
 constexpr int kRr = 2, kBb = 3, kPq = 4;
+// convergence decided on the device: kStop = 0 (running), 1 (r·r met the threshold),
+// 2 (r·r not finite); kStopIt = the iteration at whose start it was decided
+constexpr int kStop = 5, kStopIt = 6;
 
 __device__ __forceinline__ double block_sum(double v, double *sh) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
@@ -80,6 +83,64 @@ __device__ __forceinline__ double sum_partials(const double *__restrict__ part, 
 // partial count, not only the row count, sets the cost of a reduction.
 constexpr int kPcgGrid = 1024;
 static_assert(kPcgGrid <= kMaxParts, "partial slots");
+
+// Σ a[0..np) and Σ b[0..np) in one pass (fixed order, the same in every block).
+__device__ __forceinline__ void sum_partials2(const double *__restrict__ a,
+                                              const double *__restrict__ b, int np, double *sh,
+                                              double &ra, double &rb) {
+  double x = 0.0, y = 0.0;
+  for (int k = threadIdx.x; k < np; k += (int)blockDim.x) {
+    x += a[k];
+    y += b[k];
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    x += __shfl_xor(x, o);
+    y += __shfl_xor(y, o);
+  }
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) {
+    sh[2 * w] = x;
+    sh[2 * w + 1] = y;
+  }
+  __syncthreads();
+  ra = rb = 0.0;
+  for (int k = 0; k < (int)(blockDim.x >> 6); ++k) {
+    ra += sh[2 * k];
+    rb += sh[2 * k + 1];
+  }
+}
+
+// True when the solve has stopped (scal[kStop] set by an earlier kernel).  Read once
+// per workgroup and shared, so the whole workgroup returns or none of it does.
+__device__ __forceinline__ bool block_stopped(const double *scal) {
+  __shared__ int flag;
+  if (threadIdx.x == 0) flag = scal[kStop] != 0.0;
+  __syncthreads();
+  return flag != 0;
+}
+
+// Start of an iteration: r·z (for β) and, after the first iteration of a start,
+// r·r of the previous update against thr = (rtol ‖b‖)²; true when the solve stops
+// here (block 0 records why and at which iteration).  Every block sums the same
+// partials in the same order, so every block takes the same decision.
+__device__ __forceinline__ bool iteration_start(double *scal, const double *__restrict__ part_rz,
+                                                const double *__restrict__ part_rr, int np,
+                                                int first, double thr, int it, double *sh,
+                                                double &rz) {
+  double rr = 0.0;
+  if (first) {
+    rz = sum_partials(part_rz, np, sh);
+    return false;
+  }
+  sum_partials2(part_rz, part_rr, np, sh, rz, rr);
+  const bool stop = !(rr > thr) || !isfinite(rr);
+  if (stop && blockIdx.x == 0 && threadIdx.x == 0) {
+    scal[kStop] = isfinite(rr) ? 1.0 : 2.0;
+    scal[kStopIt] = (double)it;
+  }
+  return stop;
+}
 
 inline unsigned grid_rows(int64_t rows, int rows_per_block) {
   return (unsigned)std::min<int64_t>(
@@ -150,10 +211,13 @@ __global__ __launch_bounds__(kBlock) void spmv_dir_kernel(
     int64_t n, const int32_t *__restrict__ cp, const int32_t *__restrict__ rv,
     const double *__restrict__ val, const double *__restrict__ z,
     const double *__restrict__ pold, double *__restrict__ pnew, double *__restrict__ q,
-    double *scal, int par, int first, const double *__restrict__ part_rz, int np_rz,
+    double *scal, int par, int first, const double *__restrict__ part_rz,
+    const double *__restrict__ part_rr, int np_rz, double thr, int it,
     double *__restrict__ part_pq) {
-  __shared__ double sh[kBlock / 64];
-  const double rz = sum_partials(part_rz, np_rz, sh);
+  __shared__ double sh[2 * (kBlock / 64)];
+  if (block_stopped(scal)) return;
+  double rz;
+  if (iteration_start(scal, part_rz, part_rr, np_rz, first, thr, it, sh, rz)) return;
   const double beta = first ? 0.0 : rz / scal[kRz + (par ^ 1)];
   if (blockIdx.x == 0 && threadIdx.x == 0) scal[kRz + par] = rz;
   const int sub = threadIdx.x % kSpmvLanes;
@@ -190,6 +254,7 @@ __global__ __launch_bounds__(kBlock) void update_kernel(
     const double *__restrict__ q, const double *scal, const double *__restrict__ part_pq,
     int np_pq, double *__restrict__ part_rz, double *__restrict__ part_rr) {
   __shared__ double sh[kBlock / 64];
+  if (block_stopped(scal)) return;
   const double alpha = scal[kRz + par] / sum_partials(part_pq, np_pq, sh);
   double rz = 0.0, rr = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
@@ -219,6 +284,7 @@ __global__ __launch_bounds__(kBlock) void update_lower_kernel(
     const double *scal, const double *__restrict__ part_pq, int np_pq,
     double *__restrict__ part_rr) {
   __shared__ double sh[kBlock / 64];
+  if (block_stopped(scal)) return;
   const double alpha = scal[kRz + par] / sum_partials(part_pq, np_pq, sh);
   const int sub = threadIdx.x % kApplyLanes;
   const int64_t stride = (int64_t)gridDim.x * (kBlock / kApplyLanes);
@@ -267,8 +333,9 @@ __global__ __launch_bounds__(kBlock) void lower_kernel(int64_t n, const float *_
 __global__ __launch_bounds__(kBlock) void upper_rz_kernel(
     int64_t n, const float *__restrict__ ge, const int32_t *__restrict__ gi,
     const double *__restrict__ y, const double *__restrict__ r, double *__restrict__ z,
-    double *__restrict__ part_rz) {
+    const double *scal, double *__restrict__ part_rz) {
   __shared__ double sh[kBlock / 64];
+  if (block_stopped(scal)) return;
   const int sub = threadIdx.x % kApplyLanes;
   const int64_t stride = (int64_t)gridDim.x * (kBlock / kApplyLanes);
   double rz = 0.0;
@@ -368,10 +435,13 @@ __global__ void diag_inv_box_kernel(int64_t n, const double *__restrict__ a27,
 __global__ __launch_bounds__(kBlock) void spmv_dir_box_kernel(
     Box B, const double *__restrict__ a27, const double *__restrict__ z,
     const double *__restrict__ pold, double *__restrict__ pnew, double *__restrict__ q,
-    double *scal, int par, int first, const double *__restrict__ part_rz, int np_rz,
+    double *scal, int par, int first, const double *__restrict__ part_rz,
+    const double *__restrict__ part_rr, int np_rz, double thr, int it,
     double *__restrict__ part_pq) {
-  __shared__ double sh[kBlock / 64];
-  const double rz = sum_partials(part_rz, np_rz, sh);
+  __shared__ double sh[2 * (kBlock / 64)];
+  if (block_stopped(scal)) return;
+  double rz;
+  if (iteration_start(scal, part_rz, part_rr, np_rz, first, thr, it, sh, rz)) return;
   const double beta = first ? 0.0 : rz / scal[kRz + (par ^ 1)];
   if (blockIdx.x == 0 && threadIdx.x == 0) scal[kRz + par] = rz;
   double dot = 0.0;
@@ -403,6 +473,7 @@ __global__ __launch_bounds__(kBlock) void update_lower_box_kernel(
     const double *__restrict__ q, double *__restrict__ y, const double *scal,
     const double *__restrict__ part_pq, int np_pq, double *__restrict__ part_rr) {
   __shared__ double sh[kBlock / 64];
+  if (block_stopped(scal)) return;
   const double alpha = scal[kRz + par] / sum_partials(part_pq, np_pq, sh);
   double rr = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < B.n;
@@ -449,8 +520,10 @@ __global__ __launch_bounds__(kBlock) void lower_box_kernel(Box B, const float *_
 // 26 − o of i is
 __global__ __launch_bounds__(kBlock) void upper_rz_box_kernel(
     Box B, const float *__restrict__ g14, const double *__restrict__ y,
-    const double *__restrict__ r, double *__restrict__ z, double *__restrict__ part_rz) {
+    const double *__restrict__ r, double *__restrict__ z, const double *scal,
+    double *__restrict__ part_rz) {
   __shared__ double sh[kBlock / 64];
+  if (block_stopped(scal)) return;
   double rz = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < B.n;
        i += (int64_t)gridDim.x * kBlock) {
@@ -560,16 +633,17 @@ void pcg_device(Ctx &c, int precond, double rtol, int maxit, gh_solve_stats *st)
     spmv_device(c, c.s_x.get(), c.s_q.get());
     hipLaunchKernelGGL(init_kernel, dim3(g_vec), dim3(kBlock), 0, s, n, c.s_b.get(), c.s_q.get(),
                        dinv, factored, rbuf[0], c.s_z.get(), part_rz, part_rr, part_bb);
+    GH_CHECK_HIP(hipMemsetAsync(scal + kStop, 0, 2 * sizeof(double), s));
     if (factored && box) {
       hipLaunchKernelGGL(lower_box_kernel, dim3(g_apply), dim3(kBlock), 0, s, B, c.g14.get(),
                          rbuf[0], c.s_y.get());
       hipLaunchKernelGGL(upper_rz_box_kernel, dim3(g_apply), dim3(kBlock), 0, s, B,
-                         c.g14.get(), c.s_y.get(), rbuf[0], c.s_z.get(), part_rz);
+                         c.g14.get(), c.s_y.get(), rbuf[0], c.s_z.get(), scal, part_rz);
     } else if (factored) {
       hipLaunchKernelGGL(lower_kernel, dim3(g_apply), dim3(kBlock), 0, s, n, c.gle.get(),
                          c.gli.get(), rbuf[0], c.s_y.get());
       hipLaunchKernelGGL(upper_rz_kernel, dim3(g_apply), dim3(kBlock), 0, s, n, c.gue.get(),
-                         c.gui.get(), c.s_y.get(), rbuf[0], c.s_z.get(), part_rz);
+                         c.gui.get(), c.s_y.get(), rbuf[0], c.s_z.get(), scal, part_rz);
     }
     GH_CHECK_HIP(hipGetLastError());
     GH_CHECK_HIP(hipMemcpyAsync(hrr.data(), part_rr, sizeof(double) * g_vec,
@@ -586,35 +660,41 @@ void pcg_device(Ctx &c, int precond, double rtol, int maxit, gh_solve_stats *st)
     }
     if (it >= maxit || restarts > 3) break;
     ++restarts;
-    const int check = 8;
+    // iterations are enqueued in chunks; each spmv_dir decides on the device whether
+    // the previous update met the tolerance, and once it has, every later kernel of
+    // the chunk returns at once.  The host reads the decision after each chunk.
+    const int chunk = 16;
+    const double thr = (rtol * bnorm) * (rtol * bnorm);
     int k = 0;  // iteration since this start: selects the buffers and scalar slots
-    while (it < maxit) {
-      for (int m = 0; m < check && it < maxit; ++m, ++it, ++k) {
+    bool stopped = false;
+    while (it < maxit && !stopped) {
+      for (int m = 0; m < chunk && it + m < maxit; ++m, ++k) {
         const int par = k & 1;
+        const int itk = it + m;
         double *pold = pbuf[par], *pnew = pbuf[par ^ 1];
         if (box)
           hipLaunchKernelGGL(spmv_dir_box_kernel, dim3(g_spmv), dim3(kBlock), 0, s, B,
                              c.a27.get(), c.s_z.get(), pold, pnew, c.s_q.get(), scal, par,
-                             k == 0 ? 1 : 0, part_rz, np_rz, part_pq);
+                             k == 0 ? 1 : 0, part_rz, part_rr, np_rz, thr, itk, part_pq);
         else
           hipLaunchKernelGGL(spmv_dir_kernel, dim3(g_spmv), dim3(kBlock), 0, s, n,
                              c.colptr.get(), c.rowval.get(), c.nzval.get(), c.s_z.get(), pold,
-                             pnew, c.s_q.get(), scal, par, k == 0 ? 1 : 0, part_rz, np_rz,
-                             part_pq);
+                             pnew, c.s_q.get(), scal, par, k == 0 ? 1 : 0, part_rz, part_rr,
+                             np_rz, thr, itk, part_pq);
         if (factored && box) {
           double *rold = rbuf[par], *rnew = rbuf[par ^ 1];
           hipLaunchKernelGGL(update_lower_box_kernel, dim3(g_apply), dim3(kBlock), 0, s, B, par,
                              c.g14.get(), c.s_x.get(), rold, rnew, pnew, c.s_q.get(),
                              c.s_y.get(), scal, part_pq, (int)g_spmv, part_rr);
           hipLaunchKernelGGL(upper_rz_box_kernel, dim3(g_apply), dim3(kBlock), 0, s, B,
-                             c.g14.get(), c.s_y.get(), rnew, c.s_z.get(), part_rz);
+                             c.g14.get(), c.s_y.get(), rnew, c.s_z.get(), scal, part_rz);
         } else if (factored) {
           double *rold = rbuf[par], *rnew = rbuf[par ^ 1];
           hipLaunchKernelGGL(update_lower_kernel, dim3(g_apply), dim3(kBlock), 0, s, n, par,
                              c.gle.get(), c.gli.get(), c.s_x.get(), rold, rnew, pnew, c.s_q.get(),
                              c.s_y.get(), scal, part_pq, (int)g_spmv, part_rr);
           hipLaunchKernelGGL(upper_rz_kernel, dim3(g_apply), dim3(kBlock), 0, s, n, c.gue.get(),
-                             c.gui.get(), c.s_y.get(), rnew, c.s_z.get(), part_rz);
+                             c.gui.get(), c.s_y.get(), rnew, c.s_z.get(), scal, part_rz);
         } else {
           hipLaunchKernelGGL(update_kernel, dim3(g_vec), dim3(kBlock), 0, s, n, par, dinv,
                              c.s_x.get(), rbuf[0], c.s_z.get(), pnew, c.s_q.get(), scal, part_pq,
@@ -622,13 +702,18 @@ void pcg_device(Ctx &c, int precond, double rtol, int maxit, gh_solve_stats *st)
         }
       }
       GH_CHECK_HIP(hipGetLastError());
-      GH_CHECK_HIP(hipMemcpyAsync(hrr.data(), part_rr, sizeof(double) * np_rr_iter,
-                                  hipMemcpyDeviceToHost, s));
+      double st2[2] = {0.0, 0.0};
+      GH_CHECK_HIP(hipMemcpyAsync(st2, scal + kStop, sizeof(st2), hipMemcpyDeviceToHost, s));
       GH_CHECK_HIP(hipStreamSynchronize(s));
-      const double rr = host_sum(hrr, np_rr_iter);
-      if (!std::isfinite(rr)) break;
-      if (std::sqrt(rr) / bnorm <= rtol) break;
+      if (st2[0] != 0.0) {
+        it = (int)st2[1];  // iterations completed before the stop
+        stopped = true;
+      } else {
+        it = std::min(it + chunk, maxit);
+      }
     }
+    // the true residual of the next start decides convergence (and, if the stopped
+    // r·r was not finite, the restart count ends the solve)
   }
   GH_CHECK_HIP(hipEventRecord(e1, s));
   GH_CHECK_HIP(hipEventSynchronize(e1));
