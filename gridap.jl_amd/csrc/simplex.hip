@@ -26,6 +26,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 
 #include "gh_internal.h"
 
@@ -538,6 +539,10 @@ void compact_contributions(Ctx &c, int lbits) {
   c.noff.release();
   c.bbase.release();
   if (!c.nnz || !c.cstart.get() || !c.perm.get()) return;
+  // GRIDAP_HIP_COMPACT_LISTS=0 keeps the int32 lists (the fallback sum), so the two
+  // sums can be compared on the same system
+  if (const char *env = std::getenv("GRIDAP_HIP_COMPACT_LISTS"))
+    if (env[0] == '0') return;
   hipStream_t s = c.stream;
   int32_t ncontrib = 0;
   GH_CHECK_HIP(hipMemcpy(&ncontrib, c.cstart.get() + c.nnz, 4, hipMemcpyDeviceToHost));
