@@ -6,14 +6,14 @@
 // 81x81 element matrix is built from the 27x27 gradient products
 //     P_ij[a][b] = Σ_q w_q |J_q| g_ai(q) g_bj(q),   g = ∂φ/∂x = J⁻ᵀ ∇̂φ,
 //     K[(a,i),(b,j)] = E_c (λ̂ P_ij[a][b] + μ̂ P_ji[a][b] + μ̂ δ_ij Σ_k P_kk[a][b]),
-// λ̂ = ν/((1+ν)(1-2ν)), μ̂ = 1/(2(1+ν)).  One wave per cell: the CartesianMap
-// Jacobian is diagonal, so g = ĝ/h and w|J| = w·h0·h1·h2 are formed in FP64 from
-// the reference tabulation as the operands are loaded; the nine products run on
-// the matrix cores (v_mfma_f32_32x32x2f32, FP32) and the blocks are formed and
-// stored from the accumulators, or the six distinct ones in FP64 on the vector
-// units; the 81x81 matrix is written column by column (kHexColLd entries per
-// column, so a column's entries from one cell are contiguous), and the sum/rhs
-// passes of the element-list path assemble them.
+// λ̂ = ν/((1+ν)(1-2ν)), μ̂ = 1/(2(1+ν)).  Persistent waves, each walking cells: the
+// CartesianMap Jacobian is diagonal, so g = ĝ/h and w|J| = w·h0·h1·h2 are formed in
+// FP64 from the reference tabulation once per workgroup; the products run on
+// the matrix cores (v_mfma_f32_32x32x2f32 in FP32 mode, v_mfma_f64_16x16x4f64 in
+// FP64 mode) and the blocks are formed and stored from the accumulators; the 81x81
+// matrix is written column by column (kHexColLd entries per column, so a column's
+// entries from one cell are contiguous), and the sum/rhs passes of the element-list
+// path assemble them.
 #include <algorithm>
 #include <cmath>
 #include <vector>
@@ -38,56 +38,6 @@ struct HexArgs {
   void *E_out;                // ncells x 81 x 81 (T)
   double *mvec;               // ncells x 27: Σ_q w |J| φ_a
 };
-
-// FP64 products on the vector units, one wave per cell: the six distinct P_ij
-// (i <= j) go to LDS and the 81x81 matrix is written column by column.
-__global__ __launch_bounds__(64) void hex_elasticity_fp64_kernel(HexArgs A) {
-  const int lane = threadIdx.x;
-  const int64_t cell = blockIdx.x;
-  const double *rg = A.ref, *rw = A.ref + kNq * kNn * 3, *rs = rw + kNq;
-  // CartesianMap: J = diag(h) at every point (src/Geometry/CartesianGrids.jl:259-269),
-  // so g = ∂φ/∂x = ĝ / h and w|J| = w h0 h1 h2 (FP64)
-  const double det = A.h[0] * A.h[1] * A.h[2];
-  const double ih[3] = {1.0 / A.h[0], 1.0 / A.h[1], 1.0 / A.h[2]};
-  // m_a = Σ_q w |J| φ̂_a (FP64)
-  if (lane < kNn) {
-    double m = 0.0;
-    for (int q = 0; q < kNq; ++q) m += rw[q] * det * rs[q * kNn + lane];
-    A.mvec[cell * kNn + lane] = m;
-  }
-  const double Ec = A.E ? A.E[cell] : 1.0;
-  double *out = reinterpret_cast<double *>(A.E_out) + cell * (int64_t)(kNloc * kHexColLd);
-  __shared__ double P[6][kNn][kNn + 1];
-  static const int pi[6] = {0, 1, 2, 0, 0, 1}, pj[6] = {0, 1, 2, 1, 2, 2};
-  for (int e = lane; e < 6 * kNn * kNn; e += 64) {
-    const int pp = e / (kNn * kNn), ab = e % (kNn * kNn), a = ab / kNn, b = ab % kNn;
-    const int i = pi[pp], j = pj[pp];
-    double s = 0.0;
-    for (int q = 0; q < kNq; ++q)
-      s += rw[q] * rg[(q * kNn + a) * 3 + i] * rg[(q * kNn + b) * 3 + j];
-    P[pp][a][b] = s * det * ih[i] * ih[j];
-  }
-  __syncthreads();
-  const double lam = Ec * A.lam, mu = Ec * A.mu;
-  auto pidx = [](int i, int j) {  // slot of the product P_ij with i <= j
-    return i == j ? i : (i == 0 ? (j == 1 ? 3 : 4) : 5);
-  };
-  for (int e = lane; e < kNloc * kNloc; e += 64) {  // column qd, row p
-    const int qd = e / kNloc, p = e - qd * kNloc;
-    const int a = p % kNn, i = p / kNn, b = qd % kNn, j = qd / kNn;
-    double pij, pji;  // P_ji[a][b] = P_ij[b][a]; stored products have i <= j
-    if (i <= j) {
-      pij = P[pidx(i, j)][a][b];
-      pji = P[pidx(i, j)][b][a];
-    } else {
-      pij = P[pidx(j, i)][b][a];
-      pji = P[pidx(j, i)][a][b];
-    }
-    double v = lam * pij + mu * pji;
-    if (i == j) v += mu * (P[0][a][b] + P[1][a][b] + P[2][a][b]);
-    out[qd * kHexColLd + hex_row_slot(p)] = v;
-  }
-}
 
 // FP32 products on the matrix cores, persistent: kHexWaves waves per block, each wave
 // walks cells blockIdx·kHexWaves + wave, stepping by the grid's wave count.  On the
@@ -209,6 +159,139 @@ __attribute__((amdgpu_waves_per_eu(2, 2))) void hex_elasticity_mfma_kernel(
   }
 }
 
+// FP64 products on the matrix cores (v_mfma_f64_16x16x4f64), persistent, kHex64Waves
+// waves per workgroup.  Lane layout of the instruction (checked on MI355X): A lane l =
+// (row l % 16, k l / 16), B lane l = (column l % 16, k l / 16), result value t of lane
+// l = (row 4 t + l / 16, column l % 16).  A 27x27 product is 2x2 tiles of 16x16 over
+// 7 k-steps of 4 quadrature points (zero padding beyond 27): 28 instructions.  The
+// operands (A = g_i w|J|, B = g_j, formed in FP64 as in the FP32 kernel) sit in LDS
+// for the workgroup.  Σ_k P_kk accumulates in one accumulator set; column block j
+// needs P_jj again (recomputed) and the pairs P_ij, P_ji, i != j, which are kept in
+// registers until both of their blocks are placed.  Blocks go through the wave's LDS
+// tile [27 columns][84 row slots] and leave as one contiguous 18 KB column block.
+constexpr int kHex64Waves = 2;
+constexpr int64_t kHex64Blocks = 512;  // persistent grid: 2 workgroups per CU
+constexpr int kHex64S = 7;             // k-steps of 4 quadrature points
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(64 * kHex64Waves) void hex_elasticity_mfma64_kernel(
+    HexArgs A, const double *__restrict__ Ecell, const double *__restrict__ ref,
+    double *__restrict__ Eout, double *__restrict__ mvec) {
+  __shared__ double opa[2][kHex64S][3][64], opb[2][kHex64S][3][64];
+  __shared__ __align__(16) double tiles[kHex64Waves][kNn][kHexColLd];
+  const double *rg = ref, *rw = ref + kNq * kNn * 3, *rs = rw + kNq;
+  const double det = A.h[0] * A.h[1] * A.h[2];
+  const double ih[3] = {1.0 / A.h[0], 1.0 / A.h[1], 1.0 / A.h[2]};
+  for (int e = threadIdx.x; e < 2 * kHex64S * 3 * 64; e += 64 * kHex64Waves) {
+    const int t = e / (kHex64S * 3 * 64), s = (e / (3 * 64)) % kHex64S, k = (e / 64) % 3;
+    const int l = e % 64;
+    const int node = 16 * t + (l & 15), q = 4 * s + (l >> 4);
+    double av = 0.0, bv = 0.0;
+    if (q < kNq && node < kNn) {
+      bv = rg[(q * kNn + node) * 3 + k] * ih[k];
+      av = bv * rw[q] * det;
+    }
+    opa[t][s][k][l] = av;
+    opb[t][s][k][l] = bv;
+  }
+  const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  double m = 0.0;
+  if (lane < kNn)
+    for (int q = 0; q < kNq; ++q) m += rw[q] * det * rs[q * kNn + lane];
+  __syncthreads();
+  struct Prod {
+    v4d c[2][2];
+  };
+  auto zero = [](Prod &P) {
+#pragma unroll
+    for (int ta = 0; ta < 2; ++ta)
+#pragma unroll
+      for (int tb = 0; tb < 2; ++tb) P.c[ta][tb] = v4d{0.0, 0.0, 0.0, 0.0};
+  };
+  auto accum = [&](Prod &P, int i, int j) {
+#pragma unroll
+    for (int s = 0; s < kHex64S; ++s)
+#pragma unroll
+      for (int ta = 0; ta < 2; ++ta)
+#pragma unroll
+        for (int tb = 0; tb < 2; ++tb)
+          P.c[ta][tb] = __builtin_amdgcn_mfma_f64_16x16x4f64(opa[ta][s][i][lane],
+                                                             opb[tb][s][j][lane],
+                                                             P.c[ta][tb], 0, 0, 0);
+  };
+  auto prod = [&](int i, int j) {
+    Prod P;
+    zero(P);
+    accum(P, i, j);
+    return P;
+  };
+  double *tile = &tiles[wave][0][0];
+  const int r4 = lane >> 4, cl = lane & 15;
+  for (int64_t cell = (int64_t)blockIdx.x * kHex64Waves + wave; cell < A.ncells;
+       cell += (int64_t)gridDim.x * kHex64Waves) {
+    if (lane < kNn) mvec[cell * kNn + lane] = m;
+    const double Ec = Ecell ? Ecell[cell] : 1.0;
+    const double lam = Ec * A.lam, mu = Ec * A.mu;
+    double *out = Eout + cell * (int64_t)(kNloc * kHexColLd);
+    // block (i, j) = λ̂ x + μ̂ y (+ μ̂ tr): row a = 16 ta + 4 t + l/16, column b = 16 tb + l%16
+    auto put = [&](int i, const Prod &x, const Prod &y, const Prod *tr) {
+#pragma unroll
+      for (int ta = 0; ta < 2; ++ta)
+#pragma unroll
+        for (int tb = 0; tb < 2; ++tb)
+#pragma unroll
+          for (int t = 0; t < 4; ++t) {
+            const int a = 16 * ta + 4 * t + r4, b = 16 * tb + cl;
+            if (a > kNn || b >= kNn) continue;  // row 27 is the zero padding row
+            double v = lam * x.c[ta][tb][t] + mu * y.c[ta][tb][t];
+            if (tr) v += mu * tr->c[ta][tb][t];
+            tile[b * kHexColLd + 28 * i + a] = v;
+          }
+    };
+    auto flush = [&](int j) {
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      double2 *dst = reinterpret_cast<double2 *>(out + kNn * j * kHexColLd);
+      const double2 *src = reinterpret_cast<const double2 *>(tile);
+      for (int it = lane; it < kNn * kHexColLd / 2; it += 64) dst[it] = src[it];
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    };
+    Prod tr;
+    zero(tr);
+    accum(tr, 0, 0);
+    accum(tr, 1, 1);
+    accum(tr, 2, 2);
+    const Prod p10 = prod(1, 0), p01 = prod(0, 1);
+    {
+      const Prod d = prod(0, 0);
+      put(0, d, d, &tr);
+    }
+    put(1, p10, p01, nullptr);
+    const Prod p20 = prod(2, 0), p02 = prod(0, 2);
+    put(2, p20, p02, nullptr);
+    flush(0);
+    put(0, p01, p10, nullptr);
+    {
+      const Prod d = prod(1, 1);
+      put(1, d, d, &tr);
+    }
+    const Prod p21 = prod(2, 1), p12 = prod(1, 2);
+    put(2, p21, p12, nullptr);
+    flush(1);
+    put(0, p02, p20, nullptr);
+    put(1, p12, p21, nullptr);
+    {
+      const Prod d = prod(2, 2);
+      put(2, d, d, &tr);
+    }
+    flush(2);
+  }
+}
+
 template <class T>
 __global__ void hex_rhs_kernel(int64_t nfree, const int32_t *__restrict__ adjptr,
                                const int32_t *__restrict__ adj, const double *__restrict__ mvec,
@@ -284,8 +367,13 @@ void assemble_elasticity(Ctx &c, int precision) {
                          dim3(64 * kHexWaves), 0, c.stream, A, A.E, A.ref,
                          (float *)A.E_out, A.mvec);
     } else
-      hipLaunchKernelGGL(hex_elasticity_fp64_kernel, dim3((unsigned)c.ncells), dim3(64), 0,
-                         c.stream, A);
+    {
+      const int64_t want = (c.ncells + kHex64Waves - 1) / kHex64Waves;
+      hipLaunchKernelGGL(hex_elasticity_mfma64_kernel,
+                         dim3((unsigned)std::min<int64_t>(want, kHex64Blocks)),
+                         dim3(64 * kHex64Waves), 0, c.stream, A, A.E, A.ref,
+                         (double *)A.E_out, A.mvec);
+    }
   }
   GH_CHECK_HIP(hipGetLastError());
   elemlist_sum(c, f32);
