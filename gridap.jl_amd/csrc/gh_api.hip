@@ -213,8 +213,10 @@ int gh_symbolic(gh_ctx *ctx, int64_t *nnz_out) {
   return guarded(ctx, [&](gh::Ctx &c) {
     GH_REQUIRE(c.have_space, GH_ERR_STATE, "upload the FE space before gh_symbolic");
     invalidate_pattern(c);
-    gh::symbolic_general(c, gh::simplex_eligible(c) || gh::hexq2_eligible(c));
-    gh::check_structured(c);
+    if (!gh::symbolic_lattice(c)) {
+      gh::symbolic_general(c, gh::simplex_eligible(c) || gh::hexq2_eligible(c));
+      gh::check_structured(c);
+    }
     gh::elemlist_symbolic(c);
     if (nnz_out) *nnz_out = c.nnz;
   });

@@ -183,6 +183,9 @@ struct Ctx {
 // element entries that sum into it (cstart, perm)
 void symbolic_general(Ctx &c, bool contributions);
 void check_structured(Ctx &c);
+// the pattern of a Cartesian scalar-Q1 space written directly (no sort); false when the
+// numbering is not CLagrangian node order (then symbolic_general + check_structured)
+bool symbolic_lattice(Ctx &c);
 void build_coloring(Ctx &c);
 // assemble.hip
 void assemble_diffusion(Ctx &c, int precision, int path);

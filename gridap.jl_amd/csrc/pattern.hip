@@ -202,6 +202,83 @@ __global__ void lattice_check_pattern(Lattice L, int64_t nnodes,
   fmask[n] = m;
 }
 
+// ---- lattice symbolic phase (no sort) ----
+// With the numbering verified to be node order (free dof = 1 + number of free nodes
+// before it), column n of the pattern is the free lattice neighbours of node n in
+// ascending node order, which is ascending dof order: two nodes of a Q1 lattice share
+// a cell exactly when every coordinate differs by at most one.
+
+// Per node: the free-neighbour mask (bit o, o = Σ_k (δ_k+1) 3^k), its column length
+// (0 for a node that owns no column), and the numbering checks.
+__global__ void lattice_masks(Lattice L, int64_t nnodes, const int32_t *__restrict__ node_dof,
+                              const int32_t *__restrict__ scan, uint32_t *__restrict__ fmask,
+                              int32_t *__restrict__ count, int32_t *bad) {
+  int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= nnodes) return;
+  fmask[n] = 0u;
+  count[n] = 0;
+  const int32_t d = node_dof[n];
+  if (d == 0) {
+    atomicOr(bad, 2);
+    return;
+  }
+  if (d < 0) return;
+  if (d != scan[n] + 1) {
+    atomicOr(bad, 4);
+    return;
+  }
+  int64_t xi[3] = {0, 0, 0}, r = n;
+  for (int k = 0; k < L.D; ++k) {
+    xi[k] = r % L.np[k];
+    r /= L.np[k];
+  }
+  int nb = 1;
+  for (int k = 0; k < L.D; ++k) nb *= 3;
+  uint32_t m = 0u;
+  for (int o = 0; o < nb; ++o) {
+    int oo = o;
+    int64_t node = 0, stride = 1;
+    bool in = true;
+    for (int k = 0; k < L.D; ++k) {
+      const int64_t v = xi[k] + (oo % 3) - 1;
+      oo /= 3;
+      in = in && v >= 0 && v < L.np[k];
+      node += v * stride;
+      stride *= L.np[k];
+    }
+    if (in && node_dof[node] > 0) m |= 1u << o;
+  }
+  fmask[n] = m;
+  count[n] = __popc(m);
+}
+
+// colptr and rowval of every free node's column
+__global__ void lattice_columns(Lattice L, int64_t nnodes, const int32_t *__restrict__ node_dof,
+                                const uint32_t *__restrict__ fmask,
+                                const int32_t *__restrict__ start, int32_t *__restrict__ colptr,
+                                int32_t *__restrict__ rowval) {
+  int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= nnodes) return;
+  const int32_t d = node_dof[n];
+  if (d <= 0) return;
+  int32_t p = start[n];
+  colptr[d - 1] = p;
+  const uint32_t m = fmask[n];
+  int64_t stride[3] = {1, L.np[0], L.np[0] * L.np[1]};
+  int nb = 1;
+  for (int k = 0; k < L.D; ++k) nb *= 3;
+  for (int o = 0; o < nb; ++o) {
+    if (!((m >> o) & 1u)) continue;
+    int oo = o;
+    int64_t node = n;
+    for (int k = 0; k < L.D; ++k) {
+      node += ((oo % 3) - 1) * stride[k];
+      oo /= 3;
+    }
+    rowval[p++] = node_dof[node] - 1;
+  }
+}
+
 // nzval start of each tile's columns: colptr of the first free node's dof, -1
 // when the tile has no free node.  The numeric kernel reads it with one scalar
 // load instead of deriving it from the loaded dofs.
@@ -344,32 +421,146 @@ void symbolic_general(Ctx &c, bool contributions) {
 
 // Free dofs forming a full box of lattice nodes (numbered in node order, which the
 // structured check has verified): the dof offsets of the 27 neighbours are constant.
+// In node order a box's corners hold dofs 1 and N; it is a box when the corners span
+// exactly N nodes and no free node lies outside them.
+__global__ void box_corners(int64_t nnodes, int64_t nfree, const int32_t *__restrict__ node_dof,
+                            int64_t *__restrict__ corner) {
+  int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= nnodes) return;
+  const int32_t d = node_dof[n];
+  if (d == 1) corner[0] = n;
+  if (d == nfree) corner[1] = n;
+}
+
+__global__ void box_outside(int64_t nnodes, int64_t np0, int64_t np1, const int64_t *lo,
+                            const int64_t *hi, const int32_t *__restrict__ node_dof,
+                            int32_t *bad) {
+  int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= nnodes || node_dof[n] <= 0) return;
+  const int64_t x[3] = {n % np0, (n / np0) % np1, n / (np0 * np1)};
+  for (int d = 0; d < 3; ++d)
+    if (x[d] < lo[d] || x[d] > hi[d]) atomicOr(bad, 1);
+}
+
 void detect_box(Ctx &c) {
   c.box = false;
   c.a27.release();
   c.g14.release();
-  if (!c.structured || c.dim != 3) return;
-  std::vector<int32_t> nd(c.nnodes);
-  GH_CHECK_HIP(hipMemcpy(nd.data(), c.node_dof.get(), sizeof(int32_t) * c.nnodes,
-                         hipMemcpyDeviceToHost));
+  if (!c.structured || c.dim != 3 || c.nfree <= 0) return;
+  hipStream_t s = c.stream;
+  DBuf<int64_t> dv;
+  dv.resize(8);  // corners, then lo[3], hi[3]
+  hipLaunchKernelGGL(box_corners, dim3(grid_for(c.nnodes)), dim3(kBlock), 0, s, c.nnodes,
+                     c.nfree, c.node_dof.get(), dv.get());
+  int64_t corner[2] = {0, 0};
+  GH_CHECK_HIP(hipMemcpyAsync(corner, dv.get(), sizeof(corner), hipMemcpyDeviceToHost, s));
+  GH_CHECK_HIP(hipStreamSynchronize(s));
   const int64_t np0 = c.part[0] + 1, np1 = c.part[1] + 1;
-  int64_t lo[3] = {INT64_MAX, INT64_MAX, INT64_MAX}, hi[3] = {-1, -1, -1};
-  int64_t nfree = 0;
-  for (int64_t n = 0; n < c.nnodes; ++n) {
-    if (nd[n] <= 0) continue;
-    const int64_t x[3] = {n % np0, (n / np0) % np1, n / (np0 * np1)};
-    for (int d = 0; d < 3; ++d) {
-      lo[d] = std::min(lo[d], x[d]);
-      hi[d] = std::max(hi[d], x[d]);
-    }
-    ++nfree;
+  int64_t lh[6];
+  for (int k = 0; k < 2; ++k) {
+    const int64_t n = corner[k];
+    lh[3 * k + 0] = n % np0;
+    lh[3 * k + 1] = (n / np0) % np1;
+    lh[3 * k + 2] = n / (np0 * np1);
   }
-  if (nfree != c.nfree || nfree == 0) return;
   int64_t vol = 1;
-  for (int d = 0; d < 3; ++d) vol *= hi[d] - lo[d] + 1;
-  if (vol != nfree) return;
-  for (int d = 0; d < 3; ++d) c.box_m[d] = hi[d] - lo[d] + 1;
+  for (int d = 0; d < 3; ++d) {
+    if (lh[3 + d] < lh[d]) return;
+    vol *= lh[3 + d] - lh[d] + 1;
+  }
+  if (vol != c.nfree) return;
+  DBuf<int32_t> bad;
+  bad.resize(1);
+  GH_CHECK_HIP(hipMemsetAsync(bad.get(), 0, sizeof(int32_t), s));
+  GH_CHECK_HIP(hipMemcpyAsync(dv.get() + 2, lh, sizeof(lh), hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(box_outside, dim3(grid_for(c.nnodes)), dim3(kBlock), 0, s, c.nnodes, np0,
+                     np1, dv.get() + 2, dv.get() + 5, c.node_dof.get(), bad.get());
+  GH_CHECK_HIP(hipGetLastError());
+  int32_t hbad = 0;
+  GH_CHECK_HIP(hipMemcpyAsync(&hbad, bad.get(), 4, hipMemcpyDeviceToHost, s));
+  GH_CHECK_HIP(hipStreamSynchronize(s));
+  if (hbad) return;
+  for (int d = 0; d < 3; ++d) c.box_m[d] = lh[3 + d] - lh[d] + 1;
   c.box = true;
+}
+
+// The pattern of a Cartesian scalar-Q1 space without the sort: lattice node ↔ dof map
+// from cell_dofs (consistency checked), numbering checked to be node order, then each
+// column written directly.  False (nothing set) when a check fails; the caller then
+// runs symbolic_general.  On success the lattice kernel's inputs (node_dof,
+// node_fmask, tile starts) are set as check_structured sets them.
+bool symbolic_lattice(Ctx &c) {
+  c.structured = false;
+  if (!(c.cartesian && c.reffe_kind == GH_REFFE_QK && c.order == 1 && c.ncomp == 1 &&
+        c.nloc == (1 << c.dim) && c.nfree > 0 && c.nnodes < (int64_t(1) << 31)))
+    return false;
+  hipStream_t s = c.stream;
+  Lattice L;
+  L.D = c.dim;
+  for (int d = 0; d < 3; ++d) {
+    L.nc[d] = d < c.dim ? c.part[d] : 1;
+    L.np[d] = d < c.dim ? c.part[d] + 1 : 1;
+  }
+  DBuf<int32_t> node_dof, bad, f, scan, count, start;
+  DBuf<uint32_t> fmask;
+  node_dof.resize(c.nnodes);
+  fmask.resize(c.nnodes);
+  bad.resize(1);
+  f.resize(c.nnodes);
+  scan.resize(c.nnodes);
+  count.resize(c.nnodes);
+  start.resize(c.nnodes);
+  GH_CHECK_HIP(hipMemsetAsync(node_dof.get(), 0, sizeof(int32_t) * c.nnodes, s));
+  GH_CHECK_HIP(hipMemsetAsync(bad.get(), 0, sizeof(int32_t), s));
+  const int64_t t = c.ncells * c.nloc;
+  hipLaunchKernelGGL(lattice_scatter, dim3(grid_for(t)), dim3(kBlock), 0, s, L, c.ncells, c.nloc,
+                     c.cell_dofs.get(), node_dof.get());
+  hipLaunchKernelGGL(lattice_verify, dim3(grid_for(t)), dim3(kBlock), 0, s, L, c.ncells, c.nloc,
+                     c.cell_dofs.get(), node_dof.get(), bad.get());
+  hipLaunchKernelGGL(free_flags, dim3(grid_for(c.nnodes)), dim3(kBlock), 0, s, c.nnodes,
+                     node_dof.get(), f.get(), bad.get());
+  GH_CHECK_HIP(hipGetLastError());
+  exclusive_sum(f.get(), scan.get(), c.nnodes, s);
+  hipLaunchKernelGGL(lattice_masks, dim3(grid_for(c.nnodes)), dim3(kBlock), 0, s, L, c.nnodes,
+                     node_dof.get(), scan.get(), fmask.get(), count.get(), bad.get());
+  GH_CHECK_HIP(hipGetLastError());
+  exclusive_sum(count.get(), start.get(), c.nnodes, s);
+  int32_t hbad = 0, last_scan = 0, last_f = 0, last_start = 0, last_count = 0;
+  GH_CHECK_HIP(hipMemcpy(&hbad, bad.get(), 4, hipMemcpyDeviceToHost));
+  GH_CHECK_HIP(hipMemcpy(&last_scan, scan.get() + c.nnodes - 1, 4, hipMemcpyDeviceToHost));
+  GH_CHECK_HIP(hipMemcpy(&last_f, f.get() + c.nnodes - 1, 4, hipMemcpyDeviceToHost));
+  GH_CHECK_HIP(hipMemcpy(&last_start, start.get() + c.nnodes - 1, 4, hipMemcpyDeviceToHost));
+  GH_CHECK_HIP(hipMemcpy(&last_count, count.get() + c.nnodes - 1, 4, hipMemcpyDeviceToHost));
+  if (hbad != 0 || (int64_t)last_scan + last_f != c.nfree) return false;
+  const int64_t nnz = (int64_t)last_start + last_count;
+  c.nnz = nnz;
+  c.colptr.resize(c.nfree + 1);
+  c.rowval.resize(std::max<int64_t>(nnz, 1));
+  hipLaunchKernelGGL(lattice_columns, dim3(grid_for(c.nnodes)), dim3(kBlock), 0, s, L, c.nnodes,
+                     node_dof.get(), fmask.get(), start.get(), c.colptr.get(), c.rowval.get());
+  const int32_t nnz32 = (int32_t)nnz;
+  GH_CHECK_HIP(hipMemcpyAsync(c.colptr.get() + c.nfree, &nnz32, 4, hipMemcpyHostToDevice, s));
+  GH_CHECK_HIP(hipGetLastError());
+  // the lattice kernel's inputs
+  std::swap(c.node_dof.p, node_dof.p);
+  std::swap(c.node_dof.n, node_dof.n);
+  std::swap(c.node_fmask.p, fmask.p);
+  std::swap(c.node_fmask.n, fmask.n);
+  const int64_t ntiles = (c.nnodes + kTileNodes - 1) / kTileNodes;
+  c.tile_p0.resize(ntiles);
+  c.tile_d0.resize(ntiles);
+  hipLaunchKernelGGL(lattice_tile_p0, dim3(grid_for(ntiles)), dim3(kBlock), 0, s, c.nnodes,
+                     ntiles, c.node_dof.get(), c.colptr.get(), c.tile_p0.get(), c.tile_d0.get());
+  GH_CHECK_HIP(hipGetLastError());
+  GH_CHECK_HIP(hipStreamSynchronize(s));
+  c.cstart.release();
+  c.perm.release();
+  c.have_pattern = true;
+  c.structured = true;
+  c.nzval.resize(std::max<int64_t>(nnz, 1));
+  c.b.resize(std::max<int64_t>(c.nfree, 1));
+  detect_box(c);
+  return true;
 }
 
 void check_structured(Ctx &c) {
