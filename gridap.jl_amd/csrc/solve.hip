@@ -24,6 +24,7 @@
 // partials itself at the convergence checks.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 
 #include "gh_internal.h"
 
@@ -580,8 +581,11 @@ void pcg_device(Ctx &c, int precond, double rtol, int maxit, gh_solve_stats *st)
   double *dinv = nullptr;
   const int factored = precond == GH_PRECOND_FSAI;
   // box lattice: stencil kernels (A as 27 offset arrays, G as 14), one thread per row
+  // GRIDAP_HIP_STENCIL_PCG=0 runs the general (CSR) kernels on a box lattice too, so
+  // the two forms can be compared on the same system
+  const char *stencil_env = std::getenv("GRIDAP_HIP_STENCIL_PCG");
   bool box = c.box && n > 0 && n < (int64_t(1) << 31) &&
-             (!factored || c.g14.get() != nullptr);
+             (!factored || c.g14.get() != nullptr) && !(stencil_env && stencil_env[0] == '0');
   const Box B{n, c.box_m[0], c.box_m[1], c.box_m[2]};
   if (box) {
     c.a27.resize((size_t)27 * nn);

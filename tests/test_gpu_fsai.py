@@ -246,3 +246,23 @@ def test_trained_network_beats_jacobi_on_heldout():
         _, su = inst.solve("fsai")
         assert abs(su.iterations - sj.iterations) <= 8
     inst.ctx.close()
+
+
+def test_stencil_pcg_matches_general_pcg(case, monkeypatch):
+    """On a box lattice PCG runs stencil kernels (A as 27 offset arrays, G as 14, Gᵀ from
+    mirrored offsets); GRIDAP_HIP_STENCIL_PCG=0 runs the general CSR / 16-slot kernels on
+    the same system.  Both must reach the same solution with the same iteration count
+    (up to rounding: the sums run in another order)."""
+    announce("fsai: stencil PCG vs general PCG (Jacobi and FSAI)")
+    ctx = case["ctx"]
+    ctx.mlp_set(random_params(9, scale4=0.05))
+    ctx.fsai_build()
+    for precond in (_lib.GH_PRECOND_JACOBI, _lib.GH_PRECOND_FSAI):
+        monkeypatch.setenv("GRIDAP_HIP_STENCIL_PCG", "1")
+        xs, ss = ctx.pcg(precond=precond, rtol=1e-10)
+        monkeypatch.setenv("GRIDAP_HIP_STENCIL_PCG", "0")
+        xg, sg = ctx.pcg(precond=precond, rtol=1e-10)
+        assert ss.converged == 1 and sg.converged == 1
+        assert ss.precond_used == precond and sg.precond_used == precond
+        assert abs(ss.iterations - sg.iterations) <= 1
+        np.testing.assert_allclose(xs, xg, rtol=1e-8, atol=1e-12 * np.max(np.abs(xg)))
