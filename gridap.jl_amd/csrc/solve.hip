@@ -38,7 +38,6 @@ constexpr int kApplyLanes = 4;   // lanes per row in the FSAI applications
 
 // scalar slots
 constexpr int kRz = 0;  // [0..1]: r·z, double-buffered by iteration parity
-constexpr int kRr = 2, kBb = 3, kPq = 4;
 // convergence decided on the device: kStop = 0 (running), 1 (r·r met the threshold),
 // 2 (r·r not finite); kStopIt = the iteration at whose start it was decided
 constexpr int kStop = 5, kStopIt = 6;
@@ -626,7 +625,6 @@ void pcg_device(Ctx &c, int precond, double rtol, int maxit, gh_solve_stats *st)
   const unsigned g_apply = box ? g_vec : grid_rows(n, kBlock / kApplyLanes);
   // partial counts of each quantity, by producer
   const int np_rz = factored ? (int)g_apply : (int)g_vec;
-  const int np_rr_iter = factored ? (int)g_apply : (int)g_vec;
   std::vector<double> hrr(kMaxParts), hbb(kMaxParts);
 
   double bnorm = 1.0, relres = INFINITY;
