@@ -42,6 +42,17 @@ void invalidate_pattern(gh::Ctx &c) {
   c.fsai_built = false;
   c.elemlist = false;
   c.sref_ready = false;
+  // state derived from the previous pattern (set again by the symbolic phase when it
+  // applies to the new one)
+  c.box = false;
+  c.a27.release();
+  c.g14.release();
+  c.slots = false;
+  c.packed = false;
+  c.inverse = false;
+  c.cslot.release();
+  c.noff.release();
+  c.bbase.release();
 }
 
 }  // namespace
