@@ -266,3 +266,36 @@ def test_stencil_pcg_matches_general_pcg(case, monkeypatch):
         assert ss.precond_used == precond and sg.precond_used == precond
         assert abs(ss.iterations - sg.iterations) <= 1
         np.testing.assert_allclose(xs, xg, rtol=1e-8, atol=1e-12 * np.max(np.abs(xg)))
+
+
+def test_context_reuse_drops_box_state(oracle):
+    """One context, two spaces in turn: the first has its free dofs on a box of lattice
+    nodes (stencil PCG), the second does not (a single Dirichlet vertex).  Nothing of the
+    first pattern's box form may reach the second solve: both match scipy."""
+    part = (5, 4, 3)
+    announce("context reuse: box space, then a non-box space, PCG vs scipy")
+    model = G.CartesianDiscreteModel((0, 1, 0, 1, 0, 1), part)
+    reffe = G.ReferenceFE(G.lagrangian, G.Float64, 1)
+    q = G.Quadrature(3, 2)
+    shp, grad = reffe.tabulate(3, q.coords)
+    C = int(np.prod(part))
+    kappa = np.exp(np.random.default_rng(5).normal(0, 1, C))
+    ctx = _lib.Context(0)
+    ctx.mesh_cartesian(model.origin, model.sizes, model.partition)
+    for tags in ("boundary", ["tag_01"]):
+        V = G.TestFESpace(model, reffe, dirichlet_tags=tags)
+        cd = V.get_cell_dof_ids()
+        ptrs = (np.arange(cd.shape[0] + 1, dtype=np.int64) * cd.shape[1] + 1).astype(np.int32)
+        ctx.space_upload(_lib.GH_REFFE_QK, 1, 1, cd.ravel(), ptrs, V.nfree, V.ndirichlet)
+        ctx.reffe_tabulation(q.weights, shp, grad)
+        ctx.symbolic()
+        ctx.coef_upload(kappa=kappa, fsrc=np.ones(C))
+        ctx.assemble(_lib.GH_FORM_DIFFUSION, _lib.GH_PREC_FP64, _lib.GH_PATH_AUTO)
+        x, st = ctx.pcg(precond=_lib.GH_PRECOND_JACOBI, rtol=1e-10)
+        assert st.converged == 1
+        colptr, rowval = ctx.pattern_get()
+        nz, b = ctx.values_get()
+        A = oracle.to_scipy(colptr, rowval, nz, V.nfree).tocsc()
+        xs = spla.spsolve(A, b)
+        assert np.max(np.abs(x - xs)) <= 1e-8 * np.max(np.abs(xs))
+    ctx.close()
