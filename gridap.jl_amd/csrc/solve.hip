@@ -357,9 +357,12 @@ __global__ __launch_bounds__(kBlock) void upper_rz_kernel(
 
 // ---- box lattice: stencil kernels ---------------------------------------------------
 // Row i ↔ box node (x, y, z); neighbour offset o = (dx+1) + 3 (dy+1) + 9 (dz+1) has dof
-// offset dx + m0 dy + m0 m1 dz, ascending in o.  A is held as a27[o·N + i] (zero where
-// the neighbour is outside the box), G as g14[o·N + i] for o ≤ 13 (lower offsets);
-// Gᵀ row i collects g14[o·N + i − off(o)] from the mirrored neighbours.
+// offset dx + m0 dy + m0 m1 dz, ascending in o.  The assembled lattice matrix is
+// symmetric bit for bit (both entries of a pair sum the same cells in the same order),
+// so A is held as its lower half a27[o·N + i], o ≤ 13 (zero where the neighbour is
+// outside the box), and A(i, i + off(o)) for o > 13 is read as row i + off(o)'s entry
+// at offset 26 − o.  G is g14[o·N + i] for o ≤ 13 (lower offsets); Gᵀ row i collects
+// g14[o·N + i − off(o)] from the mirrored neighbours.
 
 struct Box {
   int64_t n, m0, m1, m2;
@@ -384,10 +387,10 @@ __device__ __forceinline__ int64_t box_off(const Box &B, int o) {
   return (int64_t)(o % 3 - 1) + B.m0 * ((o / 3) % 3 - 1) + B.m0 * B.m1 * (o / 9 - 1);
 }
 
-// a27 from the CSR rows of the box-lattice matrix.  A lattice row holds exactly the
-// neighbours that lie in the box, in ascending column order (the symbolic phase has
-// verified the pattern), which is ascending offset order: the row's k-th entry
-// belongs to the k-th in-box offset.  A block of kConvRows rows first stages its
+// a27 (lower half, o ≤ 13) from the CSR rows of the box-lattice matrix.  A lattice
+// row holds exactly the neighbours that lie in the box, in ascending column order (the
+// symbolic phase has verified the pattern), which is ascending offset order: the
+// row's k-th entry belongs to the k-th in-box offset.  A block of kConvRows rows first stages its
 // contiguous run of values in LDS with coalesced loads; then all lanes write array o
 // together (coalesced, zeros outside the box written explicitly).  A row whose
 // length differs from its in-box neighbour count is flagged.  32-bit indices
@@ -417,7 +420,7 @@ __global__ __launch_bounds__(kConvRows) void stencil_values_kernel(
   for (int o = 0; o < 27; ++o) {
     double v = 0.0;
     if (((vm >> o) & 1u) && ok) v = sv[p++];
-    a27[(size_t)o * n + i] = v;
+    if (o <= 13) a27[(size_t)o * n + i] = v;  // the upper half is read from the mirror
   }
   if (!ok) atomicOr(bad, 1);
 }
@@ -456,7 +459,9 @@ __global__ __launch_bounds__(kBlock) void spmv_dir_box_kernel(
       if (!((vm >> o) & 1u)) continue;
       const int64_t j = i + box_off(B, o);
       const double pj = first ? z[j] : z[j] + beta * pold[j];
-      s += a27[(int64_t)o * B.n + i] * pj;
+      // A(i, j) for an upper offset is A(j, i), row j's lower entry at offset 26 - o
+      const double a = o <= 13 ? a27[(int64_t)o * B.n + i] : a27[(int64_t)(26 - o) * B.n + j];
+      s += a * pj;
     }
     const double pn = first ? z[i] : z[i] + beta * pold[i];
     pnew[i] = pn;
@@ -587,7 +592,7 @@ void pcg_device(Ctx &c, int precond, double rtol, int maxit, gh_solve_stats *st)
              (!factored || c.g14.get() != nullptr) && !(stencil_env && stencil_env[0] == '0');
   const Box B{n, c.box_m[0], c.box_m[1], c.box_m[2]};
   if (box) {
-    c.a27.resize((size_t)27 * nn);
+    c.a27.resize((size_t)14 * nn);
     DBuf<int> bad;
     bad.resize(1);
     GH_CHECK_HIP(hipMemsetAsync(bad.get(), 0, sizeof(int), s));
