@@ -11,7 +11,8 @@ full-assembly time).
 Prints one JSON line.  `roofline` uses the algorithmic bytes per launch
 8*NNZ + 32*C + 8*C + 8*N and the kernel's average duration from HIP events on the
 library's stream.  `cpu_baseline` times the CPU oracle's numeric re-assembly of
-the same system (1 thread).
+the same system (1 thread); `cpu_baseline_omp` the same loop on every host core
+the process may use (OpenMP, parity-coloured cells).
 """
 import argparse
 import ctypes as C
@@ -229,6 +230,23 @@ def main():
         }
         out["gpu_vs_cpu_rel_frobenius"] = float(np.linalg.norm(nzval - nz_cpu) /
                                                 np.linalg.norm(nz_cpu))
+        # the same loop on all host cores this process may use (OpenMP, cells split into
+        # 2^D parity colours so that threads never add into one entry)
+        nthreads = len(os.sched_getaffinity(0))
+        if os.environ.get("OMP_NUM_THREADS", "").isdigit():
+            nthreads = min(nthreads, int(os.environ["OMP_NUM_THREADS"]))
+        t = time.time()
+        nz_omp, _ = O.numeric_refill_cartesian_par(D, model.sizes, model.partition, cd,
+                                                   q.weights, shp, grad, kappa, None, colptr,
+                                                   rowval, nthreads)
+        omp_s = time.time() - t
+        out["cpu_baseline_omp"] = {
+            "value": nnz / omp_s, "unit": "NNZ/s", "cores": nthreads, "kind": "port",
+            "sample": "the same restatement on %d OpenMP threads (cells in 2^3 parity "
+                      "colours), full %d^3 workload once, %.2f s" % (nthreads, n, omp_s),
+            "rel_diff_vs_1_thread": float(np.abs(nz_omp - nz_cpu).max() /
+                                          np.abs(nz_cpu).max()),
+        }
     ctx.close()
 
     if not args.no_fsai:

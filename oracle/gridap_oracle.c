@@ -755,6 +755,66 @@ int gor_numeric_refill_cartesian(int D, const double *sizes, int64_t ncells, int
   return status;
 }
 
+/* The same numeric loop on `nthreads` OpenMP threads, for timing only.  Gridap's
+ * loop is sequential; here the cells of a Cartesian partition part[0..D-1] (cell
+ * c = i0 + part0·(i1 + part1·i2)) are split into 2^D colours by the parity of their
+ * coordinates.  Two cells of one colour share no node, so the threads of one colour
+ * never add into the same nonzero or rhs entry.  Within a nonzero the additions come
+ * in colour order, not cell order, so the result agrees with the sequential loop to
+ * rounding only.  Returns -1 if an entry is missing. */
+int gor_numeric_refill_cartesian_par(int D, const double *sizes, const int64_t *part,
+                                     int nloc, const int32_t *cell_dofs, int npts,
+                                     const double *qw, const double *shp,
+                                     const double *shp_grad, const double *kappa,
+                                     const double *fsrc, int64_t n, const int64_t *colptr,
+                                     const int64_t *rowval, double *nzval, double *b,
+                                     int nthreads) {
+  int64_t p0 = part[0], p1 = D > 1 ? part[1] : 1, p2 = D > 2 ? part[2] : 1;
+  int status = 0;
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+  for (int64_t k = 0; k < colptr[n] - 1; ++k) nzval[k] = 0.0;
+  for (int64_t k = 0; k < n; ++k) b[k] = 0.0;
+  for (int col = 0; col < (1 << D); ++col) {
+    int64_t o0 = col & 1, o1 = (col >> 1) & 1, o2 = (col >> 2) & 1;
+    int64_t m0 = (p0 - o0 + 1) / 2, m1 = (p1 - o1 + 1) / 2, m2 = (p2 - o2 + 1) / 2;
+    int64_t mc = m0 * m1 * m2;
+#pragma omp parallel num_threads(nthreads) reduction(| : status)
+    {
+      double *gphys = (double *)malloc(sizeof(double) * npts * nloc * D);
+      double *dV = (double *)malloc(sizeof(double) * npts);
+      double *Jt = (double *)calloc(npts * D * D, sizeof(double));
+      double *K = (double *)malloc(sizeof(double) * nloc * nloc);
+      double *r = (double *)malloc(sizeof(double) * nloc);
+      for (int p = 0; p < npts; ++p)
+        for (int d = 0; d < D; ++d) Jt[(p * D + d) * D + d] = sizes[d];
+#pragma omp for schedule(static)
+      for (int64_t t = 0; t < mc; ++t) {
+        int64_t i0 = o0 + 2 * (t % m0), i1 = o1 + 2 * ((t / m0) % m1),
+                i2 = o2 + 2 * (t / (m0 * m1));
+        int64_t c = i0 + p0 * (i1 + p1 * i2);
+        diffusion_cell(D, nloc, npts, qw, shp_grad, Jt, kappa ? kappa[c] : 1.0, K, gphys, dV);
+        source_cell(nloc, npts, shp, dV, fsrc ? fsrc[c] : 1.0, r);
+        const int32_t *dofs = cell_dofs + c * nloc;
+        for (int lj = 0; lj < nloc; ++lj) {
+          int64_t j = dofs[lj];
+          if (j <= 0) continue;
+          for (int li = 0; li < nloc; ++li) {
+            int64_t i = dofs[li];
+            if (i <= 0) continue;
+            int64_t p = nz_index(colptr, rowval, i, j);
+            if (p < 0) { status = -1; continue; }
+            nzval[p - 1] += K[lj * nloc + li];
+          }
+        }
+        for (int li = 0; li < nloc; ++li)
+          if (dofs[li] > 0) b[dofs[li] - 1] += r[li];
+      }
+      free(gphys); free(dV); free(Jt); free(K); free(r);
+    }
+  }
+  return status ? -1 : 0;
+}
+
 /* assemble_vector with ArrayBuilder: dense zeros, then b[i] += v for i > 0 in
  * cell order (src/Algebra/AlgebraInterfaces.jl:202-210,380-435). */
 void gor_assemble_vector(int64_t ncells, int nloc, const int32_t *cell_dofs,

@@ -87,6 +87,15 @@ int gor_numeric_refill_cartesian(int D, const double *sizes, int64_t ncells, int
                                  const double *kappa, const double *fsrc, int64_t n,
                                  const int64_t *colptr, const int64_t *rowval, double *nzval,
                                  double *b);
+/* gor_numeric_refill_cartesian on nthreads OpenMP threads (parity colouring of the
+ * Cartesian cells; agrees with it to rounding).  For timing the CPU baseline. */
+int gor_numeric_refill_cartesian_par(int D, const double *sizes, const int64_t *part,
+                                     int nloc, const int32_t *cell_dofs, int npts,
+                                     const double *qw, const double *shp,
+                                     const double *shp_grad, const double *kappa,
+                                     const double *fsrc, int64_t n, const int64_t *colptr,
+                                     const int64_t *rowval, double *nzval, double *b,
+                                     int nthreads);
 void gor_assemble_vector(int64_t ncells, int nloc, const int32_t *cell_dofs,
                          const double *cellvecs, int64_t nrows, double *b);
 

@@ -74,6 +74,9 @@ def lib():
         L.gor_numeric_refill_cartesian.argtypes = [i32, _f64p, i64, i32, _i32p, i32, _f64p,
                                                    _f64p, _f64p, vp, vp, i64, _i64p, _i64p,
                                                    _f64p, _f64p]
+        L.gor_numeric_refill_cartesian_par.argtypes = [i32, _f64p, _i64p, i32, _i32p, i32,
+                                                       _f64p, _f64p, _f64p, vp, vp, i64,
+                                                       _i64p, _i64p, _f64p, _f64p, i32]
         L.gor_csc_spmv.argtypes = [i64, _i64p, _i64p, _f64p, _f64p, _f64p]
         L.gor_pcg_jacobi.argtypes = [i64, _i64p, _i64p, _f64p, _f64p, _f64p, dbl, i32,
                                      C.POINTER(C.c_int), C.POINTER(dbl)]
@@ -330,6 +333,27 @@ def numeric_refill_cartesian(D, sizes, cell_dofs, qw, shp, grad, kappa, fsrc, co
         D, np.asarray(sizes, dtype=np.float64), cd.shape[0], cd.shape[1], cd.ravel(), len(qw),
         np.ascontiguousarray(qw), np.ascontiguousarray(shp).ravel(),
         np.ascontiguousarray(grad).ravel(), _ptr(k), _ptr(f), n, colptr, rowval, nzval, b)
+    if st != 0:
+        raise RuntimeError("entry missing from the pattern")
+    return nzval[:len(rowval)], b[:n]
+
+
+def numeric_refill_cartesian_par(D, sizes, part, cell_dofs, qw, shp, grad, kappa, fsrc,
+                                 colptr, rowval, nthreads):
+    """numeric_refill_cartesian on nthreads OpenMP threads (parity-coloured cells); agrees
+    with the sequential loop to rounding.  Timing baseline only."""
+    cd = np.ascontiguousarray(cell_dofs, dtype=np.int32)
+    assert cd.shape[0] == int(np.prod(part)) and len(part) == D
+    n = len(colptr) - 1
+    nzval = np.zeros(max(len(rowval), 1))
+    b = np.zeros(max(n, 1))
+    k = None if kappa is None else np.ascontiguousarray(kappa, dtype=np.float64)
+    f = None if fsrc is None else np.ascontiguousarray(fsrc, dtype=np.float64)
+    st = lib().gor_numeric_refill_cartesian_par(
+        D, np.asarray(sizes, dtype=np.float64), np.asarray(part, dtype=np.int64), cd.shape[1],
+        cd.ravel(), len(qw), np.ascontiguousarray(qw), np.ascontiguousarray(shp).ravel(),
+        np.ascontiguousarray(grad).ravel(), _ptr(k), _ptr(f), n, colptr, rowval, nzval, b,
+        int(nthreads))
     if st != 0:
         raise RuntimeError("entry missing from the pattern")
     return nzval[:len(rowval)], b[:n]
