@@ -48,11 +48,12 @@ void invalidate_pattern(gh::Ctx &c) {
   c.a27.release();
   c.g14.release();
   c.slots = false;
-  c.packed = false;
+  c.metric = false;
   c.inverse = false;
   c.cslot.release();
   c.noff.release();
   c.bbase.release();
+  c.bstart.release();
 }
 
 }  // namespace

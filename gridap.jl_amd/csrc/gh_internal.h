@@ -112,21 +112,22 @@ struct Ctx {
   DBuf<int32_t> cstart;             // nnz+1: contribution range of each nonzero
   DBuf<int32_t> perm;               // contributions: emit index cell·nloc² + lj·nloc + li
   // compact contribution lists (replace cstart/perm when they fit): the column sum
-  // takes sum_nc consecutive columns per block; per contribution a 16-bit slot (block
-  // adjacency position << obits | entry offset), per nonzero its first contribution
-  // relative to its block (noff), per block its first contribution (bbase)
+  // takes one range of consecutive columns per block (bstart: first column of each);
+  // per contribution a 16-bit slot (block adjacency position << obits | entry offset),
+  // per nonzero its first contribution relative to its block (noff), per block its
+  // first contribution (bbase)
   bool slots = false;
-  bool packed = false;              // element matrices stored as packed upper triangles
+  bool metric = false;              // simplices: per-cell metric records (entries formed in the sum)
   bool inverse = false;             // cslot holds run positions per (adjacency entry, row)
   int obits = 0;                    // offset bits of a slot
-  int sum_nc = 1;                   // columns per block of the column sum
   int maxbase = 0;                  // most adjacency entries of one block
   int maxq = 0;                     // most contributions of one block
   DBuf<uint16_t> cslot;
   DBuf<uint16_t> noff;
   DBuf<int32_t> bbase;
+  DBuf<int32_t> bstart;
   DBuf<uint16_t> dmask;             // per cell: bit b set when local dof b is Dirichlet
-  DBuf<unsigned char> elem;         // per cell: the nloc x nloc element matrix (FP32/FP64)
+  DBuf<unsigned char> elem;         // per cell: the element matrix or metric record (FP32/FP64)
   DBuf<double> fdet;                // simplices: f |det DF| per cell; hexes: Σ w|J| φ_a (C x 27)
   double nu = 0.3;                  // elasticity: Poisson ratio
   double body[3] = {0, 0, 0};       // elasticity: body force

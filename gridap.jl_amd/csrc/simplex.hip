@@ -84,35 +84,33 @@ __global__ void dirichlet_mask_kernel(int64_t ncells, int nloc,
 }
 
 // ---- compact contribution lists ---------------------------------------------------
-// The column sum runs one block per range of NC consecutive columns.  A block's
-// nonzeros, its contributions (sorted by (column, row), so in nonzero order) and its
-// columns' adjacency entries are then three contiguous ranges, and 16 bits address
-// each of them relative to the range start:
+// The column sum runs one block per range of consecutive columns (bstart: the first
+// column of each range; colblk: the range of each column).  A block's nonzeros, its
+// contributions (sorted by (column, row), so in nonzero order) and its columns'
+// adjacency entries are then three contiguous ranges, and 16 bits address each of
+// them relative to the range start:
 //   slot of a contribution  = (adjacency position − block's first) << obits | offset,
-//                             the entry being E[base(cell, lj) + offset];
+//                             offset: the entry within the cell (full column row, or
+//                             packed upper-triangle entry for simplices);
 //   noff of a nonzero       = its first contribution − block's first.
 
-// out[0] = most adjacency entries of a block, out[1] = most contributions of a block
-__global__ void block_extent_kernel(int64_t nblocks, int64_t nfree, int nc,
-                                    const int32_t *__restrict__ adjptr,
-                                    const int32_t *__restrict__ colptr,
-                                    const int32_t *__restrict__ cstart,
-                                    int32_t *__restrict__ out) {
-  const int64_t bk = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (bk >= nblocks) return;
-  const int64_t j0 = bk * nc, j1 = min<int64_t>(j0 + nc, nfree);
-  atomicMax(out, adjptr[j1] - adjptr[j0]);
-  atomicMax(out + 1, cstart[colptr[j1]] - cstart[colptr[j0]]);
+// C[j] = first contribution of column j (j <= nfree).
+__global__ void column_counts_kernel(int64_t nfree, const int32_t *__restrict__ colptr,
+                                     const int32_t *__restrict__ cstart,
+                                     int32_t *__restrict__ C) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j <= nfree) C[j] = cstart[colptr[j]];
 }
 
 // Contribution q (emit index t = cell·nloc² + lj·nloc + li) of column J: the position
 // of (cell, lj) in the adjacency of J's block and the entry's offset from the
-// cell-column base, (cell·nloc + lj)·nloc (full rows, offset li) or cell·ns (packed
-// upper triangle, offset of (min, max) of li, lj).
+// cell-column base, (cell·nloc + lj)·nloc (full rows, offset li) or, for simplices
+// (ns > 0), the packed upper-triangle entry of (min, max) of li, lj.
 // With ipos (full rows): instead of the slot, the contribution's position in its
 // block's run, stored at the entry's place (adjacency position · nloc + li).
-__global__ void contrib_slot_kernel(int64_t ncontrib, int nc, int nloc, int lbits, int obits,
-                                    int ns, const int32_t *__restrict__ perm,
+__global__ void contrib_slot_kernel(int64_t ncontrib, const int32_t *__restrict__ bstart,
+                                    const int32_t *__restrict__ colblk, int nloc, int lbits,
+                                    int obits, int ns, const int32_t *__restrict__ perm,
                                     const int32_t *__restrict__ cell_dofs,
                                     const int32_t *__restrict__ adjptr,
                                     const int32_t *__restrict__ adj,
@@ -144,7 +142,7 @@ __global__ void contrib_slot_kernel(int64_t ncontrib, int nc, int nloc, int lbit
     const int a = min(li, lj), b = max(li, lj);
     off = a * nloc - a * (a - 1) / 2 + (b - a);
   }
-  const int J0 = J - J % nc;
+  const int J0 = bstart[colblk[J]];
   if (ipos) {
     ipos[(int64_t)lo * nloc + li] = (uint16_t)(q - cstart[colptr[J0]]);
     return;
@@ -155,23 +153,17 @@ __global__ void contrib_slot_kernel(int64_t ncontrib, int nc, int nloc, int lbit
 }
 
 // One wave per column: noff of its nonzeros, relative to its block's first entry.
-__global__ void nonzero_offset_kernel(int64_t nfree, int nc, const int32_t *__restrict__ colptr,
+__global__ void nonzero_offset_kernel(int64_t nfree, const int32_t *__restrict__ bstart,
+                                      const int32_t *__restrict__ colblk,
+                                      const int32_t *__restrict__ colptr,
                                       const int32_t *__restrict__ cstart,
                                       uint16_t *__restrict__ noff) {
   const int64_t j = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / 64;
   const int lane = threadIdx.x & 63;
   if (j >= nfree) return;
-  const int32_t q0 = cstart[colptr[j - j % nc]];
+  const int32_t q0 = cstart[colptr[bstart[colblk[j]]]];
   for (int32_t p = colptr[j] + lane; p < colptr[j + 1]; p += 64)
     noff[p] = (uint16_t)(cstart[p] - q0);
-}
-
-__global__ void block_base_kernel(int64_t nblocks, int64_t nfree, int nc,
-                                  const int32_t *__restrict__ colptr,
-                                  const int32_t *__restrict__ cstart,
-                                  int32_t *__restrict__ bbase) {
-  const int64_t bk = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (bk <= nblocks) bbase[bk] = cstart[colptr[min<int64_t>(bk * nc, nfree)]];
 }
 
 // ---- numeric -----------------------------------------------------------------------
@@ -181,25 +173,42 @@ __device__ __forceinline__ int sym_index(int nloc, int a, int b) {
   return lo * nloc - lo * (lo - 1) / 2 + (hi - lo);
 }
 
+// Simplex element storage for the column sum (metric records): the element matrix of an
+// affine simplex is K_ab = Σ_k g_k Ĝ^k_ab, so a cell is stored as its metric
+// g_0..g_5 and f|det DF| (one record of kMetricRec entries of T; in FP32 f|det| is a
+// double in entries 6-7, in FP64 entry 6, entry 7 zero), after a table of the
+// reference tensor as T, Ĝ^k at packed upper-triangle entry e in R[6e + k], padded
+// to metric_ref_pad entries.  The column sum forms each entry from the record and the
+// table with the same operations, in the same order, as the element kernel's
+// matrix output, so both give the same bits.
+constexpr int kMetricRec = 8;
+__host__ __device__ constexpr int metric_ref_pad(int ns) { return (6 * ns + 7) & ~7; }
+
+template <class T>
+using Vec2 = T __attribute__((ext_vector_type(2)));
+
 // Element matrices of kBlock cells: E[c][a][b] = Σ_k g_k(c) Ĝ^k_ab, g = κ|det| DF⁻¹DF⁻ᵀ,
-// as full rows or (PACKED) as the upper triangle, row by row.
-template <int NLOC, class T, bool PACKED>
+// as full rows, or (METRIC) the cell's metric record and, from block 0, the table.
+template <int NLOC, class T, bool METRIC>
 __global__ __launch_bounds__(kBlock) void simplex_element_kernel(
     int64_t ncells, const int32_t *__restrict__ cell_nodes, const double *__restrict__ xyz,
     const double *__restrict__ kappa, const double *__restrict__ fsrc,
     const double *__restrict__ gref, T *__restrict__ E, double *__restrict__ fdet) {
   constexpr int NS = NLOC * (NLOC + 1) / 2;
   __shared__ T ref[6][NS];
-  __shared__ double sg[kBlock][6];
-  for (int t = threadIdx.x; t < 6 * NS; t += kBlock) {
-    const int k = t / NS, e = t - k * NS;
-    int a = 0, rem = e;
-    while (rem >= NLOC - a) {
-      rem -= NLOC - a;
-      ++a;
+  __shared__ double sg[METRIC ? 1 : kBlock][6];
+  if (!METRIC || blockIdx.x == 0)
+    for (int t = threadIdx.x; t < 6 * NS; t += kBlock) {
+      const int k = t / NS, e = t - k * NS;
+      int a = 0, rem = e;
+      while (rem >= NLOC - a) {
+        rem -= NLOC - a;
+        ++a;
+      }
+      const T v = (T)gref[(k * NLOC + a) * NLOC + a + rem];
+      if (METRIC) E[e * 6 + k] = v;
+      else ref[k][e] = v;
     }
-    ref[k][e] = (T)gref[(k * NLOC + a) * NLOC + a + rem];
-  }
   const int64_t c0 = (int64_t)blockIdx.x * kBlock;
   const int64_t c = c0 + threadIdx.x;
   if (c < ncells) {
@@ -229,25 +238,39 @@ __global__ __launch_bounds__(kBlock) void simplex_element_kernel(
     auto dot = [](const double *u, const double *w) {
       return u[0] * w[0] + u[1] * w[1] + u[2] * w[2];
     };
-    sg[threadIdx.x][0] = s * dot(r[0], r[0]);
-    sg[threadIdx.x][1] = s * dot(r[1], r[1]);
-    sg[threadIdx.x][2] = s * dot(r[2], r[2]);
-    sg[threadIdx.x][3] = s * dot(r[0], r[1]);
-    sg[threadIdx.x][4] = s * dot(r[0], r[2]);
-    sg[threadIdx.x][5] = s * dot(r[1], r[2]);
-    fdet[c] = (fsrc ? fsrc[c] : 1.0) * ad;
-  }
-  __syncthreads();
-  const int64_t ncb = min<int64_t>(kBlock, ncells - c0);
-  constexpr int NN = PACKED ? NS : NLOC * NLOC;
-  T *out = E + c0 * NN;
-  for (int idx = threadIdx.x; idx < ncb * NN; idx += kBlock) {
-    const int cl = idx / NN, ab = idx - cl * NN;
-    const int ee = PACKED ? ab : sym_index(NLOC, ab / NLOC, ab % NLOC);
-    T v = (T)0;
+    const double g[6] = {s * dot(r[0], r[0]), s * dot(r[1], r[1]), s * dot(r[2], r[2]),
+                         s * dot(r[0], r[1]), s * dot(r[0], r[2]), s * dot(r[1], r[2])};
+    const double fd = (fsrc ? fsrc[c] : 1.0) * ad;
+    if constexpr (METRIC) {
+      T *rec = E + metric_ref_pad(NS) + c * kMetricRec;
+      Vec2<T> *r2 = reinterpret_cast<Vec2<T> *>(rec);
+      r2[0] = Vec2<T>{(T)g[0], (T)g[1]};
+      r2[1] = Vec2<T>{(T)g[2], (T)g[3]};
+      r2[2] = Vec2<T>{(T)g[4], (T)g[5]};
+      if constexpr (sizeof(T) == 4) {
+        *reinterpret_cast<double *>(rec + 6) = fd;
+      } else {
+        r2[3] = Vec2<T>{(T)fd, (T)0};
+      }
+    } else {
 #pragma unroll
-    for (int k = 0; k < 6; ++k) v += (T)sg[cl][k] * ref[k][ee];
-    out[idx] = v;
+      for (int k = 0; k < 6; ++k) sg[threadIdx.x][k] = g[k];
+      fdet[c] = fd;
+    }
+  }
+  if constexpr (!METRIC) {
+    __syncthreads();
+    const int64_t ncb = min<int64_t>(kBlock, ncells - c0);
+    constexpr int NN = NLOC * NLOC;
+    T *out = E + c0 * NN;
+    for (int idx = threadIdx.x; idx < ncb * NN; idx += kBlock) {
+      const int cl = idx / NN, ab = idx - cl * NN;
+      const int ee = sym_index(NLOC, ab / NLOC, ab % NLOC);
+      T v = (T)0;
+#pragma unroll
+      for (int k = 0; k < 6; ++k) v += (T)sg[cl][k] * ref[k][ee];
+      out[idx] = v;
+    }
   }
 }
 
@@ -292,39 +315,45 @@ struct FusedRhs {
   double *b;
 };
 
-// One block per range of NC consecutive columns [J0, J1), in two phases:
-//  * stage: every contribution of the range (a contiguous run) is read from the
-//    element matrices into LDS at its run position;
+// One block per column range [bstart[b], bstart[b+1]), in two phases:
+//  * stage: every contribution of the range (a contiguous run) is formed or read
+//    into LDS at its run position;
 //  * sum: one thread per nonzero of the range adds its entries, noff[p] up to the
 //    next nonzero's, in ascending cell order in FP64 and writes nzval once.
 // Bit-identical to simplex_sum_kernel (same entries, same order).  The loads that
 // do not depend on each other (slots or entry positions, adjacency, the first R
 // nonzero offsets per thread) are all issued before the first wait, so a block
 // costs about three memory round trips.
-// Packed storage (FULL_NLOC = 0): the bases of the range's cell-columns go to LDS and
-// each contribution gathers its entry through its slot.  Full columns (FULL_NLOC = 81,
-// the vector-Q2 hexahedron layout, kHexColLd entries per column): the block walks its
-// cell-columns entry by entry with contiguous reads and writes each entry that is a
-// contribution to its run position (cslot holds those positions, 0xffff for rows
-// that are not free).
-// LDS: maxbase bases, then maxq entries (the largest extents over all blocks).
-// Entry (row li, column lj) of a cell's element matrix in the layout of an nloc of
-// N: 81 = the vector-Q2 hexahedron's padded columns, otherwise the packed upper
-// triangle of N(N+1)/2 entries.
-template <int N>
-__device__ __forceinline__ uint32_t elem_addr(uint32_t cell, uint32_t lj, int li) {
-  if constexpr (N == 81) {
-    return (cell * 81u + lj) * (uint32_t)kHexColLd + hex_row_slot(li);
-  } else {
-    constexpr uint32_t NS = N * (N + 1) / 2;
-    const int a = min(li, (int)lj), b = max(li, (int)lj);
-    return cell * NS + (uint32_t)(a * N - a * (a - 1) / 2 + (b - a));
-  }
+// Simplices (FULL_NLOC = 0, metric records): the metric of each of the range's
+// cell-columns (one 32- or 64-byte record read per adjacency entry) and the
+// reference table go to LDS, and each contribution forms its entry Σ_k g_k Ĝ^k_e
+// from them through its slot (adjacency position, packed entry e).
+// Full columns (FULL_NLOC = 81, the vector-Q2 hexahedron layout, kHexColLd entries
+// per column): the block walks its cell-columns entry by entry with contiguous reads
+// and writes each entry that is a contribution to its run position (cslot holds
+// those positions, 0xffff for rows that are not free).
+// LDS (maxbase: the most adjacency entries of a block, even; maxq: the most
+// contributions), full columns: maxq entries; metric records: maxbase right-hand
+// side terms (when fused), the table (metric_ref_pad(ns) entries), 6 metric entries
+// per adjacency entry, then maxq entries.
+// Entry (row li, column lj) of a cell's element matrix in the vector-Q2 hexahedron's
+// padded columns.
+__device__ __forceinline__ uint32_t hex_elem_addr(uint32_t cell, uint32_t lj, int li) {
+  return (cell * 81u + lj) * (uint32_t)kHexColLd + hex_row_slot(li);
+}
+
+// Σ_k g_k Ĝ^k_e over 6 entries of T at g and r, in the element kernel's order.
+template <class T>
+__device__ __forceinline__ T metric_entry(const T *g, const T *r) {
+  T v = (T)0;
+#pragma unroll
+  for (int k = 0; k < 6; ++k) v += g[k] * r[k];
+  return v;
 }
 
 template <class T, int FULL_NLOC>
 __global__ __launch_bounds__(kBlock) void column_sum_kernel(
-    int64_t nfree, int nc, int nloc, int lbits, int obits, int ns, int maxbase,
+    const int32_t *__restrict__ bstart, int nloc, int lbits, int obits, int ns, int maxbase,
     const int32_t *__restrict__ colptr, const int32_t *__restrict__ bbase,
     const int32_t *__restrict__ adjptr, const int32_t *__restrict__ adj,
     const uint16_t *__restrict__ noff, const uint16_t *__restrict__ cslot,
@@ -332,10 +361,13 @@ __global__ __launch_bounds__(kBlock) void column_sum_kernel(
   extern __shared__ __align__(16) uint32_t shm[];
   constexpr int U = 12;  // staged entries per thread and batch
   constexpr int R = 8;   // nonzeros per thread whose offsets are prefetched
-  uint32_t *bases = shm;
-  double *rv = reinterpret_cast<double *>(shm + ((maxbase + 3) & ~3));
-  T *vals = reinterpret_cast<T *>(rv + (F.b ? maxbase : 0));
-  const int64_t j0 = (int64_t)blockIdx.x * nc, j1 = min<int64_t>(j0 + nc, nfree);
+  const int rpad = metric_ref_pad(ns);
+  unsigned char *lds = reinterpret_cast<unsigned char *>(shm);
+  double *rv = reinterpret_cast<double *>(lds);
+  T *refl = reinterpret_cast<T *>(lds + (F.b ? 8 * maxbase : 0));
+  T *gl = refl + rpad;
+  T *vals = FULL_NLOC > 0 ? reinterpret_cast<T *>(lds) : gl + 6 * maxbase;
+  const int64_t j0 = bstart[blockIdx.x], j1 = bstart[blockIdx.x + 1];
   const int a0 = adjptr[j0], na = adjptr[j1] - a0;
   const int32_t q0 = bbase[blockIdx.x], nq = bbase[blockIdx.x + 1] - q0;
   const int32_t p0 = colptr[j0], np = colptr[j1] - p0;
@@ -350,21 +382,6 @@ __global__ __launch_bounds__(kBlock) void column_sum_kernel(
     oen[r] = i + 1 < np ? noff[p0 + i + 1] : (uint32_t)nq;
   }
   if constexpr (FULL_NLOC > 0) {
-    if (F.b) {  // right-hand side terms of the range's adjacency entries (packed layout)
-      for (int k = tid; k < na; k += kBlock) {
-        const int e = adj[a0 + k];
-        const uint32_t cell = (uint32_t)(e >> lbits), lj = (uint32_t)(e & lmask);
-        double v = F.fdet[cell] * F.mref[lj];
-        uint32_t m = F.dvals ? F.dmask[cell] : 0u;
-        while (m) {
-          const int bb = __ffs(m) - 1;
-          m &= m - 1;
-          const int d = F.cell_dofs[(size_t)cell * nloc + bb];
-          v -= (double)E[elem_addr<FULL_NLOC>(cell, lj, bb)] * F.dvals[-d - 1];
-        }
-        rv[k] = v;
-      }
-    }
     const uint16_t *ip = cslot + (int64_t)a0 * FULL_NLOC;
     const int ne = na * FULL_NLOC;
     for (int eb = 0; eb < ne; eb += kBlock * U) {
@@ -382,7 +399,7 @@ __global__ __launch_bounds__(kBlock) void column_sum_kernel(
         const int e = eb + tid + u * kBlock;
         const int li = e - (e / FULL_NLOC) * FULL_NLOC;
         const uint32_t cell = (uint32_t)(ae[u] >> lbits), lj = (uint32_t)(ae[u] & lmask);
-        v[u] = pos[u] != 0xffffu ? E[elem_addr<FULL_NLOC>(cell, lj, li)] : (T)0;
+        v[u] = pos[u] != 0xffffu ? E[hex_elem_addr(cell, lj, li)] : (T)0;
       }
 #pragma unroll
       for (int u = 0; u < U; ++u)
@@ -390,26 +407,37 @@ __global__ __launch_bounds__(kBlock) void column_sum_kernel(
     }
   } else {
     const int omask = (1 << obits) - 1;
+    const T *recs = E + rpad;
     uint32_t sl[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int q = tid + u * kBlock;
       sl[u] = q < nq ? cslot[q0 + q] : 0xffffffffu;
     }
+    for (int t = tid; t < 6 * ns; t += kBlock) refl[t] = E[t];
+    __syncthreads();  // the table is read by the liftings below
     for (int k = tid; k < na; k += kBlock) {
       const int e = adj[a0 + k];
       const uint32_t cell = (uint32_t)(e >> lbits), lj = (uint32_t)(e & lmask);
-      bases[k] = ns ? cell * (uint32_t)ns : (cell * (uint32_t)nloc + lj) * (uint32_t)nloc;
+      const T *rec = recs + (size_t)cell * kMetricRec;
+      const Vec2<T> *r2 = reinterpret_cast<const Vec2<T> *>(rec);
+      const Vec2<T> x0 = r2[0], x1 = r2[1], x2 = r2[2];
+      Vec2<T> *dst = reinterpret_cast<Vec2<T> *>(gl + 6 * k);
+      dst[0] = x0;
+      dst[1] = x1;
+      dst[2] = x2;
       if (F.b) {
-        double v = F.fdet[cell] * F.mref[lj];
+        const T g[6] = {x0.x, x0.y, x1.x, x1.y, x2.x, x2.y};
+        double v = (sizeof(T) == 4 ? *reinterpret_cast<const double *>(rec + 6)
+                                   : (double)rec[6]) * F.mref[lj];
         uint32_t m = F.dvals ? F.dmask[cell] : 0u;
         while (m) {
           const int bb = __ffs(m) - 1;
           m &= m - 1;
           const int d = F.cell_dofs[(size_t)cell * nloc + bb];
           const int lo = min((int)lj, bb), hi = max((int)lj, bb);
-          v -= (double)E[(size_t)cell * ns + lo * nloc - lo * (lo - 1) / 2 + (hi - lo)] *
-               F.dvals[-d - 1];
+          const T ev = metric_entry(g, refl + 6 * (lo * nloc - lo * (lo - 1) / 2 + (hi - lo)));
+          v -= (double)ev * F.dvals[-d - 1];
         }
         rv[k] = v;
       }
@@ -423,13 +451,11 @@ __global__ __launch_bounds__(kBlock) void column_sum_kernel(
           sl[u] = q < nq ? cslot[q0 + q] : 0xffffffffu;
         }
       }
-      T v[U];
 #pragma unroll
       for (int u = 0; u < U; ++u)
-        v[u] = sl[u] != 0xffffffffu ? E[bases[sl[u] >> obits] + (sl[u] & omask)] : (T)0;
-#pragma unroll
-      for (int u = 0; u < U; ++u)
-        if (sl[u] != 0xffffffffu) vals[qb + tid + u * kBlock] = v[u];
+        if (sl[u] != 0xffffffffu)
+          vals[qb + tid + u * kBlock] =
+              metric_entry(gl + 6 * (sl[u] >> obits), refl + 6 * (sl[u] & omask));
     }
   }
   __syncthreads();
@@ -457,7 +483,9 @@ __global__ __launch_bounds__(kBlock) void column_sum_kernel(
   }
 }
 
-template <int NLOC, class T, bool PACKED>
+// b from full-row element matrices, one thread per column (when the column sum does
+// not fuse it).
+template <int NLOC, class T>
 __global__ void simplex_rhs_kernel(int64_t nfree, const int32_t *__restrict__ adjptr,
                                    const int32_t *__restrict__ adj,
                                    const double *__restrict__ fdet,
@@ -478,9 +506,7 @@ __global__ void simplex_rhs_kernel(int64_t nfree, const int32_t *__restrict__ ad
       const int bb = __ffs(m) - 1;
       m &= m - 1;
       const int d = cell_dofs[(size_t)cell * NLOC + bb];
-      const size_t ee = PACKED ? (size_t)cell * (NLOC * (NLOC + 1) / 2) + sym_index(NLOC, a, bb)
-                               : ((size_t)cell * NLOC + a) * NLOC + bb;
-      v -= (double)E[ee] * dvals[-d - 1];
+      v -= (double)E[((size_t)cell * NLOC + a) * NLOC + bb] * dvals[-d - 1];
     }
     s += v;
   }
@@ -493,19 +519,22 @@ bool elemlist_sum(Ctx &c, bool f32, const FusedRhs *rhs);
 
 namespace {
 
-template <int NLOC, class T, bool PACKED>
+template <int NLOC, class T, bool METRIC>
 void launch_simplex(Ctx &c) {
-  constexpr int NN = PACKED ? NLOC * (NLOC + 1) / 2 : NLOC * NLOC;
-  c.elem.resize((size_t)c.ncells * NN * sizeof(T));
+  constexpr int NS = NLOC * (NLOC + 1) / 2;
+  c.elem.resize(METRIC ? ((size_t)metric_ref_pad(NS) + (size_t)c.ncells * kMetricRec) * sizeof(T)
+                       : (size_t)c.ncells * NLOC * NLOC * sizeof(T));
   T *E = reinterpret_cast<T *>(c.elem.get());
-  hipLaunchKernelGGL((simplex_element_kernel<NLOC, T, PACKED>), dim3(grid_for(c.ncells)),
+  hipLaunchKernelGGL((simplex_element_kernel<NLOC, T, METRIC>), dim3(grid_for(c.ncells)),
                      dim3(kBlock), 0, c.stream, c.ncells, c.cell_nodes.get(), c.xyz.get(),
                      c.have_kappa ? c.kappa.get() : nullptr,
                      c.have_fsrc ? c.fsrc.get() : nullptr, c.sref.get(), E, c.fdet.get());
   const FusedRhs F{c.fdet.get(), c.sref.get() + 6 * NLOC * NLOC, c.dmask.get(),
                    c.cell_dofs.get(), c.have_dvals ? c.dvals.get() : nullptr, c.b.get()};
-  if (!elemlist_sum(c, sizeof(T) == 4, &F))
-    hipLaunchKernelGGL((simplex_rhs_kernel<NLOC, T, PACKED>), dim3(grid_for(c.nfree)),
+  const bool fused = elemlist_sum(c, sizeof(T) == 4, &F);
+  GH_REQUIRE(fused == METRIC, GH_ERR_STATE, "simplex storage does not match the sum");
+  if (!fused)
+    hipLaunchKernelGGL((simplex_rhs_kernel<NLOC, T>), dim3(grid_for(c.nfree)),
                        dim3(kBlock), 0, c.stream, c.nfree, c.adjptr.get(), c.adj.get(),
                        c.fdet.get(), c.sref.get() + 6 * NLOC * NLOC, c.dmask.get(),
                        c.cell_dofs.get(), (const T *)E, c.have_dvals ? c.dvals.get() : nullptr,
@@ -521,84 +550,125 @@ bool simplex_eligible(const Ctx &c) {
 }
 
 // Replace cstart/perm by the compact lists when they fit: 16-bit slots and offsets
-// within a block's ranges, and element storage addressable in 32 bits.  Simplices
-// then store packed upper triangles.  NC is the largest power of two whose blocks gather
-// at most about kBlock·12 entries on average; it halves until the block's extents fit 16 bits and the LDS.
+// within a block's ranges, and hexahedron element storage addressable in 32 bits.
+// Simplices then store metric records.  The column ranges are cut greedily in column
+// order: a range grows while it holds at most kBlock columns and a budget of
+// contributions (kBlock·8 for simplices, kBlock·12 for full columns), and its extents
+// fit 16 bits and the LDS (FP32 within kMaxColumnLds; FP64 within the device's
+// per-workgroup limit, at most kMaxColumnLds64).  Budgeted ranges keep the LDS of a
+// block near the mean: the numbering puts the vertex dofs, which touch many more cells
+// than edge or face dofs, first, and a fixed column count per range sized the LDS of
+// every block for the heaviest.
 constexpr size_t kMaxColumnLds = 48 * 1024;
+constexpr size_t kMaxColumnLds64 = 64 * 1024;
 
-size_t column_sum_lds(int maxbase, int maxq, size_t esz, bool rhs) {
-  return sizeof(uint32_t) * ((std::max(maxbase, 1) + 3) & ~3) +
-         (rhs ? sizeof(double) * std::max(maxbase, 1) : 0) + esz * std::max(maxq, 1);
+// maxbase as the kernel takes it: at least 2 and even (metric entries stay 16-byte aligned)
+int lds_maxbase(int maxbase) { return (std::max(maxbase, 1) + 1) & ~1; }
+
+size_t column_sum_lds(bool metric, int64_t maxbase, int64_t maxq, size_t esz, bool rhs, int ns) {
+  const size_t mb = (size_t)lds_maxbase((int)std::min<int64_t>(maxbase, 1 << 30));
+  const size_t mq = (size_t)std::max<int64_t>(maxq, 1);
+  if (!metric) return esz * mq;
+  return (rhs ? sizeof(double) * mb : 0) + esz * ((size_t)metric_ref_pad(ns) + 6 * mb + mq);
 }
 
 void compact_contributions(Ctx &c, int lbits) {
   c.slots = false;
-  c.packed = false;
+  c.metric = false;
   c.inverse = false;
   c.cslot.release();
   c.noff.release();
   c.bbase.release();
+  c.bstart.release();
   if (!c.nnz || !c.cstart.get() || !c.perm.get()) return;
   // GRIDAP_HIP_COMPACT_LISTS=0 keeps the int32 lists (the fallback sum), so the two
   // sums can be compared on the same system
   if (const char *env = std::getenv("GRIDAP_HIP_COMPACT_LISTS"))
     if (env[0] == '0') return;
   hipStream_t s = c.stream;
-  int32_t ncontrib = 0;
-  GH_CHECK_HIP(hipMemcpy(&ncontrib, c.cstart.get() + c.nnz, 4, hipMemcpyDeviceToHost));
-  const bool packed = c.nloc <= 16;
-  if (!packed && c.nloc != 81) return;  // full rows: the staging kernel is built for 81
-  const int ns = packed ? c.nloc * (c.nloc + 1) / 2 : 0;
+  const int64_t N = c.nfree;
+  const bool metric = c.nloc <= 16;
+  if (!metric && c.nloc != 81) return;  // full rows: the staging kernel is built for 81
+  const int ns = metric ? c.nloc * (c.nloc + 1) / 2 : 0;
   int obits = 1;
-  while ((1 << obits) < (packed ? ns : c.nloc)) ++obits;
-  const int64_t per_cell = packed ? ns : (int64_t)c.nloc * kHexColLd;
-  if (c.ncells * per_cell >= (int64_t(1) << 32)) return;
-  const double per_col = (double)ncontrib / (double)c.nfree;
-  int nc = 1;
-  while (nc < 256 && (2 * nc) * per_col <= kBlock * 12.0) nc *= 2;
-  DBuf<int32_t> dev;
-  dev.resize(2);
-  int32_t mx[2] = {0, 0};
-  for (;; nc /= 2) {
-    const int64_t nblocks = (c.nfree + nc - 1) / nc;
-    GH_CHECK_HIP(hipMemsetAsync(dev.get(), 0, 2 * sizeof(int32_t), s));
-    hipLaunchKernelGGL(block_extent_kernel, dim3(grid_for(nblocks)), dim3(kBlock), 0, s,
-                       nblocks, c.nfree, nc, c.adjptr.get(), c.colptr.get(), c.cstart.get(),
-                       dev.get());
-    GH_CHECK_HIP(hipMemcpyAsync(mx, dev.get(), 8, hipMemcpyDeviceToHost, s));
+  while ((1 << obits) < (metric ? ns : c.nloc)) ++obits;
+  if (!metric && c.ncells * (int64_t)c.nloc * kHexColLd >= (int64_t(1) << 32)) return;
+  int lds64 = 0;
+  GH_CHECK_HIP(hipDeviceGetAttribute(&lds64, hipDeviceAttributeMaxSharedMemoryPerBlock,
+                                     c.device));
+  const size_t max64 = std::min(kMaxColumnLds64, (size_t)lds64);
+  // first contribution (C) and first adjacency entry (A) of every column, on the host
+  std::vector<int32_t> C(N + 1), A(N + 1);
+  {
+    DBuf<int32_t> cd;
+    cd.resize(N + 1);
+    hipLaunchKernelGGL(column_counts_kernel, dim3(grid_for(N + 1)), dim3(kBlock), 0, s, N,
+                       c.colptr.get(), c.cstart.get(), cd.get());
+    GH_CHECK_HIP(hipGetLastError());
+    GH_CHECK_HIP(hipMemcpyAsync(C.data(), cd.get(), sizeof(int32_t) * (N + 1),
+                                hipMemcpyDeviceToHost, s));
+    GH_CHECK_HIP(hipMemcpyAsync(A.data(), c.adjptr.get(), sizeof(int32_t) * (N + 1),
+                                hipMemcpyDeviceToHost, s));
     GH_CHECK_HIP(hipStreamSynchronize(s));
-    const bool fits = ((int64_t)mx[0] << obits) <= 65536 && mx[1] <= 65535 &&
-                      column_sum_lds(mx[0], mx[1], 8, packed) <= kMaxColumnLds;
-    if (fits) break;
-    if (nc == 1) return;  // keep cstart/perm and full rows
   }
-  const int64_t nblocks = (c.nfree + nc - 1) / nc;
+  const int64_t ncontrib = C[N];
+  const int64_t budget = (int64_t)kBlock * (metric ? 8 : 12);
+  auto fits = [&](int64_t a, int64_t q) {
+    return (a << obits) <= 65536 && q <= 65535 &&
+           column_sum_lds(metric, a, q, 4, metric, ns) <= kMaxColumnLds &&
+           column_sum_lds(metric, a, q, 8, metric, ns) <= (metric ? max64 : kMaxColumnLds);
+  };
+  std::vector<int32_t> bs{0}, bq;
+  bs.reserve(N / 4 + 2);
+  int64_t maxa = 0, maxq = 0, jb = 0;
+  for (int64_t j = 1; j <= N; ++j) {
+    if (j < N) {  // extend [jb, j) to [jb, j + 1)?
+      const int64_t a = A[j + 1] - A[jb], q = C[j + 1] - C[jb];
+      if (j - jb < kBlock && q <= budget && fits(a, q)) continue;
+    }
+    const int64_t a = A[j] - A[jb], q = C[j] - C[jb];
+    if (!fits(a, q)) return;  // one column too large: keep cstart/perm and full rows
+    maxa = std::max(maxa, a);
+    maxq = std::max(maxq, q);
+    bs.push_back((int32_t)j);
+    jb = j;
+  }
+  const int64_t nblocks = (int64_t)bs.size() - 1;
+  std::vector<int32_t> colblk(N), hb(nblocks + 1);
+  for (int64_t b = 0; b < nblocks; ++b) {
+    hb[b] = C[bs[b]];
+    for (int64_t j = bs[b]; j < bs[b + 1]; ++j) colblk[j] = (int32_t)b;
+  }
+  hb[nblocks] = C[N];
   DBuf<uint16_t> cslot, noff;
-  DBuf<int32_t> bbase;
+  DBuf<int32_t> bbase, bstart, cblk, dev;
+  bstart.resize(nblocks + 1);
+  bbase.resize(nblocks + 1);
+  cblk.resize(N);
+  dev.resize(1);
+  GH_CHECK_HIP(hipMemcpyAsync(bstart.get(), bs.data(), sizeof(int32_t) * (nblocks + 1),
+                              hipMemcpyHostToDevice, s));
+  GH_CHECK_HIP(hipMemcpyAsync(bbase.get(), hb.data(), sizeof(int32_t) * (nblocks + 1),
+                              hipMemcpyHostToDevice, s));
+  GH_CHECK_HIP(hipMemcpyAsync(cblk.get(), colblk.data(), sizeof(int32_t) * N,
+                              hipMemcpyHostToDevice, s));
   // full columns: the inverse form (ipos: one run position per (free adjacency
-  // entry, row)); packed triangles: per-contribution slots (measured at 96³×6 P2:
-  // 2.31 ms per assembly with slots, 2.43 ms with the inverse form)
-  const bool inverse = !packed;
-  int32_t nadj = 0;
-  if (inverse)
-    GH_CHECK_HIP(hipMemcpy(&nadj, c.adjptr.get() + c.nfree, 4, hipMemcpyDeviceToHost));
-  const int64_t nslot = inverse ? (int64_t)nadj * c.nloc : (int64_t)ncontrib;
+  // entry, row)); simplices: per-contribution slots
+  const bool inverse = !metric;
+  const int64_t nslot = inverse ? (int64_t)A[N] * c.nloc : ncontrib;
   cslot.resize(std::max<int64_t>(nslot, 1));
   if (inverse)
     GH_CHECK_HIP(hipMemsetAsync(cslot.get(), 0xff, sizeof(uint16_t) * cslot.n, s));
   noff.resize(c.nnz);
-  bbase.resize(nblocks + 1);
   GH_CHECK_HIP(hipMemsetAsync(dev.get(), 0, sizeof(int32_t), s));
   if (ncontrib)
     hipLaunchKernelGGL(contrib_slot_kernel, dim3(grid_for(ncontrib)), dim3(kBlock), 0, s,
-                       (int64_t)ncontrib, nc, c.nloc, lbits, obits, ns, c.perm.get(),
-                       c.cell_dofs.get(), c.adjptr.get(), c.adj.get(), c.colptr.get(),
-                       c.cstart.get(), inverse ? nullptr : cslot.get(),
+                       ncontrib, bstart.get(), cblk.get(), c.nloc, lbits, obits, ns,
+                       c.perm.get(), c.cell_dofs.get(), c.adjptr.get(), c.adj.get(),
+                       c.colptr.get(), c.cstart.get(), inverse ? nullptr : cslot.get(),
                        inverse ? cslot.get() : nullptr, dev.get());
-  hipLaunchKernelGGL(nonzero_offset_kernel, dim3(grid_for(c.nfree * 64)), dim3(kBlock), 0, s,
-                     c.nfree, nc, c.colptr.get(), c.cstart.get(), noff.get());
-  hipLaunchKernelGGL(block_base_kernel, dim3(grid_for(nblocks + 1)), dim3(kBlock), 0, s,
-                     nblocks, c.nfree, nc, c.colptr.get(), c.cstart.get(), bbase.get());
+  hipLaunchKernelGGL(nonzero_offset_kernel, dim3(grid_for(N * 64)), dim3(kBlock), 0, s, N,
+                     bstart.get(), cblk.get(), c.colptr.get(), c.cstart.get(), noff.get());
   GH_CHECK_HIP(hipGetLastError());
   int32_t bad = 0;
   GH_CHECK_HIP(hipMemcpyAsync(&bad, dev.get(), 4, hipMemcpyDeviceToHost, s));
@@ -610,13 +680,14 @@ void compact_contributions(Ctx &c, int lbits) {
   std::swap(c.noff.n, noff.n);
   std::swap(c.bbase.p, bbase.p);
   std::swap(c.bbase.n, bbase.n);
+  std::swap(c.bstart.p, bstart.p);
+  std::swap(c.bstart.n, bstart.n);
   c.cstart.release();
   c.perm.release();
-  c.sum_nc = nc;
-  c.maxbase = mx[0];
-  c.maxq = mx[1];
+  c.maxbase = (int)maxa;
+  c.maxq = (int)maxq;
   c.obits = obits;
-  c.packed = packed;
+  c.metric = metric;
   c.inverse = inverse;
   c.slots = true;
   c.elem.release();
@@ -671,12 +742,13 @@ void elemlist_symbolic(Ctx &c) {
 template <class T>
 void launch_column_sum(Ctx &c, const FusedRhs &F) {
   const int lbits = c.nloc <= 16 ? 4 : 7;
-  const int ns = c.packed ? c.nloc * (c.nloc + 1) / 2 : 0;
-  const size_t shm = column_sum_lds(c.maxbase, c.maxq, sizeof(T), F.b != nullptr);
-  const int64_t nblocks = (c.nfree + c.sum_nc - 1) / c.sum_nc;
+  const int ns = c.metric ? c.nloc * (c.nloc + 1) / 2 : 0;
+  const size_t shm =
+      column_sum_lds(c.metric, c.maxbase, c.maxq, sizeof(T), F.b != nullptr, ns);
+  const int64_t nblocks = (int64_t)c.bstart.n - 1;
   auto go = [&](auto kern) {
-    hipLaunchKernelGGL(kern, dim3((unsigned)nblocks), dim3(kBlock), shm, c.stream, c.nfree,
-                       c.sum_nc, c.nloc, lbits, c.obits, ns, std::max(c.maxbase, 1),
+    hipLaunchKernelGGL(kern, dim3((unsigned)nblocks), dim3(kBlock), shm, c.stream, c.bstart.get(),
+                       c.nloc, lbits, c.obits, ns, lds_maxbase(c.maxbase),
                        c.colptr.get(), c.bbase.get(), c.adjptr.get(), c.adj.get(),
                        c.noff.get(), c.cslot.get(), (const T *)c.elem.get(), c.nzval.get(), F);
   };
@@ -684,10 +756,10 @@ void launch_column_sum(Ctx &c, const FusedRhs &F) {
   else go(column_sum_kernel<T, 81>);
 }
 
-// nzval from the element entries; with rhs (packed storage only) also b, fused
+// nzval from the element entries; with rhs (metric records only) also b, fused
 bool elemlist_sum(Ctx &c, bool f32, const FusedRhs *rhs) {
   if (!c.nnz) return false;
-  const bool fuse = rhs && c.slots && c.packed;
+  const bool fuse = rhs && c.slots && c.metric;
   FusedRhs F{};
   if (fuse) F = *rhs;
   if (c.slots) {
@@ -737,8 +809,8 @@ void assemble_simplex(Ctx &c, int precision) {
     c.sref_ready = true;
   }
   const bool f32 = precision == GH_PREC_FP32_ELEM;
-  if (c.packed) {
-    GH_REQUIRE(c.slots, GH_ERR_STATE, "packed element storage without compact lists");
+  if (c.metric) {
+    GH_REQUIRE(c.slots, GH_ERR_STATE, "metric element storage without compact lists");
     if (NL == 10) {
       if (f32) launch_simplex<10, float, true>(c);
       else launch_simplex<10, double, true>(c);
