@@ -351,6 +351,17 @@ __device__ __forceinline__ T metric_entry(const T *g, const T *r) {
   return v;
 }
 
+// metric_entry over 6 entries at g and r that are 2-entry aligned (3 paired LDS reads each).
+template <class T>
+__device__ __forceinline__ T metric_entry_lds(const T *g, const T *r) {
+  const Vec2<T> *g2 = reinterpret_cast<const Vec2<T> *>(g);
+  const Vec2<T> *r2 = reinterpret_cast<const Vec2<T> *>(r);
+  const Vec2<T> g0 = g2[0], g1 = g2[1], g2v = g2[2], r0 = r2[0], r1 = r2[1], r2v = r2[2];
+  const T gg[6] = {g0.x, g0.y, g1.x, g1.y, g2v.x, g2v.y};
+  const T rr[6] = {r0.x, r0.y, r1.x, r1.y, r2v.x, r2v.y};
+  return metric_entry(gg, rr);
+}
+
 template <class T, int FULL_NLOC>
 __global__ __launch_bounds__(kBlock) void column_sum_kernel(
     const int32_t *__restrict__ bstart, int nloc, int lbits, int obits, int ns, int maxbase,
@@ -455,7 +466,7 @@ __global__ __launch_bounds__(kBlock) void column_sum_kernel(
       for (int u = 0; u < U; ++u)
         if (sl[u] != 0xffffffffu)
           vals[qb + tid + u * kBlock] =
-              metric_entry(gl + 6 * (sl[u] >> obits), refl + 6 * (sl[u] & omask));
+              metric_entry_lds(gl + 6 * (sl[u] >> obits), refl + 6 * (sl[u] & omask));
     }
   }
   __syncthreads();
