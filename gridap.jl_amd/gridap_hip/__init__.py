@@ -18,7 +18,9 @@ the device through libgridap_hip.so (include/gridap_hip.h).
 from ._lib import GridapHipError
 from .assemblers import (AffineFEOperator, LinearFESolver, PCGSolver, SparseMatrixAssembler,
                          SparseMatrixCSC, assemble_matrix, assemble_matrix_and_vector,
-                         assemble_vector, get_matrix, get_vector, solve)
+                         assemble_matrix_and_vector_inplace, assemble_matrix_inplace,
+                         assemble_vector, assemble_vector_inplace, get_matrix, get_vector,
+                         solve)
 from .celldata import (CellField, DomainContribution, IsotropicStress, Measure, dot,
                        get_fe_basis, get_trial_fe_basis, grad, gradient, inner, integral,
                        integrate, symmetric_gradient, ε)
@@ -34,6 +36,7 @@ __all__ = [
     "SparseMatrixCSC", "TestFESpace", "TrialFESpace", "Triangulation",
     "UnstructuredDiscreteModel", "VectorValue", "simplexify",
     "assemble_matrix", "assemble_matrix_and_vector", "assemble_vector", "dot",
+    "assemble_matrix_inplace", "assemble_matrix_and_vector_inplace", "assemble_vector_inplace",
     "get_fe_basis", "get_matrix", "get_triangulation", "get_trial_fe_basis", "get_vector",
     "grad", "gradient", "inner", "integral", "integrate", "lagrangian", "solve",
     "IsotropicStress", "symmetric_gradient", "ε",

@@ -202,11 +202,17 @@ class Context:
     def assemble(self, form=GH_FORM_DIFFUSION, precision=GH_PREC_FP64, path=GH_PATH_AUTO):
         self.check(lib().gh_assemble(self.h, form, precision, path))
 
-    def values_get(self):
-        nz = np.zeros(max(self.nnz, 1))
-        b = np.zeros(max(self.nfree, 1))
-        self.check(lib().gh_values_get(self.h, ptr(nz), ptr(b)))
-        return nz[:self.nnz], b[:self.nfree]
+    def values_get(self, nz=None, b=None):
+        """nzval and b of the last assembly; into nz / b (contiguous float64 arrays of
+        nnz / nfree entries, written in place) when given, else into new arrays."""
+        for name, a, n in (("nz", nz, self.nnz), ("b", b, self.nfree)):
+            if a is not None and (not isinstance(a, np.ndarray) or a.dtype != np.float64 or
+                                  not a.flags.c_contiguous or a.shape != (n,)):
+                raise ValueError("%s must be a contiguous float64 array of %d entries" % (name, n))
+        nzo = nz if nz is not None and self.nnz > 0 else np.zeros(max(self.nnz, 1))
+        bo = b if b is not None and self.nfree > 0 else np.zeros(max(self.nfree, 1))
+        self.check(lib().gh_values_get(self.h, ptr(nzo), ptr(bo)))
+        return (nz if nz is not None else nzo[:self.nnz]), (b if b is not None else bo[:self.nfree])
 
     def spmv(self, x):
         x = np.ascontiguousarray(x, dtype=np.float64)

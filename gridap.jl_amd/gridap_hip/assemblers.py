@@ -164,8 +164,11 @@ class SparseMatrixAssembler:
                      force=np.zeros(3) if force is None else force)
         return form, kappa, fsrc, degs.pop(), extra
 
-    def assemble(self, matcontrib, veccontrib, lift):
-        """Device numeric phase; returns (SparseMatrixCSC, b)."""
+    def assemble(self, matcontrib, veccontrib, lift, A=None, b=None):
+        """Device numeric phase; returns (SparseMatrixCSC, b).  With A (a matrix on this
+        assembler's pattern) and/or b given, their values are overwritten in place and
+        they are returned (fillstored! then the numeric loop,
+        src/FESpaces/SparseMatrixAssemblers.jl:60-63)."""
         form, kappa, fsrc, degree, extra = self._lower(matcontrib, veccontrib)
         self._setup(degree)
         dvals = None
@@ -176,11 +179,17 @@ class SparseMatrixAssembler:
         self.ctx.coef_upload(kappa, fsrc, dvals)
         if form == _lib.GH_FORM_ELASTICITY:
             self.ctx.elasticity_params(extra["nu"], extra["force"])
+        if A is not None and not (A.shape == (self.space.nfree, self.space.nfree) and
+                                  np.array_equal(A.colptr, self.colptr) and
+                                  np.array_equal(A.rowval, self.rowval)):
+            # Gridap's add_entry! fails on an entry missing from the stored pattern
+            raise ValueError("A does not have this assembler's sparsity pattern")
         self.ctx.assemble(form, self.precision, self.path)
-        nz, b = self.ctx.values_get()
-        n = self.space.nfree
-        A = SparseMatrixCSC(n, n, self.colptr.copy(), self.rowval.copy(), nz)
-        return A, b
+        nz, bv = self.ctx.values_get(None if A is None else A.nzval, b)
+        if A is None:
+            n = self.space.nfree
+            A = SparseMatrixCSC(n, n, self.colptr.copy(), self.rowval.copy(), nz)
+        return A, bv
 
 
 def _contrib(form, *args):
@@ -217,6 +226,37 @@ def assemble_matrix_and_vector(a, l, U, V, assem=None):
     m = _contrib(a, get_trial_fe_basis(U), get_fe_basis(V))
     vcon = _contrib(l, get_fe_basis(V))
     return assem.assemble(m, vcon, lift=True)
+
+
+def assemble_matrix_inplace(a, A, assem, U=None, V=None):
+    """assemble_matrix!(f, A, assem, U, V) (src/FESpaces/Assemblers.jl:308-312 ->
+    SparseMatrixAssemblers.jl:60-63): overwrite the stored values of A, which must carry
+    assem's pattern, with the assembly of a(u, v).  The pattern is not rebuilt, so a
+    nonlinear (Picard or Newton) loop re-assembles its matrix at the cost of the numeric
+    phase only."""
+    U = assem.U if U is None else U
+    V = assem.V if V is None else V
+    m = _contrib(a, get_trial_fe_basis(U), get_fe_basis(V))
+    return assem.assemble(m, None, lift=False, A=A)[0]
+
+
+def assemble_vector_inplace(l, b, assem, V=None):
+    """assemble_vector!(f, b, assem, V) (src/FESpaces/Assemblers.jl:319-322): overwrite b
+    (a contiguous float64 vector of num_free_dofs entries) with the assembly of l(v)."""
+    V = assem.V if V is None else V
+    vcon = _contrib(l, get_fe_basis(V))
+    return assem.assemble(None, vcon, lift=False, b=b)[1]
+
+
+def assemble_matrix_and_vector_inplace(a, l, A, b, assem, U=None, V=None):
+    """assemble_matrix_and_vector!(f, l, A, b, assem, U, V) with the Dirichlet lifting
+    of U (src/FESpaces/Assemblers.jl:330-334,342-346): overwrite A's stored values and
+    b in place."""
+    U = assem.U if U is None else U
+    V = assem.V if V is None else V
+    m = _contrib(a, get_trial_fe_basis(U), get_fe_basis(V))
+    vcon = _contrib(l, get_fe_basis(V))
+    return assem.assemble(m, vcon, lift=True, A=A, b=b)
 
 
 class AffineFEOperator:
