@@ -370,7 +370,8 @@ __global__ __launch_bounds__(kBlock) void column_sum_kernel(
     const uint16_t *__restrict__ noff, const uint16_t *__restrict__ cslot,
     const T *__restrict__ E, double *__restrict__ nzval, FusedRhs F) {
   extern __shared__ __align__(16) uint32_t shm[];
-  constexpr int U = 12;  // staged entries per thread and batch
+  // staged entries per thread and batch (simplex ranges hold at most kBlock·8 contributions)
+  constexpr int U = FULL_NLOC > 0 ? 12 : 8;
   constexpr int R = 8;   // nonzeros per thread whose offsets are prefetched
   const int rpad = metric_ref_pad(ns);
   unsigned char *lds = reinterpret_cast<unsigned char *>(shm);
@@ -470,8 +471,9 @@ __global__ __launch_bounds__(kBlock) void column_sum_kernel(
     }
   }
   __syncthreads();
-  if (F.b && tid < j1 - j0) {
-    const int64_t J = j0 + tid;
+  // the range's column sums of b on the last threads, which hold the fewest nonzeros
+  if (F.b && tid >= kBlock - (j1 - j0)) {
+    const int64_t J = j0 + (tid - (kBlock - (j1 - j0)));
     double bs = 0.0;
     for (int k = adjptr[J] - a0, ke = adjptr[J + 1] - a0; k < ke; ++k) bs += rv[k];
     F.b[J] = bs;
