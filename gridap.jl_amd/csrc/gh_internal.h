@@ -128,7 +128,7 @@ struct Ctx {
   DBuf<int32_t> bstart;
   DBuf<uint16_t> dmask;             // per cell: bit b set when local dof b is Dirichlet
   DBuf<unsigned char> elem;         // per cell: the element matrix or metric record (FP32/FP64)
-  DBuf<double> fdet;                // simplices: f |det DF| per cell; hexes: Σ w|J| φ_a (C x 27)
+  DBuf<double> fdet;                // full-row simplices: f |det DF| per cell; hexes: Σ w|J| φ_a (C x 27)
   double nu = 0.3;                  // elasticity: Poisson ratio
   double body[3] = {0, 0, 0};       // elasticity: body force
   DBuf<double> sref;                // reference tensor Ĝ (6 x nloc x nloc) and m (nloc)

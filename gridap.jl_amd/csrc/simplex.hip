@@ -304,10 +304,10 @@ __global__ void simplex_sum_kernel(int64_t nnz, const int32_t *__restrict__ csta
 }
 
 // Right-hand side fused into the column sum of simplices (b = NULL: not fused): per
-// adjacency entry (cell, a) the term f|det| m_a minus the lifting of the cell's
-// Dirichlet dofs, and per column the sum of its terms in ascending cell order.
+// adjacency entry (cell, a) the term f|det| m_a (f|det| from the cell's metric record)
+// minus the lifting of the cell's Dirichlet dofs, and per column the sum of its terms
+// in ascending cell order.
 struct FusedRhs {
-  const double *fdet;
   const double *mref;
   const uint16_t *dmask;
   const int32_t *cell_dofs;
@@ -542,8 +542,8 @@ void launch_simplex(Ctx &c) {
                      dim3(kBlock), 0, c.stream, c.ncells, c.cell_nodes.get(), c.xyz.get(),
                      c.have_kappa ? c.kappa.get() : nullptr,
                      c.have_fsrc ? c.fsrc.get() : nullptr, c.sref.get(), E, c.fdet.get());
-  const FusedRhs F{c.fdet.get(), c.sref.get() + 6 * NLOC * NLOC, c.dmask.get(),
-                   c.cell_dofs.get(), c.have_dvals ? c.dvals.get() : nullptr, c.b.get()};
+  const FusedRhs F{c.sref.get() + 6 * NLOC * NLOC, c.dmask.get(), c.cell_dofs.get(),
+                   c.have_dvals ? c.dvals.get() : nullptr, c.b.get()};
   const bool fused = elemlist_sum(c, sizeof(T) == 4, &F);
   GH_REQUIRE(fused == METRIC, GH_ERR_STATE, "simplex storage does not match the sum");
   if (!fused)
