@@ -372,7 +372,9 @@ __global__ __launch_bounds__(kBlock) void column_sum_kernel(
   extern __shared__ __align__(16) uint32_t shm[];
   // staged entries per thread and batch (simplex ranges hold at most kBlock·8 contributions)
   constexpr int U = FULL_NLOC > 0 ? 12 : 8;
-  constexpr int R = 8;   // nonzeros per thread whose offsets are prefetched
+  // nonzeros per thread whose offsets are prefetched (a simplex range of kBlock·8
+  // contributions holds about 3·kBlock nonzeros)
+  constexpr int R = FULL_NLOC > 0 ? 8 : 4;
   const int rpad = metric_ref_pad(ns);
   unsigned char *lds = reinterpret_cast<unsigned char *>(shm);
   double *rv = reinterpret_cast<double *>(lds);
