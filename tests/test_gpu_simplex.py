@@ -173,3 +173,73 @@ def test_tet_column_sum_matches_list_sum(monkeypatch):
                           precision=precision)
         np.testing.assert_array_equal(op1.get_matrix().nzval, op2.get_matrix().nzval)
         np.testing.assert_array_equal(op1.get_vector(), op2.get_vector())
+
+
+def _fan_mesh(m):
+    """m ring vertices around a centre vertex, with an apex above and below: 2m
+    tetrahedra share the centre (vertex 0)."""
+    t = 2 * np.pi * np.arange(m) / m
+    xyz = np.vstack([[0.0, 0.0, 0.0], np.c_[np.cos(t), np.sin(t), np.zeros(m)],
+                     [0.0, 0.0, 1.0], [0.0, 0.0, -1.0]])
+    top, bot = m + 1, m + 2
+    ring = 1 + np.arange(m)
+    nxt = 1 + (np.arange(m) + 1) % m
+    cells = np.vstack([np.c_[np.zeros(m, int), ring, nxt, np.full(m, top)],
+                       np.c_[np.zeros(m, int), nxt, ring, np.full(m, bot)]])
+    return xyz, cells
+
+
+def _p1_reference(xyz, cells, kappa):
+    """Dense P1 stiffness and load (f = 1): K_ab = κ|det DF|/6 ∇λ_a·∇λ_b, b_a = |det|/24."""
+    n = xyz.shape[0]
+    K, b = np.zeros((n, n)), np.zeros(n)
+    for c, v in enumerate(cells):
+        DF = (xyz[v[1:]] - xyz[v[0]]).T
+        det = np.linalg.det(DF)
+        g = np.linalg.inv(DF)                       # rows: ∇λ_1..3
+        grads = np.vstack([-g.sum(axis=0), g])
+        K[np.ix_(v, v)] += kappa[c] * abs(det) / 6.0 * grads @ grads.T
+        b[v] += abs(det) / 24.0
+    return K, b
+
+
+def test_high_valence_column_gets_its_own_range(monkeypatch):
+    """A vertex shared by 600 tetrahedra: its column alone holds 2400 contributions,
+    more than a column range's budget, so it forms a range of its own.  P1, every dof
+    free; the compact sum and the int32-list sum agree bitwise, and both match the
+    dense P1 reference (FP64: 1e-12 relative)."""
+    announce("simplex P1 fan mesh, 600 tets on one vertex")
+    xyz, cells = _fan_mesh(300)
+    C, n = cells.shape[0], xyz.shape[0]
+    rng = np.random.default_rng(17)
+    kappa = np.exp(rng.normal(0, 1, C))
+    reffe = G.ReferenceFE(G.lagrangian, G.Float64, 1)
+    q = G.Quadrature(3, 2, simplex=True)
+    shp, grad = reffe.tabulate(3, q.coords, simplex=True)
+    ptrs = (np.arange(C + 1) * 4 + 1).astype(np.int32)
+    cd = (cells + 1).astype(np.int32)
+    out = {}
+    for env in ("1", "0"):
+        monkeypatch.setenv("GRIDAP_HIP_COMPACT_LISTS", env)
+        ctx = _lib.Context(0)
+        ctx.mesh_upload(xyz, cd.ravel(), ptrs)
+        ctx.space_upload(_lib.GH_REFFE_PK, 1, 1, cd.ravel(), ptrs, n, 0)
+        ctx.reffe_tabulation(q.weights, shp, grad)
+        ctx.symbolic()
+        ctx.coef_upload(kappa, None, None)
+        ctx.assemble(_lib.GH_FORM_DIFFUSION, _lib.GH_PREC_FP64, _lib.GH_PATH_AUTO)
+        nz, b = ctx.values_get()
+        colptr, rowval = ctx.pattern_get()
+        out[env] = (colptr, rowval, nz, b)
+        ctx.close()
+    for k in range(4):
+        np.testing.assert_array_equal(out["1"][k], out["0"][k])
+    colptr, rowval, nz, b = out["1"]
+    A = np.zeros((n, n))
+    for j in range(n):
+        for p in range(colptr[j] - 1, colptr[j + 1] - 1):
+            A[rowval[p] - 1, j] = nz[p]
+    K, bref = _p1_reference(xyz, cells, kappa)
+    assert colptr[1] - colptr[0] == n          # the centre couples to every vertex
+    assert np.max(np.abs(A - K)) <= 1e-12 * np.max(np.abs(K))
+    assert np.max(np.abs(b - bref)) <= 1e-12 * np.max(np.abs(bref))
