@@ -568,14 +568,17 @@ bool simplex_eligible(const Ctx &c) {
 // within a block's ranges, and hexahedron element storage addressable in 32 bits.
 // Simplices then store metric records.  The column ranges are cut greedily in column
 // order: a range grows while it holds at most kBlock columns and a budget of
-// contributions (kBlock·8 for simplices, kBlock·12 for full columns), and its extents
-// fit 16 bits and the LDS (FP32 within kMaxColumnLds; FP64 within the device's
-// per-workgroup limit, at most kMaxColumnLds64).  Budgeted ranges keep the LDS of a
+// contributions (kBlock·8 for simplices, kBlock·12 for full columns) and an FP32 LDS
+// target, and its extents fit 16 bits and the LDS (FP32 within kMaxColumnLds; FP64
+// within the device's per-workgroup limit, at most kMaxColumnLds64).  Budgeted ranges keep the LDS of a
 // block near the mean: the numbering puts the vertex dofs, which touch many more cells
 // than edge or face dofs, first, and a fixed column count per range sized the LDS of
 // every block for the heaviest.
 constexpr size_t kMaxColumnLds = 48 * 1024;
 constexpr size_t kMaxColumnLds64 = 64 * 1024;
+// A range also stops growing once its FP32 LDS would pass kColumnLdsTarget, so that 8
+// workgroups (32 waves) fit the 160 KB of a CU; a single column may still exceed it.
+constexpr size_t kColumnLdsTarget = 18 * 1024;
 
 // maxbase as the kernel takes it: at least 2 and even (metric entries stay 16-byte aligned)
 int lds_maxbase(int maxbase) { return (std::max(maxbase, 1) + 1) & ~1; }
@@ -639,7 +642,9 @@ void compact_contributions(Ctx &c, int lbits) {
   for (int64_t j = 1; j <= N; ++j) {
     if (j < N) {  // extend [jb, j) to [jb, j + 1)?
       const int64_t a = A[j + 1] - A[jb], q = C[j + 1] - C[jb];
-      if (j - jb < kBlock && q <= budget && fits(a, q)) continue;
+      if (j - jb < kBlock && q <= budget && fits(a, q) &&
+          column_sum_lds(metric, a, q, 4, metric, ns) <= kColumnLdsTarget)
+        continue;
     }
     const int64_t a = A[j] - A[jb], q = C[j] - C[jb];
     if (!fits(a, q)) return;  // one column too large: keep cstart/perm and full rows
