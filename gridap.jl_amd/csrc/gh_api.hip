@@ -32,6 +32,19 @@ int guarded(gh_ctx *ctx, F &&f) {
   }
 }
 
+// A new mesh voids the FE space uploaded for the previous one: its cell_dof_ids were
+// numbered on the old cells (gh_space_upload must follow gh_mesh_*).
+void invalidate_space(gh::Ctx &c) {
+  c.have_space = false;
+  c.nloc = 0;
+  c.nfree = c.ndiri = 0;
+  c.cell_dofs.release();
+  c.h_cell_dofs.clear();
+  c.h_cell_dofs.shrink_to_fit();
+  c.have_kappa = c.have_fsrc = c.have_dvals = false;
+  c.nq = 0;
+}
+
 void invalidate_pattern(gh::Ctx &c) {
   c.have_pattern = false;
   c.have_values = false;
@@ -124,6 +137,7 @@ int gh_mesh_cartesian(gh_ctx *ctx, int dim, const double *origin, const double *
     c.nv = 1 << dim;
     c.xyz.release();
     c.cell_nodes.release();
+    invalidate_space(c);
     invalidate_pattern(c);
   });
 }
@@ -158,6 +172,7 @@ int gh_mesh_upload(gh_ctx *ctx, int dim, int64_t nnodes, const double *xyz, int6
                            hipMemcpyHostToDevice));
     GH_CHECK_HIP(hipMemcpy(c.cell_nodes.get(), cn.data(), sizeof(int32_t) * cn.size(),
                            hipMemcpyHostToDevice));
+    invalidate_space(c);
     invalidate_pattern(c);
   });
 }
@@ -224,6 +239,9 @@ int gh_reffe_tabulation(gh_ctx *ctx, int nq, int nloc, const double *qw, const d
 int gh_symbolic(gh_ctx *ctx, int64_t *nnz_out) {
   return guarded(ctx, [&](gh::Ctx &c) {
     GH_REQUIRE(c.have_space, GH_ERR_STATE, "upload the FE space before gh_symbolic");
+    GH_REQUIRE(c.nloc > 0 && c.cell_dofs.n == size_t(c.ncells) * c.nloc &&
+                   c.h_cell_dofs.size() == c.cell_dofs.n,
+               GH_ERR_STATE, "the FE space does not match the mesh: upload it again");
     invalidate_pattern(c);
     if (!gh::symbolic_lattice(c)) {
       gh::symbolic_general(c, gh::simplex_eligible(c) || gh::hexq2_eligible(c));
@@ -312,6 +330,40 @@ int gh_values_get(gh_ctx *ctx, double *nzval, double *b) {
       GH_CHECK_HIP(hipMemcpyAsync(b, c.b.get(), sizeof(double) * c.nfree,
                                   hipMemcpyDeviceToHost, c.stream));
     GH_CHECK_HIP(hipStreamSynchronize(c.stream));
+  });
+}
+
+int gh_values_set(gh_ctx *ctx, const double *nzval, const double *b) {
+  return guarded(ctx, [&](gh::Ctx &c) {
+    GH_REQUIRE(c.have_pattern, GH_ERR_STATE, "no pattern: call gh_symbolic first");
+    GH_REQUIRE(nzval || c.have_values, GH_ERR_STATE,
+               "no matrix on the device: pass nzval or assemble first");
+    if (nzval && c.nnz)
+      GH_CHECK_HIP(hipMemcpyAsync(c.nzval.get(), nzval, sizeof(double) * c.nnz,
+                                  hipMemcpyHostToDevice, c.stream));
+    if (b && c.nfree)
+      GH_CHECK_HIP(hipMemcpyAsync(c.b.get(), b, sizeof(double) * c.nfree,
+                                  hipMemcpyHostToDevice, c.stream));
+    GH_CHECK_HIP(hipStreamSynchronize(c.stream));
+    c.have_values = true;
+    if (nzval) c.fsai_built = false;
+  });
+}
+
+int gh_pattern_info_get(gh_ctx *ctx, gh_pattern_info *info) {
+  return guarded(ctx, [&](gh::Ctx &c) {
+    GH_REQUIRE(info, GH_ERR_ARG, "NULL info");
+    GH_REQUIRE(c.have_pattern, GH_ERR_STATE, "no pattern: call gh_symbolic first");
+    gh_pattern_info r{};
+    r.nnz = c.nnz;
+    r.nranges = c.slots && c.bstart.n ? (int64_t)c.bstart.n - 1 : 0;
+    r.lattice = c.structured;
+    r.box = c.box;
+    r.elemlist = c.elemlist;
+    r.compact = c.slots;
+    r.metric = c.metric;
+    r.inverse = c.inverse;
+    *info = r;
   });
 }
 

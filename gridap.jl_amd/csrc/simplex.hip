@@ -12,16 +12,21 @@
 // (DF, its inverse and determinant) is FP64; the contraction is FP32 or FP64 by
 // precision mode; every sum into nzval and b is FP64.
 //
-// Three passes, each fully parallel and deterministic:
-//  * element: one thread per cell computes the metric (FP64); the block then writes
-//    every cell's element matrix (row-major, nloc x nloc) with coalesced stores, so
-//    the entries a column needs from a cell form one contiguous row;
-//  * sum: one thread per nonzero adds its entries in ascending cell order (the
-//    order of Gridap's cell loop), from the contribution list that the symbolic
-//    phase keeps (a stable sort of the (column, row) keys carries the cell index),
-//    in FP64, and writes nzval once;
-//  * rhs: one thread per free dof walks its cells (dof -> cell adjacency, ascending)
-//    and adds f|det| m_a minus the lifting of the cell's Dirichlet dofs.
+// Two passes, each fully parallel and deterministic:
+//  * element (simplex_element_kernel): one thread per cell computes DF, det DF and
+//    the metric in FP64 and writes the cell's metric record (g_0..g_5 in the element
+//    precision and f|det DF| in FP64); block 0 also writes the reference tensor.  No
+//    element matrix is stored;
+//  * column sum (column_sum_kernel): one workgroup per range of consecutive CSC
+//    columns stages the reference tensor and the metric records of the cells its
+//    columns touch in LDS, forms each contribution Σ_k g_k Ĝ^k_ab there, and one
+//    thread per nonzero adds its contributions in ascending cell order (the order of
+//    Gridap's cell loop) in FP64 and writes nzval once.  The right-hand side
+//    (f|det DF| m_a minus the lifting of the cell's Dirichlet dofs) is fused into the
+//    same pass, one thread per column.
+// A column too heavy for one range keeps the int32 contribution lists of the
+// symbolic phase and the one-thread-per-nonzero sum (simplex_sum_kernel) over full
+// element rows, with simplex_rhs_kernel for b; both forms give the same bits.
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>

@@ -45,6 +45,12 @@ class SolveStats(C.Structure):
                 ("precond_used", C.c_int32), ("reserved", C.c_int32)]
 
 
+class PatternInfo(C.Structure):
+    _fields_ = [("nnz", C.c_int64), ("nranges", C.c_int64), ("lattice", C.c_int32),
+                ("box", C.c_int32), ("elemlist", C.c_int32), ("compact", C.c_int32),
+                ("metric", C.c_int32), ("inverse", C.c_int32)]
+
+
 SIGNATURES = {
     "gh_create": (_i32, [C.POINTER(_vp), _i32]),
     "gh_destroy": (_i32, [_vp]),
@@ -59,6 +65,8 @@ SIGNATURES = {
     "gh_coef_upload": (_i32, [_vp, _vp, _vp, _vp]),
     "gh_assemble": (_i32, [_vp, _i32, _i32, _i32]),
     "gh_values_get": (_i32, [_vp, _vp, _vp]),
+    "gh_values_set": (_i32, [_vp, _vp, _vp]),
+    "gh_pattern_info_get": (_i32, [_vp, C.POINTER(PatternInfo)]),
     "gh_numeric": (_i32, [_vp, _i32, _vp, _vp, _i32, _vp, _vp, _vp]),
     "gh_spmv": (_i32, [_vp, _vp, _vp]),
     "gh_pcg": (_i32, [_vp, _i32, _vp, _vp, _dbl, _i32, C.POINTER(SolveStats)]),
@@ -213,6 +221,36 @@ class Context:
         bo = b if b is not None and self.nfree > 0 else np.zeros(max(self.nfree, 1))
         self.check(lib().gh_values_get(self.h, ptr(nzo), ptr(bo)))
         return (nz if nz is not None else nzo[:self.nnz]), (b if b is not None else bo[:self.nfree])
+
+    def values_set(self, nz=None, b=None):
+        """Load nzval (on this context's pattern) and/or b into the device."""
+        for name, a, n in (("nz", nz, self.nnz), ("b", b, self.nfree)):
+            if a is not None and (not isinstance(a, np.ndarray) or a.dtype != np.float64 or
+                                  not a.flags.c_contiguous or a.shape != (n,)):
+                raise ValueError("%s must be a contiguous float64 array of %d entries" % (name, n))
+        self.check(lib().gh_values_set(self.h, ptr(nz), ptr(b)))
+
+    def pattern_info(self):
+        """The numeric form chosen for the uploaded space (dict of gh_pattern_info)."""
+        info = PatternInfo()
+        self.check(lib().gh_pattern_info_get(self.h, C.byref(info)))
+        return {k: int(getattr(info, k)) for k, _ in PatternInfo._fields_}
+
+    def numeric(self, form, cell_coef=None, cell_source=None, precision=GH_PREC_FP64,
+                dirichlet_vals=None):
+        """gh_numeric: the one-call numeric phase from host arrays -> (nzval, b)."""
+        k = None if cell_coef is None else np.ascontiguousarray(cell_coef, dtype=np.float64)
+        f = None if cell_source is None else np.ascontiguousarray(cell_source, dtype=np.float64)
+        g = None if dirichlet_vals is None else np.ascontiguousarray(dirichlet_vals,
+                                                                     dtype=np.float64)
+        for name, a, n in (("cell_coef", k, self.ncells), ("cell_source", f, self.ncells)):
+            if a is not None and a.shape != (n,):
+                raise ValueError("%s must have one value per cell" % name)
+        nz = np.zeros(max(self.nnz, 1))
+        b = np.zeros(max(self.nfree, 1))
+        self.check(lib().gh_numeric(self.h, int(form), ptr(k), ptr(f), int(precision), ptr(nz),
+                                    ptr(b), ptr(g)))
+        return nz[:self.nnz], b[:self.nfree]
 
     def spmv(self, x):
         x = np.ascontiguousarray(x, dtype=np.float64)

@@ -71,6 +71,10 @@ class SparseMatrixAssembler:
         self.nnz = None
         self.colptr = self.rowval = None
         self._degree = None
+        # the SparseMatrixCSC whose stored values the device holds (None: the device
+        # values are not those of any matrix handed out, e.g. after a vector-only
+        # assembly); solvers load their matrix into the device when it is not this one
+        self.device_matrix = None
 
     def get_rows(self):
         return self.space.get_free_dof_ids()
@@ -189,7 +193,19 @@ class SparseMatrixAssembler:
         if A is None:
             n = self.space.nfree
             A = SparseMatrixCSC(n, n, self.colptr.copy(), self.rowval.copy(), nz)
+        self.device_matrix = A if matcontrib is not None else None
         return A, bv
+
+    def load_matrix(self, A):
+        """Make the device hold A's stored values (A carries this assembler's pattern),
+        so that the device SpMV and PCG work on A."""
+        if self.device_matrix is A:
+            return
+        if not (A.shape == (self.space.nfree, self.space.nfree) and
+                np.array_equal(A.colptr, self.colptr) and np.array_equal(A.rowval, self.rowval)):
+            raise ValueError("A does not have this assembler's sparsity pattern")
+        self.ctx.values_set(np.ascontiguousarray(A.nzval, dtype=np.float64))
+        self.device_matrix = A
 
 
 def _contrib(form, *args):
@@ -313,7 +329,13 @@ class PCGSolver:
                         "fsai": _lib.GH_PRECOND_FSAI}[precond]
         self.stats = None
 
-    def solve(self, assem, b):
+    def solve(self, assem, b, A=None):
+        """x with A x = b on the device; A (default: the matrix the device holds) must
+        carry assem's pattern and is loaded into the device when it is not the one there."""
+        if A is not None:
+            assem.load_matrix(A)
+        elif assem.device_matrix is None:
+            raise RuntimeError("no matrix on the device: pass A")
         x, st = assem.ctx.pcg(b=b, precond=self.precond, rtol=self.rtol, maxit=self.maxit)
         self.stats = st
         return x
@@ -325,5 +347,5 @@ LinearFESolver = PCGSolver
 def solve(op, solver=None):
     """uh = solve(op) (src/FESpaces/FESolvers.jl:65-69)."""
     solver = solver or PCGSolver()
-    x = solver.solve(op.assem, op.vector)
+    x = solver.solve(op.assem, op.vector, op.matrix)
     return FEFunction(op.trial, x)

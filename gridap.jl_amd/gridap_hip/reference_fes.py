@@ -155,48 +155,57 @@ def gauss_legendre01(n):
     return (x + 1.0) / 2.0, w / 2.0
 
 
-# Symmetric TET rules (barycentric orbits; weights sum to 1/6).  Degree 2: the
-# 4-point S31 rule with a = (5 - sqrt 5)/20.  Degrees 4-5: the 14-point rule of two
-# S31 orbits and one S22 orbit exact to degree 5 -- the structure of the
-# Witherden-Vincent rule Gridap selects for TET at degrees 4 and 5
-# (src/ReferenceFEs/Quadratures.jl:157-175); the values below are the interior,
-# positive-weight root of its moment equations.
-_TET5_S31 = ((0.0122488405193937, 0.09273525031089135),
-             (0.01878132095300283, 0.3108859192633008))
-_TET5_S22 = (0.00709100346284676, 0.04550370412564866)
+# Witherden-Vincent symmetric TET rules, the ones Gridap's Quadrature(TET, degree)
+# selects for degree <= 10 (src/ReferenceFEs/Quadratures.jl:157-165).  Published
+# values (F.D. Witherden, P.E. Vincent, Comput. Math. Appl. 2015) on the bi-unit TET
+# with vertices (-1,-1,-1), (1,-1,-1), (-1,1,-1), (-1,-1,1), as tabulated in
+# src/ReferenceFEs/WitherdenVincentQuadratures.jl:367-400: a centre point of weight
+# w0, orbits "S31" (w, s, t) -> points (s,s,t), (s,t,s), (t,s,s), (s,s,s), orbits
+# "S22" (w, s, t) -> (s,t,t), (t,s,t), (s,s,t), (s,t,s), (t,s,s), (t,t,s), in this
+# order (:471-513).  The map to the unit TET is x -> (x+1)/2, w -> w/2 (:1290-1299),
+# then the weights are scaled to sum to the TET measure 1/6.
+_WV_TET = {
+    1: (4.0 / 3.0, (), ()),
+    2: (None, ((0.33333333333333333333, -0.72360679774997896964, 0.17082039324993690892),),
+        ()),
+    3: (None, ((0.18162379004944980942, -0.34367339496723662642, -0.96897981509829012074),
+               (0.15170954328388352391, -0.78390550020314279176, 0.35171650060942837529)),
+        ()),
+    4: (None, ((0.15025056762402113440, -0.37822816147339878041, -0.86531551557980365878),
+               (0.097990724155149266058, -0.81452949937821754720, 0.44358849813465264159)),
+        ((0.056728027702775288584, -0.90899259174870070102, -0.091007408251299298984),)),
+}
+_WV_TET[0] = _WV_TET[1]
+_WV_TET[5] = _WV_TET[4]
 
 
 def _tet_rule(degree):
-    if degree <= 1:
-        return np.array([[0.25, 0.25, 0.25]]), np.array([1.0 / 6.0])
-    orbits31, orbits22 = [], []
-    if degree == 2:
-        orbits31 = [(1.0 / 24.0, (5.0 - 5.0 ** 0.5) / 20.0)]
-    elif degree in (4, 5):
-        orbits31, orbits22 = list(_TET5_S31), [_TET5_S22]
-    else:
-        raise NotImplementedError("TET quadrature is provided for degrees 1, 2, 4 and 5")
-    pts, wts = [], []
-    for w, a in orbits31:
-        for k in range(4):
-            lam = [a] * 4
-            lam[k] = 1.0 - 3.0 * a
-            pts.append(lam[1:])
-            wts.append(w)
-    for w, b in orbits22:
-        for i in range(4):
-            for j in range(i + 1, 4):
-                lam = [0.5 - b] * 4
-                lam[i] = lam[j] = b
-                pts.append(lam[1:])
-                wts.append(w)
-    return np.array(pts), np.array(wts)
+    """Quadrature(TET, degree) for degrees 0-5: points (n x 3) and weights."""
+    if degree not in _WV_TET:
+        raise NotImplementedError("TET quadrature is provided for degrees 0 to 5")
+    w0, s31, s22 = _WV_TET[degree]
+    rows = []
+    if w0 is not None:
+        rows.append((w0, -0.5, -0.5, -0.5))
+    for w, a, b in s31:
+        for x in ((a, a, b), (a, b, a), (b, a, a), (a, a, a)):
+            rows.append((w,) + x)
+    for w, a, b in s22:
+        for x in ((a, b, b), (b, a, b), (a, a, b), (a, b, a), (b, a, a), (b, b, a)):
+            rows.append((w,) + x)
+    wx = np.array(rows, dtype=np.float64)
+    w = wx[:, 0] / 2.0
+    pts = (wx[:, 1:] + 1.0) / 2.0
+    tot = 0.0
+    for v in w:                     # left-to-right sum, as a plain loop adds them
+        tot += v
+    return pts, w * ((1.0 / 6.0) / tot)
 
 
 class Quadrature:
     """Quadrature(polytope, degree).  n-cube: tensor-product Gauss-Legendre rule,
     div(degree,2)+1 points per axis, x fastest (src/ReferenceFEs/Quadratures.jl:191,
-    203-222).  simplex=True: the symmetric TET rules above."""
+    203-222).  simplex=True: the Witherden-Vincent TET rules above."""
 
     def __init__(self, D, degree, simplex=False):
         self.degree = degree

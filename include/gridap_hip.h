@@ -136,8 +136,31 @@ int gh_elasticity_params(gh_ctx *ctx, double nu, const double *body_force);
 /* Copy the assembled nzval[nnz] and b[nfree] to the host (either may be NULL). */
 int gh_values_get(gh_ctx *ctx, double *nzval, double *b);
 
-/* One-call numeric phase from host arrays: gh_coef_upload + gh_assemble(AUTO) +
- * gh_values_get.  qdeg must match the uploaded tabulation's rule (checked by nq). */
+/* Load a matrix with this context's pattern (nzval[nnz], the stored values of a
+ * SparseMatrixCSC whose colptr/rowval are gh_pattern_get's) and/or a right-hand side
+ * b[nfree] into the device, in place of the assembled ones; either may be NULL.  The
+ * matrix argument of solve!(x, solver, A, b) / numerical_setup(ss, A),
+ * src/Algebra/LinearSolvers.jl:202-274: gh_spmv and gh_pcg then work on this A. */
+int gh_values_set(gh_ctx *ctx, const double *nzval, const double *b);
+
+/* Which numeric form the symbolic phase chose for the uploaded space (diagnostic;
+ * no counterpart in Gridap). */
+typedef struct gh_pattern_info {
+  int64_t nnz;
+  int64_t nranges;     /* column ranges of the element-list column sum (0: not used)   */
+  int32_t lattice;     /* 1: owner-computes lattice kernel (Cartesian scalar Q1)       */
+  int32_t box;         /* 1: free dofs form a box of lattice nodes (stencil PCG)       */
+  int32_t elemlist;    /* 1: element-list path (simplices, vector Q2 hexahedra)        */
+  int32_t compact;     /* 1: compact 16-bit contribution lists (column-range sum);
+                          0 on the element-list path: int32 lists, one thread per nonzero */
+  int32_t metric;      /* 1: simplex metric records (entries formed in the column sum) */
+  int32_t inverse;     /* 1: full-column staging (vector Q2 element matrices)          */
+} gh_pattern_info;
+int gh_pattern_info_get(gh_ctx *ctx, gh_pattern_info *info);
+
+/* One-call numeric phase from host arrays: gh_coef_upload(cell_coef, cell_source,
+ * dirichlet_vals) + gh_assemble(form_kind, precision, GH_PATH_AUTO) + gh_values_get.
+ * The quadrature is the one of the uploaded tabulation (gh_reffe_tabulation). */
 int gh_numeric(gh_ctx *ctx, int form_kind, const double *cell_coef, const double *cell_source,
                int precision, double *nzval, double *b, const double *dirichlet_vals);
 

@@ -144,5 +144,96 @@ def test_elasticity_column_sum_matches_list_sum(monkeypatch):
         op1, _ = build(part, E, precision=precision)
         monkeypatch.setenv("GRIDAP_HIP_COMPACT_LISTS", "0")
         op2, _ = build(part, E, precision=precision)
+        assert op1.assem.ctx.pattern_info()["compact"] == 1
+        assert op2.assem.ctx.pattern_info()["compact"] == 0
         np.testing.assert_array_equal(op1.get_matrix().nzval, op2.get_matrix().nzval)
         np.testing.assert_array_equal(op1.get_vector(), op2.get_vector())
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_elasticity_17cube_matches_oracle(oracle, precision):
+    """4913 cells: more cells than waves in the FP32 element kernel's grid (1024
+    workgroups x 4 waves), so waves walk several cells in both modes; the whole
+    system against the oracle."""
+    part = (17, 17, 17)
+    announce("elasticity Q2 %s %s vs oracle" % (part, precision))
+    C = int(np.prod(part))
+    E = np.exp(np.random.default_rng(17).normal(0, 0.5, C))
+    op, V = build(part, E, precision=precision)
+    info = op.assem.ctx.pattern_info()
+    assert info["elemlist"] == 1 and info["compact"] == 1 and info["inverse"] == 1
+    r, cp, rv, nz, b = oracle_system(oracle, part, E, 0.3, (0, 0, -1), CLAMP_X0)
+    np.testing.assert_array_equal(V.cell_dof_ids, r["cell_dofs"])
+    A = op.get_matrix()
+    np.testing.assert_array_equal(A.colptr, cp)
+    np.testing.assert_array_equal(A.rowval, rv)
+    if precision == "fp64":
+        assert relmax(A.nzval, nz) <= 1e-12
+        assert relmax(op.get_vector(), b) <= 1e-12
+    else:
+        assert relfrob(A.nzval, nz) <= 1e-6
+        assert relfrob(op.get_vector(), b) <= 1e-6
+
+
+def _config3_E(part, seed=11):
+    return np.exp(np.random.default_rng(seed).normal(0, 0.5, int(np.prod(part))))
+
+
+def test_elasticity_48cube_fp32_vs_fp64():
+    """The benchmarked workload at full size (48^3, E=exp(N(0,0.5^2)), ν=0.3,
+    f=(0,0,-1), x=0 clamped): FP32 (matrix-core products) against FP64 within 1e-6
+    relative Frobenius, nzval and b; NNZ and free dofs equal to the closed counts."""
+    part = (48, 48, 48)
+    announce("elasticity Q2 48^3 fp32 vs fp64 (config 3)")
+    E = _config3_E(part)
+    op64, V = build(part, E, precision="fp64")
+    A64, b64 = op64.get_matrix(), op64.get_vector()
+    assert A64.nnz() == 506929500 and V.num_free_dofs() == 2709792
+    op64.assem.ctx.close()
+    announce("elasticity Q2 48^3 fp32 leg")
+    op32, _ = build(part, E, precision="fp32")
+    A32, b32 = op32.get_matrix(), op32.get_vector()
+    np.testing.assert_array_equal(A32.rowval, A64.rowval)
+    assert relfrob(A32.nzval, A64.nzval) <= 1e-6
+    assert relfrob(b32, b64) <= 1e-6
+    assert not np.array_equal(A32.nzval, A64.nzval)
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_elasticity_48cube_rigid_modes(precision):
+    """48^3 with the lognormal E and no Dirichlet tags: the three translations and
+    three rotations are in the kernel of A (checked with the device SpMV)."""
+    part = (48, 48, 48)
+    announce("elasticity Q2 48^3 rigid modes %s" % precision)
+    opn, Vn = build(part, _config3_E(part), tags=[], precision=precision)
+    xyz, comp = Vn.free_dof_coords, Vn.free_dof_comp
+    Ad = opn.assem.ctx
+    scale = np.abs(opn.get_matrix().nzval).max()
+    tol = 1e-5 if precision == "fp32" else 1e-10
+    for k in range(3):
+        t = (comp == k).astype(np.float64)
+        assert np.abs(Ad.spmv(t)).max() <= tol * scale
+    for a, b in ((0, 1), (1, 2), (2, 0)):
+        rot = np.where(comp == a, -xyz[:, b], 0.0) + np.where(comp == b, xyz[:, a], 0.0)
+        assert np.abs(Ad.spmv(rot)).max() <= tol * scale
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_elasticity_48cube_quadratic_patch(precision):
+    """48^3, E = 1, Dirichlet data u on the whole boundary, f = -div σ(u) for a
+    quadratic u: the Q2 space holds u, so A u_I = b up to rounding (max-abs residual
+    relative to max|A| max|u_I|: 1e-12 FP64, 2e-5 FP32)."""
+    part = (48, 48, 48)
+    announce("elasticity Q2 48^3 quadratic patch %s" % precision)
+    nu = 0.3
+    lam, mu = CO.lame(1.0, nu)
+    f = -np.array([lam + mu, lam + mu, lam + 3 * mu])
+    u = lambda x: np.stack([x[..., 0] * x[..., 1], x[..., 1] * x[..., 2],
+                            x[..., 2] * x[..., 0] + x[..., 0] ** 2], axis=-1)
+    op, V = build(part, 1.0, nu, f, ["boundary"], g=u, precision=precision)
+    uI = u(V.free_dof_coords)[np.arange(V.nfree), V.free_dof_comp]
+    b = op.get_vector()
+    r = op.assem.ctx.spmv(uI) - b
+    # relative to max|A| max|u_I|: FP32 entries carry ~1e-7 relative rounding each
+    rel = np.max(np.abs(r)) / (np.max(np.abs(op.get_matrix().nzval)) * np.max(np.abs(uI)))
+    assert rel <= (2e-5 if precision == "fp32" else 1e-12), rel

@@ -171,6 +171,8 @@ def test_tet_column_sum_matches_list_sum(monkeypatch):
         monkeypatch.setenv("GRIDAP_HIP_COMPACT_LISTS", "0")
         op2, _, _ = build(part, 2, kappa=kappa, tags=("tag_21", "tag_01"), g=g,
                           precision=precision)
+        assert op1.assem.ctx.pattern_info()["compact"] == 1
+        assert op2.assem.ctx.pattern_info()["compact"] == 0
         np.testing.assert_array_equal(op1.get_matrix().nzval, op2.get_matrix().nzval)
         np.testing.assert_array_equal(op1.get_vector(), op2.get_vector())
 
@@ -226,6 +228,8 @@ def test_high_valence_column_gets_its_own_range(monkeypatch):
         ctx.space_upload(_lib.GH_REFFE_PK, 1, 1, cd.ravel(), ptrs, n, 0)
         ctx.reffe_tabulation(q.weights, shp, grad)
         ctx.symbolic()
+        info = ctx.pattern_info()
+        assert info["elemlist"] == 1 and info["compact"] == (env == "1"), info
         ctx.coef_upload(kappa, None, None)
         ctx.assemble(_lib.GH_FORM_DIFFUSION, _lib.GH_PREC_FP64, _lib.GH_PATH_AUTO)
         nz, b = ctx.values_get()
@@ -243,3 +247,52 @@ def test_high_valence_column_gets_its_own_range(monkeypatch):
     assert colptr[1] - colptr[0] == n          # the centre couples to every vertex
     assert np.max(np.abs(A - K)) <= 1e-12 * np.max(np.abs(K))
     assert np.max(np.abs(b - bref)) <= 1e-12 * np.max(np.abs(bref))
+
+
+def _entry_keys(colptr, rowval, N):
+    """Key col·N + row (0-based) of every stored entry of a CSC pattern (1-based
+    colptr/rowval), ascending because columns are stored in order, rows sorted."""
+    return np.repeat(np.arange(N, dtype=np.int64), np.diff(colptr)) * N + (rowval - 1)
+
+
+def test_tet_96cube_full_size():
+    """The benchmarked P2 workload at full size (96^3 x 6 tets, jitter ±0.2h, seed 7),
+    FP64 and FP32 contraction: the column-range cut (compact lists, ~1e5 ranges) runs;
+    NNZ equals the closed count 230n^3 - 690n^2 + 732n - 273; the quadratic identity
+    A u_I = b holds (relative to max|A| max|u_I|: 1e-12 FP64, 2e-5 FP32); the values
+    are symmetric bit for bit on 200k sampled entries; every diagonal entry is
+    positive; FP32 against FP64 within 1e-6 relative Frobenius (nzval and b)."""
+    n = 96
+    part = (n, n, n)
+    u = lambda x: (1 + x[..., 0] - 2 * x[..., 1] + 0.5 * x[..., 2] + x[..., 0] ** 2 +
+                   x[..., 1] ** 2 + 2 * x[..., 1] * x[..., 2] - 0.5 * x[..., 2] ** 2)
+    C = 6 * n ** 3
+    res = {}
+    for precision in ("fp64", "fp32"):
+        announce("simplex P2 96^3x6 full size %s" % precision)
+        op, V, model = build(part, 2, seed=7, fsrc=np.full(C, -3.0), g=u, precision=precision)
+        info = op.assem.ctx.pattern_info()
+        assert info["compact"] == 1 and info["metric"] == 1 and info["nranges"] > 10000, info
+        A = op.get_matrix()
+        N = V.num_free_dofs()
+        assert A.nnz() == 230 * n ** 3 - 690 * n ** 2 + 732 * n - 273 == 197200239
+        assert N == (2 * n - 1) ** 3
+        uI = u(V.free_dof_coords)
+        b = op.get_vector()
+        r = op.assem.ctx.spmv(uI) - b
+        rel = np.max(np.abs(r)) / (np.max(np.abs(A.nzval)) * np.max(np.abs(uI)))
+        assert rel <= (1e-12 if precision == "fp64" else 2e-5), rel
+        key = _entry_keys(A.colptr, A.rowval, N)
+        sample = np.random.default_rng(96).integers(0, A.nnz(), 200000)
+        col, row = key[sample] // N, key[sample] % N
+        mirror = np.searchsorted(key, row * N + col)
+        assert np.all(key[mirror] == row * N + col)            # structurally symmetric
+        np.testing.assert_array_equal(A.nzval[sample], A.nzval[mirror])
+        dpos = np.searchsorted(key, np.arange(N, dtype=np.int64) * (N + 1))
+        assert np.all(key[dpos] == np.arange(N, dtype=np.int64) * (N + 1))
+        assert np.all(A.nzval[dpos] > 0)
+        res[precision] = (A.nzval, b)
+        del key
+        op.assem.ctx.close()
+    assert relfrob(res["fp32"][0], res["fp64"][0]) <= 1e-6
+    assert relfrob(res["fp32"][1], res["fp64"][1]) <= 1e-6

@@ -106,3 +106,48 @@ def test_product_does_not_reach_the_oracle():
             if f.endswith((".py", ".hip", ".h", ".cpp")):
                 text = open(os.path.join(dirpath, f)).read()
                 assert "gridap_oracle" not in text and "import oracle" not in text, f
+
+
+def _tet_moment_error(P, W, degree):
+    """Max |Σ w x^a y^b z^c - a!b!c!/(a+b+c+3)!| over monomials of total degree == degree."""
+    from math import factorial
+    err = 0.0
+    for a in range(degree + 1):
+        for b in range(degree + 1 - a):
+            c = degree - a - b
+            exact = factorial(a) * factorial(b) * factorial(c) / factorial(a + b + c + 3)
+            err = max(err, abs(np.sum(W * P[:, 0] ** a * P[:, 1] ** b * P[:, 2] ** c) - exact))
+    return err
+
+
+@pytest.mark.parametrize("degree,npts,exact_to", [(1, 1, 1), (2, 4, 2), (3, 8, 3),
+                                                  (4, 14, 5), (5, 14, 5)])
+def test_tet_rules_are_witherden_vincent(degree, npts, exact_to):
+    """Quadrature(TET, degree): Gridap's point count, interior points, positive weights
+    summing to 1/6, exact for every monomial up to the rule's degree and not beyond
+    (so a mistyped digit of the transcribed table fails)."""
+    q = G.Quadrature(3, degree, simplex=True)
+    P, W = q.coords, q.weights
+    assert len(W) == npts and np.all(W > 0)
+    assert np.all(P > 0) and np.all(P.sum(1) < 1)
+    assert abs(W.sum() - 1.0 / 6.0) <= 6e-17
+    for d in range(exact_to + 1):
+        assert _tet_moment_error(P, W, d) <= 6e-17, d
+    assert _tet_moment_error(P, W, exact_to + 1) > 1e-7
+
+
+def test_tet_degree4_rule_equals_the_moment_equation_root():
+    """The transcribed 14-point rule is the interior positive-weight root of the
+    degree-5 moment equations for two S31 orbits and one S22 orbit, which the oracle
+    derives independently by least squares; points listed in Gridap's orbit order
+    (S31: (s,s,t), (s,t,s), (t,s,s), (s,s,s) in bi-unit coordinates, then S22)."""
+    import conforming_oracle as CO
+    q = G.Quadrature(3, 4, simplex=True)
+    P2, W2 = CO.tet_degree5_rule()
+    key = lambda P, W: np.lexsort(np.c_[P, W].T)
+    k1, k2 = key(q.coords, q.weights), key(P2, W2)
+    np.testing.assert_allclose(q.coords[k1], P2[k2], atol=1e-14)
+    np.testing.assert_allclose(q.weights[k1], W2[k2], atol=1e-16)
+    # orbit order of the table: the first S31 orbit's fourth point is (s,s,s) mapped
+    s = q.coords[3]
+    assert s[0] == s[1] == s[2] and np.all(q.coords[0][:2] == s[:2]) and q.coords[0][2] != s[2]
