@@ -354,26 +354,6 @@ struct CellArgs {
   double *nzval, *b;
 };
 
-template <int D>
-__device__ __forceinline__ double det_inv(const double *J, double *Ji) {
-  if (D == 2) {
-    double det = J[0] * J[3] - J[1] * J[2];
-    double r = 1.0 / det;
-    Ji[0] = J[3] * r; Ji[1] = -J[1] * r; Ji[2] = -J[2] * r; Ji[3] = J[0] * r;
-    return det;
-  } else {
-    double a = J[0], b = J[1], cc = J[2], d = J[3], e = J[4], f = J[5], g = J[6], h = J[7],
-           i = J[8];
-    double c0 = e * i - f * h, c1 = f * g - d * i, c2 = d * h - e * g;
-    double det = a * c0 + b * c1 + cc * c2;
-    double r = 1.0 / det;
-    Ji[0] = c0 * r; Ji[1] = (cc * h - b * i) * r; Ji[2] = (b * f - cc * e) * r;
-    Ji[3] = c1 * r; Ji[4] = (a * i - cc * g) * r; Ji[5] = (cc * d - a * f) * r;
-    Ji[6] = c2 * r; Ji[7] = (b * g - a * h) * r; Ji[8] = (a * e - b * d) * r;
-    return det;
-  }
-}
-
 template <int D, bool F32>
 __global__ __launch_bounds__(128) void cell_q1_kernel(const CellArgs A) {
   constexpr int NL = 1 << D;

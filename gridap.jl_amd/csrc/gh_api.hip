@@ -42,6 +42,7 @@ void invalidate_space(gh::Ctx &c) {
   c.h_cell_dofs.clear();
   c.h_cell_dofs.shrink_to_fit();
   c.have_kappa = c.have_fsrc = c.have_dvals = false;
+  c.have_state = false;
   c.nq = 0;
 }
 
@@ -86,6 +87,13 @@ int gh_create(gh_ctx **out, int device) {
       hipStreamCreateWithFlags(&ctx->c.stream, hipStreamNonBlocking) != hipSuccess) {
     delete ctx;
     return GH_ERR_HIP;
+  }
+  try {
+    gh::scratch_init(device);  // once per device: the allocation-free solve's array
+  } catch (const gh::Error &e) {
+    (void)hipStreamDestroy(ctx->c.stream);
+    delete ctx;
+    return e.code;
   }
   *out = ctx;
   return GH_OK;
@@ -217,6 +225,7 @@ int gh_space_upload(gh_ctx *ctx, int reffe_kind, int order, int ncomp, const int
     c.have_space = true;
     invalidate_pattern(c);
     c.have_dvals = false;
+    c.have_state = false;
   });
 }
 
@@ -289,7 +298,8 @@ int gh_coef_upload(gh_ctx *ctx, const double *kappa, const double *fsrc,
 
 int gh_assemble(gh_ctx *ctx, int form_kind, int precision, int path) {
   return guarded(ctx, [&](gh::Ctx &c) {
-    GH_REQUIRE(form_kind == GH_FORM_DIFFUSION || form_kind == GH_FORM_ELASTICITY,
+    GH_REQUIRE(form_kind == GH_FORM_DIFFUSION || form_kind == GH_FORM_ELASTICITY ||
+                   form_kind == GH_FORM_PLAPLACIAN,
                GH_ERR_UNSUPPORTED, "unknown form kind");
     GH_REQUIRE(precision == GH_PREC_FP64 || precision == GH_PREC_FP32_ELEM, GH_ERR_ARG,
                "unknown precision mode");
@@ -297,7 +307,11 @@ int gh_assemble(gh_ctx *ctx, int form_kind, int precision, int path) {
                GH_ERR_ARG, "unknown numeric path");
     GH_REQUIRE(c.have_pattern, GH_ERR_STATE, "no pattern: call gh_symbolic first");
     GH_REQUIRE(c.nq > 0, GH_ERR_STATE, "no reference tabulation uploaded");
-    if (form_kind == GH_FORM_ELASTICITY) {
+    if (form_kind == GH_FORM_PLAPLACIAN) {
+      GH_REQUIRE(path != GH_PATH_STRUCTURED, GH_ERR_UNSUPPORTED,
+                 "the lattice path assembles linear diffusion only");
+      gh::assemble_plaplacian(c, precision);
+    } else if (form_kind == GH_FORM_ELASTICITY) {
       GH_REQUIRE(path != GH_PATH_STRUCTURED, GH_ERR_UNSUPPORTED,
                  "the lattice path assembles scalar diffusion only");
       gh::assemble_elasticity(c, precision);
@@ -317,6 +331,32 @@ int gh_elasticity_params(gh_ctx *ctx, double nu, const double *body_force) {
     c.nu = nu;
     for (int d = 0; d < 3; ++d) c.body[d] = body_force && d < c.dim ? body_force[d] : 0.0;
     c.have_values = false;
+  });
+}
+
+int gh_fefunction_upload(gh_ctx *ctx, const double *free_vals, const double *dirichlet_vals) {
+  return guarded(ctx, [&](gh::Ctx &c) {
+    GH_REQUIRE(c.have_space, GH_ERR_STATE, "upload the FE space first");
+    GH_REQUIRE(free_vals || c.nfree == 0, GH_ERR_ARG, "NULL free values");
+    c.ustate.resize(std::max<int64_t>(c.nfree, 1));
+    if (c.nfree)
+      GH_CHECK_HIP(hipMemcpyAsync(c.ustate.get(), free_vals, sizeof(double) * c.nfree,
+                                  hipMemcpyHostToDevice, c.stream));
+    c.have_dvals = dirichlet_vals != nullptr && c.ndiri > 0;
+    if (c.have_dvals) {
+      c.dvals.resize(c.ndiri);
+      GH_CHECK_HIP(hipMemcpyAsync(c.dvals.get(), dirichlet_vals, sizeof(double) * c.ndiri,
+                                  hipMemcpyHostToDevice, c.stream));
+    }
+    GH_CHECK_HIP(hipStreamSynchronize(c.stream));
+    c.have_state = true;
+  });
+}
+
+int gh_plaplacian_params(gh_ctx *ctx, double p) {
+  return guarded(ctx, [&](gh::Ctx &c) {
+    GH_REQUIRE(std::isfinite(p) && p >= 2.0, GH_ERR_ARG, "p must be a finite number >= 2");
+    c.plap_p = p;
   });
 }
 
@@ -435,6 +475,20 @@ int gh_pcg(gh_ctx *ctx, int precond, const double *b, double *x, double rtol, in
       throw gh::Error(GH_ERR_NOCONV, "PCG did not reach the requested tolerance");
   });
   return st;
+}
+
+int gh_pcg_noalloc(gh_ctx *ctx, int precond, const double *b, double *x, double rtol,
+                   int maxit, gh_solve_stats *stats) {
+  return guarded(ctx, [&](gh::Ctx &c) {
+    GH_REQUIRE(c.have_values, GH_ERR_STATE, "no assembled matrix");
+    GH_REQUIRE(x, GH_ERR_ARG, "NULL solution vector");
+    GH_REQUIRE(rtol > 0 && maxit >= 0, GH_ERR_ARG, "bad tolerance or iteration cap");
+    gh_solve_stats local{};
+    gh::pcg_noalloc(c, precond, b, x, rtol, maxit, &local);
+    if (stats) *stats = local;
+    if (!local.converged)
+      throw gh::Error(GH_ERR_NOCONV, "PCG did not reach the requested tolerance");
+  });
 }
 
 int gh_mlp_nparams(int64_t *n) {

@@ -66,6 +66,27 @@ struct DBuf {
 constexpr int kHexColLd = 84;
 __host__ __device__ inline int hex_row_slot(int li) { return li + li / 27; }
 
+// det J and J⁻¹ of a D x D matrix (row-major), FP64.
+template <int D>
+__host__ __device__ inline double det_inv(const double *J, double *Ji) {
+  if (D == 2) {
+    double det = J[0] * J[3] - J[1] * J[2];
+    double r = 1.0 / det;
+    Ji[0] = J[3] * r; Ji[1] = -J[1] * r; Ji[2] = -J[2] * r; Ji[3] = J[0] * r;
+    return det;
+  } else {
+    double a = J[0], b = J[1], cc = J[2], d = J[3], e = J[4], f = J[5], g = J[6], h = J[7],
+           i = J[8];
+    double c0 = e * i - f * h, c1 = f * g - d * i, c2 = d * h - e * g;
+    double det = a * c0 + b * c1 + cc * c2;
+    double r = 1.0 / det;
+    Ji[0] = c0 * r; Ji[1] = (cc * h - b * i) * r; Ji[2] = (b * f - cc * e) * r;
+    Ji[3] = c1 * r; Ji[4] = (a * i - cc * g) * r; Ji[5] = (cc * d - a * f) * r;
+    Ji[6] = c2 * r; Ji[7] = (b * g - a * h) * r; Ji[8] = (a * e - b * d) * r;
+    return det;
+  }
+}
+
 struct Ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -139,6 +160,12 @@ struct Ctx {
   std::vector<int64_t> color_ptr;   // ncolors+1
   DBuf<int32_t> color_cells;        // cells grouped by colour, ascending inside a colour
 
+  // ---- nonlinear forms: the FE function uh (free values; Dirichlet values in dvals) ----
+  DBuf<double> ustate;
+  bool have_state = false;
+  double plap_p = 2.0;              // p of GH_FORM_PLAPLACIAN
+  DBuf<double> nl_tab;              // packed tabulation of the nonlinear kernels
+
   // ---- coefficients / values ----
   DBuf<double> kappa, fsrc, dvals;
   bool have_kappa = false, have_fsrc = false, have_dvals = false;
@@ -195,12 +222,18 @@ bool simplex_eligible(const Ctx &c);
 void elemlist_symbolic(Ctx &c);         // dof -> cell adjacency, Dirichlet masks
 void elemlist_sum(Ctx &c, bool f32);     // nzval from the element entries
 void assemble_simplex(Ctx &c, int precision);
+// nonlinear.hip
+void assemble_plaplacian(Ctx &c, int precision);  // residual -> b, Jacobian -> nzval
 // hexq2.hip
 bool hexq2_eligible(const Ctx &c);
 void assemble_elasticity(Ctx &c, int precision);
 // solve.hip
 void spmv_device(Ctx &c, const double *x, double *y);
 void pcg_device(Ctx &c, int precond, double rtol, int maxit, gh_solve_stats *st);
+// the allocation-free solve: vectors in the library's fixed device array
+void scratch_init(int device);
+void pcg_noalloc(Ctx &c, int precond, const double *b, double *x, double rtol, int maxit,
+                 gh_solve_stats *st);
 // fsai.hip
 constexpr int kFsaiEllSlots = 16;  // slots per row of the solver's G and Gᵀ
 constexpr int kMlpIn = 10, kMlpInPad = 16, kMlpHidden = 64, kMlpOut = 36, kMlpOutPad = 40;

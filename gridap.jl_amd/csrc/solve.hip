@@ -23,8 +23,10 @@
 // spmv_dir publishes r·z for the next iteration's β.  The host sums the r·r
 // partials itself at the convergence checks.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdlib>
+#include <mutex>
 
 #include "gh_internal.h"
 
@@ -562,33 +564,47 @@ void spmv_device(Ctx &c, const double *x, double *y) {
 
 namespace {
 // Σ of np partials on the host (convergence checks only; fixed order).
-double host_sum(const std::vector<double> &v, int np) {
+double host_sum(const double *v, int np) {
   double a = 0.0;
   for (int k = 0; k < np; ++k) a += v[k];
   return a;
 }
 }  // namespace
 
-void pcg_device(Ctx &c, int precond, double rtol, int maxit, gh_solve_stats *st) {
+namespace {
+
+// Device vectors of one solve: the solution x (initial guess on entry) and b, the CG
+// vectors, the diagonal inverse, partial sums (4 x kMaxParts) and scalars (8).
+struct PcgBufs {
+  double *x, *b, *r, *r2, *z, *p, *p2, *q, *y, *dinv, *part, *scal;
+};
+
+__global__ void clear_stop_kernel(double *scal) {
+  if (threadIdx.x < 2) scal[kStop + threadIdx.x] = 0.0;
+}
+
+// The solve on the given buffers.  box: the stencil form may be used (needs c.a27
+// and, for FSAI, c.g14); timed: HIP events around the solve (device_ms), else 0.
+void pcg_run(Ctx &c, const PcgBufs &V, int precond, double rtol, int maxit, bool allow_box,
+             bool timed, gh_solve_stats *st) {
   const int64_t n = c.nfree;
   hipStream_t s = c.stream;
   const size_t nn = (size_t)std::max<int64_t>(n, 1);
-  for (auto *bf : {&c.s_r, &c.s_z, &c.s_p, &c.s_p2, &c.s_q}) bf->resize(nn);
-  c.s_part.resize(4 * kMaxParts);
-  c.s_scal.resize(8);
   // the solve is timed from here: per-matrix preparation (D^-1, the stencil arrays)
   // included
-  hipEvent_t e0, e1;
-  GH_CHECK_HIP(hipEventCreate(&e0));
-  GH_CHECK_HIP(hipEventCreate(&e1));
-  GH_CHECK_HIP(hipEventRecord(e0, s));
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (timed) {
+    GH_CHECK_HIP(hipEventCreate(&e0));
+    GH_CHECK_HIP(hipEventCreate(&e1));
+    GH_CHECK_HIP(hipEventRecord(e0, s));
+  }
   double *dinv = nullptr;
   const int factored = precond == GH_PRECOND_FSAI;
   // box lattice: stencil kernels (A as 27 offset arrays, G as 14), one thread per row
   // GRIDAP_HIP_STENCIL_PCG=0 runs the general (CSR) kernels on a box lattice too, so
   // the two forms can be compared on the same system
   const char *stencil_env = std::getenv("GRIDAP_HIP_STENCIL_PCG");
-  bool box = c.box && n > 0 && n < (int64_t(1) << 31) &&
+  bool box = allow_box && c.box && n > 0 && n < (int64_t(1) << 31) &&
              (!factored || c.g14.get() != nullptr) && !(stencil_env && stencil_env[0] == '0');
   const Box B{n, c.box_m[0], c.box_m[1], c.box_m[2]};
   if (box) {
@@ -606,56 +622,52 @@ void pcg_device(Ctx &c, int precond, double rtol, int maxit, gh_solve_stats *st)
     box = hbad == 0;
   }
   if (precond == GH_PRECOND_JACOBI) {
-    c.s_dinv.resize(nn);
     if (box)
       hipLaunchKernelGGL(diag_inv_box_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
-                         n, c.a27.get(), c.s_dinv.get());
+                         n, c.a27.get(), V.dinv);
     else
       hipLaunchKernelGGL(diag_inv_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n,
-                         c.colptr.get(), c.rowval.get(), c.nzval.get(), c.s_dinv.get());
-    dinv = c.s_dinv.get();
+                         c.colptr.get(), c.rowval.get(), c.nzval.get(), V.dinv);
+    dinv = V.dinv;
   }
-  if (factored) {
+  if (factored)
     GH_REQUIRE(c.fsai_built, GH_ERR_STATE, "no FSAI factor for this matrix: call gh_fsai_build");
-    c.s_y.resize(nn);
-    c.s_r2.resize(nn);
-  }
-  double *scal = c.s_scal.get();
-  double *part_pq = c.s_part.get(), *part_rz = part_pq + kMaxParts;
+  double *scal = V.scal;
+  double *part_pq = V.part, *part_rz = part_pq + kMaxParts;
   double *part_rr = part_rz + kMaxParts, *part_bb = part_rr + kMaxParts;
-  double *rbuf[2] = {c.s_r.get(), factored ? c.s_r2.get() : c.s_r.get()};
-  double *pbuf[2] = {c.s_p.get(), c.s_p2.get()};
+  double *rbuf[2] = {V.r, factored ? V.r2 : V.r};
+  double *pbuf[2] = {V.p, V.p2};
   const unsigned g_vec = grid_rows(n, kBlock);
   const unsigned g_spmv = box ? g_vec : grid_rows(n, kBlock / kSpmvLanes);
   const unsigned g_apply = box ? g_vec : grid_rows(n, kBlock / kApplyLanes);
   // partial counts of each quantity, by producer
   const int np_rz = factored ? (int)g_apply : (int)g_vec;
-  std::vector<double> hrr(kMaxParts), hbb(kMaxParts);
+  double hrr[kPcgGrid], hbb[kPcgGrid];  // g_vec <= kPcgGrid partials
 
   double bnorm = 1.0, relres = INFINITY;
   int it = 0, restarts = 0;
   bool converged = false;
   while (true) {
     // (re)start from the true residual r = b - A x
-    spmv_device(c, c.s_x.get(), c.s_q.get());
-    hipLaunchKernelGGL(init_kernel, dim3(g_vec), dim3(kBlock), 0, s, n, c.s_b.get(), c.s_q.get(),
-                       dinv, factored, rbuf[0], c.s_z.get(), part_rz, part_rr, part_bb);
-    GH_CHECK_HIP(hipMemsetAsync(scal + kStop, 0, 2 * sizeof(double), s));
+    spmv_device(c, V.x, V.q);
+    hipLaunchKernelGGL(init_kernel, dim3(g_vec), dim3(kBlock), 0, s, n, V.b, V.q,
+                       dinv, factored, rbuf[0], V.z, part_rz, part_rr, part_bb);
+    hipLaunchKernelGGL(clear_stop_kernel, dim3(1), dim3(64), 0, s, scal);
     if (factored && box) {
       hipLaunchKernelGGL(lower_box_kernel, dim3(g_apply), dim3(kBlock), 0, s, B, c.g14.get(),
-                         rbuf[0], c.s_y.get());
+                         rbuf[0], V.y);
       hipLaunchKernelGGL(upper_rz_box_kernel, dim3(g_apply), dim3(kBlock), 0, s, B,
-                         c.g14.get(), c.s_y.get(), rbuf[0], c.s_z.get(), scal, part_rz);
+                         c.g14.get(), V.y, rbuf[0], V.z, scal, part_rz);
     } else if (factored) {
       hipLaunchKernelGGL(lower_kernel, dim3(g_apply), dim3(kBlock), 0, s, n, c.gle.get(),
-                         c.gli.get(), rbuf[0], c.s_y.get());
+                         c.gli.get(), rbuf[0], V.y);
       hipLaunchKernelGGL(upper_rz_kernel, dim3(g_apply), dim3(kBlock), 0, s, n, c.gue.get(),
-                         c.gui.get(), c.s_y.get(), rbuf[0], c.s_z.get(), scal, part_rz);
+                         c.gui.get(), V.y, rbuf[0], V.z, scal, part_rz);
     }
     GH_CHECK_HIP(hipGetLastError());
-    GH_CHECK_HIP(hipMemcpyAsync(hrr.data(), part_rr, sizeof(double) * g_vec,
+    GH_CHECK_HIP(hipMemcpyAsync(hrr, part_rr, sizeof(double) * g_vec,
                                 hipMemcpyDeviceToHost, s));
-    GH_CHECK_HIP(hipMemcpyAsync(hbb.data(), part_bb, sizeof(double) * g_vec,
+    GH_CHECK_HIP(hipMemcpyAsync(hbb, part_bb, sizeof(double) * g_vec,
                                 hipMemcpyDeviceToHost, s));
     GH_CHECK_HIP(hipStreamSynchronize(s));
     const double bb = host_sum(hbb, (int)g_vec);
@@ -681,30 +693,30 @@ void pcg_device(Ctx &c, int precond, double rtol, int maxit, gh_solve_stats *st)
         double *pold = pbuf[par], *pnew = pbuf[par ^ 1];
         if (box)
           hipLaunchKernelGGL(spmv_dir_box_kernel, dim3(g_spmv), dim3(kBlock), 0, s, B,
-                             c.a27.get(), c.s_z.get(), pold, pnew, c.s_q.get(), scal, par,
+                             c.a27.get(), V.z, pold, pnew, V.q, scal, par,
                              k == 0 ? 1 : 0, part_rz, part_rr, np_rz, thr, itk, part_pq);
         else
           hipLaunchKernelGGL(spmv_dir_kernel, dim3(g_spmv), dim3(kBlock), 0, s, n,
-                             c.colptr.get(), c.rowval.get(), c.nzval.get(), c.s_z.get(), pold,
-                             pnew, c.s_q.get(), scal, par, k == 0 ? 1 : 0, part_rz, part_rr,
+                             c.colptr.get(), c.rowval.get(), c.nzval.get(), V.z, pold,
+                             pnew, V.q, scal, par, k == 0 ? 1 : 0, part_rz, part_rr,
                              np_rz, thr, itk, part_pq);
         if (factored && box) {
           double *rold = rbuf[par], *rnew = rbuf[par ^ 1];
           hipLaunchKernelGGL(update_lower_box_kernel, dim3(g_apply), dim3(kBlock), 0, s, B, par,
-                             c.g14.get(), c.s_x.get(), rold, rnew, pnew, c.s_q.get(),
-                             c.s_y.get(), scal, part_pq, (int)g_spmv, part_rr);
+                             c.g14.get(), V.x, rold, rnew, pnew, V.q,
+                             V.y, scal, part_pq, (int)g_spmv, part_rr);
           hipLaunchKernelGGL(upper_rz_box_kernel, dim3(g_apply), dim3(kBlock), 0, s, B,
-                             c.g14.get(), c.s_y.get(), rnew, c.s_z.get(), scal, part_rz);
+                             c.g14.get(), V.y, rnew, V.z, scal, part_rz);
         } else if (factored) {
           double *rold = rbuf[par], *rnew = rbuf[par ^ 1];
           hipLaunchKernelGGL(update_lower_kernel, dim3(g_apply), dim3(kBlock), 0, s, n, par,
-                             c.gle.get(), c.gli.get(), c.s_x.get(), rold, rnew, pnew, c.s_q.get(),
-                             c.s_y.get(), scal, part_pq, (int)g_spmv, part_rr);
+                             c.gle.get(), c.gli.get(), V.x, rold, rnew, pnew, V.q,
+                             V.y, scal, part_pq, (int)g_spmv, part_rr);
           hipLaunchKernelGGL(upper_rz_kernel, dim3(g_apply), dim3(kBlock), 0, s, n, c.gue.get(),
-                             c.gui.get(), c.s_y.get(), rnew, c.s_z.get(), scal, part_rz);
+                             c.gui.get(), V.y, rnew, V.z, scal, part_rz);
         } else {
           hipLaunchKernelGGL(update_kernel, dim3(g_vec), dim3(kBlock), 0, s, n, par, dinv,
-                             c.s_x.get(), rbuf[0], c.s_z.get(), pnew, c.s_q.get(), scal, part_pq,
+                             V.x, rbuf[0], V.z, pnew, V.q, scal, part_pq,
                              (int)g_spmv, part_rz, part_rr);
         }
       }
@@ -722,16 +734,99 @@ void pcg_device(Ctx &c, int precond, double rtol, int maxit, gh_solve_stats *st)
     // the true residual of the next start decides convergence (and, if the stopped
     // r·r was not finite, the restart count ends the solve)
   }
-  GH_CHECK_HIP(hipEventRecord(e1, s));
-  GH_CHECK_HIP(hipEventSynchronize(e1));
   float ms = 0.f;
-  GH_CHECK_HIP(hipEventElapsedTime(&ms, e0, e1));
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
+  if (timed) {
+    GH_CHECK_HIP(hipEventRecord(e1, s));
+    GH_CHECK_HIP(hipEventSynchronize(e1));
+    GH_CHECK_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+  } else {
+    GH_CHECK_HIP(hipStreamSynchronize(s));
+  }
   st->iterations = it;
   st->converged = converged ? 1 : 0;
   st->rel_residual = relres;
   st->device_ms = ms;
+}
+
+}  // namespace
+
+void pcg_device(Ctx &c, int precond, double rtol, int maxit, gh_solve_stats *st) {
+  const size_t nn = (size_t)std::max<int64_t>(c.nfree, 1);
+  for (auto *bf : {&c.s_r, &c.s_z, &c.s_p, &c.s_p2, &c.s_q}) bf->resize(nn);
+  c.s_part.resize(4 * kMaxParts);
+  c.s_scal.resize(8);
+  if (precond == GH_PRECOND_JACOBI) c.s_dinv.resize(nn);
+  if (precond == GH_PRECOND_FSAI) {
+    c.s_y.resize(nn);
+    c.s_r2.resize(nn);
+  }
+  const PcgBufs V{c.s_x.get(), c.s_b.get(), c.s_r.get(), c.s_r2.get(), c.s_z.get(),
+                  c.s_p.get(), c.s_p2.get(), c.s_q.get(), c.s_y.get(), c.s_dinv.get(),
+                  c.s_part.get(), c.s_scal.get()};
+  pcg_run(c, V, precond, rtol, maxit, true, true, st);
+}
+
+// ---- the solve that allocates nothing (gh_pcg_noalloc) ------------------------------
+// Its vectors live in one fixed device array, `scratch`, which every GPU the library
+// runs on holds until the process ends: x, b, r, z, p, p', q, D^-1 of at most
+// GH_NOALLOC_MAX_N entries each, then the partial sums and the scalars.  Every call on
+// one GPU shares that GPU's copy (calls that overlap on one GPU race), and nothing in
+// it is read before the call writes it.
+namespace {
+constexpr size_t kNoAllocVec = (size_t)GH_NOALLOC_MAX_N;
+constexpr size_t kScratchDoubles = 8 * kNoAllocVec + 4 * (size_t)kMaxParts + 8;
+__device__ __align__(256) double scratch[kScratchDoubles];
+
+constexpr int kMaxDevices = 64;
+double *g_scratch[kMaxDevices] = {};  // device address of `scratch` on each GPU
+std::mutex g_scratch_mu;
+}  // namespace
+
+// Look up this GPU's address of `scratch` once (the calling thread has the device
+// current).  Host code uses that address, never the array's name.
+void scratch_init(int device) {
+  GH_REQUIRE(device >= 0 && device < kMaxDevices, GH_ERR_ARG, "device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch[device]) return;
+  void *p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || p == nullptr)
+    throw Error(GH_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+  g_scratch[device] = static_cast<double *>(p);
+}
+
+void pcg_noalloc(Ctx &c, int precond, const double *b, double *x, double rtol, int maxit,
+                 gh_solve_stats *st) {
+  GH_REQUIRE(precond == GH_PRECOND_NONE || precond == GH_PRECOND_JACOBI, GH_ERR_UNSUPPORTED,
+             "the allocation-free solve provides Jacobi or no preconditioner");
+  GH_REQUIRE(c.nfree <= GH_NOALLOC_MAX_N, GH_ERR_ARG,
+             "num_free_dofs exceeds GH_NOALLOC_MAX_N, the allocation-free solve's limit");
+  double *base = nullptr;
+  {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    base = c.device >= 0 && c.device < kMaxDevices ? g_scratch[c.device] : nullptr;
+  }
+  GH_REQUIRE(base, GH_ERR_STATE, "scratch address not set up for this device");
+  const size_t n = (size_t)c.nfree;
+  const size_t L = kNoAllocVec;
+  const PcgBufs V{base, base + L, base + 2 * L, nullptr, base + 3 * L, base + 4 * L,
+                  base + 5 * L, base + 6 * L, nullptr, base + 7 * L, base + 8 * L,
+                  base + 8 * L + 4 * (size_t)kMaxParts};
+  const auto t0 = std::chrono::steady_clock::now();
+  GH_CHECK_HIP(hipMemcpyAsync(V.x, x, sizeof(double) * n, hipMemcpyHostToDevice, c.stream));
+  if (b)
+    GH_CHECK_HIP(hipMemcpyAsync(V.b, b, sizeof(double) * n, hipMemcpyHostToDevice, c.stream));
+  else
+    GH_CHECK_HIP(hipMemcpyAsync(V.b, c.b.get(), sizeof(double) * n, hipMemcpyDeviceToDevice,
+                                c.stream));
+  pcg_run(c, V, precond, rtol, maxit, false, false, st);
+  GH_CHECK_HIP(hipMemcpyAsync(x, V.x, sizeof(double) * n, hipMemcpyDeviceToHost, c.stream));
+  GH_CHECK_HIP(hipStreamSynchronize(c.stream));
+  st->device_ms =
+      std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  st->precond_used = precond;
 }
 
 }  // namespace gh

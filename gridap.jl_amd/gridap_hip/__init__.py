@@ -21,13 +21,15 @@ from .assemblers import (AffineFEOperator, LinearFESolver, PCGSolver, SparseMatr
                          assemble_matrix_and_vector_inplace, assemble_matrix_inplace,
                          assemble_vector, assemble_vector_inplace, get_matrix, get_vector,
                          solve)
-from .celldata import (CellField, DomainContribution, IsotropicStress, Measure, dot,
-                       get_fe_basis, get_trial_fe_basis, grad, gradient, inner, integral,
-                       integrate, symmetric_gradient, ε)
+from .celldata import (CellField, DomainContribution, IsotropicStress, Measure, PLaplacian,
+                       compose, dot, get_fe_basis, get_trial_fe_basis, grad, gradient, inner,
+                       integral, integrate, symmetric_gradient, ε)
 from .fespaces import FEFunction, FESpace, TestFESpace, TrialFESpace
 from .geometry import (CartesianDiscreteModel, FaceLabeling, Triangulation, UnstructuredDiscreteModel,
                        get_triangulation, simplexify)
 from .reference_fes import Float64, Quadrature, ReferenceFE, VectorValue, lagrangian
+from .nonlinear import (FEOperator, FESolver, NewtonRaphsonSolver, jacobian, residual,
+                        residual_and_jacobian, solve_inplace)
 
 __all__ = [
     "AffineFEOperator", "CartesianDiscreteModel", "CellField", "DomainContribution",
@@ -40,4 +42,6 @@ __all__ = [
     "get_fe_basis", "get_matrix", "get_triangulation", "get_trial_fe_basis", "get_vector",
     "grad", "gradient", "inner", "integral", "integrate", "lagrangian", "solve",
     "IsotropicStress", "symmetric_gradient", "ε",
+    "FEOperator", "FESolver", "NewtonRaphsonSolver", "PLaplacian", "compose", "jacobian",
+    "residual", "residual_and_jacobian", "solve_inplace",
 ]

@@ -27,12 +27,14 @@ GH_FORM_DIFFUSION = 1
 GH_PREC_FP64 = 0
 GH_PREC_FP32_ELEM = 1
 GH_FORM_ELASTICITY = 2
+GH_FORM_PLAPLACIAN = 3
 GH_PATH_AUTO = 0
 GH_PATH_GENERAL = 1
 GH_PATH_STRUCTURED = 2
 GH_PRECOND_NONE = 0
 GH_PRECOND_JACOBI = 1
 GH_PRECOND_FSAI = 2
+GH_NOALLOC_MAX_N = 262144
 
 # Every symbol the header declares, with (restype, argtypes).
 _i32, _i64, _dbl, _vp = C.c_int32, C.c_int64, C.c_double, C.c_void_p
@@ -66,10 +68,13 @@ SIGNATURES = {
     "gh_assemble": (_i32, [_vp, _i32, _i32, _i32]),
     "gh_values_get": (_i32, [_vp, _vp, _vp]),
     "gh_values_set": (_i32, [_vp, _vp, _vp]),
+    "gh_fefunction_upload": (_i32, [_vp, _vp, _vp]),
+    "gh_plaplacian_params": (_i32, [_vp, _dbl]),
     "gh_pattern_info_get": (_i32, [_vp, C.POINTER(PatternInfo)]),
     "gh_numeric": (_i32, [_vp, _i32, _vp, _vp, _i32, _vp, _vp, _vp]),
     "gh_spmv": (_i32, [_vp, _vp, _vp]),
     "gh_pcg": (_i32, [_vp, _i32, _vp, _vp, _dbl, _i32, C.POINTER(SolveStats)]),
+    "gh_pcg_noalloc": (_i32, [_vp, _i32, _vp, _vp, _dbl, _i32, C.POINTER(SolveStats)]),
     "gh_mlp_nparams": (_i32, [_pi64]),
     "gh_mlp_init": (_i32, [_vp, C.c_uint64]),
     "gh_mlp_set": (_i32, [_vp, _vp, _i64]),
@@ -222,6 +227,18 @@ class Context:
         self.check(lib().gh_values_get(self.h, ptr(nzo), ptr(bo)))
         return (nz if nz is not None else nzo[:self.nnz]), (b if b is not None else bo[:self.nfree])
 
+    def fefunction_upload(self, free_vals, dirichlet_vals=None):
+        """The FE function uh of a nonlinear form (free and Dirichlet values)."""
+        u = np.ascontiguousarray(free_vals, dtype=np.float64)
+        if u.shape != (self.nfree,):
+            raise ValueError("free values must have num_free_dofs entries")
+        g = None if dirichlet_vals is None else np.ascontiguousarray(dirichlet_vals,
+                                                                     dtype=np.float64)
+        self.check(lib().gh_fefunction_upload(self.h, ptr(u), ptr(g)))
+
+    def plaplacian_params(self, p):
+        self.check(lib().gh_plaplacian_params(self.h, float(p)))
+
     def values_set(self, nz=None, b=None):
         """Load nzval (on this context's pattern) and/or b into the device."""
         for name, a, n in (("nz", nz, self.nnz), ("b", b, self.nfree)):
@@ -272,6 +289,24 @@ class Context:
         st = SolveStats()
         code = lib().gh_pcg(self.h, precond, ptr(bb), ptr(x), float(rtol), int(maxit),
                             C.byref(st))
+        if code != GH_OK and (code != GH_ERR_NOCONV or raise_on_noconv):
+            self.check(code)
+        return x, st
+
+    def pcg_noalloc(self, b=None, x0=None, precond=GH_PRECOND_JACOBI, rtol=1e-8, maxit=10000,
+                    raise_on_noconv=True):
+        """gh_pcg_noalloc: the same solve with no device allocation (vectors in the
+        library's fixed device array; nfree <= GH_NOALLOC_MAX_N)."""
+        bb = None if b is None else np.ascontiguousarray(b, dtype=np.float64)
+        n = self.nfree
+        if bb is not None and bb.shape != (n,):
+            raise ValueError("b must have num_free_dofs entries")
+        x = np.zeros(n) if x0 is None else np.ascontiguousarray(x0, dtype=np.float64).copy()
+        if x.shape != (n,):
+            raise ValueError("x0 must have num_free_dofs entries")
+        st = SolveStats()
+        code = lib().gh_pcg_noalloc(self.h, precond, ptr(bb), ptr(x), float(rtol), int(maxit),
+                                    C.byref(st))
         if code != GH_OK and (code != GH_ERR_NOCONV or raise_on_noconv):
             self.check(code)
         return x, st

@@ -49,6 +49,8 @@ def _coef_product(a, b):
         return a
     va = a.values if isinstance(a, CellField) else a
     vb = b.values if isinstance(b, CellField) else b
+    if isinstance(va, numbers.Number) and isinstance(vb, numbers.Number):
+        return CellField(float(va) * float(vb))
     return CellField(np.asarray(va, dtype=np.float64) * np.asarray(vb, dtype=np.float64))
 
 
@@ -61,9 +63,22 @@ class _Expr:
     __rmul__ = __mul__
 
     def __add__(self, other):
-        return _Sum([self, other])
+        return _Sum(_terms(self) + _terms(other))
 
     __radd__ = __add__
+
+    def __neg__(self):
+        return _Sum([_scale(-1.0, t) for t in _terms(self)])
+
+    def __sub__(self, other):
+        return self + (-other)
+
+    def __rsub__(self, other):
+        return other + (-self)
+
+
+def _terms(e):
+    return list(e.terms) if isinstance(e, _Sum) else [e]
 
 
 class FEBasis(_Expr):
@@ -141,6 +156,69 @@ class _Sum(_Expr):
         self.terms = terms
 
 
+class _GradState(_Expr):
+    """∇uh of an FE function (the state of a nonlinear form)."""
+
+    def __init__(self, uh):
+        self.uh = uh
+
+
+class _Flux(_Expr):
+    """flux(∇uh) = |∇uh|^(p-2) ∇uh (PLaplacianTests.jl:15)."""
+
+    def __init__(self, p, state):
+        self.p, self.state = p, state
+
+
+class _DFlux(_Expr):
+    """dflux(∇du, ∇uh) = (p-2)|∇uh|^(p-4)(∇uh⋅∇du)∇uh + |∇uh|^(p-2)∇du
+    (PLaplacianTests.jl:16)."""
+
+    def __init__(self, p, state):
+        self.p, self.state = p, state
+
+
+class PLaplacianResidualTerm(_Expr):
+    """coef * ∇v⋅flux(∇uh)."""
+
+    def __init__(self, p, state, coef=None):
+        self.p, self.state, self.coef = p, state, coef
+
+
+class PLaplacianJacobianTerm(_Expr):
+    """coef * ∇v⋅dflux(∇du, ∇uh)."""
+
+    def __init__(self, p, state, coef=None):
+        self.p, self.state, self.coef = p, state, coef
+
+
+class PLaplacian:
+    """The constitutive law of test/GridapTests/PLaplacianTests.jl:14-17:
+    law = PLaplacian(p); law.flux(∇(u)) and law.dflux(∇(du), ∇(u)) stand for Gridap's
+    flux∘∇(u) and dflux∘(∇(du),∇(u)) inside res(u,v) and jac(u,du,v)."""
+
+    def __init__(self, p):
+        self.p = float(p)
+        if not self.p >= 2.0:
+            raise ValueError("p must be >= 2")
+
+    def flux(self, grad_u):
+        if not isinstance(grad_u, _GradState):
+            raise NotImplementedError("flux applies to ∇(uh) of an FE function")
+        return _Flux(self.p, grad_u.uh)
+
+    def dflux(self, grad_du, grad_u):
+        if not (isinstance(grad_du, _Grad) and grad_du.basis.trial and
+                isinstance(grad_u, _GradState)):
+            raise NotImplementedError("dflux applies to (∇(du) of the trial basis, ∇(uh))")
+        return _DFlux(self.p, grad_u.uh)
+
+
+def compose(f, *args):
+    """f∘args (Gridap's ∘ on CellFields)."""
+    return f(*args)
+
+
 def _scale(factor, expr):
     if isinstance(factor, _Expr) and not isinstance(expr, _Expr):
         factor, expr = expr, factor
@@ -150,6 +228,10 @@ def _scale(factor, expr):
         return SourceTerm(_as_coef(factor))
     if isinstance(expr, (DiffusionTerm, SourceTerm)):
         return type(expr)(_coef_product(expr.coef, _as_coef(factor)))
+    if isinstance(expr, (PLaplacianResidualTerm, PLaplacianJacobianTerm)):
+        return type(expr)(expr.p, expr.state, _coef_product(expr.coef, _as_coef(factor)))
+    if isinstance(expr, _Sum):
+        return _Sum([_scale(factor, t) for t in expr.terms])
     raise NotImplementedError("unsupported weak-form product")
 
 
@@ -167,8 +249,11 @@ def _as_coef(x):
 
 
 def gradient(x):
+    from .fespaces import FEFunction
+    if isinstance(x, FEFunction):
+        return _GradState(x)
     if not isinstance(x, FEBasis):
-        raise NotImplementedError("gradient of an FE basis only")
+        raise NotImplementedError("gradient of an FE basis or an FE function only")
     return _Grad(x)
 
 
@@ -188,6 +273,10 @@ def inner(a, b):
     from .reference_fes import VectorValue
     if isinstance(a, _Grad) and isinstance(b, _Grad) and a.basis.trial != b.basis.trial:
         return DiffusionTerm()
+    if isinstance(a, _Grad) and not a.basis.trial and isinstance(b, _Flux):
+        return PLaplacianResidualTerm(b.p, b.state)
+    if isinstance(a, _Grad) and not a.basis.trial and isinstance(b, _DFlux):
+        return PLaplacianJacobianTerm(b.p, b.state)
     if isinstance(a, _SymGrad) and isinstance(b, _Stress) and not a.basis.trial and \
             b.eps.basis.trial:
         return ElasticityTerm(b.material.E, b.material.nu)
@@ -238,7 +327,8 @@ class DomainContribution:
 def integrate(expr, measure):
     terms = expr.terms if isinstance(expr, _Sum) else [expr]
     for t in terms:
-        if not isinstance(t, (DiffusionTerm, SourceTerm, ElasticityTerm, BodyForceTerm)):
+        if not isinstance(t, (DiffusionTerm, SourceTerm, ElasticityTerm, BodyForceTerm,
+                              PLaplacianResidualTerm, PLaplacianJacobianTerm)):
             raise NotImplementedError("unsupported integrand %r" % (t,))
     return DomainContribution([(t, measure) for t in terms])
 

@@ -44,6 +44,12 @@ extern "C" {
 #define GH_FORM_ELASTICITY 2 /* a(u,v)=∫ ε(v) : σ(ε(u)) dΩ, σ = λ tr(ε) I + 2 μ ε with λ, μ from a
                               * per-cell Young's modulus (the κ array) and ν; l(v)=∫ v⋅f dΩ with f
                               * constant (gh_elasticity_params); vector Q2 on Cartesian hexes */
+#define GH_FORM_PLAPLACIAN 3 /* residual and Jacobian at uh (gh_fefunction_upload) of
+                              * res(u,v) = ∫ κ ∇v⋅(|∇u|^(p-2)∇u) - f v dΩ, jac(u,du,v) = ∫ κ ∇v⋅
+                              * ((p-2)|∇u|^(p-4)(∇u⋅∇du)∇u + |∇u|^(p-2)∇du) dΩ, κ (NULL = 1) and
+                              * f (NULL = 0) per cell from gh_coef_upload, p >= 2
+                              * (gh_plaplacian_params); gh_assemble writes the residual to b and
+                              * the Jacobian to nzval; scalar Q1 on quads/hexes */
 
 /* ---- element precision ---- */
 #define GH_PREC_FP64 0      /* element arithmetic and accumulation in FP64           */
@@ -133,6 +139,16 @@ int gh_assemble(gh_ctx *ctx, int form_kind, int precision, int path);
  * (ε(v) ⊙ (σ∘ε(u)), src/TensorValues/ and test/GridapTests/IsotropicDamageTests.jl:13-17). */
 int gh_elasticity_params(gh_ctx *ctx, double nu, const double *body_force);
 
+/* The FE function uh at which a nonlinear form's residual and Jacobian are assembled:
+ * its free values[nfree] and Dirichlet values[ndiri] (NULL = 0), as
+ * get_free_dof_values(uh) / get_dirichlet_dof_values(uh) of a Gridap FEFunction.
+ * Replaces the uh argument of residual!/jacobian!(…, op, uh),
+ * src/FESpaces/FEOperatorsFromWeakForm.jl:57-95. */
+int gh_fefunction_upload(gh_ctx *ctx, const double *free_vals, const double *dirichlet_vals);
+
+/* Exponent p >= 2 of GH_FORM_PLAPLACIAN (test/GridapTests/PLaplacianTests.jl:15-17). */
+int gh_plaplacian_params(gh_ctx *ctx, double p);
+
 /* Copy the assembled nzval[nnz] and b[nfree] to the host (either may be NULL). */
 int gh_values_get(gh_ctx *ctx, double *nzval, double *b);
 
@@ -173,6 +189,19 @@ int gh_spmv(gh_ctx *ctx, const double *x, double *y);
  * Stands in for solve!(x, LUSolver(), op), src/Algebra/LinearSolvers.jl:202-274. */
 int gh_pcg(gh_ctx *ctx, int precond, const double *b, double *x, double rtol, int maxit,
            gh_solve_stats *stats);
+
+/* gh_pcg without any device allocation.  No counterpart in Gridap: for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Jacobi (GH_PRECOND_JACOBI) or no preconditioner,
+ * the general CSR kernels, nfree <= GH_NOALLOC_MAX_N.  Its vectors live in one fixed
+ * device array of the library (8 x GH_NOALLOC_MAX_N doubles plus partial sums, about
+ * 17 MB), which every GPU the library runs on holds until the process ends; all calls
+ * on one GPU share it, so calls that overlap on one GPU race.  Same results as gh_pcg
+ * with the same preconditioner on the CSR kernels (GRIDAP_HIP_STENCIL_PCG=0);
+ * stats->device_ms is the host wall time of the call. */
+#define GH_NOALLOC_MAX_N 262144
+int gh_pcg_noalloc(gh_ctx *ctx, int precond, const double *b, double *x, double rtol,
+                   int maxit, gh_solve_stats *stats);
 
 /* ---- learned FSAI preconditioner (3-D Cartesian mesh, scalar Q1) ----
  * No counterpart in Gridap (it ships LU only, src/Algebra/LinearSolvers.jl:249-303).

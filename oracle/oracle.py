@@ -415,3 +415,70 @@ def poisson_q1(domain, partition, kappa=None, fsrc=None, dirichlet_tags=("bounda
                 cell_dofs=cell_dofs, qx=qx, qw=qw, shp=shp, grad=grad, cellmats=mats,
                 cellvecs=vecs, dirichlet_vals=diri_vals, colptr=colptr, rowval=rowval,
                 nzval=nzval, b=b)
+
+
+# ---- nonlinear forms: residual and Jacobian at an FE function --------------------------
+
+def cell_values(cell_dofs, free_vals, dirichlet_vals):
+    """The cell-wise dof values of an FE function: free values at ids > 0, Dirichlet
+    values at ids < 0 (scatter_free_and_dirichlet_values,
+    src/FESpaces/FESpaceInterface.jl, with Gridap's signed dof ids)."""
+    cd = np.asarray(cell_dofs, dtype=np.int64)
+    fv = np.asarray(free_vals, dtype=np.float64)
+    dv = np.asarray(dirichlet_vals, dtype=np.float64) if dirichlet_vals is not None else None
+    out = np.zeros(cd.shape)
+    pos = cd > 0
+    out[pos] = fv[cd[pos] - 1]
+    if dv is not None and dv.size:
+        out[~pos] = dv[-cd[~pos] - 1]
+    return out
+
+
+def plaplacian_cells(D, cell_nodes, xyz, qw, shp, grad, u_cell, p, kappa=None, fsrc=None,
+                     cartesian_sizes=None):
+    """Cell residuals and Jacobians of the p-Laplacian of
+    test/GridapTests/PLaplacianTests.jl:15-17,31-32 (flux(∇u) = |∇u|^(p-2) ∇u,
+    dflux(∇du,∇u) = (p-2)|∇u|^(p-4)(∇u⋅∇du)∇u + |∇u|^(p-2)∇du), scaled by κ per cell,
+    with the source term -∫ f v, at the FE function whose cell values are u_cell:
+        r_a  = Σ_q w|J| (κ ∇φ_a⋅flux(∇u) - f φ_a)
+        J_ab = Σ_q w|J| κ ∇φ_a⋅dflux(∇φ_b, ∇u)
+    (residual!/jacobian! of src/FESpaces/FEOperatorsFromWeakForm.jl:57-95, integrated
+    as IntegrationMap, src/Fields/FieldsInterfaces.jl:737-775; the geometry as in
+    cell_matrices_diffusion: the isoparametric Jt, or diag(sizes) for CartesianMap).
+    Returns (cellmats [c, lj, li], cellvecs [c, a])."""
+    cn = np.asarray(cell_nodes, dtype=np.int64) - 1
+    nc, nl = cn.shape
+    qw = np.asarray(qw, dtype=np.float64)
+    shp = np.asarray(shp, dtype=np.float64).reshape(len(qw), nl)
+    grad = np.asarray(grad, dtype=np.float64).reshape(len(qw), nl, D)
+    if cartesian_sizes is not None:
+        Jt = np.broadcast_to(np.diag(np.asarray(cartesian_sizes, dtype=np.float64)[:D]),
+                             (nc, len(qw), D, D))
+    else:
+        X = np.asarray(xyz, dtype=np.float64)[cn]                       # [c, a, D]
+        Jt = np.einsum("qai,caj->cqij", grad, X)
+    Ji = np.linalg.inv(Jt)
+    dV = np.abs(np.linalg.det(Jt)) * qw[None, :]                         # [c, q]
+    g = np.einsum("cqrk,qak->cqar", Ji, grad)                            # [c, q, a, r]
+    gu = np.einsum("ca,cqar->cqr", np.asarray(u_cell, dtype=np.float64), g)
+    n2 = np.sum(gu * gu, axis=-1)
+    nrm = np.sqrt(n2)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        s0 = np.ones_like(nrm) if p == 2 else np.where(nrm > 0, nrm ** (p - 2), 0.0)
+        s1 = np.zeros_like(nrm) if p == 2 else np.where(nrm > 0, (p - 2) * nrm ** (p - 4), 0.0)
+    kap = np.ones(nc) if kappa is None else np.asarray(kappa, dtype=np.float64)
+    f = np.zeros(nc) if fsrc is None else np.asarray(fsrc, dtype=np.float64)
+    ga = np.einsum("cqr,cqar->cqa", gu, g)                                # ∇u⋅∇φ_a
+    r = np.einsum("cq,cqa->ca", dV * kap[:, None] * s0, ga) - \
+        np.einsum("cq,qa->ca", dV * f[:, None], shp)
+    K = np.einsum("cq,cqar,cqbr->cab", dV * kap[:, None] * s0, g, g) + \
+        np.einsum("cq,cqa,cqb->cab", dV * kap[:, None] * s1, ga, ga)
+    return np.ascontiguousarray(K.transpose(0, 2, 1)), r                 # [c, lj, li]
+
+
+def plaplacian_system(cell_dofs, nfree, cellmats, cellvecs):
+    """Global Jacobian (CSC) and residual from the cell arrays (assemble_matrix! and
+    assemble_vector!, Dirichlet rows and columns skipped; no lifting)."""
+    colptr, rowval, nzval = assemble_csc(cell_dofs, cellmats, nfree, nfree)
+    r = assemble_vector(cell_dofs, cellvecs, nfree)
+    return colptr, rowval, nzval, r

@@ -1,0 +1,143 @@
+"""Newton FEOperator on the device: residual and Jacobian of the p-Laplacian at an FE
+function against the oracle, and Newton's method on PLaplacianTests.jl's driver
+(needs an MI355X).
+
+Bar: FP64 residual and Jacobian within 1e-12 relative (max-abs error / max-abs value)
+of the oracle, pattern bit-exact; FP32 element math within 1e-6 relative Frobenius;
+Newton iterates within 1e-10 of the oracle's Newton (direct solves) and the
+manufactured solution u = x + y reproduced to 1e-9.
+"""
+import sys
+
+import numpy as np
+import pytest
+import scipy.sparse.linalg as spla
+
+import gridap_hip as G
+
+pytestmark = pytest.mark.gpu
+
+
+def announce(msg):
+    print("[gpu-case] " + msg, flush=True)
+    sys.stdout.flush()
+
+
+def relmax(a, b):
+    return np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300) if a.size else 0.0
+
+
+def relfrob(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def _operator(part, p, kappa=None, fsrc=None, g=None, map=None, tags=("boundary",),
+              precision="fp64"):
+    D = len(part)
+    model = G.CartesianDiscreteModel((0, 1) * D, part, map=map)
+    V = G.TestFESpace(model, G.ReferenceFE(G.lagrangian, G.Float64, 1),
+                      dirichlet_tags=list(tags))
+    U = G.TrialFESpace(V, g)
+    Ω = G.Triangulation(model)
+    dΩ = G.Measure(Ω, 2)
+    law = G.PLaplacian(p)
+    k = 1.0 if kappa is None else G.CellField(kappa, Ω)
+    f = 0.0 if fsrc is None else G.CellField(fsrc, Ω)
+    res = lambda u, v: G.integrate(k * G.inner(G.gradient(v), law.flux(G.gradient(u))) -
+                                   f * v, dΩ)
+    jac = lambda u, du, v: G.integrate(
+        k * G.inner(G.gradient(v), law.dflux(G.gradient(du), G.gradient(u))), dΩ)
+    assem = G.SparseMatrixAssembler(U, V, precision=precision)
+    return G.FEOperator(res, jac, U, V, assem), V, U, model
+
+
+def _oracle_system(oracle, model, V, uh, p, kappa, fsrc):
+    D = model.D
+    cd = V.cell_dof_ids
+    xyz = model.node_coordinates()
+    cn = model.cell_node_ids()
+    qx, qw = oracle.tensor_quadrature(D, 2)
+    shp, grad = oracle.qk_tabulate(D, 1, qx)
+    uc = oracle.cell_values(cd, uh.free_values, uh.dirichlet_values)
+    cs = model.sizes if model.is_affine_cartesian() else None
+    K, r = oracle.plaplacian_cells(D, cn, xyz, qw, shp, grad, uc, p, kappa, fsrc,
+                                   cartesian_sizes=cs)
+    return oracle.plaplacian_system(cd, V.nfree, K, r)
+
+
+@pytest.mark.parametrize("part,mapped,p", [((9, 7), False, 3.0), ((9, 7), True, 4.0),
+                                           ((7, 6, 5), False, 3.0), ((6, 5, 4), True, 3.5)])
+def test_residual_and_jacobian_match_oracle(oracle, part, mapped, p):
+    announce("p-Laplacian residual+Jacobian %s mapped=%s p=%g" % (part, mapped, p))
+    D = len(part)
+    rng = np.random.default_rng(len(part) * 10 + int(mapped))
+    C = int(np.prod(part))
+    kappa = np.exp(rng.normal(size=C))
+    fsrc = rng.uniform(-1, 1, C)
+    mp = (lambda x: x + 0.04 * np.sin(3 * x[..., ::-1])) if mapped else None
+    g = lambda x: 1.0 + x[..., 0] - 0.5 * x[..., 1]
+    op, V, U, model = _operator(part, p, kappa, fsrc, g=g, map=mp)
+    uh = G.FEFunction(U, rng.normal(size=V.nfree))
+    b, A = G.residual_and_jacobian(op, uh)
+    cp, rv, nz, r = _oracle_system(oracle, model, V, uh, p, kappa, fsrc)
+    np.testing.assert_array_equal(A.colptr, cp)
+    np.testing.assert_array_equal(A.rowval, rv)
+    assert relmax(A.nzval, nz) <= 1e-12
+    assert relmax(b, r) <= 1e-12
+    # in-place re-assembly at another uh, into the same A and b
+    uh2 = G.FEFunction(U, rng.normal(size=V.nfree))
+    b2, A2 = op.residual_and_jacobian_inplace(b, A, uh2)
+    assert A2 is A and b2 is b
+    cp, rv, nz, r = _oracle_system(oracle, model, V, uh2, p, kappa, fsrc)
+    assert relmax(A.nzval, nz) <= 1e-12 and relmax(b, r) <= 1e-12
+    # FP32 element math
+    op32, *_ = _operator(part, p, kappa, fsrc, g=g, map=mp, precision="fp32")
+    b32, A32 = G.residual_and_jacobian(op32, uh2)
+    assert relfrob(A32.nzval, nz) <= 1e-6 and relfrob(b32, r) <= 1e-6
+
+
+def test_newton_plaplacian_driver():
+    """PLaplacianTests.jl:8-47 with Q1 (u = x + y is in the space): p = 3, f = 0, random
+    initial guess, NewtonRaphsonSolver with device PCG; the solution is the interpolant
+    of u and the last steps converge quadratically."""
+    announce("p-Laplacian Newton driver 4x4 and 24x20")
+    g = lambda x: x[..., 0] + x[..., 1]
+    for part in ((4, 4), (24, 20)):
+        op, V, U, model = _operator(part, 3.0, g=g)
+        uh0 = G.FEFunction(U, np.random.default_rng(0).uniform(0, 1, V.nfree))
+        nls = G.NewtonRaphsonSolver(G.PCGSolver(rtol=1e-13), 1e-12, 20)
+        uh, _ = G.solve_inplace(uh0, G.FESolver(nls), op)
+        uI = g(model.node_coordinates()[V.free_dof_to_node - 1])
+        assert np.max(np.abs(uh.free_values - uI)) <= 1e-9
+        h = nls.history
+        assert h[-1] < 1e-12 * h[0]
+        assert h[-1] <= 10.0 * h[-2] ** 2 / h[0] + 1e-15   # quadratic at the end
+
+
+def test_newton_iterates_match_oracle(oracle):
+    """3-D 12x10x8, κ lognormal, f per cell, p = 3: every Newton iterate of the device
+    (residual/Jacobian assembled on the device, PCG to 1e-13) against the oracle's
+    Newton with direct solves, from the same initial guess."""
+    announce("p-Laplacian Newton iterates vs oracle 12x10x8")
+    part = (12, 10, 8)
+    rng = np.random.default_rng(7)
+    C = int(np.prod(part))
+    kappa = np.exp(0.5 * rng.normal(size=C))
+    fsrc = rng.uniform(0.5, 1.5, C)
+    g = lambda x: 0.2 * x[..., 0] - x[..., 2]
+    op, V, U, model = _operator(part, 3.0, kappa, fsrc, g=g)
+    x = rng.uniform(-0.2, 0.2, V.nfree)
+    xo = x.copy()
+    pcg = G.PCGSolver(rtol=1e-13)
+    r0 = None
+    for it in range(6):
+        b, A = op.residual_and_jacobian(G.FEFunction(U, x))
+        cp, rv, nz, r = _oracle_system(oracle, model, V, G.FEFunction(U, xo), 3.0, kappa, fsrc)
+        r0 = np.abs(r).max() if r0 is None else r0
+        # residuals against the initial residual's scale (late iterates are rounding noise)
+        assert np.abs(b - r).max() <= 1e-10 * r0, it
+        assert relmax(A.nzval, nz) <= 1e-10, it
+        x = x + pcg.solve(op.assem, -b, A)
+        xo = xo + spla.spsolve(oracle.to_scipy(cp, rv, nz, V.nfree).tocsc(), -r)
+        assert relmax(x, xo) <= 1e-9, it
+    assert np.abs(r).max() <= 1e-10 * r0   # converged
