@@ -32,6 +32,21 @@ int guarded(gh_ctx *ctx, F &&f) {
   }
 }
 
+void clear_boundary(gh::Ctx &c) {
+  c.nbfaces = 0;
+  c.h_face_cell.clear();
+  c.bface_cell.release();
+  c.bface_side.release();
+  c.balpha.release();
+  c.bbeta.release();
+  c.bg.release();
+  c.bh.release();
+  c.have_balpha = c.have_bbeta = c.have_bg = c.have_bh = false;
+  c.bface_ncolors = 0;
+  c.bface_color_ptr.clear();
+  c.bface_order.release();
+}
+
 // A new mesh voids the FE space uploaded for the previous one: its cell_dof_ids were
 // numbered on the old cells (gh_space_upload must follow gh_mesh_*).
 void invalidate_space(gh::Ctx &c) {
@@ -44,6 +59,9 @@ void invalidate_space(gh::Ctx &c) {
   c.have_kappa = c.have_fsrc = c.have_dvals = false;
   c.have_state = false;
   c.nq = 0;
+  c.nqf = 0;
+  c.bface_tab_ok = false;
+  clear_boundary(c);
 }
 
 void invalidate_pattern(gh::Ctx &c) {
@@ -226,6 +244,7 @@ int gh_space_upload(gh_ctx *ctx, int reffe_kind, int order, int ncomp, const int
     invalidate_pattern(c);
     c.have_dvals = false;
     c.have_state = false;
+    clear_boundary(c);
   });
 }
 
@@ -242,6 +261,74 @@ int gh_reffe_tabulation(gh_ctx *ctx, int nq, int nloc, const double *qw, const d
     c.grad.assign(grad, grad + size_t(nq) * nloc * c.dim);
     c.have_values = false;
     c.sref_ready = false;
+  });
+}
+
+int gh_face_tabulation(gh_ctx *ctx, int nsides, int nqf, const double *qw, const double *shp,
+                       const double *grad) {
+  return guarded(ctx, [&](gh::Ctx &c) {
+    GH_REQUIRE(c.have_space, GH_ERR_STATE, "upload the FE space before the face tabulation");
+    GH_REQUIRE(nsides == 2 * c.dim, GH_ERR_ARG, "one tabulation per facet of the n-cube");
+    GH_REQUIRE(nqf >= 1 && nqf <= 64, GH_ERR_ARG, "face quadrature size out of range");
+    GH_REQUIRE(qw && shp && grad, GH_ERR_ARG, "NULL face tabulation array");
+    const int NL = c.nloc, D = c.dim, stride = 1 + NL + NL * D;
+    std::vector<double> tab((size_t)nsides * nqf * stride);
+    for (int s = 0; s < nsides; ++s)
+      for (int q = 0; q < nqf; ++q) {
+        double *t = &tab[((size_t)s * nqf + q) * stride];
+        t[0] = qw[q];
+        for (int a = 0; a < NL; ++a) t[1 + a] = shp[((size_t)s * nqf + q) * NL + a];
+        for (int a = 0; a < NL * D; ++a)
+          t[1 + NL + a] = grad[((size_t)s * nqf + q) * NL * D + a];
+      }
+    c.bface_tab.resize(tab.size());
+    GH_CHECK_HIP(hipMemcpy(c.bface_tab.get(), tab.data(), sizeof(double) * tab.size(),
+                           hipMemcpyHostToDevice));
+    c.nqf = nqf;
+    c.bface_tab_ok = true;
+    c.have_values = false;
+  });
+}
+
+int gh_boundary_upload(gh_ctx *ctx, int64_t nfaces, const int32_t *face_cells,
+                       const int32_t *face_sides, const double *alpha, const double *beta,
+                       const double *gq, const double *hq) {
+  return guarded(ctx, [&](gh::Ctx &c) {
+    GH_REQUIRE(c.have_space, GH_ERR_STATE, "upload the FE space before boundary terms");
+    GH_REQUIRE(nfaces >= 0 && nfaces < (int64_t(1) << 31), GH_ERR_ARG, "bad face count");
+    clear_boundary(c);
+    c.have_values = false;
+    if (nfaces == 0) return;
+    GH_REQUIRE(face_cells && face_sides, GH_ERR_ARG, "NULL face arrays");
+    GH_REQUIRE(c.bface_tab_ok && c.nqf > 0, GH_ERR_STATE,
+               "upload the face tabulation (gh_face_tabulation) first");
+    std::vector<int32_t> fc(nfaces), fs(nfaces);
+    for (int64_t f = 0; f < nfaces; ++f) {
+      GH_REQUIRE(face_cells[f] >= 1 && face_cells[f] <= c.ncells, GH_ERR_RANGE,
+                 "face cell out of range");
+      GH_REQUIRE(face_sides[f] >= 1 && face_sides[f] <= 2 * c.dim, GH_ERR_RANGE,
+                 "face side out of range");
+      fc[f] = face_cells[f] - 1;
+      fs[f] = face_sides[f] - 1;
+    }
+    c.bface_cell.resize(nfaces);
+    c.bface_side.resize(nfaces);
+    GH_CHECK_HIP(hipMemcpy(c.bface_cell.get(), fc.data(), sizeof(int32_t) * nfaces,
+                           hipMemcpyHostToDevice));
+    GH_CHECK_HIP(hipMemcpy(c.bface_side.get(), fs.data(), sizeof(int32_t) * nfaces,
+                           hipMemcpyHostToDevice));
+    auto up = [&](gh::DBuf<double> &buf, bool &flag, const double *src, size_t n) {
+      flag = src != nullptr;
+      if (!flag) return;
+      buf.resize(n);
+      GH_CHECK_HIP(hipMemcpy(buf.get(), src, sizeof(double) * n, hipMemcpyHostToDevice));
+    };
+    up(c.balpha, c.have_balpha, alpha, nfaces);
+    up(c.bbeta, c.have_bbeta, beta, nfaces);
+    up(c.bg, c.have_bg, gq, (size_t)nfaces * c.nqf);
+    up(c.bh, c.have_bh, hq, (size_t)nfaces * c.nqf);
+    c.h_face_cell.swap(fc);
+    c.nbfaces = nfaces;
   });
 }
 
@@ -319,6 +406,7 @@ int gh_assemble(gh_ctx *ctx, int form_kind, int precision, int path) {
       GH_REQUIRE(!gh::hexq2_eligible(c), GH_ERR_UNSUPPORTED,
                  "vector Q2 spaces assemble the elasticity form");
       gh::assemble_diffusion(c, precision, path);
+      gh::assemble_boundary(c);
     }
     c.have_values = true;
     c.fsai_built = false;

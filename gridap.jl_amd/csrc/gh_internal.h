@@ -166,6 +166,19 @@ struct Ctx {
   double plap_p = 2.0;              // p of GH_FORM_PLAPLACIAN
   DBuf<double> nl_tab;              // packed tabulation of the nonlinear kernels
 
+  // ---- boundary faces (BoundaryTriangulation) and their terms ----
+  int64_t nbfaces = 0;
+  std::vector<int32_t> h_face_cell; // 0-based cell of each face (host copy, colouring)
+  DBuf<int32_t> bface_cell, bface_side;  // 0-based cell and n-cube facet of each face
+  int nqf = 0;                      // quadrature points per face
+  bool bface_tab_ok = false;        // face tabulation matches the element
+  DBuf<double> bface_tab;           // per side, per point: w, shp[nloc], grad[nloc*dim]
+  DBuf<double> balpha, bbeta, bg, bh;
+  bool have_balpha = false, have_bbeta = false, have_bg = false, have_bh = false;
+  int bface_ncolors = 0;
+  std::vector<int64_t> bface_color_ptr;
+  DBuf<int32_t> bface_order;        // faces grouped by colour
+
   // ---- coefficients / values ----
   DBuf<double> kappa, fsrc, dvals;
   bool have_kappa = false, have_fsrc = false, have_dvals = false;
@@ -222,6 +235,8 @@ bool simplex_eligible(const Ctx &c);
 void elemlist_symbolic(Ctx &c);         // dof -> cell adjacency, Dirichlet masks
 void elemlist_sum(Ctx &c, bool f32);     // nzval from the element entries
 void assemble_simplex(Ctx &c, int precision);
+// boundary.hip
+void assemble_boundary(Ctx &c);  // adds the boundary-face terms to nzval and b
 // nonlinear.hip
 void assemble_plaplacian(Ctx &c, int precision);  // residual -> b, Jacobian -> nzval
 // hexq2.hip

@@ -22,11 +22,11 @@ from .assemblers import (AffineFEOperator, LinearFESolver, PCGSolver, SparseMatr
                          assemble_vector, assemble_vector_inplace, get_matrix, get_vector,
                          solve)
 from .celldata import (CellField, DomainContribution, IsotropicStress, Measure, PLaplacian,
-                       compose, dot, get_fe_basis, get_trial_fe_basis, grad, gradient, inner,
-                       integral, integrate, symmetric_gradient, ε)
-from .fespaces import FEFunction, FESpace, TestFESpace, TrialFESpace
-from .geometry import (CartesianDiscreteModel, FaceLabeling, Triangulation, UnstructuredDiscreteModel,
-                       get_triangulation, simplexify)
+                       compose, dot, get_fe_basis, get_normal_vector, get_trial_fe_basis, grad,
+                       gradient, inner, integral, integrate, symmetric_gradient, ε)
+from .fespaces import FEFunction, FESpace, TestFESpace, TrialFESpace, interpolate
+from .geometry import (BoundaryTriangulation, CartesianDiscreteModel, FaceLabeling, Triangulation,
+                       UnstructuredDiscreteModel, get_triangulation, simplexify)
 from .reference_fes import Float64, Quadrature, ReferenceFE, VectorValue, lagrangian
 from .nonlinear import (FEOperator, FESolver, NewtonRaphsonSolver, jacobian, residual,
                         residual_and_jacobian, solve_inplace)
@@ -44,4 +44,5 @@ __all__ = [
     "IsotropicStress", "symmetric_gradient", "ε",
     "FEOperator", "FESolver", "NewtonRaphsonSolver", "PLaplacian", "compose", "jacobian",
     "residual", "residual_and_jacobian", "solve_inplace",
+    "BoundaryTriangulation", "get_normal_vector", "interpolate",
 ]

@@ -62,6 +62,8 @@ SIGNATURES = {
     "gh_mesh_upload": (_i32, [_vp, _i32, _i64, _vp, _i64, _vp, _vp]),
     "gh_space_upload": (_i32, [_vp, _i32, _i32, _i32, _vp, _vp, _i64, _i64, _i64]),
     "gh_reffe_tabulation": (_i32, [_vp, _i32, _i32, _vp, _vp, _vp]),
+    "gh_face_tabulation": (_i32, [_vp, _i32, _i32, _vp, _vp, _vp]),
+    "gh_boundary_upload": (_i32, [_vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp]),
     "gh_symbolic": (_i32, [_vp, _pi64]),
     "gh_pattern_get": (_i32, [_vp, _vp, _vp]),
     "gh_coef_upload": (_i32, [_vp, _vp, _vp, _vp]),
@@ -188,6 +190,25 @@ class Context:
         grad = np.ascontiguousarray(grad, dtype=np.float64)
         self.check(lib().gh_reffe_tabulation(self.h, len(qw), shp.shape[1], ptr(qw),
                                               ptr(shp), ptr(grad)))
+
+    def face_tabulation(self, qw, shp, grad):
+        """Per n-cube facet: the cell's reference tabulation at the facet points;
+        shp [nsides, nqf, nloc], grad [nsides, nqf, nloc, dim]."""
+        qw = np.ascontiguousarray(qw, dtype=np.float64)
+        shp = np.ascontiguousarray(shp, dtype=np.float64)
+        grad = np.ascontiguousarray(grad, dtype=np.float64)
+        self.check(lib().gh_face_tabulation(self.h, shp.shape[0], len(qw), ptr(qw), ptr(shp),
+                                             ptr(grad)))
+
+    def boundary_upload(self, face_cells, face_sides, alpha=None, beta=None, gq=None, hq=None):
+        fc = np.ascontiguousarray(face_cells, dtype=np.int32)
+        fs = np.ascontiguousarray(face_sides, dtype=np.int32)
+        if fc.shape != fs.shape:
+            raise ValueError("face_cells and face_sides differ in length")
+        arrs = [None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+                for a in (alpha, beta, gq, hq)]
+        self.check(lib().gh_boundary_upload(self.h, len(fc), ptr(fc), ptr(fs),
+                                             *[ptr(a) for a in arrs]))
 
     def symbolic(self):
         nnz = C.c_int64()

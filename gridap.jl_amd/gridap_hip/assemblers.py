@@ -12,8 +12,10 @@ import numpy as np
 
 from . import _lib
 from ._lib import Context
+from .boundary import lower_boundary
 from .celldata import (BodyForceTerm, DiffusionTerm, DomainContribution, ElasticityTerm,
-                       SourceTerm, get_fe_basis, get_trial_fe_basis)
+                       SourceTerm, _cell_coef, get_fe_basis, get_trial_fe_basis)
+from .geometry import BoundaryTriangulation
 from .fespaces import FEFunction, TrialFESpace
 from .reference_fes import Quadrature
 
@@ -75,6 +77,8 @@ class SparseMatrixAssembler:
         # values are not those of any matrix handed out, e.g. after a vector-only
         # assembly); solvers load their matrix into the device when it is not this one
         self.device_matrix = None
+        self._face_degree = None    # degree of the face tabulation on the device
+        self._has_boundary = False  # boundary terms uploaded
 
     def get_rows(self):
         return self.space.get_free_dof_ids()
@@ -110,9 +114,24 @@ class SparseMatrixAssembler:
             self.ctx.reffe_tabulation(q.weights, shp, grad)
             self._degree = degree
 
+    @staticmethod
+    def _split(contrib):
+        """(cell terms, boundary terms) of a DomainContribution (or None)."""
+        if contrib is None:
+            return None, []
+        cell = [(t, m) for t, m in contrib.terms if not isinstance(m.trian, BoundaryTriangulation)]
+        face = [(t, m) for t, m in contrib.terms if isinstance(m.trian, BoundaryTriangulation)]
+        return (DomainContribution(cell) if cell else None), face
+
     def _lower(self, matcontrib, veccontrib):
-        """Weak form -> (form kind, per-cell coefficient, per-cell source, degree, extra)."""
+        """Weak form -> (form kind, per-cell coefficient, per-cell source, degree, extra);
+        boundary terms go to extra["boundary"] (lower_boundary's per-face data)."""
         ncells = self.space.model.num_cells()
+        had_mat = matcontrib is not None
+        matcontrib, mface = self._split(matcontrib)
+        veccontrib, vface = self._split(veccontrib)
+        if had_mat and matcontrib is None:
+            raise NotImplementedError("a bilinear form needs a cell (volume) term")
         kappa, fsrc, degs = None, None, set()
         nu, force = None, None
         have_mat = have_vec = False
@@ -128,7 +147,8 @@ class SparseMatrixAssembler:
                     kinds.add(_lib.GH_FORM_DIFFUSION)
                 else:
                     raise NotImplementedError("bilinear forms: κ ∇v⋅∇u or ε(v) : σ(ε(u))")
-                k = term.coef.per_cell(ncells) if term.coef is not None else None
+                coef = _cell_coef(term.coef)
+                k = coef.per_cell(ncells) if coef is not None else None
                 k = np.ones(ncells) if k is None else k
                 kappa = k if kappa is None else kappa + k
                 degs.add(meas.degree)
@@ -141,7 +161,8 @@ class SparseMatrixAssembler:
                     force = f if force is None else force + f
                 elif isinstance(term, SourceTerm):
                     kinds.add(_lib.GH_FORM_DIFFUSION)
-                    f = term.coef.per_cell(ncells) if term.coef is not None else None
+                    coef = _cell_coef(term.coef)
+                    f = coef.per_cell(ncells) if coef is not None else None
                     f = np.ones(ncells) if f is None else f
                     fsrc = f if fsrc is None else fsrc + f
                 else:
@@ -152,6 +173,12 @@ class SparseMatrixAssembler:
             raise NotImplementedError("one form kind per operator")
         if len(degs) > 1:
             raise NotImplementedError("matrix and vector terms must share one Measure")
+        if not degs and (mface or vface):
+            # boundary terms only (e.g. a Neumann load): the cell part is empty
+            form = _lib.GH_FORM_DIFFUSION
+            kinds.add(form)
+            degs.add(2 * self.space.order)
+            fsrc = np.zeros(ncells)
         if not degs:
             raise ValueError("empty weak form")
         form = kinds.pop()
@@ -166,7 +193,26 @@ class SparseMatrixAssembler:
             kappa = np.zeros(ncells)
         extra = dict(nu=0.3 if nu is None else nu,
                      force=np.zeros(3) if force is None else force)
+        if mface or vface:
+            if form != _lib.GH_FORM_DIFFUSION:
+                raise NotImplementedError("boundary terms are provided with the diffusion form")
+            extra["boundary"] = lower_boundary(self.space.model, self.space.reffe, mface, vface)
         return form, kappa, fsrc, degs.pop(), extra
+
+    def _upload_boundary(self, bnd):
+        if bnd is None:
+            if self._has_boundary:
+                self.ctx.boundary_upload(np.zeros(0, np.int32), np.zeros(0, np.int32))
+                self._has_boundary = False
+            return
+        fs = bnd["faces"]
+        key = (fs.nqf, fs.qw.tobytes())
+        if self._face_degree != key:
+            self.ctx.face_tabulation(fs.qw, fs.shp, fs.grad)
+            self._face_degree = key
+        self.ctx.boundary_upload(fs.cells + 1, fs.sides + 1, bnd["alpha"], bnd["beta"],
+                                 bnd["gq"], bnd["hq"])
+        self._has_boundary = True
 
     def assemble(self, matcontrib, veccontrib, lift, A=None, b=None):
         """Device numeric phase; returns (SparseMatrixCSC, b).  With A (a matrix on this
@@ -181,6 +227,7 @@ class SparseMatrixAssembler:
             if dv is not None and dv.size and np.any(dv != 0):
                 dvals = dv
         self.ctx.coef_upload(kappa, fsrc, dvals)
+        self._upload_boundary(extra.get("boundary"))
         if form == _lib.GH_FORM_ELASTICITY:
             self.ctx.elasticity_params(extra["nu"], extra["force"])
         if A is not None and not (A.shape == (self.space.nfree, self.space.nfree) and

@@ -12,12 +12,21 @@ form kind plus per-cell coefficients, which is what the device kernels consume:
     l(v)   = integrate(dot(v, VectorValue(f1, f2, f3)), dΩ)       -> body force
 
 κ, f and E are numbers or per-cell arrays (CellField(values, Ω)).
+
+Over a BoundaryTriangulation Γ (measure dΓ, normal n = get_normal_vector(Γ)):
+
+    ∫( α*v*u + β*(v*(n⋅∇(u)) + (n⋅∇(v))*u) )dΓ   -> face matrix (Nitsche, Robin)
+    ∫( v*g + (n⋅∇(v))*h )dΓ                        -> face vector (Neumann, Nitsche)
+
+where α, β are numbers or per-cell arrays and g, h may also be functions of x, FE
+functions uh, n⋅∇(uh), or products of these; they are evaluated at the face
+quadrature points by the host.
 """
 import numbers
 
 import numpy as np
 
-from .geometry import Triangulation
+from .geometry import BoundaryTriangulation, Triangulation
 
 
 class CellField:
@@ -42,11 +51,46 @@ class CellField:
     __rmul__ = __mul__
 
 
+class PointField:
+    """A coefficient g(x) evaluated at the quadrature points (boundary terms)."""
+
+    def __init__(self, func):
+        self.func = func
+
+
+class FEValue:
+    """The values of an FE function uh at the quadrature points."""
+
+    def __init__(self, uh):
+        self.uh = uh
+
+
+class NormalGradValue:
+    """n⋅∇uh of an FE function at the boundary quadrature points."""
+
+    def __init__(self, uh):
+        self.uh = uh
+
+
+class CoefProduct:
+    """Product of coefficients, evaluated factor by factor."""
+
+    def __init__(self, factors):
+        self.factors = factors
+
+
+_POINT_COEFS = (PointField, FEValue, NormalGradValue, CoefProduct)
+
+
 def _coef_product(a, b):
     if a is None:
         return b
     if b is None:
         return a
+    if isinstance(a, _POINT_COEFS) or isinstance(b, _POINT_COEFS):
+        fa = a.factors if isinstance(a, CoefProduct) else [a]
+        fb = b.factors if isinstance(b, CoefProduct) else [b]
+        return CoefProduct(fa + fb)
     va = a.values if isinstance(a, CellField) else a
     vb = b.values if isinstance(b, CellField) else b
     if isinstance(va, numbers.Number) and isinstance(vb, numbers.Number):
@@ -156,6 +200,41 @@ class _Sum(_Expr):
         self.terms = terms
 
 
+class _Normal:
+    """get_normal_vector(Γ): the outward unit normal of a boundary triangulation."""
+
+    def __init__(self, trian):
+        self.trian = trian
+
+
+class _NormalGrad(_Expr):
+    """n⋅∇(v) or n⋅∇(u) of an FE basis."""
+
+    def __init__(self, basis):
+        self.basis = basis
+
+
+class MassTerm(_Expr):
+    """coef * v u (test times trial)."""
+
+    def __init__(self, coef=None):
+        self.coef = coef
+
+
+class NitscheTerm(_Expr):
+    """coef * v (n⋅∇u) (part "v_dnu") or coef * (n⋅∇v) u (part "dnv_u")."""
+
+    def __init__(self, part, coef=None):
+        self.part, self.coef = part, coef
+
+
+class NormalLoadTerm(_Expr):
+    """coef * (n⋅∇v) (a boundary load on the normal derivative of the test function)."""
+
+    def __init__(self, coef=None):
+        self.coef = coef
+
+
 class _GradState(_Expr):
     """∇uh of an FE function (the state of a nonlinear form)."""
 
@@ -219,33 +298,91 @@ def compose(f, *args):
     return f(*args)
 
 
+def _is_coef(x):
+    from .fespaces import FEFunction
+    if isinstance(x, (_Expr, FEBasis)):
+        return False
+    return (isinstance(x, (numbers.Number, np.ndarray, CellField, FEFunction)) or
+            isinstance(x, _POINT_COEFS) or callable(x))
+
+
+def _product(x, y):
+    """x * y for one ordering of the two factors, or None."""
+    cy = _is_coef(y)
+    if isinstance(x, FEBasis) and not x.trial:                       # v * y
+        if isinstance(y, FEBasis) and y.trial:
+            return MassTerm()
+        if isinstance(y, _NormalGrad) and y.basis.trial:
+            return NitscheTerm("v_dnu")
+        if cy:
+            return SourceTerm(_as_coef(y))
+    if isinstance(x, _NormalGrad) and not x.basis.trial:            # (n⋅∇v) * y
+        if isinstance(y, FEBasis) and y.trial:
+            return NitscheTerm("dnv_u")
+        if cy:
+            return NormalLoadTerm(_as_coef(y))
+    if isinstance(x, SourceTerm):                                    # (c v) * y
+        if isinstance(y, FEBasis) and y.trial:
+            return MassTerm(x.coef)
+        if isinstance(y, _NormalGrad) and y.basis.trial:
+            return NitscheTerm("v_dnu", x.coef)
+        if cy:
+            return SourceTerm(_coef_product(x.coef, _as_coef(y)))
+    if isinstance(x, NormalLoadTerm):                                # (c n⋅∇v) * y
+        if isinstance(y, FEBasis) and y.trial:
+            return NitscheTerm("dnv_u", x.coef)
+        if cy:
+            return NormalLoadTerm(_coef_product(x.coef, _as_coef(y)))
+    if cy:
+        if isinstance(x, (DiffusionTerm, MassTerm)):
+            return type(x)(_coef_product(x.coef, _as_coef(y)))
+        if isinstance(x, NitscheTerm):
+            return NitscheTerm(x.part, _coef_product(x.coef, _as_coef(y)))
+        if isinstance(x, (PLaplacianResidualTerm, PLaplacianJacobianTerm)):
+            return type(x)(x.p, x.state, _coef_product(x.coef, _as_coef(y)))
+        if isinstance(x, _Sum):
+            return _Sum([_scale(y, t) for t in x.terms])
+    return None
+
+
 def _scale(factor, expr):
-    if isinstance(factor, _Expr) and not isinstance(expr, _Expr):
-        factor, expr = expr, factor
-    if isinstance(expr, FEBasis):
-        if expr.trial:
-            raise NotImplementedError("only ∫ f v source terms are provided")
-        return SourceTerm(_as_coef(factor))
-    if isinstance(expr, (DiffusionTerm, SourceTerm)):
-        return type(expr)(_coef_product(expr.coef, _as_coef(factor)))
-    if isinstance(expr, (PLaplacianResidualTerm, PLaplacianJacobianTerm)):
-        return type(expr)(expr.p, expr.state, _coef_product(expr.coef, _as_coef(factor)))
-    if isinstance(expr, _Sum):
-        return _Sum([_scale(factor, t) for t in expr.terms])
+    """factor * expr, in either order of the two operands."""
+    for x, y in ((expr, factor), (factor, expr)):
+        r = _product(x, y)
+        if r is not None:
+            return r
+    if isinstance(expr, FEBasis) and expr.trial and _is_coef(factor):
+        raise NotImplementedError("only ∫ f v source terms are provided")
     raise NotImplementedError("unsupported weak-form product")
 
 
 def _as_coef(x):
-    if isinstance(x, CellField):
+    from .fespaces import FEFunction
+    if isinstance(x, (CellField,) + _POINT_COEFS):
         return x
     if isinstance(x, numbers.Number):
         return CellField(float(x))
     if isinstance(x, np.ndarray):
         return CellField(x)
+    if isinstance(x, FEFunction):
+        return FEValue(x)
     if callable(x):
-        raise NotImplementedError("coefficients must be constant per cell "
-                                  "(a number or a per-cell array)")
+        return PointField(x)
     raise TypeError("unsupported coefficient %r" % (x,))
+
+
+def _cell_coef(coef):
+    """Per-cell coefficient of a cell integral (constant on each cell)."""
+    if coef is not None and not isinstance(coef, CellField):
+        raise NotImplementedError("coefficients of cell integrals must be constant per cell "
+                                  "(a number or a per-cell array)")
+    return coef
+
+
+def get_normal_vector(trian):
+    if not isinstance(trian, BoundaryTriangulation):
+        raise TypeError("get_normal_vector needs a BoundaryTriangulation")
+    return _Normal(trian)
 
 
 def gradient(x):
@@ -270,7 +407,16 @@ def symmetric_gradient(x):
 
 
 def inner(a, b):
+    from .fespaces import FEFunction
     from .reference_fes import VectorValue
+    if isinstance(b, _Normal):
+        a, b = b, a
+    if isinstance(a, _Normal):
+        if isinstance(b, _Grad):
+            return _NormalGrad(b.basis)
+        if isinstance(b, _GradState):
+            return NormalGradValue(b.uh)
+        raise NotImplementedError("n⋅x is provided for gradients only")
     if isinstance(a, _Grad) and isinstance(b, _Grad) and a.basis.trial != b.basis.trial:
         return DiffusionTerm()
     if isinstance(a, _Grad) and not a.basis.trial and isinstance(b, _Flux):
@@ -288,7 +434,10 @@ def inner(a, b):
         return _scale(b, a)
     if isinstance(b, FEBasis) and not isinstance(a, _Expr):
         return _scale(a, b)
-    raise NotImplementedError("inner/dot of these operands is not provided")
+    try:
+        return _scale(a, b)
+    except NotImplementedError:
+        raise NotImplementedError("inner/dot of these operands is not provided") from None
 
 
 dot = inner
@@ -298,8 +447,8 @@ class Measure:
     """Measure(Ω, degree): the tensor-product Gauss rule of that degree."""
 
     def __init__(self, trian, degree):
-        if not isinstance(trian, Triangulation):
-            raise TypeError("Measure(Ω, degree) needs a Triangulation")
+        if not isinstance(trian, (Triangulation, BoundaryTriangulation)):
+            raise TypeError("Measure(Ω, degree) needs a Triangulation or BoundaryTriangulation")
         self.trian = trian
         self.degree = int(degree)
 
@@ -324,12 +473,18 @@ class DomainContribution:
         return NotImplemented
 
 
+_CELL_TERMS = (DiffusionTerm, SourceTerm, ElasticityTerm, BodyForceTerm,
+               PLaplacianResidualTerm, PLaplacianJacobianTerm)
+_FACE_TERMS = (MassTerm, NitscheTerm, SourceTerm, NormalLoadTerm)
+
+
 def integrate(expr, measure):
     terms = expr.terms if isinstance(expr, _Sum) else [expr]
+    boundary = isinstance(measure.trian, BoundaryTriangulation)
     for t in terms:
-        if not isinstance(t, (DiffusionTerm, SourceTerm, ElasticityTerm, BodyForceTerm,
-                              PLaplacianResidualTerm, PLaplacianJacobianTerm)):
-            raise NotImplementedError("unsupported integrand %r" % (t,))
+        if not isinstance(t, _FACE_TERMS if boundary else _CELL_TERMS):
+            raise NotImplementedError("unsupported integrand %r over %s" %
+                                      (t, "a boundary" if boundary else "the cells"))
     return DomainContribution([(t, measure) for t in terms])
 
 

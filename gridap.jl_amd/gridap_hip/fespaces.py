@@ -15,7 +15,7 @@ from .reference_fes import LagrangianRefFE
 
 class FESpace:
     def __init__(self, model, reffe, conformity="H1", dirichlet_tags=(), dirichlet_masks=None,
-                 trian=None):
+                 trian=None, labels=None):
         if isinstance(model, Triangulation):
             model = model.model
         if not isinstance(model, (CartesianDiscreteModel, UnstructuredDiscreteModel)):
@@ -41,7 +41,7 @@ class FESpace:
         if isinstance(dirichlet_tags, (str, int, np.integer)):
             dirichlet_tags = [dirichlet_tags]
         self.dirichlet_tags = list(dirichlet_tags)
-        labels = model.get_face_labeling()
+        labels = labels if labels is not None else model.get_face_labeling()
         if self.simplex:
             self._number_simplex(labels)
             return
@@ -293,3 +293,19 @@ class FEFunction:
                        self.dirichlet_values[np.maximum(-n2d, 1) - 1] if V.ndirichlet
                        else 0.0)
         return out
+
+
+def interpolate(u, U):
+    """interpolate(u, U): the FE function of U whose free dofs take u's values at their
+    nodes (Lagrangian dofs are point evaluations) and whose Dirichlet dofs are U's
+    Dirichlet values (src/FESpaces/FESpaceInterface.jl, interpolate)."""
+    V = U.space if isinstance(U, TrialFESpace) else U
+    if getattr(V, "conforming", False):
+        x, comp = V.free_dof_coords, getattr(V, "free_dof_comp", None)
+    else:
+        x = V.model.node_coordinates()[V.free_dof_to_node - 1]
+        comp = V.free_dof_to_comp - 1 if V.ncomp > 1 else None
+    vals = np.asarray(u(x), dtype=np.float64)
+    if vals.ndim > 1:
+        vals = vals[np.arange(len(vals)), comp]
+    return FEFunction(U, vals.reshape(-1))

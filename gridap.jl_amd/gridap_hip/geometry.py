@@ -46,6 +46,17 @@ class FaceLabeling:
         self.tag_names += ["interior", "boundary"]
         self.tag_entities = [[i] for i in range(1, nf)] + [[nf], list(range(1, nf))]
 
+    def add_tag_from_tags(self, name, tags):
+        """add_tag_from_tags!(labels, name, tags) (src/Geometry/FaceLabelings.jl): a new
+        tag whose entities are the union of the given tags' entities."""
+        if isinstance(tags, (str, int, np.integer)):
+            tags = [tags]
+        ents = sorted({e for t in tags for e in self.tag_to_entities(t)})
+        if name in self.tag_names:
+            raise ValueError("tag %r already exists" % name)
+        self.tag_names.append(name)
+        self.tag_entities.append(ents)
+
     def tag_to_entities(self, tag):
         if isinstance(tag, str):
             if tag not in self.tag_names:
@@ -166,6 +177,50 @@ class Triangulation:
 
 def get_triangulation(model):
     return Triangulation(model)
+
+
+class BoundaryTriangulation:
+    """BoundaryTriangulation(model[, labels]; tags="boundary") on a Cartesian model
+    (src/Geometry/BoundaryTriangulations.jl:122-183): the boundary facets whose entity
+    (the box side holding them) belongs to the tags, each as (cell, local facet) with
+    the facet in the n-cube face order (QUAD: y=0, y=1, x=0, x=1; HEX: z=0, z=1, y=0,
+    y=1, x=0, x=1).  Facets are listed side by side, cells ascending on each side."""
+
+    def __init__(self, model, labels=None, tags="boundary"):
+        if not isinstance(model, CartesianDiscreteModel) or not model.is_affine_cartesian():
+            raise NotImplementedError("boundary triangulations of Cartesian models only")
+        self.model = model
+        labels = labels if labels is not None else model.get_face_labeling()
+        if isinstance(tags, (str, int, np.integer)):
+            tags = [tags]
+        ents = {e for t in tags for e in labels.tag_to_entities(t)}
+        D, part = model.D, model.partition
+        cells, sides = [], []
+        ci = np.indices(part).reshape(D, -1, order="F")
+        for side in range(2 * D):
+            axis = D - 1 - side // 2
+            hi = side & 1
+            free = ((1 << D) - 1) & ~(1 << axis)
+            ent = ncube_face_id(D, free, (hi << axis))
+            if ent not in ents:
+                continue
+            on = ci[axis] == (part[axis] - 1 if hi else 0)
+            c = np.nonzero(on)[0]
+            cells.append(c)
+            sides.append(np.full(len(c), side))
+        self.face_cells = (np.concatenate(cells) if cells else np.zeros(0, np.int64)).astype(np.int64)
+        self.face_sides = (np.concatenate(sides) if sides else np.zeros(0, np.int64)).astype(np.int64)
+
+    def num_cells(self):
+        return len(self.face_cells)
+
+    def normals(self):
+        """Outward unit normal of every face, [nfaces, D]."""
+        D = self.model.D
+        n = np.zeros((len(self.face_sides), D))
+        axis = D - 1 - self.face_sides // 2
+        n[np.arange(len(axis)), axis] = np.where(self.face_sides & 1, 1.0, -1.0)
+        return n
 
 
 # ---- simplicial models ------------------------------------------------------------

@@ -13,7 +13,7 @@ import numpy as np
 from . import _lib
 from .assemblers import PCGSolver, SparseMatrixAssembler, SparseMatrixCSC, _space
 from .celldata import (PLaplacianJacobianTerm, PLaplacianResidualTerm, SourceTerm,
-                       get_fe_basis, get_trial_fe_basis)
+                       _cell_coef, get_fe_basis, get_trial_fe_basis)
 from .fespaces import FEFunction, TrialFESpace
 from .reference_fes import Quadrature
 
@@ -65,7 +65,8 @@ class FEOperator:
             if isinstance(t, PLaplacianResidualTerm):
                 if t.state is not uh:
                     raise ValueError("the residual must be evaluated at the given uh")
-                k = t.coef.per_cell(ncells) if t.coef is not None else None
+                coef = _cell_coef(t.coef)
+                k = coef.per_cell(ncells) if coef is not None else None
                 k = np.ones(ncells) if k is None else k
                 if p is not None and p != t.p:
                     raise NotImplementedError("one p-Laplacian exponent per operator")
@@ -73,7 +74,8 @@ class FEOperator:
                 kappa = k if kappa is None else kappa + k
             elif isinstance(t, SourceTerm):
                 # res += ∫ c v  <=>  the kernel's -∫ f v with f = -c
-                c = t.coef.per_cell(ncells) if t.coef is not None else None
+                coef = _cell_coef(t.coef)
+                c = coef.per_cell(ncells) if coef is not None else None
                 fsrc = fsrc - (np.ones(ncells) if c is None else c)
             else:
                 raise NotImplementedError("residuals: κ ∇v⋅flux(∇u) and source terms f v")
@@ -88,7 +90,8 @@ class FEOperator:
                 if not isinstance(t, PLaplacianJacobianTerm) or t.p != p:
                     raise NotImplementedError("the Jacobian must be κ ∇v⋅dflux(∇du, ∇u) of the "
                                               "residual's law")
-                k = t.coef.per_cell(ncells) if t.coef is not None else None
+                coef = _cell_coef(t.coef)
+                k = coef.per_cell(ncells) if coef is not None else None
                 k = np.ones(ncells) if k is None else k
                 kj = k if kj is None else kj + k
             if kj is None or not np.array_equal(kj, kappa):

@@ -224,3 +224,26 @@ class Quadrature:
         for d in range(D):
             w = w * w1[idx[d]]
         self.weights = w
+
+
+def facet_points(D, side, pts_f):
+    """Points of the reference n-cube facet `side` (n-cube face order: pairs of
+    opposite facets, normal axis from the last to the first) for facet-local points
+    pts_f [n, D-1] (in the remaining axes, ascending)."""
+    axis = D - 1 - side // 2
+    pts = np.zeros((pts_f.shape[0], D))
+    others = [k for k in range(D) if k != axis]
+    for j, k in enumerate(others):
+        pts[:, k] = pts_f[:, j]
+    pts[:, axis] = float(side & 1)
+    return pts
+
+
+def facet_quadrature(D, degree):
+    """The tensor Gauss-Legendre rule of `degree` on the reference facet (a (D-1)-cube),
+    as Quadrature(facet polytope, degree): div(degree,2)+1 points per axis, x fastest."""
+    q = Quadrature(D - 1, degree) if D > 2 else None
+    if D == 2:
+        x1, w1 = gauss_legendre01(degree // 2 + 1)
+        return x1[:, None], w1
+    return q.coords, q.weights

@@ -110,6 +110,27 @@ int gh_space_upload(gh_ctx *ctx, int reffe_kind, int order, int ncomp,
 int gh_reffe_tabulation(gh_ctx *ctx, int nq, int nloc, const double *qw, const double *shp,
                         const double *grad);
 
+/* Boundary integrals (Cartesian meshes, scalar Q1).  The reference tabulation at the
+ * facet quadrature points, per facet (side) of the n-cube in Gridap's face order
+ * (QUAD: y=0, y=1, x=0, x=1; HEX: z=0, z=1, y=0, y=1, x=0, x=1): facet weights qw[nqf]
+ * (of the reference facet), cell shape values shp[nsides*nqf*nloc] and reference
+ * gradients grad[nsides*nqf*nloc*dim].  Replaces the face-to-cell evaluation of a
+ * BoundaryTriangulation measure, src/Geometry/BoundaryTriangulations.jl:122-183. */
+int gh_face_tabulation(gh_ctx *ctx, int nsides, int nqf, const double *qw, const double *shp,
+                       const double *grad);
+
+/* The boundary faces and the terms integrated over them, added by gh_assemble
+ * (GH_FORM_DIFFUSION) into the same nzval and b: face f is facet face_sides[f]
+ * (1-based) of cell face_cells[f] (1-based) with outward normal n, and
+ *   A += ∫_F α φ_a φ_b + β (φ_a n⋅∇φ_b + n⋅∇φ_a φ_b),  b += ∫_F φ_a g + n⋅∇φ_a h
+ * with α[f], β[f] per face (NULL = 0) and g, h at the face quadrature points,
+ * gq[f*nqf + q], hq[f*nqf + q] (NULL = 0), followed by the Dirichlet lifting of the face
+ * matrix.  Neumann: g = n⋅∇u; Nitsche (test/GridapTests/PoissonTests.jl:101-108):
+ * α = γ/h, β = -1, g = (γ/h) u_D, h = -u_D.  nfaces = 0 removes the boundary terms. */
+int gh_boundary_upload(gh_ctx *ctx, int64_t nfaces, const int32_t *face_cells,
+                       const int32_t *face_sides, const double *alpha, const double *beta,
+                       const double *gq, const double *hq);
+
 /* Symbolic phase: counter -> allocation -> sorted unique rows per column, built on
  * the device.  nnz_out = stored entries of create_from_nz.
  * Replaces symbolic_loop_matrix! + nz_allocation + create_from_nz pattern,

@@ -482,3 +482,127 @@ def plaplacian_system(cell_dofs, nfree, cellmats, cellvecs):
     colptr, rowval, nzval = assemble_csc(cell_dofs, cellmats, nfree, nfree)
     r = assemble_vector(cell_dofs, cellvecs, nfree)
     return colptr, rowval, nzval, r
+
+
+# ---- boundary integrals ------------------------------------------------------------------
+
+def boundary_faces(model, tag_entities):
+    """The boundary facets of a Cartesian model whose entity is in tag_entities, found
+    from the vertex labels: a cell facet (the 2^(D-1) local vertices with one local
+    coordinate fixed) whose vertices all lie on one box side; its entity is that box
+    side's face id (BoundaryTriangulation(model, tags), src/Geometry/
+    BoundaryTriangulations.jl:122-183, with FaceLabeling entities,
+    src/Geometry/CartesianDiscreteModels.jl:137-170).  Returns [(cell, local vertex
+    list, fixed axis, fixed value)] in cell order, facets of one cell in axis order."""
+    D = model["D"]
+    xyz, cn = model["xyz"], np.asarray(model["cell_nodes"]) - 1
+    lo = xyz.min(0)
+    hi = xyz.max(0)
+    out = []
+    ents = set(tag_entities)
+    for c in range(cn.shape[0]):
+        for axis in range(D):
+            for val in (0, 1):
+                loc = [a for a in range(1 << D) if ((a >> axis) & 1) == val]
+                X = xyz[cn[c, loc]]
+                side = lo[axis] if val == 0 else hi[axis]
+                if not np.all(np.abs(X[:, axis] - side) < 1e-12):
+                    continue
+                free = ((1 << D) - 1) & ~(1 << axis)
+                ent = ncube_face_id(D, free, val << axis)
+                if ent in ents:
+                    out.append((c, loc, axis, val))
+    return out
+
+
+def boundary_face_arrays(model, faces, degree, alpha=0.0, beta=0.0, g=None, h=None):
+    """Face matrices [f, lj, li] and vectors [f, a] (cell-local numbering) of
+        ∫_F α φ_a φ_b + β (φ_a n⋅∇φ_b + n⋅∇φ_a φ_b)   and   ∫_F φ_a g + n⋅∇φ_a h
+    for Q1 cells: the facet quadrature (tensor Gauss of `degree` on the (D-1)-cube) is
+    mapped into the cell's reference coordinates, the cell basis and its isoparametric
+    gradients are evaluated there, the facet measure is sqrt(det(TᵀT)) of the facet
+    tangents T, and the normal is the unit normal of the facet oriented away from the
+    cell centre (the geometry of src/Geometry/BoundaryTriangulations.jl:122-183 and
+    get_normal_vector).  g, h: callables (x, n) -> value (None = 0)."""
+    D = model["D"]
+    xyz, cn = model["xyz"], np.asarray(model["cell_nodes"]) - 1
+    x1, w1 = gauss_legendre01(degree // 2 + 1)
+    if D == 2:
+        eta, we = x1[:, None], w1
+    else:                                                  # first facet coordinate fastest
+        eta = np.array([[x1[i], x1[j]] for j in range(len(x1)) for i in range(len(x1))])
+        we = np.array([w1[i] * w1[j] for j in range(len(x1)) for i in range(len(x1))])
+    nl = 1 << D
+    mats = np.zeros((len(faces), nl, nl))
+    vecs = np.zeros((len(faces), nl))
+    for k, (c, loc, axis, val) in enumerate(faces):
+        others = [d for d in range(D) if d != axis]
+        xi = np.zeros((len(we), D))
+        xi[:, others] = eta
+        xi[:, axis] = val
+        shp, grad = qk_tabulate(D, 1, xi)                 # [q, a], [q, a, D]
+        X = xyz[cn[c]]                                     # [a, D]
+        centre = X.mean(0)
+        for q in range(len(we)):
+            Jt = grad[q].T @ X                             # Jt[i, j] = ∂x_j/∂ξ_i
+            gphys = np.linalg.solve(Jt, grad[q].T).T       # [a, D]
+            T = Jt[others]                                 # facet tangents (rows)
+            dS = np.sqrt(np.linalg.det(T @ T.T)) * we[q]
+            if D == 2:
+                n = np.array([T[0, 1], -T[0, 0]])
+            else:
+                n = np.cross(T[0], T[1])
+            n = n / np.linalg.norm(n)
+            x = shp[q] @ X
+            if np.dot(n, x - centre) < 0:
+                n = -n
+            dn = gphys @ n
+            phi = shp[q]
+            mats[k] += dS * (alpha * np.outer(phi, phi) +
+                             beta * (np.outer(phi, dn) + np.outer(dn, phi)))
+            gv = 0.0 if g is None else g(x, n)
+            hv = 0.0 if h is None else h(x, n)
+            vecs[k] += dS * (phi * gv + dn * hv)
+    # [f, li, lj] -> [f, lj, li]
+    return np.ascontiguousarray(mats.transpose(0, 2, 1)), vecs
+
+
+def poisson_q1_with_boundary(domain, partition, dirichlet_entities, face_terms, degree=2,
+                             face_degree=2, g=None, fsrc=None):
+    """Q1 Poisson with boundary terms, assembled as Gridap's AffineFEOperator does with
+    a weak form over Ω and boundary triangulations: cell arrays then face arrays in one
+    assembly (the cell contributions first), Dirichlet lifting of both
+    (src/FESpaces/Assemblers.jl:485-541, src/CellData/AttachDirichlet.jl).
+    dirichlet_entities: the entity ids of the Dirichlet tag; face_terms: [(entity ids,
+    alpha, beta, g(x, n), h(x, n))]."""
+    D = len(partition)
+    model = cartesian_model(domain, partition)
+    ents = sorted(set(dirichlet_entities))
+    node_to_tag = face_tag_index(model["vertex_labels"], [ents] if ents else [])
+    n2d, nfree, ndiri = clagrangian(node_to_tag)
+    cd = clagrangian_cell_dofs(model["cell_nodes"], n2d)
+    qx, qw = tensor_quadrature(D, degree)
+    shp, grad = qk_tabulate(D, 1, qx)
+    mats = cell_matrices_diffusion(D, model["cell_nodes"], model["xyz"], qw, grad,
+                                   cartesian_sizes=model["sizes"])
+    vecs = cell_vectors_source(D, model["cell_nodes"], model["xyz"], qw, shp, grad, fsrc,
+                               cartesian_sizes=model["sizes"])
+    all_cd, all_m, all_v = [cd], [mats], [vecs]
+    for tents, al, be, gf, hf in face_terms:
+        faces = boundary_faces(model, tents)
+        fm, fv = boundary_face_arrays(model, faces, face_degree, al, be, gf, hf)
+        all_cd.append(cd[[f[0] for f in faces]])
+        all_m.append(fm)
+        all_v.append(fv)
+    cd_all = np.ascontiguousarray(np.concatenate(all_cd), dtype=np.int32)
+    m_all = np.ascontiguousarray(np.concatenate(all_m))
+    v_all = np.ascontiguousarray(np.concatenate(all_v))
+    dvals = np.zeros(ndiri)
+    if g is not None and ndiri:
+        dnodes = np.nonzero(n2d[:, 0] < 0)[0]
+        dvals[-n2d[dnodes, 0] - 1] = [g(model["xyz"][n]) for n in dnodes]
+        v_all = attach_dirichlet(cd_all, m_all, dvals, v_all)
+    colptr, rowval, nzval = assemble_csc(cd_all, m_all, nfree, nfree)
+    b = assemble_vector(cd_all, v_all, nfree)
+    return dict(model=model, node_to_dof=n2d, nfree=nfree, ndiri=ndiri, cell_dofs=cd,
+                colptr=colptr, rowval=rowval, nzval=nzval, b=b, dirichlet_vals=dvals)

@@ -1,0 +1,118 @@
+"""Boundary integrals on the device (needs an MI355X): Neumann and Nitsche terms of
+test/GridapTests/PoissonTests.jl:9-108 assembled into the same CSC and b as the cell
+terms, against the oracle.
+
+Bar: pattern bit-exact, nzval and b within 1e-12 relative (max-abs error / max-abs
+value) on both numeric paths (lattice + face kernel, general + face kernel); the
+driver's manufactured solution reproduced by the device solve (PoissonTests.jl:123-124
+asks 1e-8 relative).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import gridap_hip as G
+
+pytestmark = pytest.mark.gpu
+
+U_EX = lambda x: 1 + x[..., 0] + 2 * x[..., 1] + 0.5 * x[..., 0] * x[..., 1]
+GU_EX = lambda x: np.array([1 + 0.5 * x[1], 2 + 0.5 * x[0]])
+
+
+def announce(msg):
+    print("[gpu-case] " + msg, flush=True)
+    sys.stdout.flush()
+
+
+def relmax(a, b):
+    return np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300) if a.size else 0.0
+
+
+def poisson_driver(part, path="auto", gam=10.0):
+    """PoissonTests.jl:9-108 with Q1 and a u that Q1 holds."""
+    model = G.CartesianDiscreteModel((0, 1, 0, 1), part)
+    labels = model.get_face_labeling()
+    labels.add_tag_from_tags("dirichlet", [1, 2, 5])
+    labels.add_tag_from_tags("neumann", [7, 8])
+    labels.add_tag_from_tags("nitsche", 6)
+    h = 1.0 / part[0]
+    Ω = G.Triangulation(model)
+    Γn = G.BoundaryTriangulation(model, labels, tags="neumann")
+    Γd = G.BoundaryTriangulation(model, labels, tags="nitsche")
+    dΩ, dΓn, dΓd = G.Measure(Ω, 2), G.Measure(Γn, 2), G.Measure(Γd, 2)
+    nn, nd = G.get_normal_vector(Γn), G.get_normal_vector(Γd)
+    V = G.TestFESpace(model, G.ReferenceFE(G.lagrangian, G.Float64, 1),
+                      dirichlet_tags="dirichlet", labels=labels)
+    U = G.TrialFESpace(V, U_EX)
+    uh = G.interpolate(U_EX, U)
+    a = lambda u, v: (G.integrate(G.inner(G.gradient(v), G.gradient(u)), dΩ) +
+                      G.integrate((gam / h) * v * u - v * G.dot(nd, G.gradient(u)) -
+                                  G.dot(nd, G.gradient(v)) * u, dΓd))
+    l = lambda v: (G.integrate(0.0 * v, dΩ) +
+                   G.integrate(v * G.dot(nn, G.gradient(uh)), dΓn) +
+                   G.integrate((gam / h) * v * uh - G.dot(nd, G.gradient(v)) * uh, dΓd))
+    assem = G.SparseMatrixAssembler(U, V, path=path)
+    return G.AffineFEOperator(a, l, U, V, assem), V, model
+
+
+def oracle_driver(oracle, part, gam=10.0):
+    h = 1.0 / part[0]
+    U1 = lambda x: U_EX(np.asarray(x))
+    terms = [([7, 8], 0.0, 0.0, lambda x, n: GU_EX(x) @ n, None),
+             ([6], gam / h, -1.0, lambda x, n: gam / h * U1(x), lambda x, n: -U1(x))]
+    return oracle.poisson_q1_with_boundary((0, 1, 0, 1), part, [1, 2, 5], terms, g=U1,
+                                           fsrc=np.zeros(int(np.prod(part))))
+
+
+@pytest.mark.parametrize("path", ["auto", "general"])
+@pytest.mark.parametrize("part", [(4, 4), (13, 9)])
+def test_poisson_driver_matches_oracle(oracle, part, path):
+    announce("boundary terms PoissonTests driver %s path=%s" % (part, path))
+    op, V, model = poisson_driver(part, path)
+    ref = oracle_driver(oracle, part)
+    A, b = op.get_matrix(), op.get_vector()
+    np.testing.assert_array_equal(A.colptr, ref["colptr"])
+    np.testing.assert_array_equal(A.rowval, ref["rowval"])
+    assert relmax(A.nzval, ref["nzval"]) <= 1e-12
+    assert relmax(b, ref["b"]) <= 1e-12
+    info = op.assem.ctx.pattern_info()
+    assert info["lattice"] == 1          # the face kernel runs after either cell kernel
+
+
+def test_poisson_driver_solution():
+    announce("boundary terms PoissonTests driver 32x24 solve")
+    op, V, model = poisson_driver((32, 24))
+    uh = G.solve(op, G.PCGSolver(rtol=1e-12))
+    x = model.node_coordinates()[V.free_dof_to_node - 1]
+    uI = U_EX(x)
+    assert np.max(np.abs(uh.get_free_dof_values() - uI)) / np.max(np.abs(uI)) <= 1e-8
+
+
+def test_3d_robin_and_neumann_match_oracle(oracle):
+    """3-D 9x8x7: Robin (α = 3) on z = 0 and z = 1, a Neumann load g(x) on x = 1,
+    Dirichlet on x = 0, κ = 1, f = 1."""
+    part = (9, 8, 7)
+    announce("boundary terms 3-D Robin + Neumann %s" % (part,))
+    model = G.CartesianDiscreteModel((0, 1) * 3, part)
+    labels = model.get_face_labeling()
+    Ω = G.Triangulation(model)
+    Γr = G.BoundaryTriangulation(model, labels, tags=["tag_21", "tag_22"])
+    Γn = G.BoundaryTriangulation(model, labels, tags="tag_26")
+    dΩ, dΓr, dΓn = G.Measure(Ω, 2), G.Measure(Γr, 2), G.Measure(Γn, 2)
+    V = G.TestFESpace(model, G.ReferenceFE(G.lagrangian, G.Float64, 1), dirichlet_tags="tag_25")
+    U = G.TrialFESpace(V, lambda x: 1.0 + x[..., 1])
+    g = lambda x: np.sin(3 * x[..., 1]) + x[..., 2]
+    a = lambda u, v: (G.integrate(G.inner(G.gradient(v), G.gradient(u)), dΩ) +
+                      G.integrate(3.0 * v * u, dΓr))
+    l = lambda v: G.integrate(1.0 * v, dΩ) + G.integrate(v * g, dΓn)
+    op = G.AffineFEOperator(a, l, U, V)
+    gx = lambda x: 1.0 + x[1]
+    ref = oracle.poisson_q1_with_boundary(
+        (0, 1) * 3, part, [25],
+        [([21, 22], 3.0, 0.0, None, None),
+         ([26], 0.0, 0.0, lambda x, n: np.sin(3 * x[1]) + x[2], None)], g=gx)
+    A, b = op.get_matrix(), op.get_vector()
+    np.testing.assert_array_equal(A.rowval, ref["rowval"])
+    assert relmax(A.nzval, ref["nzval"]) <= 1e-12
+    assert relmax(b, ref["b"]) <= 1e-12

@@ -111,7 +111,9 @@ def test_newton_plaplacian_driver():
         assert np.max(np.abs(uh.free_values - uI)) <= 1e-9
         h = nls.history
         assert h[-1] < 1e-12 * h[0]
-        assert h[-1] <= 10.0 * h[-2] ** 2 / h[0] + 1e-15   # quadratic at the end
+        # quadratic convergence above the rounding floor: r_{k+1}/r_0 <= C (r_k/r_0)^2
+        tail = [x / h[0] for x in h if x > 1e-13 * h[0]]
+        assert len(tail) >= 2 and tail[-1] <= 1e3 * tail[-2] ** 2, h
 
 
 def test_newton_iterates_match_oracle(oracle):
@@ -130,7 +132,7 @@ def test_newton_iterates_match_oracle(oracle):
     xo = x.copy()
     pcg = G.PCGSolver(rtol=1e-13)
     r0 = None
-    for it in range(6):
+    for it in range(9):                # the oracle's Newton converges at iteration 8
         b, A = op.residual_and_jacobian(G.FEFunction(U, x))
         cp, rv, nz, r = _oracle_system(oracle, model, V, G.FEFunction(U, xo), 3.0, kappa, fsrc)
         r0 = np.abs(r).max() if r0 is None else r0
