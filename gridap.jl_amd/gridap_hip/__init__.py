@@ -30,6 +30,7 @@ from .geometry import (BoundaryTriangulation, CartesianDiscreteModel, FaceLabeli
 from .reference_fes import Float64, Quadrature, ReferenceFE, VectorValue, lagrangian
 from .nonlinear import (FEOperator, FESolver, NewtonRaphsonSolver, jacobian, residual,
                         residual_and_jacobian, solve_inplace)
+from .io import DiscreteModelFromFile, from_json, from_json_file, to_json, to_json_file
 
 __all__ = [
     "AffineFEOperator", "CartesianDiscreteModel", "CellField", "DomainContribution",
@@ -45,4 +46,5 @@ __all__ = [
     "FEOperator", "FESolver", "NewtonRaphsonSolver", "PLaplacian", "compose", "jacobian",
     "residual", "residual_and_jacobian", "solve_inplace",
     "BoundaryTriangulation", "get_normal_vector", "interpolate",
+    "DiscreteModelFromFile", "from_json", "from_json_file", "to_json", "to_json_file",
 ]

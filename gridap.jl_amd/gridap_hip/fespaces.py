@@ -24,7 +24,7 @@ class FESpace:
             raise TypeError("FESpace needs a ReferenceFE(lagrangian, T, order)")
         if conformity not in ("H1", ":H1"):
             raise NotImplementedError("only H1-conforming spaces are provided")
-        self.simplex = isinstance(model, UnstructuredDiscreteModel)
+        self.simplex = bool(model.is_simplex())
         self.conforming = self.simplex or reffe.order == 2
         if self.simplex:
             if reffe.order not in (1, 2) or reffe.ncomp != 1:

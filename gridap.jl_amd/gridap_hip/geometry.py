@@ -117,6 +117,9 @@ class CartesianDiscreteModel:
     def is_affine_cartesian(self):
         return self.map is None
 
+    def is_simplex(self):
+        return False
+
     # geometry ---------------------------------------------------------------
     def node_coordinates(self):
         if self._xyz is None:

@@ -63,7 +63,7 @@ def test_noalloc_pcg_matches_pcg(monkeypatch):
 
 def test_noalloc_pcg_limits():
     announce("noalloc pcg: size limit and FSAI refused")
-    big = _system((64, 64, 64), 3, tags=("boundary",))
+    big = _system((66, 66, 66), 3, tags=("boundary",))   # 65^3 = 274,625 free dofs
     assert big.assem.ctx.nfree > _lib.GH_NOALLOC_MAX_N
     with pytest.raises(G.GridapHipError) as e:
         big.assem.ctx.pcg_noalloc()
