@@ -350,6 +350,7 @@ struct CellArgs {
   int nq;
   const double *tab;  // per q: w, shp[NLOC], grad[NLOC*D]
   const double *kappa, *fsrc, *dvals;
+  const double *fq;  // f at the quadrature points (cells x nq) or nullptr
   const int32_t *colptr, *rowval;
   double *nzval, *b;
 };
@@ -425,7 +426,7 @@ __global__ __launch_bounds__(128) void cell_q1_kernel(const CellArgs A) {
         for (int r = 0; r < D; ++r) dot += g[i][r] * g[j][r];
         K[i][j] += dot * kd;
       }
-      bv[i] += (T)tq[1 + i] * ((T)fc * dV);
+      bv[i] += (T)tq[1 + i] * ((T)(A.fq ? A.fq[cell * A.nq + q] : fc) * dV);
     }
   }
   // Dirichlet lifting b -= K u_d on the cell.
@@ -498,6 +499,7 @@ void launch_general(Ctx &c, int precision) {
   A.kappa = c.have_kappa ? c.kappa.get() : nullptr;
   A.fsrc = c.have_fsrc ? c.fsrc.get() : nullptr;
   A.dvals = c.have_dvals ? c.dvals.get() : nullptr;
+  A.fq = c.have_fq ? c.fq.get() : nullptr;
   A.colptr = c.colptr.get();
   A.rowval = c.rowval.get();
   A.nzval = c.nzval.get();
@@ -534,8 +536,12 @@ void assemble_diffusion(Ctx &c, int precision, int path) {
     assemble_simplex(c, precision);
     return;
   }
+  GH_REQUIRE(!c.have_fq || c.fq.n == (size_t)c.ncells * c.nq, GH_ERR_STATE,
+             "the quadrature-point source was uploaded for another mesh or quadrature");
   double Kc[8] = {0}, b0 = 0.0;
-  bool lattice = c.structured && path != GH_PATH_GENERAL;
+  // the lattice kernel takes f constant per cell; f at the quadrature points runs the
+  // per-cell kernel
+  bool lattice = c.structured && path != GH_PATH_GENERAL && !c.have_fq;
   if (lattice) {
     GH_REQUIRE((int)c.shp.size() == c.nq * c.nloc, GH_ERR_STATE,
                "tabulation does not match the element (dofs per cell)");

@@ -57,6 +57,8 @@ void invalidate_space(gh::Ctx &c) {
   c.h_cell_dofs.clear();
   c.h_cell_dofs.shrink_to_fit();
   c.have_kappa = c.have_fsrc = c.have_dvals = false;
+  c.have_fq = false;
+  c.fq.release();
   c.have_state = false;
   c.nq = 0;
   c.nqf = 0;
@@ -255,6 +257,10 @@ int gh_reffe_tabulation(gh_ctx *ctx, int nq, int nloc, const double *qw, const d
     GH_REQUIRE(nq >= 1 && nq <= 512 && nloc >= 1 && nloc <= 256, GH_ERR_ARG,
                "tabulation sizes out of range");
     GH_REQUIRE(qw && shp && grad, GH_ERR_ARG, "NULL tabulation array");
+    if (nq != c.nq) {
+      c.have_fq = false;
+      c.fq.release();
+    }
     c.nq = nq;
     c.qw.assign(qw, qw + nq);
     c.shp.assign(shp, shp + size_t(nq) * nloc);
@@ -380,6 +386,24 @@ int gh_coef_upload(gh_ctx *ctx, const double *kappa, const double *fsrc,
     up(c.fsrc, c.have_fsrc, fsrc, c.ncells);
     up(c.dvals, c.have_dvals, dirichlet_vals, c.ndiri);
     GH_CHECK_HIP(hipStreamSynchronize(c.stream));
+  });
+}
+
+int gh_source_qp_upload(gh_ctx *ctx, const double *fq) {
+  return guarded(ctx, [&](gh::Ctx &c) {
+    GH_REQUIRE(c.have_space && c.nq > 0, GH_ERR_STATE,
+               "upload the FE space and the tabulation before the source");
+    c.have_fq = fq != nullptr;
+    c.have_values = false;
+    if (!fq) {
+      c.fq.release();
+      return;
+    }
+    GH_REQUIRE(c.reffe_kind == GH_REFFE_QK && c.order == 1 && c.ncomp == 1, GH_ERR_UNSUPPORTED,
+               "sources at quadrature points are provided for scalar Q1 spaces");
+    const size_t n = (size_t)c.ncells * c.nq;
+    c.fq.resize(n);
+    GH_CHECK_HIP(hipMemcpy(c.fq.get(), fq, sizeof(double) * n, hipMemcpyHostToDevice));
   });
 }
 

@@ -68,6 +68,7 @@ SIGNATURES = {
     "gh_pattern_get": (_i32, [_vp, _vp, _vp]),
     "gh_coef_upload": (_i32, [_vp, _vp, _vp, _vp]),
     "gh_assemble": (_i32, [_vp, _i32, _i32, _i32]),
+    "gh_source_qp_upload": (_i32, [_vp, _vp]),
     "gh_values_get": (_i32, [_vp, _vp, _vp]),
     "gh_values_set": (_i32, [_vp, _vp, _vp]),
     "gh_fefunction_upload": (_i32, [_vp, _vp, _vp]),
@@ -228,6 +229,11 @@ class Context:
         g = None if dirichlet_vals is None else np.ascontiguousarray(dirichlet_vals,
                                                                      dtype=np.float64)
         self.check(lib().gh_coef_upload(self.h, ptr(k), ptr(f), ptr(g)))
+
+    def source_qp_upload(self, fq=None):
+        """f at the quadrature points, [ncells, nq] (None: the per-cell f again)."""
+        a = None if fq is None else np.ascontiguousarray(fq, dtype=np.float64)
+        self.check(lib().gh_source_qp_upload(self.h, ptr(a)))
 
     def elasticity_params(self, nu, body_force=None):
         f = None if body_force is None else np.ascontiguousarray(body_force, dtype=np.float64)

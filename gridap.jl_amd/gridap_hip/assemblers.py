@@ -13,6 +13,7 @@ import numpy as np
 from . import _lib
 from ._lib import Context
 from .boundary import lower_boundary
+from .celldata import CellField, CoefProduct, FEValue, PointField
 from .celldata import (BodyForceTerm, DiffusionTerm, DomainContribution, ElasticityTerm,
                        SourceTerm, _cell_coef, get_fe_basis, get_trial_fe_basis)
 from .geometry import BoundaryTriangulation
@@ -79,6 +80,7 @@ class SparseMatrixAssembler:
         self.device_matrix = None
         self._face_degree = None    # degree of the face tabulation on the device
         self._has_boundary = False  # boundary terms uploaded
+        self._has_fq = False        # a source at the quadrature points uploaded
 
     def get_rows(self):
         return self.space.get_free_dof_ids()
@@ -133,6 +135,7 @@ class SparseMatrixAssembler:
         if had_mat and matcontrib is None:
             raise NotImplementedError("a bilinear form needs a cell (volume) term")
         kappa, fsrc, degs = None, None, set()
+        point_sources = []  # (coefficient, degree) of sources f(x) given as functions
         nu, force = None, None
         have_mat = have_vec = False
         kinds = set()
@@ -161,6 +164,11 @@ class SparseMatrixAssembler:
                     force = f if force is None else force + f
                 elif isinstance(term, SourceTerm):
                     kinds.add(_lib.GH_FORM_DIFFUSION)
+                    if term.coef is not None and not isinstance(term.coef, CellField):
+                        point_sources.append((term.coef, meas.degree))
+                        degs.add(meas.degree)
+                        have_vec = True
+                        continue
                     coef = _cell_coef(term.coef)
                     f = coef.per_cell(ncells) if coef is not None else None
                     f = np.ones(ncells) if f is None else f
@@ -193,6 +201,14 @@ class SparseMatrixAssembler:
             kappa = np.zeros(ncells)
         extra = dict(nu=0.3 if nu is None else nu,
                      force=np.zeros(3) if force is None else force)
+        if point_sources:
+            # f at the quadrature points: the per-cell sources join it, fsrc is unused
+            deg = next(iter(degs))
+            fq = sum(_eval_cell_points(cf, self.space, deg) for cf, _ in point_sources)
+            if fsrc is not None:
+                fq = fq + fsrc[:, None]
+            extra["fq"] = np.ascontiguousarray(fq)
+            fsrc = np.zeros(ncells)
         if mface or vface:
             if form != _lib.GH_FORM_DIFFUSION:
                 raise NotImplementedError("boundary terms are provided with the diffusion form")
@@ -227,6 +243,10 @@ class SparseMatrixAssembler:
             if dv is not None and dv.size and np.any(dv != 0):
                 dvals = dv
         self.ctx.coef_upload(kappa, fsrc, dvals)
+        fq = extra.get("fq")
+        if fq is not None or self._has_fq:
+            self.ctx.source_qp_upload(fq)
+            self._has_fq = fq is not None
         self._upload_boundary(extra.get("boundary"))
         if form == _lib.GH_FORM_ELASTICITY:
             self.ctx.elasticity_params(extra["nu"], extra["force"])
@@ -253,6 +273,41 @@ class SparseMatrixAssembler:
             raise ValueError("A does not have this assembler's sparsity pattern")
         self.ctx.values_set(np.ascontiguousarray(A.nzval, dtype=np.float64))
         self.device_matrix = A
+
+
+def _eval_cell_points(coef, V, degree):
+    """A coefficient at the quadrature points of every cell, [ncells, nq]: functions of
+    x at the physical points (CartesianMap, or the isoparametric map of the nodes), FE
+    functions by their cell values and the basis, per-cell values broadcast."""
+    model = V.model
+    if V.simplex or V.order != 1 or V.ncomp != 1:
+        raise NotImplementedError("sources f(x) are provided for scalar Q1 spaces")
+    D = model.D
+    q = Quadrature(D, degree)
+    shp, _ = V.reffe.tabulate(D, q.coords)                    # [q, a]
+    C, nq = model.num_cells(), len(q.weights)
+    if isinstance(coef, CoefProduct):
+        out = np.ones((C, nq))
+        for c in coef.factors:
+            out = out * _eval_cell_points(c, V, degree)
+        return out
+    if isinstance(coef, CellField):
+        pc = coef.per_cell(C)
+        return np.ones((C, nq)) * (1.0 if pc is None else pc[:, None])
+    if isinstance(coef, PointField):
+        X = model.node_coordinates()[model.cell_node_ids() - 1]   # [c, a, D]
+        x = np.einsum("qa,cad->cqd", shp, X)
+        val = np.asarray(coef.func(x.reshape(-1, D)), dtype=np.float64)
+        return np.broadcast_to(val.reshape(-1), (C * nq,)).reshape(C, nq).copy()
+    if isinstance(coef, FEValue):
+        uh = coef.uh
+        cd = np.asarray(V.cell_dof_ids, dtype=np.int64)
+        fv = np.asarray(uh.free_values, dtype=np.float64)
+        dv = np.asarray(uh.dirichlet_values, dtype=np.float64)
+        uc = np.where(cd > 0, fv[np.maximum(cd, 1) - 1],
+                      dv[np.maximum(-cd, 1) - 1] if dv.size else 0.0)
+        return uc @ shp.T
+    raise NotImplementedError("unsupported source coefficient %r" % (coef,))
 
 
 def _contrib(form, *args):

@@ -147,6 +147,13 @@ int gh_pattern_get(gh_ctx *ctx, int64_t *colptr, int64_t *rowval);
 int gh_coef_upload(gh_ctx *ctx, const double *kappa, const double *fsrc,
                    const double *dirichlet_vals);
 
+/* A source f(x) given at the quadrature points of every cell, fq[cell*nq + q] (the
+ * rule of the uploaded tabulation), in place of the per-cell f of gh_coef_upload
+ * (NULL: back to the per-cell f).  The x-dependent load of
+ * test/GridapTests/PoissonTests.jl:101-108 (l(v) = ∫ v f dΩ with f a function).
+ * Scalar Q1; the assembly then runs the per-cell kernel. */
+int gh_source_qp_upload(gh_ctx *ctx, const double *fq);
+
 /* Numeric phase on device-resident inputs: nzval and b in HBM, the matrix part of
  * assemble_matrix_and_vector! with attach_dirichlet lifting.  The work is enqueued
  * on the context stream; gh_values_get / gh_spmv / gh_pcg wait for it.

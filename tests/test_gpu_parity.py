@@ -327,3 +327,43 @@ def test_config2_128cube_properties():
     rs = S @ np.ones(S.shape[0])
     diag = S.diagonal()
     assert np.max(np.abs(rs[free_far]) / diag[free_far]) <= 1e-13
+
+
+@pytest.mark.parametrize("part,mapped", [((13, 11), False), ((9, 8, 7), False),
+                                         ((8, 7, 6), True)])
+def test_source_at_quadrature_points(oracle, part, mapped):
+    """l(v) = ∫ v f dΩ with f a function of x (PoissonTests.jl:101-108), f evaluated at
+    the physical quadrature points: nzval and b against the oracle's
+    cell_vectors_source_qp (FP64, 1e-12), Dirichlet data lifted."""
+    D = len(part)
+    announce("source f(x) at quadrature points %s mapped=%s" % (part, mapped))
+    f = lambda x: np.sin(3 * x[..., 0]) + x[..., 1] * x[..., -1]
+    g = lambda x: 1.0 + x[..., 0] - x[..., 1]
+    mp = (lambda x: x + 0.03 * np.sin(4 * x[..., ::-1])) if mapped else None
+    model = G.CartesianDiscreteModel((0, 1) * D, part, map=mp)
+    V = G.TestFESpace(model, G.ReferenceFE(G.lagrangian, G.Float64, 1),
+                      dirichlet_tags="boundary")
+    U = G.TrialFESpace(V, g)
+    Ω = G.Triangulation(model)
+    dΩ = G.Measure(Ω, 2)
+    kappa = np.exp(np.random.default_rng(D).normal(size=model.num_cells()))
+    k = G.CellField(kappa, Ω)
+    op = G.AffineFEOperator(lambda u, v: G.integrate(k * G.inner(G.gradient(v), G.gradient(u)), dΩ),
+                            lambda v: G.integrate(v * f, dΩ), U, V)
+    cn, xyz = model.cell_node_ids(), model.node_coordinates()
+    qx, qw = oracle.tensor_quadrature(D, 2)
+    shp, grad = oracle.qk_tabulate(D, 1, qx)
+    xq = oracle.physical_quadrature_points(D, cn, xyz, shp)
+    fq = f(xq)
+    cs = None if mapped else model.sizes
+    mats = (oracle.cell_matrices_diffusion(D, cn, xyz, qw, grad, kappa, cartesian_sizes=cs))
+    vecs = oracle.cell_vectors_source_qp(D, cn, xyz, qw, shp, grad, fq, cartesian_sizes=cs)
+    cd = V.cell_dof_ids
+    dv = U.dirichlet_values()
+    vecs = oracle.attach_dirichlet(cd, mats, dv, vecs)
+    cp, rv, nz = oracle.assemble_csc(cd, mats, V.nfree, V.nfree)
+    b = oracle.assemble_vector(cd, vecs, V.nfree)
+    A = op.get_matrix()
+    np.testing.assert_array_equal(A.rowval, rv)
+    assert relmax(A.nzval, nz) <= FP64_TOL
+    assert relmax(op.get_vector(), b) <= FP64_TOL

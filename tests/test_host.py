@@ -78,8 +78,14 @@ def test_weak_form_lowering():
     assert c2.terms[0][0].coef.values == 2.0
     s = G.integrate(v * 3.0, dΩ) + G.integrate(1.0 * v, dΩ)
     assert len(s.terms) == 2
+    # a source f(x) is evaluated at the quadrature points; a coefficient of a bilinear
+    # term must stay constant per cell
+    assem = G.SparseMatrixAssembler(G.TrialFESpace(V), V)
+    *_, extra = assem._lower(None, G.integrate(v * (lambda x: x[..., 0]), dΩ))
+    assert extra["fq"].shape == (6, 4)
     with pytest.raises(NotImplementedError):
-        G.integrate(v * (lambda x: x[0]), dΩ)
+        assem._lower(G.integrate((lambda x: x[..., 0]) * G.inner(G.gradient(v),
+                                                                 G.gradient(u)), dΩ), None)
     with pytest.raises(NotImplementedError):
         G.FESpace(model, G.ReferenceFE(G.lagrangian, G.Float64, 2))
 
