@@ -265,9 +265,11 @@ MFMA_FP64_PEAK = 78.6e12   # MI355X dense FP64 matrix, FLOP/s
 def elasticity_leg(n, ev, steps=5, warmup=1, seed=11):
     """Numeric assembly of 3-D linear elasticity with vector Q2 on n^3 hexes: E_c =
     exp(N(0, 0.5^2)) per cell, ν = 0.3, f = (0, 0, -1), x = 0 face clamped, degree-4
-    rule (27 points).  FP32 mode runs the element gradient products on the matrix
-    cores; the FP64 mode (FP64 matrix cores) is timed too and its values are the
-    reference of the FP32 mode's relative Frobenius error."""
+    rule (27 points).  Two numeric paths on one pattern, each in FP32 and FP64 mode:
+    the owner-computes lattice kernel (the default on a Cartesian mesh: E_c K̂ gathered
+    per column, no element matrices) and the element path (element gradient products
+    on the matrix cores, element matrices summed by the column-range kernel).  The
+    FP64 lattice values are the reference of every relative Frobenius error."""
     import gridap_hip as G
     from gridap_hip import _lib
     t = time.time()
@@ -290,56 +292,72 @@ def elasticity_leg(n, ev, steps=5, warmup=1, seed=11):
     t = time.time()
     nnz = ctx.symbolic()
     symbolic_s = time.time() - t
+    lattice = bool(ctx.pattern_info()["lattice"])
     ctx.coef_upload(E, None, None)
     ctx.elasticity_params(0.3, [0.0, 0.0, -1.0])
     stream = ctx.stream()
-    # FP64 mode (element products on the FP64 matrix cores): timed, and the values
-    # are the reference for the FP32 mode's error
-    ctx.assemble(_lib.GH_FORM_ELASTICITY, _lib.GH_PREC_FP64, _lib.GH_PATH_AUTO)
-    ev.sync(stream)
-    evs = ev.create(2)
-    ev.record(evs[0], stream)
-    for _ in range(steps):
-        ctx.assemble(_lib.GH_FORM_ELASTICITY, _lib.GH_PREC_FP64, _lib.GH_PATH_AUTO)
-    ev.record(evs[1], stream)
-    ms64 = ev.elapsed_ms(evs[0], evs[1]) / steps
-    ev.destroy(evs)
-    nz64, b64 = ctx.values_get()
-    for _ in range(warmup):
-        ctx.assemble(_lib.GH_FORM_ELASTICITY, _lib.GH_PREC_FP32_ELEM, _lib.GH_PATH_AUTO)
-    ev.sync(stream)
-    evs = ev.create(2)
-    ev.record(evs[0], stream)
-    for _ in range(steps):
-        ctx.assemble(_lib.GH_FORM_ELASTICITY, _lib.GH_PREC_FP32_ELEM, _lib.GH_PATH_AUTO)
-    ev.record(evs[1], stream)
-    ms = ev.elapsed_ms(evs[0], evs[1]) / steps
-    ev.destroy(evs)
-    nz32, b32 = ctx.values_get()
+
+    def timed(prec, path):
+        for _ in range(warmup):
+            ctx.assemble(_lib.GH_FORM_ELASTICITY, prec, path)
+        ev.sync(stream)
+        evs = ev.create(2)
+        ev.record(evs[0], stream)
+        for _ in range(steps):
+            ctx.assemble(_lib.GH_FORM_ELASTICITY, prec, path)
+        ev.record(evs[1], stream)
+        ms = ev.elapsed_ms(evs[0], evs[1]) / steps
+        ev.destroy(evs)
+        nz, b = ctx.values_get()
+        return ms, nz, b
+
+    F32, F64 = _lib.GH_PREC_FP32_ELEM, _lib.GH_PREC_FP64
+    AUTO, GEN = _lib.GH_PATH_AUTO, _lib.GH_PATH_GENERAL
+    ms_l64, nz_ref, b_ref = timed(F64, AUTO)
+    ms_l32, nz, b = timed(F32, AUTO)
+    err_l32 = float(np.linalg.norm(nz - nz_ref) / np.linalg.norm(nz_ref))
+    del nz
+    ms_e32, nz, b_e32 = timed(F32, GEN)
+    err_e32 = float(np.linalg.norm(nz - nz_ref) / np.linalg.norm(nz_ref))
+    del nz
+    ms_e64, nz, _ = timed(F64, GEN)
+    err_e64 = float(np.abs(nz - nz_ref).max() / np.abs(nz_ref).max())
+    del nz
     N = V.nfree
     ctx.close()
     flops = C * 27 * 2 * (6 * 6 * 81 + 81 * 6 * 81)      # dense Voigt BᵀDB count
     algo = 8 * nnz + 4 * 81 * C + 8 * C + 8 * N
+
+    def hbm(ms):
+        return {"bound": "hbm", "achieved": algo / (ms * 1e-3) / 1e9, "peak": HBM_PEAK / 1e9,
+                "unit": "GB/s", "frac": algo / (ms * 1e-3) / HBM_PEAK,
+                "algorithmic_bytes_per_launch": algo, "kernel_ms": ms}
+
+    def mfma(ms, peak):
+        return {"bound": "mfma", "achieved": flops / (ms * 1e-3) / 1e12, "peak": peak / 1e12,
+                "unit": "TFLOP/s", "frac": flops / (ms * 1e-3) / peak, "flops_per_launch": flops}
+
     return {
         "workload": "3-D linear elasticity, vector Q2 on %d^3 HEX, degree 4, E=exp(N(0,0.5^2)) "
-                    "per cell, nu=0.3, f=(0,0,-1), x=0 face clamped; FP32 element products "
-                    "on MFMA, FP64 nzval/b" % n,
+                    "per cell, nu=0.3, f=(0,0,-1), x=0 face clamped; FP32 element math, FP64 "
+                    "nzval/b" % n,
         "cells": C, "free_dofs": N, "nnz": int(nnz),
-        "ms_per_assembly": ms, "value": nnz / (ms * 1e-3), "unit": "NNZ/s",
-        "roofline_mfma_dense_count": {"bound": "mfma", "achieved": flops / (ms * 1e-3) / 1e12,
-                                      "peak": MFMA_FP32_PEAK / 1e12, "unit": "TFLOP/s",
-                                      "frac": flops / (ms * 1e-3) / MFMA_FP32_PEAK,
-                                      "flops_per_launch": flops},
-        "roofline_hbm": {"bound": "hbm", "achieved": algo / (ms * 1e-3) / 1e9,
-                         "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                         "frac": algo / (ms * 1e-3) / HBM_PEAK,
-                         "algorithmic_bytes_per_launch": algo},
-        "fp64_ms_per_assembly": ms64,
-        "fp64_roofline_mfma_dense_count": {"bound": "mfma", "achieved": flops / (ms64 * 1e-3) / 1e12,
-                                           "peak": MFMA_FP64_PEAK / 1e12, "unit": "TFLOP/s",
-                                           "frac": flops / (ms64 * 1e-3) / MFMA_FP64_PEAK},
-        "fp32_vs_fp64_rel_frobenius": float(np.linalg.norm(nz32 - nz64) / np.linalg.norm(nz64)),
-        "rhs_rel_diff": float(np.linalg.norm(b32 - b64) / np.linalg.norm(b64)),
+        "path": "lattice owner-computes (E_c K̂ per column, no element matrices)" if lattice
+                else "element matrices on MFMA + column-range sum",
+        "ms_per_assembly": ms_l32, "value": nnz / (ms_l32 * 1e-3), "unit": "NNZ/s",
+        "roofline_hbm": hbm(ms_l32),
+        "fp32_vs_fp64_rel_frobenius": err_l32,
+        "fp64_ms_per_assembly": ms_l64, "fp64_roofline_hbm": hbm(ms_l64),
+        "element_path_mfma": {
+            "ms_per_assembly": ms_e32, "value": nnz / (ms_e32 * 1e-3),
+            "roofline_mfma_dense_count": mfma(ms_e32, MFMA_FP32_PEAK),
+            "roofline_hbm": hbm(ms_e32),
+            "fp32_vs_fp64_lattice_rel_frobenius": err_e32,
+            "fp64_ms_per_assembly": ms_e64,
+            "fp64_roofline_mfma_dense_count": mfma(ms_e64, MFMA_FP64_PEAK),
+            "fp64_vs_fp64_lattice_rel_max": err_e64,
+            "rhs_rel_diff_fp32": float(np.linalg.norm(b_e32 - b_ref) / np.linalg.norm(b_ref)),
+        },
         "symbolic_s": symbolic_s, "host_numbering_s": host_s,
     }
 

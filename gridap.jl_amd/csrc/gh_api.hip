@@ -76,6 +76,9 @@ void invalidate_pattern(gh::Ctx &c) {
   c.fsai_built = false;
   c.elemlist = false;
   c.sref_ready = false;
+  c.q2lattice = false;
+  c.q2_ref_ready = false;
+  c.q2_node_dof.release();
   // state derived from the previous pattern (set again by the symbolic phase when it
   // applies to the new one)
   c.box = false;
@@ -267,6 +270,7 @@ int gh_reffe_tabulation(gh_ctx *ctx, int nq, int nloc, const double *qw, const d
     c.grad.assign(grad, grad + size_t(nq) * nloc * c.dim);
     c.have_values = false;
     c.sref_ready = false;
+    c.q2_ref_ready = false;
   });
 }
 
@@ -350,6 +354,7 @@ int gh_symbolic(gh_ctx *ctx, int64_t *nnz_out) {
       gh::check_structured(c);
     }
     gh::elemlist_symbolic(c);
+    gh::q2lattice_prepare(c);
     if (nnz_out) *nnz_out = c.nnz;
   });
 }
@@ -423,9 +428,7 @@ int gh_assemble(gh_ctx *ctx, int form_kind, int precision, int path) {
                  "the lattice path assembles linear diffusion only");
       gh::assemble_plaplacian(c, precision);
     } else if (form_kind == GH_FORM_ELASTICITY) {
-      GH_REQUIRE(path != GH_PATH_STRUCTURED, GH_ERR_UNSUPPORTED,
-                 "the lattice path assembles scalar diffusion only");
-      gh::assemble_elasticity(c, precision);
+      gh::assemble_elasticity(c, precision, path);
     } else {
       GH_REQUIRE(!gh::hexq2_eligible(c), GH_ERR_UNSUPPORTED,
                  "vector Q2 spaces assemble the elasticity form");
@@ -509,7 +512,7 @@ int gh_pattern_info_get(gh_ctx *ctx, gh_pattern_info *info) {
     gh_pattern_info r{};
     r.nnz = c.nnz;
     r.nranges = c.slots && c.bstart.n ? (int64_t)c.bstart.n - 1 : 0;
-    r.lattice = c.structured;
+    r.lattice = c.structured || c.q2lattice;
     r.box = c.box;
     r.elemlist = c.elemlist;
     r.compact = c.slots;

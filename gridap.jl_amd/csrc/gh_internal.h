@@ -155,6 +155,14 @@ struct Ctx {
   DBuf<double> sref;                // reference tensor Ĝ (6 x nloc x nloc) and m (nloc)
   bool sref_ready = false;          // sref matches the uploaded tabulation
 
+  // ---- vector Q2 on a Cartesian mesh, owner-computes (no element matrices) ----
+  bool q2lattice = false;           // the lattice kernel assembles this space
+  DBuf<int32_t> q2_node_dof;        // (2n+1)^3 lattice nodes x 3: signed dof ids
+  DBuf<double> q2_ref;              // K̂ (81 x 81, by column) and m̂ (27), FP64
+  DBuf<float> q2_ref32;             // K̂ in FP32
+  bool q2_ref_ready = false;
+  double q2_ref_key[4] = {0, 0, 0, 0};  // ν and cell sizes of q2_ref
+
   // ---- colouring for the general path ----
   int ncolors = 0;
   std::vector<int64_t> color_ptr;   // ncolors+1
@@ -243,7 +251,10 @@ void assemble_boundary(Ctx &c);  // adds the boundary-face terms to nzval and b
 void assemble_plaplacian(Ctx &c, int precision);  // residual -> b, Jacobian -> nzval
 // hexq2.hip
 bool hexq2_eligible(const Ctx &c);
-void assemble_elasticity(Ctx &c, int precision);
+void assemble_elasticity(Ctx &c, int precision, int path);
+// hexq2_lattice.hip
+void q2lattice_prepare(Ctx &c);                 // symbolic phase: node map, pattern check
+void q2lattice_assemble(Ctx &c, bool f32);      // numeric phase: nzval and b
 // solve.hip
 void spmv_device(Ctx &c, const double *x, double *y);
 void pcg_device(Ctx &c, int precond, double rtol, int maxit, gh_solve_stats *st);

@@ -329,12 +329,20 @@ bool hexq2_eligible(const Ctx &c) {
          c.nloc == kNloc && c.cartesian;
 }
 
-void assemble_elasticity(Ctx &c, int precision) {
-  GH_REQUIRE(hexq2_eligible(c) && c.elemlist, GH_ERR_UNSUPPORTED,
+void assemble_elasticity(Ctx &c, int precision, int path) {
+  GH_REQUIRE(hexq2_eligible(c) && (c.elemlist || c.q2lattice), GH_ERR_UNSUPPORTED,
              "elasticity is provided for vector Q2 on Cartesian hexahedra");
   GH_REQUIRE(c.nq == kNq && (int)c.shp.size() == kNq * kNn &&
                  (int)c.grad.size() == kNq * kNn * 3,
              GH_ERR_STATE, "the elasticity path needs the scalar Q2 tabulation at the 27-point rule");
+  if (c.q2lattice && path != GH_PATH_GENERAL) {
+    // owner-computes lattice kernel: no element matrices
+    q2lattice_assemble(c, precision == GH_PREC_FP32_ELEM);
+    return;
+  }
+  GH_REQUIRE(path != GH_PATH_STRUCTURED, GH_ERR_UNSUPPORTED,
+             "the vector-Q2 lattice path is not available for this space");
+  GH_REQUIRE(c.elemlist, GH_ERR_STATE, "no element lists for this space");
   if (!c.sref_ready) {
     // ĝ[q][a][d], w[q], φ̂[q][a] of the uploaded (scalar Q2) tabulation
     std::vector<double> ref((size_t)kNq * kNn * 3 + kNq + (size_t)kNq * kNn, 0.0);

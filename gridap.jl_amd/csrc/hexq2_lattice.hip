@@ -1,0 +1,403 @@
+// Vector-Q2 elasticity on a Cartesian mesh, owner-computes: the assembled matrix
+// without element matrices in memory.
+//
+// On the CartesianMap every cell has the Jacobian diag(h), so the element matrix of
+// cell c is K_c = E_c K̂ with one 81x81 reference matrix
+//     K̂[(a,i),(b,j)] = λ̂ P_ij[a][b] + μ̂ P_ji[a][b] + μ̂ δ_ij Σ_k P_kk[a][b],
+//     P_ij[a][b] = Σ_q w_q |J| g_ai(q) g_bj(q),   g = ĝ/h
+// (λ̂ = ν/((1+ν)(1-2ν)), μ̂ = 1/(2(1+ν)); the weak form ε(v) : σ(ε(u)) of
+// test/GridapTests/IsotropicDamageTests.jl:13-17), formed once in FP64 and made
+// exactly symmetric.  The Q2 nodes form a lattice of (2n+1)^3 points and the
+// conforming numbering gives each node's free components consecutive dof ids, so a
+// column (node n, component i) holds, in ascending row order, the free components of
+// the nodes of the cells around n sorted by their first free dof.  One wavefront
+// owns a node: it gathers the (at most 125) neighbour nodes, ranks them by dof id
+// (the sorted row order of create_from_nz, src/Algebra/SparseMatrixCSC.jl:264-283),
+// forms every entry as Σ_c E_c K̂[(a_c(m),k),(a_c(n),i)] over the cells shared by n and
+// m in ascending cell order (Gridap's cell loop), stages the column in LDS and writes
+// its nzval run once with coalesced stores.  The Dirichlet neighbours' entries give
+// the lifting, and b = f_i Σ_c ∫φ_a - lifting is written by the same wave.  No element
+// matrix, no contribution list, no atomics; deterministic.
+//
+// The symbolic phase builds the node -> dof map of the lattice from cell_dof_ids,
+// checks that every cell agrees with it, and checks once that the rows this kernel
+// produces are exactly the stored pattern (rowval); otherwise the element-list path
+// assembles the space.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "gh_internal.h"
+
+namespace gh {
+namespace {
+
+constexpr int kQ2Waves = 8;      // waves per workgroup (they share K̂ in LDS)
+constexpr int kQ2MaxNb = 125;    // neighbour nodes of a vertex node (5^3)
+constexpr int kQ2MaxRun = 3 * kQ2MaxNb;
+
+// Gridap's local Q2 node order on the HEX (8 vertices, 12 edges, 6 faces, the
+// interior; src/ReferenceFEs/CLagrangianRefFEs.jl:626-764) as lattice offsets
+// tx + 3 ty + 9 tz inside the cell, and back.
+__constant__ int8_t kLexOfLoc[27] = {
+    0, 2, 6, 8, 18, 20, 24, 26,                  // vertices (x fastest)
+    1, 7, 19, 25, 3, 5, 21, 23, 9, 11, 15, 17,   // edges: x-, y-, z-directed
+    4, 22, 10, 16, 12, 14,                       // faces z=0, z=1, y=0, y=1, x=0, x=1
+    13};                                         // interior
+__constant__ int8_t kLocOfLex[27] = {
+    0, 8, 1, 12, 20, 13, 2, 9, 3, 16, 22, 17, 24, 26, 25, 18, 23, 19,
+    4, 10, 5, 14, 21, 15, 6, 11, 7};
+
+struct Q2Args {
+  int nc[3];                 // cells per axis
+  int nn[3];                 // lattice nodes per axis (2 nc + 1)
+  int64_t nnodes;
+  const int32_t *node_dof;   // nnodes x 3, signed Gridap dof ids
+  const int32_t *colptr;
+  const int32_t *rowval;     // check mode only
+  const double *E;           // per cell, or nullptr (E = 1)
+  const void *Kref;          // 81 x 81 (T), [col (b,j)][row (a,i)]
+  const double *mref;        // 27: Σ_q w |J| φ_a
+  double body[3];
+  const double *dvals;       // Dirichlet values or nullptr
+  double *nzval;
+  double *b;
+  int *bad;                  // check mode: set when the rows differ from rowval
+};
+
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <class T, bool CHECK>
+__global__ __launch_bounds__(64 * kQ2Waves) void q2_lattice_kernel(const Q2Args P) {
+  __shared__ T K[81 * 81];
+  __shared__ int keys[kQ2Waves][128];
+  __shared__ int nfcs[kQ2Waves][128];
+  __shared__ __align__(16) double stages[kQ2Waves][kQ2MaxRun + 2];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (!CHECK) {
+    const T *Kg = static_cast<const T *>(P.Kref);
+    for (int t = threadIdx.x; t < 81 * 81; t += blockDim.x) K[t] = Kg[t];
+    __syncthreads();
+  }
+  int *key = keys[wave];
+  int *nfc = nfcs[wave];
+  double *stage = stages[wave];
+  const int N0 = P.nn[0], N1 = P.nn[1];
+  for (int64_t node = (int64_t)blockIdx.x * kQ2Waves + wave; node < P.nnodes;
+       node += (int64_t)gridDim.x * kQ2Waves) {
+    int own[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) own[k] = P.node_dof[node * 3 + k];
+    if (own[0] <= 0 && own[1] <= 0 && own[2] <= 0) continue;  // wave-uniform
+    const int X = (int)(node % N0), Y = (int)((node / N0) % N1), Z = (int)(node / ((int64_t)N0 * N1));
+    const int c[3] = {X, Y, Z};
+    // the cells around the node (per axis: one if the coordinate is odd, else two)
+    int clo[3], chi[3], lo[3], nb[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      if (c[k] & 1) {
+        clo[k] = chi[k] = c[k] >> 1;
+      } else {
+        clo[k] = max((c[k] >> 1) - 1, 0);
+        chi[k] = min(c[k] >> 1, P.nc[k] - 1);
+      }
+      lo[k] = 2 * clo[k];
+      nb[k] = 2 * (chi[k] - clo[k]) + 3;
+    }
+    const int nbox = nb[0] * nb[1] * nb[2];
+    // neighbour slots s = lane and lane + 64
+    int md[2][3], mx[2][3];
+    bool valid[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int s = lane + 64 * h;
+      valid[h] = s < nbox;
+      int kk = 0, nf = 0;
+      if (valid[h]) {
+        mx[h][0] = lo[0] + s % nb[0];
+        mx[h][1] = lo[1] + (s / nb[0]) % nb[1];
+        mx[h][2] = lo[2] + s / (nb[0] * nb[1]);
+        const int64_t m = mx[h][0] + (int64_t)N0 * (mx[h][1] + (int64_t)N1 * mx[h][2]);
+        kk = INT_MAX;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          md[h][k] = P.node_dof[m * 3 + k];
+          if (md[h][k] > 0) {
+            kk = min(kk, md[h][k]);
+            ++nf;
+          }
+        }
+      } else {
+        kk = INT_MAX;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) md[h][k] = 0, mx[h][k] = 0;
+      }
+      key[s] = kk;
+      nfc[s] = nf;
+    }
+    wave_sync();
+    // position of each neighbour's first free component in the column: the free
+    // components of every neighbour with a smaller dof id come first
+    int pos[2] = {0, 0}, total = 0;
+    const int mine[2] = {key[lane], key[lane + 64]};
+    for (int t = 0; t < nbox; ++t) {
+      const int kt = key[t], ft = nfc[t];
+      total += ft;
+#pragma unroll
+      for (int h = 0; h < 2; ++h) pos[h] += kt < mine[h] ? ft : 0;
+    }
+    double ev[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) ev[e] = 0.0;
+    {
+      int e = 0;
+      for (int cz = clo[2]; cz <= chi[2]; ++cz)
+        for (int cy = clo[1]; cy <= chi[1]; ++cy)
+          for (int cx = clo[0]; cx <= chi[0]; ++cx, ++e)
+            ev[e] = P.E ? P.E[cx + (int64_t)P.nc[0] * (cy + (int64_t)P.nc[1] * cz)] : 1.0;
+    }
+#pragma unroll 1
+    for (int i = 0; i < 3; ++i) {
+      const int j = own[i];
+      if (j <= 0) continue;  // wave-uniform
+      const int p0 = P.colptr[j - 1];
+      const int sh = p0 & 1;  // stage[sh + t] <-> nzval[p0 + t]: 16-byte aligned chunks
+      double lift = 0.0;
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        if (!valid[h]) continue;
+        double v[3] = {0.0, 0.0, 0.0};
+        int e = 0;
+        for (int cz = clo[2]; cz <= chi[2]; ++cz)
+          for (int cy = clo[1]; cy <= chi[1]; ++cy)
+            for (int cx = clo[0]; cx <= chi[0]; ++cx, ++e) {
+              const int tx = mx[h][0] - 2 * cx, ty = mx[h][1] - 2 * cy, tz = mx[h][2] - 2 * cz;
+              if (tx < 0 || tx > 2 || ty < 0 || ty > 2 || tz < 0 || tz > 2) continue;
+              if (CHECK) continue;
+              const int am = kLocOfLex[tx + 3 * ty + 9 * tz];
+              const int an = kLocOfLex[(X - 2 * cx) + 3 * (Y - 2 * cy) + 9 * (Z - 2 * cz)];
+              const T *col = &K[(an + 27 * i) * 81];
+              const T Ec = (T)ev[e];
+#pragma unroll
+              for (int k = 0; k < 3; ++k) v[k] += (double)(Ec * col[am + 27 * k]);
+            }
+        int r = 0;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          const int d = md[h][k];
+          if (d > 0) {
+            if (CHECK) {
+              const int at = p0 + pos[h] + r;
+              if (at >= P.colptr[j] || P.rowval[at] != d - 1) *P.bad = 1;
+            } else {
+              stage[sh + pos[h] + r] = v[k];
+            }
+            ++r;
+          } else if (d < 0 && P.dvals) {
+            lift += v[k] * P.dvals[-d - 1];
+          }
+        }
+      }
+      if (CHECK) {
+        if (lane == 0 && P.colptr[j] - p0 != total) *P.bad = 1;
+        continue;
+      }
+      wave_sync();
+      double2 *dst2 = reinterpret_cast<double2 *>(P.nzval + (p0 - sh));
+      const double2 *st2 = reinterpret_cast<const double2 *>(stage);
+      const int lo2 = sh, hi2 = sh + total;
+      for (int t = (lo2 + 1) / 2 + lane; t < hi2 / 2; t += 64) dst2[t] = st2[t];
+      if (lane == 0 && (lo2 & 1)) P.nzval[p0] = stage[lo2];
+      if (lane == 1 && (hi2 & 1)) P.nzval[p0 + total - 1] = stage[hi2 - 1];
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) lift += __shfl_xor(lift, o);
+      if (lane == 0) {
+        double load = 0.0;
+        int e = 0;
+        for (int cz = clo[2]; cz <= chi[2]; ++cz)
+          for (int cy = clo[1]; cy <= chi[1]; ++cy)
+            for (int cx = clo[0]; cx <= chi[0]; ++cx, ++e)
+              load += P.mref[kLocOfLex[(X - 2 * cx) + 3 * (Y - 2 * cy) + 9 * (Z - 2 * cz)]];
+        P.b[j - 1] = P.body[i] * load - lift;
+      }
+      wave_sync();  // the stage is free for the next component
+    }
+    wave_sync();    // key/nfc are free for the next node
+  }
+}
+
+// node -> dof map of the lattice from cell_dof_ids (cells agree on shared nodes; the
+// check kernel below verifies it)
+__global__ void q2_node_scatter(int64_t ncells, int nc0, int nc1, int N0, int N1,
+                                const int32_t *__restrict__ cell_dofs,
+                                int32_t *__restrict__ node_dof) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ncells * 27) return;
+  const int64_t cell = t / 27;
+  const int a = (int)(t % 27);
+  const int cx = (int)(cell % nc0), cy = (int)((cell / nc0) % nc1), cz = (int)(cell / ((int64_t)nc0 * nc1));
+  const int lx = kLexOfLoc[a];
+  const int64_t node = (2 * cx + lx % 3) + (int64_t)N0 * ((2 * cy + (lx / 3) % 3) +
+                                                         (int64_t)N1 * (2 * cz + lx / 9));
+#pragma unroll
+  for (int k = 0; k < 3; ++k) node_dof[node * 3 + k] = cell_dofs[cell * 81 + a + 27 * k];
+}
+
+__global__ void q2_node_verify(int64_t ncells, int nc0, int nc1, int N0, int N1,
+                               const int32_t *__restrict__ cell_dofs,
+                               const int32_t *__restrict__ node_dof, int *bad) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ncells * 27) return;
+  const int64_t cell = t / 27;
+  const int a = (int)(t % 27);
+  const int cx = (int)(cell % nc0), cy = (int)((cell / nc0) % nc1), cz = (int)(cell / ((int64_t)nc0 * nc1));
+  const int lx = kLexOfLoc[a];
+  const int64_t node = (2 * cx + lx % 3) + (int64_t)N0 * ((2 * cy + (lx / 3) % 3) +
+                                                         (int64_t)N1 * (2 * cz + lx / 9));
+#pragma unroll
+  for (int k = 0; k < 3; ++k)
+    if (node_dof[node * 3 + k] != cell_dofs[cell * 81 + a + 27 * k]) *bad = 1;
+}
+
+Q2Args make_args(const Ctx &c) {
+  Q2Args A{};
+  for (int k = 0; k < 3; ++k) {
+    A.nc[k] = (int)c.part[k];
+    A.nn[k] = 2 * (int)c.part[k] + 1;
+  }
+  A.nnodes = (int64_t)A.nn[0] * A.nn[1] * A.nn[2];
+  A.node_dof = c.q2_node_dof.get();
+  A.colptr = c.colptr.get();
+  A.rowval = c.rowval.get();
+  return A;
+}
+
+unsigned q2_grid(int64_t nnodes, bool f64) {
+  const int64_t want = (nnodes + kQ2Waves - 1) / kQ2Waves;
+  return (unsigned)std::max<int64_t>(1, std::min<int64_t>(want, f64 ? 256 : 512));
+}
+
+}  // namespace
+
+// Symbolic phase of the lattice path (a Cartesian vector-Q2 space with its pattern):
+// the node -> dof map, its agreement with every cell, and the produced rows against
+// rowval.  Sets c.q2lattice.
+void q2lattice_prepare(Ctx &c) {
+  c.q2lattice = false;
+  c.q2_node_dof.release();
+  const char *env = std::getenv("GRIDAP_HIP_Q2_LATTICE");
+  if (env && env[0] == '0') return;
+  if (!(c.cartesian && c.dim == 3 && c.reffe_kind == GH_REFFE_QK && c.order == 2 &&
+        c.ncomp == 3 && c.nloc == 81 && c.have_pattern && c.nfree > 0))
+    return;
+  const int64_t nn = (2 * c.part[0] + 1) * (2 * c.part[1] + 1) * (2 * c.part[2] + 1);
+  if (nn * 3 >= (int64_t(1) << 31)) return;
+  hipStream_t s = c.stream;
+  c.q2_node_dof.resize(nn * 3);
+  GH_CHECK_HIP(hipMemsetAsync(c.q2_node_dof.get(), 0, sizeof(int32_t) * nn * 3, s));
+  Q2Args A = make_args(c);
+  const int64_t work = c.ncells * 27;
+  const unsigned g = (unsigned)((work + 255) / 256);
+  hipLaunchKernelGGL(q2_node_scatter, dim3(g), dim3(256), 0, s, c.ncells, A.nc[0], A.nc[1],
+                     A.nn[0], A.nn[1], c.cell_dofs.get(), c.q2_node_dof.get());
+  DBuf<int> bad;
+  bad.resize(1);
+  GH_CHECK_HIP(hipMemsetAsync(bad.get(), 0, sizeof(int), s));
+  hipLaunchKernelGGL(q2_node_verify, dim3(g), dim3(256), 0, s, c.ncells, A.nc[0], A.nc[1],
+                     A.nn[0], A.nn[1], c.cell_dofs.get(), c.q2_node_dof.get(), bad.get());
+  A.bad = bad.get();
+  hipLaunchKernelGGL((q2_lattice_kernel<float, true>), dim3(q2_grid(A.nnodes, false)),
+                     dim3(64 * kQ2Waves), 0, s, A);
+  GH_CHECK_HIP(hipGetLastError());
+  int hbad = 1;
+  GH_CHECK_HIP(hipMemcpyAsync(&hbad, bad.get(), sizeof(int), hipMemcpyDeviceToHost, s));
+  GH_CHECK_HIP(hipStreamSynchronize(s));
+  if (hbad) {
+    c.q2_node_dof.release();
+    return;
+  }
+  c.q2lattice = true;
+}
+
+// The numeric phase on the lattice: K̂ and m̂ from the uploaded tabulation (FP64, once
+// per tabulation, cell sizes and ν), then one launch writes nzval and b.
+void q2lattice_assemble(Ctx &c, bool f32) {
+  GH_REQUIRE(c.q2lattice, GH_ERR_STATE, "no vector-Q2 lattice for this space");
+  const double key[4] = {c.nu, c.sizes[0], c.sizes[1], c.sizes[2]};
+  const bool fresh = !(c.q2_ref_ready && std::equal(key, key + 4, c.q2_ref_key));
+  if (fresh) {
+    const int nq = c.nq, NN = 27;
+    const double vol = c.sizes[0] * c.sizes[1] * c.sizes[2];
+    std::vector<double> g((size_t)nq * NN * 3);
+    for (int q = 0; q < nq; ++q)
+      for (int a = 0; a < NN; ++a)
+        for (int d = 0; d < 3; ++d)
+          g[((size_t)q * NN + a) * 3 + d] = c.grad[((size_t)q * NN + a) * 3 + d] / c.sizes[d];
+    std::vector<double> P((size_t)9 * NN * NN, 0.0);  // P[(i*3+j)][a][b]
+    for (int q = 0; q < nq; ++q) {
+      const double w = c.qw[q] * vol;
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+          for (int a = 0; a < NN; ++a) {
+            const double ga = w * g[((size_t)q * NN + a) * 3 + i];
+            for (int b = 0; b < NN; ++b)
+              P[((size_t)(i * 3 + j) * NN + a) * NN + b] += ga * g[((size_t)q * NN + b) * 3 + j];
+          }
+    }
+    const double lam = c.nu / ((1.0 + c.nu) * (1.0 - 2.0 * c.nu));
+    const double mu = 1.0 / (2.0 * (1.0 + c.nu));
+    std::vector<double> K((size_t)81 * 81);
+    auto Pij = [&](int i, int j, int a, int b) { return P[((size_t)(i * 3 + j) * NN + a) * NN + b]; };
+    for (int i = 0; i < 3; ++i)
+      for (int a = 0; a < NN; ++a)
+        for (int j = 0; j < 3; ++j)
+          for (int b = 0; b < NN; ++b) {
+            double v = lam * Pij(i, j, a, b) + mu * Pij(j, i, a, b);
+            if (i == j) v += mu * (Pij(0, 0, a, b) + Pij(1, 1, a, b) + Pij(2, 2, a, b));
+            K[(size_t)(a + 27 * i) * 81 + (b + 27 * j)] = v;  // [row][col]
+          }
+    // exactly symmetric; stored by column ([col][row]) for the kernel
+    std::vector<double> Kc((size_t)81 * 81);
+    std::vector<float> Kf((size_t)81 * 81);
+    for (int r = 0; r < 81; ++r)
+      for (int cc = 0; cc < 81; ++cc) {
+        const double v = 0.5 * (K[(size_t)r * 81 + cc] + K[(size_t)cc * 81 + r]);
+        Kc[(size_t)cc * 81 + r] = v;
+        Kf[(size_t)cc * 81 + r] = (float)v;
+      }
+    std::vector<double> m(NN, 0.0);
+    for (int q = 0; q < nq; ++q)
+      for (int a = 0; a < NN; ++a) m[a] += c.qw[q] * vol * c.shp[(size_t)q * NN + a];
+    c.q2_ref.resize(81 * 81 + NN);
+    c.q2_ref32.resize(81 * 81);
+    GH_CHECK_HIP(hipMemcpy(c.q2_ref.get(), Kc.data(), sizeof(double) * 81 * 81,
+                           hipMemcpyHostToDevice));
+    GH_CHECK_HIP(hipMemcpy(c.q2_ref.get() + 81 * 81, m.data(), sizeof(double) * NN,
+                           hipMemcpyHostToDevice));
+    GH_CHECK_HIP(hipMemcpy(c.q2_ref32.get(), Kf.data(), sizeof(float) * 81 * 81,
+                           hipMemcpyHostToDevice));
+    std::copy(key, key + 4, c.q2_ref_key);
+    c.q2_ref_ready = true;
+  }
+  Q2Args A = make_args(c);
+  A.E = c.have_kappa ? c.kappa.get() : nullptr;
+  A.Kref = f32 ? (const void *)c.q2_ref32.get() : (const void *)c.q2_ref.get();
+  A.mref = c.q2_ref.get() + 81 * 81;
+  for (int k = 0; k < 3; ++k) A.body[k] = c.body[k];
+  A.dvals = c.have_dvals ? c.dvals.get() : nullptr;
+  A.nzval = c.nzval.get();
+  A.b = c.b.get();
+  if (f32)
+    hipLaunchKernelGGL((q2_lattice_kernel<float, false>), dim3(q2_grid(A.nnodes, false)),
+                       dim3(64 * kQ2Waves), 0, c.stream, A);
+  else
+    hipLaunchKernelGGL((q2_lattice_kernel<double, false>), dim3(q2_grid(A.nnodes, true)),
+                       dim3(64 * kQ2Waves), 0, c.stream, A);
+  GH_CHECK_HIP(hipGetLastError());
+}
+
+}  // namespace gh

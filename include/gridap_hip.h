@@ -192,7 +192,8 @@ int gh_values_set(gh_ctx *ctx, const double *nzval, const double *b);
 typedef struct gh_pattern_info {
   int64_t nnz;
   int64_t nranges;     /* column ranges of the element-list column sum (0: not used)   */
-  int32_t lattice;     /* 1: owner-computes lattice kernel (Cartesian scalar Q1)       */
+  int32_t lattice;     /* 1: owner-computes lattice kernel (Cartesian scalar Q1, or
+                          vector Q2 elasticity without element matrices)            */
   int32_t box;         /* 1: free dofs form a box of lattice nodes (stencil PCG)       */
   int32_t elemlist;    /* 1: element-list path (simplices, vector Q2 hexahedra)        */
   int32_t compact;     /* 1: compact 16-bit contribution lists (column-range sum);

@@ -34,7 +34,8 @@ def relfrob(a, b):
     return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
 
 
-def build(part, E, nu=0.3, force=(0.0, 0.0, -1.0), tags=CLAMP_X0, g=None, precision="fp64"):
+def build(part, E, nu=0.3, force=(0.0, 0.0, -1.0), tags=CLAMP_X0, g=None, precision="fp64",
+          path="auto"):
     model = G.CartesianDiscreteModel((0, 1, 0, 1, 0, 1), part)
     reffe = G.ReferenceFE(G.lagrangian, G.VectorValue[3, G.Float64], 2)
     V = G.TestFESpace(model, reffe, dirichlet_tags=list(tags))
@@ -44,7 +45,7 @@ def build(part, E, nu=0.3, force=(0.0, 0.0, -1.0), tags=CLAMP_X0, g=None, precis
     σ = G.IsotropicStress(G.CellField(E, Ω) if not np.isscalar(E) else E, nu)
     a = lambda u, v: G.integrate(G.inner(G.ε(v), σ(G.ε(u))), dΩ)
     l = lambda v: G.integrate(G.dot(v, G.VectorValue(*force)), dΩ)
-    assem = G.SparseMatrixAssembler(U, V, precision=precision)
+    assem = G.SparseMatrixAssembler(U, V, precision=precision, path=path)
     return G.AffineFEOperator(a, l, U, V, assem), V
 
 
@@ -61,12 +62,14 @@ def oracle_system(oracle, part, E, nu, force, tags, g=None):
     return r, cp, rv, nz, b
 
 
+@pytest.mark.parametrize("path", ["auto", "general"])
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
-def test_elasticity_matches_oracle(oracle, precision):
+def test_elasticity_matches_oracle(oracle, precision, path):
     part = (3, 2, 2)
-    announce("elasticity Q2 %s %s clamped x=0" % (part, precision))
+    announce("elasticity Q2 %s %s clamped x=0 path=%s" % (part, precision, path))
     E = np.exp(np.random.default_rng(3).normal(0, 0.5, 12))
-    op, V = build(part, E, precision=precision)
+    op, V = build(part, E, precision=precision, path=path)
+    assert op.assem.ctx.pattern_info()["lattice"] == (path == "auto")
     r, cp, rv, nz, b = oracle_system(oracle, part, E, 0.3, (0, 0, -1), CLAMP_X0)
     np.testing.assert_array_equal(V.cell_dof_ids, r["cell_dofs"])
     A = op.get_matrix()
@@ -141,27 +144,30 @@ def test_elasticity_column_sum_matches_list_sum(monkeypatch):
     E = np.exp(np.random.default_rng(7).normal(0, 0.5, 60))
     for precision in ("fp32", "fp64"):
         monkeypatch.setenv("GRIDAP_HIP_COMPACT_LISTS", "1")
-        op1, _ = build(part, E, precision=precision)
+        op1, _ = build(part, E, precision=precision, path="general")
         monkeypatch.setenv("GRIDAP_HIP_COMPACT_LISTS", "0")
-        op2, _ = build(part, E, precision=precision)
+        op2, _ = build(part, E, precision=precision, path="general")
         assert op1.assem.ctx.pattern_info()["compact"] == 1
         assert op2.assem.ctx.pattern_info()["compact"] == 0
         np.testing.assert_array_equal(op1.get_matrix().nzval, op2.get_matrix().nzval)
         np.testing.assert_array_equal(op1.get_vector(), op2.get_vector())
 
 
+@pytest.mark.parametrize("path", ["auto", "general"])
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
-def test_elasticity_17cube_matches_oracle(oracle, precision):
+def test_elasticity_17cube_matches_oracle(oracle, precision, path):
     """4913 cells: more cells than waves in the FP32 element kernel's grid (1024
-    workgroups x 4 waves), so waves walk several cells in both modes; the whole
+    workgroups x 4 waves), so waves walk several cells in both modes (general path);
+    the lattice kernel walks 35^3 nodes with persistent waves (auto path); the whole
     system against the oracle."""
     part = (17, 17, 17)
-    announce("elasticity Q2 %s %s vs oracle" % (part, precision))
+    announce("elasticity Q2 %s %s vs oracle path=%s" % (part, precision, path))
     C = int(np.prod(part))
     E = np.exp(np.random.default_rng(17).normal(0, 0.5, C))
-    op, V = build(part, E, precision=precision)
+    op, V = build(part, E, precision=precision, path=path)
     info = op.assem.ctx.pattern_info()
     assert info["elemlist"] == 1 and info["compact"] == 1 and info["inverse"] == 1
+    assert info["lattice"] == 1
     r, cp, rv, nz, b = oracle_system(oracle, part, E, 0.3, (0, 0, -1), CLAMP_X0)
     np.testing.assert_array_equal(V.cell_dof_ids, r["cell_dofs"])
     A = op.get_matrix()
@@ -187,16 +193,24 @@ def test_elasticity_48cube_fp32_vs_fp64():
     announce("elasticity Q2 48^3 fp32 vs fp64 (config 3)")
     E = _config3_E(part)
     op64, V = build(part, E, precision="fp64")
+    assert op64.assem.ctx.pattern_info()["lattice"] == 1
     A64, b64 = op64.get_matrix(), op64.get_vector()
     assert A64.nnz() == 506929500 and V.num_free_dofs() == 2709792
     op64.assem.ctx.close()
-    announce("elasticity Q2 48^3 fp32 leg")
-    op32, _ = build(part, E, precision="fp32")
-    A32, b32 = op32.get_matrix(), op32.get_vector()
-    np.testing.assert_array_equal(A32.rowval, A64.rowval)
-    assert relfrob(A32.nzval, A64.nzval) <= 1e-6
-    assert relfrob(b32, b64) <= 1e-6
-    assert not np.array_equal(A32.nzval, A64.nzval)
+    for prec, path in (("fp32", "auto"), ("fp64", "general"), ("fp32", "general")):
+        announce("elasticity Q2 48^3 %s path=%s against the FP64 lattice" % (prec, path))
+        op, _ = build(part, E, precision=prec, path=path)
+        A, b = op.get_matrix(), op.get_vector()
+        op.assem.ctx.close()
+        np.testing.assert_array_equal(A.rowval, A64.rowval)
+        if prec == "fp64":
+            assert relmax(A.nzval, A64.nzval) <= 1e-12   # the MFMA element path, FP64
+            assert relmax(b, b64) <= 1e-12
+        else:
+            assert relfrob(A.nzval, A64.nzval) <= 1e-6
+            assert relfrob(b, b64) <= 1e-6
+            assert not np.array_equal(A.nzval, A64.nzval)
+        del A, b
 
 
 @pytest.mark.parametrize("precision", ["fp32", "fp64"])
@@ -237,3 +251,22 @@ def test_elasticity_48cube_quadratic_patch(precision):
     # relative to max|A| max|u_I|: FP32 entries carry ~1e-7 relative rounding each
     rel = np.max(np.abs(r)) / (np.max(np.abs(op.get_matrix().nzval)) * np.max(np.abs(uI)))
     assert rel <= (2e-5 if precision == "fp32" else 1e-12), rel
+
+
+def test_elasticity_lattice_symmetric_and_switch(monkeypatch):
+    """The lattice kernel's matrix is symmetric bit for bit (K̂ exactly symmetric, both
+    entries of a pair sum the same cells in the same order); GRIDAP_HIP_Q2_LATTICE=0
+    leaves the element-list path, which agrees to rounding."""
+    part = (6, 5, 4)
+    announce("elasticity Q2 lattice symmetry and switch %s" % (part,))
+    E = np.exp(np.random.default_rng(6).normal(0, 0.5, int(np.prod(part))))
+    op, V = build(part, E, precision="fp32")
+    S = op.get_matrix().to_scipy()
+    assert (S != S.T).nnz == 0
+    monkeypatch.setenv("GRIDAP_HIP_Q2_LATTICE", "0")
+    op2, _ = build(part, E, precision="fp64")
+    assert op2.assem.ctx.pattern_info()["lattice"] == 0
+    monkeypatch.delenv("GRIDAP_HIP_Q2_LATTICE")
+    op3, _ = build(part, E, precision="fp64")
+    assert relmax(op3.get_matrix().nzval, op2.get_matrix().nzval) <= 1e-12
+    assert relmax(op3.get_vector(), op2.get_vector()) <= 1e-12
