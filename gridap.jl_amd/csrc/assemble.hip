@@ -309,6 +309,7 @@ bool lattice_reference_symmetric(const Ctx &c, double *Kc, double *b0) {
 
 template <int D>
 void launch_lattice(Ctx &c, const double *Kc, double b0) {
+  c.last_kernel = GH_KERNEL_Q1_LATTICE;
   LatticeArgs P{};
   for (int k = 0; k < 3; ++k) {
     P.nn[k] = (int)(c.part[k] + 1);
@@ -465,6 +466,7 @@ __global__ __launch_bounds__(128) void cell_q1_kernel(const CellArgs A) {
 }
 
 void launch_general(Ctx &c, int precision) {
+  c.last_kernel = GH_KERNEL_CELL;
   GH_REQUIRE(c.reffe_kind == GH_REFFE_QK && c.order == 1 && c.ncomp == 1 &&
                  c.nloc == (1 << c.dim),
              GH_ERR_UNSUPPORTED, "general path supports scalar Q1 on quads/hexes");

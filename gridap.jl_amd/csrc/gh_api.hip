@@ -69,6 +69,7 @@ void invalidate_space(gh::Ctx &c) {
 void invalidate_pattern(gh::Ctx &c) {
   c.have_pattern = false;
   c.have_values = false;
+  c.last_kernel = GH_KERNEL_NONE;
   c.structured = false;
   c.ncolors = 0;
   c.color_ptr.clear();
@@ -518,6 +519,7 @@ int gh_pattern_info_get(gh_ctx *ctx, gh_pattern_info *info) {
     r.compact = c.slots;
     r.metric = c.metric;
     r.inverse = c.inverse;
+    r.last_kernel = c.last_kernel;
     *info = r;
   });
 }

@@ -156,6 +156,7 @@ struct Ctx {
   bool sref_ready = false;          // sref matches the uploaded tabulation
 
   // ---- vector Q2 on a Cartesian mesh, owner-computes (no element matrices) ----
+  int last_kernel = 0;              // GH_KERNEL_* of the last numeric phase
   bool q2lattice = false;           // the lattice kernel assembles this space
   DBuf<int32_t> q2_node_dof;        // (2n+1)^3 lattice nodes x 3: signed dof ids
   DBuf<double> q2_ref;              // K̂ (81 x 81, by column) and m̂ (27), FP64

@@ -327,6 +327,7 @@ void q2lattice_prepare(Ctx &c) {
 // per tabulation, cell sizes and ν), then one launch writes nzval and b.
 void q2lattice_assemble(Ctx &c, bool f32) {
   GH_REQUIRE(c.q2lattice, GH_ERR_STATE, "no vector-Q2 lattice for this space");
+  c.last_kernel = GH_KERNEL_Q2_LATTICE;
   const double key[4] = {c.nu, c.sizes[0], c.sizes[1], c.sizes[2]};
   const bool fresh = !(c.q2_ref_ready && std::equal(key, key + 4, c.q2_ref_key));
   if (fresh) {

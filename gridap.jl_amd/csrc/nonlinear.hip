@@ -190,6 +190,7 @@ void assemble_plaplacian(Ctx &c, int precision) {
   GH_REQUIRE(!c.have_kappa || c.kappa.n == (size_t)c.ncells, GH_ERR_STATE,
              "kappa was uploaded for a different mesh");
   build_coloring(c);
+  c.last_kernel = GH_KERNEL_CELL;
   const int NL = c.nloc, D = c.dim;
   const int stride = 1 + NL + NL * D;
   std::vector<double> tab(size_t(c.nq) * stride);

@@ -787,6 +787,7 @@ bool elemlist_sum(Ctx &c, bool f32, const FusedRhs *rhs) {
   const bool fuse = rhs && c.slots && c.metric;
   FusedRhs F{};
   if (fuse) F = *rhs;
+  c.last_kernel = c.slots ? GH_KERNEL_COLUMN_SUM : GH_KERNEL_LIST_SUM;
   if (c.slots) {
     if (f32) launch_column_sum<float>(c, F);
     else launch_column_sum<double>(c, F);

@@ -31,6 +31,12 @@ GH_FORM_PLAPLACIAN = 3
 GH_PATH_AUTO = 0
 GH_PATH_GENERAL = 1
 GH_PATH_STRUCTURED = 2
+GH_KERNEL_NONE = 0
+GH_KERNEL_Q1_LATTICE = 1
+GH_KERNEL_CELL = 2
+GH_KERNEL_COLUMN_SUM = 3
+GH_KERNEL_LIST_SUM = 4
+GH_KERNEL_Q2_LATTICE = 5
 GH_PRECOND_NONE = 0
 GH_PRECOND_JACOBI = 1
 GH_PRECOND_FSAI = 2
@@ -50,7 +56,8 @@ class SolveStats(C.Structure):
 class PatternInfo(C.Structure):
     _fields_ = [("nnz", C.c_int64), ("nranges", C.c_int64), ("lattice", C.c_int32),
                 ("box", C.c_int32), ("elemlist", C.c_int32), ("compact", C.c_int32),
-                ("metric", C.c_int32), ("inverse", C.c_int32)]
+                ("metric", C.c_int32), ("inverse", C.c_int32), ("last_kernel", C.c_int32),
+                ("reserved", C.c_int32)]
 
 
 SIGNATURES = {

@@ -61,6 +61,14 @@ extern "C" {
 #define GH_PATH_STRUCTURED 2 /* owner-computes lattice kernel (error if not eligible)  */
 
 /* ---- preconditioners ---- */
+/* Numeric kernels (gh_pattern_info.last_kernel): which form assembled the values. */
+#define GH_KERNEL_NONE 0       /* no numeric phase since the pattern was built          */
+#define GH_KERNEL_Q1_LATTICE 1 /* owner-computes scalar Q1 lattice kernel               */
+#define GH_KERNEL_CELL 2       /* per-cell kernel + coloured deterministic scatter      */
+#define GH_KERNEL_COLUMN_SUM 3 /* element lists: column ranges with 16-bit lists        */
+#define GH_KERNEL_LIST_SUM 4   /* element lists: int32 lists, one thread per nonzero    */
+#define GH_KERNEL_Q2_LATTICE 5 /* owner-computes vector-Q2 lattice kernel               */
+
 #define GH_PRECOND_NONE 0
 #define GH_PRECOND_JACOBI 1
 #define GH_PRECOND_FSAI 2 /* learned factorized sparse approximate inverse, M = GᵀG;
@@ -200,6 +208,8 @@ typedef struct gh_pattern_info {
                           0 on the element-list path: int32 lists, one thread per nonzero */
   int32_t metric;      /* 1: simplex metric records (entries formed in the column sum) */
   int32_t inverse;     /* 1: full-column staging (vector Q2 element matrices)          */
+  int32_t last_kernel; /* GH_KERNEL_* of the last numeric phase on this pattern          */
+  int32_t reserved;
 } gh_pattern_info;
 int gh_pattern_info_get(gh_ctx *ctx, gh_pattern_info *info);
 

@@ -69,7 +69,10 @@ def test_elasticity_matches_oracle(oracle, precision, path):
     announce("elasticity Q2 %s %s clamped x=0 path=%s" % (part, precision, path))
     E = np.exp(np.random.default_rng(3).normal(0, 0.5, 12))
     op, V = build(part, E, precision=precision, path=path)
-    assert op.assem.ctx.pattern_info()["lattice"] == (path == "auto")
+    info = op.assem.ctx.pattern_info()
+    assert info["lattice"] == 1 and info["elemlist"] == 1
+    assert info["last_kernel"] == (_lib.GH_KERNEL_Q2_LATTICE if path == "auto"
+                                   else _lib.GH_KERNEL_COLUMN_SUM)
     r, cp, rv, nz, b = oracle_system(oracle, part, E, 0.3, (0, 0, -1), CLAMP_X0)
     np.testing.assert_array_equal(V.cell_dof_ids, r["cell_dofs"])
     A = op.get_matrix()

@@ -173,6 +173,8 @@ def test_tet_column_sum_matches_list_sum(monkeypatch):
                           precision=precision)
         assert op1.assem.ctx.pattern_info()["compact"] == 1
         assert op2.assem.ctx.pattern_info()["compact"] == 0
+        assert op1.assem.ctx.pattern_info()["last_kernel"] == _lib.GH_KERNEL_COLUMN_SUM
+        assert op2.assem.ctx.pattern_info()["last_kernel"] == _lib.GH_KERNEL_LIST_SUM
         np.testing.assert_array_equal(op1.get_matrix().nzval, op2.get_matrix().nzval)
         np.testing.assert_array_equal(op1.get_vector(), op2.get_vector())
 
@@ -232,6 +234,10 @@ def test_high_valence_column_gets_its_own_range(monkeypatch):
         assert info["elemlist"] == 1 and info["compact"] == (env == "1"), info
         ctx.coef_upload(kappa, None, None)
         ctx.assemble(_lib.GH_FORM_DIFFUSION, _lib.GH_PREC_FP64, _lib.GH_PATH_AUTO)
+        info = ctx.pattern_info()
+        assert info["last_kernel"] == (_lib.GH_KERNEL_COLUMN_SUM if env == "1"
+                                       else _lib.GH_KERNEL_LIST_SUM), info
+        assert (info["nranges"] > 0) == (env == "1"), info
         nz, b = ctx.values_get()
         colptr, rowval = ctx.pattern_get()
         out[env] = (colptr, rowval, nz, b)
