@@ -80,6 +80,9 @@ void invalidate_pattern(gh::Ctx &c) {
   c.q2lattice = false;
   c.q2_ref_ready = false;
   c.q2_node_dof.release();
+  c.q2_node_class.release();
+  c.q2_irregular.release();
+  c.q2_nirregular = 0;
   // state derived from the previous pattern (set again by the symbolic phase when it
   // applies to the new one)
   c.box = false;
@@ -520,6 +523,7 @@ int gh_pattern_info_get(gh_ctx *ctx, gh_pattern_info *info) {
     r.metric = c.metric;
     r.inverse = c.inverse;
     r.last_kernel = c.last_kernel;
+    r.fallback_nodes = c.q2lattice ? (int32_t)c.q2_nirregular : 0;
     *info = r;
   });
 }

@@ -25,6 +25,7 @@
 // assembles the space.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <vector>
 
 #include "gh_internal.h"
@@ -63,6 +64,8 @@ struct Q2Args {
   double *nzval;
   double *b;
   int *bad;                  // check mode: set when the rows differ from rowval
+  const int32_t *nodes;      // the nodes to assemble, or nullptr: every node
+  int64_t nlist;             // their count (with nodes)
 };
 
 __device__ __forceinline__ void wave_sync() {
@@ -87,8 +90,10 @@ __global__ __launch_bounds__(64 * kQ2Waves) void q2_lattice_kernel(const Q2Args 
   int *nfc = nfcs[wave];
   double *stage = stages[wave];
   const int N0 = P.nn[0], N1 = P.nn[1];
-  for (int64_t node = (int64_t)blockIdx.x * kQ2Waves + wave; node < P.nnodes;
-       node += (int64_t)gridDim.x * kQ2Waves) {
+  const int64_t nwork = P.nodes ? P.nlist : P.nnodes;
+  for (int64_t w = (int64_t)blockIdx.x * kQ2Waves + wave; w < nwork;
+       w += (int64_t)gridDim.x * kQ2Waves) {
+    const int64_t node = P.nodes ? (int64_t)P.nodes[w] : w;
     int own[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) own[k] = P.node_dof[node * 3 + k];
@@ -263,6 +268,194 @@ __global__ void q2_node_verify(int64_t ncells, int nc0, int nc1, int N0, int N1,
     if (node_dof[node * 3 + k] != cell_dofs[cell * 81 + a + 27 * k]) *bad = 1;
 }
 
+// ---- column templates ---------------------------------------------------------------
+// Away from the boundary every node of one parity type (x, y, z even or odd) sees the
+// same column: the same cells around it, the same neighbours, the same sorted row
+// order (the conforming numbering is translation invariant there) and the same
+// Dirichlet neighbours.  The symbolic phase groups the nodes by a position class per
+// axis (the first and last four lattice positions each form a class, the rest split
+// by parity), builds each class's template from one node and checks every node
+// against it (rows equal to rowval, Dirichlet neighbours where the template has them):
+//   slot s < len: the s-th row of the column (neighbour offset and component),
+//   slot len <= s < len + nd: a Dirichlet neighbour (its entries give the lifting),
+//   pos[e][t]: the slot of local row t = a + 27 k of the template's e-th cell.
+// The column of a templated node is then Σ_e E_e K̂[:, (a_e(n), i)] scattered by pos
+// into an LDS accumulator, cell by cell in ascending cell order (Gridap's cell loop);
+// the products and sums are those of the ranking kernel above, so both give the same
+// bits.  Nodes that match no template keep the ranking kernel.
+constexpr int kQ2Slots = 3 * kQ2MaxNb;  // neighbour nodes x components
+constexpr int kQ2AxisClasses = 10;
+constexpr int kQ2Classes = kQ2AxisClasses * kQ2AxisClasses * kQ2AxisClasses;
+
+struct Q2ClassHdr {
+  int32_t ne, len, nd, pad;
+  int8_t an[8];     // local Q2 node of the column's node in cell e
+  int8_t dc[3][8];  // cell e = (X >> 1, Y >> 1, Z >> 1) + dc[.][e]
+};
+
+__host__ __device__ inline int q2_axis_class(int x, int n) {
+  if (x < 4) return x;
+  if (x > n - 5) return 4 + (n - 1 - x);
+  return 8 + (x & 1);
+}
+
+// neighbour of `node` addressed by a slot code (offset code * 3 + component), or -1
+// outside the lattice
+__device__ __forceinline__ int64_t q2_neighbour(const Q2Args &P, int X, int Y, int Z, int code,
+                                                int *comp) {
+  *comp = code % 3;
+  const int o = code / 3;
+  const int mx = X + o % 5 - 2, my = Y + (o / 5) % 5 - 2, mz = Z + o / 25 - 2;
+  if (mx < 0 || my < 0 || mz < 0 || mx >= P.nn[0] || my >= P.nn[1] || mz >= P.nn[2]) return -1;
+  return mx + (int64_t)P.nn[0] * (my + (int64_t)P.nn[1] * mz);
+}
+
+// One wave per node: the node's class when its column equals the template's, else -1.
+__global__ __launch_bounds__(256) void q2_class_check(const Q2Args P,
+                                                      const int16_t *__restrict__ clsmap,
+                                                      const Q2ClassHdr *__restrict__ hdr,
+                                                      const int16_t *__restrict__ code,
+                                                      int16_t *__restrict__ node_class) {
+  const int lane = threadIdx.x & 63;
+  const int64_t nw = (int64_t)gridDim.x * 4;
+  for (int64_t node = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); node < P.nnodes; node += nw) {
+    const int X = (int)(node % P.nn[0]), Y = (int)((node / P.nn[0]) % P.nn[1]);
+    const int Z = (int)(node / ((int64_t)P.nn[0] * P.nn[1]));
+    const int cls = clsmap[q2_axis_class(X, P.nn[0]) +
+                           kQ2AxisClasses * (q2_axis_class(Y, P.nn[1]) +
+                                             kQ2AxisClasses * q2_axis_class(Z, P.nn[2]))];
+    bool ok = cls >= 0;
+    if (ok) {
+      const Q2ClassHdr &H = hdr[cls];
+      // the cells around the node are the template's, in the same order
+      const int c[3] = {X, Y, Z};
+      int clo[3], chi[3];
+      for (int k = 0; k < 3; ++k) {
+        if (c[k] & 1) {
+          clo[k] = chi[k] = c[k] >> 1;
+        } else {
+          clo[k] = max((c[k] >> 1) - 1, 0);
+          chi[k] = min(c[k] >> 1, P.nc[k] - 1);
+        }
+      }
+      int e = 0;
+      for (int cz = clo[2]; cz <= chi[2]; ++cz)
+        for (int cy = clo[1]; cy <= chi[1]; ++cy)
+          for (int cx = clo[0]; cx <= chi[0]; ++cx, ++e)
+            if (e >= 8 || cx - (X >> 1) != H.dc[0][e] || cy - (Y >> 1) != H.dc[1][e] ||
+                cz - (Z >> 1) != H.dc[2][e])
+              ok = false;
+      if (e != H.ne) ok = false;
+      const int16_t *cc = code + (int64_t)cls * kQ2Slots;
+      for (int i = 0; i < 3 && ok; ++i) {
+        const int j = P.node_dof[node * 3 + i];
+        if (j <= 0) continue;
+        const int p0 = P.colptr[j - 1];
+        if (P.colptr[j] - p0 != H.len) {
+          ok = false;
+          break;
+        }
+        for (int s = lane; s < H.len; s += 64) {
+          int k;
+          const int64_t m = q2_neighbour(P, X, Y, Z, cc[s], &k);
+          if (m < 0 || P.node_dof[m * 3 + k] != P.rowval[p0 + s] + 1) ok = false;
+        }
+      }
+      for (int s = lane; s < H.nd && ok; s += 64) {
+        int k;
+        const int64_t m = q2_neighbour(P, X, Y, Z, cc[H.len + s], &k);
+        if (m < 0 || P.node_dof[m * 3 + k] >= 0) ok = false;
+      }
+    }
+    ok = __all(ok);
+    if (lane == 0) node_class[node] = ok ? (int16_t)cls : (int16_t)-1;
+  }
+}
+
+template <class T>
+__global__ __launch_bounds__(64 * kQ2Waves) void q2_class_kernel(
+    const Q2Args P, const Q2ClassHdr *__restrict__ hdr, const int32_t *__restrict__ pos32,
+    const int16_t *__restrict__ code, const int16_t *__restrict__ node_class) {
+  __shared__ T K[81 * 81];
+  __shared__ double msh[27];
+  __shared__ __align__(16) double accs[kQ2Waves][kQ2Slots + 3];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const T *Kg = static_cast<const T *>(P.Kref);
+  for (int t = threadIdx.x; t < 81 * 81; t += blockDim.x) K[t] = Kg[t];
+  if (threadIdx.x < 27) msh[threadIdx.x] = P.mref[threadIdx.x];
+  __syncthreads();
+  double *acc = accs[wave];
+  const int N0 = P.nn[0], N1 = P.nn[1];
+  for (int64_t node = (int64_t)blockIdx.x * kQ2Waves + wave; node < P.nnodes;
+       node += (int64_t)gridDim.x * kQ2Waves) {
+    const int cls = node_class[node];
+    if (cls < 0) continue;  // wave-uniform
+    int own[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) own[k] = P.node_dof[node * 3 + k];
+    if (own[0] <= 0 && own[1] <= 0 && own[2] <= 0) continue;
+    const Q2ClassHdr *H = hdr + cls;
+    const int ne = H->ne, len = H->len, nd = H->nd;
+    const int X = (int)(node % N0), Y = (int)((node / N0) % N1), Z = (int)(node / ((int64_t)N0 * N1));
+    const int bx = X >> 1, by = Y >> 1, bz = Z >> 1;
+    double ev[8], msum = 0.0;
+    int an[8];
+    uint32_t pp[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      ev[e] = 0.0;
+      an[e] = 0;
+      pp[e] = 0;
+      if (e < ne) {
+        an[e] = H->an[e];
+        const int64_t cell = (bx + H->dc[0][e]) +
+                             (int64_t)P.nc[0] * ((by + H->dc[1][e]) + (int64_t)P.nc[1] * (bz + H->dc[2][e]));
+        ev[e] = P.E ? P.E[cell] : 1.0;
+        pp[e] = (uint32_t)pos32[((int64_t)cls * 8 + e) * 64 + lane];
+        msum += msh[an[e]];
+      }
+    }
+#pragma unroll 1
+    for (int i = 0; i < 3; ++i) {
+      const int j = own[i];
+      if (j <= 0) continue;  // wave-uniform
+      const int p0 = P.colptr[j - 1];
+      const int sh = p0 & 1;  // acc[sh + t] <-> nzval[p0 + t]: 16-byte aligned chunks
+      for (int t = lane; t < len + nd; t += 64) acc[sh + t] = 0.0;
+      wave_sync();
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        if (e < ne) {
+          const T *kc = &K[(an[e] + 27 * i) * 81];
+          const T Ec = (T)ev[e];
+          acc[sh + (int)(pp[e] & 0xffffu)] += (double)(Ec * kc[lane]);
+          if (lane < 81 - 64) acc[sh + (int)(pp[e] >> 16)] += (double)(Ec * kc[lane + 64]);
+          wave_sync();
+        }
+      }
+      double2 *dst2 = reinterpret_cast<double2 *>(P.nzval + (p0 - sh));
+      const double2 *st2 = reinterpret_cast<const double2 *>(acc);
+      const int lo2 = sh, hi2 = sh + len;
+      for (int t = (lo2 + 1) / 2 + lane; t < hi2 / 2; t += 64) dst2[t] = st2[t];
+      if (lane == 0 && (lo2 & 1)) P.nzval[p0] = acc[lo2];
+      if (lane == 1 && (hi2 & 1)) P.nzval[p0 + len - 1] = acc[hi2 - 1];
+      double lift = 0.0;
+      if (P.dvals) {
+        const int16_t *cc = code + (int64_t)cls * kQ2Slots + len;
+        for (int t = lane; t < nd; t += 64) {
+          int k;
+          const int64_t m = q2_neighbour(P, X, Y, Z, cc[t], &k);
+          lift += acc[sh + len + t] * P.dvals[-P.node_dof[m * 3 + k] - 1];
+        }
+      }
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) lift += __shfl_xor(lift, o);
+      if (lane == 0) P.b[j - 1] = P.body[i] * msum - lift;
+      wave_sync();  // the accumulator is free for the next component
+    }
+  }
+}
+
 Q2Args make_args(const Ctx &c) {
   Q2Args A{};
   for (int k = 0; k < 3; ++k) {
@@ -274,6 +467,151 @@ Q2Args make_args(const Ctx &c) {
   A.colptr = c.colptr.get();
   A.rowval = c.rowval.get();
   return A;
+}
+
+// Host copy of kLexOfLoc (the template builder runs on the host).
+const int8_t kLexOfLocHost[27] = {0, 2, 6, 8, 18, 20, 24, 26, 1, 7, 19, 25, 3, 5,
+                                  21, 23, 9, 11, 15, 17, 4, 22, 10, 16, 12, 14, 13};
+
+// The column templates of a lattice space (the node map must be on the device): one per
+// position class, built from the first node of the class; every node is then checked
+// against its class's template.  Nodes without one are listed for the ranking kernel.
+void q2_build_classes(Ctx &c, const Q2Args &A) {
+  const int64_t nn = A.nnodes;
+  hipStream_t s = c.stream;
+  std::vector<int32_t> nd((size_t)nn * 3);
+  GH_CHECK_HIP(hipMemcpyAsync(nd.data(), c.q2_node_dof.get(), sizeof(int32_t) * nd.size(),
+                              hipMemcpyDeviceToHost, s));
+  GH_CHECK_HIP(hipStreamSynchronize(s));
+  int first[3][kQ2AxisClasses];
+  for (int k = 0; k < 3; ++k) {
+    std::fill(first[k], first[k] + kQ2AxisClasses, -1);
+    for (int x = A.nn[k] - 1; x >= 0; --x) first[k][q2_axis_class(x, A.nn[k])] = x;
+  }
+  std::vector<int16_t> clsmap(kQ2Classes, -1);
+  std::vector<Q2ClassHdr> hdr;
+  std::vector<int32_t> pos;
+  std::vector<int16_t> codes;
+  std::vector<int> slot_of(kQ2Slots);
+  for (int az = 0; az < kQ2AxisClasses; ++az)
+    for (int ay = 0; ay < kQ2AxisClasses; ++ay)
+      for (int ax = 0; ax < kQ2AxisClasses; ++ax) {
+        const int rx = first[0][ax], ry = first[1][ay], rz = first[2][az];
+        if (rx < 0 || ry < 0 || rz < 0) continue;
+        const int r[3] = {rx, ry, rz};
+        int clo[3], chi[3];
+        for (int k = 0; k < 3; ++k) {
+          if (r[k] & 1) {
+            clo[k] = chi[k] = r[k] >> 1;
+          } else {
+            clo[k] = std::max((r[k] >> 1) - 1, 0);
+            chi[k] = std::min(r[k] >> 1, A.nc[k] - 1);
+          }
+        }
+        // the window: every node of the cells around the representative
+        std::vector<std::pair<int32_t, int>> freev;
+        std::vector<int> dirv;
+        bool valid = true;
+        for (int mz = 2 * clo[2]; mz <= 2 * chi[2] + 2; ++mz)
+          for (int my = 2 * clo[1]; my <= 2 * chi[1] + 2; ++my)
+            for (int mx = 2 * clo[0]; mx <= 2 * chi[0] + 2; ++mx) {
+              const int64_t m = mx + (int64_t)A.nn[0] * (my + (int64_t)A.nn[1] * mz);
+              const int oc = (mx - rx + 2) + 5 * (my - ry + 2) + 25 * (mz - rz + 2);
+              for (int k = 0; k < 3; ++k) {
+                const int32_t d = nd[(size_t)m * 3 + k];
+                if (d > 0) freev.push_back({d, oc * 3 + k});
+                else if (d < 0) dirv.push_back(oc * 3 + k);
+                else valid = false;
+              }
+            }
+        if (!valid) continue;
+        std::sort(freev.begin(), freev.end());
+        std::fill(slot_of.begin(), slot_of.end(), -1);
+        Q2ClassHdr H{};
+        H.len = (int)freev.size();
+        H.nd = (int)dirv.size();
+        std::vector<int16_t> cc(kQ2Slots, 0);
+        for (int t = 0; t < H.len; ++t) {
+          slot_of[freev[t].second] = t;
+          cc[t] = (int16_t)freev[t].second;
+        }
+        for (int t = 0; t < H.nd; ++t) {
+          slot_of[dirv[t]] = H.len + t;
+          cc[H.len + t] = (int16_t)dirv[t];
+        }
+        std::vector<int32_t> pc((size_t)8 * 64, 0);
+        int e = 0;
+        for (int cz = clo[2]; cz <= chi[2]; ++cz)
+          for (int cy = clo[1]; cy <= chi[1]; ++cy)
+            for (int cx = clo[0]; cx <= chi[0]; ++cx, ++e) {
+              H.dc[0][e] = (int8_t)(cx - (rx >> 1));
+              H.dc[1][e] = (int8_t)(cy - (ry >> 1));
+              H.dc[2][e] = (int8_t)(cz - (rz >> 1));
+              const int an_lex = (rx - 2 * cx) + 3 * (ry - 2 * cy) + 9 * (rz - 2 * cz);
+              H.an[e] = -1;
+              for (int a = 0; a < 27; ++a)
+                if (kLexOfLocHost[a] == an_lex) H.an[e] = (int8_t)a;
+              for (int t = 0; t < 81; ++t) {
+                const int a = t % 27, k = t / 27, lx = kLexOfLocHost[a];
+                const int mx = 2 * cx + lx % 3, my = 2 * cy + (lx / 3) % 3, mz = 2 * cz + lx / 9;
+                const int oc = (mx - rx + 2) + 5 * (my - ry + 2) + 25 * (mz - rz + 2);
+                const int sl = slot_of[oc * 3 + k];
+                if (sl < 0 || H.an[e] < 0) valid = false;
+                if (t < 64) pc[(size_t)e * 64 + t] |= sl & 0xffff;
+                else pc[(size_t)e * 64 + (t - 64)] |= (sl & 0xffff) << 16;
+              }
+            }
+        H.ne = e;
+        if (!valid) continue;
+        clsmap[ax + kQ2AxisClasses * (ay + kQ2AxisClasses * az)] = (int16_t)hdr.size();
+        hdr.push_back(H);
+        pos.insert(pos.end(), pc.begin(), pc.end());
+        codes.insert(codes.end(), cc.begin(), cc.end());
+      }
+  c.q2_node_class.resize(nn);
+  if (hdr.empty()) {
+    GH_CHECK_HIP(hipMemsetAsync(c.q2_node_class.get(), 0xff, sizeof(int16_t) * nn, s));
+  } else {
+    c.q2_cls_hdr.resize(hdr.size() * sizeof(Q2ClassHdr));
+    c.q2_cls_pos.resize(pos.size());
+    c.q2_cls_code.resize(codes.size());
+    DBuf<int16_t> dmap;
+    dmap.resize(clsmap.size());
+    GH_CHECK_HIP(hipMemcpyAsync(c.q2_cls_hdr.get(), hdr.data(), hdr.size() * sizeof(Q2ClassHdr),
+                                hipMemcpyHostToDevice, s));
+    GH_CHECK_HIP(hipMemcpyAsync(c.q2_cls_pos.get(), pos.data(), pos.size() * sizeof(int32_t),
+                                hipMemcpyHostToDevice, s));
+    GH_CHECK_HIP(hipMemcpyAsync(c.q2_cls_code.get(), codes.data(), codes.size() * sizeof(int16_t),
+                                hipMemcpyHostToDevice, s));
+    GH_CHECK_HIP(hipMemcpyAsync(dmap.get(), clsmap.data(), clsmap.size() * sizeof(int16_t),
+                                hipMemcpyHostToDevice, s));
+    const unsigned g = (unsigned)std::min<int64_t>((nn + 3) / 4, 8192);
+    hipLaunchKernelGGL(q2_class_check, dim3(g), dim3(256), 0, s, A, dmap.get(),
+                       (const Q2ClassHdr *)c.q2_cls_hdr.get(), c.q2_cls_code.get(),
+                       c.q2_node_class.get());
+    GH_CHECK_HIP(hipGetLastError());
+    GH_CHECK_HIP(hipStreamSynchronize(s));  // dmap is released at scope end
+  }
+  std::vector<int16_t> ncls((size_t)nn);
+  GH_CHECK_HIP(hipMemcpyAsync(ncls.data(), c.q2_node_class.get(), sizeof(int16_t) * nn,
+                              hipMemcpyDeviceToHost, s));
+  GH_CHECK_HIP(hipStreamSynchronize(s));
+  std::vector<int32_t> irr;
+  for (int64_t m = 0; m < nn; ++m)
+    if (ncls[m] < 0 && (nd[m * 3] > 0 || nd[m * 3 + 1] > 0 || nd[m * 3 + 2] > 0))
+      irr.push_back((int32_t)m);
+  c.q2_nirregular = (int64_t)irr.size();
+  c.q2_irregular.resize(irr.size());
+  if (!irr.empty())
+    GH_CHECK_HIP(hipMemcpy(c.q2_irregular.get(), irr.data(), sizeof(int32_t) * irr.size(),
+                           hipMemcpyHostToDevice));
+}
+
+unsigned q2_class_grid(int64_t nnodes, bool f64) {
+  const int64_t want = (nnodes + kQ2Waves - 1) / kQ2Waves;
+  const char *env = std::getenv("GRIDAP_HIP_Q2_GRID");
+  const int64_t cap = env && std::atoi(env) > 0 ? std::atoi(env) : (f64 ? 512 : 1024);
+  return (unsigned)std::max<int64_t>(1, std::min<int64_t>(want, cap));
 }
 
 unsigned q2_grid(int64_t nnodes, bool f64) {
@@ -319,6 +657,16 @@ void q2lattice_prepare(Ctx &c) {
   if (hbad) {
     c.q2_node_dof.release();
     return;
+  }
+  const char *tenv = std::getenv("GRIDAP_HIP_Q2_TEMPLATES");
+  if (tenv && tenv[0] == '0') {
+    // every node through the ranking kernel
+    c.q2_node_class.resize(A.nnodes);
+    GH_CHECK_HIP(hipMemsetAsync(c.q2_node_class.get(), 0xff, sizeof(int16_t) * A.nnodes, s));
+    c.q2_irregular.release();
+    c.q2_nirregular = -1;
+  } else {
+    q2_build_classes(c, A);
   }
   c.q2lattice = true;
 }
@@ -392,11 +740,29 @@ void q2lattice_assemble(Ctx &c, bool f32) {
   A.dvals = c.have_dvals ? c.dvals.get() : nullptr;
   A.nzval = c.nzval.get();
   A.b = c.b.get();
+  // templated nodes
+  if (c.q2_nirregular >= 0) {
+    const Q2ClassHdr *hdr = (const Q2ClassHdr *)c.q2_cls_hdr.get();
+    if (f32)
+      hipLaunchKernelGGL((q2_class_kernel<float>), dim3(q2_class_grid(A.nnodes, false)),
+                         dim3(64 * kQ2Waves), 0, c.stream, A, hdr, c.q2_cls_pos.get(),
+                         c.q2_cls_code.get(), c.q2_node_class.get());
+    else
+      hipLaunchKernelGGL((q2_class_kernel<double>), dim3(q2_class_grid(A.nnodes, true)),
+                         dim3(64 * kQ2Waves), 0, c.stream, A, hdr, c.q2_cls_pos.get(),
+                         c.q2_cls_code.get(), c.q2_node_class.get());
+    GH_CHECK_HIP(hipGetLastError());
+    if (c.q2_nirregular == 0) return;
+    A.nodes = c.q2_irregular.get();
+    A.nlist = c.q2_nirregular;
+  }
+  // the rest through the ranking kernel
+  const int64_t work = A.nodes ? A.nlist : A.nnodes;
   if (f32)
-    hipLaunchKernelGGL((q2_lattice_kernel<float, false>), dim3(q2_grid(A.nnodes, false)),
+    hipLaunchKernelGGL((q2_lattice_kernel<float, false>), dim3(q2_grid(work, false)),
                        dim3(64 * kQ2Waves), 0, c.stream, A);
   else
-    hipLaunchKernelGGL((q2_lattice_kernel<double, false>), dim3(q2_grid(A.nnodes, true)),
+    hipLaunchKernelGGL((q2_lattice_kernel<double, false>), dim3(q2_grid(work, true)),
                        dim3(64 * kQ2Waves), 0, c.stream, A);
   GH_CHECK_HIP(hipGetLastError());
 }

@@ -57,7 +57,7 @@ class PatternInfo(C.Structure):
     _fields_ = [("nnz", C.c_int64), ("nranges", C.c_int64), ("lattice", C.c_int32),
                 ("box", C.c_int32), ("elemlist", C.c_int32), ("compact", C.c_int32),
                 ("metric", C.c_int32), ("inverse", C.c_int32), ("last_kernel", C.c_int32),
-                ("reserved", C.c_int32)]
+                ("fallback_nodes", C.c_int32)]
 
 
 SIGNATURES = {

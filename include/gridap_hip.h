@@ -209,7 +209,8 @@ typedef struct gh_pattern_info {
   int32_t metric;      /* 1: simplex metric records (entries formed in the column sum) */
   int32_t inverse;     /* 1: full-column staging (vector Q2 element matrices)          */
   int32_t last_kernel; /* GH_KERNEL_* of the last numeric phase on this pattern          */
-  int32_t reserved;
+  int32_t fallback_nodes; /* vector-Q2 lattice: nodes without a column template, which the
+                          ranking kernel assembles (-1: templates switched off)      */
 } gh_pattern_info;
 int gh_pattern_info_get(gh_ctx *ctx, gh_pattern_info *info);
 

@@ -273,3 +273,82 @@ def test_elasticity_lattice_symmetric_and_switch(monkeypatch):
     op3, _ = build(part, E, precision="fp64")
     assert relmax(op3.get_matrix().nzval, op2.get_matrix().nzval) <= 1e-12
     assert relmax(op3.get_vector(), op2.get_vector()) <= 1e-12
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_elasticity_templates_match_ranking(monkeypatch, precision):
+    """Column templates (one per position class, checked node by node against rowval)
+    and the per-node ranking kernel add the same products in the same order: nzval
+    bit-identical, b within 1e-14 (the lifting sums in another order).  15x13x11
+    cells: every node has a template; Dirichlet data on the clamped face."""
+    part = (15, 13, 11)
+    announce("elasticity Q2 %s templates vs ranking %s" % (part, precision))
+    E = _config3_E(part, seed=5)
+    g = lambda x: np.stack([0.1 * x[..., 1], -0.2 * x[..., 2], 0.3 + x[..., 1] * x[..., 2]],
+                           axis=-1)
+    op1, _ = build(part, E, precision=precision, g=g)
+    info1 = op1.assem.ctx.pattern_info()
+    assert info1["last_kernel"] == _lib.GH_KERNEL_Q2_LATTICE and info1["fallback_nodes"] == 0
+    monkeypatch.setenv("GRIDAP_HIP_Q2_TEMPLATES", "0")
+    op2, _ = build(part, E, precision=precision, g=g)
+    assert op2.assem.ctx.pattern_info()["fallback_nodes"] == -1
+    np.testing.assert_array_equal(op1.get_matrix().nzval, op2.get_matrix().nzval)
+    assert relmax(op1.get_vector(), op2.get_vector()) <= 1e-14
+
+
+def test_elasticity_templates_with_irregular_nodes(oracle, monkeypatch):
+    """40 single free components turned Dirichlet at random (the other free dofs
+    renumbered in order): the nodes around them match no template and go through the
+    ranking kernel, the rest through the templates.  FP64 against the oracle's
+    assembly of the same cell matrices with the same ids (1e-12), and bitwise against
+    the ranking kernel alone."""
+    part = (9, 8, 7)
+    announce("elasticity Q2 %s templates with irregular nodes" % (part,))
+    C = int(np.prod(part))
+    E = _config3_E(part, seed=8)
+    model = G.CartesianDiscreteModel((0, 1, 0, 1, 0, 1), part)
+    reffe = G.ReferenceFE(G.lagrangian, G.VectorValue[3, G.Float64], 2)
+    V = G.TestFESpace(model, reffe, dirichlet_tags=CLAMP_X0)
+    cd = V.cell_dof_ids.astype(np.int64)
+    rng = np.random.default_rng(9)
+    S = rng.choice(np.arange(1, V.nfree + 1), size=40, replace=False)
+    keep = np.ones(V.nfree + 1, bool)
+    keep[0] = False
+    keep[S] = False
+    newid = np.zeros(V.nfree + 1, np.int64)
+    newid[keep] = np.arange(1, keep.sum() + 1)
+    newid[S] = -(V.ndirichlet + 1 + np.arange(S.size))
+    cd2 = np.where(cd > 0, newid[np.maximum(cd, 0)], cd).astype(np.int32)
+    nfree2, ndiri2 = int(keep.sum()), V.ndirichlet + S.size
+    q = G.Quadrature(3, 4)
+    shp, grad = V.reffe.tabulate(3, q.coords, simplex=False)
+    ptrs = (np.arange(C + 1) * 81 + 1).astype(np.int32)
+    out = {}
+    for env in ("1", "0"):
+        monkeypatch.setenv("GRIDAP_HIP_Q2_TEMPLATES", env)
+        ctx = _lib.Context(0)
+        ctx.mesh_cartesian(model.origin, model.sizes, model.partition)
+        ctx.space_upload(_lib.GH_REFFE_QK, 2, 3, cd2.ravel(), ptrs, nfree2, ndiri2)
+        ctx.symbolic()
+        ctx.reffe_tabulation(q.weights, shp, grad)
+        ctx.coef_upload(E, None, None)
+        ctx.elasticity_params(0.3, (0.0, 0.0, -1.0))
+        ctx.assemble(_lib.GH_FORM_ELASTICITY, _lib.GH_PREC_FP64, _lib.GH_PATH_AUTO)
+        info = ctx.pattern_info()
+        assert info["last_kernel"] == _lib.GH_KERNEL_Q2_LATTICE, info
+        if env == "1":
+            assert 0 < info["fallback_nodes"] < (2 * part[0] + 1) * (2 * part[1] + 1) * (2 * part[2] + 1)
+        nz, b = ctx.values_get()
+        out[env] = (ctx.pattern_get(), nz, b)
+        ctx.close()
+    np.testing.assert_array_equal(out["1"][1], out["0"][1])
+    r = CO.elasticity_q2_box(oracle, part, E, 0.3, np.array([0.0, 0.0, -1.0]),
+                             [entities(CLAMP_X0)])
+    np.testing.assert_array_equal(r["cell_dofs"], V.cell_dof_ids)
+    cp, rv, nz = oracle.assemble_csc(cd2, r["cellmats"], nfree2, nfree2)
+    bo = oracle.assemble_vector(cd2, r["cellvecs"], nfree2)
+    (colptr, rowval), nzd, bd = out["1"]
+    np.testing.assert_array_equal(colptr, cp)
+    np.testing.assert_array_equal(rowval, rv)
+    assert relmax(nzd, nz) <= 1e-12
+    assert relmax(bd, bo) <= 1e-12
