@@ -80,7 +80,8 @@ void invalidate_pattern(gh::Ctx &c) {
   c.q2lattice = false;
   c.q2_ref_ready = false;
   c.q2_node_dof.release();
-  c.q2_node_class.release();
+  c.q2_rec.release();
+  c.q2_nrec = 0;
   c.q2_irregular.release();
   c.q2_nirregular = 0;
   // state derived from the previous pattern (set again by the symbolic phase when it

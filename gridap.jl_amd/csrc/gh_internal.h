@@ -160,9 +160,10 @@ struct Ctx {
   bool q2lattice = false;           // the lattice kernel assembles this space
   DBuf<int32_t> q2_node_dof;        // (2n+1)^3 lattice nodes x 3: signed dof ids
   DBuf<double> q2_ref;              // K̂ (81 x 81, by column) and m̂ (27), FP64
-  DBuf<int16_t> q2_node_class;      // per lattice node: its column template, -1: none
+  DBuf<int32_t> q2_rec;             // per templated node: {node, template, dofs, column starts}
+  int64_t q2_nrec = 0;
   DBuf<unsigned char> q2_cls_hdr;   // per template: Q2ClassHdr
-  DBuf<int32_t> q2_cls_pos;         // per template, cell, lane: two packed 16-bit slots
+  DBuf<int32_t> q2_cls_pos;         // per template, cell, lane: slots of rows lane, 64 + lane
   DBuf<int16_t> q2_cls_code;        // per template, slot: neighbour offset code * 3 + comp
   DBuf<int32_t> q2_irregular;       // lattice nodes without a template (ranking kernel)
   int64_t q2_nirregular = 0;

@@ -372,67 +372,120 @@ __global__ __launch_bounds__(256) void q2_class_check(const Q2Args P,
   }
 }
 
+// One record per templated node with a free component, written by the symbolic phase:
+// {lattice position x | y << 10 | z << 20, template, dof of component 0, 1, 2 (signed),
+// first nonzero of each free component's column (-1: Dirichlet)}.
+struct Q2NodeRec {
+  int32_t xyz, cls, own[3], p0[3];
+};
+constexpr int kQ2AccLd = 376;  // LDS accumulator of one column (>= kQ2Slots + 1, even)
+
+// Read-only data through the constant address space: a wave-uniform address becomes a
+// scalar load, whose wait (lgkmcnt) is not held up by the wave's outstanding nzval
+// stores the way a vector load's wait (vmcnt) is.
 template <class T>
-__global__ __launch_bounds__(64 * kQ2Waves) void q2_class_kernel(
-    const Q2Args P, const Q2ClassHdr *__restrict__ hdr, const int32_t *__restrict__ pos32,
-    const int16_t *__restrict__ code, const int16_t *__restrict__ node_class) {
+__device__ __forceinline__ T ldc(const T *p) {
+  typedef const __attribute__((address_space(4))) T ConstT;
+  return *(ConstT *)p;
+}
+
+__device__ __forceinline__ int4 ldc4(const int4 *p) {
+  const int *q = reinterpret_cast<const int *>(p);
+  return make_int4(ldc(q), ldc(q + 1), ldc(q + 2), ldc(q + 3));
+}
+
+__device__ __forceinline__ int sbyte(int w, int k) { return (int)(int8_t)(w >> (8 * k)); }
+
+// One wave per node, one component's column at a time: lane L takes the cell's local
+// rows t = L and t = 64 + L (L < 17) of the column; pos2 holds, per template, cell and
+// lane, the slots of those two rows (16 bits each).  The records are sorted by
+// template, so the per-lane slot tables are loaded once per template change; every
+// other read is a wave-uniform scalar load, so no wait in the node loop is held up by
+// the wave's outstanding nzval stores.
+template <class T, int W>
+__global__ __launch_bounds__(64 * W) void q2_class_kernel(
+    const Q2Args P, const int4 *__restrict__ hdr, const uint32_t *__restrict__ pos2,
+    const int16_t *__restrict__ code, const int4 *__restrict__ rec, int64_t nrec) {
   __shared__ T K[81 * 81];
-  __shared__ double msh[27];
-  __shared__ __align__(16) double accs[kQ2Waves][kQ2Slots + 3];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  // per wave: one column's accumulator; one more region, shared by the waves, takes
+  // the entries of inactive lanes (their values are never read)
+  __shared__ __align__(16) double accs[(W + 1) * kQ2AccLd];
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const T *Kg = static_cast<const T *>(P.Kref);
   for (int t = threadIdx.x; t < 81 * 81; t += blockDim.x) K[t] = Kg[t];
-  if (threadIdx.x < 27) msh[threadIdx.x] = P.mref[threadIdx.x];
   __syncthreads();
-  double *acc = accs[wave];
-  const int N0 = P.nn[0], N1 = P.nn[1];
-  for (int64_t node = (int64_t)blockIdx.x * kQ2Waves + wave; node < P.nnodes;
-       node += (int64_t)gridDim.x * kQ2Waves) {
-    const int cls = node_class[node];
-    if (cls < 0) continue;  // wave-uniform
-    int own[3];
+  double *acc = accs + wave * kQ2AccLd;
+  const int dummy = (W - wave) * kQ2AccLd + lane;  // the shared region, from acc
+  const bool hi = lane < 81 - 64;                  // the lane also takes row 64 + lane
+  const int64_t nw = (int64_t)gridDim.x * W, gw = (int64_t)blockIdx.x * W + wave;
+  const int nc0 = P.nc[0], nc01 = P.nc[0] * P.nc[1];
+  // the current template
+  int cur = -1, ne = 0, len = 0, nd = 0;
+  int an[8], coff[8];
+  uint32_t pp[8];
+  double msum = 0.0;
 #pragma unroll
-    for (int k = 0; k < 3; ++k) own[k] = P.node_dof[node * 3 + k];
-    if (own[0] <= 0 && own[1] <= 0 && own[2] <= 0) continue;
-    const Q2ClassHdr *H = hdr + cls;
-    const int ne = H->ne, len = H->len, nd = H->nd;
-    const int X = (int)(node % N0), Y = (int)((node / N0) % N1), Z = (int)(node / ((int64_t)N0 * N1));
-    const int bx = X >> 1, by = Y >> 1, bz = Z >> 1;
-    double ev[8], msum = 0.0;
-    int an[8];
-    uint32_t pp[8];
+  for (int e = 0; e < 8; ++e) an[e] = coff[e] = 0, pp[e] = 0;
+  // strided over the sorted records: every wave sees each template's share of the
+  // nodes (balanced work) and still changes template rarely
+  for (int64_t r = gw; r < nrec; r += nw) {
+    const int4 ra = ldc4(rec + 2 * r), rb = ldc4(rec + 2 * r + 1);
+    const int xyz = ra.x, cls = ra.y;
+    const int own[3] = {ra.z, ra.w, rb.x};
+    const int p0s[3] = {rb.y, rb.z, rb.w};
+    if (cls != cur) {  // wave-uniform; rare: the records are sorted by template
+      cur = cls;
+      const int4 h0 = ldc4(hdr + 3 * cls), h1 = ldc4(hdr + 3 * cls + 1), h2 = ldc4(hdr + 3 * cls + 2);
+      ne = h0.x;
+      len = h0.y;
+      nd = h0.z;
+      msum = 0.0;
 #pragma unroll
-    for (int e = 0; e < 8; ++e) {
-      ev[e] = 0.0;
-      an[e] = 0;
-      pp[e] = 0;
-      if (e < ne) {
-        an[e] = H->an[e];
-        const int64_t cell = (bx + H->dc[0][e]) +
-                             (int64_t)P.nc[0] * ((by + H->dc[1][e]) + (int64_t)P.nc[1] * (bz + H->dc[2][e]));
-        ev[e] = P.E ? P.E[cell] : 1.0;
-        pp[e] = (uint32_t)pos32[((int64_t)cls * 8 + e) * 64 + lane];
-        msum += msh[an[e]];
+      for (int e = 0; e < 8; ++e) {
+        const int k = e & 3;
+        an[e] = sbyte(e < 4 ? h1.x : h1.y, k);
+        const int dx = sbyte(e < 4 ? h1.z : h1.w, k), dy = sbyte(e < 4 ? h2.x : h2.y, k),
+                  dz = sbyte(e < 4 ? h2.z : h2.w, k);
+        coff[e] = dx + nc0 * dy + nc01 * dz;
+        if (e < ne) {
+          pp[e] = pos2[((int64_t)cls * 8 + e) * 64 + lane];
+          msum += ldc(P.mref + an[e]);
+        }
       }
+      // wait for the slot tables here, on this rare path, so that the common path
+      // carries no wait on vector loads (which would also wait for the stores)
+#pragma unroll
+      for (int e = 0; e < 8; ++e) asm volatile("" ::"v"(pp[e]));
     }
+    const int X = xyz & 1023, Y = (xyz >> 10) & 1023, Z = xyz >> 20;
+    const int64_t cbase = (X >> 1) + (int64_t)nc0 * (Y >> 1) + (int64_t)nc01 * (Z >> 1);
+    T Ec[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) Ec[e] = e < ne ? (T)(P.E ? ldc(P.E + cbase + coff[e]) : 1.0) : (T)0;
 #pragma unroll 1
     for (int i = 0; i < 3; ++i) {
       const int j = own[i];
       if (j <= 0) continue;  // wave-uniform
-      const int p0 = P.colptr[j - 1];
-      const int sh = p0 & 1;  // acc[sh + t] <-> nzval[p0 + t]: 16-byte aligned chunks
-      for (int t = lane; t < len + nd; t += 64) acc[sh + t] = 0.0;
-      wave_sync();
+      const int p0 = p0s[i];
+      const int sh = p0 & 1;  // acc[sh + slot] <-> nzval[p0 + slot]: 16-byte aligned chunks
+      double2 *acc2 = reinterpret_cast<double2 *>(acc);
+      for (int t = lane; t < kQ2AccLd / 2; t += 64) acc2[t] = make_double2(0.0, 0.0);
 #pragma unroll
       for (int e = 0; e < 8; ++e) {
         if (e < ne) {
           const T *kc = &K[(an[e] + 27 * i) * 81];
-          const T Ec = (T)ev[e];
-          acc[sh + (int)(pp[e] & 0xffffu)] += (double)(Ec * kc[lane]);
-          if (lane < 81 - 64) acc[sh + (int)(pp[e] >> 16)] += (double)(Ec * kc[lane + 64]);
-          wave_sync();
+          // the lane's two rows of this cell have distinct slots: both reads, then both
+          // writes; the next cell's reads follow these writes in the wave's LDS order
+          const int a0 = sh + (int)(pp[e] & 0xffffu);
+          const int a1 = hi ? sh + (int)(pp[e] >> 16) : dummy;
+          const T k0 = kc[lane], k1 = kc[hi ? 64 + lane : lane];
+          const double v0 = acc[a0], v1 = acc[a1];
+          acc[a0] = v0 + (double)(Ec[e] * k0);
+          acc[a1] = v1 + (double)(Ec[e] * k1);
         }
       }
+      wave_sync();
       double2 *dst2 = reinterpret_cast<double2 *>(P.nzval + (p0 - sh));
       const double2 *st2 = reinterpret_cast<const double2 *>(acc);
       const int lo2 = sh, hi2 = sh + len;
@@ -490,7 +543,7 @@ void q2_build_classes(Ctx &c, const Q2Args &A) {
   }
   std::vector<int16_t> clsmap(kQ2Classes, -1);
   std::vector<Q2ClassHdr> hdr;
-  std::vector<int32_t> pos;
+  std::vector<uint32_t> pos;
   std::vector<int16_t> codes;
   std::vector<int> slot_of(kQ2Slots);
   for (int az = 0; az < kQ2AxisClasses; ++az)
@@ -539,7 +592,7 @@ void q2_build_classes(Ctx &c, const Q2Args &A) {
           slot_of[dirv[t]] = H.len + t;
           cc[H.len + t] = (int16_t)dirv[t];
         }
-        std::vector<int32_t> pc((size_t)8 * 64, 0);
+        std::vector<uint32_t> pc((size_t)8 * 64, 0);  // [e][lane]: rows lane, 64 + lane
         int e = 0;
         for (int cz = clo[2]; cz <= chi[2]; ++cz)
           for (int cy = clo[1]; cy <= chi[1]; ++cy)
@@ -557,8 +610,7 @@ void q2_build_classes(Ctx &c, const Q2Args &A) {
                 const int oc = (mx - rx + 2) + 5 * (my - ry + 2) + 25 * (mz - rz + 2);
                 const int sl = slot_of[oc * 3 + k];
                 if (sl < 0 || H.an[e] < 0) valid = false;
-                if (t < 64) pc[(size_t)e * 64 + t] |= sl & 0xffff;
-                else pc[(size_t)e * 64 + (t - 64)] |= (sl & 0xffff) << 16;
+                pc[(size_t)e * 64 + t % 64] |= (uint32_t)(sl & 0xffff) << (t < 64 ? 0 : 16);
               }
             }
         H.ne = e;
@@ -568,9 +620,10 @@ void q2_build_classes(Ctx &c, const Q2Args &A) {
         pos.insert(pos.end(), pc.begin(), pc.end());
         codes.insert(codes.end(), cc.begin(), cc.end());
       }
-  c.q2_node_class.resize(nn);
+  DBuf<int16_t> node_class;
+  node_class.resize(nn);
   if (hdr.empty()) {
-    GH_CHECK_HIP(hipMemsetAsync(c.q2_node_class.get(), 0xff, sizeof(int16_t) * nn, s));
+    GH_CHECK_HIP(hipMemsetAsync(node_class.get(), 0xff, sizeof(int16_t) * nn, s));
   } else {
     c.q2_cls_hdr.resize(hdr.size() * sizeof(Q2ClassHdr));
     c.q2_cls_pos.resize(pos.size());
@@ -579,7 +632,7 @@ void q2_build_classes(Ctx &c, const Q2Args &A) {
     dmap.resize(clsmap.size());
     GH_CHECK_HIP(hipMemcpyAsync(c.q2_cls_hdr.get(), hdr.data(), hdr.size() * sizeof(Q2ClassHdr),
                                 hipMemcpyHostToDevice, s));
-    GH_CHECK_HIP(hipMemcpyAsync(c.q2_cls_pos.get(), pos.data(), pos.size() * sizeof(int32_t),
+    GH_CHECK_HIP(hipMemcpyAsync(c.q2_cls_pos.get(), pos.data(), pos.size() * sizeof(uint32_t),
                                 hipMemcpyHostToDevice, s));
     GH_CHECK_HIP(hipMemcpyAsync(c.q2_cls_code.get(), codes.data(), codes.size() * sizeof(int16_t),
                                 hipMemcpyHostToDevice, s));
@@ -588,18 +641,44 @@ void q2_build_classes(Ctx &c, const Q2Args &A) {
     const unsigned g = (unsigned)std::min<int64_t>((nn + 3) / 4, 8192);
     hipLaunchKernelGGL(q2_class_check, dim3(g), dim3(256), 0, s, A, dmap.get(),
                        (const Q2ClassHdr *)c.q2_cls_hdr.get(), c.q2_cls_code.get(),
-                       c.q2_node_class.get());
+                       node_class.get());
     GH_CHECK_HIP(hipGetLastError());
     GH_CHECK_HIP(hipStreamSynchronize(s));  // dmap is released at scope end
   }
   std::vector<int16_t> ncls((size_t)nn);
-  GH_CHECK_HIP(hipMemcpyAsync(ncls.data(), c.q2_node_class.get(), sizeof(int16_t) * nn,
+  std::vector<int32_t> cp((size_t)c.nfree + 1);
+  GH_CHECK_HIP(hipMemcpyAsync(ncls.data(), node_class.get(), sizeof(int16_t) * nn,
+                              hipMemcpyDeviceToHost, s));
+  GH_CHECK_HIP(hipMemcpyAsync(cp.data(), c.colptr.get(), sizeof(int32_t) * cp.size(),
                               hipMemcpyDeviceToHost, s));
   GH_CHECK_HIP(hipStreamSynchronize(s));
   std::vector<int32_t> irr;
-  for (int64_t m = 0; m < nn; ++m)
-    if (ncls[m] < 0 && (nd[m * 3] > 0 || nd[m * 3 + 1] > 0 || nd[m * 3 + 2] > 0))
+  std::vector<Q2NodeRec> recs;
+  for (int64_t m = 0; m < nn; ++m) {
+    const int32_t *o = &nd[(size_t)m * 3];
+    if (o[0] <= 0 && o[1] <= 0 && o[2] <= 0) continue;
+    if (ncls[m] < 0) {
       irr.push_back((int32_t)m);
+      continue;
+    }
+    Q2NodeRec R{};
+    const int64_t X = m % A.nn[0], Y = (m / A.nn[0]) % A.nn[1], Z = m / ((int64_t)A.nn[0] * A.nn[1]);
+    R.xyz = (int32_t)(X | (Y << 10) | (Z << 20));
+    R.cls = ncls[m];
+    for (int k = 0; k < 3; ++k) {
+      R.own[k] = o[k];
+      R.p0[k] = o[k] > 0 ? cp[o[k] - 1] : -1;
+    }
+    recs.push_back(R);
+  }
+  // by template, then lattice order: a wave's run of records rarely changes template
+  std::stable_sort(recs.begin(), recs.end(),
+                   [](const Q2NodeRec &x, const Q2NodeRec &y) { return x.cls < y.cls; });
+  c.q2_nrec = (int64_t)recs.size();
+  c.q2_rec.resize(recs.size() * 8);
+  if (!recs.empty())
+    GH_CHECK_HIP(hipMemcpy(c.q2_rec.get(), recs.data(), sizeof(Q2NodeRec) * recs.size(),
+                           hipMemcpyHostToDevice));
   c.q2_nirregular = (int64_t)irr.size();
   c.q2_irregular.resize(irr.size());
   if (!irr.empty())
@@ -607,10 +686,12 @@ void q2_build_classes(Ctx &c, const Q2Args &A) {
                            hipMemcpyHostToDevice));
 }
 
-unsigned q2_class_grid(int64_t nnodes, bool f64) {
-  const int64_t want = (nnodes + kQ2Waves - 1) / kQ2Waves;
+unsigned q2_class_grid(int64_t nnodes, bool f64, int waves) {
+  const int64_t want = (nnodes + waves - 1) / waves;
   const char *env = std::getenv("GRIDAP_HIP_Q2_GRID");
-  const int64_t cap = env && std::atoi(env) > 0 ? std::atoi(env) : (f64 ? 512 : 1024);
+  // measured at 48^3: FP32 8 waves per workgroup (3 workgroups per CU), 2048 workgroups;
+  // FP64 16 waves (one workgroup per CU by LDS), 512 workgroups
+  const int64_t cap = env && std::atoi(env) > 0 ? std::atoi(env) : (f64 ? 512 : 2048);
   return (unsigned)std::max<int64_t>(1, std::min<int64_t>(want, cap));
 }
 
@@ -659,10 +740,10 @@ void q2lattice_prepare(Ctx &c) {
     return;
   }
   const char *tenv = std::getenv("GRIDAP_HIP_Q2_TEMPLATES");
-  if (tenv && tenv[0] == '0') {
+  if ((tenv && tenv[0] == '0') || A.nn[0] > 1024 || A.nn[1] > 1024 || A.nn[2] > 1024) {
     // every node through the ranking kernel
-    c.q2_node_class.resize(A.nnodes);
-    GH_CHECK_HIP(hipMemsetAsync(c.q2_node_class.get(), 0xff, sizeof(int16_t) * A.nnodes, s));
+    c.q2_rec.release();
+    c.q2_nrec = 0;
     c.q2_irregular.release();
     c.q2_nirregular = -1;
   } else {
@@ -742,15 +823,26 @@ void q2lattice_assemble(Ctx &c, bool f32) {
   A.b = c.b.get();
   // templated nodes
   if (c.q2_nirregular >= 0) {
-    const Q2ClassHdr *hdr = (const Q2ClassHdr *)c.q2_cls_hdr.get();
-    if (f32)
-      hipLaunchKernelGGL((q2_class_kernel<float>), dim3(q2_class_grid(A.nnodes, false)),
-                         dim3(64 * kQ2Waves), 0, c.stream, A, hdr, c.q2_cls_pos.get(),
-                         c.q2_cls_code.get(), c.q2_node_class.get());
-    else
-      hipLaunchKernelGGL((q2_class_kernel<double>), dim3(q2_class_grid(A.nnodes, true)),
-                         dim3(64 * kQ2Waves), 0, c.stream, A, hdr, c.q2_cls_pos.get(),
-                         c.q2_cls_code.get(), c.q2_node_class.get());
+    static_assert(sizeof(Q2ClassHdr) == 48 && sizeof(Q2NodeRec) == 32, "record layouts");
+    const int4 *hdr = (const int4 *)c.q2_cls_hdr.get();
+    const int4 *rec = (const int4 *)c.q2_rec.get();
+    const char *wenv = std::getenv("GRIDAP_HIP_Q2_WAVES");
+    const bool w8 = wenv ? std::atoi(wenv) == 8 : f32;
+    if (c.q2_nrec > 0) {
+      const unsigned g = q2_class_grid(c.q2_nrec, !f32, w8 ? 8 : 16);
+      auto go = [&](auto kern, int w) {
+        hipLaunchKernelGGL(kern, dim3(g), dim3(64 * w), 0, c.stream, A, hdr,
+                           (const uint32_t *)c.q2_cls_pos.get(), c.q2_cls_code.get(), rec,
+                           c.q2_nrec);
+      };
+      if (f32) {
+        if (w8) go(q2_class_kernel<float, 8>, 8);
+        else go(q2_class_kernel<float, 16>, 16);
+      } else {
+        if (w8) go(q2_class_kernel<double, 8>, 8);
+        else go(q2_class_kernel<double, 16>, 16);
+      }
+    }
     GH_CHECK_HIP(hipGetLastError());
     if (c.q2_nirregular == 0) return;
     A.nodes = c.q2_irregular.get();
