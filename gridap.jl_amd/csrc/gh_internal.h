@@ -56,6 +56,10 @@ struct DBuf {
     GH_CHECK_HIP(hipMalloc(&p, count * sizeof(T)));
     n = count;
   }
+  // grow-only: keeps the buffer (and its contents) when it is large enough
+  void reserve(size_t count) {
+    if (count > n) resize(count);
+  }
   T *get() const { return p; }
 };
 
@@ -87,10 +91,39 @@ __host__ __device__ inline double det_inv(const double *J, double *Ji) {
   }
 }
 
+// Pinned host memory (small read-backs of the symbolic phase).
+template <class T>
+struct HBuf {
+  T *p = nullptr;
+  size_t n = 0;
+  HBuf() = default;
+  HBuf(const HBuf &) = delete;
+  HBuf &operator=(const HBuf &) = delete;
+  ~HBuf() {
+    if (p) (void)hipHostFree(p);
+  }
+  T *get(size_t count) {
+    if (count > n) {
+      if (p) (void)hipHostFree(p);
+      p = nullptr;
+      n = 0;
+      GH_CHECK_HIP(hipHostMalloc((void **)&p, count * sizeof(T), hipHostMallocDefault));
+      n = count;
+    }
+    return p;
+  }
+};
+
 struct Ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   std::string err;
+  // workspace of the lattice symbolic phase, kept between calls (it is rebuilt for
+  // every pattern, so reusing it saves the allocations, not any result)
+  DBuf<int32_t> ws_f, ws_scan, ws_count, ws_start, ws_flag;
+  DBuf<int64_t> ws_box;
+  DBuf<unsigned char> ws_cub;
+  HBuf<int64_t> h_small;
 
   // ---- mesh ----
   int dim = 0;

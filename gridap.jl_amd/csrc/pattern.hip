@@ -311,6 +311,15 @@ void exclusive_sum(const T *in, T *out, int64_t n, hipStream_t s) {
   GH_CHECK_HIP(hipStreamSynchronize(s));
 }
 
+// The same scan with a kept temporary buffer: enqueued only.
+template <class T>
+void exclusive_sum_ws(const T *in, T *out, int64_t n, hipStream_t s, DBuf<unsigned char> &ws) {
+  size_t bytes = 0;
+  GH_CHECK_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, in, out, (int)n, s));
+  ws.reserve(std::max<size_t>(bytes, 1));
+  GH_CHECK_HIP(hipcub::DeviceScan::ExclusiveSum(ws.get(), bytes, in, out, (int)n, s));
+}
+
 }  // namespace
 
 void symbolic_general(Ctx &c, bool contributions) {
@@ -489,11 +498,243 @@ void detect_box(Ctx &c) {
 // column written directly.  False (nothing set) when a check fails; the caller then
 // runs symbolic_general.  On success the lattice kernel's inputs (node_dof,
 // node_fmask, tile starts) are set as check_structured sets them.
+// 32-bit forms of the lattice kernels (every index of the lattice fits): the 64-bit
+// divisions of the general forms dominate their time.
+template <class I>
+__device__ __forceinline__ I cart_node_i(int D, const int64_t *np, I cell, const int64_t *nc,
+                                         int a) {
+  I node = 0, stride = 1, r = cell;
+  for (int d = 0; d < D; ++d) {
+    const I ci = r % (I)nc[d];
+    r /= (I)nc[d];
+    node += (ci + ((a >> d) & 1)) * stride;
+    stride *= (I)np[d];
+  }
+  return node;
+}
+
+__global__ void lattice_scatter32(Lattice L, int32_t total, int nloc,
+                                  const int32_t *__restrict__ cell_dofs,
+                                  int32_t *__restrict__ node_dof) {
+  const int32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= total) return;
+  const int32_t c = t / nloc;
+  node_dof[cart_node_i<int32_t>(L.D, L.np, c, L.nc, t - c * nloc)] = cell_dofs[t];
+}
+
+__global__ void lattice_verify32(Lattice L, int32_t total, int nloc,
+                                 const int32_t *__restrict__ cell_dofs,
+                                 const int32_t *__restrict__ node_dof, int32_t *bad) {
+  const int32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= total) return;
+  const int32_t c = t / nloc;
+  if (node_dof[cart_node_i<int32_t>(L.D, L.np, c, L.nc, t - c * nloc)] != cell_dofs[t])
+    atomicOr(bad, 1);
+}
+
+// lattice_masks with 32-bit indices
+__global__ void lattice_masks32(Lattice L, int32_t nnodes, const int32_t *__restrict__ node_dof,
+                                const int32_t *__restrict__ scan, uint32_t *__restrict__ fmask,
+                                int32_t *__restrict__ count, int32_t *bad) {
+  const int32_t n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= nnodes) return;
+  const int32_t d = node_dof[n];
+  uint32_t m = 0u;
+  if (d == 0) atomicOr(bad, 2);
+  if (d > 0 && d != scan[n] + 1) atomicOr(bad, 4);
+  if (d > 0 && d == scan[n] + 1) {
+    int32_t xi[3] = {0, 0, 0}, r = n;
+    for (int k = 0; k < L.D; ++k) {
+      xi[k] = r % (int32_t)L.np[k];
+      r /= (int32_t)L.np[k];
+    }
+    int nb = 1;
+    for (int k = 0; k < L.D; ++k) nb *= 3;
+    for (int o = 0; o < nb; ++o) {
+      int oo = o;
+      int32_t node = 0, stride = 1;
+      bool in = true;
+      for (int k = 0; k < L.D; ++k) {
+        const int32_t v = xi[k] + (oo % 3) - 1;
+        oo /= 3;
+        in = in && v >= 0 && v < (int32_t)L.np[k];
+        node += v * stride;
+        stride *= (int32_t)L.np[k];
+      }
+      if (in && node_dof[node] > 0) m |= 1u << o;
+    }
+  }
+  fmask[n] = m;
+  count[n] = __popc(m);
+}
+
+// One thread per (node, stencil offset): the row of each free neighbour at its place in
+// the node's column (consecutive threads write consecutive rows), and colptr.
+__global__ void lattice_rows32(Lattice L, int32_t nnodes, int nb,
+                               const int32_t *__restrict__ node_dof,
+                               const uint32_t *__restrict__ fmask,
+                               const int32_t *__restrict__ start, int32_t *__restrict__ colptr,
+                               int32_t *__restrict__ rowval) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int32_t n = (int32_t)(t / nb);
+  const int o = (int)(t - (int64_t)n * nb);
+  if (n >= nnodes) return;
+  const int32_t d = node_dof[n];
+  if (d <= 0) return;
+  const int32_t p = start[n];
+  if (o == 0) colptr[d - 1] = p;
+  const uint32_t m = fmask[n];
+  if (!((m >> o) & 1u)) return;
+  int32_t off = 0, stride = 1;
+  int oo = o;
+  for (int k = 0; k < L.D; ++k) {
+    off += ((oo % 3) - 1) * stride;
+    oo /= 3;
+    stride *= (int32_t)L.np[k];
+  }
+  rowval[p + __popc(m & ((1u << o) - 1u))] = node_dof[n + off] - 1;
+}
+
+// Clears the flags and the box record: box[0], box[1] = node of dof 1 and of the last
+// dof, box[2..4] / box[5..7] = their lattice coordinates, box[8] = 1 when the free nodes
+// span exactly that box of the lattice.
+__global__ void lattice_init(int32_t *flag, int64_t *box) {
+  if (threadIdx.x == 0) flag[0] = 0;
+  if (threadIdx.x < 9) box[threadIdx.x] = threadIdx.x < 2 ? -1 : 0;
+}
+
+// out = {flags, free count, nnz}; colptr[nfree] = nnz.
+__global__ void lattice_summary(int64_t nnodes, int64_t nfree, const int32_t *flag,
+                                const int32_t *f, const int32_t *scan, const int32_t *count,
+                                const int32_t *start, int32_t *colptr, int64_t *out) {
+  if (threadIdx.x != 0) return;
+  const int64_t nnz = (int64_t)start[nnodes - 1] + count[nnodes - 1];
+  out[0] = flag[0];
+  out[1] = (int64_t)scan[nnodes - 1] + f[nnodes - 1];
+  out[2] = nnz;
+  colptr[nfree] = (int32_t)nnz;
+}
+
+__global__ void box_bounds(int64_t np0, int64_t np1, int64_t nfree, int64_t *box) {
+  if (threadIdx.x != 0) return;
+  if (box[0] < 0 || box[1] < 0) return;
+  int64_t vol = 1;
+  for (int k = 0; k < 2; ++k) {
+    const int64_t n = box[k];
+    box[2 + 3 * k] = n % np0;
+    box[3 + 3 * k] = (n / np0) % np1;
+    box[4 + 3 * k] = n / (np0 * np1);
+  }
+  for (int d = 0; d < 3; ++d) {
+    if (box[5 + d] < box[2 + d]) return;
+    vol *= box[5 + d] - box[2 + d] + 1;
+  }
+  box[8] = vol == nfree;
+}
+
+__global__ void box_outside32(int32_t nnodes, int32_t np0, int32_t np1, int64_t *box,
+                              const int32_t *__restrict__ node_dof) {
+  const int32_t n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= nnodes || box[8] == 0 || node_dof[n] <= 0) return;
+  const int32_t x[3] = {n % np0, (n / np0) % np1, n / (np0 * np1)};
+  for (int d = 0; d < 3; ++d)
+    if (x[d] < box[2 + d] || x[d] > box[5 + d]) box[8] = 0;
+}
+
+// Lattice pattern with 32-bit indices: a fixed number of launches, the workspace
+// kept in the context, and two reads back (the counts, then the box).
+bool symbolic_lattice32(Ctx &c) {
+  hipStream_t s = c.stream;
+  Lattice L;
+  L.D = c.dim;
+  for (int d = 0; d < 3; ++d) {
+    L.nc[d] = d < c.dim ? c.part[d] : 1;
+    L.np[d] = d < c.dim ? c.part[d] + 1 : 1;
+  }
+  const int32_t nn = (int32_t)c.nnodes;
+  c.node_dof.resize(c.nnodes);
+  c.node_fmask.resize(c.nnodes);
+  c.ws_f.resize(c.nnodes);
+  c.ws_scan.resize(c.nnodes);
+  c.ws_count.resize(c.nnodes);
+  c.ws_start.resize(c.nnodes);
+  c.ws_flag.resize(1);
+  c.ws_box.resize(9);
+  c.colptr.resize(c.nfree + 1);
+  int64_t *h = c.h_small.get(16);
+  hipLaunchKernelGGL(lattice_init, dim3(1), dim3(64), 0, s, c.ws_flag.get(), c.ws_box.get());
+  const int32_t t = (int32_t)(c.ncells * c.nloc);
+  hipLaunchKernelGGL(lattice_scatter32, dim3(grid_for(t)), dim3(kBlock), 0, s, L, t, c.nloc,
+                     c.cell_dofs.get(), c.node_dof.get());
+  hipLaunchKernelGGL(lattice_verify32, dim3(grid_for(t)), dim3(kBlock), 0, s, L, t, c.nloc,
+                     c.cell_dofs.get(), c.node_dof.get(), c.ws_flag.get());
+  hipLaunchKernelGGL(free_flags, dim3(grid_for(nn)), dim3(kBlock), 0, s, c.nnodes,
+                     c.node_dof.get(), c.ws_f.get(), c.ws_flag.get());
+  GH_CHECK_HIP(hipGetLastError());
+  exclusive_sum_ws(c.ws_f.get(), c.ws_scan.get(), c.nnodes, s, c.ws_cub);
+  hipLaunchKernelGGL(lattice_masks32, dim3(grid_for(nn)), dim3(kBlock), 0, s, L, nn,
+                     c.node_dof.get(), c.ws_scan.get(), c.node_fmask.get(), c.ws_count.get(),
+                     c.ws_flag.get());
+  GH_CHECK_HIP(hipGetLastError());
+  exclusive_sum_ws(c.ws_count.get(), c.ws_start.get(), c.nnodes, s, c.ws_cub);
+  hipLaunchKernelGGL(lattice_summary, dim3(1), dim3(64), 0, s, c.nnodes, c.nfree,
+                     c.ws_flag.get(), c.ws_f.get(), c.ws_scan.get(), c.ws_count.get(),
+                     c.ws_start.get(), c.colptr.get(), c.ws_box.get());
+  GH_CHECK_HIP(hipGetLastError());
+  GH_CHECK_HIP(hipMemcpyAsync(h, c.ws_box.get(), 3 * sizeof(int64_t), hipMemcpyDeviceToHost, s));
+  GH_CHECK_HIP(hipStreamSynchronize(s));
+  if (h[0] != 0 || h[1] != c.nfree) return false;
+  const int64_t nnz = h[2];
+  c.nnz = nnz;
+  c.rowval.resize(std::max<int64_t>(nnz, 1));
+  int nb = 1;
+  for (int k = 0; k < c.dim; ++k) nb *= 3;
+  hipLaunchKernelGGL(lattice_rows32, dim3(grid_for((int64_t)nn * nb)), dim3(kBlock), 0, s, L, nn,
+                     nb, c.node_dof.get(), c.node_fmask.get(), c.ws_start.get(), c.colptr.get(),
+                     c.rowval.get());
+  const int64_t ntiles = (c.nnodes + kTileNodes - 1) / kTileNodes;
+  c.tile_p0.resize(ntiles);
+  c.tile_d0.resize(ntiles);
+  hipLaunchKernelGGL(lattice_tile_p0, dim3(grid_for(ntiles)), dim3(kBlock), 0, s, c.nnodes,
+                     ntiles, c.node_dof.get(), c.colptr.get(), c.tile_p0.get(), c.tile_d0.get());
+  GH_CHECK_HIP(hipGetLastError());
+  // box of free nodes (the stencil PCG), decided on the device
+  c.box = false;
+  c.a27.release();
+  c.g14.release();
+  if (c.dim == 3) {
+    hipLaunchKernelGGL(lattice_init, dim3(1), dim3(64), 0, s, c.ws_flag.get(), c.ws_box.get());
+    hipLaunchKernelGGL(box_corners, dim3(grid_for(nn)), dim3(kBlock), 0, s, c.nnodes, c.nfree,
+                       c.node_dof.get(), c.ws_box.get());
+    hipLaunchKernelGGL(box_bounds, dim3(1), dim3(64), 0, s, L.np[0], L.np[1], c.nfree,
+                       c.ws_box.get());
+    hipLaunchKernelGGL(box_outside32, dim3(grid_for(nn)), dim3(kBlock), 0, s, nn,
+                       (int32_t)L.np[0], (int32_t)L.np[1], c.ws_box.get(), c.node_dof.get());
+    GH_CHECK_HIP(hipGetLastError());
+    GH_CHECK_HIP(hipMemcpyAsync(h + 3, c.ws_box.get(), 9 * sizeof(int64_t),
+                                hipMemcpyDeviceToHost, s));
+  }
+  GH_CHECK_HIP(hipStreamSynchronize(s));
+  if (c.dim == 3 && h[3 + 8] == 1) {
+    for (int d = 0; d < 3; ++d) c.box_m[d] = h[3 + 5 + d] - h[3 + 2 + d] + 1;
+    c.box = true;
+  }
+  c.cstart.release();
+  c.perm.release();
+  c.have_pattern = true;
+  c.structured = true;
+  c.nzval.resize(std::max<int64_t>(nnz, 1));
+  c.b.resize(std::max<int64_t>(c.nfree, 1));
+  return true;
+}
+
 bool symbolic_lattice(Ctx &c) {
   c.structured = false;
   if (!(c.cartesian && c.reffe_kind == GH_REFFE_QK && c.order == 1 && c.ncomp == 1 &&
         c.nloc == (1 << c.dim) && c.nfree > 0 && c.nnodes < (int64_t(1) << 31)))
     return false;
+  if (c.ncells * c.nloc < (int64_t(1) << 31) && c.nnodes * 27 < (int64_t(1) << 31))
+    return symbolic_lattice32(c);
   hipStream_t s = c.stream;
   Lattice L;
   L.D = c.dim;
