@@ -351,6 +351,7 @@ struct CellArgs {
   int nq;
   const double *tab;  // per q: w, shp[NLOC], grad[NLOC*D]
   const double *kappa, *fsrc, *dvals;
+  const double *sigma;  // mass coefficient per cell, or nullptr (no mass term)
   const double *fq;  // f at the quadrature points (cells x nq) or nullptr
   const int32_t *colptr, *rowval;
   double *nzval, *b;
@@ -429,6 +430,13 @@ __global__ __launch_bounds__(128) void cell_q1_kernel(const CellArgs A) {
       }
       bv[i] += (T)tq[1 + i] * ((T)(A.fq ? A.fq[cell * A.nq + q] : fc) * dV);
     }
+    if (A.sigma) {  // + σ φ_i φ_j at this point
+      const T sd = (T)A.sigma[cell] * dV;
+#pragma unroll
+      for (int i = 0; i < NL; ++i)
+#pragma unroll
+        for (int j = 0; j < NL; ++j) K[i][j] += (T)tq[1 + i] * (T)tq[1 + j] * sd;
+    }
   }
   // Dirichlet lifting b -= K u_d on the cell.
   if (A.dvals) {
@@ -500,6 +508,7 @@ void launch_general(Ctx &c, int precision) {
   A.tab = dtab.get();
   A.kappa = c.have_kappa ? c.kappa.get() : nullptr;
   A.fsrc = c.have_fsrc ? c.fsrc.get() : nullptr;
+  A.sigma = c.have_sigma ? c.sigma.get() : nullptr;
   A.dvals = c.have_dvals ? c.dvals.get() : nullptr;
   A.fq = c.have_fq ? c.fq.get() : nullptr;
   A.colptr = c.colptr.get();
@@ -531,6 +540,8 @@ void assemble_diffusion(Ctx &c, int precision, int path) {
              "kappa was uploaded for a different mesh");
   GH_REQUIRE(!c.have_dvals || c.dvals.n == (size_t)c.ndiri, GH_ERR_STATE,
              "Dirichlet values were uploaded for a different space");
+  GH_REQUIRE(!c.have_sigma || (c.reffe_kind == GH_REFFE_QK && c.order == 1 && c.ncomp == 1),
+             GH_ERR_UNSUPPORTED, "the cell mass term is provided for scalar Q1 spaces");
   if (c.elemlist && simplex_eligible(c)) {
     GH_REQUIRE(path != GH_PATH_STRUCTURED, GH_ERR_UNSUPPORTED,
                "the lattice path needs a Cartesian mesh, a scalar Q1 CLagrangian space and a "
@@ -541,9 +552,9 @@ void assemble_diffusion(Ctx &c, int precision, int path) {
   GH_REQUIRE(!c.have_fq || c.fq.n == (size_t)c.ncells * c.nq, GH_ERR_STATE,
              "the quadrature-point source was uploaded for another mesh or quadrature");
   double Kc[8] = {0}, b0 = 0.0;
-  // the lattice kernel takes f constant per cell; f at the quadrature points runs the
-  // per-cell kernel
-  bool lattice = c.structured && path != GH_PATH_GENERAL && !c.have_fq;
+  // the lattice kernel takes f constant per cell and no mass term; f at the quadrature
+  // points or a mass term runs the per-cell kernel
+  bool lattice = c.structured && path != GH_PATH_GENERAL && !c.have_fq && !c.have_sigma;
   if (lattice) {
     GH_REQUIRE((int)c.shp.size() == c.nq * c.nloc, GH_ERR_STATE,
                "tabulation does not match the element (dofs per cell)");

@@ -57,6 +57,8 @@ void invalidate_space(gh::Ctx &c) {
   c.h_cell_dofs.clear();
   c.h_cell_dofs.shrink_to_fit();
   c.have_kappa = c.have_fsrc = c.have_dvals = false;
+  c.have_sigma = false;
+  c.sigma.release();
   c.have_fq = false;
   c.fq.release();
   c.have_state = false;
@@ -395,6 +397,24 @@ int gh_coef_upload(gh_ctx *ctx, const double *kappa, const double *fsrc,
     up(c.kappa, c.have_kappa, kappa, c.ncells);
     up(c.fsrc, c.have_fsrc, fsrc, c.ncells);
     up(c.dvals, c.have_dvals, dirichlet_vals, c.ndiri);
+    GH_CHECK_HIP(hipStreamSynchronize(c.stream));
+  });
+}
+
+int gh_mass_upload(gh_ctx *ctx, const double *sigma) {
+  return guarded(ctx, [&](gh::Ctx &c) {
+    GH_REQUIRE(c.have_space, GH_ERR_STATE, "upload the FE space before coefficients");
+    c.have_values = false;
+    c.have_sigma = sigma != nullptr && c.ncells > 0;
+    if (!c.have_sigma) {
+      c.sigma.release();
+      return;
+    }
+    GH_REQUIRE(c.reffe_kind == GH_REFFE_QK && c.order == 1 && c.ncomp == 1, GH_ERR_UNSUPPORTED,
+               "the cell mass term is provided for scalar Q1 spaces");
+    c.sigma.resize(c.ncells);
+    GH_CHECK_HIP(hipMemcpyAsync(c.sigma.get(), sigma, sizeof(double) * c.ncells,
+                                hipMemcpyHostToDevice, c.stream));
     GH_CHECK_HIP(hipStreamSynchronize(c.stream));
   });
 }

@@ -231,6 +231,8 @@ struct Ctx {
   // ---- coefficients / values ----
   DBuf<double> kappa, fsrc, dvals;
   bool have_kappa = false, have_fsrc = false, have_dvals = false;
+  DBuf<double> sigma;               // per cell: mass (reaction) coefficient of ∫ σ u v
+  bool have_sigma = false;
   DBuf<double> fq;                  // f at the quadrature points, cells x nq (general path)
   bool have_fq = false;
   bool have_values = false;

@@ -31,6 +31,8 @@ from .reference_fes import Float64, Quadrature, ReferenceFE, VectorValue, lagran
 from .nonlinear import (FEOperator, FESolver, NewtonRaphsonSolver, jacobian, residual,
                         residual_and_jacobian, solve_inplace)
 from .io import DiscreteModelFromFile, from_json, from_json_file, to_json, to_json_file
+from .multifield import (BlockMultiFieldStyle, BlockSparseMatrix, BlockVector,
+                         ConsecutiveMultiFieldStyle, MultiFieldFESpace)
 
 __all__ = [
     "AffineFEOperator", "CartesianDiscreteModel", "CellField", "DomainContribution",
@@ -47,4 +49,6 @@ __all__ = [
     "residual", "residual_and_jacobian", "solve_inplace",
     "BoundaryTriangulation", "get_normal_vector", "interpolate",
     "DiscreteModelFromFile", "from_json", "from_json_file", "to_json", "to_json_file",
+    "BlockMultiFieldStyle", "BlockSparseMatrix", "BlockVector", "ConsecutiveMultiFieldStyle",
+    "MultiFieldFESpace",
 ]

@@ -74,6 +74,7 @@ SIGNATURES = {
     "gh_symbolic": (_i32, [_vp, _pi64]),
     "gh_pattern_get": (_i32, [_vp, _vp, _vp]),
     "gh_coef_upload": (_i32, [_vp, _vp, _vp, _vp]),
+    "gh_mass_upload": (_i32, [_vp, _vp]),
     "gh_assemble": (_i32, [_vp, _i32, _i32, _i32]),
     "gh_source_qp_upload": (_i32, [_vp, _vp]),
     "gh_values_get": (_i32, [_vp, _vp, _vp]),
@@ -236,6 +237,11 @@ class Context:
         g = None if dirichlet_vals is None else np.ascontiguousarray(dirichlet_vals,
                                                                      dtype=np.float64)
         self.check(lib().gh_coef_upload(self.h, ptr(k), ptr(f), ptr(g)))
+
+    def mass_upload(self, sigma=None):
+        """Per-cell mass coefficient σ of ∫ σ u v (None: no mass term)."""
+        a = None if sigma is None else np.ascontiguousarray(sigma, dtype=np.float64)
+        self.check(lib().gh_mass_upload(self.h, ptr(a)))
 
     def source_qp_upload(self, fq=None):
         """f at the quadrature points, [ncells, nq] (None: the per-cell f again)."""

@@ -15,7 +15,7 @@ from ._lib import Context
 from .boundary import lower_boundary
 from .celldata import CellField, CoefProduct, FEValue, PointField
 from .celldata import (BodyForceTerm, DiffusionTerm, DomainContribution, ElasticityTerm,
-                       SourceTerm, _cell_coef, get_fe_basis, get_trial_fe_basis)
+                       MassTerm, SourceTerm, _cell_coef, get_fe_basis, get_trial_fe_basis)
 from .geometry import BoundaryTriangulation
 from .fespaces import FEFunction, TrialFESpace
 from .reference_fes import Quadrature
@@ -61,6 +61,12 @@ class SparseMatrixAssembler:
     _PATH = {"auto": _lib.GH_PATH_AUTO, "general": _lib.GH_PATH_GENERAL,
              "structured": _lib.GH_PATH_STRUCTURED}
 
+    def __new__(cls, U, V, *args, **kwargs):
+        from .multifield import MultiFieldFESpace, MultiFieldSparseMatrixAssembler
+        if isinstance(U, MultiFieldFESpace) or isinstance(V, MultiFieldFESpace):
+            return MultiFieldSparseMatrixAssembler(U, V, *args, **kwargs)
+        return super().__new__(cls)
+
     def __init__(self, U, V, device=0, precision="fp64", path="auto"):
         Us, Vs = _space(U), _space(V)
         if Us is not Vs:
@@ -81,6 +87,7 @@ class SparseMatrixAssembler:
         self._face_degree = None    # degree of the face tabulation on the device
         self._has_boundary = False  # boundary terms uploaded
         self._has_fq = False        # a source at the quadrature points uploaded
+        self._has_sigma = False     # a cell mass coefficient uploaded
 
     def get_rows(self):
         return self.space.get_free_dof_ids()
@@ -134,7 +141,8 @@ class SparseMatrixAssembler:
         veccontrib, vface = self._split(veccontrib)
         if had_mat and matcontrib is None:
             raise NotImplementedError("a bilinear form needs a cell (volume) term")
-        kappa, fsrc, degs = None, None, set()
+        kappa, fsrc, sigma, degs = None, None, None, set()
+        have_stiff = False
         point_sources = []  # (coefficient, degree) of sources f(x) given as functions
         nu, force = None, None
         have_mat = have_vec = False
@@ -148,14 +156,24 @@ class SparseMatrixAssembler:
                     nu = term.nu
                 elif isinstance(term, DiffusionTerm):
                     kinds.add(_lib.GH_FORM_DIFFUSION)
+                elif isinstance(term, MassTerm):
+                    # σ u v: the mass coefficient of the diffusion form
+                    kinds.add(_lib.GH_FORM_DIFFUSION)
+                    coef = _cell_coef(term.coef)
+                    sg = coef.per_cell(ncells) if coef is not None else None
+                    sg = np.ones(ncells) if sg is None else sg
+                    sigma = sg if sigma is None else sigma + sg
+                    degs.add(meas.degree)
+                    have_mat = True
+                    continue
                 else:
-                    raise NotImplementedError("bilinear forms: κ ∇v⋅∇u or ε(v) : σ(ε(u))")
+                    raise NotImplementedError("bilinear forms: κ ∇v⋅∇u, σ u v or ε(v) : σ(ε(u))")
                 coef = _cell_coef(term.coef)
                 k = coef.per_cell(ncells) if coef is not None else None
                 k = np.ones(ncells) if k is None else k
                 kappa = k if kappa is None else kappa + k
                 degs.add(meas.degree)
-                have_mat = True
+                have_mat = have_stiff = True
         if veccontrib is not None:
             for term, meas in veccontrib.terms:
                 if isinstance(term, BodyForceTerm):
@@ -197,10 +215,10 @@ class SparseMatrixAssembler:
                 fsrc = None
             if not have_vec:
                 fsrc = np.zeros(ncells)
-        if not have_mat:
+        if not have_stiff:
             kappa = np.zeros(ncells)
         extra = dict(nu=0.3 if nu is None else nu,
-                     force=np.zeros(3) if force is None else force)
+                     force=np.zeros(3) if force is None else force, sigma=sigma)
         if point_sources:
             # f at the quadrature points: the per-cell sources join it, fsrc is unused
             deg = next(iter(degs))
@@ -230,19 +248,25 @@ class SparseMatrixAssembler:
                                  bnd["gq"], bnd["hq"])
         self._has_boundary = True
 
-    def assemble(self, matcontrib, veccontrib, lift, A=None, b=None):
+    def assemble(self, matcontrib, veccontrib, lift, A=None, b=None, trial=None):
         """Device numeric phase; returns (SparseMatrixCSC, b).  With A (a matrix on this
         assembler's pattern) and/or b given, their values are overwritten in place and
         they are returned (fillstored! then the numeric loop,
-        src/FESpaces/SparseMatrixAssemblers.jl:60-63)."""
+        src/FESpaces/SparseMatrixAssemblers.jl:60-63).  trial: the space whose Dirichlet
+        values are lifted (default: the assembler's U)."""
         form, kappa, fsrc, degree, extra = self._lower(matcontrib, veccontrib)
         self._setup(degree)
         dvals = None
         if lift:
-            dv = self.U.dirichlet_values() if isinstance(self.U, TrialFESpace) else None
+            U = self.U if trial is None else trial
+            dv = U.dirichlet_values() if isinstance(U, TrialFESpace) else None
             if dv is not None and dv.size and np.any(dv != 0):
                 dvals = dv
         self.ctx.coef_upload(kappa, fsrc, dvals)
+        sigma = extra.get("sigma")
+        if sigma is not None or self._has_sigma:
+            self.ctx.mass_upload(sigma)
+            self._has_sigma = sigma is not None
         fq = extra.get("fq")
         if fq is not None or self._has_fq:
             self.ctx.source_qp_upload(fq)
@@ -317,10 +341,15 @@ def _contrib(form, *args):
     return out
 
 
+def _is_assembler(x):
+    from .multifield import MultiFieldSparseMatrixAssembler
+    return isinstance(x, (SparseMatrixAssembler, MultiFieldSparseMatrixAssembler))
+
+
 def assemble_matrix(a, U, V=None, assem=None):
     """assemble_matrix(a, U, V) or assemble_matrix(a, assem, U, V)
     (src/FESpaces/Assemblers.jl:372-375)."""
-    if isinstance(U, SparseMatrixAssembler):
+    if _is_assembler(U):
         assem, U, V = U, V, assem
     V = U if V is None else V
     assem = assem or SparseMatrixAssembler(U, V)
@@ -330,7 +359,7 @@ def assemble_matrix(a, U, V=None, assem=None):
 
 def assemble_vector(l, V, assem=None):
     """assemble_vector(l, V) (src/FESpaces/Assemblers.jl:382-385)."""
-    if isinstance(V, SparseMatrixAssembler):
+    if _is_assembler(V):
         assem, V = V, assem
     assem = assem or SparseMatrixAssembler(V, V)
     vcon = _contrib(l, get_fe_basis(V))

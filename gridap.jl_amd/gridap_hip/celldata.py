@@ -6,6 +6,7 @@ Gridap builds lazy cell arrays from `∫(κ*∇(v)⋅∇(u))dΩ`
 form kind plus per-cell coefficients, which is what the device kernels consume:
 
     a(u,v) = integrate(κ * inner(gradient(v), gradient(u)), dΩ)   -> diffusion, κ
+    a(u,v) = integrate(σ * u * v, dΩ)                             -> mass, σ (scalar Q1)
     l(v)   = integrate(f * v, dΩ)                                 -> source, f
     a(u,v) = integrate(inner(ε(v), σ(ε(u))), dΩ), σ = IsotropicStress(E, ν)
                                                                   -> elasticity, E, ν
@@ -100,6 +101,7 @@ def _coef_product(a, b):
 
 class _Expr:
     coef = None
+    fields = None   # multi-field: (test field,) or (test field, trial field)
 
     def __mul__(self, other):
         return _scale(other, self)
@@ -126,11 +128,19 @@ def _terms(e):
 
 
 class FEBasis(_Expr):
-    """get_fe_basis(V) (test, v) or get_trial_fe_basis(U) (trial, u)."""
+    """get_fe_basis(V) (test, v) or get_trial_fe_basis(U) (trial, u); field: the index of
+    the field in a MultiFieldFESpace (None for a single-field space)."""
 
-    def __init__(self, space, trial):
+    def __init__(self, space, trial, field=None):
         self.space = space
         self.trial = trial
+        self.field = field
+
+
+def _f(term, fields):
+    """term with its multi-field (test[, trial]) field indices recorded."""
+    term.fields = fields
+    return term
 
 
 class _Grad(_Expr):
@@ -311,23 +321,24 @@ def _product(x, y):
     cy = _is_coef(y)
     if isinstance(x, FEBasis) and not x.trial:                       # v * y
         if isinstance(y, FEBasis) and y.trial:
-            return MassTerm()
+            return _f(MassTerm(), (x.field, y.field))
         if isinstance(y, _NormalGrad) and y.basis.trial:
             return NitscheTerm("v_dnu")
         if cy:
-            return SourceTerm(_as_coef(y))
+            return _f(SourceTerm(_as_coef(y)), (x.field,))
     if isinstance(x, _NormalGrad) and not x.basis.trial:            # (n⋅∇v) * y
         if isinstance(y, FEBasis) and y.trial:
             return NitscheTerm("dnv_u")
         if cy:
             return NormalLoadTerm(_as_coef(y))
     if isinstance(x, SourceTerm):                                    # (c v) * y
+        tf = x.fields[0] if x.fields else None
         if isinstance(y, FEBasis) and y.trial:
-            return MassTerm(x.coef)
+            return _f(MassTerm(x.coef), (tf, y.field))
         if isinstance(y, _NormalGrad) and y.basis.trial:
             return NitscheTerm("v_dnu", x.coef)
         if cy:
-            return SourceTerm(_coef_product(x.coef, _as_coef(y)))
+            return _f(SourceTerm(_coef_product(x.coef, _as_coef(y))), x.fields)
     if isinstance(x, NormalLoadTerm):                                # (c n⋅∇v) * y
         if isinstance(y, FEBasis) and y.trial:
             return NitscheTerm("dnv_u", x.coef)
@@ -335,7 +346,7 @@ def _product(x, y):
             return NormalLoadTerm(_coef_product(x.coef, _as_coef(y)))
     if cy:
         if isinstance(x, (DiffusionTerm, MassTerm)):
-            return type(x)(_coef_product(x.coef, _as_coef(y)))
+            return _f(type(x)(_coef_product(x.coef, _as_coef(y))), x.fields)
         if isinstance(x, NitscheTerm):
             return NitscheTerm(x.part, _coef_product(x.coef, _as_coef(y)))
         if isinstance(x, (PLaplacianResidualTerm, PLaplacianJacobianTerm)):
@@ -418,7 +429,8 @@ def inner(a, b):
             return NormalGradValue(b.uh)
         raise NotImplementedError("n⋅x is provided for gradients only")
     if isinstance(a, _Grad) and isinstance(b, _Grad) and a.basis.trial != b.basis.trial:
-        return DiffusionTerm()
+        v, u = (b, a) if a.basis.trial else (a, b)
+        return _f(DiffusionTerm(), (v.basis.field, u.basis.field))
     if isinstance(a, _Grad) and not a.basis.trial and isinstance(b, _Flux):
         return PLaplacianResidualTerm(b.p, b.state)
     if isinstance(a, _Grad) and not a.basis.trial and isinstance(b, _DFlux):
@@ -473,7 +485,7 @@ class DomainContribution:
         return NotImplemented
 
 
-_CELL_TERMS = (DiffusionTerm, SourceTerm, ElasticityTerm, BodyForceTerm,
+_CELL_TERMS = (DiffusionTerm, MassTerm, SourceTerm, ElasticityTerm, BodyForceTerm,
                PLaplacianResidualTerm, PLaplacianJacobianTerm)
 _FACE_TERMS = (MassTerm, NitscheTerm, SourceTerm, NormalLoadTerm)
 
@@ -500,8 +512,16 @@ def integral(expr):
 
 
 def get_fe_basis(V):
+    """v = get_fe_basis(V); for a MultiFieldFESpace the tuple (v1, v2, ...)."""
+    from .multifield import MultiFieldFESpace
+    if isinstance(V, MultiFieldFESpace):
+        return tuple(FEBasis(Vi, trial=False, field=i) for i, Vi in enumerate(V.spaces))
     return FEBasis(V, trial=False)
 
 
 def get_trial_fe_basis(U):
+    """u = get_trial_fe_basis(U); for a MultiFieldFESpace the tuple (u1, u2, ...)."""
+    from .multifield import MultiFieldFESpace
+    if isinstance(U, MultiFieldFESpace):
+        return tuple(FEBasis(Ui, trial=True, field=i) for i, Ui in enumerate(U.spaces))
     return FEBasis(U, trial=True)

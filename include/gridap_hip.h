@@ -155,6 +155,12 @@ int gh_pattern_get(gh_ctx *ctx, int64_t *colptr, int64_t *rowval);
 int gh_coef_upload(gh_ctx *ctx, const double *kappa, const double *fsrc,
                    const double *dirichlet_vals);
 
+/* Per-cell mass (reaction) coefficient σ of the diffusion form: a(u,v) += ∫ σ u v dΩ
+ * (NULL: no mass term).  The u⋅v terms of the multi-field forms of
+ * test/MultiFieldTests/BlockSparseMatrixAssemblersTests.jl:101-104 (∫ u2⋅v2 - u1⋅v2).
+ * Scalar Q1; the assembly then runs the per-cell kernel. */
+int gh_mass_upload(gh_ctx *ctx, const double *sigma);
+
 /* A source f(x) given at the quadrature points of every cell, fq[cell*nq + q] (the
  * rule of the uploaded tabulation), in place of the per-cell f of gh_coef_upload
  * (NULL: back to the per-cell f).  The x-dependent load of

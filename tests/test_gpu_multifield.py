@@ -1,0 +1,202 @@
+"""Multi-field assembly on the device against the oracle (needs an MI355X).
+
+Mirrors test/MultiFieldTests/BlockSparseMatrixAssemblersTests.jl: two and three copies
+of a Q1 space on a 5x5 QUAD model with Dirichlet data sol(x) = x1 + x2 on the boundary,
+biform2/liform2 and biform3/liform3 (:101-104), assembled with the consecutive style
+(one matrix) and with the block styles BlockMultiFieldStyle() and
+BlockMultiFieldStyle(2, (1, n-1)) for trial and test: every block layout equals the
+monolithic matrix (:48-75), and the monolithic one equals the oracle (pattern bit-exact,
+values and b within 1e-12).  The mass term ∫ σ u v alone is checked on a 3-D mesh.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import gridap_hip as G
+from gridap_hip import _lib
+import multifield_oracle as MF
+
+pytestmark = pytest.mark.gpu
+
+
+def announce(msg):
+    print("[gpu-case] " + msg, flush=True)
+    sys.stdout.flush()
+
+
+def relmax(a, b):
+    return np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300) if a.size else 0.0
+
+
+sol = lambda x: x[..., 0] + x[..., 1]
+
+
+def biform2(u, v, dΩ):
+    (u1, u2), (v1, v2) = u, v
+    return G.integrate(G.inner(G.gradient(u1), G.gradient(v1)) + G.inner(u2, v2) -
+                       G.inner(u1, v2), dΩ)
+
+
+def liform2(v, dΩ):
+    v1, v2 = v
+    return G.integrate(v1 * 1.0 - v2 * 1.0, dΩ)
+
+
+def biform3(u, v, dΩ):
+    (u1, u2, u3), (v1, v2, v3) = u, v
+    return G.integrate(G.inner(G.gradient(u1), G.gradient(v1)) + G.inner(u2, v2) -
+                       G.inner(u1, v2) - G.inner(u3, v2) - G.inner(u2, v3), dΩ)
+
+
+def liform3(v, dΩ):
+    v1, v2, v3 = v
+    return G.integrate(v1 * 1.0 - v2 * 1.0 + v3 * 2.0, dΩ)
+
+
+# the touched pairs (test field, trial field) of each form and their mass/stiffness
+PAIRS2 = {(0, 0): ("stiff", 1.0), (1, 1): ("mass", 1.0), (1, 0): ("mass", -1.0)}
+VECS2 = {0: 1.0, 1: -1.0}
+PAIRS3 = {(0, 0): ("stiff", 1.0), (1, 1): ("mass", 1.0), (1, 0): ("mass", -1.0),
+          (1, 2): ("mass", -1.0), (2, 1): ("mass", -1.0)}
+VECS3 = {0: 1.0, 1: -1.0, 2: 2.0}
+
+
+def oracle_system(n, pairs, vecs, lift=True):
+    f = MF.q1_field((0, 1, 0, 1), (5, 5), g=lambda x: x[0] + x[1])
+    cd = f["cell_dofs"]
+    mats = {k: s * f[kind] for k, (kind, s) in pairs.items()}
+    cv = {i: s * f["src"] for i, s in vecs.items()}
+    return f, MF.assemble([cd] * n, [f["nfree"]] * n, [f["ndiri"]] * n, mats, cv,
+                          dvals=[f["dvals"]] * n if lift else None)
+
+
+def spaces(n, style=None):
+    model = G.CartesianDiscreteModel((0, 1, 0, 1), (5, 5))
+    reffe = G.ReferenceFE(G.lagrangian, G.Float64, 1)
+    V = G.TestFESpace(model, reffe, dirichlet_tags="boundary")
+    U = G.TrialFESpace(V, sol)
+    Ω = G.Triangulation(model)
+    dΩ = G.Measure(Ω, 2)
+    return V, U, dΩ
+
+
+def slice_csc(A, rows, cols):
+    """Rows and columns (0-based index arrays, each a union of field ranges) of a 1-based
+    SparseMatrixCSC, keeping stored entries (explicit zeros included)."""
+    rmap = -np.ones(A.m, dtype=np.int64)
+    rmap[rows] = np.arange(len(rows))
+    cp, rv, nz = [1], [], []
+    for c in cols:
+        lo, hi = A.colptr[c] - 1, A.colptr[c + 1] - 1
+        r = rmap[A.rowval[lo:hi] - 1]
+        keep = r >= 0
+        rv.append(r[keep] + 1)
+        nz.append(A.nzval[lo:hi][keep])
+        cp.append(cp[-1] + int(keep.sum()))
+    return (np.array(cp), np.concatenate(rv) if rv else np.zeros(0, np.int64),
+            np.concatenate(nz) if nz else np.zeros(0))
+
+
+@pytest.mark.parametrize("n", [2, 3])
+def test_multifield_consecutive_matches_oracle(oracle, n):
+    announce("multi-field consecutive, %d fields, 5x5 QUAD" % n)
+    V, U, dΩ = spaces(n)
+    Y = G.MultiFieldFESpace([V] * n)
+    X = G.MultiFieldFESpace([U] * n)
+    a = (lambda u, v: biform2(u, v, dΩ)) if n == 2 else (lambda u, v: biform3(u, v, dΩ))
+    l = (lambda v: liform2(v, dΩ)) if n == 2 else (lambda v: liform3(v, dΩ))
+    pairs, vecs = (PAIRS2, VECS2) if n == 2 else (PAIRS3, VECS3)
+    _, (cp, rv, nz, b) = oracle_system(n, pairs, vecs)
+    assem = G.SparseMatrixAssembler(X, Y)
+    A1 = G.assemble_matrix(a, assem, X, Y)
+    np.testing.assert_array_equal(A1.colptr, cp)
+    np.testing.assert_array_equal(A1.rowval, rv)
+    assert relmax(A1.nzval, nz) <= 1e-12
+    A2, b2 = G.assemble_matrix_and_vector(a, l, X, Y, assem)
+    np.testing.assert_array_equal(A2.rowval, rv)
+    assert relmax(A2.nzval, nz) <= 1e-12
+    assert relmax(b2, b) <= 1e-12
+    # the vector alone: sources only, no lifting
+    _, (_, _, _, bv) = oracle_system(n, pairs, vecs, lift=False)
+    b1 = G.assemble_vector(l, assem, Y)
+    assert relmax(b1, bv) <= 1e-12
+    op = G.AffineFEOperator(a, l, X, Y)
+    assert relmax(op.get_matrix().nzval, nz) <= 1e-12 and relmax(op.get_vector(), b) <= 1e-12
+
+
+@pytest.mark.parametrize("n", [2, 3])
+@pytest.mark.parametrize("trial_style", ["block", "block2"])
+@pytest.mark.parametrize("test_style", ["block", "block2"])
+def test_multifield_blocks_equal_monolithic(n, trial_style, test_style):
+    announce("multi-field blocks %s x %s, %d fields" % (test_style, trial_style, n))
+    V, U, dΩ = spaces(n)
+    style = {"block": G.BlockMultiFieldStyle(), "block2": G.BlockMultiFieldStyle(2, (1, n - 1))}
+    a = (lambda u, v: biform2(u, v, dΩ)) if n == 2 else (lambda u, v: biform3(u, v, dΩ))
+    l = (lambda v: liform2(v, dΩ)) if n == 2 else (lambda v: liform3(v, dΩ))
+    X, Y = G.MultiFieldFESpace([U] * n), G.MultiFieldFESpace([V] * n)
+    A1, b1 = G.assemble_matrix_and_vector(a, l, X, Y)
+    Xb = G.MultiFieldFESpace([U] * n, style=style[trial_style])
+    Yb = G.MultiFieldFESpace([V] * n, style=style[test_style])
+    Ab, bb = G.assemble_matrix_and_vector(a, l, Xb, Yb)
+    assert isinstance(Ab, G.BlockSparseMatrix) and isinstance(bb, G.BlockVector)
+    nf = V.nfree
+    rgroups, cgroups = Yb.groups(), Xb.groups()
+    assert len(Ab.blocks) == len(rgroups) and len(Ab.blocks[0]) == len(cgroups)
+    # each block: the monolithic matrix on its fields, stored entries included
+    for r, rg in enumerate(rgroups):
+        rows = np.concatenate([np.arange(i * nf, (i + 1) * nf) for i in rg])
+        for c, cg in enumerate(cgroups):
+            cols = np.concatenate([np.arange(j * nf, (j + 1) * nf) for j in cg])
+            cp, rv, nz = slice_csc(A1, rows, cols)
+            B = Ab.blocks[r][c]
+            assert B.shape == (len(rows), len(cols))
+            np.testing.assert_array_equal(B.colptr, cp)
+            np.testing.assert_array_equal(B.rowval, rv)
+            np.testing.assert_array_equal(B.nzval, nz)
+        np.testing.assert_array_equal(bb.blocks[r], b1[rows])
+    # mul!: the block product equals the monolithic one when both layouts agree
+    if rgroups == cgroups:
+        x = np.random.default_rng(n).normal(size=A1.n)
+        perm = np.concatenate([np.arange(j * nf, (j + 1) * nf) for g in cgroups for j in g])
+        y1 = A1.to_scipy() @ x
+        yb = Ab.to_scipy() @ x[perm]
+        np.testing.assert_allclose(yb, y1[perm], rtol=1e-14, atol=1e-14)
+
+
+def test_mass_term_3d_matches_oracle(oracle):
+    """∫ κ ∇v⋅∇u + σ u v with per-cell κ and σ on a 3-D Cartesian mesh (the per-cell
+    kernel), FP64 against the oracle; FP32 element math within 1e-6."""
+    part = (7, 6, 5)
+    announce("mass + stiffness %s" % (part,))
+    C = int(np.prod(part))
+    rng = np.random.default_rng(5)
+    kappa = np.exp(rng.normal(size=C))
+    sigma = np.exp(rng.normal(size=C))
+    f = MF.q1_field((0, 1, 0, 1, 0, 1), part, g=lambda x: 1.0 + x[0] * x[2])
+    mats = kappa[:, None, None] * f["stiff"] + sigma[:, None, None] * f["mass"]
+    cp, rv, nz, b = MF.assemble([f["cell_dofs"]], [f["nfree"]], [f["ndiri"]], {(0, 0): mats},
+                                {0: f["src"]}, dvals=[f["dvals"]])
+    model = G.CartesianDiscreteModel((0, 1, 0, 1, 0, 1), part)
+    V = G.TestFESpace(model, G.ReferenceFE(G.lagrangian, G.Float64, 1),
+                      dirichlet_tags="boundary")
+    U = G.TrialFESpace(V, lambda x: 1.0 + x[..., 0] * x[..., 2])
+    Ω = G.Triangulation(model)
+    dΩ = G.Measure(Ω, 2)
+    k, s = G.CellField(kappa, Ω), G.CellField(sigma, Ω)
+    a = lambda u, v: G.integrate(k * G.inner(G.gradient(v), G.gradient(u)) + s * (u * v), dΩ)
+    l = lambda v: G.integrate(1.0 * v, dΩ)
+    for prec, tol in (("fp64", 1e-12), ("fp32", 1e-6)):
+        op = G.AffineFEOperator(a, l, U, V, G.SparseMatrixAssembler(U, V, precision=prec))
+        A = op.get_matrix()
+        assert op.assem.ctx.pattern_info()["last_kernel"] == _lib.GH_KERNEL_CELL
+        np.testing.assert_array_equal(A.colptr, cp)
+        np.testing.assert_array_equal(A.rowval, rv)
+        assert relmax(A.nzval, nz) <= tol
+        assert relmax(op.get_vector(), b) <= tol
+    # the mass matrix alone is SPD: the device PCG solves it
+    m = lambda u, v: G.integrate(s * (v * u), dΩ)
+    opm = G.AffineFEOperator(m, l, U, V)
+    uh = G.solve(opm)
+    r = opm.get_vector() - opm.get_matrix().to_scipy() @ uh.free_values
+    assert np.linalg.norm(r) <= 1e-10 * np.linalg.norm(opm.get_vector())
