@@ -23,6 +23,8 @@
 #include <algorithm>
 #include <climits>
 #include <cmath>
+#include <type_traits>
+#include <vector>
 
 #include "gh_internal.h"
 
@@ -50,6 +52,10 @@ struct LatticeArgs {
   double *b;
   double Kc[8];  // reference stiffness (κ=1) by class a⊕b of the node pair
   double b0;     // reference load ∫φ_a (f=1), equal for every a on a box cell
+  // mapped lattice: per cell, the upper triangle of the element matrix (row-major)
+  // then the element load, entry-major ([k][cell]), FP32 or FP64
+  const void *elem;
+  int ncells;
 };
 
 template <int D>
@@ -109,10 +115,11 @@ struct TileIn {
   int xi[3];
   uint32_t fm;
   double kc[1 << D];
+  int cid[1 << D];  // mapped lattice: the incident cells (-1 outside)
   double fsum;
 };
 
-template <int D>
+template <int D, bool MAPPED>
 __device__ __forceinline__ void load_tile(const LatticeArgs &P, int tile, int lane,
                                           TileIn<D> &T) {
   constexpr int NV = 1 << D;
@@ -146,8 +153,12 @@ __device__ __forceinline__ void load_tile(const LatticeArgs &P, int tile, int la
       cid += ck * cstride;
       cstride *= P.nc[k];
     }
-    T.kc[e] = in ? (P.kappa ? P.kappa[cid] : 1.0) : 0.0;
-    T.fsum += in ? (P.fsrc ? P.fsrc[cid] : 1.0) : 0.0;
+    if (MAPPED) {
+      T.cid[e] = in ? cid : -1;
+    } else {
+      T.kc[e] = in ? (P.kappa ? P.kappa[cid] : 1.0) : 0.0;
+      T.fsum += in ? (P.fsrc ? P.fsrc[cid] : 1.0) : 0.0;
+    }
   }
 }
 
@@ -159,9 +170,14 @@ __device__ __forceinline__ void load_tile(const LatticeArgs &P, int tile, int la
 // (a few dozen flops per column; the kernel is bound by the nzval stores).
 // Column offsets come from the free-neighbour masks (one load per node instead
 // of 3^D); the tile is staged in LDS and stored in 16-byte chunks.
-template <int D>
+//
+// EK = 0: Cartesian cells (K̂ and κ); EK = 4 / 8: mapped cells, whose element matrices
+// and loads the element kernel below stored in FP32 / FP64: the entry of column n at
+// offset δ is Σ over the incident cells e, ascending, of K_e[a_e(n)][a_e(n+δ)].
+template <int D, int EK>
 __global__ __launch_bounds__(64 * kLatticeWaves) void lattice_q1_kernel(const LatticeArgs P,
                                                                         int ntiles) {
+  using ET = typename std::conditional<EK == 4, float, double>::type;
   constexpr int NB = Pow3<D>::v;  // stencil entries per column
   constexpr int NV = 1 << D;      // incident cells per node
   constexpr int R = kLatticeStageRounds;
@@ -176,7 +192,7 @@ __global__ __launch_bounds__(64 * kLatticeWaves) void lattice_q1_kernel(const La
   const int P0 = P.tile_p0[tile];  // uniform: every load below is independent of it
   const int dbase = P.tile_d0[tile];
   TileIn<D> cur;
-  load_tile<D>(P, tile, lane, cur);
+  load_tile<D, EK != 0>(P, tile, lane, cur);
   if (P0 < 0) return;  // wave-uniform: no free node in the tile
   const uint32_t fm = cur.fm;
   // Free dofs are numbered in node order (checked in the symbolic phase): the
@@ -186,6 +202,37 @@ __global__ __launch_bounds__(64 * kLatticeWaves) void lattice_q1_kernel(const La
 
   // acc[o] = K̂[class(o)] Σ_{cells holding node and neighbour o} κ_c
   double acc[NB];
+  double bsum = 0.0;
+  if (EK != 0) {
+    constexpr int NT = NV * (NV + 1) / 2;
+    const ET *E = static_cast<const ET *>(P.elem);
+    const int64_t nce = P.ncells;
+#pragma unroll
+    for (int o = 0; o < NB; ++o) acc[o] = 0.0;
+#pragma unroll
+    for (int e = 0; e < NV; ++e) {
+      const int cid = cur.cid[e];
+      if (cid < 0) continue;
+      const int a = (~e) & (NV - 1);  // the node's local index in cell e
+      bsum += (double)E[(int64_t)(NT + a) * nce + cid];
+#pragma unroll
+      for (int o = 0; o < NB; ++o) {
+        int oo = o, bl = 0;
+        bool ok = true;
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+          const int bk = ((a >> k) & 1) + oo % 3 - 1;
+          oo /= 3;
+          ok = ok && bk >= 0 && bk <= 1;
+          bl |= (bk & 1) << k;
+        }
+        if (!ok) continue;
+        const int lo = a < bl ? a : bl, hi = a < bl ? bl : a;
+        const int t = lo * NV - lo * (lo - 1) / 2 + (hi - lo);
+        acc[o] += (double)E[(int64_t)t * nce + cid];
+      }
+    }
+  } else
 #pragma unroll
   for (int o = 0; o < NB; ++o) {
     double sum = 0.0;
@@ -214,7 +261,7 @@ __global__ __launch_bounds__(64 * kLatticeWaves) void lattice_q1_kernel(const La
     acc[o] = P.Kc[cls] * sum;
   }
 
-  double bacc = P.b0 * cur.fsum;
+  double bacc = EK != 0 ? bsum : P.b0 * cur.fsum;
   if (P.dvals && d0 > 0) {
     // Dirichlet lifting b -= Σ_δ A(n, n+δ) u_d(n+δ) over Dirichlet neighbours.
 #pragma unroll
@@ -332,9 +379,177 @@ void launch_lattice(Ctx &c, const double *Kc, double b0) {
   P.b0 = b0;
   const int ntiles = (int)((c.nnodes + kTileNodes - 1) / kTileNodes);
   const unsigned grid = (unsigned)((ntiles + kLatticeWaves - 1) / kLatticeWaves);
-  hipLaunchKernelGGL((lattice_q1_kernel<D>), dim3(grid), dim3(kTileNodes * kLatticeWaves), 0,
+  hipLaunchKernelGGL((lattice_q1_kernel<D, 0>), dim3(grid), dim3(kTileNodes * kLatticeWaves), 0,
                      c.stream, P, ntiles);
   GH_CHECK_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// Mapped lattice: element kernel (per cell, isoparametric Jacobian per point) then
+// the owner-computes column kernel over the stored element matrices
+// ---------------------------------------------------------------------------
+
+struct QElemArgs {
+  int nc[3], nn[3];
+  int ncells;
+  int nq;
+  const double *xyz;
+  const double *tab;  // per q: w, shp[NV], grad[NV*D]
+  const double *kappa, *fsrc;
+};
+
+// The element matrix and load of every cell, with the arithmetic of cell_q1_kernel
+// (FP64 Jacobian, FP32 or FP64 element math); stored as the upper triangle of the
+// symmetric matrix (row-major) and the load, entry-major.
+template <int D, bool F32>
+__global__ __launch_bounds__(256) void q1_lattice_elem_kernel(const QElemArgs A,
+                                                              typename std::conditional<F32, float, double>::type *E) {
+  using T = typename std::conditional<F32, float, double>::type;
+  constexpr int NL = 1 << D;
+  constexpr int NT = NL * (NL + 1) / 2;
+  const int cell = blockIdx.x * blockDim.x + threadIdx.x;
+  if (cell >= A.ncells) return;
+  int ci[3] = {0, 0, 0};
+  {
+    int r = cell;
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      ci[k] = r % A.nc[k];
+      r /= A.nc[k];
+    }
+  }
+  double X[NL][D];
+#pragma unroll
+  for (int a = 0; a < NL; ++a) {
+    int node = 0, stride = 1;
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      node += (ci[k] + ((a >> k) & 1)) * stride;
+      stride *= A.nn[k];
+    }
+#pragma unroll
+    for (int k = 0; k < D; ++k) X[a][k] = A.xyz[(int64_t)node * D + k];
+  }
+  const double kap = A.kappa ? A.kappa[cell] : 1.0;
+  const double fc = A.fsrc ? A.fsrc[cell] : 1.0;
+  T K[NT], F[NL];
+#pragma unroll
+  for (int t = 0; t < NT; ++t) K[t] = 0;
+#pragma unroll
+  for (int a = 0; a < NL; ++a) F[a] = 0;
+  const int stride = 1 + NL + NL * D;
+  for (int q = 0; q < A.nq; ++q) {
+    const double *tq = A.tab + q * stride;
+    double J[D * D], Ji[D * D];
+#pragma unroll
+    for (int i = 0; i < D; ++i)
+#pragma unroll
+      for (int j = 0; j < D; ++j) {
+        double s = 0.0;
+#pragma unroll
+        for (int a = 0; a < NL; ++a) s += tq[1 + NL + a * D + i] * X[a][j];
+        J[i * D + j] = s;
+      }
+    const double det = det_inv<D>(J, Ji);
+    const T dV = (T)(fabs(det) * tq[0]);
+    T g[NL][D];
+#pragma unroll
+    for (int a = 0; a < NL; ++a)
+#pragma unroll
+      for (int r = 0; r < D; ++r) {
+        double s = 0.0;
+#pragma unroll
+        for (int k = 0; k < D; ++k) s += Ji[r * D + k] * tq[1 + NL + a * D + k];
+        g[a][r] = (T)s;
+      }
+    const T kd = (T)kap * dV;
+    int t = 0;
+#pragma unroll
+    for (int i = 0; i < NL; ++i) {
+#pragma unroll
+      for (int j = i; j < NL; ++j, ++t) {
+        T dot = 0;
+#pragma unroll
+        for (int r = 0; r < D; ++r) dot += g[i][r] * g[j][r];
+        K[t] += dot * kd;
+      }
+      F[i] += (T)tq[1 + i] * ((T)fc * dV);
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < NT; ++t) E[(int64_t)t * A.ncells + cell] = K[t];
+#pragma unroll
+  for (int a = 0; a < NL; ++a) E[(int64_t)(NT + a) * A.ncells + cell] = F[a];
+}
+
+template <int D>
+void launch_lattice_mapped(Ctx &c, int precision) {
+  c.last_kernel = GH_KERNEL_Q1_LATTICE;
+  constexpr int NL = 1 << D;
+  constexpr int NT = NL * (NL + 1) / 2;
+  GH_REQUIRE(c.nv == NL && c.xyz.n == (size_t)c.nnodes * D, GH_ERR_STATE,
+             "the mapped lattice needs the node coordinates");
+  const bool f32 = precision == GH_PREC_FP32_ELEM;
+  const size_t esz = f32 ? sizeof(float) : sizeof(double);
+  c.qelem.resize((size_t)(NT + NL) * c.ncells * esz);
+  const int stride = 1 + NL + NL * D;
+  std::vector<double> tab(size_t(c.nq) * stride);
+  for (int q = 0; q < c.nq; ++q) {
+    tab[q * stride] = c.qw[q];
+    for (int a = 0; a < NL; ++a) tab[q * stride + 1 + a] = c.shp[q * NL + a];
+    for (int a = 0; a < NL * D; ++a) tab[q * stride + 1 + NL + a] = c.grad[q * NL * D + a];
+  }
+  c.nl_tab.resize(tab.size());
+  GH_CHECK_HIP(hipMemcpyAsync(c.nl_tab.get(), tab.data(), sizeof(double) * tab.size(),
+                              hipMemcpyHostToDevice, c.stream));
+  QElemArgs A{};
+  for (int k = 0; k < 3; ++k) {
+    A.nc[k] = (int)c.part[k];
+    A.nn[k] = (int)(c.part[k] + 1);
+  }
+  A.ncells = (int)c.ncells;
+  A.nq = c.nq;
+  A.xyz = c.xyz.get();
+  A.tab = c.nl_tab.get();
+  A.kappa = c.have_kappa ? c.kappa.get() : nullptr;
+  A.fsrc = c.have_fsrc ? c.fsrc.get() : nullptr;
+  const unsigned eg = (unsigned)((c.ncells + 255) / 256);
+  if (f32)
+    hipLaunchKernelGGL((q1_lattice_elem_kernel<D, true>), dim3(eg), dim3(256), 0, c.stream, A,
+                       (float *)c.qelem.get());
+  else
+    hipLaunchKernelGGL((q1_lattice_elem_kernel<D, false>), dim3(eg), dim3(256), 0, c.stream, A,
+                       (double *)c.qelem.get());
+  GH_CHECK_HIP(hipGetLastError());
+  LatticeArgs P{};
+  for (int k = 0; k < 3; ++k) {
+    P.nn[k] = (int)(c.part[k] + 1);
+    P.nc[k] = (int)c.part[k];
+  }
+  P.nnodes = (int)c.nnodes;
+  P.inv_nn0 = 1.0 / P.nn[0];
+  P.inv_nn1 = 1.0 / P.nn[1];
+  P.node_dof = c.node_dof.get();
+  P.fmask = c.node_fmask.get();
+  P.tile_p0 = c.tile_p0.get();
+  P.tile_d0 = c.tile_d0.get();
+  P.colptr = c.colptr.get();
+  P.dvals = c.have_dvals ? c.dvals.get() : nullptr;
+  P.nzval = c.nzval.get();
+  P.b = c.b.get();
+  P.elem = c.qelem.get();
+  P.ncells = (int)c.ncells;
+  const int ntiles = (int)((c.nnodes + kTileNodes - 1) / kTileNodes);
+  const unsigned grid = (unsigned)((ntiles + kLatticeWaves - 1) / kLatticeWaves);
+  if (f32)
+    hipLaunchKernelGGL((lattice_q1_kernel<D, 4>), dim3(grid), dim3(kTileNodes * kLatticeWaves),
+                       0, c.stream, P, ntiles);
+  else
+    hipLaunchKernelGGL((lattice_q1_kernel<D, 8>), dim3(grid), dim3(kTileNodes * kLatticeWaves),
+                       0, c.stream, P, ntiles);
+  GH_CHECK_HIP(hipGetLastError());
+  // the packed tabulation's host copy lives until here
+  GH_CHECK_HIP(hipStreamSynchronize(c.stream));
 }
 
 // ---------------------------------------------------------------------------
@@ -555,6 +770,16 @@ void assemble_diffusion(Ctx &c, int precision, int path) {
   // the lattice kernel takes f constant per cell and no mass term; f at the quadrature
   // points or a mass term runs the per-cell kernel
   bool lattice = c.structured && path != GH_PATH_GENERAL && !c.have_fq && !c.have_sigma;
+  if (lattice && !c.cartesian) {
+    // mapped cells of a lattice: element matrices per cell, owner-computes columns
+    GH_REQUIRE((int)c.shp.size() == c.nq * c.nloc && (int)c.grad.size() == c.nq * c.nloc * c.dim,
+               GH_ERR_STATE, "tabulation does not match the element (dofs per cell)");
+    if (c.dim == 3)
+      launch_lattice_mapped<3>(c, precision);
+    else
+      launch_lattice_mapped<2>(c, precision);
+    return;
+  }
   if (lattice) {
     GH_REQUIRE((int)c.shp.size() == c.nq * c.nloc, GH_ERR_STATE,
                "tabulation does not match the element (dofs per cell)");

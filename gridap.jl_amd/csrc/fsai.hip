@@ -677,7 +677,7 @@ constexpr int kResidualBlocks = 512;
 }  // namespace
 
 void fsai_prepare(Ctx &c) {
-  GH_REQUIRE(c.structured && c.dim == 3, GH_ERR_UNSUPPORTED,
+  GH_REQUIRE(c.structured && c.cartesian && c.dim == 3, GH_ERR_UNSUPPORTED,
              "the learned FSAI needs a 3-D Cartesian mesh with a scalar Q1 CLagrangian space");
   GH_REQUIRE(c.have_values, GH_ERR_STATE, "assemble the matrix before the FSAI set-up");
   const int64_t n = c.nfree, C = c.ncells;

@@ -47,6 +47,42 @@ void clear_boundary(gh::Ctx &c) {
   c.bface_order.release();
 }
 
+// An uploaded quadrilateral/hexahedral mesh whose cells are those of a Cartesian grid:
+// lexicographic node ids, lexicographic cells, Gridap's local node order (x fastest),
+// as a CartesianDiscreteModel with a map has (src/Geometry/CartesianGrids.jl:107-124).
+// The lattice pattern and the owner-computes kernels then apply, with the node
+// coordinates for the geometry.  Sets part and returns true when it is one.
+bool lattice_topology(int dim, int64_t nnodes, int64_t ncells, int nv,
+                      const std::vector<int32_t> &cn, int64_t *part) {
+  if (nv != (1 << dim) || ncells < 1 || cn[0] != 0 || cn[1] != 1) return false;
+  const int64_t n0 = (int64_t)cn[2] - cn[0];
+  if (n0 < 2) return false;
+  int64_t p[3] = {n0 - 1, 1, 1};
+  int64_t n01 = 0;
+  if (dim == 3) {
+    n01 = (int64_t)cn[4] - cn[0];
+    if (n01 < 2 * n0 || n01 % n0) return false;
+    p[1] = n01 / n0 - 1;
+    if (ncells % (p[0] * p[1])) return false;
+    p[2] = ncells / (p[0] * p[1]);
+    if (n01 * (p[2] + 1) != nnodes) return false;
+  } else {
+    if (ncells % p[0]) return false;
+    p[1] = ncells / p[0];
+    if (n0 * (p[1] + 1) != nnodes) return false;
+  }
+  for (int64_t k = 0; k < ncells; ++k) {
+    const int64_t cx = k % p[0], cy = (k / p[0]) % p[1], cz = dim == 3 ? k / (p[0] * p[1]) : 0;
+    const int64_t base = cx + n0 * (cy + (p[1] + 1) * cz);
+    for (int a = 0; a < nv; ++a) {
+      const int64_t node = base + (a & 1) + n0 * ((a >> 1) & 1) + n01 * ((a >> 2) & 1);
+      if (cn[(size_t)k * nv + a] != node) return false;
+    }
+  }
+  for (int d = 0; d < 3; ++d) part[d] = d < dim ? p[d] : 1;
+  return true;
+}
+
 // A new mesh voids the FE space uploaded for the previous one: its cell_dof_ids were
 // numbered on the old cells (gh_space_upload must follow gh_mesh_*).
 void invalidate_space(gh::Ctx &c) {
@@ -165,6 +201,7 @@ int gh_mesh_cartesian(gh_ctx *ctx, int dim, const double *origin, const double *
     GH_REQUIRE(nn < (int64_t(1) << 31), GH_ERR_ARG, "too many nodes for 32-bit ids");
     c.dim = dim;
     c.cartesian = true;
+    c.lattice_topo = true;
     for (int d = 0; d < 3; ++d) {
       c.origin[d] = d < dim ? origin[d] : 0.0;
       c.sizes[d] = d < dim ? sizes[d] : 1.0;
@@ -201,6 +238,7 @@ int gh_mesh_upload(gh_ctx *ctx, int dim, int64_t nnodes, const double *xyz, int6
     }
     c.dim = dim;
     c.cartesian = false;
+    c.lattice_topo = lattice_topology(dim, nnodes, ncells, nv, cn, c.part);
     c.nnodes = nnodes;
     c.ncells = ncells;
     c.nv = nv;

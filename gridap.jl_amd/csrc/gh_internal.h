@@ -128,6 +128,7 @@ struct Ctx {
   // ---- mesh ----
   int dim = 0;
   bool cartesian = false;
+  bool lattice_topo = false;        // cells and nodes of a Cartesian grid (part), any coordinates
   double origin[3] = {0, 0, 0}, sizes[3] = {1, 1, 1};
   int64_t part[3] = {1, 1, 1};
   int64_t nnodes = 0, ncells = 0;
@@ -203,6 +204,7 @@ struct Ctx {
   DBuf<float> q2_ref32;             // K̂ in FP32
   bool q2_ref_ready = false;
   double q2_ref_key[4] = {0, 0, 0, 0};  // ν and cell sizes of q2_ref
+  DBuf<unsigned char> qelem;        // mapped lattice: per-cell stiffness triangle and load (SoA)
 
   // ---- colouring for the general path ----
   int ncolors = 0;

@@ -730,7 +730,7 @@ bool symbolic_lattice32(Ctx &c) {
 
 bool symbolic_lattice(Ctx &c) {
   c.structured = false;
-  if (!(c.cartesian && c.reffe_kind == GH_REFFE_QK && c.order == 1 && c.ncomp == 1 &&
+  if (!(c.lattice_topo && c.reffe_kind == GH_REFFE_QK && c.order == 1 && c.ncomp == 1 &&
         c.nloc == (1 << c.dim) && c.nfree > 0 && c.nnodes < (int64_t(1) << 31)))
     return false;
   if (c.ncells * c.nloc < (int64_t(1) << 31) && c.nnodes * 27 < (int64_t(1) << 31))
@@ -810,7 +810,7 @@ void check_structured(Ctx &c) {
   c.node_fmask.release();
   c.tile_p0.release();
   c.tile_d0.release();
-  if (!(c.cartesian && c.reffe_kind == GH_REFFE_QK && c.order == 1 && c.ncomp == 1 &&
+  if (!(c.lattice_topo && c.reffe_kind == GH_REFFE_QK && c.order == 1 && c.ncomp == 1 &&
         c.nloc == (1 << c.dim) && c.nfree > 0))
     return;
   hipStream_t s = c.stream;

@@ -72,6 +72,11 @@ def oracle_problem(oracle, domain, part, kappa=None, fsrc=None, tags=("boundary"
     shp, grad = oracle.qk_tabulate(D, 1, qx)
     mats = oracle.cell_matrices_diffusion(D, m["cell_nodes"], xyz, qw, grad, kappa)
     vecs = oracle.cell_vectors_source(D, m["cell_nodes"], xyz, qw, shp, grad, fsrc)
+    if g is not None and ndiri:
+        dv = np.zeros(ndiri)
+        dnodes = np.nonzero(n2d[:, 0] < 0)[0]
+        dv[-n2d[dnodes, 0] - 1] = [g(xyz[n]) for n in dnodes]
+        vecs = oracle.attach_dirichlet(cd, mats, dv, vecs)
     colptr, rowval, nzval = oracle.assemble_csc(cd, mats, nfree, nfree)
     b = oracle.assemble_vector(cd, vecs, nfree)
     return dict(colptr=colptr, rowval=rowval, nzval=nzval, b=b, nfree=nfree)
@@ -166,10 +171,13 @@ def test_partial_dirichlet_tags(oracle):
     check_against(ref, op, "fp64")
 
 
-def test_mapped_grid_general_path(oracle):
+@pytest.mark.parametrize("path", ["auto", "general"])
+def test_mapped_grid_general_path(oracle, path):
     """Non-affine geometry (jittered interior nodes): isoparametric Jacobian per
-    quadrature point on the general path."""
-    announce("3d 8x7x6 jittered mapped grid")
+    quadrature point.  auto: the mesh keeps the lattice topology, so the element
+    matrices are stored per cell and the owner-computes lattice kernel sums them;
+    general: the per-cell kernel with the coloured scatter."""
+    announce("3d 8x7x6 jittered mapped grid path=%s" % path)
     part = (8, 7, 6)
     rng = np.random.default_rng(15)
     h = np.array([1 / 8, 1 / 7, 1 / 6])
@@ -181,12 +189,40 @@ def test_mapped_grid_general_path(oracle):
         return out
 
     kappa = np.exp(rng.normal(size=int(np.prod(part))))
-    op, V, U, model = build((0, 1) * 3, part, kappa=kappa, map=jitter)
+    op, V, U, model = build((0, 1) * 3, part, kappa=kappa, map=jitter, path=path)
+    kern = _lib.GH_KERNEL_Q1_LATTICE if path == "auto" else _lib.GH_KERNEL_CELL
+    assert op.assem.ctx.pattern_info()["last_kernel"] == kern
     ref = oracle_problem(oracle, (0, 1) * 3, part, kappa=kappa, xyz=model.node_coordinates())
     check_against(ref, op, "fp64")
     op32, *_ = build((0, 1) * 3, part, kappa=kappa, map=lambda x: model.node_coordinates(),
-                     precision="fp32")
+                     precision="fp32", path=path)
     check_against(ref, op32, "fp32")
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_mapped_lattice_matches_cell_kernel(oracle, precision):
+    """A smoothly mapped 2-D and 3-D lattice with κ lognormal, Dirichlet data g on part
+    of the boundary (lifting): the mapped lattice kernel against the oracle and against
+    the per-cell kernel (same element arithmetic, another summation order)."""
+    for dom, part in (((0, 1, 0, 1), (17, 13)), ((0, 1) * 3, (11, 9, 10))):
+        announce("mapped lattice %s %s" % (part, precision))
+        D = len(part)
+        warp = lambda x: x + 0.04 * np.prod(np.sin(np.pi * x), axis=-1)[..., None]
+        kappa = np.exp(np.random.default_rng(D).normal(size=int(np.prod(part))))
+        g = lambda x: 1.0 + x[..., 0] - 0.5 * x[..., 1]
+        tags = ("tag_5", "tag_7") if D == 2 else ("tag_21", "tag_23")
+        op, V, U, model = build(dom, part, kappa=kappa, map=warp, g=g, tags=tags,
+                                precision=precision)
+        assert op.assem.ctx.pattern_info()["last_kernel"] == _lib.GH_KERNEL_Q1_LATTICE
+        ref = oracle_problem(oracle, dom, part, kappa=kappa, xyz=model.node_coordinates(),
+                             tags=tags, g=lambda x: 1.0 + x[0] - 0.5 * x[1])
+        check_against(ref, op, precision)
+        op2, *_ = build(dom, part, kappa=kappa, map=warp, g=g, tags=tags, precision=precision,
+                        path="general")
+        A1, A2 = op.get_matrix(), op2.get_matrix()
+        tol = 1e-13 if precision == "fp64" else 1e-6
+        assert relmax(A1.nzval, A2.nzval) <= tol
+        assert relmax(op.get_vector(), op2.get_vector()) <= tol
 
 
 @pytest.mark.parametrize("path", ["auto", "general"])
