@@ -532,10 +532,13 @@ __global__ void lattice_verify32(Lattice L, int32_t total, int nloc,
     atomicOr(bad, 1);
 }
 
-// lattice_masks with 32-bit indices
-__global__ void lattice_masks32(Lattice L, int32_t nnodes, const int32_t *__restrict__ node_dof,
-                                const int32_t *__restrict__ scan, uint32_t *__restrict__ fmask,
-                                int32_t *__restrict__ count, int32_t *bad) {
+// lattice_masks with 32-bit indices and the stencil unrolled (D known at compile time)
+template <int D>
+__global__ __launch_bounds__(256) void lattice_masks32(Lattice L, int32_t nnodes,
+                                                       const int32_t *__restrict__ node_dof,
+                                                       const int32_t *__restrict__ scan,
+                                                       uint32_t *__restrict__ fmask,
+                                                       int32_t *__restrict__ count, int32_t *bad) {
   const int32_t n = blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= nnodes) return;
   const int32_t d = node_dof[n];
@@ -543,23 +546,26 @@ __global__ void lattice_masks32(Lattice L, int32_t nnodes, const int32_t *__rest
   if (d == 0) atomicOr(bad, 2);
   if (d > 0 && d != scan[n] + 1) atomicOr(bad, 4);
   if (d > 0 && d == scan[n] + 1) {
-    int32_t xi[3] = {0, 0, 0}, r = n;
-    for (int k = 0; k < L.D; ++k) {
-      xi[k] = r % (int32_t)L.np[k];
-      r /= (int32_t)L.np[k];
+    int32_t xi[3] = {0, 0, 0}, np[3] = {1, 1, 1}, r = n;
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      np[k] = (int32_t)L.np[k];
+      xi[k] = r % np[k];
+      r /= np[k];
     }
-    int nb = 1;
-    for (int k = 0; k < L.D; ++k) nb *= 3;
-    for (int o = 0; o < nb; ++o) {
+    constexpr int NB = D == 3 ? 27 : 9;
+#pragma unroll
+    for (int o = 0; o < NB; ++o) {
       int oo = o;
       int32_t node = 0, stride = 1;
       bool in = true;
-      for (int k = 0; k < L.D; ++k) {
+#pragma unroll
+      for (int k = 0; k < D; ++k) {
         const int32_t v = xi[k] + (oo % 3) - 1;
         oo /= 3;
-        in = in && v >= 0 && v < (int32_t)L.np[k];
+        in = in && v >= 0 && v < np[k];
         node += v * stride;
-        stride *= (int32_t)L.np[k];
+        stride *= np[k];
       }
       if (in && node_dof[node] > 0) m |= 1u << o;
     }
@@ -568,31 +574,46 @@ __global__ void lattice_masks32(Lattice L, int32_t nnodes, const int32_t *__rest
   count[n] = __popc(m);
 }
 
-// One thread per (node, stencil offset): the row of each free neighbour at its place in
-// the node's column (consecutive threads write consecutive rows), and colptr.
-__global__ void lattice_rows32(Lattice L, int32_t nnodes, int nb,
-                               const int32_t *__restrict__ node_dof,
-                               const uint32_t *__restrict__ fmask,
-                               const int32_t *__restrict__ start, int32_t *__restrict__ colptr,
-                               int32_t *__restrict__ rowval) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int32_t n = (int32_t)(t / nb);
-  const int o = (int)(t - (int64_t)n * nb);
-  if (n >= nnodes) return;
-  const int32_t d = node_dof[n];
-  if (d <= 0) return;
-  const int32_t p = start[n];
-  if (o == 0) colptr[d - 1] = p;
-  const uint32_t m = fmask[n];
-  if (!((m >> o) & 1u)) return;
-  int32_t off = 0, stride = 1;
-  int oo = o;
-  for (int k = 0; k < L.D; ++k) {
-    off += ((oo % 3) - 1) * stride;
-    oo /= 3;
-    stride *= (int32_t)L.np[k];
+// colptr and rowval: one thread per node writes its column's rows into the
+// workgroup's LDS run (the columns of 256 consecutive nodes are one contiguous run of
+// rowval), which the workgroup then stores coalesced.
+template <int D>
+__global__ __launch_bounds__(256) void lattice_rows32(Lattice L, int32_t nnodes, int64_t nnz,
+                                                      const int32_t *__restrict__ node_dof,
+                                                      const uint32_t *__restrict__ fmask,
+                                                      const int32_t *__restrict__ start,
+                                                      int32_t *__restrict__ colptr,
+                                                      int32_t *__restrict__ rowval) {
+  constexpr int NB = D == 3 ? 27 : 9;
+  __shared__ int32_t buf[256 * NB];
+  const int32_t n0 = blockIdx.x * 256, n = n0 + (int32_t)threadIdx.x;
+  const int32_t base = start[n0];
+  const int64_t endp = n0 + 256 < nnodes ? (int64_t)start[n0 + 256] : nnz;
+  if (n < nnodes) {
+    const int32_t d = node_dof[n];
+    if (d > 0) {
+      const int32_t p = start[n];
+      colptr[d - 1] = p;
+      const uint32_t m = fmask[n];
+      int k = p - base;
+      int32_t off[3] = {1, (int32_t)L.np[0], (int32_t)(L.np[0] * L.np[1])};
+#pragma unroll
+      for (int o = 0; o < NB; ++o) {
+        if (!((m >> o) & 1u)) continue;
+        int oo = o;
+        int32_t nb = n;
+#pragma unroll
+        for (int j = 0; j < D; ++j) {
+          nb += ((oo % 3) - 1) * off[j];
+          oo /= 3;
+        }
+        buf[k++] = node_dof[nb] - 1;
+      }
+    }
   }
-  rowval[p + __popc(m & ((1u << o) - 1u))] = node_dof[n + off] - 1;
+  __syncthreads();
+  const int len = (int)(endp - base);
+  for (int t = threadIdx.x; t < len; t += 256) rowval[(int64_t)base + t] = buf[t];
 }
 
 // Clears the flags and the box record: box[0], box[1] = node of dof 1 and of the last
@@ -672,9 +693,14 @@ bool symbolic_lattice32(Ctx &c) {
                      c.node_dof.get(), c.ws_f.get(), c.ws_flag.get());
   GH_CHECK_HIP(hipGetLastError());
   exclusive_sum_ws(c.ws_f.get(), c.ws_scan.get(), c.nnodes, s, c.ws_cub);
-  hipLaunchKernelGGL(lattice_masks32, dim3(grid_for(nn)), dim3(kBlock), 0, s, L, nn,
-                     c.node_dof.get(), c.ws_scan.get(), c.node_fmask.get(), c.ws_count.get(),
-                     c.ws_flag.get());
+  if (c.dim == 3)
+    hipLaunchKernelGGL(lattice_masks32<3>, dim3(grid_for(nn)), dim3(kBlock), 0, s, L, nn,
+                       c.node_dof.get(), c.ws_scan.get(), c.node_fmask.get(), c.ws_count.get(),
+                       c.ws_flag.get());
+  else
+    hipLaunchKernelGGL(lattice_masks32<2>, dim3(grid_for(nn)), dim3(kBlock), 0, s, L, nn,
+                       c.node_dof.get(), c.ws_scan.get(), c.node_fmask.get(), c.ws_count.get(),
+                       c.ws_flag.get());
   GH_CHECK_HIP(hipGetLastError());
   exclusive_sum_ws(c.ws_count.get(), c.ws_start.get(), c.nnodes, s, c.ws_cub);
   hipLaunchKernelGGL(lattice_summary, dim3(1), dim3(64), 0, s, c.nnodes, c.nfree,
@@ -687,11 +713,14 @@ bool symbolic_lattice32(Ctx &c) {
   const int64_t nnz = h[2];
   c.nnz = nnz;
   c.rowval.resize(std::max<int64_t>(nnz, 1));
-  int nb = 1;
-  for (int k = 0; k < c.dim; ++k) nb *= 3;
-  hipLaunchKernelGGL(lattice_rows32, dim3(grid_for((int64_t)nn * nb)), dim3(kBlock), 0, s, L, nn,
-                     nb, c.node_dof.get(), c.node_fmask.get(), c.ws_start.get(), c.colptr.get(),
-                     c.rowval.get());
+  if (c.dim == 3)
+    hipLaunchKernelGGL(lattice_rows32<3>, dim3(grid_for(nn)), dim3(256), 0, s, L, nn, nnz,
+                       c.node_dof.get(), c.node_fmask.get(), c.ws_start.get(), c.colptr.get(),
+                       c.rowval.get());
+  else
+    hipLaunchKernelGGL(lattice_rows32<2>, dim3(grid_for(nn)), dim3(256), 0, s, L, nn, nnz,
+                       c.node_dof.get(), c.node_fmask.get(), c.ws_start.get(), c.colptr.get(),
+                       c.rowval.get());
   const int64_t ntiles = (c.nnodes + kTileNodes - 1) / kTileNodes;
   c.tile_p0.resize(ntiles);
   c.tile_d0.resize(ntiles);
