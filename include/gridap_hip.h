@@ -226,11 +226,15 @@ int gh_pattern_info_get(gh_ctx *ctx, gh_pattern_info *info);
 int gh_numeric(gh_ctx *ctx, int form_kind, const double *cell_coef, const double *cell_source,
                int precision, double *nzval, double *b, const double *dirichlet_vals);
 
-/* y = A x with the assembled matrix (host vectors of length nfree).
+/* y = A x with the assembled matrix (host vectors of length nfree).  The kernels read
+ * the CSC columns as rows, i.e. they compute Aᵀx, which is A x for the symmetric
+ * matrices of these forms: every numeric path except the vector-Q2 element path
+ * (GH_PATH_GENERAL, whose FP32/FP64 matrix-core products round the two entries of a
+ * pair differently in the last bits) stores A symmetric bit for bit.
  * Replaces the SpMV of residual!, src/Algebra/LinearSolvers.jl:36-58. */
 int gh_spmv(gh_ctx *ctx, const double *x, double *y);
 
-/* Preconditioned CG in FP64 on the assembled (SPD) matrix; b NULL = the assembled
+/* Preconditioned CG in FP64 on the assembled (SPD, see gh_spmv) matrix; b NULL = the assembled
  * b.  x holds the initial guess on entry and the solution on exit.
  * Stands in for solve!(x, LUSolver(), op), src/Algebra/LinearSolvers.jl:202-274. */
 int gh_pcg(gh_ctx *ctx, int precond, const double *b, double *x, double rtol, int maxit,
