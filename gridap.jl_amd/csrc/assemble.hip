@@ -74,21 +74,7 @@ constexpr int kLatticeStageRounds = 2;
 // own LDS and synchronises only with itself.
 constexpr int kLatticeWaves = 4;
 
-// LDS writes of this wave visible to its other lanes (and its reads finished)
-__device__ __forceinline__ void wave_lds_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
-// Workgroups are dealt round-robin to the 8 XCDs; remap so XCD x sweeps the
-// contiguous x-th slice of the lattice and the neighbouring lines/planes its
-// nodes need are in that XCD's L2.  Bijective on [0, nblocks).
-__device__ __forceinline__ int xcd_remap(int b, int nblocks) {
-  constexpr int kXcds = 8;
-  const int q = nblocks / kXcds, r = nblocks % kXcds, x = b % kXcds;
-  return x * q + min(x, r) + b / kXcds;
-}
 
 // floor(n / d) for 0 <= n < 2^31 from a double reciprocal, corrected exactly.
 __device__ __forceinline__ int div_by(int n, int d, double inv) {
@@ -297,14 +283,14 @@ __global__ __launch_bounds__(64 * kLatticeWaves) void lattice_q1_kernel(const La
     if (end == beg) continue;  // wave-uniform
     const int Pr = P0 + beg;
     const int sh = Pr & 1;  // stage[sh + k] <-> nzval[Pr + k]; 16-byte aligned chunks
-    if (r > 0) wave_lds_sync();  // the previous round's stage has been read out
+    if (r > 0) wave_sync();  // the previous round's stage has been read out
     if (d0 > 0 && lane / LPR == r) {
       int w = sh + off - beg;
 #pragma unroll
       for (int o = 0; o < NB; ++o)
         if ((fm >> o) & 1u) stage[w++] = acc[o];
     }
-    wave_lds_sync();
+    wave_sync();
     const int lo = sh, hi = sh + (end - beg);
     double2 *dst2 = reinterpret_cast<double2 *>(P.nzval + (Pr - sh));
     const double2 *st2 = reinterpret_cast<const double2 *>(stage);

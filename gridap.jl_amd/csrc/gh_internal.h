@@ -14,6 +14,22 @@ namespace gh {
 // Lattice nodes per tile of the lattice kernel (one wavefront).
 constexpr int kTileNodes = 64;
 
+// Workgroups are dealt round-robin to the 8 XCDs; remap so XCD x takes the contiguous
+// x-th slice of the work and the data that neighbouring work items share stays in
+// that XCD's L2.  Bijective on [0, nblocks).
+__device__ __forceinline__ int xcd_remap(int b, int nblocks) {
+  constexpr int kXcds = 8;
+  const int q = nblocks / kXcds, r = nblocks % kXcds, x = b % kXcds;
+  return x * q + min(x, r) + b / kXcds;
+}
+
+// LDS written by some lanes of a wavefront is visible to the wavefront's other lanes.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Error carried from the point of failure to the ABI boundary, where it becomes a
 // status code plus gh_last_error() text.
 struct Error : std::runtime_error {

@@ -68,11 +68,6 @@ struct Q2Args {
   int64_t nlist;             // their count (with nodes)
 };
 
-__device__ __forceinline__ void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 template <class T, bool CHECK>
 __global__ __launch_bounds__(64 * kQ2Waves) void q2_lattice_kernel(const Q2Args P) {
