@@ -316,6 +316,25 @@ int gh_reffe_tabulation(gh_ctx *ctx, int nq, int nloc, const double *qw, const d
     c.have_values = false;
     c.sref_ready = false;
     c.q2_ref_ready = false;
+    c.have_gtab = false;  // the geometry tabulation belongs to the previous points
+    c.gtab.release();
+  });
+}
+
+int gh_geometry_tabulation(gh_ctx *ctx, int nq, int nv, const double *grad) {
+  return guarded(ctx, [&](gh::Ctx &c) {
+    GH_REQUIRE(c.dim > 0 && !c.cartesian, GH_ERR_STATE,
+               "the geometry tabulation belongs to a mesh uploaded with its nodes");
+    GH_REQUIRE(nq == c.nq, GH_ERR_ARG,
+               "the geometry tabulation must use the points of the FE tabulation");
+    GH_REQUIRE(nv == c.nv, GH_ERR_ARG, "one geometry shape function per cell node");
+    GH_REQUIRE(grad, GH_ERR_ARG, "NULL geometry tabulation");
+    c.gtab.resize((size_t)nq * nv * c.dim);
+    GH_CHECK_HIP(hipMemcpy(c.gtab.get(), grad, sizeof(double) * c.gtab.n,
+                           hipMemcpyHostToDevice));
+    c.have_gtab = true;
+    c.gtab_nq = nq;
+    c.have_values = false;
   });
 }
 
@@ -395,7 +414,8 @@ int gh_symbolic(gh_ctx *ctx, int64_t *nnz_out) {
                GH_ERR_STATE, "the FE space does not match the mesh: upload it again");
     invalidate_pattern(c);
     if (!gh::symbolic_lattice(c)) {
-      gh::symbolic_general(c, gh::simplex_eligible(c) || gh::hexq2_eligible(c));
+      gh::symbolic_general(c, gh::simplex_eligible(c) || gh::hexq2_eligible(c) ||
+                                  gh::hexgen_eligible(c));
       gh::check_structured(c);
     }
     gh::elemlist_symbolic(c);
@@ -491,11 +511,23 @@ int gh_assemble(gh_ctx *ctx, int form_kind, int precision, int path) {
                  "the lattice path assembles linear diffusion only");
       gh::assemble_plaplacian(c, precision);
     } else if (form_kind == GH_FORM_ELASTICITY) {
-      gh::assemble_elasticity(c, precision, path);
+      if (gh::hexgen_eligible(c)) {
+        GH_REQUIRE(path != GH_PATH_STRUCTURED, GH_ERR_UNSUPPORTED,
+                   "the lattice path assembles vector Q2 on Cartesian hexahedra");
+        gh::hexgen_assemble(c, precision, true);
+      } else {
+        gh::assemble_elasticity(c, precision, path);
+      }
     } else {
       GH_REQUIRE(!gh::hexq2_eligible(c), GH_ERR_UNSUPPORTED,
                  "vector Q2 spaces assemble the elasticity form");
-      gh::assemble_diffusion(c, precision, path);
+      if (gh::hexgen_eligible(c)) {
+        GH_REQUIRE(path != GH_PATH_STRUCTURED, GH_ERR_UNSUPPORTED,
+                   "the lattice path assembles scalar Q1 spaces");
+        gh::hexgen_assemble(c, precision, false);
+      } else {
+        gh::assemble_diffusion(c, precision, path);
+      }
       gh::assemble_boundary(c);
     }
     c.have_values = true;

@@ -221,6 +221,10 @@ struct Ctx {
   bool q2_ref_ready = false;
   double q2_ref_key[4] = {0, 0, 0, 0};  // ν and cell sizes of q2_ref
   DBuf<unsigned char> qelem;        // mapped lattice: per-cell stiffness triangle and load (SoA)
+  DBuf<double> evec;                // general n-cube element lists: per cell its load entries
+  DBuf<double> gtab;                // geometry map gradients ∇̂N_v at the tabulation's points
+  bool have_gtab = false;
+  int gtab_nq = 0;
 
   // ---- colouring for the general path ----
   int ncolors = 0;
@@ -303,6 +307,8 @@ void build_coloring(Ctx &c);
 void assemble_diffusion(Ctx &c, int precision, int path);
 // simplex.hip
 bool simplex_eligible(const Ctx &c);
+bool hexgen_eligible(const Ctx &c);      // scalar Q2 / vector Q1, Q2 n-cubes for hexgen
+void hexgen_assemble(Ctx &c, int precision, bool elasticity);
 void elemlist_symbolic(Ctx &c);         // dof -> cell adjacency, Dirichlet masks
 void elemlist_sum(Ctx &c, bool f32);     // nzval from the element entries
 void assemble_simplex(Ctx &c, int precision);

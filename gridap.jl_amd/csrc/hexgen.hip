@@ -1,0 +1,297 @@
+// Element-list assembly of the n-cube Lagrangian spaces the specialised kernels do not
+// take: scalar Q2 diffusion ∫ κ ∇v·∇u, f v (test/GridapTests/PoissonTests.jl, order 2),
+// vector Q1 and Q2 isotropic elasticity ∫ ε(v) : σ(ε(u)), b·v
+// (test/GridapTests/IsotropicDamageTests.jl:76-85), on Cartesian cells or on the
+// isoparametric Q1 map of each cell's vertices (a CartesianDiscreteModel with a map,
+// src/Geometry/CartesianGrids.jl:107-124).  The cell integrals follow IntegrationMap
+// (src/Fields/FieldsInterfaces.jl:737-760): per quadrature point the Jacobian
+// J = Σ_v X_v ⊗ ∇̂N_v of the geometry map, w|det J| and the physical gradients
+// J⁻ᵀ∇̂φ_a (push_∇, src/Fields/FieldsInterfaces.jl:94-98), then the integrand summed
+// over the points.
+//
+// Two passes over element matrices stored as full columns:
+//  * hexgen_element_kernel: one workgroup per cell; the points' weights and physical
+//    gradients go to LDS (FP64), each thread then forms the blocks of its pairs of
+//    scalar nodes a <= b (both (a,b) and (b,a) written from one value, so the matrix is
+//    symmetric bit for bit), and the cell's load minus the lifting of its Dirichlet
+//    dofs (attach_dirichlet, src/CellData/AttachDirichlet.jl:59-84).  Element
+//    arithmetic in FP64; the stored matrices in the precision mode's type;
+//  * the contribution-list sum (one thread per nonzero, ascending cells) of the
+//    element-list path for nzval, and hexgen_rhs_kernel (one thread per free dof,
+//    ascending cells) for b.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "gh_internal.h"
+
+namespace gh {
+namespace {
+
+constexpr int kHgThreads = 128;
+constexpr int kHgMaxNn = 27;   // scalar shape functions (Q2 hexahedron)
+constexpr int kHgMaxNq = 64;   // quadrature points
+
+struct HgArgs {
+  int D, nn, nc, nloc, nq, nv;
+  int elasticity;
+  double lam, mu;              // Lamé parameters for E = 1 (scaled by the cell's E)
+  double h[3];                 // Cartesian cell sizes (cell_nodes == nullptr)
+  const int32_t *cell_nodes;   // 0-based geometry nodes (nv per cell), or nullptr
+  const double *xyz;           // node coordinates, D per node
+  const double *tab;           // per point: w, φ_a (nn), ∇̂φ_a (nn·D)
+  const double *gtab;          // per point: ∇̂N_v (nv·D) of the geometry map
+  const double *coef;          // κ (diffusion) or E (elasticity) per cell, or nullptr: 1
+  const double *fsrc;          // f per cell (diffusion), or nullptr: 1
+  double body[3];              // body force (elasticity)
+  const int32_t *cell_dofs;
+  const double *dvals;         // Dirichlet values, or nullptr
+  void *E;                     // per cell: nloc columns of nloc rows
+  double *Ev;                  // per cell: nloc load entries
+};
+
+__device__ __forceinline__ double det_inv(int D, const double J[3][3], double Ji[3][3]) {
+  if (D == 2) {
+    const double det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+    const double r = 1.0 / det;
+    Ji[0][0] = J[1][1] * r;
+    Ji[0][1] = -J[0][1] * r;
+    Ji[1][0] = -J[1][0] * r;
+    Ji[1][1] = J[0][0] * r;
+    return det;
+  }
+  const double c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+  const double c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+  const double c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+  const double det = J[0][0] * c00 + J[0][1] * c01 + J[0][2] * c02;
+  const double r = 1.0 / det;
+  Ji[0][0] = c00 * r;
+  Ji[1][0] = c01 * r;
+  Ji[2][0] = c02 * r;
+  Ji[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * r;
+  Ji[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * r;
+  Ji[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * r;
+  Ji[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * r;
+  Ji[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * r;
+  Ji[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * r;
+  return det;
+}
+
+template <class T>
+__global__ __launch_bounds__(kHgThreads) void hexgen_element_kernel(HgArgs A) {
+  extern __shared__ double sh[];
+  const int D = A.D, nn = A.nn, nq = A.nq, nloc = A.nloc;
+  double *wq = sh;                                  // nq: w |det J|
+  double *gp = sh + nq;                             // nq x nn x D: J⁻ᵀ∇̂φ
+  int32_t *cd = reinterpret_cast<int32_t *>(gp + (size_t)nq * nn * D);  // nloc dofs
+  const int64_t cell = blockIdx.x;
+  const int tid = threadIdx.x;
+  const int stride = 1 + nn + nn * D;
+  for (int a = tid; a < nloc; a += kHgThreads) cd[a] = A.cell_dofs[cell * nloc + a];
+  for (int q = tid; q < nq; q += kHgThreads) {
+    const double *tq = A.tab + (size_t)q * stride;
+    double J[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}}, Ji[3][3];
+    double det;
+    if (A.cell_nodes) {
+      for (int d = 0; d < D; ++d)
+        for (int e = 0; e < D; ++e) J[d][e] = 0.0;
+      const double *gg = A.gtab + (size_t)q * A.nv * D;
+      for (int v = 0; v < A.nv; ++v) {
+        const double *X = A.xyz + (size_t)A.cell_nodes[cell * A.nv + v] * D;
+        for (int d = 0; d < D; ++d)
+          for (int e = 0; e < D; ++e) J[d][e] += X[d] * gg[v * D + e];
+      }
+      det = det_inv(D, J, Ji);
+    } else {
+      det = 1.0;
+      for (int d = 0; d < 3; ++d)
+        for (int e = 0; e < 3; ++e) Ji[d][e] = 0.0;
+      for (int d = 0; d < D; ++d) {
+        det *= A.h[d];
+        Ji[d][d] = 1.0 / A.h[d];
+      }
+    }
+    wq[q] = tq[0] * fabs(det);
+    const double *gr = tq + 1 + nn;
+    for (int a = 0; a < nn; ++a)
+      for (int d = 0; d < D; ++d) {
+        double g = 0.0;
+        for (int e = 0; e < D; ++e) g += Ji[e][d] * gr[a * D + e];
+        gp[((size_t)q * nn + a) * D + d] = g;
+      }
+  }
+  __syncthreads();
+  T *E = static_cast<T *>(A.E) + (size_t)cell * nloc * nloc;
+  const double cf = A.coef ? A.coef[cell] : 1.0;
+  const int npairs = nn * (nn + 1) / 2;
+  for (int p = tid; p < npairs; p += kHgThreads) {
+    int a = 0, r = p;  // p -> (a, b), a <= b, row by row of the upper triangle
+    while (r >= nn - a) {
+      r -= nn - a;
+      ++a;
+    }
+    const int b = a + r;
+    if (!A.elasticity) {
+      double s = 0.0;
+      for (int q = 0; q < nq; ++q) {
+        const double *ga = gp + ((size_t)q * nn + a) * D, *gb = gp + ((size_t)q * nn + b) * D;
+        double dot = 0.0;
+        for (int d = 0; d < D; ++d) dot += ga[d] * gb[d];
+        s += wq[q] * dot;
+      }
+      const T v = (T)(cf * s);
+      E[b * nloc + a] = v;
+      E[a * nloc + b] = v;
+      continue;
+    }
+    // K[(a,i),(b,j)] = Σ_q w [λ g_ai g_bj + μ (g_aj g_bi + δ_ij g_a·g_b)]
+    double k[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    for (int q = 0; q < nq; ++q) {
+      const double *ga = gp + ((size_t)q * nn + a) * D, *gb = gp + ((size_t)q * nn + b) * D;
+      double dot = 0.0;
+      for (int d = 0; d < D; ++d) dot += ga[d] * gb[d];
+      const double w = wq[q];
+      for (int i = 0; i < D; ++i)
+        for (int j = 0; j < D; ++j)
+          k[i][j] += w * (A.lam * ga[i] * gb[j] + A.mu * (ga[j] * gb[i] + (i == j ? dot : 0.0)));
+    }
+    for (int i = 0; i < D; ++i)
+      for (int j = 0; j < D; ++j) {
+        if (a == b && j < i) continue;  // the diagonal block from its upper triangle
+        const int li = a + nn * i, lj = b + nn * j;
+        const T v = (T)(cf * k[i][j]);
+        E[lj * nloc + li] = v;
+        E[li * nloc + lj] = v;
+      }
+  }
+  __syncthreads();  // the cell's matrix is read below
+  // load minus the lifting of the cell's Dirichlet dofs
+  const double f = A.fsrc ? A.fsrc[cell] : 1.0;
+  for (int li = tid; li < nloc; li += kHgThreads) {
+    const int a = li % nn, i = li / nn;
+    double m = 0.0;
+    for (int q = 0; q < nq; ++q) m += wq[q] * A.tab[(size_t)q * stride + 1 + a];
+    double v = (A.elasticity ? A.body[i] : f) * m;
+    if (A.dvals)
+      for (int lk = 0; lk < nloc; ++lk)
+        if (cd[lk] < 0) v -= (double)E[lk * nloc + li] * A.dvals[-cd[lk] - 1];
+    A.Ev[cell * nloc + li] = v;
+  }
+}
+
+// b[j] = Σ over j's cell-local entries (ascending cells) of the cell's load entry.
+__global__ void hexgen_rhs_kernel(int64_t nfree, int nloc, int lbits,
+                                  const int32_t *__restrict__ adjptr,
+                                  const int32_t *__restrict__ adj,
+                                  const double *__restrict__ Ev, double *__restrict__ b) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nfree) return;
+  const int lmask = (1 << lbits) - 1;
+  double s = 0.0;
+  for (int k = adjptr[j]; k < adjptr[j + 1]; ++k) {
+    const int e = adj[k];
+    s += Ev[(int64_t)(e >> lbits) * nloc + (e & lmask)];
+  }
+  b[j] = s;
+}
+
+}  // namespace
+
+bool hexgen_eligible(const Ctx &c) {
+  if (c.reffe_kind != GH_REFFE_QK || (c.dim != 2 && c.dim != 3) || hexq2_eligible(c))
+    return false;
+  if (!(c.cartesian || c.nv == (1 << c.dim))) return false;
+  const bool scalar_q2 = c.ncomp == 1 && c.order == 2;
+  const bool vector = c.ncomp == c.dim && (c.order == 1 || c.order == 2);
+  int nn = 1;
+  for (int d = 0; d < c.dim; ++d) nn *= c.order + 1;
+  return (scalar_q2 || vector) && c.nloc == nn * c.ncomp && nn <= kHgMaxNn;
+}
+
+void hexgen_assemble(Ctx &c, int precision, bool elasticity) {
+  GH_REQUIRE(hexgen_eligible(c) && (elasticity ? c.ncomp == c.dim : c.ncomp == 1),
+             GH_ERR_UNSUPPORTED,
+             elasticity ? "elasticity needs a vector space with one component per dimension"
+                        : "diffusion needs a scalar space");
+  GH_REQUIRE(c.elemlist && !c.slots, GH_ERR_STATE, "no element lists for this space");
+  GH_REQUIRE(!c.have_sigma && !c.have_fq, GH_ERR_UNSUPPORTED,
+             "the cell mass term and sources at quadrature points are provided for scalar Q1");
+  const int D = c.dim, nloc = c.nloc, nn = nloc / c.ncomp;
+  GH_REQUIRE(c.nq >= 1 && c.nq <= kHgMaxNq && (int)c.qw.size() == c.nq &&
+                 (int)c.shp.size() == c.nq * nn && (int)c.grad.size() == c.nq * nn * D,
+             GH_ERR_STATE, "the tabulation does not match the element (scalar shape functions)");
+  // per point: w, φ, ∇̂φ
+  const int stride = 1 + nn + nn * D;
+  std::vector<double> tab((size_t)c.nq * stride);
+  for (int q = 0; q < c.nq; ++q) {
+    tab[(size_t)q * stride] = c.qw[q];
+    for (int a = 0; a < nn; ++a) tab[(size_t)q * stride + 1 + a] = c.shp[(size_t)q * nn + a];
+    for (int a = 0; a < nn * D; ++a)
+      tab[(size_t)q * stride + 1 + nn + a] = c.grad[(size_t)q * nn * D + a];
+  }
+  DBuf<double> dtab, dgeo;
+  dtab.resize(tab.size());
+  GH_CHECK_HIP(hipMemcpyAsync(dtab.get(), tab.data(), sizeof(double) * tab.size(),
+                              hipMemcpyHostToDevice, c.stream));
+  const double *gtab = nullptr;
+  if (!c.cartesian) {
+    if (nn == c.nv) {  // Q1: the geometry map is the FE basis
+      dgeo.resize((size_t)c.nq * nn * D);
+      GH_CHECK_HIP(hipMemcpyAsync(dgeo.get(), c.grad.data(), sizeof(double) * dgeo.n,
+                                  hipMemcpyHostToDevice, c.stream));
+      gtab = dgeo.get();
+    } else {
+      GH_REQUIRE(c.have_gtab && c.gtab_nq == c.nq, GH_ERR_STATE,
+                 "a mapped mesh with a Q2 space needs the geometry tabulation "
+                 "(gh_geometry_tabulation) at the points of the FE tabulation");
+      gtab = c.gtab.get();
+    }
+  }
+  const bool f32 = precision == GH_PREC_FP32_ELEM;
+  const size_t esz = f32 ? sizeof(float) : sizeof(double);
+  c.elem.resize((size_t)c.ncells * nloc * nloc * esz);
+  c.evec.resize((size_t)c.ncells * nloc);
+  HgArgs A{};
+  A.D = D;
+  A.nn = nn;
+  A.nc = c.ncomp;
+  A.nloc = nloc;
+  A.nq = c.nq;
+  A.nv = c.nv;
+  A.elasticity = elasticity;
+  A.lam = c.nu / ((1.0 + c.nu) * (1.0 - 2.0 * c.nu));
+  A.mu = 1.0 / (2.0 * (1.0 + c.nu));
+  for (int d = 0; d < 3; ++d) {
+    A.h[d] = c.sizes[d];
+    A.body[d] = c.body[d];
+  }
+  A.cell_nodes = c.cartesian ? nullptr : c.cell_nodes.get();
+  A.xyz = c.cartesian ? nullptr : c.xyz.get();
+  A.tab = dtab.get();
+  A.gtab = gtab;
+  A.coef = c.have_kappa ? c.kappa.get() : nullptr;
+  A.fsrc = c.have_fsrc ? c.fsrc.get() : nullptr;
+  A.cell_dofs = c.cell_dofs.get();
+  A.dvals = c.have_dvals ? c.dvals.get() : nullptr;
+  A.E = c.elem.get();
+  A.Ev = c.evec.get();
+  const size_t shm = sizeof(double) * ((size_t)c.nq + (size_t)c.nq * nn * D) +
+                     sizeof(int32_t) * nloc;
+  if (f32)
+    hipLaunchKernelGGL(hexgen_element_kernel<float>, dim3((unsigned)c.ncells), dim3(kHgThreads),
+                       shm, c.stream, A);
+  else
+    hipLaunchKernelGGL(hexgen_element_kernel<double>, dim3((unsigned)c.ncells),
+                       dim3(kHgThreads), shm, c.stream, A);
+  GH_CHECK_HIP(hipGetLastError());
+  elemlist_sum(c, f32);
+  const int lbits = nloc <= 16 ? 4 : 7;
+  hipLaunchKernelGGL(hexgen_rhs_kernel, dim3((unsigned)std::max<int64_t>(1, (c.nfree + 255) / 256)),
+                     dim3(256), 0, c.stream, c.nfree, nloc, lbits, c.adjptr.get(), c.adj.get(),
+                     c.evec.get(), c.b.get());
+  GH_CHECK_HIP(hipGetLastError());
+  // the tabulation buffers are freed on return: wait for the kernels that read them
+  GH_CHECK_HIP(hipStreamSynchronize(c.stream));
+}
+
+}  // namespace gh

@@ -604,6 +604,9 @@ void compact_contributions(Ctx &c, int lbits) {
   c.bbase.release();
   c.bstart.release();
   if (!c.nnz || !c.cstart.get() || !c.perm.get()) return;
+  // the compact forms exist for simplices (metric records) and the Cartesian vector-Q2
+  // element layout; the general n-cube element lists keep the int32 lists
+  if (!simplex_eligible(c) && !hexq2_eligible(c)) return;
   // GRIDAP_HIP_COMPACT_LISTS=0 keeps the int32 lists (the fallback sum), so the two
   // sums can be compared on the same system
   if (const char *env = std::getenv("GRIDAP_HIP_COMPACT_LISTS"))
@@ -724,7 +727,7 @@ void elemlist_symbolic(Ctx &c) {
   c.adj.release();
   c.dmask.release();
   c.elem.release();
-  if (!(simplex_eligible(c) || hexq2_eligible(c)) || c.nfree == 0) return;
+  if (!(simplex_eligible(c) || hexq2_eligible(c) || hexgen_eligible(c)) || c.nfree == 0) return;
   hipStream_t s = c.stream;
   const int64_t total = c.ncells * c.nloc;
   const int lbits = c.nloc <= 16 ? 4 : 7;
@@ -794,11 +797,11 @@ bool elemlist_sum(Ctx &c, bool f32, const FusedRhs *rhs) {
   } else if (f32)
     hipLaunchKernelGGL((simplex_sum_kernel<float>), dim3(grid_for(c.nnz)), dim3(kBlock), 0,
                        c.stream, c.nnz, c.cstart.get(), c.perm.get(),
-                       (const float *)c.elem.get(), c.nloc == 81, c.nzval.get());
+                       (const float *)c.elem.get(), hexq2_eligible(c), c.nzval.get());
   else
     hipLaunchKernelGGL((simplex_sum_kernel<double>), dim3(grid_for(c.nnz)), dim3(kBlock), 0,
                        c.stream, c.nnz, c.cstart.get(), c.perm.get(),
-                       (const double *)c.elem.get(), c.nloc == 81, c.nzval.get());
+                       (const double *)c.elem.get(), hexq2_eligible(c), c.nzval.get());
   GH_CHECK_HIP(hipGetLastError());
   return fuse;
 }

@@ -69,6 +69,7 @@ SIGNATURES = {
     "gh_mesh_upload": (_i32, [_vp, _i32, _i64, _vp, _i64, _vp, _vp]),
     "gh_space_upload": (_i32, [_vp, _i32, _i32, _i32, _vp, _vp, _i64, _i64, _i64]),
     "gh_reffe_tabulation": (_i32, [_vp, _i32, _i32, _vp, _vp, _vp]),
+    "gh_geometry_tabulation": (_i32, [_vp, _i32, _i32, _vp]),
     "gh_face_tabulation": (_i32, [_vp, _i32, _i32, _vp, _vp, _vp]),
     "gh_boundary_upload": (_i32, [_vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp]),
     "gh_symbolic": (_i32, [_vp, _pi64]),
@@ -199,6 +200,13 @@ class Context:
         grad = np.ascontiguousarray(grad, dtype=np.float64)
         self.check(lib().gh_reffe_tabulation(self.h, len(qw), shp.shape[1], ptr(qw),
                                               ptr(shp), ptr(grad)))
+
+    def geometry_tabulation(self, grad):
+        """Gradients of the geometry map's shape functions at the points of the
+        reference tabulation, [nq, nv, dim]."""
+        grad = np.ascontiguousarray(grad, dtype=np.float64)
+        self.check(lib().gh_geometry_tabulation(self.h, grad.shape[0], grad.shape[1],
+                                                 ptr(grad)))
 
     def face_tabulation(self, qw, shp, grad):
         """Per n-cube facet: the cell's reference tabulation at the facet points;

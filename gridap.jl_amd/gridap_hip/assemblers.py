@@ -18,7 +18,7 @@ from .celldata import (BodyForceTerm, DiffusionTerm, DomainContribution, Elastic
                        MassTerm, SourceTerm, _cell_coef, get_fe_basis, get_trial_fe_basis)
 from .geometry import BoundaryTriangulation
 from .fespaces import FEFunction, TrialFESpace
-from .reference_fes import Quadrature
+from .reference_fes import Float64, Quadrature, ReferenceFE, lagrangian
 
 
 class SparseMatrixCSC:
@@ -121,6 +121,10 @@ class SparseMatrixAssembler:
             q = Quadrature(model.D, degree, simplex=self._simplex)
             shp, grad = V.reffe.tabulate(model.D, q.coords, simplex=self._simplex)
             self.ctx.reffe_tabulation(q.weights, shp, grad)
+            if not self._simplex and V.order != 1 and not model.is_affine_cartesian():
+                # the cell map is the Q1 map of the cell's vertices (get_cell_map)
+                _, ggrad = ReferenceFE(lagrangian, Float64, 1).tabulate(model.D, q.coords)
+                self.ctx.geometry_tabulation(ggrad)
             self._degree = degree
 
     @staticmethod

@@ -118,6 +118,14 @@ int gh_space_upload(gh_ctx *ctx, int reffe_kind, int order, int ncomp,
 int gh_reffe_tabulation(gh_ctx *ctx, int nq, int nloc, const double *qw, const double *shp,
                         const double *grad);
 
+/* Gradients of the geometry map's shape functions (the Q1 basis of the cell's nv
+ * vertices) at the points of the reference tabulation, grad[nq*nv*dim] (point-major),
+ * for meshes uploaded with their nodes when the FE basis is not the geometry basis
+ * (Q2 spaces on mapped hexahedra).  Replaces the evaluation of the cell map's gradient,
+ * get_cell_map (src/Geometry/Grids.jl:234-241), at the quadrature points; cleared by the
+ * next gh_reffe_tabulation. */
+int gh_geometry_tabulation(gh_ctx *ctx, int nq, int nv, const double *grad);
+
 /* Boundary integrals (Cartesian meshes, scalar Q1).  The reference tabulation at the
  * facet quadrature points, per facet (side) of the n-cube in Gridap's face order
  * (QUAD: y=0, y=1, x=0, x=1; HEX: z=0, z=1, y=0, y=1, x=0, x=1): facet weights qw[nqf]

@@ -547,3 +547,59 @@ def elasticity_q2_box(oracle, partition, E, nu, body_force, tag_entities, g=None
                 vecs[c] = vecs[c] - mats[c][:, msk] @ dvals[-cd[c, msk] - 1]
     return dict(cells=cells, cell_dofs=cd, nfree=nfree, ndiri=ndiri, cellmats=mats,
                 cellvecs=vecs, dirichlet_vals=dvals, Khat=Khat, m=m, h=h)
+
+
+# ---- n-cube elements on the isoparametric Q1 map ---------------------------------------
+# get_cell_map (src/Geometry/Grids.jl:234-241): x(ξ) = Σ_v X_v N_v(ξ) with the Q1 basis of
+# the cell's vertices; IntegrationMap (src/Fields/FieldsInterfaces.jl:737-760) sums the
+# integrand times w|det J| over the points; gradients are pushed forward with J⁻ᵀ
+# (push_∇, src/Fields/FieldsInterfaces.jl:94-98).
+
+def physical_gradients(X, qw, ggrad, grad):
+    """w|det J| [C, nq] and the physical gradients J⁻ᵀ∇̂φ [C, nq, nn, D] of the FE basis,
+    J[c, q] = Σ_v X[c, v] ⊗ ggrad[q, v] (X: [C, nv, D] vertex coordinates; ggrad:
+    [nq, nv, D] Q1 gradients; grad: [nq, nn, D] FE gradients)."""
+    X = np.asarray(X, dtype=np.float64)
+    J = np.einsum("cvd,qve->cqde", X, np.asarray(ggrad, dtype=np.float64))
+    det = np.linalg.det(J)
+    Jinv = np.linalg.inv(J)                                  # [c, q, e, d]
+    g = np.einsum("cqed,qae->cqad", Jinv, np.asarray(grad, dtype=np.float64))
+    return np.asarray(qw, dtype=np.float64)[None, :] * np.abs(det), g
+
+
+def ncube_diffusion_cells(wdet, g, kappa=None):
+    """[C, nn, nn]: κ_c Σ_q w|det J| ∇φ_a·∇φ_b."""
+    K = np.einsum("cq,cqad,cqbd->cab", wdet, g, g)
+    return K if kappa is None else np.asarray(kappa, dtype=np.float64)[:, None, None] * K
+
+
+def ncube_elasticity_cells(wdet, g, nu, E=None):
+    """[C, D·nn, D·nn] with component-blocked local dofs (node + nn·comp):
+    K[(a,i),(b,j)] = E_c Σ_q w|det J| [λ g_ai g_bj + μ (g_aj g_bi + δ_ij g_a·g_b)],
+    λ, μ of E = 1 and ν (lame)."""
+    C, nq, nn, D = g.shape
+    lam, mu = lame(1.0, nu)
+    K = np.zeros((C, D * nn, D * nn))
+    dot = np.einsum("cq,cqad,cqbd->cab", wdet, g, g)
+    for i in range(D):
+        for j in range(D):
+            blk = lam * np.einsum("cq,cqa,cqb->cab", wdet, g[..., i], g[..., j]) \
+                + mu * np.einsum("cq,cqa,cqb->cab", wdet, g[..., j], g[..., i])
+            if i == j:
+                blk = blk + mu * dot
+            K[:, i * nn:(i + 1) * nn, j * nn:(j + 1) * nn] = blk
+    return K if E is None else np.asarray(E, dtype=np.float64)[:, None, None] * K
+
+
+def ncube_loads(wdet, shp, values):
+    """[C, ncomp·nn]: values[c, k] Σ_q w|det J| φ_a for component k (component-blocked);
+    values: [C, ncomp]."""
+    m = np.einsum("cq,qa->ca", wdet, np.asarray(shp, dtype=np.float64))
+    v = np.asarray(values, dtype=np.float64)
+    return np.concatenate([v[:, k:k + 1] * m for k in range(v.shape[1])], axis=1)
+
+
+def q1_map_points(X, n1):
+    """Physical coordinates [C, npts, D] of reference points under each cell's Q1 map
+    (n1: [npts, nv] Q1 shape values at the points)."""
+    return np.einsum("pv,cvd->cpd", np.asarray(n1, dtype=np.float64), X)
