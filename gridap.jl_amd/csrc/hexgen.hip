@@ -11,10 +11,10 @@
 //
 // Two passes over element matrices stored as full columns:
 //  * hexgen_element_kernel: one workgroup per cell; the points' weights and physical
-//    gradients go to LDS (FP64), each thread then forms the blocks of its pairs of
-//    scalar nodes a <= b (both (a,b) and (b,a) written from one value, so the matrix is
-//    symmetric bit for bit), and the cell's load minus the lifting of its Dirichlet
-//    dofs (attach_dirichlet, src/CellData/AttachDirichlet.jl:59-84).  Element
+//    gradients go to LDS (FP64), each thread then forms the block of its ordered pair
+//    of scalar nodes (a, b) (the products w (g_a g_b) commute, so the blocks of (a, b)
+//    and (b, a) are transposes bit for bit), and the cell's load minus the lifting of
+//    its Dirichlet dofs (attach_dirichlet, src/CellData/AttachDirichlet.jl:59-84).  Element
 //    arithmetic in FP64; the stored matrices in the precision mode's type;
 //  * the contribution-list sum (one thread per nonzero, ascending cells) of the
 //    element-list path for nzval, and hexgen_rhs_kernel (one thread per free dof,
@@ -28,7 +28,7 @@
 namespace gh {
 namespace {
 
-constexpr int kHgThreads = 128;
+constexpr int kHgThreads = 256;
 constexpr int kHgMaxNn = 27;   // scalar shape functions (Q2 hexahedron)
 constexpr int kHgMaxNq = 64;   // quadrature points
 
@@ -50,8 +50,9 @@ struct HgArgs {
   double *Ev;                  // per cell: nloc load entries
 };
 
-__device__ __forceinline__ double det_inv(int D, const double J[3][3], double Ji[3][3]) {
-  if (D == 2) {
+template <int D>
+__device__ __forceinline__ double det_inv(const double J[3][3], double Ji[3][3]) {
+  if constexpr (D == 2) {
     const double det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
     const double r = 1.0 / det;
     Ji[0][0] = J[1][1] * r;
@@ -77,10 +78,10 @@ __device__ __forceinline__ double det_inv(int D, const double J[3][3], double Ji
   return det;
 }
 
-template <class T>
+template <class T, int D>
 __global__ __launch_bounds__(kHgThreads) void hexgen_element_kernel(HgArgs A) {
   extern __shared__ double sh[];
-  const int D = A.D, nn = A.nn, nq = A.nq, nloc = A.nloc;
+  const int nn = A.nn, nq = A.nq, nloc = A.nloc;
   double *wq = sh;                                  // nq: w |det J|
   double *gp = sh + nq;                             // nq x nn x D: J⁻ᵀ∇̂φ
   int32_t *cd = reinterpret_cast<int32_t *>(gp + (size_t)nq * nn * D);  // nloc dofs
@@ -101,7 +102,7 @@ __global__ __launch_bounds__(kHgThreads) void hexgen_element_kernel(HgArgs A) {
         for (int d = 0; d < D; ++d)
           for (int e = 0; e < D; ++e) J[d][e] += X[d] * gg[v * D + e];
       }
-      det = det_inv(D, J, Ji);
+      det = det_inv<D>(J, Ji);
     } else {
       det = 1.0;
       for (int d = 0; d < 3; ++d)
@@ -123,14 +124,11 @@ __global__ __launch_bounds__(kHgThreads) void hexgen_element_kernel(HgArgs A) {
   __syncthreads();
   T *E = static_cast<T *>(A.E) + (size_t)cell * nloc * nloc;
   const double cf = A.coef ? A.coef[cell] : 1.0;
-  const int npairs = nn * (nn + 1) / 2;
-  for (int p = tid; p < npairs; p += kHgThreads) {
-    int a = 0, r = p;  // p -> (a, b), a <= b, row by row of the upper triangle
-    while (r >= nn - a) {
-      r -= nn - a;
-      ++a;
-    }
-    const int b = a + r;
+  // one thread per ordered pair (a, b), consecutive threads on consecutive rows a, so
+  // the column stores are contiguous runs; every product is formed as w (g_a g_b),
+  // which is the same number for (b, a), so the matrix is symmetric bit for bit
+  for (int p = tid; p < nn * nn; p += kHgThreads) {
+    const int a = p % nn, b = p / nn;
     if (!A.elasticity) {
       double s = 0.0;
       for (int q = 0; q < nq; ++q) {
@@ -139,29 +137,29 @@ __global__ __launch_bounds__(kHgThreads) void hexgen_element_kernel(HgArgs A) {
         for (int d = 0; d < D; ++d) dot += ga[d] * gb[d];
         s += wq[q] * dot;
       }
-      const T v = (T)(cf * s);
-      E[b * nloc + a] = v;
-      E[a * nloc + b] = v;
+      E[b * nloc + a] = (T)(cf * s);
       continue;
     }
     // K[(a,i),(b,j)] = Σ_q w [λ g_ai g_bj + μ (g_aj g_bi + δ_ij g_a·g_b)]
-    double k[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    //                = λ G_ij + μ (G_ji + δ_ij tr G),  G_ij = Σ_q w (g_ai g_bj)
+    double G[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
     for (int q = 0; q < nq; ++q) {
       const double *ga = gp + ((size_t)q * nn + a) * D, *gb = gp + ((size_t)q * nn + b) * D;
-      double dot = 0.0;
-      for (int d = 0; d < D; ++d) dot += ga[d] * gb[d];
       const double w = wq[q];
+      double gav[3], gbv[3];
+      for (int d = 0; d < D; ++d) {
+        gav[d] = ga[d];
+        gbv[d] = gb[d];
+      }
       for (int i = 0; i < D; ++i)
-        for (int j = 0; j < D; ++j)
-          k[i][j] += w * (A.lam * ga[i] * gb[j] + A.mu * (ga[j] * gb[i] + (i == j ? dot : 0.0)));
+        for (int j = 0; j < D; ++j) G[i][j] += w * (gav[i] * gbv[j]);
     }
-    for (int i = 0; i < D; ++i)
-      for (int j = 0; j < D; ++j) {
-        if (a == b && j < i) continue;  // the diagonal block from its upper triangle
-        const int li = a + nn * i, lj = b + nn * j;
-        const T v = (T)(cf * k[i][j]);
-        E[lj * nloc + li] = v;
-        E[li * nloc + lj] = v;
+    double tr = 0.0;
+    for (int d = 0; d < D; ++d) tr += G[d][d];
+    for (int j = 0; j < D; ++j)
+      for (int i = 0; i < D; ++i) {
+        const double k = A.lam * G[i][j] + A.mu * (G[j][i] + (i == j ? tr : 0.0));
+        E[(b + nn * j) * nloc + a + nn * i] = (T)(cf * k);
       }
   }
   __syncthreads();  // the cell's matrix is read below
@@ -277,12 +275,16 @@ void hexgen_assemble(Ctx &c, int precision, bool elasticity) {
   A.Ev = c.evec.get();
   const size_t shm = sizeof(double) * ((size_t)c.nq + (size_t)c.nq * nn * D) +
                      sizeof(int32_t) * nloc;
-  if (f32)
-    hipLaunchKernelGGL(hexgen_element_kernel<float>, dim3((unsigned)c.ncells), dim3(kHgThreads),
-                       shm, c.stream, A);
-  else
-    hipLaunchKernelGGL(hexgen_element_kernel<double>, dim3((unsigned)c.ncells),
-                       dim3(kHgThreads), shm, c.stream, A);
+  auto go = [&](auto kern) {
+    hipLaunchKernelGGL(kern, dim3((unsigned)c.ncells), dim3(kHgThreads), shm, c.stream, A);
+  };
+  if (D == 2) {
+    if (f32) go(hexgen_element_kernel<float, 2>);
+    else go(hexgen_element_kernel<double, 2>);
+  } else {
+    if (f32) go(hexgen_element_kernel<float, 3>);
+    else go(hexgen_element_kernel<double, 3>);
+  }
   GH_CHECK_HIP(hipGetLastError());
   elemlist_sum(c, f32);
   const int lbits = nloc <= 16 ? 4 : 7;

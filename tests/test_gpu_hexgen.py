@@ -212,14 +212,16 @@ def test_scalar_q2_quadratic_patch_24cube(precision):
 
 @pytest.mark.parametrize("order", [1, 2])
 def test_mapped_elasticity_rigid_motions(order):
-    """No Dirichlet dofs: the six rigid motions, interpolated at the dof nodes of the
-    mapped mesh, lie in the kernel of A (1e-11 relative to max|A| max|r|), FP64."""
+    """No Dirichlet dofs: A is symmetric bit for bit, and the six rigid motions,
+    interpolated at the dof nodes of the mapped mesh, lie in its kernel (1e-11 relative
+    to max|A| max|r|), FP64."""
     part = (16, 14, 12) if order == 1 else (8, 7, 6)
     announce("hexgen mapped rigid motions order %d %s" % (order, part))
     C = int(np.prod(part))
     E = np.exp(np.random.default_rng(order).normal(size=C))
     op, V, model = product(part, order, True, (), None, E, (0.0, 0.0, -1.0), mapped=True)
     A = op.get_matrix().to_scipy()
+    assert (A != A.T).nnz == 0          # symmetric bit for bit
     if order == 1:
         x = model.node_coordinates()[V.free_dof_to_node - 1]
         comp = V.free_dof_to_comp - 1
