@@ -1,7 +1,8 @@
 // Boundary integrals into the same CSC and right-hand side: the terms of a weak form
 // integrated over a BoundaryTriangulation (src/Geometry/BoundaryTriangulations.jl:
 // 122-183) of a Cartesian mesh, as in the Neumann and Nitsche drivers of
-// test/GridapTests/PoissonTests.jl:101-108.
+// test/GridapTests/PoissonTests.jl:101-108 (scalar Q1, and the driver's own order 2 on
+// quadrilaterals).
 //
 // Per boundary face F (a facet `side` of cell c, outward unit normal n) with the face
 // quadrature (reference weights w_q on the facet, the cell's reference tabulation at
@@ -38,9 +39,9 @@ struct FaceArgs {
   double *nzval, *b;
 };
 
-template <int D>
+// NL: the cell's scalar shape functions (4, 8: Q1; 9: Q2 quadrilateral)
+template <int D, int NL>
 __global__ __launch_bounds__(128) void boundary_face_kernel(const FaceArgs A) {
-  constexpr int NL = 1 << D;
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= A.nfaces) return;
   const int64_t f = A.faces[t];
@@ -172,9 +173,12 @@ void boundary_coloring(Ctx &c) {
 
 void assemble_boundary(Ctx &c) {
   if (c.nbfaces == 0) return;
-  GH_REQUIRE(c.cartesian && c.reffe_kind == GH_REFFE_QK && c.order == 1 && c.ncomp == 1 &&
-                 c.nloc == (1 << c.dim),
-             GH_ERR_UNSUPPORTED, "boundary terms are provided for scalar Q1 on Cartesian meshes");
+  const bool q1 = c.order == 1 && c.nloc == (1 << c.dim);
+  const bool q2quad = c.order == 2 && c.dim == 2 && c.nloc == 9;
+  GH_REQUIRE(c.cartesian && c.reffe_kind == GH_REFFE_QK && c.ncomp == 1 && (q1 || q2quad),
+             GH_ERR_UNSUPPORTED,
+             "boundary terms are provided for scalar Q1, and Q2 on quadrilaterals, on "
+             "Cartesian meshes");
   GH_REQUIRE(c.bface_tab_ok, GH_ERR_STATE, "no face tabulation: call gh_face_tabulation");
   if (c.bface_ncolors == 0) boundary_coloring(c);
   FaceArgs A{};
@@ -198,10 +202,12 @@ void assemble_boundary(Ctx &c) {
     A.nfaces = c.bface_color_ptr[col + 1] - c.bface_color_ptr[col];
     if (A.nfaces == 0) continue;
     const unsigned grid = (unsigned)((A.nfaces + 127) / 128);
-    if (c.dim == 2)
-      hipLaunchKernelGGL(boundary_face_kernel<2>, dim3(grid), dim3(128), 0, c.stream, A);
+    if (q2quad)
+      hipLaunchKernelGGL((boundary_face_kernel<2, 9>), dim3(grid), dim3(128), 0, c.stream, A);
+    else if (c.dim == 2)
+      hipLaunchKernelGGL((boundary_face_kernel<2, 4>), dim3(grid), dim3(128), 0, c.stream, A);
     else
-      hipLaunchKernelGGL(boundary_face_kernel<3>, dim3(grid), dim3(128), 0, c.stream, A);
+      hipLaunchKernelGGL((boundary_face_kernel<3, 8>), dim3(grid), dim3(128), 0, c.stream, A);
     GH_CHECK_HIP(hipGetLastError());
   }
 }

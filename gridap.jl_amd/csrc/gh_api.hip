@@ -487,8 +487,10 @@ int gh_source_qp_upload(gh_ctx *ctx, const double *fq) {
       c.fq.release();
       return;
     }
-    GH_REQUIRE(c.reffe_kind == GH_REFFE_QK && c.order == 1 && c.ncomp == 1, GH_ERR_UNSUPPORTED,
-               "sources at quadrature points are provided for scalar Q1 spaces");
+    GH_REQUIRE(c.reffe_kind == GH_REFFE_QK && c.ncomp == 1 &&
+                   (c.order == 1 || gh::hexgen_eligible(c)),
+               GH_ERR_UNSUPPORTED,
+               "sources at quadrature points are provided for scalar Q1 and Q2 spaces");
     const size_t n = (size_t)c.ncells * c.nq;
     c.fq.resize(n);
     GH_CHECK_HIP(hipMemcpy(c.fq.get(), fq, sizeof(double) * n, hipMemcpyHostToDevice));

@@ -43,6 +43,7 @@ struct HgArgs {
   const double *gtab;          // per point: ∇̂N_v (nv·D) of the geometry map
   const double *coef;          // κ (diffusion) or E (elasticity) per cell, or nullptr: 1
   const double *fsrc;          // f per cell (diffusion), or nullptr: 1
+  const double *fq;            // f per cell and point (diffusion; replaces fsrc), or nullptr
   double body[3];              // body force (elasticity)
   const int32_t *cell_dofs;
   const double *dvals;         // Dirichlet values, or nullptr
@@ -168,8 +169,15 @@ __global__ __launch_bounds__(kHgThreads) void hexgen_element_kernel(HgArgs A) {
   for (int li = tid; li < nloc; li += kHgThreads) {
     const int a = li % nn, i = li / nn;
     double m = 0.0;
-    for (int q = 0; q < nq; ++q) m += wq[q] * A.tab[(size_t)q * stride + 1 + a];
-    double v = (A.elasticity ? A.body[i] : f) * m;
+    double v;
+    if (A.fq) {  // Σ_q w|J| f_q φ_a
+      for (int q = 0; q < nq; ++q)
+        m += wq[q] * A.fq[cell * nq + q] * A.tab[(size_t)q * stride + 1 + a];
+      v = m;
+    } else {
+      for (int q = 0; q < nq; ++q) m += wq[q] * A.tab[(size_t)q * stride + 1 + a];
+      v = (A.elasticity ? A.body[i] : f) * m;
+    }
     if (A.dvals)
       for (int lk = 0; lk < nloc; ++lk)
         if (cd[lk] < 0) v -= (double)E[lk * nloc + li] * A.dvals[-cd[lk] - 1];
@@ -212,8 +220,9 @@ void hexgen_assemble(Ctx &c, int precision, bool elasticity) {
              elasticity ? "elasticity needs a vector space with one component per dimension"
                         : "diffusion needs a scalar space");
   GH_REQUIRE(c.elemlist && !c.slots, GH_ERR_STATE, "no element lists for this space");
-  GH_REQUIRE(!c.have_sigma && !c.have_fq, GH_ERR_UNSUPPORTED,
-             "the cell mass term and sources at quadrature points are provided for scalar Q1");
+  GH_REQUIRE(!c.have_sigma, GH_ERR_UNSUPPORTED, "the cell mass term is provided for scalar Q1");
+  GH_REQUIRE(!c.have_fq || (!elasticity && c.fq.n == (size_t)c.ncells * c.nq), GH_ERR_STATE,
+             "the quadrature-point source belongs to another mesh or quadrature");
   const int D = c.dim, nloc = c.nloc, nn = nloc / c.ncomp;
   GH_REQUIRE(c.nq >= 1 && c.nq <= kHgMaxNq && (int)c.qw.size() == c.nq &&
                  (int)c.shp.size() == c.nq * nn && (int)c.grad.size() == c.nq * nn * D,
@@ -269,6 +278,7 @@ void hexgen_assemble(Ctx &c, int precision, bool elasticity) {
   A.gtab = gtab;
   A.coef = c.have_kappa ? c.kappa.get() : nullptr;
   A.fsrc = c.have_fsrc ? c.fsrc.get() : nullptr;
+  A.fq = c.have_fq ? c.fq.get() : nullptr;
   A.cell_dofs = c.cell_dofs.get();
   A.dvals = c.have_dvals ? c.dvals.get() : nullptr;
   A.E = c.elem.get();

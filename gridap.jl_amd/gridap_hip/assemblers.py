@@ -308,11 +308,14 @@ def _eval_cell_points(coef, V, degree):
     x at the physical points (CartesianMap, or the isoparametric map of the nodes), FE
     functions by their cell values and the basis, per-cell values broadcast."""
     model = V.model
-    if V.simplex or V.order != 1 or V.ncomp != 1:
-        raise NotImplementedError("sources f(x) are provided for scalar Q1 spaces")
+    if V.simplex or V.order not in (1, 2) or V.ncomp != 1:
+        raise NotImplementedError("sources f(x) are provided for scalar Q1 and Q2 spaces "
+                                  "on n-cube meshes")
     D = model.D
     q = Quadrature(D, degree)
-    shp, _ = V.reffe.tabulate(D, q.coords)                    # [q, a]
+    shp, _ = V.reffe.tabulate(D, q.coords)                    # [q, a]: the FE basis
+    # the cell map: the Q1 basis of the cell's vertices (get_cell_map)
+    gshp = shp if V.order == 1 else ReferenceFE(lagrangian, Float64, 1).tabulate(D, q.coords)[0]
     C, nq = model.num_cells(), len(q.weights)
     if isinstance(coef, CoefProduct):
         out = np.ones((C, nq))
@@ -323,8 +326,8 @@ def _eval_cell_points(coef, V, degree):
         pc = coef.per_cell(C)
         return np.ones((C, nq)) * (1.0 if pc is None else pc[:, None])
     if isinstance(coef, PointField):
-        X = model.node_coordinates()[model.cell_node_ids() - 1]   # [c, a, D]
-        x = np.einsum("qa,cad->cqd", shp, X)
+        X = model.node_coordinates()[model.cell_node_ids() - 1]   # [c, v, D]
+        x = np.einsum("qv,cvd->cqd", gshp, X)
         val = np.asarray(coef.func(x.reshape(-1, D)), dtype=np.float64)
         return np.broadcast_to(val.reshape(-1), (C * nq,)).reshape(C, nq).copy()
     if isinstance(coef, FEValue):

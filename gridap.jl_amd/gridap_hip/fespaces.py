@@ -30,8 +30,8 @@ class FESpace:
             if reffe.order not in (1, 2) or reffe.ncomp != 1:
                 raise NotImplementedError("simplicial spaces: scalar P1 and P2")
         elif reffe.order == 2:
-            if model.D != 3:
-                raise NotImplementedError("Q2 spaces are provided on 3-D Cartesian models")
+            if model.D not in (2, 3):
+                raise NotImplementedError("Q2 spaces are provided on 2-D and 3-D Cartesian models")
         elif reffe.order != 1:
             raise NotImplementedError("n-cube spaces of order > 2 are not provided")
         self.model = model
@@ -120,64 +120,86 @@ class FESpace:
         self.node_and_comp_to_dof = None
 
     def _number_q2(self, labels, dirichlet_masks):
-        """Conforming (vector) Q2 numbering on a Cartesian hex grid
-        (src/FESpaces/ConformingFESpaces.jl:367-636): the grid topology numbers edges
-        and quad faces by first touch; faces are visited by dimension (vertices,
-        edges, quads, cells) in id order, each owning one node and its ncomp dofs
-        (node-major); free ids 1.., Dirichlet ids -1.. in that order, a component of a
-        tagged face being Dirichlet when its tag's mask says so.  Local dofs are
-        component-blocked: node + 27 * comp, nodes = 8 vertices, 12 edges, 6 faces, 1
-        interior in Gridap's face order."""
+        """Conforming (vector) Q2 numbering on a Cartesian quadrilateral or hexahedral
+        grid (src/FESpaces/ConformingFESpaces.jl:367-636): the grid topology numbers
+        edges (and quad faces) by first touch; faces are visited by dimension
+        (vertices, edges, quads, cells) in id order, each owning one node and its ncomp
+        dofs (node-major); free ids 1.., Dirichlet ids -1.. in that order, a component
+        of a tagged face being Dirichlet when its tag's mask says so.  Local dofs are
+        component-blocked: node + nn * comp, nodes = the vertices, edges, (quad faces)
+        and the interior in Gridap's face order (nn = 9 or 27)."""
         from .geometry import _first_touch, ncube_face_id
         m = self.model
+        D = m.D
         nc = self.ncomp
-        cells = m.cell_node_ids().astype(np.int64) - 1            # C x 8, x-fastest
+        cells = m.cell_node_ids().astype(np.int64) - 1            # C x 2^D, x-fastest
         nn = m.num_nodes()
         C = cells.shape[0]
-        hex_edges = ((0, 1), (2, 3), (4, 5), (6, 7), (0, 2), (1, 3), (4, 6), (5, 7),
-                     (0, 4), (1, 5), (2, 6), (3, 7))
-        hex_faces = ((0, 1, 2, 3), (4, 5, 6, 7), (0, 1, 4, 5), (2, 3, 6, 7), (0, 2, 4, 6),
-                     (1, 3, 5, 7))
+        if D == 2:
+            ncube_edges = ((0, 1), (2, 3), (0, 2), (1, 3))
+            edge_axes = (0, 0, 1, 1)
+            ncube_faces, face_axes = (), ()
+        else:
+            ncube_edges = ((0, 1), (2, 3), (4, 5), (6, 7), (0, 2), (1, 3), (4, 6), (5, 7),
+                           (0, 4), (1, 5), (2, 6), (3, 7))
+            edge_axes = (0,) * 4 + (1,) * 4 + (2,) * 4
+            ncube_faces = ((0, 1, 2, 3), (4, 5, 6, 7), (0, 1, 4, 5), (2, 3, 6, 7),
+                           (0, 2, 4, 6), (1, 3, 5, 7))
+            face_axes = (2, 2, 1, 1, 0, 0)
+        nle, nlf = len(ncube_edges), len(ncube_faces)
         ekeys = np.stack([np.minimum(cells[:, a], cells[:, b]) * 3 + ax
-                          for (a, b), ax in zip(hex_edges, (0,) * 4 + (1,) * 4 + (2,) * 4)],
-                         axis=1).reshape(-1)
+                          for (a, b), ax in zip(ncube_edges, edge_axes)], axis=1).reshape(-1)
         cell_edges, ne = _first_touch(ekeys)
-        cell_edges = cell_edges.reshape(C, 12)
-        fkeys = np.stack([cells[:, f[0]] * 3 + ax
-                          for f, ax in zip(hex_faces, (2, 2, 1, 1, 0, 0))], axis=1).reshape(-1)
-        cell_faces, nf = _first_touch(fkeys)
-        cell_faces = cell_faces.reshape(C, 6)
+        cell_edges = cell_edges.reshape(C, nle)
+        if nlf:
+            fkeys = np.stack([cells[:, f[0]] * 3 + ax
+                              for f, ax in zip(ncube_faces, face_axes)], axis=1).reshape(-1)
+            cell_faces, nf = _first_touch(fkeys)
+            cell_faces = cell_faces.reshape(C, nlf)
+        else:
+            cell_faces, nf = np.zeros((C, 0), dtype=np.int64), 0
         # entity of every face = the box face holding all its vertices
         part = m.partition
-        n0, n1 = part[0] + 1, part[1] + 1
-        ijk = lambda v: np.stack([v % n0, (v // n0) % n1, v // (n0 * n1)])
-        table = np.zeros((8, 8), dtype=np.int32)
-        for mk in range(8):
-            for an in range(8):
-                table[mk, an] = ncube_face_id(3, mk, an & ~mk)
+        npts = [p + 1 for p in part]
+
+        def ijk(v):
+            out, r = [], v
+            for d in range(D):
+                out.append(r % npts[d])
+                r = r // npts[d]
+            return out
+        nmask = 1 << D
+        table = np.zeros((nmask, nmask), dtype=np.int32)
+        for mk in range(nmask):
+            for an in range(nmask):
+                table[mk, an] = ncube_face_id(D, mk, an & ~mk)
 
         def entity(vs):                                   # vs: list of vertex-id arrays
             L = [ijk(v) for v in vs]
             free = np.zeros(len(vs[0]), dtype=np.int64)
             anchor = np.zeros(len(vs[0]), dtype=np.int64)
-            for d in range(3):
+            for d in range(D):
                 lo = np.all([l[d] == 0 for l in L], axis=0)
                 hi = np.all([l[d] == part[d] for l in L], axis=0)
                 free |= (~(lo | hi)).astype(np.int64) << d
                 anchor |= hi.astype(np.int64) << d
             return table[free, anchor & ~free]
 
+        he = np.array(ncube_edges)
         first_e = np.zeros(ne, dtype=np.int64)
-        first_e[cell_edges.reshape(-1)] = np.arange(C * 12)
-        ce, le = first_e // 12, first_e % 12
-        edge_ent = entity([cells[ce, np.array(hex_edges)[le, 0]], cells[ce, np.array(hex_edges)[le, 1]]])
-        first_f = np.zeros(nf, dtype=np.int64)
-        first_f[cell_faces.reshape(-1)] = np.arange(C * 6)
-        cf, lf = first_f // 6, first_f % 6
-        hf = np.array(hex_faces)
-        face_ent = entity([cells[cf, hf[lf, k]] for k in range(4)])
-        ents = [m.get_face_labeling().vertex_entity.astype(np.int64), edge_ent, face_ent,
-                np.full(C, 27, dtype=np.int64)]
+        first_e[cell_edges.reshape(-1)] = np.arange(C * nle)
+        ce, le = first_e // nle, first_e % nle
+        ev = np.stack([cells[ce, he[le, 0]], cells[ce, he[le, 1]]], 1)
+        ents = [m.get_face_labeling().vertex_entity.astype(np.int64), entity([ev[:, 0], ev[:, 1]])]
+        if nlf:
+            hf = np.array(ncube_faces)
+            first_f = np.zeros(nf, dtype=np.int64)
+            first_f[cell_faces.reshape(-1)] = np.arange(C * nlf)
+            cf, lf = first_f // nlf, first_f % nlf
+            fv = np.stack([cells[cf, hf[lf, k]] for k in range(4)], 1)
+            ents.append(entity([fv[:, k] for k in range(4)]))
+        interior = 3 ** D   # the n-cube's own face id (9 for QUAD, 27 for HEX)
+        ents.append(np.full(C, interior, dtype=np.int64))
         # tag index per face (first tag holding the entity), then per component
         tags = self.dirichlet_tags
         ntags = len(tags)
@@ -196,25 +218,27 @@ class FESpace:
         ids = np.empty(len(isd), dtype=np.int64)
         ids[~isd] = np.arange(1, np.count_nonzero(~isd) + 1)
         ids[isd] = -np.arange(1, np.count_nonzero(isd) + 1)
-        off = np.cumsum([0, nn * nc, ne * nc, nf * nc])
-        node_face = np.concatenate([cells, off[1] // nc + cell_edges, off[2] // nc + cell_faces,
-                                    off[3] // nc + np.arange(C)[:, None]], axis=1)   # C x 27
+        off = np.cumsum([0, nn, ne, nf])                  # first node of each dimension
+        node_face = np.concatenate([cells, off[1] + cell_edges, off[2] + cell_faces,
+                                    off[3] + np.arange(C)[:, None]], axis=1)   # C x nn_loc
         cd = np.concatenate([ids[node_face * nc + k] for k in range(nc)], axis=1)
         self.cell_dof_ids = np.ascontiguousarray(cd, dtype=np.int32)
         self.nfree = int(np.count_nonzero(~isd))
         self.ndirichlet = int(np.count_nonzero(isd))
-        # coordinates of every Q2 node (vertices, edge / face / cell midpoints), per dof
+        # coordinates of every Q2 node (vertices, edge / face / cell midpoints), per dof:
+        # the means of their vertices, the Q1 cell map at the reference midpoints
         xyz = m.node_coordinates()
-        ev = np.stack([cells[ce, np.array(hex_edges)[le, 0]], cells[ce, np.array(hex_edges)[le, 1]]], 1)
-        fv = np.stack([cells[cf, hf[lf, k]] for k in range(4)], 1)
-        nodes = np.concatenate([xyz, xyz[ev].mean(1), xyz[fv].mean(1), xyz[cells].mean(1)])
+        parts = [xyz, xyz[ev].mean(1)]
+        if nlf:
+            parts.append(xyz[fv].mean(1))
+        parts.append(xyz[cells].mean(1))
+        nodes = np.concatenate(parts)
         comp = np.tile(np.arange(nc), len(nodes))
         pts = np.repeat(nodes, nc, axis=0)
         self.free_dof_coords, self.free_dof_comp = pts[~isd], comp[~isd]
         self.dirichlet_dof_coords, self.dirichlet_dof_comp = pts[isd], comp[isd]
         self.node_and_comp_to_dof = None
 
-    # Gridap accessors --------------------------------------------------------
     def get_cell_dof_ids(self):
         return self.cell_dof_ids
 

@@ -603,3 +603,53 @@ def q1_map_points(X, n1):
     """Physical coordinates [C, npts, D] of reference points under each cell's Q1 map
     (n1: [npts, nv] Q1 shape values at the points)."""
     return np.einsum("pv,cvd->cpd", np.asarray(n1, dtype=np.float64), X)
+
+
+def poisson_qk_with_boundary(oracle, domain, partition, order, dirichlet_entities,
+                             face_terms, degree, face_degree, g=None, f=None):
+    """Scalar Q_order Poisson ∫ ∇v⋅∇u with a source f(x) and boundary terms on a
+    Cartesian model, assembled as Gridap's AffineFEOperator does with a weak form over
+    Ω and boundary triangulations (test/GridapTests/PoissonTests.jl:9-108): the
+    conforming numbering, cell arrays then face arrays in one assembly
+    (src/FESpaces/Assemblers.jl:485-541), the Dirichlet lifting of both
+    (src/CellData/AttachDirichlet.jl:59-84) with g at the dof nodes.  face_terms:
+    [(entity ids, alpha, beta, g(x, n), h(x, n))]; f: callable of x (None: 0)."""
+    D = len(partition)
+    model = oracle.cartesian_model(domain, partition)
+    _, cd, nfree, ndiri, _ = cartesian_numbering(partition, order, 1,
+                                                 [set(dirichlet_entities)])
+    X = model["xyz"][model["cell_nodes"] - 1]
+    C = X.shape[0]
+    qx, qw = oracle.tensor_quadrature(D, degree)
+    shp, grad = oracle.qk_tabulate(D, order, qx)
+    n1q, g1 = oracle.qk_tabulate(D, 1, qx)
+    wdet, gp = physical_gradients(X, qw, g1, grad)
+    mats = ncube_diffusion_cells(wdet, gp)
+    if f is None:
+        vecs = np.zeros((C, shp.shape[1]))
+    else:
+        xq = q1_map_points(X, n1q)                                  # [C, nq, D]
+        fq = np.asarray(f(xq.reshape(-1, D)), dtype=np.float64).reshape(C, -1)
+        vecs = np.einsum("cq,cq,qa->ca", wdet, fq, shp)
+    all_cd, all_m, all_v = [cd], [mats], [vecs]
+    for tents, al, be, gf, hf in face_terms:
+        faces = oracle.boundary_faces(model, tents)
+        fm, fv = oracle.boundary_face_arrays(model, faces, face_degree, al, be, gf, hf,
+                                             order=order)
+        all_cd.append(cd[[fc[0] for fc in faces]])
+        all_m.append(fm)
+        all_v.append(fv)
+    cd_all = np.ascontiguousarray(np.concatenate(all_cd), dtype=np.int32)
+    m_all = np.ascontiguousarray(np.concatenate(all_m))
+    v_all = np.ascontiguousarray(np.concatenate(all_v))
+    dvals = np.zeros(ndiri)
+    if g is not None and ndiri:
+        n1, _ = oracle.qk_tabulate(D, 1, oracle.qk_nodes(D, order))
+        P = q1_map_points(X, n1)                                    # [C, nn, D]
+        neg = cd < 0
+        dvals[-cd[neg] - 1] = [g(x) for x in P[neg]]
+        v_all = oracle.attach_dirichlet(cd_all, m_all, dvals, v_all)
+    colptr, rowval, nzval = oracle.assemble_csc(cd_all, m_all, nfree, nfree)
+    b = oracle.assemble_vector(cd_all, v_all, nfree)
+    return dict(model=model, cell_dofs=cd, nfree=nfree, ndiri=ndiri, colptr=colptr,
+                rowval=rowval, nzval=nzval, b=b, dirichlet_vals=dvals)

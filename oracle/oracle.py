@@ -515,12 +515,13 @@ def boundary_faces(model, tag_entities):
     return out
 
 
-def boundary_face_arrays(model, faces, degree, alpha=0.0, beta=0.0, g=None, h=None):
+def boundary_face_arrays(model, faces, degree, alpha=0.0, beta=0.0, g=None, h=None, order=1):
     """Face matrices [f, lj, li] and vectors [f, a] (cell-local numbering) of
         ∫_F α φ_a φ_b + β (φ_a n⋅∇φ_b + n⋅∇φ_a φ_b)   and   ∫_F φ_a g + n⋅∇φ_a h
-    for Q1 cells: the facet quadrature (tensor Gauss of `degree` on the (D-1)-cube) is
-    mapped into the cell's reference coordinates, the cell basis and its isoparametric
-    gradients are evaluated there, the facet measure is sqrt(det(TᵀT)) of the facet
+    for Q_order cells on the Q1 map of their vertices: the facet quadrature (tensor
+    Gauss of `degree` on the (D-1)-cube) is mapped into the cell's reference
+    coordinates, the cell basis and its gradients pushed forward by the Q1 map's
+    Jacobian are evaluated there, the facet measure is sqrt(det(TᵀT)) of the facet
     tangents T, and the normal is the unit normal of the facet oriented away from the
     cell centre (the geometry of src/Geometry/BoundaryTriangulations.jl:122-183 and
     get_normal_vector).  g, h: callables (x, n) -> value (None = 0)."""
@@ -532,7 +533,7 @@ def boundary_face_arrays(model, faces, degree, alpha=0.0, beta=0.0, g=None, h=No
     else:                                                  # first facet coordinate fastest
         eta = np.array([[x1[i], x1[j]] for j in range(len(x1)) for i in range(len(x1))])
         we = np.array([w1[i] * w1[j] for j in range(len(x1)) for i in range(len(x1))])
-    nl = 1 << D
+    nl = (order + 1) ** D
     mats = np.zeros((len(faces), nl, nl))
     vecs = np.zeros((len(faces), nl))
     for k, (c, loc, axis, val) in enumerate(faces):
@@ -540,11 +541,12 @@ def boundary_face_arrays(model, faces, degree, alpha=0.0, beta=0.0, g=None, h=No
         xi = np.zeros((len(we), D))
         xi[:, others] = eta
         xi[:, axis] = val
-        shp, grad = qk_tabulate(D, 1, xi)                 # [q, a], [q, a, D]
-        X = xyz[cn[c]]                                     # [a, D]
+        shp1, grad1 = qk_tabulate(D, 1, xi)               # the Q1 cell map
+        shp, grad = qk_tabulate(D, order, xi)             # [q, a], [q, a, D]
+        X = xyz[cn[c]]                                     # [v, D]
         centre = X.mean(0)
         for q in range(len(we)):
-            Jt = grad[q].T @ X                             # Jt[i, j] = ∂x_j/∂ξ_i
+            Jt = grad1[q].T @ X                            # Jt[i, j] = ∂x_j/∂ξ_i
             gphys = np.linalg.solve(Jt, grad[q].T).T       # [a, D]
             T = Jt[others]                                 # facet tangents (rows)
             dS = np.sqrt(np.linalg.det(T @ T.T)) * we[q]
@@ -553,7 +555,7 @@ def boundary_face_arrays(model, faces, degree, alpha=0.0, beta=0.0, g=None, h=No
             else:
                 n = np.cross(T[0], T[1])
             n = n / np.linalg.norm(n)
-            x = shp[q] @ X
+            x = shp1[q] @ X
             if np.dot(n, x - centre) < 0:
                 n = -n
             dn = gphys @ n

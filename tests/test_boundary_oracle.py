@@ -80,3 +80,64 @@ def test_poisson_driver_manufactured_solution(oracle, part):
     assert abs(A - A.T).max() <= 1e-15 * abs(A).max()
     _, bad = _driver(oracle, part, normal_sign=-1.0)
     assert bad > 1e-3
+
+
+# ---- the driver at its own order 2 (PoissonTests.jl:12, u_scal = x1² + x2, f = -Δu) ----
+
+U2 = lambda x: x[..., 0] ** 2 + x[..., 1]
+GU2 = lambda x: np.array([2 * x[0], 1.0])
+
+
+def _terms(part, gam=10.0, normal_sign=1.0, u=U, gu=GU):
+    h = 1.0 / part[0]
+    return [([7, 8], 0.0, 0.0, lambda x, n: normal_sign * (gu(x) @ n), None),
+            ([6], gam / h, -1.0, lambda x, n: gam / h * u(x), lambda x, n: -u(x))]
+
+
+def test_qk_driver_order1_equals_q1_driver(oracle):
+    """The Q_k restatement (conforming numbering, numpy cell integrals, Q_k face arrays)
+    at order 1 equals the pinned Q1 driver."""
+    import conforming_oracle as CO
+    part = (5, 4)
+    r1 = oracle.poisson_q1_with_boundary((0, 1, 0, 1), part, [1, 2, 5], _terms(part), g=U,
+                                         fsrc=np.zeros(int(np.prod(part))))
+    rk = CO.poisson_qk_with_boundary(oracle, (0, 1, 0, 1), part, 1, [1, 2, 5], _terms(part),
+                                     2, 2, g=U)
+    np.testing.assert_array_equal(rk["colptr"], r1["colptr"])
+    np.testing.assert_array_equal(rk["rowval"], r1["rowval"])
+    np.testing.assert_allclose(rk["nzval"], r1["nzval"], rtol=0, atol=1e-13)
+    np.testing.assert_allclose(rk["b"], r1["b"], rtol=0, atol=1e-13)
+
+
+def _free_coords(oracle, r, order):
+    import conforming_oracle as CO
+    m, cd = r["model"], r["cell_dofs"]
+    X = m["xyz"][m["cell_nodes"] - 1]
+    n1, _ = oracle.qk_tabulate(2, 1, oracle.qk_nodes(2, order))
+    P = CO.q1_map_points(X, n1)
+    x = np.zeros((r["nfree"], 2))
+    x[cd[cd > 0] - 1] = P[cd > 0]
+    return x
+
+
+@pytest.mark.parametrize("part", [(4, 4), (6, 5)])
+def test_qk_driver_order2_manufactured_solution(oracle, part):
+    """PoissonTests.jl at its own order 2 with u = x1² + x2 (in Q2) and f = -Δu = -2:
+    the solution at the dof nodes is u to rounding (the test asks 1e-8 in L2); a wrong
+    normal sign breaks it."""
+    import conforming_oracle as CO
+    u1 = lambda x: float(x[0] ** 2 + x[1])
+
+    def solve(sign):
+        r = CO.poisson_qk_with_boundary(oracle, (0, 1, 0, 1), part, 2, [1, 2, 5],
+                                        _terms(part, normal_sign=sign, u=u1, gu=GU2),
+                                        4, 4, g=u1, f=lambda x: np.full(len(x), -2.0))
+        A = oracle.to_scipy(r["colptr"], r["rowval"], r["nzval"], r["nfree"])
+        sol = spla.spsolve(A.tocsc(), r["b"])
+        uI = U2(_free_coords(oracle, r, 2))
+        return A, np.abs(sol - uI).max() / np.abs(uI).max()
+    A, err = solve(1.0)
+    assert err <= 1e-11
+    assert abs(A - A.T).max() <= 1e-15 * abs(A).max()
+    _, bad = solve(-1.0)
+    assert bad > 1e-3

@@ -116,3 +116,71 @@ def test_3d_robin_and_neumann_match_oracle(oracle):
     np.testing.assert_array_equal(A.rowval, ref["rowval"])
     assert relmax(A.nzval, ref["nzval"]) <= 1e-12
     assert relmax(b, ref["b"]) <= 1e-12
+
+
+# ---- the driver at its own order (PoissonTests.jl:12: order = 2, degree = order) -------
+
+U2 = lambda x: x[..., 0] ** 2 + x[..., 1]          # u_scal (PoissonTests.jl:35)
+GU2 = lambda x: np.array([2 * x[0], 1.0])
+
+
+def poisson_driver_q2(part, gam=10.0, precision="fp64"):
+    """PoissonTests.jl:9-108 as written for scalar data: Q2, degree 2, f = -Δu = -2 as a
+    function of x, Neumann data n⋅∇uh of the interpolant, Nitsche on tag 6."""
+    order = 2
+    model = G.CartesianDiscreteModel((0, 1, 0, 1), part)
+    labels = model.get_face_labeling()
+    labels.add_tag_from_tags("dirichlet", [1, 2, 5])
+    labels.add_tag_from_tags("neumann", [7, 8])
+    labels.add_tag_from_tags("nitsche", 6)
+    h = 1.0 / part[0]
+    Ω = G.Triangulation(model)
+    Γn = G.BoundaryTriangulation(model, labels, tags="neumann")
+    Γd = G.BoundaryTriangulation(model, labels, tags="nitsche")
+    dΩ, dΓn, dΓd = G.Measure(Ω, order), G.Measure(Γn, order), G.Measure(Γd, order)
+    nn, nd = G.get_normal_vector(Γn), G.get_normal_vector(Γd)
+    V = G.TestFESpace(model, G.ReferenceFE(G.lagrangian, G.Float64, order),
+                      dirichlet_tags="dirichlet", labels=labels)
+    U = G.TrialFESpace(V, U2)
+    uh = G.interpolate(U2, U)
+    f = lambda x: np.full(x.shape[:-1], -2.0)
+    a = lambda u, v: (G.integrate(G.inner(G.gradient(v), G.gradient(u)), dΩ) +
+                      G.integrate((gam / h) * v * u - v * G.dot(nd, G.gradient(u)) -
+                                  G.dot(nd, G.gradient(v)) * u, dΓd))
+    l = lambda v: (G.integrate(v * f, dΩ) +
+                   G.integrate(v * G.dot(nn, G.gradient(uh)), dΓn) +
+                   G.integrate((gam / h) * v * uh - G.dot(nd, G.gradient(v)) * uh, dΓd))
+    assem = G.SparseMatrixAssembler(U, V, precision=precision)
+    return G.AffineFEOperator(a, l, U, V, assem), V
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("part", [(4, 4), (11, 7)])
+def test_poisson_driver_order2_matches_oracle(oracle, part, precision):
+    """The scalar case of the driver at its own order 2 (general n-cube element lists,
+    source at the quadrature points, Q2 face kernel): pattern bit-exact, nzval and b
+    against the oracle (1e-12 FP64, 1e-6 relative Frobenius FP32), and the device
+    solve reproduces u = x1² + x2 (PoissonTests.jl:123 asks 1e-8 relative in L2)."""
+    import conforming_oracle as CO
+    announce("PoissonTests driver order 2 %s %s" % (part, precision))
+    op, V = poisson_driver_q2(part, precision=precision)
+    gam, h = 10.0, 1.0 / part[0]
+    u1 = lambda x: float(x[0] ** 2 + x[1])
+    terms = [([7, 8], 0.0, 0.0, lambda x, n: GU2(x) @ n, None),
+             ([6], gam / h, -1.0, lambda x, n: gam / h * u1(x), lambda x, n: -u1(x))]
+    ref = CO.poisson_qk_with_boundary(oracle, (0, 1, 0, 1), part, 2, [1, 2, 5], terms, 2, 2,
+                                      g=u1, f=lambda x: np.full(len(x), -2.0))
+    np.testing.assert_array_equal(V.cell_dof_ids, ref["cell_dofs"])
+    A, b = op.get_matrix(), op.get_vector()
+    np.testing.assert_array_equal(A.colptr, ref["colptr"])
+    np.testing.assert_array_equal(A.rowval, ref["rowval"])
+    if precision == "fp64":
+        assert relmax(A.nzval, ref["nzval"]) <= 1e-12
+        assert relmax(b, ref["b"]) <= 1e-12
+    else:
+        assert np.linalg.norm(A.nzval - ref["nzval"]) <= 1e-6 * np.linalg.norm(ref["nzval"])
+        assert np.linalg.norm(b - ref["b"]) <= 1e-6 * np.linalg.norm(ref["b"])
+    if precision == "fp64":
+        x = G.solve(op).get_free_dof_values()
+        uI = U2(V.free_dof_coords)
+        assert np.abs(x - uI).max() <= 1e-8 * np.abs(uI).max()

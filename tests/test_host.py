@@ -87,7 +87,7 @@ def test_weak_form_lowering():
         assem._lower(G.integrate((lambda x: x[..., 0]) * G.inner(G.gradient(v),
                                                                  G.gradient(u)), dΩ), None)
     with pytest.raises(NotImplementedError):
-        G.FESpace(model, G.ReferenceFE(G.lagrangian, G.Float64, 2))
+        G.FESpace(model, G.ReferenceFE(G.lagrangian, G.Float64, 3))
 
 
 def _header_symbols():
