@@ -40,10 +40,13 @@ extern "C" {
 #define GH_REFFE_PK 2 /* simplex P_k (TRI/TET)             */
 
 /* ---- weak forms ---- */
-#define GH_FORM_DIFFUSION 1 /* a(u,v)=∫ κ ∇v⋅∇u dΩ,  l(v)=∫ f v dΩ, κ and f constant per cell */
+#define GH_FORM_DIFFUSION 1 /* a(u,v)=∫ κ ∇v⋅∇u dΩ,  l(v)=∫ f v dΩ, κ and f constant per cell
+                             * (or f at the quadrature points, gh_source_qp_upload); scalar
+                             * Q1, P1, P2, and Q2 on quads/hexes */
 #define GH_FORM_ELASTICITY 2 /* a(u,v)=∫ ε(v) : σ(ε(u)) dΩ, σ = λ tr(ε) I + 2 μ ε with λ, μ from a
                               * per-cell Young's modulus (the κ array) and ν; l(v)=∫ v⋅f dΩ with f
-                              * constant (gh_elasticity_params); vector Q2 on Cartesian hexes */
+                              * constant (gh_elasticity_params); vector Q1 on quads/hexes and
+                              * vector Q2 on hexes, Cartesian or mapped */
 #define GH_FORM_PLAPLACIAN 3 /* residual and Jacobian at uh (gh_fefunction_upload) of
                               * res(u,v) = ∫ κ ∇v⋅(|∇u|^(p-2)∇u) - f v dΩ, jac(u,du,v) = ∫ κ ∇v⋅
                               * ((p-2)|∇u|^(p-4)(∇u⋅∇du)∇u + |∇u|^(p-2)∇du) dΩ, κ (NULL = 1) and
@@ -66,7 +69,9 @@ extern "C" {
 #define GH_KERNEL_Q1_LATTICE 1 /* owner-computes scalar Q1 lattice kernel               */
 #define GH_KERNEL_CELL 2       /* per-cell kernel + coloured deterministic scatter      */
 #define GH_KERNEL_COLUMN_SUM 3 /* element lists: column ranges with 16-bit lists        */
-#define GH_KERNEL_LIST_SUM 4   /* element lists: int32 lists, one thread per nonzero    */
+#define GH_KERNEL_LIST_SUM 4   /* element lists: int32 lists, one thread per nonzero (with
+                                  the general n-cube element kernel: scalar Q2, vector Q1,
+                                  mapped vector Q2)                                     */
 #define GH_KERNEL_Q2_LATTICE 5 /* owner-computes vector-Q2 lattice kernel               */
 
 #define GH_PRECOND_NONE 0
@@ -126,8 +131,9 @@ int gh_reffe_tabulation(gh_ctx *ctx, int nq, int nloc, const double *qw, const d
  * next gh_reffe_tabulation. */
 int gh_geometry_tabulation(gh_ctx *ctx, int nq, int nv, const double *grad);
 
-/* Boundary integrals (Cartesian meshes, scalar Q1).  The reference tabulation at the
- * facet quadrature points, per facet (side) of the n-cube in Gridap's face order
+/* Boundary integrals (Cartesian meshes; scalar Q1, and scalar Q2 on quadrilaterals).
+ * The reference tabulation at the facet quadrature points, per facet (side) of the
+ * n-cube in Gridap's face order
  * (QUAD: y=0, y=1, x=0, x=1; HEX: z=0, z=1, y=0, y=1, x=0, x=1): facet weights qw[nqf]
  * (of the reference facet), cell shape values shp[nsides*nqf*nloc] and reference
  * gradients grad[nsides*nqf*nloc*dim].  Replaces the face-to-cell evaluation of a
@@ -173,7 +179,8 @@ int gh_mass_upload(gh_ctx *ctx, const double *sigma);
  * rule of the uploaded tabulation), in place of the per-cell f of gh_coef_upload
  * (NULL: back to the per-cell f).  The x-dependent load of
  * test/GridapTests/PoissonTests.jl:101-108 (l(v) = ∫ v f dΩ with f a function).
- * Scalar Q1; the assembly then runs the per-cell kernel. */
+ * Scalar Q1 (the assembly then runs the per-cell kernel) and scalar Q2 (the general
+ * n-cube element kernel). */
 int gh_source_qp_upload(gh_ctx *ctx, const double *fq);
 
 /* Numeric phase on device-resident inputs: nzval and b in HBM, the matrix part of
