@@ -467,6 +467,59 @@ def q1_mapped_leg(n, ev, steps=5, warmup=1, seed=2024):
         else:
             out[name]["rel_frobenius_vs_fp64"] = float(np.linalg.norm(nz - ref) /
                                                        np.linalg.norm(ref))
+    # the per-cell kernel with coloured scatter on the same mesh (GH_PATH_GENERAL), and
+    # the same mesh with its nodes shuffled inside runs of 64 consecutive ids (no lattice
+    # topology left, the locality of a mesh generator's numbering kept): the general
+    # n-cube element lists
+    for name, prec in (("fp64", _lib.GH_PREC_FP64), ("fp32", _lib.GH_PREC_FP32_ELEM)):
+        ctx.assemble(_lib.GH_FORM_DIFFUSION, prec, _lib.GH_PATH_GENERAL)
+        ev.sync(stream)
+        evs = ev.create(2)
+        ev.record(evs[0], stream)
+        for _ in range(steps):
+            ctx.assemble(_lib.GH_FORM_DIFFUSION, prec, _lib.GH_PATH_GENERAL)
+        ev.record(evs[1], stream)
+        out[name]["cell_kernel_ms"] = ev.elapsed_ms(evs[0], evs[1]) / steps
+        ev.destroy(evs)
+    ctx.close()
+    rng = np.random.default_rng(seed + 1)
+    perm = np.arange(nnodes)
+    for k0 in range(0, nnodes, 64):
+        perm[k0:k0 + 64] = k0 + rng.permutation(min(64, nnodes - k0))
+    xyz2 = np.empty_like(model.node_coordinates())
+    xyz2[perm] = model.node_coordinates()
+    cn2 = (perm[cn - 1] + 1).astype(np.int32)
+    isd = np.zeros(nnodes, dtype=bool)
+    isd[perm] = V.node_and_comp_to_dof[:, 0] < 0
+    ids = np.empty(nnodes, dtype=np.int64)
+    ids[~isd] = np.arange(1, np.count_nonzero(~isd) + 1)
+    ids[isd] = -np.arange(1, np.count_nonzero(isd) + 1)
+    cd2 = ids[cn2 - 1].astype(np.int32)
+    ctx = _lib.Context(0)
+    ctx.mesh_upload(xyz2, cn2.ravel(), ptr(cn2))
+    ctx.space_upload(_lib.GH_REFFE_QK, 1, 1, cd2.ravel(), ptr(cd2), N, int(np.count_nonzero(isd)))
+    ctx.reffe_tabulation(q.weights, shp, grad)
+    t = time.time()
+    nnz2 = ctx.symbolic()
+    sym2 = time.time() - t
+    ctx.coef_upload(kappa, None, None)
+    stream = ctx.stream()
+    un = {"workload": "the same mesh with its nodes shuffled inside runs of 64 ids (no "
+                      "lattice topology): general n-cube element lists", "nnz": int(nnz2),
+          "symbolic_s": sym2}
+    for name, prec in (("fp64", _lib.GH_PREC_FP64), ("fp32", _lib.GH_PREC_FP32_ELEM)):
+        ctx.assemble(_lib.GH_FORM_DIFFUSION, prec, _lib.GH_PATH_AUTO)
+        ev.sync(stream)
+        evs = ev.create(2)
+        ev.record(evs[0], stream)
+        for _ in range(steps):
+            ctx.assemble(_lib.GH_FORM_DIFFUSION, prec, _lib.GH_PATH_AUTO)
+        ev.record(evs[1], stream)
+        ms = ev.elapsed_ms(evs[0], evs[1]) / steps
+        ev.destroy(evs)
+        un[name] = {"ms_per_assembly": ms, "value": nnz2 / (ms * 1e-3), "unit": "NNZ/s",
+                    "kernel": ctx.pattern_info()["last_kernel"]}
+    out["unstructured"] = un
     ctx.close()
     return out
 

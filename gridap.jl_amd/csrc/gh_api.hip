@@ -468,8 +468,8 @@ int gh_mass_upload(gh_ctx *ctx, const double *sigma) {
       c.sigma.release();
       return;
     }
-    GH_REQUIRE(c.reffe_kind == GH_REFFE_QK && c.order == 1 && c.ncomp == 1, GH_ERR_UNSUPPORTED,
-               "the cell mass term is provided for scalar Q1 spaces");
+    GH_REQUIRE(c.reffe_kind == GH_REFFE_QK && c.ncomp == 1 && (c.order == 1 || c.order == 2),
+               GH_ERR_UNSUPPORTED, "the cell mass term is provided for scalar Q1 and Q2 spaces");
     c.sigma.resize(c.ncells);
     GH_CHECK_HIP(hipMemcpyAsync(c.sigma.get(), sigma, sizeof(double) * c.ncells,
                                 hipMemcpyHostToDevice, c.stream));

@@ -1,5 +1,6 @@
 // Element-list assembly of the n-cube Lagrangian spaces the specialised kernels do not
-// take: scalar Q2 diffusion ∫ κ ∇v·∇u, f v (test/GridapTests/PoissonTests.jl, order 2),
+// take: scalar Q2 diffusion ∫ κ ∇v·∇u + σ u v, f v (test/GridapTests/PoissonTests.jl,
+// order 2), scalar Q1 diffusion on meshes without the lattice topology,
 // vector Q1 and Q2 isotropic elasticity ∫ ε(v) : σ(ε(u)), b·v
 // (test/GridapTests/IsotropicDamageTests.jl:76-85), on Cartesian cells or on the
 // isoparametric Q1 map of each cell's vertices (a CartesianDiscreteModel with a map,
@@ -34,6 +35,8 @@ constexpr int kHgMaxNq = 64;   // quadrature points
 
 struct HgArgs {
   int D, nn, nc, nloc, nq, nv;
+  int cpb;                     // cells per workgroup (kHgThreads / cpb threads each)
+  int64_t ncells;
   int elasticity;
   double lam, mu;              // Lamé parameters for E = 1 (scaled by the cell's E)
   double h[3];                 // Cartesian cell sizes (cell_nodes == nullptr)
@@ -44,6 +47,7 @@ struct HgArgs {
   const double *coef;          // κ (diffusion) or E (elasticity) per cell, or nullptr: 1
   const double *fsrc;          // f per cell (diffusion), or nullptr: 1
   const double *fq;            // f per cell and point (diffusion; replaces fsrc), or nullptr
+  const double *sigma;         // mass coefficient σ per cell (diffusion), or nullptr
   double body[3];              // body force (elasticity)
   const int32_t *cell_dofs;
   const double *dvals;         // Dirichlet values, or nullptr
@@ -79,18 +83,26 @@ __device__ __forceinline__ double det_inv(const double J[3][3], double Ji[3][3])
   return det;
 }
 
+// LDS of one cell of the element kernel, in doubles: weights, gradients, dofs.
+__host__ __device__ inline size_t hexgen_slot_doubles(int nq, int nn, int D, int nloc) {
+  return (size_t)nq + (size_t)nq * nn * D + ((size_t)nloc + 1) / 2;
+}
+
 template <class T, int D>
 __global__ __launch_bounds__(kHgThreads) void hexgen_element_kernel(HgArgs A) {
   extern __shared__ double sh[];
   const int nn = A.nn, nq = A.nq, nloc = A.nloc;
-  double *wq = sh;                                  // nq: w |det J|
-  double *gp = sh + nq;                             // nq x nn x D: J⁻ᵀ∇̂φ
+  // cpb cells per workgroup, tpc threads each (small elements share a workgroup)
+  const int tpc = kHgThreads / A.cpb;
+  const int slot = threadIdx.x / tpc, tid = threadIdx.x - slot * tpc;
+  const int64_t cell = (int64_t)blockIdx.x * A.cpb + slot;
+  const bool live = cell < A.ncells;                // no early return: barriers follow
+  double *wq = sh + (size_t)slot * hexgen_slot_doubles(nq, nn, D, nloc);  // nq: w |det J|
+  double *gp = wq + nq;                             // nq x nn x D: J⁻ᵀ∇̂φ
   int32_t *cd = reinterpret_cast<int32_t *>(gp + (size_t)nq * nn * D);  // nloc dofs
-  const int64_t cell = blockIdx.x;
-  const int tid = threadIdx.x;
   const int stride = 1 + nn + nn * D;
-  for (int a = tid; a < nloc; a += kHgThreads) cd[a] = A.cell_dofs[cell * nloc + a];
-  for (int q = tid; q < nq; q += kHgThreads) {
+  for (int a = tid; live && a < nloc; a += tpc) cd[a] = A.cell_dofs[cell * nloc + a];
+  for (int q = tid; live && q < nq; q += tpc) {
     const double *tq = A.tab + (size_t)q * stride;
     double J[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}}, Ji[3][3];
     double det;
@@ -124,11 +136,11 @@ __global__ __launch_bounds__(kHgThreads) void hexgen_element_kernel(HgArgs A) {
   }
   __syncthreads();
   T *E = static_cast<T *>(A.E) + (size_t)cell * nloc * nloc;
-  const double cf = A.coef ? A.coef[cell] : 1.0;
+  const double cf = live && A.coef ? A.coef[cell] : 1.0;
   // one thread per ordered pair (a, b), consecutive threads on consecutive rows a, so
   // the column stores are contiguous runs; every product is formed as w (g_a g_b),
   // which is the same number for (b, a), so the matrix is symmetric bit for bit
-  for (int p = tid; p < nn * nn; p += kHgThreads) {
+  for (int p = tid; live && p < nn * nn; p += tpc) {
     const int a = p % nn, b = p / nn;
     if (!A.elasticity) {
       double s = 0.0;
@@ -138,7 +150,14 @@ __global__ __launch_bounds__(kHgThreads) void hexgen_element_kernel(HgArgs A) {
         for (int d = 0; d < D; ++d) dot += ga[d] * gb[d];
         s += wq[q] * dot;
       }
-      E[b * nloc + a] = (T)(cf * s);
+      double v = cf * s;
+      if (A.sigma) {  // + σ Σ_q w|J| φ_a φ_b (the products commute: symmetric bit for bit)
+        double m = 0.0;
+        for (int q = 0; q < nq; ++q)
+          m += wq[q] * (A.tab[(size_t)q * stride + 1 + a] * A.tab[(size_t)q * stride + 1 + b]);
+        v += A.sigma[cell] * m;
+      }
+      E[b * nloc + a] = (T)v;
       continue;
     }
     // K[(a,i),(b,j)] = Σ_q w [λ g_ai g_bj + μ (g_aj g_bi + δ_ij g_a·g_b)]
@@ -165,8 +184,8 @@ __global__ __launch_bounds__(kHgThreads) void hexgen_element_kernel(HgArgs A) {
   }
   __syncthreads();  // the cell's matrix is read below
   // load minus the lifting of the cell's Dirichlet dofs
-  const double f = A.fsrc ? A.fsrc[cell] : 1.0;
-  for (int li = tid; li < nloc; li += kHgThreads) {
+  const double f = live && A.fsrc ? A.fsrc[cell] : 1.0;
+  for (int li = tid; live && li < nloc; li += tpc) {
     const int a = li % nn, i = li / nn;
     double m = 0.0;
     double v;
@@ -209,9 +228,12 @@ bool hexgen_eligible(const Ctx &c) {
   if (!(c.cartesian || c.nv == (1 << c.dim))) return false;
   const bool scalar_q2 = c.ncomp == 1 && c.order == 2;
   const bool vector = c.ncomp == c.dim && (c.order == 1 || c.order == 2);
+  // scalar Q1 without the lattice topology (unstructured hexahedra or quadrilaterals):
+  // element lists instead of the coloured per-cell scatter
+  const bool scalar_q1 = c.ncomp == 1 && c.order == 1 && !c.cartesian && !c.lattice_topo;
   int nn = 1;
   for (int d = 0; d < c.dim; ++d) nn *= c.order + 1;
-  return (scalar_q2 || vector) && c.nloc == nn * c.ncomp && nn <= kHgMaxNn;
+  return (scalar_q2 || vector || scalar_q1) && c.nloc == nn * c.ncomp && nn <= kHgMaxNn;
 }
 
 void hexgen_assemble(Ctx &c, int precision, bool elasticity) {
@@ -220,7 +242,8 @@ void hexgen_assemble(Ctx &c, int precision, bool elasticity) {
              elasticity ? "elasticity needs a vector space with one component per dimension"
                         : "diffusion needs a scalar space");
   GH_REQUIRE(c.elemlist && !c.slots, GH_ERR_STATE, "no element lists for this space");
-  GH_REQUIRE(!c.have_sigma, GH_ERR_UNSUPPORTED, "the cell mass term is provided for scalar Q1");
+  GH_REQUIRE(!c.have_sigma || (!elasticity && c.sigma.n == (size_t)c.ncells), GH_ERR_STATE,
+             "the cell mass term belongs to the diffusion form of this mesh");
   GH_REQUIRE(!c.have_fq || (!elasticity && c.fq.n == (size_t)c.ncells * c.nq), GH_ERR_STATE,
              "the quadrature-point source belongs to another mesh or quadrature");
   const int D = c.dim, nloc = c.nloc, nn = nloc / c.ncomp;
@@ -279,14 +302,21 @@ void hexgen_assemble(Ctx &c, int precision, bool elasticity) {
   A.coef = c.have_kappa ? c.kappa.get() : nullptr;
   A.fsrc = c.have_fsrc ? c.fsrc.get() : nullptr;
   A.fq = c.have_fq ? c.fq.get() : nullptr;
+  A.sigma = c.have_sigma ? c.sigma.get() : nullptr;
   A.cell_dofs = c.cell_dofs.get();
   A.dvals = c.have_dvals ? c.dvals.get() : nullptr;
   A.E = c.elem.get();
   A.Ev = c.evec.get();
-  const size_t shm = sizeof(double) * ((size_t)c.nq + (size_t)c.nq * nn * D) +
-                     sizeof(int32_t) * nloc;
+  // cells per workgroup: the most (a power of two, at most 16) whose node pairs still
+  // give every thread one
+  int cpb = 1;
+  while (cpb < 16 && 2 * cpb * nn * nn <= kHgThreads) cpb *= 2;
+  A.cpb = cpb;
+  A.ncells = c.ncells;
+  const size_t shm = sizeof(double) * hexgen_slot_doubles(c.nq, nn, D, nloc) * cpb;
   auto go = [&](auto kern) {
-    hipLaunchKernelGGL(kern, dim3((unsigned)c.ncells), dim3(kHgThreads), shm, c.stream, A);
+    hipLaunchKernelGGL(kern, dim3((unsigned)((c.ncells + cpb - 1) / cpb)), dim3(kHgThreads),
+                       shm, c.stream, A);
   };
   if (D == 2) {
     if (f32) go(hexgen_element_kernel<float, 2>);

@@ -239,3 +239,74 @@ def test_mapped_elasticity_rigid_motions(order):
     amax = np.max(np.abs(A.data))
     for r in modes:
         assert np.max(np.abs(A @ r)) <= 1e-11 * amax * np.max(np.abs(r))
+
+
+def _unstructured_q1(oracle, part, seed):
+    """A Cartesian Q1 mesh with its nodes renumbered at random and moved (no lattice
+    topology left): coordinates, 1-based cell nodes, CLagrangian cell dofs of the
+    boundary-Dirichlet space in the new node order."""
+    D = len(part)
+    m = oracle.cartesian_model((0.0, 1.0) * D, part)
+    rng = np.random.default_rng(seed)
+    nn = m["xyz"].shape[0]
+    perm = rng.permutation(nn)                 # new id of old node k: perm[k]
+    xyz = np.empty_like(m["xyz"])
+    xyz[perm] = warp(m["xyz"])
+    cn = (perm[m["cell_nodes"] - 1] + 1).astype(np.int32)
+    lab = np.empty_like(m["vertex_labels"])
+    lab[perm] = m["vertex_labels"]
+    ents = CO.boundary_entities(D)
+    node_to_tag = oracle.face_tag_index(lab, [sorted(ents)])
+    n2d, nfree, ndiri = oracle.clagrangian(node_to_tag)
+    cd = oracle.clagrangian_cell_dofs(cn, n2d)
+    return xyz, cn, cd, nfree, ndiri, n2d
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("part", [(17, 13), (9, 8, 7)])
+def test_unstructured_q1_element_lists_match_oracle(oracle, part, precision):
+    """Scalar Q1 on a mesh without the lattice topology (renumbered, moved nodes),
+    through the C ABI: ∫ κ ∇v⋅∇u + σ u v with κ, σ, f per cell and Dirichlet data go to
+    the general n-cube element lists (no coloured scatter); pattern bit-exact, values
+    within 1e-12 (FP64) / 1e-6 (FP32) of the oracle."""
+    D = len(part)
+    announce("unstructured Q1 element lists %s %s" % (part, precision))
+    xyz, cn, cd, nfree, ndiri, n2d = _unstructured_q1(oracle, part, 7)
+    C = cn.shape[0]
+    rng = np.random.default_rng(D)
+    kappa, sigma, fsrc = np.exp(rng.normal(size=C)), np.exp(rng.normal(size=C)), rng.normal(size=C)
+    dv = rng.normal(size=ndiri)
+    qx, qw = oracle.tensor_quadrature(D, 2)
+    shp, grad = oracle.qk_tabulate(D, 1, qx)
+    ptrs = (np.arange(C + 1) * (1 << D) + 1).astype(np.int32)
+    ctx = _lib.Context(0)
+    ctx.mesh_upload(xyz, cn.ravel(), ptrs)
+    ctx.space_upload(_lib.GH_REFFE_QK, 1, 1, np.ascontiguousarray(cd, np.int32).ravel(), ptrs,
+                     nfree, ndiri)
+    ctx.reffe_tabulation(qw, shp, grad.reshape(len(qw), -1))
+    ctx.symbolic()
+    ctx.coef_upload(kappa, fsrc, dv)
+    ctx.mass_upload(sigma)
+    prec = _lib.GH_PREC_FP64 if precision == "fp64" else _lib.GH_PREC_FP32_ELEM
+    ctx.assemble(_lib.GH_FORM_DIFFUSION, prec, _lib.GH_PATH_AUTO)
+    info = ctx.pattern_info()
+    assert info["lattice"] == 0 and info["last_kernel"] == _lib.GH_KERNEL_LIST_SUM, info
+    nz, b = ctx.values_get()
+    colptr, rowval = ctx.pattern_get()
+    ctx.close()
+    X = xyz[cn - 1]
+    wdet, gp = CO.physical_gradients(X, qw, grad, grad)
+    mats = CO.ncube_diffusion_cells(wdet, gp, kappa) + \
+        sigma[:, None, None] * np.einsum("cq,qa,qb->cab", wdet, shp, shp)
+    Kc = oracle.cell_matrices_diffusion(D, cn, xyz, qw, grad, kappa=kappa)
+    assert np.abs(CO.ncube_diffusion_cells(wdet, gp, kappa) - Kc).max() <= 1e-12 * np.abs(Kc).max()
+    vecs = CO.ncube_loads(wdet, shp, fsrc[:, None])
+    vecs = oracle.attach_dirichlet(cd, mats, dv, vecs)
+    cp, rv, nzr = oracle.assemble_csc(cd, mats, nfree, nfree)
+    br = oracle.assemble_vector(cd, vecs, nfree)
+    np.testing.assert_array_equal(colptr, cp)
+    np.testing.assert_array_equal(rowval, rv)
+    if precision == "fp64":
+        assert relmax(nz, nzr) <= 1e-12 and relmax(b, br) <= 1e-12
+    else:
+        assert relfrob(nz, nzr) <= 1e-6 and relfrob(b, br) <= 1e-6
