@@ -17,9 +17,12 @@
 //    and (b, a) are transposes bit for bit), and the cell's load minus the lifting of
 //    its Dirichlet dofs (attach_dirichlet, src/CellData/AttachDirichlet.jl:59-84).  Element
 //    arithmetic in FP64; the stored matrices in the precision mode's type;
-//  * the contribution-list sum (one thread per nonzero, ascending cells) of the
-//    element-list path for nzval, and hexgen_rhs_kernel (one thread per free dof,
-//    ascending cells) for b.
+//  * the column-range sum of the element-list path in its full-column form (each range
+//    walks its cells' columns with contiguous reads and stages the entries in LDS at
+//    their run positions, one thread per nonzero then adds them in ascending cell
+//    order), or the contribution-list sum when a column does not fit the 16-bit
+//    ranges, for nzval; hexgen_rhs_kernel (one thread per free dof, ascending cells)
+//    for b.
 #include <algorithm>
 #include <cmath>
 #include <vector>
@@ -204,6 +207,101 @@ __global__ __launch_bounds__(kHgThreads) void hexgen_element_kernel(HgArgs A) {
   }
 }
 
+// Scalar Q1 (NL = 2^D nodes, the FE basis is the cell map's): one thread per cell,
+// the upper triangle of the element matrix accumulated in registers point by point
+// (FP64), then stored as full columns (both halves from one value) with the load
+// minus the lifting.  A workgroup per cell leaves these small cells latency-bound.
+template <class T, int D>
+__global__ __launch_bounds__(256) void hexgen_q1_kernel(HgArgs A) {
+  constexpr int NL = 1 << D;
+  constexpr int NT = NL * (NL + 1) / 2;
+  const int64_t cell = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (cell >= A.ncells) return;
+  double X[NL][D];
+#pragma unroll
+  for (int v = 0; v < NL; ++v)
+#pragma unroll
+    for (int d = 0; d < D; ++d)
+      X[v][d] = A.cell_nodes ? A.xyz[(size_t)A.cell_nodes[cell * NL + v] * D + d] : 0.0;
+  const double kap = A.coef ? A.coef[cell] : 1.0;
+  const double sig = A.sigma ? A.sigma[cell] : 0.0;
+  const double fc = A.fsrc ? A.fsrc[cell] : 1.0;
+  double K[NT], F[NL];
+#pragma unroll
+  for (int t = 0; t < NT; ++t) K[t] = 0.0;
+#pragma unroll
+  for (int a = 0; a < NL; ++a) F[a] = 0.0;
+  const int stride = 1 + NL + NL * D;
+  for (int q = 0; q < A.nq; ++q) {
+    const double *tq = A.tab + (size_t)q * stride;
+    double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, Ji[3][3];
+    double det;
+    if (A.cell_nodes) {
+#pragma unroll
+      for (int v = 0; v < NL; ++v)
+#pragma unroll
+        for (int d = 0; d < D; ++d)
+#pragma unroll
+          for (int e = 0; e < D; ++e) J[d][e] += X[v][d] * tq[1 + NL + v * D + e];
+      det = det_inv<D>(J, Ji);
+    } else {
+      det = 1.0;
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        det *= A.h[d];
+#pragma unroll
+        for (int e = 0; e < D; ++e) Ji[d][e] = d == e ? 1.0 / A.h[d] : 0.0;
+      }
+    }
+    const double w = tq[0] * fabs(det);
+    double g[NL][D];
+#pragma unroll
+    for (int a = 0; a < NL; ++a)
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        double s = 0.0;
+#pragma unroll
+        for (int e = 0; e < D; ++e) s += Ji[e][d] * tq[1 + NL + a * D + e];
+        g[a][d] = s;
+      }
+    const double fq = A.fq ? A.fq[cell * A.nq + q] : fc;
+    int t = 0;
+#pragma unroll
+    for (int a = 0; a < NL; ++a) {
+      F[a] += w * fq * tq[1 + a];
+#pragma unroll
+      for (int b = a; b < NL; ++b, ++t) {
+        double dot = 0.0;
+#pragma unroll
+        for (int d = 0; d < D; ++d) dot += g[a][d] * g[b][d];
+        K[t] += w * (kap * dot + sig * (tq[1 + a] * tq[1 + b]));
+      }
+    }
+  }
+  T *E = static_cast<T *>(A.E) + (size_t)cell * NL * NL;
+  int t = 0;
+#pragma unroll
+  for (int a = 0; a < NL; ++a)
+#pragma unroll
+    for (int b = a; b < NL; ++b, ++t) {
+      E[b * NL + a] = (T)K[t];
+      E[a * NL + b] = (T)K[t];
+    }
+  const int32_t *cd = A.cell_dofs + cell * NL;
+#pragma unroll
+  for (int a = 0; a < NL; ++a) {
+    double v = F[a];
+    if (A.dvals)
+#pragma unroll
+      for (int b = 0; b < NL; ++b) {
+        const int32_t d = cd[b];
+        const int lo = a < b ? a : b, hi = a < b ? b : a;
+        if (d < 0) v -= (double)(T)K[lo * NL - lo * (lo - 1) / 2 + (hi - lo)] * A.dvals[-d - 1];
+      }
+    A.Ev[cell * NL + a] = v;
+  }
+}
+
 // b[j] = Σ over j's cell-local entries (ascending cells) of the cell's load entry.
 __global__ void hexgen_rhs_kernel(int64_t nfree, int nloc, int lbits,
                                   const int32_t *__restrict__ adjptr,
@@ -241,7 +339,7 @@ void hexgen_assemble(Ctx &c, int precision, bool elasticity) {
              GH_ERR_UNSUPPORTED,
              elasticity ? "elasticity needs a vector space with one component per dimension"
                         : "diffusion needs a scalar space");
-  GH_REQUIRE(c.elemlist && !c.slots, GH_ERR_STATE, "no element lists for this space");
+  GH_REQUIRE(c.elemlist, GH_ERR_STATE, "no element lists for this space");
   GH_REQUIRE(!c.have_sigma || (!elasticity && c.sigma.n == (size_t)c.ncells), GH_ERR_STATE,
              "the cell mass term belongs to the diffusion form of this mesh");
   GH_REQUIRE(!c.have_fq || (!elasticity && c.fq.n == (size_t)c.ncells * c.nq), GH_ERR_STATE,
@@ -318,7 +416,18 @@ void hexgen_assemble(Ctx &c, int precision, bool elasticity) {
     hipLaunchKernelGGL(kern, dim3((unsigned)((c.ncells + cpb - 1) / cpb)), dim3(kHgThreads),
                        shm, c.stream, A);
   };
-  if (D == 2) {
+  auto go1 = [&](auto kern) {
+    hipLaunchKernelGGL(kern, dim3((unsigned)((c.ncells + 255) / 256)), dim3(256), 0, c.stream, A);
+  };
+  if (c.order == 1 && c.ncomp == 1) {
+    if (D == 2) {
+      if (f32) go1(hexgen_q1_kernel<float, 2>);
+      else go1(hexgen_q1_kernel<double, 2>);
+    } else {
+      if (f32) go1(hexgen_q1_kernel<float, 3>);
+      else go1(hexgen_q1_kernel<double, 3>);
+    }
+  } else if (D == 2) {
     if (f32) go(hexgen_element_kernel<float, 2>);
     else go(hexgen_element_kernel<double, 2>);
   } else {

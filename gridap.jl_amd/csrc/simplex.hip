@@ -334,7 +334,8 @@ struct FusedRhs {
 // reference table go to LDS, and each contribution forms its entry Σ_k g_k Ĝ^k_e
 // from them through its slot (adjacency position, packed entry e).
 // Full columns (FULL_NLOC = 81, the vector-Q2 hexahedron layout, kHexColLd entries
-// per column): the block walks its cell-columns entry by entry with contiguous reads
+// per column; FULL_NLOC = -1, nloc rows per column of the general n-cube element
+// matrices): the block walks its cell-columns entry by entry with contiguous reads
 // and writes each entry that is a contribution to its run position (cslot holds
 // those positions, 0xffff for rows that are not free).
 // LDS (maxbase: the most adjacency entries of a block, even; maxq: the most
@@ -376,16 +377,16 @@ __global__ __launch_bounds__(kBlock) void column_sum_kernel(
     const T *__restrict__ E, double *__restrict__ nzval, FusedRhs F) {
   extern __shared__ __align__(16) uint32_t shm[];
   // staged entries per thread and batch (simplex ranges hold at most kBlock·8 contributions)
-  constexpr int U = FULL_NLOC > 0 ? 12 : 8;
+  constexpr int U = FULL_NLOC != 0 ? 12 : 8;
   // nonzeros per thread whose offsets are prefetched (a simplex range of kBlock·8
   // contributions holds about 3·kBlock nonzeros)
-  constexpr int R = FULL_NLOC > 0 ? 8 : 4;
+  constexpr int R = FULL_NLOC != 0 ? 8 : 4;
   const int rpad = metric_ref_pad(ns);
   unsigned char *lds = reinterpret_cast<unsigned char *>(shm);
   double *rv = reinterpret_cast<double *>(lds);
   T *refl = reinterpret_cast<T *>(lds + (F.b ? 8 * maxbase : 0));
   T *gl = refl + rpad;
-  T *vals = FULL_NLOC > 0 ? reinterpret_cast<T *>(lds) : gl + 6 * maxbase;
+  T *vals = FULL_NLOC != 0 ? reinterpret_cast<T *>(lds) : gl + 6 * maxbase;
   const int64_t j0 = bstart[blockIdx.x], j1 = bstart[blockIdx.x + 1];
   const int a0 = adjptr[j0], na = adjptr[j1] - a0;
   const int32_t q0 = bbase[blockIdx.x], nq = bbase[blockIdx.x + 1] - q0;
@@ -400,9 +401,12 @@ __global__ __launch_bounds__(kBlock) void column_sum_kernel(
     ost[r] = i < np ? noff[p0 + i] : 0u;
     oen[r] = i + 1 < np ? noff[p0 + i + 1] : (uint32_t)nq;
   }
-  if constexpr (FULL_NLOC > 0) {
-    const uint16_t *ip = cslot + (int64_t)a0 * FULL_NLOC;
-    const int ne = na * FULL_NLOC;
+  if constexpr (FULL_NLOC != 0) {
+    // full columns: FULL_NLOC > 0, the vector-Q2 hexahedron's padded columns; FULL_NLOC
+    // < 0, nloc rows per column of the general n-cube element matrices
+    const int fl = FULL_NLOC > 0 ? FULL_NLOC : nloc;
+    const uint16_t *ip = cslot + (int64_t)a0 * fl;
+    const int ne = na * fl;
     for (int eb = 0; eb < ne; eb += kBlock * U) {
       uint32_t pos[U];
       int32_t ae[U];
@@ -410,15 +414,17 @@ __global__ __launch_bounds__(kBlock) void column_sum_kernel(
       for (int u = 0; u < U; ++u) {
         const int e = eb + tid + u * kBlock;
         pos[u] = e < ne ? ip[e] : 0xffffu;
-        ae[u] = e < ne ? adj[a0 + e / FULL_NLOC] : 0;
+        ae[u] = e < ne ? adj[a0 + e / fl] : 0;
       }
       T v[U];
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         const int e = eb + tid + u * kBlock;
-        const int li = e - (e / FULL_NLOC) * FULL_NLOC;
+        const int li = e - (e / fl) * fl;
         const uint32_t cell = (uint32_t)(ae[u] >> lbits), lj = (uint32_t)(ae[u] & lmask);
-        v[u] = pos[u] != 0xffffu ? E[hex_elem_addr(cell, lj, li)] : (T)0;
+        const uint32_t at = FULL_NLOC > 0 ? hex_elem_addr(cell, lj, li)
+                                          : (cell * (uint32_t)fl + lj) * (uint32_t)fl + li;
+        v[u] = pos[u] != 0xffffu ? E[at] : (T)0;
       }
 #pragma unroll
       for (int u = 0; u < U; ++u)
@@ -604,21 +610,26 @@ void compact_contributions(Ctx &c, int lbits) {
   c.bbase.release();
   c.bstart.release();
   if (!c.nnz || !c.cstart.get() || !c.perm.get()) return;
-  // the compact forms exist for simplices (metric records) and the Cartesian vector-Q2
-  // element layout; the general n-cube element lists keep the int32 lists
-  if (!simplex_eligible(c) && !hexq2_eligible(c)) return;
   // GRIDAP_HIP_COMPACT_LISTS=0 keeps the int32 lists (the fallback sum), so the two
   // sums can be compared on the same system
   if (const char *env = std::getenv("GRIDAP_HIP_COMPACT_LISTS"))
     if (env[0] == '0') return;
   hipStream_t s = c.stream;
   const int64_t N = c.nfree;
-  const bool metric = c.nloc <= 16;
-  if (!metric && c.nloc != 81) return;  // full rows: the staging kernel is built for 81
+  // simplices: metric records; otherwise full columns (the padded vector-Q2 layout of
+  // the Cartesian hexahedron, or nloc rows per column of the general n-cube element
+  // matrices), addressed in 32 bits
+  const bool metric = simplex_eligible(c);
+  const bool padded = hexq2_eligible(c);
+  // general n-cube elements: full columns for the large ones (vector Q2: the mapped 32³
+  // sum 0.95 -> 0.31 ms); the lists for the small ones, whose gathers stay near each
+  // other (scalar Q2 48³: 0.37 ms with the lists, 0.79 ms with full columns)
+  if (!metric && !padded && !(hexgen_eligible(c) && c.nloc >= 48)) return;
   const int ns = metric ? c.nloc * (c.nloc + 1) / 2 : 0;
   int obits = 1;
   while ((1 << obits) < (metric ? ns : c.nloc)) ++obits;
-  if (!metric && c.ncells * (int64_t)c.nloc * kHexColLd >= (int64_t(1) << 32)) return;
+  if (!metric && c.ncells * (int64_t)c.nloc * (padded ? kHexColLd : c.nloc) >= (int64_t(1) << 32))
+    return;
   int lds64 = 0;
   GH_CHECK_HIP(hipDeviceGetAttribute(&lds64, hipDeviceAttributeMaxSharedMemoryPerBlock,
                                      c.device));
@@ -781,7 +792,8 @@ void launch_column_sum(Ctx &c, const FusedRhs &F) {
                        c.noff.get(), c.cslot.get(), (const T *)c.elem.get(), c.nzval.get(), F);
   };
   if (!c.inverse) go(column_sum_kernel<T, 0>);
-  else go(column_sum_kernel<T, 81>);
+  else if (hexq2_eligible(c)) go(column_sum_kernel<T, 81>);
+  else go(column_sum_kernel<T, -1>);
 }
 
 // nzval from the element entries; with rhs (metric records only) also b, fused

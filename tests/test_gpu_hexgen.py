@@ -21,6 +21,9 @@ import conforming_oracle as CO
 
 pytestmark = pytest.mark.gpu
 
+# the element-list sums: column ranges with full-column staging, or the int32 lists
+SUMS = (_lib.GH_KERNEL_COLUMN_SUM, _lib.GH_KERNEL_LIST_SUM)
+
 
 def announce(msg):
     print("[gpu-case] " + msg, flush=True)
@@ -111,7 +114,7 @@ def check(op, V, ref, precision):
     A = op.get_matrix()
     np.testing.assert_array_equal(A.colptr, cp)
     np.testing.assert_array_equal(A.rowval, rv)
-    assert op.assem.ctx.pattern_info()["last_kernel"] == _lib.GH_KERNEL_LIST_SUM
+    assert op.assem.ctx.pattern_info()["last_kernel"] in SUMS
     if precision == "fp64":
         assert relmax(A.nzval, nz) <= 1e-12
         assert relmax(op.get_vector(), b) <= 1e-12
@@ -179,8 +182,8 @@ def test_identity_map_matches_cartesian_vector_q2():
     tags = ("tag_21",)
     op1, V1, _ = product(part, 2, True, tags, g, E, body, identity_map=True)
     op2, V2, _ = product(part, 2, True, tags, g, E, body)
-    assert op1.assem.ctx.pattern_info()["last_kernel"] == _lib.GH_KERNEL_LIST_SUM
-    assert op2.assem.ctx.pattern_info()["last_kernel"] != _lib.GH_KERNEL_LIST_SUM
+    assert op1.assem.ctx.pattern_info()["last_kernel"] in SUMS
+    assert op2.assem.ctx.pattern_info()["last_kernel"] not in SUMS
     A1, A2 = op1.get_matrix(), op2.get_matrix()
     np.testing.assert_array_equal(A1.colptr, A2.colptr)
     np.testing.assert_array_equal(A1.rowval, A2.rowval)
@@ -290,7 +293,7 @@ def test_unstructured_q1_element_lists_match_oracle(oracle, part, precision):
     prec = _lib.GH_PREC_FP64 if precision == "fp64" else _lib.GH_PREC_FP32_ELEM
     ctx.assemble(_lib.GH_FORM_DIFFUSION, prec, _lib.GH_PATH_AUTO)
     info = ctx.pattern_info()
-    assert info["lattice"] == 0 and info["last_kernel"] == _lib.GH_KERNEL_LIST_SUM, info
+    assert info["lattice"] == 0 and info["last_kernel"] in SUMS, info
     nz, b = ctx.values_get()
     colptr, rowval = ctx.pattern_get()
     ctx.close()
@@ -310,3 +313,35 @@ def test_unstructured_q1_element_lists_match_oracle(oracle, part, precision):
         assert relmax(nz, nzr) <= 1e-12 and relmax(b, br) <= 1e-12
     else:
         assert relfrob(nz, nzr) <= 1e-6 and relfrob(b, br) <= 1e-6
+
+
+@pytest.mark.parametrize("case", ["mapped_vector_q2", "mapped_vector_q2_wide"])
+def test_column_ranges_match_list_sum(monkeypatch, case):
+    """Vector Q2 (81 entries per element column, the full-column form of the range sum;
+    smaller elements keep the lists): the column-range sum and the int32-list sum
+    (GRIDAP_HIP_COMPACT_LISTS=0) add the same element entries in the same order: nzval
+    and b bit-identical, FP32 and FP64."""
+    announce("hexgen column ranges vs list sum %s" % case)
+    part, order, el, tags, mapped = {
+        "mapped_vector_q2": ((4, 3, 3), 2, True, ("tag_21",), True),
+        "mapped_vector_q2_wide": ((9, 7, 6), 2, True, ("tag_21", "tag_26"), True)}[case]
+    D = len(part)
+    C = int(np.prod(part))
+    coef = np.exp(np.random.default_rng(C).normal(size=C))
+    src = (0.1, -1.0, 0.3)[:D] if el else np.random.default_rng(1).normal(size=C)
+    g = (lambda x: np.stack([0.01 * x[..., -1]] * D, axis=-1)) if el else \
+        (lambda x: 1.0 + x[..., 0] * x[..., 1])
+    for precision in ("fp64", "fp32"):
+        out = {}
+        for env in ("1", "0"):
+            monkeypatch.setenv("GRIDAP_HIP_COMPACT_LISTS", env)
+            op, V, _ = product(part, order, el, tags, g, coef, src, mapped=mapped,
+                               precision=precision)
+            info = op.assem.ctx.pattern_info()
+            assert info["compact"] == (env == "1") and info["inverse"] == (env == "1"), info
+            assert info["last_kernel"] == (_lib.GH_KERNEL_COLUMN_SUM if env == "1"
+                                           else _lib.GH_KERNEL_LIST_SUM), info
+            out[env] = (op.get_matrix().nzval, op.get_vector())
+            op.assem.ctx.close()
+        np.testing.assert_array_equal(out["1"][0], out["0"][0])
+        np.testing.assert_array_equal(out["1"][1], out["0"][1])
