@@ -379,9 +379,12 @@ struct QElemArgs {
   int nc[3], nn[3];
   int ncells;
   int nq;
-  const double *xyz;
+  const double *xyz;  // node coordinates, or nullptr: the Cartesian cells of sizes h
+  double h[3];
   const double *tab;  // per q: w, shp[NV], grad[NV*D]
   const double *kappa, *fsrc;
+  const double *sigma;  // mass coefficient per cell, or nullptr
+  const double *fq;     // source per cell and point (replaces fsrc), or nullptr
 };
 
 // The element matrix and load of every cell, with the arithmetic of cell_q1_kernel
@@ -414,7 +417,8 @@ __global__ __launch_bounds__(256) void q1_lattice_elem_kernel(const QElemArgs A,
       stride *= A.nn[k];
     }
 #pragma unroll
-    for (int k = 0; k < D; ++k) X[a][k] = A.xyz[(int64_t)node * D + k];
+    for (int k = 0; k < D; ++k)
+      X[a][k] = A.xyz ? A.xyz[(int64_t)node * D + k] : (double)(ci[k] + ((a >> k) & 1)) * A.h[k];
   }
   const double kap = A.kappa ? A.kappa[cell] : 1.0;
   const double fc = A.fsrc ? A.fsrc[cell] : 1.0;
@@ -449,6 +453,8 @@ __global__ __launch_bounds__(256) void q1_lattice_elem_kernel(const QElemArgs A,
         g[a][r] = (T)s;
       }
     const T kd = (T)kap * dV;
+    const T sd = A.sigma ? (T)A.sigma[cell] * dV : (T)0;
+    const T fd = (T)(A.fq ? A.fq[(int64_t)cell * A.nq + q] : fc) * dV;
     int t = 0;
 #pragma unroll
     for (int i = 0; i < NL; ++i) {
@@ -458,8 +464,9 @@ __global__ __launch_bounds__(256) void q1_lattice_elem_kernel(const QElemArgs A,
 #pragma unroll
         for (int r = 0; r < D; ++r) dot += g[i][r] * g[j][r];
         K[t] += dot * kd;
+        if (A.sigma) K[t] += (T)tq[1 + i] * (T)tq[1 + j] * sd;
       }
-      F[i] += (T)tq[1 + i] * ((T)fc * dV);
+      F[i] += (T)tq[1 + i] * fd;
     }
   }
 #pragma unroll
@@ -473,7 +480,7 @@ void launch_lattice_mapped(Ctx &c, int precision) {
   c.last_kernel = GH_KERNEL_Q1_LATTICE;
   constexpr int NL = 1 << D;
   constexpr int NT = NL * (NL + 1) / 2;
-  GH_REQUIRE(c.nv == NL && c.xyz.n == (size_t)c.nnodes * D, GH_ERR_STATE,
+  GH_REQUIRE(c.cartesian || (c.nv == NL && c.xyz.n == (size_t)c.nnodes * D), GH_ERR_STATE,
              "the mapped lattice needs the node coordinates");
   const bool f32 = precision == GH_PREC_FP32_ELEM;
   const size_t esz = f32 ? sizeof(float) : sizeof(double);
@@ -495,10 +502,13 @@ void launch_lattice_mapped(Ctx &c, int precision) {
   }
   A.ncells = (int)c.ncells;
   A.nq = c.nq;
-  A.xyz = c.xyz.get();
+  A.xyz = c.cartesian ? nullptr : c.xyz.get();
+  for (int k = 0; k < 3; ++k) A.h[k] = c.sizes[k];
   A.tab = c.nl_tab.get();
   A.kappa = c.have_kappa ? c.kappa.get() : nullptr;
   A.fsrc = c.have_fsrc ? c.fsrc.get() : nullptr;
+  A.sigma = c.have_sigma ? c.sigma.get() : nullptr;
+  A.fq = c.have_fq ? c.fq.get() : nullptr;
   const unsigned eg = (unsigned)((c.ncells + 255) / 256);
   if (f32)
     hipLaunchKernelGGL((q1_lattice_elem_kernel<D, true>), dim3(eg), dim3(256), 0, c.stream, A,
@@ -753,11 +763,12 @@ void assemble_diffusion(Ctx &c, int precision, int path) {
   GH_REQUIRE(!c.have_fq || c.fq.n == (size_t)c.ncells * c.nq, GH_ERR_STATE,
              "the quadrature-point source was uploaded for another mesh or quadrature");
   double Kc[8] = {0}, b0 = 0.0;
-  // the lattice kernel takes f constant per cell and no mass term; f at the quadrature
-  // points or a mass term runs the per-cell kernel
-  bool lattice = c.structured && path != GH_PATH_GENERAL && !c.have_fq && !c.have_sigma;
-  if (lattice && !c.cartesian) {
-    // mapped cells of a lattice: element matrices per cell, owner-computes columns
+  // the lattice kernel takes Cartesian cells with f constant per cell and no mass term;
+  // mapped cells, f at the quadrature points or a mass term store the element matrices
+  // per cell first (the same owner-computes column kernel then sums them)
+  bool lattice = c.structured && path != GH_PATH_GENERAL;
+  if (lattice && (!c.cartesian || c.have_fq || c.have_sigma)) {
+    // element matrices per cell, owner-computes columns
     GH_REQUIRE((int)c.shp.size() == c.nq * c.nloc && (int)c.grad.size() == c.nq * c.nloc * c.dim,
                GH_ERR_STATE, "tabulation does not match the element (dofs per cell)");
     if (c.dim == 3)

@@ -165,8 +165,9 @@ def test_multifield_blocks_equal_monolithic(n, trial_style, test_style):
 
 
 def test_mass_term_3d_matches_oracle(oracle):
-    """∫ κ ∇v⋅∇u + σ u v with per-cell κ and σ on a 3-D Cartesian mesh (the per-cell
-    kernel), FP64 against the oracle; FP32 element math within 1e-6."""
+    """∫ κ ∇v⋅∇u + σ u v with per-cell κ and σ on a 3-D Cartesian mesh, on the lattice
+    route (element matrices per cell, owner-computes columns) and on the per-cell
+    kernel: FP64 against the oracle; FP32 element math within 1e-6."""
     part = (7, 6, 5)
     announce("mass + stiffness %s" % (part,))
     C = int(np.prod(part))
@@ -186,10 +187,15 @@ def test_mass_term_3d_matches_oracle(oracle):
     k, s = G.CellField(kappa, Ω), G.CellField(sigma, Ω)
     a = lambda u, v: G.integrate(k * G.inner(G.gradient(v), G.gradient(u)) + s * (u * v), dΩ)
     l = lambda v: G.integrate(1.0 * v, dΩ)
-    for prec, tol in (("fp64", 1e-12), ("fp32", 1e-6)):
-        op = G.AffineFEOperator(a, l, U, V, G.SparseMatrixAssembler(U, V, precision=prec))
+    for prec, tol, path, kern in (("fp64", 1e-12, "auto", _lib.GH_KERNEL_Q1_LATTICE),
+                                  ("fp32", 1e-6, "auto", _lib.GH_KERNEL_Q1_LATTICE),
+                                  ("fp64", 1e-12, "general", _lib.GH_KERNEL_CELL)):
+        # auto: element matrices per cell, then the owner-computes lattice kernel;
+        # general: the per-cell kernel with coloured scatter
+        op = G.AffineFEOperator(a, l, U, V, G.SparseMatrixAssembler(U, V, precision=prec,
+                                                                    path=path))
         A = op.get_matrix()
-        assert op.assem.ctx.pattern_info()["last_kernel"] == _lib.GH_KERNEL_CELL
+        assert op.assem.ctx.pattern_info()["last_kernel"] == kern
         np.testing.assert_array_equal(A.colptr, cp)
         np.testing.assert_array_equal(A.rowval, rv)
         assert relmax(A.nzval, nz) <= tol
