@@ -128,6 +128,11 @@ def main():
     ap.add_argument("--no-elasticity", action="store_true",
                     help="skip the vector-Q2 elasticity leg (48^3 hexes)")
     ap.add_argument("--elast-n", type=int, default=48)
+    ap.add_argument("--no-plaplacian", action="store_true",
+                    help="skip the p-Laplacian residual + Jacobian leg")
+    ap.add_argument("--no-boundary", action="store_true", help="skip the boundary-terms leg")
+    ap.add_argument("--no-multifield", action="store_true", help="skip the two-field leg")
+    ap.add_argument("--mf-n", type=int, default=48)
     ap.add_argument("--seed", type=int, default=2024)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -297,6 +302,12 @@ def main():
         out["ncube_general"] = ncube_general_leg(ev)
     if not args.no_elasticity:
         out["q2_elasticity_assembly"] = elasticity_leg(args.elast_n, ev)
+    if not args.no_plaplacian:
+        out["plaplacian_assembly"] = plaplacian_leg(args.n, ev)
+    if not args.no_boundary:
+        out["boundary_terms_assembly"] = boundary_leg(args.n, ev)
+    if not args.no_multifield:
+        out["multifield_assembly"] = multifield_leg(args.mf_n)
     print(json.dumps(out), flush=True)
 
 
@@ -691,6 +702,222 @@ def p2_tet_leg(n, ev, steps=10, warmup=2, seed=7):
         "element_tflops_dense_count": flops / (ms * 1e-3) / 1e12,
         "symbolic_s": symbolic_s, "host_numbering_s": host_s, "finite": ok,
     }
+
+
+def timed_ms(ev, stream, fn, steps, warmup=1):
+    """Mean duration of fn() over steps calls, HIP events on stream around them (fn's
+    work is enqueued on that stream)."""
+    for _ in range(warmup):
+        fn()
+    ev.sync(stream)
+    evs = ev.create(2)
+    ev.record(evs[0], stream)
+    for _ in range(steps):
+        fn()
+    ev.record(evs[1], stream)
+    ms = ev.elapsed_ms(evs[0], evs[1]) / steps
+    ev.destroy(evs)
+    return ms
+
+
+def plaplacian_leg(n, ev, p=3.0, steps=5, seed=31):
+    """Residual and Jacobian of the p-Laplacian (PLaplacianTests.jl's form, p = 3, κ and f
+    per cell) at a random FE function on the n^3 Q1 hexahedral mesh, Dirichlet boundary:
+    one Newton step's assembly.  Lattice route (element Jacobians and residuals per cell,
+    owner-computes columns) and, for comparison, the coloured per-cell scatter; the CPU
+    oracle (numpy) on a 48^3 sample."""
+    import gridap_hip as G
+    from gridap_hip import _lib
+    model = G.CartesianDiscreteModel((0, 1, 0, 1, 0, 1), (n, n, n))
+    reffe = G.ReferenceFE(G.lagrangian, G.Float64, 1)
+    V = G.TestFESpace(model, reffe, dirichlet_tags="boundary")
+    C_ = model.num_cells()
+    N = V.num_free_dofs()
+    rng = np.random.default_rng(seed)
+    kappa = np.exp(rng.standard_normal(C_))
+    fsrc = rng.uniform(-1, 1, C_)
+    u = rng.standard_normal(N)
+    q = G.Quadrature(3, 2)
+    shp, grad = reffe.tabulate(3, q.coords)
+    cd = V.get_cell_dof_ids()
+    ptrs = (np.arange(C_ + 1, dtype=np.int64) * cd.shape[1] + 1).astype(np.int32)
+    ctx = _lib.Context(0)
+    ctx.mesh_cartesian(model.origin, model.sizes, model.partition)
+    ctx.space_upload(_lib.GH_REFFE_QK, 1, 1, cd.ravel(), ptrs, N, V.num_dirichlet_dofs())
+    ctx.reffe_tabulation(q.weights, shp, grad)
+    nnz = ctx.symbolic()
+    ctx.coef_upload(kappa, fsrc, None)
+    ctx.fefunction_upload(u, np.zeros(V.num_dirichlet_dofs()))
+    ctx.plaplacian_params(p)
+    stream = ctx.stream()
+    # nzval and the residual written; κ, f, the cell dof ids and uh read
+    algo = 8 * nnz + 8 * N + 16 * C_ + 32 * C_ + 8 * N
+    out = {"workload": "p-Laplacian (p=%g) residual + Jacobian at a random FE function, "
+                       "%d^3 HEX Q1, degree 2, κ=exp(N(0,1)), f~U(-1,1) per cell, "
+                       "Dirichlet boundary" % (p, n),
+           "cells": C_, "free_dofs": N, "nnz": int(nnz), "algorithmic_bytes_per_launch": algo}
+    ref = None
+    for name, prec in (("fp64", _lib.GH_PREC_FP64), ("fp32", _lib.GH_PREC_FP32_ELEM)):
+        run = lambda: ctx.assemble(_lib.GH_FORM_PLAPLACIAN, prec, _lib.GH_PATH_AUTO)
+        ms = timed_ms(ev, stream, run, steps)
+        kern = ctx.pattern_info()["last_kernel"]
+        nz, b = ctx.values_get()
+        assert np.all(np.isfinite(nz)) and np.all(np.isfinite(b))
+        cell = timed_ms(ev, stream, lambda: ctx.assemble(_lib.GH_FORM_PLAPLACIAN, prec,
+                                                         _lib.GH_PATH_GENERAL), steps)
+        nzc, bc = ctx.values_get()
+        r = {"ms_per_assembly": ms, "value": nnz / (ms * 1e-3), "unit": "NNZ/s",
+             "kernel": kern, "cell_kernel_ms": cell,
+             "rel_max_vs_cell_kernel": float(np.abs(nz - nzc).max() / np.abs(nzc).max()),
+             "roofline": {"bound": "hbm", "achieved": algo / (ms * 1e-3) / 1e9,
+                          "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                          "frac": algo / (ms * 1e-3) / HBM_PEAK}}
+        if ref is None:
+            ref = nz
+        else:
+            r["rel_frobenius_vs_fp64"] = float(np.linalg.norm(nz - ref) / np.linalg.norm(ref))
+        out[name] = r
+    ctx.close()
+    # CPU oracle on a bounded sample
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    ns = 48
+    ms_ = G.CartesianDiscreteModel((0, 1, 0, 1, 0, 1), (ns, ns, ns))
+    Vs = G.TestFESpace(ms_, reffe, dirichlet_tags="boundary")
+    cds = Vs.get_cell_dof_ids()
+    us = rng.standard_normal(Vs.num_free_dofs())
+    t = time.time()
+    uc = O.cell_values(cds, us, np.zeros(Vs.num_dirichlet_dofs()))
+    K, rr = O.plaplacian_cells(3, ms_.cell_node_ids(), ms_.node_coordinates(), q.weights, shp,
+                               grad, uc, p, kappa[:ms_.num_cells()], fsrc[:ms_.num_cells()],
+                               cartesian_sizes=ms_.sizes)
+    _, _, nzs, _ = O.plaplacian_system(cds, Vs.num_free_dofs(), K, rr)
+    cpu_s = time.time() - t
+    out["cpu_baseline"] = {"value": len(nzs) / cpu_s, "unit": "NNZ/s", "cores": 1,
+                           "kind": "port",
+                           "sample": "%d^3 cells (%d nonzeros): the oracle's numpy "
+                                     "restatement of residual!/jacobian! + assembly, %.2f s"
+                                     % (ns, len(nzs), cpu_s)}
+    return out
+
+
+def _boundary_cells(n):
+    """Cells of the n^3 lattice with a face on the boundary (x fastest)."""
+    i = np.arange(n)
+    x, y, z = np.meshgrid(i, i, i, indexing="ij")
+    on = (x == 0) | (x == n - 1) | (y == 0) | (y == n - 1) | (z == 0) | (z == n - 1)
+    return (x + n * (y + n * z))[on]
+
+
+def boundary_leg(n, ev, steps=5):
+    """Cell terms plus boundary terms in one assembly: ∫κ∇u⋅∇v + ∫_Γ α u v = ∫ f v +
+    ∫_Γ g v on the n^3 Q1 hexahedral mesh with a Robin condition (α = 2) and a Neumann
+    load g(x) on all six faces (no Dirichlet dofs), the same system a
+    BoundaryTriangulation term list assembles (PoissonTests.jl:101-108).  The face kernel
+    time is the difference with the cell terms alone on the same pattern."""
+    import gridap_hip as G
+    model = G.CartesianDiscreteModel((0, 1, 0, 1, 0, 1), (n, n, n))
+    labels = model.get_face_labeling()
+    Ω = G.Triangulation(model)
+    Γ = G.BoundaryTriangulation(model, labels, tags="boundary")
+    dΩ, dΓ = G.Measure(Ω, 2), G.Measure(Γ, 2)
+    V = G.TestFESpace(model, G.ReferenceFE(G.lagrangian, G.Float64, 1))
+    U = G.TrialFESpace(V)
+    kappa = np.exp(np.random.default_rng(5).standard_normal(model.num_cells()))
+    k = G.CellField(kappa, Ω)
+    g = lambda x: np.sin(3 * x[..., 0]) + x[..., 1] * x[..., 2]
+    a = lambda u, v: (G.integrate(k * G.inner(G.gradient(v), G.gradient(u)), dΩ) +
+                      G.integrate(2.0 * v * u, dΓ))
+    l = lambda v: G.integrate(1.0 * v, dΩ) + G.integrate(v * g, dΓ)
+    t = time.time()
+    op = G.AffineFEOperator(a, l, U, V)
+    first_s = time.time() - t
+    assem = op.assem
+    ctx = assem.ctx
+    stream = ctx.stream()
+    nnz = int(assem.colptr[-1] - 1)
+    nf = 6 * n * n
+    from gridap_hip import _lib
+    run = lambda: ctx.assemble(_lib.GH_FORM_DIFFUSION, assem.precision, assem.path)
+    ms = timed_ms(ev, stream, run, steps)
+    kern = ctx.pattern_info()["last_kernel"]
+    ctx.boundary_upload(np.zeros(0, np.int32), np.zeros(0, np.int32))
+    ms_cells = timed_ms(ev, stream, run, steps)
+    ctx.close()
+    # what the face terms move: per face its cell, side, α and 4 g values (48 B), its
+    # 8 loads written and read back (128 B), its matrix triangle written (288 B) and its
+    # 64 contributions gathered with their 4-B indices (768 B): ~1 KB with the loads;
+    # each distinct nonzero the faces touch read and written once (16 B), and each
+    # touched b entry (16 B)
+    cd = V.get_cell_dof_ids()
+    fd = cd[_boundary_cells(n)].astype(np.int64)
+    pairs = np.unique((fd[:, :, None] * (1 << 32) + fd[:, None, :]).ravel())
+    ndst = len(pairs)
+    nown = len(np.unique(fd))
+    face_bytes = nf * (48 + 128 + 288 + 768) + ndst * 16 + nown * 16
+    dms = max(ms - ms_cells, 1e-6)
+    return {"workload": "Q1 Poisson %d^3 HEX with Robin (α=2) + Neumann g(x) terms on all "
+                        "%d boundary faces, κ=exp(N(0,1)); lattice cell kernel, then the face kernel and the per-nonzero face sum" % (n, nf),
+            "nnz": nnz, "boundary_faces": nf, "face_nonzeros": ndst, "first_assembly_s": first_s,
+            "ms_per_assembly": ms, "value": nnz / (ms * 1e-3), "unit": "NNZ/s",
+            "kernel": kern, "cells_only_ms": ms_cells, "face_terms_ms": ms - ms_cells,
+            "face_roofline": {"bound": "hbm", "achieved": face_bytes / (dms * 1e-3) / 1e9,
+                              "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                              "frac": face_bytes / (dms * 1e-3) / HBM_PEAK,
+                              "algorithmic_bytes_per_launch": face_bytes}}
+
+
+def multifield_leg(n, reps=3):
+    """Two-field Q1 system (BlockSparseMatrixAssemblersTests.jl's biform2/liform2:
+    ∫∇u1⋅∇v1 + u2 v2 - u1 v2, ∫v1 - v2) on the n^3 hexahedral mesh, Dirichlet boundary,
+    consecutive layout: assemble_matrix_and_vector end to end (host lowering, the three
+    touched pairs on the device, their host copies and the consecutive composition),
+    wall clock; and the composition alone."""
+    import gridap_hip as G
+    from gridap_hip import multifield as MFm
+    model = G.CartesianDiscreteModel((0, 1, 0, 1, 0, 1), (n, n, n))
+    reffe = G.ReferenceFE(G.lagrangian, G.Float64, 1)
+    V = G.TestFESpace(model, reffe, dirichlet_tags="boundary")
+    U = G.TrialFESpace(V, lambda x: x[..., 0] + x[..., 1])
+    dΩ = G.Measure(G.Triangulation(model), 2)
+    X, Y = G.MultiFieldFESpace([U, U]), G.MultiFieldFESpace([V, V])
+    a = lambda u, v: G.integrate(G.inner(G.gradient(u[0]), G.gradient(v[0])) +
+                                 G.inner(u[1], v[1]) - G.inner(u[0], v[1]), dΩ)
+    l = lambda v: G.integrate(v[0] * 1.0 - v[1] * 1.0, dΩ)
+    assem = G.SparseMatrixAssembler(X, Y)
+    t = time.time()
+    A, b = G.assemble_matrix_and_vector(a, l, X, Y, assem)
+    first_s = time.time() - t
+    # time the host composition inside the same calls
+    compose, spent = MFm.compose_blocks, []
+
+    def timed_compose(*args, **kw):
+        t0 = time.time()
+        r = compose(*args, **kw)
+        spent.append(time.time() - t0)
+        return r
+
+    MFm.compose_blocks = timed_compose
+    walls, comps = [], []
+    try:
+        for _ in range(reps):
+            del spent[:]
+            t = time.time()
+            A, b = G.assemble_matrix_and_vector(a, l, X, Y, assem)
+            walls.append(time.time() - t)
+            comps.append(sum(spent))
+    finally:
+        MFm.compose_blocks = compose
+    assert np.all(np.isfinite(A.nzval)) and np.all(np.isfinite(b))
+    k = int(np.argmin(walls))
+    nnz = int(A.colptr[-1] - 1)
+    return {"workload": "two-field Q1 system (∇u1⋅∇v1 + u2 v2 - u1 v2) on %d^3 HEX, "
+                        "Dirichlet boundary, consecutive layout; assemble_matrix_and_vector "
+                        "end to end (host lowering, 3 device pair assemblies, host copies, "
+                        "host composition), wall clock" % n,
+            "free_dofs": 2 * V.num_free_dofs(), "nnz": nnz, "first_assembly_s": first_s,
+            "wall_s_per_assembly": walls[k], "host_compose_s": comps[k],
+            "value": nnz / walls[k], "unit": "NNZ/s"}
 
 
 def fsai_leg(k):

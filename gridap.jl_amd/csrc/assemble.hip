@@ -55,6 +55,8 @@ struct LatticeArgs {
   // mapped lattice: per cell, the upper triangle of the element matrix (row-major)
   // then the element load, entry-major ([k][cell]), FP32 or FP64
   const void *elem;
+  // element loads in FP64 ([a][cell]) in place of elem's load slots, or nullptr
+  const double *eload;
   int ncells;
 };
 
@@ -200,7 +202,7 @@ __global__ __launch_bounds__(64 * kLatticeWaves) void lattice_q1_kernel(const La
       const int cid = cur.cid[e];
       if (cid < 0) continue;
       const int a = (~e) & (NV - 1);  // the node's local index in cell e
-      bsum += (double)E[(int64_t)(NT + a) * nce + cid];
+      bsum += P.eload ? P.eload[(int64_t)a * nce + cid] : (double)E[(int64_t)(NT + a) * nce + cid];
 #pragma unroll
       for (int o = 0; o < NB; ++o) {
         int oo = o, bl = 0;
@@ -475,6 +477,40 @@ __global__ __launch_bounds__(256) void q1_lattice_elem_kernel(const QElemArgs A,
   for (int a = 0; a < NL; ++a) E[(int64_t)(NT + a) * A.ncells + cell] = F[a];
 }
 
+// The owner-computes column kernel over the element matrices stored in c.qelem
+// (FP32 when f32, else FP64); eload: FP64 element loads in place of qelem's load
+// slots, or nullptr; lift: subtract the Dirichlet lifting from b.
+template <int D>
+void lattice_columns(Ctx &c, bool f32, const double *eload, bool lift) {
+  LatticeArgs P{};
+  for (int k = 0; k < 3; ++k) {
+    P.nn[k] = (int)(c.part[k] + 1);
+    P.nc[k] = (int)c.part[k];
+  }
+  P.nnodes = (int)c.nnodes;
+  P.inv_nn0 = 1.0 / P.nn[0];
+  P.inv_nn1 = 1.0 / P.nn[1];
+  P.node_dof = c.node_dof.get();
+  P.fmask = c.node_fmask.get();
+  P.tile_p0 = c.tile_p0.get();
+  P.tile_d0 = c.tile_d0.get();
+  P.colptr = c.colptr.get();
+  P.dvals = lift && c.have_dvals ? c.dvals.get() : nullptr;
+  P.nzval = c.nzval.get();
+  P.b = c.b.get();
+  P.elem = c.qelem.get();
+  P.eload = eload;
+  P.ncells = (int)c.ncells;
+  const int ntiles = (int)((c.nnodes + kTileNodes - 1) / kTileNodes);
+  const unsigned grid = (unsigned)((ntiles + kLatticeWaves - 1) / kLatticeWaves);
+  const dim3 block(kTileNodes * kLatticeWaves);
+  if (f32)
+    hipLaunchKernelGGL((lattice_q1_kernel<D, 4>), dim3(grid), block, 0, c.stream, P, ntiles);
+  else
+    hipLaunchKernelGGL((lattice_q1_kernel<D, 8>), dim3(grid), block, 0, c.stream, P, ntiles);
+  GH_CHECK_HIP(hipGetLastError());
+}
+
 template <int D>
 void launch_lattice_mapped(Ctx &c, int precision) {
   c.last_kernel = GH_KERNEL_Q1_LATTICE;
@@ -517,33 +553,7 @@ void launch_lattice_mapped(Ctx &c, int precision) {
     hipLaunchKernelGGL((q1_lattice_elem_kernel<D, false>), dim3(eg), dim3(256), 0, c.stream, A,
                        (double *)c.qelem.get());
   GH_CHECK_HIP(hipGetLastError());
-  LatticeArgs P{};
-  for (int k = 0; k < 3; ++k) {
-    P.nn[k] = (int)(c.part[k] + 1);
-    P.nc[k] = (int)c.part[k];
-  }
-  P.nnodes = (int)c.nnodes;
-  P.inv_nn0 = 1.0 / P.nn[0];
-  P.inv_nn1 = 1.0 / P.nn[1];
-  P.node_dof = c.node_dof.get();
-  P.fmask = c.node_fmask.get();
-  P.tile_p0 = c.tile_p0.get();
-  P.tile_d0 = c.tile_d0.get();
-  P.colptr = c.colptr.get();
-  P.dvals = c.have_dvals ? c.dvals.get() : nullptr;
-  P.nzval = c.nzval.get();
-  P.b = c.b.get();
-  P.elem = c.qelem.get();
-  P.ncells = (int)c.ncells;
-  const int ntiles = (int)((c.nnodes + kTileNodes - 1) / kTileNodes);
-  const unsigned grid = (unsigned)((ntiles + kLatticeWaves - 1) / kLatticeWaves);
-  if (f32)
-    hipLaunchKernelGGL((lattice_q1_kernel<D, 4>), dim3(grid), dim3(kTileNodes * kLatticeWaves),
-                       0, c.stream, P, ntiles);
-  else
-    hipLaunchKernelGGL((lattice_q1_kernel<D, 8>), dim3(grid), dim3(kTileNodes * kLatticeWaves),
-                       0, c.stream, P, ntiles);
-  GH_CHECK_HIP(hipGetLastError());
+  lattice_columns<D>(c, f32, nullptr, true);
   // the packed tabulation's host copy lives until here
   GH_CHECK_HIP(hipStreamSynchronize(c.stream));
 }
@@ -745,6 +755,13 @@ void launch_general(Ctx &c, int precision) {
 }
 
 }  // namespace
+
+void launch_lattice_columns(Ctx &c, bool f32, const double *eload, bool lift) {
+  if (c.dim == 3)
+    lattice_columns<3>(c, f32, eload, lift);
+  else
+    lattice_columns<2>(c, f32, eload, lift);
+}
 
 void assemble_diffusion(Ctx &c, int precision, int path) {
   GH_REQUIRE(!c.have_kappa || c.kappa.n == (size_t)c.ncells, GH_ERR_STATE,

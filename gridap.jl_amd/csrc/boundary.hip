@@ -13,19 +13,33 @@
 // penalty and consistency data, evaluated by the host), then the Dirichlet lifting
 // b_a -= Σ_{b Dirichlet} A_ab u_b of the face matrix (src/CellData/AttachDirichlet.jl:
 // 59-84).  On a Cartesian mesh ∇φ = ∇̂φ / h per axis, |J_F| is the product of the cell
-// sizes along the facet and n = ±e_axis.  One thread per face; faces are launched colour
-// by colour (no two faces of one colour share a free dof) after the cell kernels, so
-// the additions are race-free and deterministic.
+// sizes along the facet and n = ±e_axis.
+//
+// Two kernels after the cell kernels, on lists built once per face upload and pattern:
+//  * face kernel, one thread per face: the face matrix (upper triangle; K_ab and K_ba are
+//    the same products summed in the same order) and the lifted face load, stored
+//    entry-major ([k][face], coalesced stores);
+//  * sum kernel, one thread per destination: a nonzero (row i, column j) of two free dofs
+//    of a face's cell sums its face contributions (faces ascending) and adds the sum
+//    into nzval once; one thread per free dof of a boundary cell adds its face loads
+//    into b.  No two threads write one value, so the additions are race-free and
+//    deterministic, and the assembled matrix keeps the symmetry of the cell terms
+//    bitwise.
 #include <algorithm>
+#include <climits>
 #include <cmath>
+#include <vector>
 
 #include "gh_internal.h"
 
 namespace gh {
 namespace {
 
+// bdst_pos value of a destination with no entry in the pattern (not met when the pattern
+// is the cell pattern of the same space)
+constexpr int32_t kNoEntry = INT32_MIN;
+
 struct FaceArgs {
-  const int32_t *faces;      // face indices of this colour
   int64_t nfaces;
   const int32_t *face_cell;  // 0-based cell of each face
   const int32_t *face_side;  // 0-based n-cube facet of each face
@@ -35,16 +49,15 @@ struct FaceArgs {
   const double *alpha, *beta, *gq, *hq;  // per face / per face and point (or nullptr)
   const double *dvals;
   double h[3];
-  const int32_t *colptr, *rowval;
-  double *nzval, *b;
+  double *K, *r;             // out: [t][face] upper triangle, [a][face] load
 };
 
-// NL: the cell's scalar shape functions (4, 8: Q1; 9: Q2 quadrilateral)
+// The matrix and load of every boundary face.
 template <int D, int NL>
 __global__ __launch_bounds__(128) void boundary_face_kernel(const FaceArgs A) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= A.nfaces) return;
-  const int64_t f = A.faces[t];
+  constexpr int NT = NL * (NL + 1) / 2;
+  const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= A.nfaces) return;
   const int64_t cell = A.face_cell[f];
   const int side = A.face_side[f];
   // n-cube facet order (src/ReferenceFEs/ExtrusionPolytopes.jl): facets come in pairs
@@ -58,16 +71,11 @@ __global__ __launch_bounds__(128) void boundary_face_kernel(const FaceArgs A) {
   const double inv_h = 1.0 / A.h[axis];
   const double al = A.alpha ? A.alpha[f] : 0.0;
   const double be = A.beta ? A.beta[f] : 0.0;
-  int32_t dofs[NL];
+  double K[NT], r[NL];
 #pragma unroll
-  for (int a = 0; a < NL; ++a) dofs[a] = A.cell_dofs[cell * NL + a];
-  double K[NL][NL], r[NL];
+  for (int t = 0; t < NT; ++t) K[t] = 0.0;
 #pragma unroll
-  for (int a = 0; a < NL; ++a) {
-    r[a] = 0.0;
-#pragma unroll
-    for (int bb = 0; bb < NL; ++bb) K[a][bb] = 0.0;
-  }
+  for (int a = 0; a < NL; ++a) r[a] = 0.0;
   const int stride = 1 + NL + NL * D;
   for (int q = 0; q < A.nqf; ++q) {
     const double *tq = A.tab + ((size_t)side * A.nqf + q) * stride;
@@ -81,95 +89,179 @@ __global__ __launch_bounds__(128) void boundary_face_kernel(const FaceArgs A) {
       dn[a] = sgn * tq[1 + NL + a * D + axis] * inv_h;
       r[a] += dS * (phi[a] * g + dn[a] * hh);
     }
+    int t = 0;
 #pragma unroll
     for (int a = 0; a < NL; ++a)
 #pragma unroll
-      for (int bb = 0; bb < NL; ++bb)
-        K[a][bb] += dS * (al * phi[a] * phi[bb] + be * (phi[a] * dn[bb] + dn[a] * phi[bb]));
+      for (int b = a; b < NL; ++b, ++t)
+        K[t] += dS * (al * (phi[a] * phi[b]) + be * (phi[a] * dn[b] + dn[a] * phi[b]));
   }
   if (A.dvals) {
+    // Dirichlet lifting r_a -= Σ_b K_ab u_b over the face's Dirichlet dofs
+    // (src/CellData/AttachDirichlet.jl:59-84)
+    double ud[NL];
+#pragma unroll
+    for (int b = 0; b < NL; ++b) {
+      const int32_t d = A.cell_dofs[cell * NL + b];
+      ud[b] = d < 0 ? A.dvals[-d - 1] : 0.0;
+    }
 #pragma unroll
     for (int a = 0; a < NL; ++a) {
       double s = 0.0;
 #pragma unroll
-      for (int bb = 0; bb < NL; ++bb)
-        if (dofs[bb] < 0) s += K[a][bb] * A.dvals[-dofs[bb] - 1];
+      for (int b = 0; b < NL; ++b) {
+        const int lo = a < b ? a : b, hi = a < b ? b : a;
+        s += K[lo * NL - lo * (lo - 1) / 2 + (hi - lo)] * ud[b];
+      }
       r[a] -= s;
     }
   }
 #pragma unroll
-  for (int lj = 0; lj < NL; ++lj) {
-    const int32_t j = dofs[lj];
-    if (j <= 0) continue;
-    const int32_t p0 = A.colptr[j - 1], p1 = A.colptr[j];
+  for (int t = 0; t < NT; ++t) A.K[t * A.nfaces + f] = K[t];
 #pragma unroll
-    for (int li = 0; li < NL; ++li) {
-      const int32_t i = dofs[li];
-      if (i <= 0) continue;
-      int lo = p0, hi = p1;
-      while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        if (A.rowval[mid] < i - 1) lo = mid + 1;
-        else hi = mid;
-      }
-      if (lo < p1 && A.rowval[lo] == i - 1) A.nzval[lo] += K[li][lj];
-    }
-  }
-#pragma unroll
-  for (int li = 0; li < NL; ++li)
-    if (dofs[li] > 0) A.b[dofs[li] - 1] += r[li];
+  for (int a = 0; a < NL; ++a) A.r[a * A.nfaces + f] = r[a];
 }
 
-}  // namespace
+struct SumArgs {
+  int64_t ndst, nown;
+  const int32_t *dst_ptr, *dst_ci, *dst_pos;  // contributions: indices into K
+  const int32_t *own_dof, *own_ptr, *own_fa;  // loads: indices into r
+  const double *K, *r;
+  double *nzval, *b;
+};
 
-// Greedy colouring of the boundary faces on shared free dofs (faces ascending inside a
-// colour); faces of one cell always differ in colour.
-void boundary_coloring(Ctx &c) {
+// Threads [0, ndst): one nonzero each; [ndst, ndst + nown): one load row each.
+__global__ __launch_bounds__(256) void boundary_sum_kernel(const SumArgs A) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < A.ndst) {
+    const int32_t pos = A.dst_pos[t];
+    double s = 0.0;
+    for (int32_t k = A.dst_ptr[t]; k < A.dst_ptr[t + 1]; ++k) s += A.K[A.dst_ci[k]];
+    if (pos >= 0) A.nzval[pos] += s;
+  } else if (t < A.ndst + A.nown) {
+    const int64_t o = t - A.ndst;
+    double s = 0.0;
+    for (int32_t k = A.own_ptr[o]; k < A.own_ptr[o + 1]; ++k) s += A.r[A.own_fa[k]];
+    A.b[A.own_dof[o] - 1] += s;
+  }
+}
+
+// The nzval position of each destination: a binary search of its row in its column
+// (nz_index, src/Algebra/SparseMatrixCSC.jl:14-22), once per pattern.
+__global__ __launch_bounds__(256) void boundary_positions_kernel(
+    int64_t n, const int32_t *row, const int32_t *col, const int32_t *colptr,
+    const int32_t *rowval, int32_t *pos) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int32_t i = row[t] - 1, j = col[t];
+  const int32_t p1 = colptr[j];
+  int lo = colptr[j - 1], hi = p1;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (rowval[mid] < i) lo = mid + 1;
+    else hi = mid;
+  }
+  pos[t] = lo < p1 && rowval[lo] == i ? lo : kNoEntry;
+}
+
+template <class T>
+void upload(DBuf<T> &d, const std::vector<T> &h) {
+  d.resize(std::max<size_t>(h.size(), 1));
+  if (!h.empty())
+    GH_CHECK_HIP(hipMemcpy(d.get(), h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice));
+}
+
+// Destination and owner lists of the uploaded faces (free dofs ascending; each
+// destination's rows ascending inside its column; contributions faces ascending).
+void boundary_lists(Ctx &c) {
   const int64_t F = c.nbfaces;
   const int nl = c.nloc;
+  const int nt = nl * (nl + 1) / 2;
+  GH_REQUIRE((int64_t)nt * F < INT32_MAX, GH_ERR_UNSUPPORTED,
+             "too many boundary faces for the 32-bit face lists");
   const std::vector<int32_t> &cd = c.h_cell_dofs;
-  std::vector<int32_t> color(F, -1);
-  // free dof -> faces already coloured that hold it
-  std::vector<std::vector<int32_t>> dof_faces;
-  std::vector<int64_t> stamp(8, -1);
-  int ncol = 0;
-  std::vector<int32_t> slot(c.nfree + 1, -1);  // free dof -> its list in dof_faces
+  std::vector<int32_t> cnt(c.nfree + 2, 0);
   for (int64_t f = 0; f < F; ++f) {
     const int64_t cell = c.h_face_cell[f];
     for (int a = 0; a < nl; ++a) {
       const int32_t d = cd[cell * nl + a];
-      if (d <= 0) continue;
-      if (slot[d] < 0) {
-        slot[d] = (int32_t)dof_faces.size();
-        dof_faces.emplace_back();
-      }
-      for (int32_t other : dof_faces[slot[d]]) {
-        const int col = color[other];
-        if ((size_t)col >= stamp.size()) stamp.resize(2 * col + 2, -1);
-        stamp[col] = f;
-      }
-    }
-    int chosen = 0;
-    while ((size_t)chosen < stamp.size() && stamp[chosen] == f) ++chosen;
-    if ((size_t)chosen >= stamp.size()) stamp.resize(2 * chosen + 2, -1);
-    color[f] = chosen;
-    ncol = std::max(ncol, chosen + 1);
-    for (int a = 0; a < nl; ++a) {
-      const int32_t d = cd[cell * nl + a];
-      if (d > 0) dof_faces[slot[d]].push_back((int32_t)f);
+      if (d > 0) cnt[d]++;
     }
   }
-  c.bface_color_ptr.assign(ncol + 1, 0);
-  for (int64_t f = 0; f < F; ++f) c.bface_color_ptr[color[f] + 1]++;
-  for (int k = 0; k < ncol; ++k) c.bface_color_ptr[k + 1] += c.bface_color_ptr[k];
-  std::vector<int32_t> order(std::max<int64_t>(F, 1));
-  std::vector<int64_t> cur(c.bface_color_ptr.begin(), c.bface_color_ptr.end() - 1);
-  for (int64_t f = 0; f < F; ++f) order[cur[color[f]]++] = (int32_t)f;
-  c.bface_order.resize(order.size());
-  GH_CHECK_HIP(hipMemcpy(c.bface_order.get(), order.data(), sizeof(int32_t) * order.size(),
-                         hipMemcpyHostToDevice));
-  c.bface_ncolors = ncol;
+  // owner entries (face, local index), faces ascending
+  std::vector<int32_t> own, optr(1, 0), slot(c.nfree + 1, -1);
+  for (int64_t d = 1; d <= c.nfree; ++d)
+    if (cnt[d] > 0) {
+      slot[d] = optr.back();
+      own.push_back((int32_t)d);
+      optr.push_back(optr.back() + cnt[d]);
+    }
+  std::vector<int32_t> ef(optr.back()), ea(optr.back());
+  for (int64_t f = 0; f < F; ++f) {
+    const int64_t cell = c.h_face_cell[f];
+    for (int a = 0; a < nl; ++a) {
+      const int32_t d = cd[cell * nl + a];
+      if (d > 0) {
+        ef[slot[d]] = (int32_t)f;
+        ea[slot[d]++] = a;
+      }
+    }
+  }
+  std::vector<int32_t> ofa(ef.size());
+  for (size_t k = 0; k < ef.size(); ++k) ofa[k] = (int32_t)(ea[k] * F + ef[k]);
+  // destinations of each owner column j: rows i of the faces' cells
+  std::vector<int32_t> dptr(1, 0), dci, drow, dcol;
+  std::vector<std::pair<int32_t, int32_t>> tmp;  // (row, contribution)
+  for (size_t o = 0; o < own.size(); ++o) {
+    tmp.clear();
+    for (int32_t k = optr[o]; k < optr[o + 1]; ++k) {
+      const int64_t f = ef[k];
+      const int a = ea[k];
+      const int64_t cell = c.h_face_cell[f];
+      for (int b = 0; b < nl; ++b) {
+        const int32_t i = cd[cell * nl + b];
+        if (i <= 0) continue;
+        const int lo = std::min(a, b), hi = std::max(a, b);
+        const int t = lo * nl - lo * (lo - 1) / 2 + (hi - lo);
+        tmp.emplace_back(i, (int32_t)(t * F + f));
+      }
+    }
+    // rows ascending; stable, so each row keeps its faces ascending
+    std::stable_sort(tmp.begin(), tmp.end(),
+                     [](const std::pair<int32_t, int32_t> &x,
+                        const std::pair<int32_t, int32_t> &y) { return x.first < y.first; });
+    for (size_t k = 0; k < tmp.size(); ++k) {
+      if (k == 0 || tmp[k].first != tmp[k - 1].first) {
+        if (k > 0) dptr.push_back((int32_t)dci.size());
+        drow.push_back(tmp[k].first);
+        dcol.push_back(own[o]);
+      }
+      dci.push_back(tmp[k].second);
+    }
+    if (!tmp.empty()) dptr.push_back((int32_t)dci.size());
+  }
+  upload(c.bdst_ptr, dptr);
+  upload(c.bdst_ci, dci);
+  upload(c.bdst_row, drow);
+  upload(c.bdst_col, dcol);
+  upload(c.bown_dof, own);
+  upload(c.bown_ptr, optr);
+  upload(c.bown_fa, ofa);
+  const int64_t nd = (int64_t)drow.size();
+  c.bdst_pos.resize(std::max<int64_t>(nd, 1));
+  if (nd > 0) {
+    hipLaunchKernelGGL(boundary_positions_kernel, dim3((unsigned)((nd + 255) / 256)), dim3(256),
+                       0, c.stream, nd, c.bdst_row.get(), c.bdst_col.get(), c.colptr.get(),
+                       c.rowval.get(), c.bdst_pos.get());
+    GH_CHECK_HIP(hipGetLastError());
+  }
+  c.bface_K.resize(std::max<int64_t>((int64_t)nt * F, 1));
+  c.bface_r.resize(std::max<int64_t>((int64_t)nl * F, 1));
+  c.bown_n = (int64_t)own.size();
+  c.bdst_n = nd;  // built: kept until the faces, the space or the pattern change
 }
+
+}  // namespace
 
 void assemble_boundary(Ctx &c) {
   if (c.nbfaces == 0) return;
@@ -180,8 +272,9 @@ void assemble_boundary(Ctx &c) {
              "boundary terms are provided for scalar Q1, and Q2 on quadrilaterals, on "
              "Cartesian meshes");
   GH_REQUIRE(c.bface_tab_ok, GH_ERR_STATE, "no face tabulation: call gh_face_tabulation");
-  if (c.bface_ncolors == 0) boundary_coloring(c);
+  if (c.bdst_n < 0) boundary_lists(c);
   FaceArgs A{};
+  A.nfaces = c.nbfaces;
   A.face_cell = c.bface_cell.get();
   A.face_side = c.bface_side.get();
   A.cell_dofs = c.cell_dofs.get();
@@ -193,21 +286,33 @@ void assemble_boundary(Ctx &c) {
   A.hq = c.have_bh ? c.bh.get() : nullptr;
   A.dvals = c.have_dvals ? c.dvals.get() : nullptr;
   for (int k = 0; k < 3; ++k) A.h[k] = c.sizes[k];
-  A.colptr = c.colptr.get();
-  A.rowval = c.rowval.get();
-  A.nzval = c.nzval.get();
-  A.b = c.b.get();
-  for (int col = 0; col < c.bface_ncolors; ++col) {
-    A.faces = c.bface_order.get() + c.bface_color_ptr[col];
-    A.nfaces = c.bface_color_ptr[col + 1] - c.bface_color_ptr[col];
-    if (A.nfaces == 0) continue;
-    const unsigned grid = (unsigned)((A.nfaces + 127) / 128);
-    if (q2quad)
-      hipLaunchKernelGGL((boundary_face_kernel<2, 9>), dim3(grid), dim3(128), 0, c.stream, A);
-    else if (c.dim == 2)
-      hipLaunchKernelGGL((boundary_face_kernel<2, 4>), dim3(grid), dim3(128), 0, c.stream, A);
-    else
-      hipLaunchKernelGGL((boundary_face_kernel<3, 8>), dim3(grid), dim3(128), 0, c.stream, A);
+  A.K = c.bface_K.get();
+  A.r = c.bface_r.get();
+  const unsigned fgrid = (unsigned)((c.nbfaces + 127) / 128);
+  if (q2quad)
+    hipLaunchKernelGGL((boundary_face_kernel<2, 9>), dim3(fgrid), dim3(128), 0, c.stream, A);
+  else if (c.dim == 2)
+    hipLaunchKernelGGL((boundary_face_kernel<2, 4>), dim3(fgrid), dim3(128), 0, c.stream, A);
+  else
+    hipLaunchKernelGGL((boundary_face_kernel<3, 8>), dim3(fgrid), dim3(128), 0, c.stream, A);
+  GH_CHECK_HIP(hipGetLastError());
+  SumArgs S{};
+  S.ndst = c.bdst_n;
+  S.nown = c.bown_n;
+  S.dst_ptr = c.bdst_ptr.get();
+  S.dst_ci = c.bdst_ci.get();
+  S.dst_pos = c.bdst_pos.get();
+  S.own_dof = c.bown_dof.get();
+  S.own_ptr = c.bown_ptr.get();
+  S.own_fa = c.bown_fa.get();
+  S.K = c.bface_K.get();
+  S.r = c.bface_r.get();
+  S.nzval = c.nzval.get();
+  S.b = c.b.get();
+  const int64_t nt = S.ndst + S.nown;
+  if (nt > 0) {
+    hipLaunchKernelGGL(boundary_sum_kernel, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0,
+                       c.stream, S);
     GH_CHECK_HIP(hipGetLastError());
   }
 }

@@ -42,9 +42,13 @@ void clear_boundary(gh::Ctx &c) {
   c.bg.release();
   c.bh.release();
   c.have_balpha = c.have_bbeta = c.have_bg = c.have_bh = false;
-  c.bface_ncolors = 0;
-  c.bface_color_ptr.clear();
-  c.bface_order.release();
+  c.bdst_n = -1;
+  c.bown_n = 0;
+  for (gh::DBuf<int32_t> *v : {&c.bdst_ptr, &c.bdst_ci, &c.bdst_row, &c.bdst_col, &c.bdst_pos,
+                               &c.bown_dof, &c.bown_ptr, &c.bown_fa})
+    v->release();
+  c.bface_K.release();
+  c.bface_r.release();
 }
 
 // An uploaded quadrilateral/hexahedral mesh whose cells are those of a Cartesian grid:
@@ -134,6 +138,7 @@ void invalidate_pattern(gh::Ctx &c) {
   c.noff.release();
   c.bbase.release();
   c.bstart.release();
+  c.bdst_n = -1;  // the face lists hold nzval positions of the old pattern
 }
 
 }  // namespace
@@ -509,9 +514,7 @@ int gh_assemble(gh_ctx *ctx, int form_kind, int precision, int path) {
     GH_REQUIRE(c.have_pattern, GH_ERR_STATE, "no pattern: call gh_symbolic first");
     GH_REQUIRE(c.nq > 0, GH_ERR_STATE, "no reference tabulation uploaded");
     if (form_kind == GH_FORM_PLAPLACIAN) {
-      GH_REQUIRE(path != GH_PATH_STRUCTURED, GH_ERR_UNSUPPORTED,
-                 "the lattice path assembles linear diffusion only");
-      gh::assemble_plaplacian(c, precision);
+      gh::assemble_plaplacian(c, precision, path);
     } else if (form_kind == GH_FORM_ELASTICITY) {
       if (gh::hexgen_eligible(c)) {
         GH_REQUIRE(path != GH_PATH_STRUCTURED, GH_ERR_UNSUPPORTED,

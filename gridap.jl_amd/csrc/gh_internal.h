@@ -239,16 +239,22 @@ struct Ctx {
 
   // ---- boundary faces (BoundaryTriangulation) and their terms ----
   int64_t nbfaces = 0;
-  std::vector<int32_t> h_face_cell; // 0-based cell of each face (host copy, colouring)
+  std::vector<int32_t> h_face_cell; // 0-based cell of each face (host copy, owner lists)
   DBuf<int32_t> bface_cell, bface_side;  // 0-based cell and n-cube facet of each face
   int nqf = 0;                      // quadrature points per face
   bool bface_tab_ok = false;        // face tabulation matches the element
   DBuf<double> bface_tab;           // per side, per point: w, shp[nloc], grad[nloc*dim]
   DBuf<double> balpha, bbeta, bg, bh;
   bool have_balpha = false, have_bbeta = false, have_bg = false, have_bh = false;
-  int bface_ncolors = 0;
-  std::vector<int64_t> bface_color_ptr;
-  DBuf<int32_t> bface_order;        // faces grouped by colour
+  // face assembly lists (boundary_lists; built once per face upload and pattern):
+  // destinations = (column j, row i) entries of free dofs shared by a face's cell, each
+  // with its face-matrix contributions; owners = free dofs of boundary cells, each with
+  // its face loads
+  int64_t bdst_n = -1;              // destinations (-1: lists not built)
+  int64_t bown_n = 0;               // owners
+  DBuf<int32_t> bdst_ptr, bdst_ci, bdst_row, bdst_col, bdst_pos;
+  DBuf<int32_t> bown_dof, bown_ptr, bown_fa;
+  DBuf<double> bface_K, bface_r;    // face matrices (upper triangle) and loads, [k][face]
 
   // ---- coefficients / values ----
   DBuf<double> kappa, fsrc, dvals;
@@ -315,7 +321,12 @@ void assemble_simplex(Ctx &c, int precision);
 // boundary.hip
 void assemble_boundary(Ctx &c);  // adds the boundary-face terms to nzval and b
 // nonlinear.hip
-void assemble_plaplacian(Ctx &c, int precision);  // residual -> b, Jacobian -> nzval
+void assemble_plaplacian(Ctx &c, int precision, int path);  // residual -> b, Jacobian -> nzval
+// Owner-computes column kernel of the Q1 lattice over the element matrices in c.qelem
+// (upper triangle, FP32 when f32, else FP64; loads in qelem's load slots, or FP64 in
+// eload); lift: subtract the Dirichlet lifting from b.  Enqueued on c.stream.
+void launch_lattice_columns(Ctx &c, bool f32, const double *eload, bool lift);
+
 // hexq2.hip
 bool hexq2_eligible(const Ctx &c);
 void assemble_elasticity(Ctx &c, int precision, int path);

@@ -13,76 +13,49 @@
 // with ∇u = Σ_b u_b ∇φ_b and u_b the cell's values of uh (free values, or Dirichlet
 // values at Dirichlet dofs).  Rows and columns of Dirichlet dofs are skipped (the
 // residual has no lifting: uh already carries its Dirichlet values).  For p = 2 this is
-// the linear diffusion operator, and J is symmetric for every p.
+// the linear diffusion operator, and J is symmetric for every p (bitwise: J_ab and J_ba
+// are the same products, summed in the same order).
 //
-// One thread per cell (Q1 on quads or hexahedra, Cartesian or isoparametric map), the
-// element Jacobian in FP64 (or FP32 element math), added into nzval by a binary search
-// of its column; cells are launched colour by colour (no two cells of one colour share
-// a free dof), so the scatter is race-free and deterministic.
+// Q1 on quads or hexahedra, Cartesian or isoparametric map, one thread per cell, the
+// element Jacobian in FP64 (or FP32 element math), the residual in FP64 always.  Two
+// ways into nzval:
+//  * lattice meshes (Cartesian topology, nodes numbered lexicographically): the
+//    element Jacobians and residuals are stored per cell and the owner-computes column
+//    kernel of the linear lattice path sums them (each free node writes its column
+//    once, coalesced; incident cells in ascending order);
+//  * other meshes: each thread adds its element into nzval by a binary search of the
+//    column; cells are launched colour by colour (no two cells of one colour share a
+//    free dof), so the scatter is race-free and deterministic.
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 
 #include "gh_internal.h"
 
 namespace gh {
 namespace {
 
-struct NlArgs {
-  const int32_t *cells;  // cells of this colour
-  int64_t ncells;
-  const int32_t *cell_dofs;
-  const int32_t *cell_nodes;  // nullptr: Cartesian analytic
-  const double *xyz;
-  double h[3];
-  int nq;
-  const double *tab;  // per q: w, shp[NLOC], grad[NLOC*D]
-  const double *kappa, *fsrc;
-  const double *ufree, *udiri;  // the FE function uh: free and Dirichlet values
-  double p;
-  const int32_t *colptr, *rowval;
-  double *nzval, *b;
-};
-
-template <int D, bool F32>
-__global__ __launch_bounds__(128) void plaplacian_q1_kernel(const NlArgs A) {
+// The upper triangle Kt (row-major: (i, j >= i)) of the element Jacobian and the
+// residual r of one Q1 cell with nodes X (mapped) or box sizes h (Cartesian), at the
+// cell values u of uh.  tab: per q, w, shp[NL], grad[NL*D].
+template <int D, typename T>
+__device__ __forceinline__ void plap_element(const double (&X)[1 << D][D], bool mapped,
+                                             const double *h, const double (&u)[1 << D],
+                                             double kap, double fc, double pm2,
+                                             const double *tab, int nq,
+                                             T (&Kt)[(1 << D) * ((1 << D) + 1) / 2],
+                                             double (&r)[1 << D]) {
   constexpr int NL = 1 << D;
-  using T = typename std::conditional<F32, float, double>::type;
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= A.ncells) return;
-  const int64_t cell = A.cells[t];
-  int32_t dofs[NL];
-  double u[NL];
+  constexpr int NT = NL * (NL + 1) / 2;
 #pragma unroll
-  for (int a = 0; a < NL; ++a) {
-    const int32_t d = A.cell_dofs[cell * NL + a];
-    dofs[a] = d;
-    u[a] = d > 0 ? A.ufree[d - 1] : (A.udiri ? A.udiri[-d - 1] : 0.0);
-  }
-  double X[NL][D];
-  if (A.cell_nodes) {
+  for (int t = 0; t < NT; ++t) Kt[t] = 0;
 #pragma unroll
-    for (int a = 0; a < NL; ++a) {
-      const int64_t nd = A.cell_nodes[cell * NL + a];
-#pragma unroll
-      for (int k = 0; k < D; ++k) X[a][k] = A.xyz[nd * D + k];
-    }
-  }
-  const double kap = A.kappa ? A.kappa[cell] : 1.0;
-  const double fc = A.fsrc ? A.fsrc[cell] : 0.0;
-  const double pm2 = A.p - 2.0;
-  T K[NL][NL];
-  double r[NL];
-#pragma unroll
-  for (int i = 0; i < NL; ++i) {
-    r[i] = 0.0;
-#pragma unroll
-    for (int j = 0; j < NL; ++j) K[i][j] = 0;
-  }
+  for (int a = 0; a < NL; ++a) r[a] = 0.0;
   const int stride = 1 + NL + NL * D;
-  for (int q = 0; q < A.nq; ++q) {
-    const double *tq = A.tab + q * stride;
+  for (int q = 0; q < nq; ++q) {
+    const double *tq = tab + q * stride;
     double J[D * D], Ji[D * D];
-    if (A.cell_nodes) {
+    if (mapped) {
 #pragma unroll
       for (int i = 0; i < D; ++i)
 #pragma unroll
@@ -96,7 +69,7 @@ __global__ __launch_bounds__(128) void plaplacian_q1_kernel(const NlArgs A) {
 #pragma unroll
       for (int i = 0; i < D * D; ++i) J[i] = 0.0;
 #pragma unroll
-      for (int k = 0; k < D; ++k) J[k * D + k] = A.h[k];
+      for (int k = 0; k < D; ++k) J[k * D + k] = h[k];
     }
     const double dV = fabs(det_inv<D>(J, Ji)) * tq[0];
     double g[NL][D], gu[D];
@@ -129,16 +102,76 @@ __global__ __launch_bounds__(128) void plaplacian_q1_kernel(const NlArgs A) {
       r[a] += dV * (kap * s0 * s - fc * tq[1 + a]);
     }
     const T w0 = (T)(kap * dV * s0), w1 = (T)(kap * dV * s1);
+    int t = 0;
 #pragma unroll
     for (int i = 0; i < NL; ++i)
 #pragma unroll
-      for (int j = 0; j < NL; ++j) {
+      for (int j = i; j < NL; ++j, ++t) {
         T dot = 0;
 #pragma unroll
         for (int k = 0; k < D; ++k) dot += (T)g[i][k] * (T)g[j][k];
-        K[i][j] += w0 * dot + w1 * ((T)ga[i] * (T)ga[j]);
+        Kt[t] += w0 * dot + w1 * ((T)ga[i] * (T)ga[j]);
       }
   }
+}
+
+// Index of (i, j) in the row-major upper triangle of an NL x NL matrix.
+template <int NL>
+__device__ __forceinline__ int tri_index(int i, int j) {
+  const int lo = i < j ? i : j, hi = i < j ? j : i;
+  return lo * NL - lo * (lo - 1) / 2 + (hi - lo);
+}
+
+__device__ __forceinline__ double uh_value(int32_t d, const double *ufree,
+                                           const double *udiri) {
+  return d > 0 ? ufree[d - 1] : (udiri ? udiri[-d - 1] : 0.0);
+}
+
+struct NlArgs {
+  const int32_t *cells;  // cells of this colour
+  int64_t ncells;
+  const int32_t *cell_dofs;
+  const int32_t *cell_nodes;  // nullptr: Cartesian analytic
+  const double *xyz;
+  double h[3];
+  int nq;
+  const double *tab;  // per q: w, shp[NLOC], grad[NLOC*D]
+  const double *kappa, *fsrc;
+  const double *ufree, *udiri;  // the FE function uh: free and Dirichlet values
+  double p;
+  const int32_t *colptr, *rowval;
+  double *nzval, *b;
+};
+
+template <int D, bool F32>
+__global__ __launch_bounds__(128) void plaplacian_q1_kernel(const NlArgs A) {
+  constexpr int NL = 1 << D;
+  constexpr int NT = NL * (NL + 1) / 2;
+  using T = typename std::conditional<F32, float, double>::type;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= A.ncells) return;
+  const int64_t cell = A.cells[t];
+  int32_t dofs[NL];
+  double u[NL];
+#pragma unroll
+  for (int a = 0; a < NL; ++a) {
+    dofs[a] = A.cell_dofs[cell * NL + a];
+    u[a] = uh_value(dofs[a], A.ufree, A.udiri);
+  }
+  double X[NL][D];
+  if (A.cell_nodes) {
+#pragma unroll
+    for (int a = 0; a < NL; ++a) {
+      const int64_t nd = A.cell_nodes[cell * NL + a];
+#pragma unroll
+      for (int k = 0; k < D; ++k) X[a][k] = A.xyz[nd * D + k];
+    }
+  }
+  const double kap = A.kappa ? A.kappa[cell] : 1.0;
+  const double fc = A.fsrc ? A.fsrc[cell] : 0.0;
+  T K[NT];
+  double r[NL];
+  plap_element<D, T>(X, A.cell_nodes != nullptr, A.h, u, kap, fc, A.p - 2.0, A.tab, A.nq, K, r);
   // scatter: columns lj, rows li, Dirichlet ids skipped
 #pragma unroll
   for (int lj = 0; lj < NL; ++lj) {
@@ -155,12 +188,70 @@ __global__ __launch_bounds__(128) void plaplacian_q1_kernel(const NlArgs A) {
         if (A.rowval[mid] < i - 1) lo = mid + 1;
         else hi = mid;
       }
-      if (lo < p1 && A.rowval[lo] == i - 1) A.nzval[lo] += (double)K[li][lj];
+      if (lo < p1 && A.rowval[lo] == i - 1) A.nzval[lo] += (double)K[tri_index<NL>(li, lj)];
     }
   }
 #pragma unroll
   for (int li = 0; li < NL; ++li)
     if (dofs[li] > 0) A.b[dofs[li] - 1] += r[li];
+}
+
+struct NlLatArgs {
+  int nc[3], nn[3];
+  int ncells;
+  int nq;
+  const int32_t *cell_dofs;
+  const double *xyz;  // node coordinates, or nullptr: the Cartesian cells of sizes h
+  double h[3];
+  const double *tab;
+  const double *kappa, *fsrc;
+  const double *ufree, *udiri;
+  double p;
+};
+
+// Lattice meshes: the element Jacobian's upper triangle (E, [t][cell]) and the FP64
+// residual (R, [a][cell]) of every cell, for the owner-computes column kernel.
+template <int D, bool F32>
+__global__ __launch_bounds__(256) void plaplacian_lattice_elem_kernel(
+    const NlLatArgs A, typename std::conditional<F32, float, double>::type *E, double *R) {
+  constexpr int NL = 1 << D;
+  constexpr int NT = NL * (NL + 1) / 2;
+  using T = typename std::conditional<F32, float, double>::type;
+  const int cell = blockIdx.x * blockDim.x + threadIdx.x;
+  if (cell >= A.ncells) return;
+  double X[NL][D], u[NL];
+  if (A.xyz) {
+    int ci[3] = {0, 0, 0};
+    int rem = cell;
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      ci[k] = rem % A.nc[k];
+      rem /= A.nc[k];
+    }
+#pragma unroll
+    for (int a = 0; a < NL; ++a) {
+      int node = 0, stride = 1;
+#pragma unroll
+      for (int k = 0; k < D; ++k) {
+        node += (ci[k] + ((a >> k) & 1)) * stride;
+        stride *= A.nn[k];
+      }
+#pragma unroll
+      for (int k = 0; k < D; ++k) X[a][k] = A.xyz[(int64_t)node * D + k];
+    }
+  }
+#pragma unroll
+  for (int a = 0; a < NL; ++a)
+    u[a] = uh_value(A.cell_dofs[(int64_t)cell * NL + a], A.ufree, A.udiri);
+  const double kap = A.kappa ? A.kappa[cell] : 1.0;
+  const double fc = A.fsrc ? A.fsrc[cell] : 0.0;
+  T K[NT];
+  double r[NL];
+  plap_element<D, T>(X, A.xyz != nullptr, A.h, u, kap, fc, A.p - 2.0, A.tab, A.nq, K, r);
+#pragma unroll
+  for (int t = 0; t < NT; ++t) E[(int64_t)t * A.ncells + cell] = K[t];
+#pragma unroll
+  for (int a = 0; a < NL; ++a) R[(int64_t)a * A.ncells + cell] = r[a];
 }
 
 __global__ void zero_kernel(int64_t n, double *__restrict__ v) {
@@ -177,7 +268,7 @@ void zero(Ctx &c, double *v, int64_t n) {
 
 }  // namespace
 
-void assemble_plaplacian(Ctx &c, int precision) {
+void assemble_plaplacian(Ctx &c, int precision, int path) {
   GH_REQUIRE(c.reffe_kind == GH_REFFE_QK && c.order == 1 && c.ncomp == 1 &&
                  c.nloc == (1 << c.dim),
              GH_ERR_UNSUPPORTED, "the p-Laplacian form is provided for scalar Q1 on quads/hexes");
@@ -189,8 +280,9 @@ void assemble_plaplacian(Ctx &c, int precision) {
              GH_ERR_STATE, "no FE function: call gh_fefunction_upload");
   GH_REQUIRE(!c.have_kappa || c.kappa.n == (size_t)c.ncells, GH_ERR_STATE,
              "kappa was uploaded for a different mesh");
-  build_coloring(c);
-  c.last_kernel = GH_KERNEL_CELL;
+  const bool lattice = c.structured && path != GH_PATH_GENERAL;
+  GH_REQUIRE(path != GH_PATH_STRUCTURED || lattice, GH_ERR_UNSUPPORTED,
+             "the lattice path needs a Cartesian-topology mesh with a scalar Q1 CLagrangian space");
   const int NL = c.nloc, D = c.dim;
   const int stride = 1 + NL + NL * D;
   std::vector<double> tab(size_t(c.nq) * stride);
@@ -202,6 +294,50 @@ void assemble_plaplacian(Ctx &c, int precision) {
   c.nl_tab.resize(tab.size());
   GH_CHECK_HIP(hipMemcpyAsync(c.nl_tab.get(), tab.data(), sizeof(double) * tab.size(),
                               hipMemcpyHostToDevice, c.stream));
+  const bool f32 = precision == GH_PREC_FP32_ELEM;
+  if (lattice) {
+    GH_REQUIRE(c.cartesian || c.xyz.n == (size_t)c.nnodes * D, GH_ERR_STATE,
+               "the mapped lattice needs the node coordinates");
+    c.last_kernel = GH_KERNEL_Q1_LATTICE;
+    const int NT = NL * (NL + 1) / 2;
+    // Jacobian triangles (FP32 or FP64) then FP64 residuals; NT is even, so the
+    // residual block starts 8-byte aligned
+    const size_t ebytes = (size_t)NT * c.ncells * (f32 ? sizeof(float) : sizeof(double));
+    c.qelem.resize(ebytes + (size_t)NL * c.ncells * sizeof(double));
+    double *R = reinterpret_cast<double *>(c.qelem.get() + ebytes);
+    NlLatArgs A{};
+    for (int k = 0; k < 3; ++k) {
+      A.nc[k] = (int)c.part[k];
+      A.nn[k] = (int)(c.part[k] + 1);
+      A.h[k] = c.sizes[k];
+    }
+    A.ncells = (int)c.ncells;
+    A.nq = c.nq;
+    A.cell_dofs = c.cell_dofs.get();
+    A.xyz = c.cartesian ? nullptr : c.xyz.get();
+    A.tab = c.nl_tab.get();
+    A.kappa = c.have_kappa ? c.kappa.get() : nullptr;
+    A.fsrc = c.have_fsrc ? c.fsrc.get() : nullptr;
+    A.ufree = c.ustate.get();
+    A.udiri = c.have_dvals ? c.dvals.get() : nullptr;
+    A.p = c.plap_p;
+    const unsigned eg = (unsigned)((c.ncells + 255) / 256);
+    void *E = c.qelem.get();
+    if (D == 2) {
+      if (f32) hipLaunchKernelGGL((plaplacian_lattice_elem_kernel<2, true>), dim3(eg), dim3(256), 0, c.stream, A, (float *)E, R);
+      else hipLaunchKernelGGL((plaplacian_lattice_elem_kernel<2, false>), dim3(eg), dim3(256), 0, c.stream, A, (double *)E, R);
+    } else {
+      if (f32) hipLaunchKernelGGL((plaplacian_lattice_elem_kernel<3, true>), dim3(eg), dim3(256), 0, c.stream, A, (float *)E, R);
+      else hipLaunchKernelGGL((plaplacian_lattice_elem_kernel<3, false>), dim3(eg), dim3(256), 0, c.stream, A, (double *)E, R);
+    }
+    GH_CHECK_HIP(hipGetLastError());
+    // the residual carries no lifting: uh already holds its Dirichlet values
+    launch_lattice_columns(c, f32, R, false);
+    GH_CHECK_HIP(hipStreamSynchronize(c.stream));
+    return;
+  }
+  build_coloring(c);
+  c.last_kernel = GH_KERNEL_CELL;
   zero(c, c.nzval.get(), c.nnz);
   zero(c, c.b.get(), c.nfree);
   NlArgs A{};
@@ -220,7 +356,6 @@ void assemble_plaplacian(Ctx &c, int precision) {
   A.rowval = c.rowval.get();
   A.nzval = c.nzval.get();
   A.b = c.b.get();
-  const bool f32 = precision == GH_PREC_FP32_ELEM;
   for (int col = 0; col < c.ncolors; ++col) {
     A.cells = c.color_cells.get() + c.color_ptr[col];
     A.ncells = c.color_ptr[col + 1] - c.color_ptr[col];

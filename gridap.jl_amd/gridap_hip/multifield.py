@@ -137,42 +137,52 @@ def _same_space(A, B):
             np.array_equal(np.asarray(a.cell_dof_ids), np.asarray(b.cell_dof_ids)))
 
 
-def compose_blocks(sub, row_fields, col_fields, nrows, ncols):
-    """One matrix over the fields row_fields x col_fields from the field-pair matrices
-    sub[(i, j)] (SparseMatrixCSC, 1-based; missing pairs are untouched): column
-    (j, k) is the concatenation over i of the rows of sub[(i, j)] column k, each shifted
-    by the offset of field i within row_fields."""
-    from .assemblers import SparseMatrixCSC
+def compose_plan(sub, row_fields, col_fields, nrows, ncols):
+    """The layout of compose_blocks for the patterns of sub: (m, n, colptr, rowval,
+    [(pair, destination indices)]).  It depends on the patterns only, so an assembler
+    reuses it for every assembly on its fixed patterns."""
     roff = np.concatenate([[0], np.cumsum([nrows[i] for i in row_fields])]).astype(np.int64)
-    colptrs, rowvals, nzvals = [np.ones(1, dtype=np.int64)], [], []
+    colptrs, rowvals, scatter = [np.ones(1, dtype=np.int64)], [], []
     base = 0
     for j in col_fields:
         nc = ncols[j]
-        parts = [(r, sub[(i, j)]) for r, i in enumerate(row_fields) if (i, j) in sub]
+        parts = [(r, (i, j)) for r, i in enumerate(row_fields) if (i, j) in sub]
         cnt = np.zeros(nc, dtype=np.int64)
-        for _, B in parts:
-            cnt += np.diff(B.colptr)
+        for _, key in parts:
+            cnt += np.diff(sub[key].colptr)
         start = np.concatenate([[0], np.cumsum(cnt)]).astype(np.int64)
         total = int(start[-1])
         rv = np.empty(total, dtype=np.int64)
-        nz = np.empty(total, dtype=np.float64)
         done = np.zeros(nc, dtype=np.int64)   # entries already placed in each column
-        for r, B in parts:
+        for r, key in parts:
+            B = sub[key]
             c = np.diff(B.colptr)
             first = np.repeat(start[:-1] + done, c)
             within = np.arange(B.nnz(), dtype=np.int64) - np.repeat(B.colptr[:-1] - 1, c)
             dst = first + within
             rv[dst] = B.rowval + roff[r]
-            nz[dst] = B.nzval
+            scatter.append((key, dst + base))
             done += c
         colptrs.append(start[1:] + 1 + base)
         rowvals.append(rv)
-        nzvals.append(nz)
         base += total
     cp = np.concatenate(colptrs)
     rv = np.concatenate(rowvals) if rowvals else np.zeros(0, dtype=np.int64)
-    nz = np.concatenate(nzvals) if nzvals else np.zeros(0)
-    return SparseMatrixCSC(int(roff[-1]), int(sum(ncols[j] for j in col_fields)), cp, rv, nz)
+    return int(roff[-1]), int(sum(ncols[j] for j in col_fields)), cp, rv, scatter
+
+
+def compose_blocks(sub, row_fields, col_fields, nrows, ncols, plan=None):
+    """One matrix over the fields row_fields x col_fields from the field-pair matrices
+    sub[(i, j)] (SparseMatrixCSC, 1-based; missing pairs are untouched): column
+    (j, k) is the concatenation over i of the rows of sub[(i, j)] column k, each shifted
+    by the offset of field i within row_fields.  plan: compose_plan of the same
+    patterns (computed here when None)."""
+    from .assemblers import SparseMatrixCSC
+    m, n, cp, rv, scatter = plan or compose_plan(sub, row_fields, col_fields, nrows, ncols)
+    nz = np.empty(len(rv), dtype=np.float64)
+    for key, dst in scatter:
+        nz[dst] = sub[key].nzval
+    return SparseMatrixCSC(m, n, cp.copy(), rv.copy(), nz)
 
 
 class MultiFieldSparseMatrixAssembler:
@@ -189,6 +199,14 @@ class MultiFieldSparseMatrixAssembler:
         self.U, self.V = U, V
         self.device, self.precision, self.path = device, precision, path
         self._assems = []   # (test space, single-field assembler)
+        self._plans = {}    # compose_plan per (row fields, column fields, touched pairs)
+
+    def _compose(self, sub, rg, cg, nrows, ncols):
+        # the pair patterns are the fixed patterns of the single-field assemblers
+        key = (tuple(rg), tuple(cg), tuple(sorted(k for k in sub if k[0] in rg and k[1] in cg)))
+        if key not in self._plans:
+            self._plans[key] = compose_plan(sub, rg, cg, nrows, ncols)
+        return compose_blocks(sub, rg, cg, nrows, ncols, plan=self._plans[key])
 
     def get_rows(self):
         return np.arange(1, self.V.num_free_dofs() + 1)
@@ -246,7 +264,7 @@ class MultiFieldSparseMatrixAssembler:
         groups = self.V.groups()
         mat = vec = None
         if matcontrib is not None:
-            blocks = [[compose_blocks(sub, rg, cg, nrows, ncols) for cg in self.U.groups()]
+            blocks = [[self._compose(sub, rg, cg, nrows, ncols) for cg in self.U.groups()]
                       for rg in groups]
             mat = blocks[0][0] if not isinstance(self.V.style, BlockMultiFieldStyle) \
                 else BlockSparseMatrix(blocks)

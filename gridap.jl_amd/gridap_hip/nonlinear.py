@@ -117,7 +117,7 @@ class FEOperator:
         if A is not None and not (np.array_equal(A.colptr, assem.colptr) and
                                   np.array_equal(A.rowval, assem.rowval)):
             raise ValueError("A does not have this assembler's sparsity pattern")
-        ctx.assemble(_lib.GH_FORM_PLAPLACIAN, assem.precision, _lib.GH_PATH_GENERAL)
+        ctx.assemble(_lib.GH_FORM_PLAPLACIAN, assem.precision, assem.path)
         nz, r = ctx.values_get(None if A is None else A.nzval, b)
         if A is None:
             n = assem.space.nfree

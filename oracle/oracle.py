@@ -570,13 +570,13 @@ def boundary_face_arrays(model, faces, degree, alpha=0.0, beta=0.0, g=None, h=No
 
 
 def poisson_q1_with_boundary(domain, partition, dirichlet_entities, face_terms, degree=2,
-                             face_degree=2, g=None, fsrc=None):
+                             face_degree=2, g=None, fsrc=None, kappa=None):
     """Q1 Poisson with boundary terms, assembled as Gridap's AffineFEOperator does with
     a weak form over Ω and boundary triangulations: cell arrays then face arrays in one
     assembly (the cell contributions first), Dirichlet lifting of both
     (src/FESpaces/Assemblers.jl:485-541, src/CellData/AttachDirichlet.jl).
     dirichlet_entities: the entity ids of the Dirichlet tag; face_terms: [(entity ids,
-    alpha, beta, g(x, n), h(x, n))]."""
+    alpha, beta, g(x, n), h(x, n))]; kappa: the diffusion coefficient per cell (1)."""
     D = len(partition)
     model = cartesian_model(domain, partition)
     ents = sorted(set(dirichlet_entities))
@@ -585,7 +585,7 @@ def poisson_q1_with_boundary(domain, partition, dirichlet_entities, face_terms, 
     cd = clagrangian_cell_dofs(model["cell_nodes"], n2d)
     qx, qw = tensor_quadrature(D, degree)
     shp, grad = qk_tabulate(D, 1, qx)
-    mats = cell_matrices_diffusion(D, model["cell_nodes"], model["xyz"], qw, grad,
+    mats = cell_matrices_diffusion(D, model["cell_nodes"], model["xyz"], qw, grad, kappa=kappa,
                                    cartesian_sizes=model["sizes"])
     vecs = cell_vectors_source(D, model["cell_nodes"], model["xyz"], qw, shp, grad, fsrc,
                                cartesian_sizes=model["sizes"])

@@ -116,6 +116,10 @@ def test_3d_robin_and_neumann_match_oracle(oracle):
     np.testing.assert_array_equal(A.rowval, ref["rowval"])
     assert relmax(A.nzval, ref["nzval"]) <= 1e-12
     assert relmax(b, ref["b"]) <= 1e-12
+    # the face kernel adds each face matrix (symmetric bitwise) into column j by the
+    # owner of j, faces ascending: the assembled matrix stays symmetric bitwise
+    S = A.to_scipy()
+    assert (S - S.T).count_nonzero() == 0
 
 
 # ---- the driver at its own order (PoissonTests.jl:12: order = 2, degree = order) -------
@@ -184,3 +188,68 @@ def test_poisson_driver_order2_matches_oracle(oracle, part, precision):
         x = G.solve(op).get_free_dof_values()
         uI = U2(V.free_dof_coords)
         assert np.abs(x - uI).max() <= 1e-8 * np.abs(uI).max()
+
+
+@pytest.mark.parametrize("mass", [False, True])
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_3d_faces_folded_into_lattice_match_face_kernel(oracle, mass, precision):
+    """3-D 10x9x8 with Nitsche terms (α = γ/h, β = -1) on z = 1, Robin (α = 3) on y = 0, a
+    Neumann load on x = 1 and Dirichlet data on x = 0.  On the lattice the faces are
+    folded into the owner-computes columns (with a cell mass term: the stored element
+    matrices, FP32 or FP64); path "general" runs the per-cell kernel and the separate
+    face kernel.  Both agree to 1e-12 (FP64), and without the mass term the FP64 matrix
+    equals the oracle's."""
+    part = (10, 9, 8)
+    announce("boundary faces folded into the lattice %s mass=%s %s" % (part, mass, precision))
+    h = 1.0 / part[2]
+    gam = 10.0
+    ue = lambda x: 1.0 + x[..., 0] - 0.5 * x[..., 2]
+    model = G.CartesianDiscreteModel((0, 1) * 3, part)
+    labels = model.get_face_labeling()
+    Ω = G.Triangulation(model)
+    Γd = G.BoundaryTriangulation(model, labels, tags="tag_22")
+    Γr = G.BoundaryTriangulation(model, labels, tags="tag_23")
+    Γn = G.BoundaryTriangulation(model, labels, tags="tag_26")
+    dΩ, dΓd, dΓr, dΓn = (G.Measure(t, 2) for t in (Ω, Γd, Γr, Γn))
+    nd = G.get_normal_vector(Γd)
+    V = G.TestFESpace(model, G.ReferenceFE(G.lagrangian, G.Float64, 1), dirichlet_tags="tag_25")
+    U = G.TrialFESpace(V, ue)
+    uh = G.interpolate(ue, U)
+    kappa = np.exp(np.random.default_rng(4).normal(size=model.num_cells()))
+    k = G.CellField(kappa, Ω)
+    g = lambda x: np.cos(2 * x[..., 1]) + x[..., 2]
+    sig = 2.5 if mass else 0.0
+
+    def a(u, v):
+        vol = k * G.inner(G.gradient(v), G.gradient(u))
+        if mass:
+            vol = vol + sig * v * u
+        return (G.integrate(vol, dΩ) +
+                G.integrate((gam / h) * v * u - v * G.dot(nd, G.gradient(u)) -
+                            G.dot(nd, G.gradient(v)) * u, dΓd) +
+                G.integrate(3.0 * v * u, dΓr))
+
+    l = lambda v: (G.integrate(1.0 * v, dΩ) + G.integrate(v * g, dΓn) +
+                   G.integrate((gam / h) * v * uh - G.dot(nd, G.gradient(v)) * uh, dΓd))
+    ops = {path: G.AffineFEOperator(a, l, U, V, G.SparseMatrixAssembler(U, V, precision=precision,
+                                                                        path=path))
+           for path in ("auto", "general")}
+    assert ops["auto"].assem.ctx.pattern_info()["last_kernel"] == G._lib.GH_KERNEL_Q1_LATTICE
+    assert ops["general"].assem.ctx.pattern_info()["last_kernel"] == G._lib.GH_KERNEL_CELL
+    A, b = ops["auto"].get_matrix(), ops["auto"].get_vector()
+    Ag, bg = ops["general"].get_matrix(), ops["general"].get_vector()
+    np.testing.assert_array_equal(A.rowval, Ag.rowval)
+    tol = 1e-12 if precision == "fp64" else 1e-6
+    assert relmax(A.nzval, Ag.nzval) <= tol and relmax(b, bg) <= tol
+    S = A.to_scipy()
+    assert (S - S.T).count_nonzero() == 0
+    if not mass and precision == "fp64":
+        U1 = lambda x: ue(np.asarray(x))
+        ref = oracle.poisson_q1_with_boundary(
+            (0, 1) * 3, part, [25],
+            [([22], gam / h, -1.0, lambda x, n: gam / h * U1(x), lambda x, n: -U1(x)),
+             ([23], 3.0, 0.0, None, None),
+             ([26], 0.0, 0.0, lambda x, n: np.cos(2 * x[1]) + x[2], None)],
+            g=U1, kappa=kappa)
+        np.testing.assert_array_equal(A.rowval, ref["rowval"])
+        assert relmax(A.nzval, ref["nzval"]) <= 1e-12 and relmax(b, ref["b"]) <= 1e-12
