@@ -52,7 +52,9 @@ extern "C" {
                               * ((p-2)|∇u|^(p-4)(∇u⋅∇du)∇u + |∇u|^(p-2)∇du) dΩ, κ (NULL = 1) and
                               * f (NULL = 0) per cell from gh_coef_upload, p >= 2
                               * (gh_plaplacian_params); gh_assemble writes the residual to b and
-                              * the Jacobian to nzval; scalar Q1 on quads/hexes */
+                              * the Jacobian to nzval; scalar Q1 on quads/hexes.  Lattice
+                              * meshes take the owner-computes column route unless path is
+                              * GH_PATH_GENERAL (the coloured per-cell scatter) */
 
 /* ---- element precision ---- */
 #define GH_PREC_FP64 0      /* element arithmetic and accumulation in FP64           */
@@ -148,7 +150,9 @@ int gh_face_tabulation(gh_ctx *ctx, int nsides, int nqf, const double *qw, const
  * with α[f], β[f] per face (NULL = 0) and g, h at the face quadrature points,
  * gq[f*nqf + q], hq[f*nqf + q] (NULL = 0), followed by the Dirichlet lifting of the face
  * matrix.  Neumann: g = n⋅∇u; Nitsche (test/GridapTests/PoissonTests.jl:101-108):
- * α = γ/h, β = -1, g = (γ/h) u_D, h = -u_D.  nfaces = 0 removes the boundary terms. */
+ * α = γ/h, β = -1, g = (γ/h) u_D, h = -u_D.  nfaces = 0 removes the boundary terms.
+ * The first gh_assemble after an upload (or a new pattern) builds the face lists: for
+ * each nonzero the faces touch, its face contributions (faces ascending). */
 int gh_boundary_upload(gh_ctx *ctx, int64_t nfaces, const int32_t *face_cells,
                        const int32_t *face_sides, const double *alpha, const double *beta,
                        const double *gq, const double *hq);
