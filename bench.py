@@ -562,6 +562,26 @@ def q1_mapped_leg(n, ev, steps=5, warmup=1, seed=2024):
         ev.destroy(evs)
         un[name] = {"ms_per_assembly": ms, "value": nnz2 / (ms * 1e-3), "unit": "NNZ/s",
                     "kernel": ctx.pattern_info()["last_kernel"]}
+    # the p-Laplacian (p = 3) residual + Jacobian at a random FE function on the same
+    # mesh: element lists, and the coloured per-cell scatter (path "general")
+    ctx.fefunction_upload(rng.standard_normal(N), np.zeros(int(np.count_nonzero(isd))))
+    ctx.plaplacian_params(3.0)
+    pl = {}
+    for name, prec in (("fp64", _lib.GH_PREC_FP64), ("fp32", _lib.GH_PREC_FP32_ELEM)):
+        for key, path in (("ms_per_assembly", _lib.GH_PATH_AUTO),
+                          ("cell_kernel_ms", _lib.GH_PATH_GENERAL)):
+            ctx.assemble(_lib.GH_FORM_PLAPLACIAN, prec, path)
+            ev.sync(stream)
+            evs = ev.create(2)
+            ev.record(evs[0], stream)
+            for _ in range(steps):
+                ctx.assemble(_lib.GH_FORM_PLAPLACIAN, prec, path)
+            ev.record(evs[1], stream)
+            pl.setdefault(name, {})[key] = ev.elapsed_ms(evs[0], evs[1]) / steps
+            ev.destroy(evs)
+            if path == _lib.GH_PATH_AUTO:
+                pl[name]["kernel"] = ctx.pattern_info()["last_kernel"]
+    un["plaplacian_p3"] = pl
     out["unstructured"] = un
     ctx.close()
     return out

@@ -315,6 +315,9 @@ void assemble_diffusion(Ctx &c, int precision, int path);
 bool simplex_eligible(const Ctx &c);
 bool hexgen_eligible(const Ctx &c);      // scalar Q2 / vector Q1, Q2 n-cubes for hexgen
 void hexgen_assemble(Ctx &c, int precision, bool elasticity);
+// b[j] = Σ over j's adjacency (ascending cells) of the cell loads in c.evec
+// ([cell][local], FP64); enqueued on c.stream.
+void hexgen_rhs(Ctx &c);
 void elemlist_symbolic(Ctx &c);         // dof -> cell adjacency, Dirichlet masks
 void elemlist_sum(Ctx &c, bool f32);     // nzval from the element entries
 void assemble_simplex(Ctx &c, int precision);

@@ -334,6 +334,14 @@ bool hexgen_eligible(const Ctx &c) {
   return (scalar_q2 || vector || scalar_q1) && c.nloc == nn * c.ncomp && nn <= kHgMaxNn;
 }
 
+void hexgen_rhs(Ctx &c) {
+  const int lbits = c.nloc <= 16 ? 4 : 7;
+  hipLaunchKernelGGL(hexgen_rhs_kernel, dim3((unsigned)std::max<int64_t>(1, (c.nfree + 255) / 256)),
+                     dim3(256), 0, c.stream, c.nfree, c.nloc, lbits, c.adjptr.get(), c.adj.get(),
+                     c.evec.get(), c.b.get());
+  GH_CHECK_HIP(hipGetLastError());
+}
+
 void hexgen_assemble(Ctx &c, int precision, bool elasticity) {
   GH_REQUIRE(hexgen_eligible(c) && (elasticity ? c.ncomp == c.dim : c.ncomp == 1),
              GH_ERR_UNSUPPORTED,
@@ -436,11 +444,7 @@ void hexgen_assemble(Ctx &c, int precision, bool elasticity) {
   }
   GH_CHECK_HIP(hipGetLastError());
   elemlist_sum(c, f32);
-  const int lbits = nloc <= 16 ? 4 : 7;
-  hipLaunchKernelGGL(hexgen_rhs_kernel, dim3((unsigned)std::max<int64_t>(1, (c.nfree + 255) / 256)),
-                     dim3(256), 0, c.stream, c.nfree, nloc, lbits, c.adjptr.get(), c.adj.get(),
-                     c.evec.get(), c.b.get());
-  GH_CHECK_HIP(hipGetLastError());
+  hexgen_rhs(c);
   // the tabulation buffers are freed on return: wait for the kernels that read them
   GH_CHECK_HIP(hipStreamSynchronize(c.stream));
 }

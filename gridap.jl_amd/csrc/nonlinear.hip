@@ -23,9 +23,14 @@
 //    element Jacobians and residuals are stored per cell and the owner-computes column
 //    kernel of the linear lattice path sums them (each free node writes its column
 //    once, coalesced; incident cells in ascending order);
-//  * other meshes: each thread adds its element into nzval by a binary search of the
-//    column; cells are launched colour by colour (no two cells of one colour share a
-//    free dof), so the scatter is race-free and deterministic.
+//  * quadrilateral / hexahedral meshes without the lattice topology: the element
+//    Jacobians (full columns) and residuals are stored per cell and the element-list
+//    sums of the general n-cube path add them per nonzero and per b row, cells
+//    ascending;
+//  * on request (GH_PATH_GENERAL), or where neither applies: each thread adds its
+//    element into nzval by a binary search of the column; cells are launched colour by
+//    colour (no two cells of one colour share a free dof), so the scatter is race-free
+//    and deterministic.
 #include <algorithm>
 #include <cmath>
 #include <type_traits>
@@ -254,6 +259,113 @@ __global__ __launch_bounds__(256) void plaplacian_lattice_elem_kernel(
   for (int a = 0; a < NL; ++a) R[(int64_t)a * A.ncells + cell] = r[a];
 }
 
+// Cartesian lattice cells: the geometry is the same box for every cell, so the
+// physical gradients g_a(q) = ∇̂φ_a(q)/h and their products P_q[ab] = g_a(q)⋅g_b(q)
+// come from the table (wave-uniform loads) and only uh's values live in registers:
+// J_ab += w0 P_q[ab] + w1 (∇u⋅g_a)(∇u⋅g_b), which keeps the kernel at three waves per
+// SIMD instead of one.  Table per q: w|det|, φ[NL], g[NL*D], P[NT].
+template <int D, bool F32>
+__global__ __launch_bounds__(256) void plaplacian_cart_elem_kernel(
+    const NlLatArgs A, typename std::conditional<F32, float, double>::type *E, double *R) {
+  constexpr int NL = 1 << D;
+  constexpr int NT = NL * (NL + 1) / 2;
+  constexpr int stride = 1 + NL + NL * D + NT;
+  using T = typename std::conditional<F32, float, double>::type;
+  const int cell = blockIdx.x * blockDim.x + threadIdx.x;
+  if (cell >= A.ncells) return;
+  double u[NL];
+#pragma unroll
+  for (int a = 0; a < NL; ++a)
+    u[a] = uh_value(A.cell_dofs[(int64_t)cell * NL + a], A.ufree, A.udiri);
+  const double kap = A.kappa ? A.kappa[cell] : 1.0;
+  const double fc = A.fsrc ? A.fsrc[cell] : 0.0;
+  const double pm2 = A.p - 2.0;
+  T K[NT];
+  double r[NL];
+#pragma unroll
+  for (int t = 0; t < NT; ++t) K[t] = 0;
+#pragma unroll
+  for (int a = 0; a < NL; ++a) r[a] = 0.0;
+  for (int q = 0; q < A.nq; ++q) {
+    const double *tq = A.tab + q * stride;
+    const double *gq = tq + 1 + NL;
+    const double dV = tq[0];
+    double gu[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      double s = 0.0;
+#pragma unroll
+      for (int a = 0; a < NL; ++a) s += u[a] * gq[a * D + k];
+      gu[k] = s;
+    }
+    double n2 = 0.0;
+#pragma unroll
+    for (int k = 0; k < D; ++k) n2 += gu[k] * gu[k];
+    const double nrm = sqrt(n2);
+    const double s0 = pm2 == 0.0 ? 1.0 : (nrm > 0.0 ? pow(nrm, pm2) : 0.0);
+    const double s1 = (pm2 != 0.0 && nrm > 0.0) ? pm2 * s0 / n2 : 0.0;
+    double ga[NL];
+#pragma unroll
+    for (int a = 0; a < NL; ++a) {
+      double s = 0.0;
+#pragma unroll
+      for (int k = 0; k < D; ++k) s += gu[k] * gq[a * D + k];
+      ga[a] = s;
+      r[a] += dV * (kap * s0 * s - fc * tq[1 + a]);
+    }
+    const T w0 = (T)(kap * dV * s0), w1 = (T)(kap * dV * s1);
+    const double *P = gq + NL * D;
+    int t = 0;
+#pragma unroll
+    for (int i = 0; i < NL; ++i)
+#pragma unroll
+      for (int j = i; j < NL; ++j, ++t) K[t] += w0 * (T)P[t] + w1 * ((T)ga[i] * (T)ga[j]);
+  }
+#pragma unroll
+  for (int t = 0; t < NT; ++t) E[(int64_t)t * A.ncells + cell] = K[t];
+#pragma unroll
+  for (int a = 0; a < NL; ++a) R[(int64_t)a * A.ncells + cell] = r[a];
+}
+
+// Meshes without the lattice topology: the element Jacobian as full columns
+// (E[cell][b][a], both halves from one value) and the FP64 residual (R[cell][a]) of
+// every cell, for the element-list sums.
+template <int D, bool F32>
+__global__ __launch_bounds__(256) void plaplacian_list_elem_kernel(
+    const NlArgs A, typename std::conditional<F32, float, double>::type *E, double *R) {
+  constexpr int NL = 1 << D;
+  constexpr int NT = NL * (NL + 1) / 2;
+  using T = typename std::conditional<F32, float, double>::type;
+  const int64_t cell = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (cell >= A.ncells) return;
+  double X[NL][D], u[NL];
+#pragma unroll
+  for (int a = 0; a < NL; ++a) {
+    u[a] = uh_value(A.cell_dofs[cell * NL + a], A.ufree, A.udiri);
+    if (A.cell_nodes) {
+      const int64_t nd = A.cell_nodes[cell * NL + a];
+#pragma unroll
+      for (int k = 0; k < D; ++k) X[a][k] = A.xyz[nd * D + k];
+    }
+  }
+  const double kap = A.kappa ? A.kappa[cell] : 1.0;
+  const double fc = A.fsrc ? A.fsrc[cell] : 0.0;
+  T K[NT];
+  double r[NL];
+  plap_element<D, T>(X, A.cell_nodes != nullptr, A.h, u, kap, fc, A.p - 2.0, A.tab, A.nq, K, r);
+  T *Ec = E + cell * NL * NL;
+  int t = 0;
+#pragma unroll
+  for (int a = 0; a < NL; ++a)
+#pragma unroll
+    for (int b = a; b < NL; ++b, ++t) {
+      Ec[b * NL + a] = K[t];
+      Ec[a * NL + b] = K[t];
+    }
+#pragma unroll
+  for (int a = 0; a < NL; ++a) R[cell * NL + a] = r[a];
+}
+
 __global__ void zero_kernel(int64_t n, double *__restrict__ v) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x)
@@ -284,12 +396,31 @@ void assemble_plaplacian(Ctx &c, int precision, int path) {
   GH_REQUIRE(path != GH_PATH_STRUCTURED || lattice, GH_ERR_UNSUPPORTED,
              "the lattice path needs a Cartesian-topology mesh with a scalar Q1 CLagrangian space");
   const int NL = c.nloc, D = c.dim;
-  const int stride = 1 + NL + NL * D;
-  std::vector<double> tab(size_t(c.nq) * stride);
+  const int NT = NL * (NL + 1) / 2;
+  // Cartesian lattice cells take the table of plaplacian_cart_elem_kernel (per q: w|det|,
+  // φ, g = ∇̂φ/h, P_ab = g_a⋅g_b for a <= b), the other kernels per q: w, φ, ∇̂φ
+  const bool cart_lattice = lattice && c.cartesian;
+  const int stride = cart_lattice ? 1 + NL + NL * D + NT : 1 + NL + NL * D;
+  std::vector<double> tab(size_t(c.nq) * stride, 0.0);
+  double det = 1.0;
+  for (int k = 0; k < D; ++k) det *= c.sizes[k];
   for (int q = 0; q < c.nq; ++q) {
-    tab[q * stride] = c.qw[q];
-    for (int a = 0; a < NL; ++a) tab[q * stride + 1 + a] = c.shp[q * NL + a];
-    for (int a = 0; a < NL * D; ++a) tab[q * stride + 1 + NL + a] = c.grad[q * NL * D + a];
+    double *tq = &tab[(size_t)q * stride];
+    tq[0] = cart_lattice ? c.qw[q] * std::fabs(det) : c.qw[q];
+    for (int a = 0; a < NL; ++a) tq[1 + a] = c.shp[q * NL + a];
+    double *g = tq + 1 + NL;
+    for (int a = 0; a < NL; ++a)
+      for (int k = 0; k < D; ++k)
+        g[a * D + k] = c.grad[(q * NL + a) * D + k] / (cart_lattice ? c.sizes[k] : 1.0);
+    if (cart_lattice) {
+      int t = 0;
+      for (int i = 0; i < NL; ++i)
+        for (int j = i; j < NL; ++j, ++t) {
+          double dot = 0.0;
+          for (int k = 0; k < D; ++k) dot += g[i * D + k] * g[j * D + k];
+          g[NL * D + t] = dot;
+        }
+    }
   }
   c.nl_tab.resize(tab.size());
   GH_CHECK_HIP(hipMemcpyAsync(c.nl_tab.get(), tab.data(), sizeof(double) * tab.size(),
@@ -299,7 +430,6 @@ void assemble_plaplacian(Ctx &c, int precision, int path) {
     GH_REQUIRE(c.cartesian || c.xyz.n == (size_t)c.nnodes * D, GH_ERR_STATE,
                "the mapped lattice needs the node coordinates");
     c.last_kernel = GH_KERNEL_Q1_LATTICE;
-    const int NT = NL * (NL + 1) / 2;
     // Jacobian triangles (FP32 or FP64) then FP64 residuals; NT is even, so the
     // residual block starts 8-byte aligned
     const size_t ebytes = (size_t)NT * c.ncells * (f32 ? sizeof(float) : sizeof(double));
@@ -323,7 +453,15 @@ void assemble_plaplacian(Ctx &c, int precision, int path) {
     A.p = c.plap_p;
     const unsigned eg = (unsigned)((c.ncells + 255) / 256);
     void *E = c.qelem.get();
-    if (D == 2) {
+    if (c.cartesian) {
+      if (D == 2) {
+        if (f32) hipLaunchKernelGGL((plaplacian_cart_elem_kernel<2, true>), dim3(eg), dim3(256), 0, c.stream, A, (float *)E, R);
+        else hipLaunchKernelGGL((plaplacian_cart_elem_kernel<2, false>), dim3(eg), dim3(256), 0, c.stream, A, (double *)E, R);
+      } else {
+        if (f32) hipLaunchKernelGGL((plaplacian_cart_elem_kernel<3, true>), dim3(eg), dim3(256), 0, c.stream, A, (float *)E, R);
+        else hipLaunchKernelGGL((plaplacian_cart_elem_kernel<3, false>), dim3(eg), dim3(256), 0, c.stream, A, (double *)E, R);
+      }
+    } else if (D == 2) {
       if (f32) hipLaunchKernelGGL((plaplacian_lattice_elem_kernel<2, true>), dim3(eg), dim3(256), 0, c.stream, A, (float *)E, R);
       else hipLaunchKernelGGL((plaplacian_lattice_elem_kernel<2, false>), dim3(eg), dim3(256), 0, c.stream, A, (double *)E, R);
     } else {
@@ -336,10 +474,6 @@ void assemble_plaplacian(Ctx &c, int precision, int path) {
     GH_CHECK_HIP(hipStreamSynchronize(c.stream));
     return;
   }
-  build_coloring(c);
-  c.last_kernel = GH_KERNEL_CELL;
-  zero(c, c.nzval.get(), c.nnz);
-  zero(c, c.b.get(), c.nfree);
   NlArgs A{};
   A.cell_dofs = c.cell_dofs.get();
   A.cell_nodes = c.cartesian ? nullptr : c.cell_nodes.get();
@@ -356,6 +490,31 @@ void assemble_plaplacian(Ctx &c, int precision, int path) {
   A.rowval = c.rowval.get();
   A.nzval = c.nzval.get();
   A.b = c.b.get();
+  if (path != GH_PATH_GENERAL && c.elemlist && hexgen_eligible(c)) {
+    // element-list sums (sets last_kernel)
+    c.elem.resize((size_t)c.ncells * NL * NL * (f32 ? sizeof(float) : sizeof(double)));
+    c.evec.resize((size_t)c.ncells * NL);
+    A.ncells = c.ncells;
+    const unsigned eg = (unsigned)((c.ncells + 255) / 256);
+    void *E = c.elem.get();
+    double *R = c.evec.get();
+    if (D == 2) {
+      if (f32) hipLaunchKernelGGL((plaplacian_list_elem_kernel<2, true>), dim3(eg), dim3(256), 0, c.stream, A, (float *)E, R);
+      else hipLaunchKernelGGL((plaplacian_list_elem_kernel<2, false>), dim3(eg), dim3(256), 0, c.stream, A, (double *)E, R);
+    } else {
+      if (f32) hipLaunchKernelGGL((plaplacian_list_elem_kernel<3, true>), dim3(eg), dim3(256), 0, c.stream, A, (float *)E, R);
+      else hipLaunchKernelGGL((plaplacian_list_elem_kernel<3, false>), dim3(eg), dim3(256), 0, c.stream, A, (double *)E, R);
+    }
+    GH_CHECK_HIP(hipGetLastError());
+    elemlist_sum(c, f32);
+    hexgen_rhs(c);
+    GH_CHECK_HIP(hipStreamSynchronize(c.stream));
+    return;
+  }
+  build_coloring(c);
+  c.last_kernel = GH_KERNEL_CELL;
+  zero(c, c.nzval.get(), c.nnz);
+  zero(c, c.b.get(), c.nfree);
   for (int col = 0; col < c.ncolors; ++col) {
     A.cells = c.color_cells.get() + c.color_ptr[col];
     A.ncells = c.color_ptr[col + 1] - c.color_ptr[col];

@@ -171,3 +171,73 @@ def test_newton_iterates_match_oracle(oracle):
         xo = xo + spla.spsolve(oracle.to_scipy(cp, rv, nz, V.nfree).tocsc(), -r)
         assert relmax(x, xo) <= 1e-9, it
     assert np.abs(r).max() <= 1e-10 * r0   # converged
+
+
+def _unstructured_q1(oracle, part, seed):
+    """A Cartesian Q1 mesh with its nodes renumbered at random and moved (no lattice
+    topology left): coordinates, 1-based cell nodes, CLagrangian cell dofs of the
+    boundary-Dirichlet space in the new node order."""
+    import conforming_oracle as CO
+    D = len(part)
+    m = oracle.cartesian_model((0.0, 1.0) * D, part)
+    rng = np.random.default_rng(seed)
+    nn = m["xyz"].shape[0]
+    perm = rng.permutation(nn)                 # new id of old node k: perm[k]
+    xyz = np.empty_like(m["xyz"])
+    xyz[perm] = m["xyz"] + 0.05 * np.prod(np.sin(np.pi * m["xyz"]), axis=-1)[:, None]
+    cn = (perm[m["cell_nodes"] - 1] + 1).astype(np.int32)
+    lab = np.empty_like(m["vertex_labels"])
+    lab[perm] = m["vertex_labels"]
+    node_to_tag = oracle.face_tag_index(lab, [sorted(CO.boundary_entities(D))])
+    n2d, nfree, ndiri = oracle.clagrangian(node_to_tag)
+    cd = oracle.clagrangian_cell_dofs(cn, n2d)
+    return xyz, cn, cd, nfree, ndiri
+
+
+@pytest.mark.parametrize("part", [(9, 7), (6, 5, 4)])
+def test_unstructured_mesh_element_lists_match_oracle(oracle, part):
+    """A Q1 mesh without the lattice topology (renumbered, moved nodes), through the C
+    ABI: the residual and Jacobian go to the element-list sums (no coloured scatter),
+    within 1e-12 of the oracle (FP64) and 1e-6 (FP32 element math); the coloured
+    scatter (path "general") agrees."""
+    from gridap_hip import _lib
+    D = len(part)
+    announce("p-Laplacian element lists, unstructured %s" % (part,))
+    xyz, cn, cd, nfree, ndiri = _unstructured_q1(oracle, part, 11)
+    C = cn.shape[0]
+    rng = np.random.default_rng(D + 20)
+    kappa, fsrc = np.exp(rng.normal(size=C)), rng.uniform(-1, 1, C)
+    u, dv = rng.normal(size=nfree), rng.normal(size=ndiri)
+    p = 3.0
+    qx, qw = oracle.tensor_quadrature(D, 2)
+    shp, grad = oracle.qk_tabulate(D, 1, qx)
+    ptrs = (np.arange(C + 1) * (1 << D) + 1).astype(np.int32)
+    ctx = _lib.Context(0)
+    ctx.mesh_upload(xyz, cn.ravel(), ptrs)
+    ctx.space_upload(_lib.GH_REFFE_QK, 1, 1, np.ascontiguousarray(cd, np.int32).ravel(), ptrs,
+                     nfree, ndiri)
+    ctx.reffe_tabulation(qw, shp, grad.reshape(len(qw), -1))
+    ctx.symbolic()
+    ctx.coef_upload(kappa, fsrc, None)
+    ctx.fefunction_upload(u, dv)
+    ctx.plaplacian_params(p)
+    uc = oracle.cell_values(cd, u, dv)
+    K, r = oracle.plaplacian_cells(D, cn, xyz, qw, shp, grad, uc, p, kappa, fsrc)
+    cp, rv, nzr, br = oracle.plaplacian_system(cd, nfree, K, r)
+    colptr, rowval = ctx.pattern_get()
+    np.testing.assert_array_equal(colptr, cp)
+    np.testing.assert_array_equal(rowval, rv)
+    out = {}
+    for prec, path in ((_lib.GH_PREC_FP64, _lib.GH_PATH_AUTO), (_lib.GH_PREC_FP32_ELEM, _lib.GH_PATH_AUTO),
+                       (_lib.GH_PREC_FP64, _lib.GH_PATH_GENERAL)):
+        ctx.assemble(_lib.GH_FORM_PLAPLACIAN, prec, path)
+        out[(prec, path)] = ctx.values_get() + (ctx.pattern_info()["last_kernel"],)
+    ctx.close()
+    nz, b, kern = out[(_lib.GH_PREC_FP64, _lib.GH_PATH_AUTO)]
+    assert kern in (_lib.GH_KERNEL_COLUMN_SUM, _lib.GH_KERNEL_LIST_SUM)
+    assert relmax(nz, nzr) <= 1e-12 and relmax(b, br) <= 1e-12
+    nz32, b32, _ = out[(_lib.GH_PREC_FP32_ELEM, _lib.GH_PATH_AUTO)]
+    assert relfrob(nz32, nzr) <= 1e-6 and relmax(b32, br) <= 1e-12   # residual in FP64
+    nzc, bc, kc = out[(_lib.GH_PREC_FP64, _lib.GH_PATH_GENERAL)]
+    assert kc == _lib.GH_KERNEL_CELL
+    assert relmax(nzc, nz) <= 1e-13 and relmax(bc, b) <= 1e-13
