@@ -15,7 +15,7 @@ OUT_DIR = os.path.join(HERE, "gridap_hip")
 LIB = os.path.join(OUT_DIR, "libgridap_hip.so")
 OBJ_DIR = os.path.join(HERE, "_build")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-SOURCES = ["gh_api.hip", "pattern.hip", "assemble.hip", "solve.hip", "fsai.hip", "simplex.hip", "hexq2.hip", "nonlinear.hip", "boundary.hip", "hexq2_lattice.hip", "hexgen.hip"]
+SOURCES = ["gh_api.hip", "pattern.hip", "assemble.hip", "solve.hip", "fsai.hip", "simplex.hip", "hexq2.hip", "nonlinear.hip", "boundary.hip", "hexq2_lattice.hip", "hexgen.hip", "rect.hip"]
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
     # keep the compiler from emitting packed-FP32 VALU instructions

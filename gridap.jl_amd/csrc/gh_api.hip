@@ -296,10 +296,73 @@ int gh_space_upload(gh_ctx *ctx, int reffe_kind, int order, int ncomp, const int
                            hipMemcpyHostToDevice));
     c.h_cell_dofs.swap(cd);
     c.have_space = true;
+    c.have_trial = false;  // a new test space ends the previous rectangular pair
+    c.tcell_dofs.release();
+    c.tshp.clear();
+    c.tgrad.clear();
+    c.tnq = 0;
     invalidate_pattern(c);
     c.have_dvals = false;
     c.have_state = false;
     clear_boundary(c);
+  });
+}
+
+int gh_trial_space_upload(gh_ctx *ctx, int reffe_kind, int order, int ncomp,
+                          const int32_t *data, const int32_t *ptrs, int64_t ncells,
+                          int64_t nfree, int64_t ndiri) {
+  return guarded(ctx, [&](gh::Ctx &c) {
+    GH_REQUIRE(c.have_space, GH_ERR_STATE, "upload the test space before the trial space");
+    GH_REQUIRE(ncells == c.ncells, GH_ERR_ARG, "cell count differs from the mesh");
+    GH_REQUIRE(data && ptrs, GH_ERR_ARG, "NULL cell_dof_ids array");
+    GH_REQUIRE(reffe_kind == GH_REFFE_QK && c.reffe_kind == GH_REFFE_QK && ncomp == 1 &&
+                   c.ncomp == 1 && order >= 1 && order <= 2 && c.order >= 1 && c.order <= 2,
+               GH_ERR_UNSUPPORTED,
+               "rectangular pairs are provided between scalar Q1/Q2 spaces on quadrilaterals "
+               "and hexahedra");
+    GH_REQUIRE(nfree >= 0 && ndiri >= 0 && nfree < (int64_t(1) << 31) &&
+                   ndiri < (int64_t(1) << 31),
+               GH_ERR_ARG, "dof counts out of range");
+    GH_REQUIRE(ptrs[0] == 1, GH_ERR_ARG, "cell_dof_ids ptrs must start at 1");
+    const int nloc = ptrs[1] - ptrs[0];
+    GH_REQUIRE(nloc >= 1 && nloc <= 256, GH_ERR_UNSUPPORTED, "unsupported dofs per cell");
+    std::vector<int32_t> cd(size_t(ncells) * nloc);
+    for (int64_t k = 0; k < ncells; ++k) {
+      GH_REQUIRE(ptrs[k + 1] - ptrs[k] == nloc, GH_ERR_UNSUPPORTED,
+                 "cells with different numbers of dofs are not supported");
+      const int32_t *src = data + (ptrs[k] - 1);
+      for (int a = 0; a < nloc; ++a)
+        GH_REQUIRE(src[a] != 0 && src[a] <= nfree && -src[a] <= ndiri, GH_ERR_RANGE,
+                   "cell dof id out of range");
+      std::memcpy(&cd[size_t(k) * nloc], src, sizeof(int32_t) * nloc);
+    }
+    c.torder = order;
+    c.tnloc = nloc;
+    c.tnfree = nfree;
+    c.tndiri = ndiri;
+    c.tcell_dofs.resize(cd.size());
+    GH_CHECK_HIP(hipMemcpy(c.tcell_dofs.get(), cd.data(), sizeof(int32_t) * cd.size(),
+                           hipMemcpyHostToDevice));
+    c.tshp.clear();
+    c.tgrad.clear();
+    c.tnq = 0;
+    c.have_trial = true;
+    invalidate_pattern(c);
+    c.have_dvals = false;
+    c.have_state = false;
+    clear_boundary(c);
+  });
+}
+
+int gh_trial_tabulation(gh_ctx *ctx, int nq, int nloc, const double *shp, const double *grad) {
+  return guarded(ctx, [&](gh::Ctx &c) {
+    GH_REQUIRE(c.have_trial, GH_ERR_STATE, "upload the trial space first");
+    GH_REQUIRE(nq >= 1 && nloc == c.tnloc && shp && grad, GH_ERR_ARG,
+               "the trial tabulation needs nq >= 1 points and the trial space's dofs per cell");
+    c.tshp.assign(shp, shp + (size_t)nq * nloc);
+    c.tgrad.assign(grad, grad + (size_t)nq * nloc * c.dim);
+    c.tnq = nq;
+    c.have_values = false;
   });
 }
 
@@ -373,6 +436,8 @@ int gh_boundary_upload(gh_ctx *ctx, int64_t nfaces, const int32_t *face_cells,
                        const int32_t *face_sides, const double *alpha, const double *beta,
                        const double *gq, const double *hq) {
   return guarded(ctx, [&](gh::Ctx &c) {
+    GH_REQUIRE(!c.have_trial, GH_ERR_UNSUPPORTED,
+               "boundary terms belong to square (same-space) pairs");
     GH_REQUIRE(c.have_space, GH_ERR_STATE, "upload the FE space before boundary terms");
     GH_REQUIRE(nfaces >= 0 && nfaces < (int64_t(1) << 31), GH_ERR_ARG, "bad face count");
     clear_boundary(c);
@@ -418,6 +483,11 @@ int gh_symbolic(gh_ctx *ctx, int64_t *nnz_out) {
                    c.h_cell_dofs.size() == c.cell_dofs.n,
                GH_ERR_STATE, "the FE space does not match the mesh: upload it again");
     invalidate_pattern(c);
+    if (c.have_trial) {
+      gh::rect_symbolic(c);
+      if (nnz_out) *nnz_out = c.nnz;
+      return;
+    }
     if (!gh::symbolic_lattice(c)) {
       gh::symbolic_general(c, gh::simplex_eligible(c) || gh::hexq2_eligible(c) ||
                                   gh::hexgen_eligible(c));
@@ -432,7 +502,7 @@ int gh_symbolic(gh_ctx *ctx, int64_t *nnz_out) {
 int gh_pattern_get(gh_ctx *ctx, int64_t *colptr, int64_t *rowval) {
   return guarded(ctx, [&](gh::Ctx &c) {
     GH_REQUIRE(c.have_pattern, GH_ERR_STATE, "no pattern: call gh_symbolic first");
-    std::vector<int32_t> cp(c.nfree + 1), rv(c.nnz);
+    std::vector<int32_t> cp((c.have_trial ? c.tnfree : c.nfree) + 1), rv(c.nnz);
     GH_CHECK_HIP(hipMemcpyAsync(cp.data(), c.colptr.get(), sizeof(int32_t) * cp.size(),
                                 hipMemcpyDeviceToHost, c.stream));
     if (c.nnz)
@@ -459,7 +529,8 @@ int gh_coef_upload(gh_ctx *ctx, const double *kappa, const double *fsrc,
     };
     up(c.kappa, c.have_kappa, kappa, c.ncells);
     up(c.fsrc, c.have_fsrc, fsrc, c.ncells);
-    up(c.dvals, c.have_dvals, dirichlet_vals, c.ndiri);
+    // Dirichlet values of the trial space (the test space's own for a square pair)
+    up(c.dvals, c.have_dvals, dirichlet_vals, c.have_trial ? c.tndiri : c.ndiri);
     GH_CHECK_HIP(hipStreamSynchronize(c.stream));
   });
 }
@@ -484,6 +555,8 @@ int gh_mass_upload(gh_ctx *ctx, const double *sigma) {
 
 int gh_source_qp_upload(gh_ctx *ctx, const double *fq) {
   return guarded(ctx, [&](gh::Ctx &c) {
+    GH_REQUIRE(!c.have_trial || !fq, GH_ERR_UNSUPPORTED,
+               "sources belong to the test field's own (square) pair");
     GH_REQUIRE(c.have_space && c.nq > 0, GH_ERR_STATE,
                "upload the FE space and the tabulation before the source");
     c.have_fq = fq != nullptr;
@@ -513,7 +586,14 @@ int gh_assemble(gh_ctx *ctx, int form_kind, int precision, int path) {
                GH_ERR_ARG, "unknown numeric path");
     GH_REQUIRE(c.have_pattern, GH_ERR_STATE, "no pattern: call gh_symbolic first");
     GH_REQUIRE(c.nq > 0, GH_ERR_STATE, "no reference tabulation uploaded");
-    if (form_kind == GH_FORM_PLAPLACIAN) {
+    if (c.have_trial) {
+      GH_REQUIRE(form_kind == GH_FORM_DIFFUSION && path != GH_PATH_STRUCTURED,
+                 GH_ERR_UNSUPPORTED,
+                 "a rectangular pair assembles κ ∇ψ⋅∇φ + σ ψ φ on the general path");
+      GH_REQUIRE(!c.have_fsrc && !c.have_fq, GH_ERR_UNSUPPORTED,
+                 "sources belong to the test field's own (square) pair");
+      gh::rect_assemble(c, precision);
+    } else if (form_kind == GH_FORM_PLAPLACIAN) {
       gh::assemble_plaplacian(c, precision, path);
     } else if (form_kind == GH_FORM_ELASTICITY) {
       if (gh::hexgen_eligible(c)) {
@@ -551,6 +631,7 @@ int gh_elasticity_params(gh_ctx *ctx, double nu, const double *body_force) {
 
 int gh_fefunction_upload(gh_ctx *ctx, const double *free_vals, const double *dirichlet_vals) {
   return guarded(ctx, [&](gh::Ctx &c) {
+    GH_REQUIRE(!c.have_trial, GH_ERR_UNSUPPORTED, "nonlinear forms need a square pair");
     GH_REQUIRE(c.have_space, GH_ERR_STATE, "upload the FE space first");
     GH_REQUIRE(free_vals || c.nfree == 0, GH_ERR_ARG, "NULL free values");
     c.ustate.resize(std::max<int64_t>(c.nfree, 1));
@@ -635,6 +716,7 @@ int gh_numeric(gh_ctx *ctx, int form_kind, const double *cell_coef, const double
 
 int gh_spmv(gh_ctx *ctx, const double *x, double *y) {
   return guarded(ctx, [&](gh::Ctx &c) {
+    GH_REQUIRE(!c.have_trial, GH_ERR_UNSUPPORTED, "SpMV needs a square matrix (no trial space)");
     GH_REQUIRE(c.have_values, GH_ERR_STATE, "no assembled matrix");
     GH_REQUIRE(x && y, GH_ERR_ARG, "NULL vector");
     c.s_x.resize(c.nfree);
@@ -651,6 +733,7 @@ int gh_spmv(gh_ctx *ctx, const double *x, double *y) {
 int gh_pcg(gh_ctx *ctx, int precond, const double *b, double *x, double rtol, int maxit,
            gh_solve_stats *stats) {
   int st = guarded(ctx, [&](gh::Ctx &c) {
+    GH_REQUIRE(!c.have_trial, GH_ERR_UNSUPPORTED, "PCG needs a square matrix (no trial space)");
     GH_REQUIRE(c.have_values, GH_ERR_STATE, "no assembled matrix");
     GH_REQUIRE(x, GH_ERR_ARG, "NULL solution vector");
     GH_REQUIRE(precond == GH_PRECOND_NONE || precond == GH_PRECOND_JACOBI ||
@@ -697,6 +780,7 @@ int gh_pcg(gh_ctx *ctx, int precond, const double *b, double *x, double rtol, in
 int gh_pcg_noalloc(gh_ctx *ctx, int precond, const double *b, double *x, double rtol,
                    int maxit, gh_solve_stats *stats) {
   return guarded(ctx, [&](gh::Ctx &c) {
+    GH_REQUIRE(!c.have_trial, GH_ERR_UNSUPPORTED, "PCG needs a square matrix (no trial space)");
     GH_REQUIRE(c.have_values, GH_ERR_STATE, "no assembled matrix");
     GH_REQUIRE(x, GH_ERR_ARG, "NULL solution vector");
     GH_REQUIRE(rtol > 0 && maxit >= 0, GH_ERR_ARG, "bad tolerance or iteration cap");
@@ -798,6 +882,7 @@ int gh_grf_kappa(gh_ctx *ctx, int nmodes, const double *omega, const double *pha
 
 int gh_fsai_build(gh_ctx *ctx) {
   return guarded(ctx, [&](gh::Ctx &c) {
+    GH_REQUIRE(!c.have_trial, GH_ERR_UNSUPPORTED, "FSAI needs a square matrix (no trial space)");
     GH_REQUIRE(c.have_values, GH_ERR_STATE, "assemble the matrix first");
     gh::fsai_forward_build(c, true);
     GH_CHECK_HIP(hipStreamSynchronize(c.stream));
@@ -806,6 +891,7 @@ int gh_fsai_build(gh_ctx *ctx) {
 
 int gh_fsai_train_step(gh_ctx *ctx, int nprobes, uint64_t seed, float lr, double *loss) {
   return guarded(ctx, [&](gh::Ctx &c) {
+    GH_REQUIRE(!c.have_trial, GH_ERR_UNSUPPORTED, "FSAI needs a square matrix (no trial space)");
     GH_REQUIRE(c.have_values, GH_ERR_STATE, "assemble the matrix first");
     GH_REQUIRE(lr >= 0.0f && std::isfinite(lr), GH_ERR_ARG, "bad learning rate");
     const double l = gh::fsai_train_step(c, nprobes, seed, lr);

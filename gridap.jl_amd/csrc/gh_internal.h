@@ -159,6 +159,15 @@ struct Ctx {
   DBuf<int32_t> cell_dofs;          // ncells x nloc, Gridap signed 1-based ids
   std::vector<int32_t> h_cell_dofs; // host copy (colouring)
 
+  // ---- trial space of a rectangular field pair (gh_trial_space_upload): the columns,
+  //      while the space above gives the rows ----
+  bool have_trial = false;
+  int torder = 0, tnloc = 0;
+  int64_t tnfree = 0, tndiri = 0;
+  DBuf<int32_t> tcell_dofs;         // ncells x tnloc, Gridap signed 1-based ids
+  int tnq = 0;                      // points of the trial tabulation
+  std::vector<double> tshp, tgrad;  // trial values and reference gradients there
+
   // ---- reference tabulation ----
   int nq = 0;
   std::vector<double> qw, shp, grad;
@@ -318,6 +327,8 @@ void hexgen_assemble(Ctx &c, int precision, bool elasticity);
 // b[j] = Σ over j's adjacency (ascending cells) of the cell loads in c.evec
 // ([cell][local], FP64); enqueued on c.stream.
 void hexgen_rhs(Ctx &c);
+void rect_symbolic(Ctx &c);                  // pattern of a rectangular field pair
+void rect_assemble(Ctx &c, int precision);   // its values and lifting loads
 void elemlist_symbolic(Ctx &c);         // dof -> cell adjacency, Dirichlet masks
 void elemlist_sum(Ctx &c, bool f32);     // nzval from the element entries
 void assemble_simplex(Ctx &c, int precision);

@@ -68,6 +68,8 @@ SIGNATURES = {
     "gh_mesh_cartesian": (_i32, [_vp, _i32, _vp, _vp, _vp]),
     "gh_mesh_upload": (_i32, [_vp, _i32, _i64, _vp, _i64, _vp, _vp]),
     "gh_space_upload": (_i32, [_vp, _i32, _i32, _i32, _vp, _vp, _i64, _i64, _i64]),
+    "gh_trial_space_upload": (_i32, [_vp, _i32, _i32, _i32, _vp, _vp, _i64, _i64, _i64]),
+    "gh_trial_tabulation": (_i32, [_vp, _i32, _i32, _vp, _vp]),
     "gh_reffe_tabulation": (_i32, [_vp, _i32, _i32, _vp, _vp, _vp]),
     "gh_geometry_tabulation": (_i32, [_vp, _i32, _i32, _vp]),
     "gh_face_tabulation": (_i32, [_vp, _i32, _i32, _vp, _vp, _vp]),
@@ -146,6 +148,7 @@ class Context:
         self.device = device
         self.nfree = 0  # sizes of the arrays the library returns, tracked here so the
         self.nnz = 0    # host buffers always match what the library writes
+        self.ncols = 0  # matrix columns: nfree, or the trial space's free dofs
         self.ncells = 0
 
     def check(self, st):
@@ -192,7 +195,26 @@ class Context:
         self.check(lib().gh_space_upload(self.h, reffe_kind, order, ncomp, ptr(d), ptr(p),
                                          len(p) - 1, int(nfree), int(ndiri)))
         self.nfree = int(nfree)
+        self.ncols = int(nfree)
         self.nnz = 0
+
+    def trial_space_upload(self, reffe_kind, order, ncomp, data, ptrs, nfree, ndiri):
+        """gh_trial_space_upload: the columns of a rectangular field pair (the space of
+        space_upload gives the rows)."""
+        d = np.ascontiguousarray(data, dtype=np.int32)
+        p = np.ascontiguousarray(ptrs, dtype=np.int32)
+        self.check(lib().gh_trial_space_upload(self.h, reffe_kind, order, ncomp, ptr(d), ptr(p),
+                                               len(p) - 1, int(nfree), int(ndiri)))
+        self.ncols = int(nfree)
+        self.nnz = 0
+
+    def trial_tabulation(self, shp, grad):
+        """The trial basis at the points of reffe_tabulation: shp [nq, nloc], grad
+        [nq, nloc, dim] (or [nq, nloc*dim])."""
+        shp = np.ascontiguousarray(shp, dtype=np.float64)
+        grad = np.ascontiguousarray(grad, dtype=np.float64)
+        self.check(lib().gh_trial_tabulation(self.h, shp.shape[0], shp.shape[1], ptr(shp),
+                                             ptr(grad)))
 
     def reffe_tabulation(self, qw, shp, grad):
         qw = np.ascontiguousarray(qw, dtype=np.float64)
@@ -234,7 +256,7 @@ class Context:
         return nnz.value
 
     def pattern_get(self):
-        colptr = np.zeros(self.nfree + 1, dtype=np.int64)
+        colptr = np.zeros(self.ncols + 1, dtype=np.int64)
         rowval = np.zeros(max(self.nnz, 1), dtype=np.int64)
         self.check(lib().gh_pattern_get(self.h, ptr(colptr), ptr(rowval)))
         return colptr, rowval[:self.nnz]

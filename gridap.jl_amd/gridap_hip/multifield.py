@@ -21,7 +21,11 @@ pair on it), and the pairs are laid out as Gridap lays them out:
 
 The right-hand side of test field i is its source terms minus the lifting
 Σ_j A_ij(:, D_j) u_Dj of every touched pair (attach_dirichlet per field pair).
-Pairs between two different spaces (rectangular blocks) are not provided.
+Pairs between two different spaces on one mesh (rectangular blocks, e.g. a Q2 and a Q1
+scalar field coupled by ∫ σ u1 v2 or ∫ κ ∇u1⋅∇v2) are assembled on the device by a
+pair context that holds the test space as rows and the trial space as columns
+(gh_trial_space_upload); vector-valued fields in rectangular pairs (Taylor–Hood
+velocity-pressure blocks) are not provided.
 """
 import numpy as np
 
@@ -185,6 +189,98 @@ def compose_blocks(sub, row_fields, col_fields, nrows, ncols, plan=None):
     return SparseMatrixCSC(m, n, cp.copy(), rv.copy(), nz)
 
 
+class RectangularPairAssembler:
+    """The block (i, j) of a multi-field form whose test field i and trial field j live
+    in different spaces on one mesh: rows are Vi's free dofs, columns Uj's
+    (SparseMatrixAssemblers.jl:174-248 with cell_rows and cell_cols from two spaces).
+    Cell terms κ ∇u⋅∇v and σ u v (κ, σ constant per cell); the lifting of Uj's
+    Dirichlet values goes to the test field's right-hand side."""
+
+    def __init__(self, U, V, device=0, precision="fp64"):
+        from . import _lib
+        self.U, self.V = U, V
+        self.device = device
+        self.precision = {"fp64": _lib.GH_PREC_FP64, "fp32": _lib.GH_PREC_FP32_ELEM}[precision] \
+            if isinstance(precision, str) else precision
+        self.ctx = None
+        self._degree = None
+        self.colptr = self.rowval = None
+
+    def _setup(self, degree):
+        from . import _lib
+        from .reference_fes import Float64, Quadrature, ReferenceFE, lagrangian
+        Vs, Us = _underlying(self.V), _underlying(self.U)
+        if getattr(Vs, "simplex", False) or getattr(Us, "simplex", False):
+            raise NotImplementedError("rectangular pairs are provided on quadrilaterals and "
+                                      "hexahedra")
+        model = Vs.model
+        if self.ctx is None:
+            ctx = _lib.Context(self.device)
+            if model.is_affine_cartesian():
+                ctx.mesh_cartesian(model.origin, model.sizes, model.partition)
+            else:
+                cn = model.cell_node_ids()
+                ptrs = (np.arange(cn.shape[0] + 1, dtype=np.int64) * cn.shape[1] + 1)
+                ctx.mesh_upload(model.node_coordinates(), cn.ravel(), ptrs.astype(np.int32))
+            for up, S in ((ctx.space_upload, Vs), (ctx.trial_space_upload, Us)):
+                cd = np.asarray(S.cell_dof_ids)
+                ptrs = (np.arange(cd.shape[0] + 1, dtype=np.int64) * cd.shape[1] + 1)
+                up(_lib.GH_REFFE_QK, S.order, S.ncomp, cd.ravel(), ptrs.astype(np.int32),
+                   S.nfree, S.ndirichlet)
+            ctx.symbolic()
+            self.colptr, self.rowval = ctx.pattern_get()
+            self.ctx = ctx
+        if degree != self._degree:
+            D = model.D
+            q = Quadrature(D, degree)
+            shp, grad = Vs.reffe.tabulate(D, q.coords)
+            self.ctx.reffe_tabulation(q.weights, shp, grad)
+            tshp, tgrad = Us.reffe.tabulate(D, q.coords)
+            self.ctx.trial_tabulation(tshp, tgrad)
+            if not model.is_affine_cartesian():
+                _, ggrad = ReferenceFE(lagrangian, Float64, 1).tabulate(D, q.coords)
+                self.ctx.geometry_tabulation(ggrad)
+            self._degree = degree
+
+    def assemble(self, matcontrib, veccontrib, lift, trial=None):
+        from . import _lib
+        from .assemblers import SparseMatrixCSC
+        from .celldata import DiffusionTerm, MassTerm, _cell_coef
+        from .fespaces import TrialFESpace
+        if veccontrib is not None:
+            raise NotImplementedError("sources belong to the test field's own pair")
+        ncells = _underlying(self.V).model.num_cells()
+        kappa = sigma = None
+        degs = set()
+        for term, meas in (matcontrib.terms if matcontrib is not None else []):
+            if not isinstance(term, (DiffusionTerm, MassTerm)):
+                raise NotImplementedError("rectangular pairs: κ ∇u⋅∇v and σ u v cell terms")
+            coef = _cell_coef(term.coef)
+            k = coef.per_cell(ncells) if coef is not None else None
+            k = np.ones(ncells) if k is None else k
+            if isinstance(term, DiffusionTerm):
+                kappa = k if kappa is None else kappa + k
+            else:
+                sigma = k if sigma is None else sigma + k
+            degs.add(meas.degree)
+        if len(degs) != 1:
+            raise NotImplementedError("the terms of a block must share one Measure")
+        self._setup(degs.pop())
+        dvals = None
+        U = self.U if trial is None else trial
+        if lift and isinstance(U, TrialFESpace):
+            dv = U.dirichlet_values()
+            if dv is not None and dv.size and np.any(dv != 0):
+                dvals = dv
+        self.ctx.coef_upload(kappa, None, dvals)
+        self.ctx.mass_upload(sigma)
+        self.ctx.assemble(_lib.GH_FORM_DIFFUSION, self.precision, _lib.GH_PATH_GENERAL)
+        nz, b = self.ctx.values_get()
+        Vs, Us = _underlying(self.V), _underlying(self.U)
+        A = SparseMatrixCSC(Vs.nfree, Us.nfree, self.colptr.copy(), self.rowval.copy(), nz)
+        return A, b
+
+
 class MultiFieldSparseMatrixAssembler:
     """SparseMatrixAssembler(X, Y) for multi-field spaces: the touched field pairs on
     the device, laid out by the style of Y (and X)."""
@@ -199,6 +295,7 @@ class MultiFieldSparseMatrixAssembler:
         self.U, self.V = U, V
         self.device, self.precision, self.path = device, precision, path
         self._assems = []   # (test space, single-field assembler)
+        self._rects = []    # ((test space, trial space), rectangular pair assembler)
         self._plans = {}    # compose_plan per (row fields, column fields, touched pairs)
 
     def _compose(self, sub, rg, cg, nrows, ncols):
@@ -218,8 +315,13 @@ class MultiFieldSparseMatrixAssembler:
         from .assemblers import SparseMatrixAssembler
         Vi, Uj = self.V.spaces[i], self.U.spaces[j]
         if not _same_space(Vi, Uj):
-            raise NotImplementedError("blocks between two different FE spaces (rectangular "
-                                      "blocks) are not provided")
+            for (test, trial), assem in self._rects:
+                if _same_space(test, Vi) and _same_space(trial, Uj):
+                    return assem
+            assem = RectangularPairAssembler(Uj, Vi, device=self.device,
+                                             precision=self.precision)
+            self._rects.append(((Vi, Uj), assem))
+            return assem
         for space, assem in self._assems:
             if _same_space(space, Vi):
                 return assem

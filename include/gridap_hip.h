@@ -118,6 +118,26 @@ int gh_space_upload(gh_ctx *ctx, int reffe_kind, int order, int ncomp,
                     const int32_t *cell_dofs_data, const int32_t *cell_dofs_ptrs,
                     int64_t ncells, int64_t nfree, int64_t ndiri);
 
+/* Trial space of a rectangular field pair: the columns, while gh_space_upload's space
+ * gives the rows (the block (i, j) of a multi-field form whose test field i and trial
+ * field j live in different spaces on one mesh; the cols of
+ * SparseMatrixAssemblers.jl:174-248 when cell_cols differ from cell_rows, and
+ * src/MultiField/BlockSparseMatrixAssemblers.jl:19-36).  Scalar Q1/Q2 spaces on
+ * quadrilaterals and hexahedra.  gh_symbolic then builds the (test nfree) x (trial
+ * nfree) pattern (empty columns allowed), gh_coef_upload's Dirichlet values are the
+ * trial space's, and gh_assemble (GH_FORM_DIFFUSION) writes
+ *   B_c[lj][li] = ∫ κ ∇ψ_li⋅∇φ_lj + σ ψ_li φ_lj   (κ: gh_coef_upload, σ: gh_mass_upload)
+ * into nzval and the lifting -B u_D into b.  Uploading a test space again ends the pair. */
+int gh_trial_space_upload(gh_ctx *ctx, int reffe_kind, int order, int ncomp,
+                          const int32_t *cell_dofs_data, const int32_t *cell_dofs_ptrs,
+                          int64_t ncells, int64_t nfree, int64_t ndiri);
+
+/* The trial basis at the points of the test tabulation (gh_reffe_tabulation): values
+ * shp[q*nloc + a] and reference gradients grad[(q*nloc + a)*dim + d] (the trial half
+ * of the cell field pair that IntegrationMap evaluates, src/Fields/FieldsInterfaces.jl:
+ * 737-760). */
+int gh_trial_tabulation(gh_ctx *ctx, int nq, int nloc, const double *shp, const double *grad);
+
 /* Reference tabulation, computed once per reference cell by the host:
  * quadrature weights qw[nq], shape values shp[nq*nloc] and reference gradients
  * grad[nq*nloc*dim] (point-major).  Replaces the per-cell-type evaluation of

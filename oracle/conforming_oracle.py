@@ -599,6 +599,24 @@ def ncube_loads(wdet, shp, values):
     return np.concatenate([v[:, k:k + 1] * m for k in range(v.shape[1])], axis=1)
 
 
+def mixed_cells(wdet, g_test, g_trial, shp_test, shp_trial, kappa=None, sigma=None):
+    """[C, nt, ns] cell matrices of a rectangular field pair (test basis ψ, trial basis φ
+    on one cell, IntegrationMap of src/Fields/FieldsInterfaces.jl:737-760 with the two
+    bases of a MultiFieldCellField block): entry [c, lj, li] =
+    Σ_q w|det J| (κ_c ∇ψ_li·∇φ_lj + σ_c ψ_li φ_lj); g_*: [C, nq, n, D] physical gradients
+    (physical_gradients), shp_*: [nq, n]; κ, σ per cell or None (term absent)."""
+    C = wdet.shape[0]
+    out = np.zeros((C, g_trial.shape[2], g_test.shape[2]))
+    if kappa is not None:
+        out += np.asarray(kappa, dtype=np.float64)[:, None, None] * \
+            np.einsum("cq,cqbd,cqad->cba", wdet, g_trial, g_test)
+    if sigma is not None:
+        out += np.asarray(sigma, dtype=np.float64)[:, None, None] * \
+            np.einsum("cq,qb,qa->cba", wdet, np.asarray(shp_trial, dtype=np.float64),
+                      np.asarray(shp_test, dtype=np.float64))
+    return out
+
+
 def q1_map_points(X, n1):
     """Physical coordinates [C, npts, D] of reference points under each cell's Q1 map
     (n1: [npts, nv] Q1 shape values at the points)."""

@@ -206,3 +206,73 @@ def test_mass_term_3d_matches_oracle(oracle):
     uh = G.solve(opm)
     r = opm.get_vector() - opm.get_matrix().to_scipy() @ uh.free_values
     assert np.linalg.norm(r) <= 1e-10 * np.linalg.norm(opm.get_vector())
+
+
+def _mixed_system(part, mapped, precision="fp64", style=None):
+    """A Q2 field and a Q1 field on one mesh (Dirichlet data on both) coupled both ways:
+    a = ∫ ∇u1⋅∇v1 + u2 v2 - u1 v2 + 0.5 ∇u2⋅∇v1, l = ∫ v1 - v2.  The blocks (2, 1) and
+    (1, 2) pair two different spaces (rectangular)."""
+    D = len(part)
+    warp = lambda x: x + 0.04 * np.prod(np.sin(np.pi * x), axis=-1)[..., None]
+    model = G.CartesianDiscreteModel((0, 1) * D, part, map=warp if mapped else None)
+    V1 = G.TestFESpace(model, G.ReferenceFE(G.lagrangian, G.Float64, 2), dirichlet_tags="boundary")
+    V2 = G.TestFESpace(model, G.ReferenceFE(G.lagrangian, G.Float64, 1), dirichlet_tags="boundary")
+    U1 = G.TrialFESpace(V1, lambda x: x[..., 0] + x[..., 1] ** 2)
+    U2 = G.TrialFESpace(V2, lambda x: 1.0 - x[..., 0])
+    dΩ = G.Measure(G.Triangulation(model), 4)
+    kw = {} if style is None else {"style": style}
+    X, Y = G.MultiFieldFESpace([U1, U2], **kw), G.MultiFieldFESpace([V1, V2], **kw)
+    a = lambda u, v: G.integrate(G.inner(G.gradient(u[0]), G.gradient(v[0])) +
+                                 G.inner(u[1], v[1]) - G.inner(u[0], v[1]) +
+                                 0.5 * G.inner(G.gradient(u[1]), G.gradient(v[0])), dΩ)
+    l = lambda v: G.integrate(v[0] * 1.0 - v[1] * 1.0, dΩ)
+    return G.assemble_matrix_and_vector(a, l, X, Y, G.SparseMatrixAssembler(X, Y, precision=precision)), \
+        (model, V1, V2, U1, U2)
+
+
+def _mixed_oracle(oracle, model, V1, V2, U1, U2):
+    import conforming_oracle as CO
+    D = model.D
+    X = model.node_coordinates()[model.cell_node_ids() - 1]
+    qx, qw = oracle.tensor_quadrature(D, 4)
+    s1, g1 = oracle.qk_tabulate(D, 1, qx)
+    s2, g2 = oracle.qk_tabulate(D, 2, qx)
+    wdet, gp1 = CO.physical_gradients(X, qw, g1, g1)
+    _, gp2 = CO.physical_gradients(X, qw, g1, g2)
+    C = X.shape[0]
+    one = np.ones(C)
+    mats = {(0, 0): CO.mixed_cells(wdet, gp2, gp2, s2, s2, kappa=one),
+            (1, 1): CO.mixed_cells(wdet, gp1, gp1, s1, s1, sigma=one),
+            (1, 0): -CO.mixed_cells(wdet, gp1, gp2, s1, s2, sigma=one),
+            (0, 1): 0.5 * CO.mixed_cells(wdet, gp2, gp1, s2, s1, kappa=one)}
+    vecs = {0: CO.ncube_loads(wdet, s2, one[:, None]), 1: -CO.ncube_loads(wdet, s1, one[:, None])}
+    cds = [np.asarray(V1.cell_dof_ids), np.asarray(V2.cell_dof_ids)]
+    return MF.assemble(cds, [V1.nfree, V2.nfree], [V1.ndirichlet, V2.ndirichlet], mats, vecs,
+                       dvals=[U1.dirichlet_values(), U2.dirichlet_values()])
+
+
+@pytest.mark.parametrize("part,mapped", [((5, 4), False), ((4, 3, 3), True)])
+def test_mixed_order_fields_match_oracle(oracle, part, mapped):
+    """Rectangular blocks between a Q2 and a Q1 field (consecutive layout): pattern
+    bit-exact, values and b within 1e-12 of the oracle (FP64), 1e-6 (FP32 element
+    math); the block layout equals the consecutive matrix entry for entry."""
+    announce("mixed-order fields (Q2, Q1) %s mapped=%s" % (part, mapped))
+    (A, b), spaces = _mixed_system(part, mapped)
+    cp, rv, nz, br = _mixed_oracle(oracle, *spaces)
+    np.testing.assert_array_equal(A.colptr, cp)
+    np.testing.assert_array_equal(A.rowval, rv)
+    assert relmax(A.nzval, nz) <= 1e-12 and relmax(b, br) <= 1e-12
+    (A32, b32), _ = _mixed_system(part, mapped, precision="fp32")
+    assert np.linalg.norm(A32.nzval - nz) <= 1e-6 * np.linalg.norm(nz)
+    assert relmax(b32, br) <= 1e-6
+    (Ab, bb), _ = _mixed_system(part, mapped, style=G.BlockMultiFieldStyle())
+    n1 = spaces[1].nfree
+    S = A.to_scipy().tocsc()
+    for (i, j), (rs, cs) in {(0, 0): (slice(0, n1), slice(0, n1)),
+                             (0, 1): (slice(0, n1), slice(n1, None)),
+                             (1, 0): (slice(n1, None), slice(0, n1)),
+                             (1, 1): (slice(n1, None), slice(n1, None))}.items():
+        blk = Ab.blocks[i][j].to_scipy().tocsc()
+        ref = S[rs, cs].tocsc()
+        assert blk.shape == ref.shape and abs(blk - ref).max() == 0, (i, j)
+    np.testing.assert_array_equal(np.concatenate(bb.blocks), b)

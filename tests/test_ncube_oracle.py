@@ -108,3 +108,32 @@ def test_mapped_elasticity_rigid_motions_and_trace(oracle, D, order):
     r = np.zeros((len(cn), D * nn))
     r[:, :nn] = P[:, :, 0]
     assert np.abs(np.einsum("cab,cb->ca", K, r)).max() > 1e-3 * np.abs(K).max()
+
+
+@pytest.mark.parametrize("D", [2, 3])
+def test_mixed_cells_pins(oracle, D):
+    """mixed_cells (the cell matrices of a rectangular field pair): with one space for
+    test and trial it is the square diffusion + mass matrix; with a Q2 test space and a
+    Q1 trial space, every Q1 basis function is its Q2 interpolant (Q1 ⊂ Q2), so the
+    block equals the square Q2 matrix times the interpolation matrix N[a, b] =
+    φ^Q1_b(x^Q2_a) — a transposed operand or a swapped index fails it."""
+    part = (3, 2, 2)[:D]
+    m, xyz, cn, X = cells_of(oracle, D, part, True)
+    qx, qw = oracle.tensor_quadrature(D, 4)
+    s1, g1 = oracle.qk_tabulate(D, 1, qx)
+    s2, g2 = oracle.qk_tabulate(D, 2, qx)
+    wdet, gp1 = CO.physical_gradients(X, qw, g1, g1)
+    _, gp2 = CO.physical_gradients(X, qw, g1, g2)
+    rng = np.random.default_rng(D)
+    kappa, sigma = np.exp(rng.normal(size=len(cn))), np.exp(rng.normal(size=len(cn)))
+    sq = CO.mixed_cells(wdet, gp1, gp1, s1, s1, kappa, sigma)
+    ref = CO.ncube_diffusion_cells(wdet, gp1, kappa) + \
+        sigma[:, None, None] * np.einsum("cq,qa,qb->cab", wdet, s1, s1)
+    np.testing.assert_allclose(sq, ref, rtol=0, atol=1e-13 * np.abs(ref).max())
+    B = CO.mixed_cells(wdet, gp2, gp1, s2, s1, kappa, sigma)          # [C, n1, n2]
+    K2 = CO.mixed_cells(wdet, gp2, gp2, s2, s2, kappa, sigma)         # [C, n2, n2]
+    N, _ = oracle.qk_tabulate(D, 1, oracle.qk_nodes(D, 2))           # [n2 nodes, n1]
+    # φ^Q1_b = Σ_a N[a, b] ψ^Q2_a, so B[c, b, li] = Σ_a N[a, b] K2[c, a, li]
+    np.testing.assert_allclose(B, np.einsum("ab,cal->cbl", N, K2), rtol=0,
+                               atol=1e-12 * np.abs(K2).max())
+    assert B.shape == (len(cn), 2 ** D, 3 ** D)
