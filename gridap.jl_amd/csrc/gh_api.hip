@@ -315,11 +315,14 @@ int gh_trial_space_upload(gh_ctx *ctx, int reffe_kind, int order, int ncomp,
     GH_REQUIRE(c.have_space, GH_ERR_STATE, "upload the test space before the trial space");
     GH_REQUIRE(ncells == c.ncells, GH_ERR_ARG, "cell count differs from the mesh");
     GH_REQUIRE(data && ptrs, GH_ERR_ARG, "NULL cell_dof_ids array");
-    GH_REQUIRE(reffe_kind == GH_REFFE_QK && c.reffe_kind == GH_REFFE_QK && ncomp == 1 &&
-                   c.ncomp == 1 && order >= 1 && order <= 2 && c.order >= 1 && c.order <= 2,
+    GH_REQUIRE(reffe_kind == GH_REFFE_QK && c.reffe_kind == GH_REFFE_QK && order >= 1 &&
+                   order <= 2 && c.order >= 1 && c.order <= 2 &&
+                   (ncomp == 1 || ncomp == c.dim) && (c.ncomp == 1 || c.ncomp == c.dim) &&
+                   (ncomp == 1 || c.ncomp == 1),
                GH_ERR_UNSUPPORTED,
-               "rectangular pairs are provided between scalar Q1/Q2 spaces on quadrilaterals "
-               "and hexahedra");
+               "rectangular pairs are provided between Q1/Q2 spaces on quadrilaterals and "
+               "hexahedra, scalar or with one component per dimension, at most one of them "
+               "vector-valued");
     GH_REQUIRE(nfree >= 0 && ndiri >= 0 && nfree < (int64_t(1) << 31) &&
                    ndiri < (int64_t(1) << 31),
                GH_ERR_ARG, "dof counts out of range");
@@ -336,8 +339,10 @@ int gh_trial_space_upload(gh_ctx *ctx, int reffe_kind, int order, int ncomp,
                    "cell dof id out of range");
       std::memcpy(&cd[size_t(k) * nloc], src, sizeof(int32_t) * nloc);
     }
+    GH_REQUIRE(nloc % ncomp == 0, GH_ERR_ARG, "dofs per cell not a multiple of ncomp");
     c.torder = order;
     c.tnloc = nloc;
+    c.tncomp = ncomp;
     c.tnfree = nfree;
     c.tndiri = ndiri;
     c.tcell_dofs.resize(cd.size());
@@ -357,8 +362,9 @@ int gh_trial_space_upload(gh_ctx *ctx, int reffe_kind, int order, int ncomp,
 int gh_trial_tabulation(gh_ctx *ctx, int nq, int nloc, const double *shp, const double *grad) {
   return guarded(ctx, [&](gh::Ctx &c) {
     GH_REQUIRE(c.have_trial, GH_ERR_STATE, "upload the trial space first");
-    GH_REQUIRE(nq >= 1 && nloc == c.tnloc && shp && grad, GH_ERR_ARG,
-               "the trial tabulation needs nq >= 1 points and the trial space's dofs per cell");
+    GH_REQUIRE(nq >= 1 && nloc == c.tnloc / c.tncomp && shp && grad, GH_ERR_ARG,
+               "the trial tabulation needs nq >= 1 points and the trial space's scalar shape "
+               "functions");
     c.tshp.assign(shp, shp + (size_t)nq * nloc);
     c.tgrad.assign(grad, grad + (size_t)nq * nloc * c.dim);
     c.tnq = nq;
@@ -578,8 +584,9 @@ int gh_source_qp_upload(gh_ctx *ctx, const double *fq) {
 int gh_assemble(gh_ctx *ctx, int form_kind, int precision, int path) {
   return guarded(ctx, [&](gh::Ctx &c) {
     GH_REQUIRE(form_kind == GH_FORM_DIFFUSION || form_kind == GH_FORM_ELASTICITY ||
-                   form_kind == GH_FORM_PLAPLACIAN,
-               GH_ERR_UNSUPPORTED, "unknown form kind");
+                   form_kind == GH_FORM_PLAPLACIAN ||
+                   (form_kind == GH_FORM_DIVERGENCE && c.have_trial),
+               GH_ERR_UNSUPPORTED, "unknown form kind (the divergence form needs a trial space)");
     GH_REQUIRE(precision == GH_PREC_FP64 || precision == GH_PREC_FP32_ELEM, GH_ERR_ARG,
                "unknown precision mode");
     GH_REQUIRE(path == GH_PATH_AUTO || path == GH_PATH_GENERAL || path == GH_PATH_STRUCTURED,
@@ -587,12 +594,14 @@ int gh_assemble(gh_ctx *ctx, int form_kind, int precision, int path) {
     GH_REQUIRE(c.have_pattern, GH_ERR_STATE, "no pattern: call gh_symbolic first");
     GH_REQUIRE(c.nq > 0, GH_ERR_STATE, "no reference tabulation uploaded");
     if (c.have_trial) {
-      GH_REQUIRE(form_kind == GH_FORM_DIFFUSION && path != GH_PATH_STRUCTURED,
+      GH_REQUIRE((form_kind == GH_FORM_DIFFUSION || form_kind == GH_FORM_DIVERGENCE) &&
+                     path != GH_PATH_STRUCTURED,
                  GH_ERR_UNSUPPORTED,
-                 "a rectangular pair assembles κ ∇ψ⋅∇φ + σ ψ φ on the general path");
+                 "a rectangular pair assembles κ ∇ψ⋅∇φ + σ ψ φ or the divergence coupling on "
+                 "the general path");
       GH_REQUIRE(!c.have_fsrc && !c.have_fq, GH_ERR_UNSUPPORTED,
                  "sources belong to the test field's own (square) pair");
-      gh::rect_assemble(c, precision);
+      gh::rect_assemble(c, form_kind, precision);
     } else if (form_kind == GH_FORM_PLAPLACIAN) {
       gh::assemble_plaplacian(c, precision, path);
     } else if (form_kind == GH_FORM_ELASTICITY) {

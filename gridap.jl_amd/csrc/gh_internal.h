@@ -162,7 +162,7 @@ struct Ctx {
   // ---- trial space of a rectangular field pair (gh_trial_space_upload): the columns,
   //      while the space above gives the rows ----
   bool have_trial = false;
-  int torder = 0, tnloc = 0;
+  int torder = 0, tnloc = 0, tncomp = 1;
   int64_t tnfree = 0, tndiri = 0;
   DBuf<int32_t> tcell_dofs;         // ncells x tnloc, Gridap signed 1-based ids
   int tnq = 0;                      // points of the trial tabulation
@@ -328,7 +328,7 @@ void hexgen_assemble(Ctx &c, int precision, bool elasticity);
 // ([cell][local], FP64); enqueued on c.stream.
 void hexgen_rhs(Ctx &c);
 void rect_symbolic(Ctx &c);                  // pattern of a rectangular field pair
-void rect_assemble(Ctx &c, int precision);   // its values and lifting loads
+void rect_assemble(Ctx &c, int form, int precision);  // its values and lifting loads
 void elemlist_symbolic(Ctx &c);         // dof -> cell adjacency, Dirichlet masks
 void elemlist_sum(Ctx &c, bool f32);     // nzval from the element entries
 void assemble_simplex(Ctx &c, int precision);

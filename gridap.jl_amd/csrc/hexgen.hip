@@ -160,7 +160,10 @@ __global__ __launch_bounds__(kHgThreads) void hexgen_element_kernel(HgArgs A) {
           m += wq[q] * (A.tab[(size_t)q * stride + 1 + a] * A.tab[(size_t)q * stride + 1 + b]);
         v += A.sigma[cell] * m;
       }
-      E[b * nloc + a] = (T)v;
+      // vector spaces: the component-wise (vector) Laplacian ∇v⊙∇u, δ_ij blocks
+      for (int j = 0; j < A.nc; ++j)
+        for (int i = 0; i < A.nc; ++i)
+          E[(b + nn * j) * nloc + a + nn * i] = (T)(i == j ? v : 0.0);
       continue;
     }
     // K[(a,i),(b,j)] = Σ_q w [λ g_ai g_bj + μ (g_aj g_bi + δ_ij g_a·g_b)]
@@ -198,7 +201,7 @@ __global__ __launch_bounds__(kHgThreads) void hexgen_element_kernel(HgArgs A) {
       v = m;
     } else {
       for (int q = 0; q < nq; ++q) m += wq[q] * A.tab[(size_t)q * stride + 1 + a];
-      v = (A.elasticity ? A.body[i] : f) * m;
+      v = (A.nc > 1 ? A.body[i] : f) * m;
     }
     if (A.dvals)
       for (int lk = 0; lk < nloc; ++lk)
@@ -343,13 +346,16 @@ void hexgen_rhs(Ctx &c) {
 }
 
 void hexgen_assemble(Ctx &c, int precision, bool elasticity) {
-  GH_REQUIRE(hexgen_eligible(c) && (elasticity ? c.ncomp == c.dim : c.ncomp == 1),
+  GH_REQUIRE(hexgen_eligible(c) && (elasticity ? c.ncomp == c.dim
+                                                : (c.ncomp == 1 || c.ncomp == c.dim)),
              GH_ERR_UNSUPPORTED,
              elasticity ? "elasticity needs a vector space with one component per dimension"
-                        : "diffusion needs a scalar space");
+                        : "diffusion needs a scalar space or one component per dimension");
   GH_REQUIRE(c.elemlist, GH_ERR_STATE, "no element lists for this space");
   GH_REQUIRE(!c.have_sigma || (!elasticity && c.sigma.n == (size_t)c.ncells), GH_ERR_STATE,
              "the cell mass term belongs to the diffusion form of this mesh");
+  GH_REQUIRE(!c.have_fq || c.ncomp == 1, GH_ERR_UNSUPPORTED,
+             "sources at quadrature points load scalar spaces");
   GH_REQUIRE(!c.have_fq || (!elasticity && c.fq.n == (size_t)c.ncells * c.nq), GH_ERR_STATE,
              "the quadrature-point source belongs to another mesh or quadrature");
   const int D = c.dim, nloc = c.nloc, nn = nloc / c.ncomp;

@@ -9,10 +9,15 @@
 // src/Algebra/AlgebraInterfaces.jl:175-187), one radix sort orders them with their
 // emit index, and a unique-flag scan writes rowval, colptr (columns may be empty: a
 // trial dof whose cells hold only Dirichlet test dofs) and the contribution lists
-// (cstart, perm).  Numeric: one thread per cell forms the block's element matrix
-//   B_c[lj][li] = Σ_q w|J| (κ_c ∇ψ_li⋅∇φ_lj + σ_c ψ_li φ_lj)
-// (ψ test, φ trial; the geometry of the cell: Cartesian sizes, or the Q1 map of its
-// vertices) and its lifting load -Σ_{lj Dirichlet} B_c[lj][li] u_D(lj); one thread per
+// (cstart, perm).  Numeric: one thread per cell forms the block's element matrix, for
+// scalar test ψ and trial φ
+//   B_c[lj][li] = Σ_q w|J| (κ_c ∇ψ_li⋅∇φ_lj + σ_c ψ_li φ_lj),
+// or the divergence coupling between a scalar and a vector space (local dofs
+// component-blocked, node + nn·component)
+//   scalar test, vector trial: B_c[(b,j)][a] = κ_c Σ_q w|J| ψ_a ∂_j φ_b,
+//   vector test, scalar trial: B_c[b][(a,i)] = κ_c Σ_q w|J| ∂_i ψ_a φ_b
+// (the geometry of the cell: Cartesian sizes, or the Q1 map of its vertices) and its
+// lifting load -Σ_{lj Dirichlet} B_c[lj][li] u_D(lj); one thread per
 // nonzero sums its entries in ascending cell order and writes nzval once; one thread
 // per test row sums the lifting loads of its cells (ascending).  Deterministic, no
 // atomics on values.
@@ -83,8 +88,10 @@ __global__ void rect_rows(int64_t n, const uint64_t *__restrict__ keys, uint64_t
 
 struct RectArgs {
   int64_t ncells;
-  int ns, nt, nq, nv;
-  const double *ttab;        // per q: w, ψ[ns], ∇̂ψ[ns*D], φ[nt], ∇̂φ[nt*D]
+  int ns, nt, nq, nv;        // local dofs of the test and trial spaces
+  int nns, ntn;              // their scalar shape functions (nodes)
+  bool div;                  // the divergence coupling (else κ ∇ψ⋅∇φ + σ ψ φ)
+  const double *ttab;        // per q: w, ψ[nns], ∇̂ψ[nns*D], φ[ntn], ∇̂φ[ntn*D]
   const double *gtab;        // Q1 map gradients per q ([nv*D]), or nullptr (Cartesian)
   const int32_t *cell_nodes; // 0-based node ids of the cell's vertices
   const double *xyz;
@@ -99,9 +106,9 @@ __global__ __launch_bounds__(kBlock) void rect_elem_kernel(const RectArgs A, T *
                                                            double *__restrict__ Ev) {
   const int64_t cell = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (cell >= A.ncells) return;
-  const int ns = A.ns, nt = A.nt;
-  const int stride = 1 + (ns + nt) * (1 + D);
-  const double kap = A.kappa ? A.kappa[cell] : 0.0;
+  const int ns = A.ns, nt = A.nt, nns = A.nns, ntn = A.ntn;
+  const int stride = 1 + (nns + ntn) * (1 + D);
+  const double kap = A.kappa ? A.kappa[cell] : (A.div ? 1.0 : 0.0);
   const double sig = A.sigma ? A.sigma[cell] : 0.0;
   T *Ec = E + cell * ns * nt;
   double *Evc = Ev + cell * ns;
@@ -131,7 +138,27 @@ __global__ __launch_bounds__(kBlock) void rect_elem_kernel(const RectArgs A, T *
       }
     }
     const double w = tq[0] * fabs(det);
-    const double *psi = tq + 1, *gpsi = psi + ns, *phi = gpsi + ns * D, *gphi = phi + nt;
+    const double *psi = tq + 1, *gpsi = psi + nns, *phi = gpsi + nns * D, *gphi = phi + ntn;
+    if (A.div && nt > ntn) {  // scalar test, vector trial: ψ_a ∂_j φ_b
+      for (int b = 0; b < ntn; ++b)
+        for (int j = 0; j < D; ++j) {
+          double g = 0.0;
+          for (int e = 0; e < D; ++e) g += Ji[e][j] * gphi[b * D + e];
+          const double wg = w * kap * g;
+          for (int a = 0; a < nns; ++a) Ec[(b + ntn * j) * ns + a] += (T)(wg * psi[a]);
+        }
+      continue;
+    }
+    if (A.div) {  // vector test, scalar trial: ∂_i ψ_a φ_b
+      for (int a = 0; a < nns; ++a)
+        for (int i = 0; i < D; ++i) {
+          double g = 0.0;
+          for (int e = 0; e < D; ++e) g += Ji[e][i] * gpsi[a * D + e];
+          const double wg = w * kap * g;
+          for (int b = 0; b < ntn; ++b) Ec[b * ns + a + nns * i] += (T)(wg * phi[b]);
+        }
+      continue;
+    }
     for (int lj = 0; lj < nt; ++lj) {
       double gj[D];
       for (int d = 0; d < D; ++d) {
@@ -293,30 +320,38 @@ void rect_symbolic(Ctx &c) {
   c.b.resize(std::max<int64_t>(c.nfree, 1));
 }
 
-void rect_assemble(Ctx &c, int precision) {
+void rect_assemble(Ctx &c, int form, int precision) {
   const int D = c.dim, ns = c.nloc, nt = c.tnloc;
+  const int nns = ns / c.ncomp, ntn = nt / c.tncomp;
+  const bool div = form == GH_FORM_DIVERGENCE;
   GH_REQUIRE(D == 2 || D == 3, GH_ERR_UNSUPPORTED, "rectangular pairs on 2-D and 3-D meshes");
-  GH_REQUIRE(c.nq > 0 && c.tnq == c.nq && (int)c.shp.size() == c.nq * ns &&
-                 (int)c.grad.size() == c.nq * ns * D && (int)c.tshp.size() == c.nq * nt &&
-                 (int)c.tgrad.size() == c.nq * nt * D,
+  GH_REQUIRE(div ? (c.ncomp == 1) != (c.tncomp == 1) : (c.ncomp == 1 && c.tncomp == 1),
+             GH_ERR_UNSUPPORTED,
+             div ? "the divergence coupling pairs a scalar space with a vector space"
+                 : "κ ∇ψ⋅∇φ + σ ψ φ pairs two scalar spaces");
+  GH_REQUIRE(c.nq > 0 && c.tnq == c.nq && (int)c.shp.size() == c.nq * nns &&
+                 (int)c.grad.size() == c.nq * nns * D && (int)c.tshp.size() == c.nq * ntn &&
+                 (int)c.tgrad.size() == c.nq * ntn * D,
              GH_ERR_STATE, "upload the test and trial tabulations at the same points");
   GH_REQUIRE(c.cartesian || (c.have_gtab && c.gtab_nq == c.nq && c.nv <= 8 &&
                              c.nv == (1 << D)),
              GH_ERR_STATE,
              "a mapped mesh needs the geometry tabulation (gh_geometry_tabulation) at the "
              "points of the tabulations");
-  GH_REQUIRE(c.have_kappa || c.have_sigma, GH_ERR_STATE,
+  GH_REQUIRE(div || c.have_kappa || c.have_sigma, GH_ERR_STATE,
              "a rectangular pair needs κ (gh_coef_upload) or σ (gh_mass_upload)");
-  const int stride = 1 + (ns + nt) * (1 + D);
+  GH_REQUIRE(!div || !c.have_sigma, GH_ERR_UNSUPPORTED,
+             "the divergence coupling has no mass term");
+  const int stride = 1 + (nns + ntn) * (1 + D);
   std::vector<double> tab((size_t)c.nq * stride);
   for (int q = 0; q < c.nq; ++q) {
     double *tq = &tab[(size_t)q * stride];
     tq[0] = c.qw[q];
-    double *psi = tq + 1, *gpsi = psi + ns, *phi = gpsi + ns * D, *gphi = phi + nt;
-    for (int a = 0; a < ns; ++a) psi[a] = c.shp[(size_t)q * ns + a];
-    for (int a = 0; a < ns * D; ++a) gpsi[a] = c.grad[(size_t)q * ns * D + a];
-    for (int a = 0; a < nt; ++a) phi[a] = c.tshp[(size_t)q * nt + a];
-    for (int a = 0; a < nt * D; ++a) gphi[a] = c.tgrad[(size_t)q * nt * D + a];
+    double *psi = tq + 1, *gpsi = psi + nns, *phi = gpsi + nns * D, *gphi = phi + ntn;
+    for (int a = 0; a < nns; ++a) psi[a] = c.shp[(size_t)q * nns + a];
+    for (int a = 0; a < nns * D; ++a) gpsi[a] = c.grad[(size_t)q * nns * D + a];
+    for (int a = 0; a < ntn; ++a) phi[a] = c.tshp[(size_t)q * ntn + a];
+    for (int a = 0; a < ntn * D; ++a) gphi[a] = c.tgrad[(size_t)q * ntn * D + a];
   }
   DBuf<double> dtab;
   dtab.resize(tab.size());
@@ -329,6 +364,9 @@ void rect_assemble(Ctx &c, int precision) {
   A.ncells = c.ncells;
   A.ns = ns;
   A.nt = nt;
+  A.nns = nns;
+  A.ntn = ntn;
+  A.div = div;
   A.nq = c.nq;
   A.nv = c.nv;
   A.ttab = dtab.get();

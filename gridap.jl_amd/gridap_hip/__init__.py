@@ -22,7 +22,8 @@ from .assemblers import (AffineFEOperator, LinearFESolver, PCGSolver, SparseMatr
                          assemble_vector, assemble_vector_inplace, get_matrix, get_vector,
                          solve)
 from .celldata import (CellField, DomainContribution, IsotropicStress, Measure, PLaplacian,
-                       compose, dot, get_fe_basis, get_normal_vector, get_trial_fe_basis, grad,
+                       compose, divergence, dot, get_fe_basis, get_normal_vector,
+                       get_trial_fe_basis, grad,
                        gradient, inner, integral, integrate, symmetric_gradient, ε)
 from .fespaces import FEFunction, FESpace, TestFESpace, TrialFESpace, interpolate
 from .geometry import (BoundaryTriangulation, CartesianDiscreteModel, FaceLabeling, Triangulation,
@@ -40,7 +41,7 @@ __all__ = [
     "Measure", "PCGSolver", "Quadrature", "ReferenceFE", "SparseMatrixAssembler",
     "SparseMatrixCSC", "TestFESpace", "TrialFESpace", "Triangulation",
     "UnstructuredDiscreteModel", "VectorValue", "simplexify",
-    "assemble_matrix", "assemble_matrix_and_vector", "assemble_vector", "dot",
+    "assemble_matrix", "assemble_matrix_and_vector", "assemble_vector", "divergence", "dot",
     "assemble_matrix_inplace", "assemble_matrix_and_vector_inplace", "assemble_vector_inplace",
     "get_fe_basis", "get_matrix", "get_triangulation", "get_trial_fe_basis", "get_vector",
     "grad", "gradient", "inner", "integral", "integrate", "lagrangian", "solve",

@@ -28,6 +28,7 @@ GH_PREC_FP64 = 0
 GH_PREC_FP32_ELEM = 1
 GH_FORM_ELASTICITY = 2
 GH_FORM_PLAPLACIAN = 3
+GH_FORM_DIVERGENCE = 4
 GH_PATH_AUTO = 0
 GH_PATH_GENERAL = 1
 GH_PATH_STRUCTURED = 2

@@ -199,6 +199,10 @@ class SparseMatrixAssembler:
                     raise NotImplementedError("linear forms: f v or v⋅f")
                 degs.add(meas.degree)
                 have_vec = True
+        if kinds == {_lib.GH_FORM_DIFFUSION, _lib.GH_FORM_ELASTICITY} and nu is None and \
+                self.space.ncomp > 1 and fsrc is None and not point_sources:
+            # ∇v⊙∇u with v⋅f on a vector space: the vector Laplacian and its body force
+            kinds = {_lib.GH_FORM_DIFFUSION}
         if len(kinds) > 1:
             raise NotImplementedError("one form kind per operator")
         if len(degs) > 1:
@@ -276,7 +280,8 @@ class SparseMatrixAssembler:
             self.ctx.source_qp_upload(fq)
             self._has_fq = fq is not None
         self._upload_boundary(extra.get("boundary"))
-        if form == _lib.GH_FORM_ELASTICITY:
+        if form == _lib.GH_FORM_ELASTICITY or self.space.ncomp > 1:
+            # the body force loads elasticity and the vector Laplacian
             self.ctx.elasticity_params(extra["nu"], extra["force"])
         if A is not None and not (A.shape == (self.space.nfree, self.space.nfree) and
                                   np.array_equal(A.colptr, self.colptr) and

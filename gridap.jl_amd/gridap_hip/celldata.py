@@ -224,6 +224,23 @@ class _NormalGrad(_Expr):
         self.basis = basis
 
 
+class _Div(_Expr):
+    """coef * (∇⋅u) of an FE basis of a vector-valued space."""
+
+    def __init__(self, basis, coef=None):
+        self.basis = basis
+        self.coef = coef
+
+
+class DivergenceTerm(_Expr):
+    """coef * q (∇⋅u) (part "q_divu": scalar test, vector trial) or coef * (∇⋅v) p (part
+    "divv_p": vector test, scalar trial), the coupling blocks of
+    test/GridapTests/StokesTaylorHoodTests.jl:54."""
+
+    def __init__(self, part, coef=None):
+        self.part, self.coef = part, coef
+
+
 class MassTerm(_Expr):
     """coef * v u (test times trial)."""
 
@@ -322,10 +339,19 @@ def _product(x, y):
     if isinstance(x, FEBasis) and not x.trial:                       # v * y
         if isinstance(y, FEBasis) and y.trial:
             return _f(MassTerm(), (x.field, y.field))
+        if isinstance(y, _Div) and y.basis.trial:                    # q * (∇⋅u)
+            return _f(DivergenceTerm("q_divu", y.coef), (x.field, y.basis.field))
         if isinstance(y, _NormalGrad) and y.basis.trial:
             return NitscheTerm("v_dnu")
         if cy:
             return _f(SourceTerm(_as_coef(y)), (x.field,))
+    if isinstance(x, _Div):
+        if not x.basis.trial and isinstance(y, FEBasis) and y.trial:  # (∇⋅v) * p
+            return _f(DivergenceTerm("divv_p", x.coef), (x.basis.field, y.field))
+        if cy:
+            return _Div(x.basis, _coef_product(x.coef, _as_coef(y)))
+    if isinstance(x, _Sum) and isinstance(y, FEBasis):               # (Σ terms) * basis
+        return _Sum([_scale(y, t) for t in x.terms])
     if isinstance(x, _NormalGrad) and not x.basis.trial:            # (n⋅∇v) * y
         if isinstance(y, FEBasis) and y.trial:
             return NitscheTerm("dnv_u")
@@ -335,6 +361,9 @@ def _product(x, y):
         tf = x.fields[0] if x.fields else None
         if isinstance(y, FEBasis) and y.trial:
             return _f(MassTerm(x.coef), (tf, y.field))
+        if isinstance(y, _Div) and y.basis.trial:
+            return _f(DivergenceTerm("q_divu", _coef_product(x.coef, y.coef)),
+                      (tf, y.basis.field))
         if isinstance(y, _NormalGrad) and y.basis.trial:
             return NitscheTerm("v_dnu", x.coef)
         if cy:
@@ -347,6 +376,8 @@ def _product(x, y):
     if cy:
         if isinstance(x, (DiffusionTerm, MassTerm)):
             return _f(type(x)(_coef_product(x.coef, _as_coef(y))), x.fields)
+        if isinstance(x, DivergenceTerm):
+            return _f(DivergenceTerm(x.part, _coef_product(x.coef, _as_coef(y))), x.fields)
         if isinstance(x, NitscheTerm):
             return NitscheTerm(x.part, _coef_product(x.coef, _as_coef(y)))
         if isinstance(x, (PLaplacianResidualTerm, PLaplacianJacobianTerm)):
@@ -408,6 +439,13 @@ def gradient(x):
 grad = gradient
 
 
+def divergence(x):
+    """∇⋅u of an FE basis of a vector-valued space."""
+    if not isinstance(x, FEBasis):
+        raise NotImplementedError("divergence of an FE basis only")
+    return _Div(x)
+
+
 def symmetric_gradient(x):
     if not isinstance(x, FEBasis):
         raise NotImplementedError("ε of an FE basis only")
@@ -439,9 +477,9 @@ def inner(a, b):
             b.eps.basis.trial:
         return ElasticityTerm(b.material.E, b.material.nu)
     if isinstance(a, FEBasis) and isinstance(b, VectorValue) and not a.trial:
-        return BodyForceTerm(b.comps)
+        return _f(BodyForceTerm(b.comps), (a.field,))
     if isinstance(b, FEBasis) and isinstance(a, VectorValue) and not b.trial:
-        return BodyForceTerm(a.comps)
+        return _f(BodyForceTerm(a.comps), (b.field,))
     if isinstance(a, FEBasis) and not isinstance(b, _Expr):
         return _scale(b, a)
     if isinstance(b, FEBasis) and not isinstance(a, _Expr):
@@ -486,7 +524,7 @@ class DomainContribution:
 
 
 _CELL_TERMS = (DiffusionTerm, MassTerm, SourceTerm, ElasticityTerm, BodyForceTerm,
-               PLaplacianResidualTerm, PLaplacianJacobianTerm)
+               PLaplacianResidualTerm, PLaplacianJacobianTerm, DivergenceTerm)
 _FACE_TERMS = (MassTerm, NitscheTerm, SourceTerm, NormalLoadTerm)
 
 

@@ -24,8 +24,9 @@ The right-hand side of test field i is its source terms minus the lifting
 Pairs between two different spaces on one mesh (rectangular blocks, e.g. a Q2 and a Q1
 scalar field coupled by ∫ σ u1 v2 or ∫ κ ∇u1⋅∇v2) are assembled on the device by a
 pair context that holds the test space as rows and the trial space as columns
-(gh_trial_space_upload); vector-valued fields in rectangular pairs (Taylor–Hood
-velocity-pressure blocks) are not provided.
+(gh_trial_space_upload), and so are the divergence couplings q (∇⋅u) and (∇⋅v) p
+between a scalar and a vector field (the velocity-pressure blocks of a Taylor–Hood
+Stokes form, test/GridapTests/StokesTaylorHoodTests.jl:54).
 """
 import numpy as np
 
@@ -193,8 +194,9 @@ class RectangularPairAssembler:
     """The block (i, j) of a multi-field form whose test field i and trial field j live
     in different spaces on one mesh: rows are Vi's free dofs, columns Uj's
     (SparseMatrixAssemblers.jl:174-248 with cell_rows and cell_cols from two spaces).
-    Cell terms κ ∇u⋅∇v and σ u v (κ, σ constant per cell); the lifting of Uj's
-    Dirichlet values goes to the test field's right-hand side."""
+    Cell terms κ ∇u⋅∇v and σ u v between scalar spaces, or the divergence coupling
+    κ q (∇⋅u) / κ (∇⋅v) p between a scalar and a vector space (κ constant per cell);
+    the lifting of Uj's Dirichlet values goes to the test field's right-hand side."""
 
     def __init__(self, U, V, device=0, precision="fp64"):
         from . import _lib
@@ -245,26 +247,37 @@ class RectangularPairAssembler:
     def assemble(self, matcontrib, veccontrib, lift, trial=None):
         from . import _lib
         from .assemblers import SparseMatrixCSC
-        from .celldata import DiffusionTerm, MassTerm, _cell_coef
+        from .celldata import DiffusionTerm, DivergenceTerm, MassTerm, _cell_coef
         from .fespaces import TrialFESpace
         if veccontrib is not None:
             raise NotImplementedError("sources belong to the test field's own pair")
-        ncells = _underlying(self.V).model.num_cells()
+        Vs, Us = _underlying(self.V), _underlying(self.U)
+        ncells = Vs.model.num_cells()
         kappa = sigma = None
-        degs = set()
+        degs, kinds = set(), set()
         for term, meas in (matcontrib.terms if matcontrib is not None else []):
-            if not isinstance(term, (DiffusionTerm, MassTerm)):
-                raise NotImplementedError("rectangular pairs: κ ∇u⋅∇v and σ u v cell terms")
+            if isinstance(term, DivergenceTerm):
+                want = (1, Vs.model.D) if term.part == "q_divu" else (Vs.model.D, 1)
+                if (Vs.ncomp, Us.ncomp) != want:
+                    raise NotImplementedError("the divergence coupling pairs a scalar and a "
+                                              "vector space as q (∇⋅u) or (∇⋅v) p")
+                kinds.add(_lib.GH_FORM_DIVERGENCE)
+            elif isinstance(term, (DiffusionTerm, MassTerm)):
+                kinds.add(_lib.GH_FORM_DIFFUSION)
+            else:
+                raise NotImplementedError("rectangular pairs: κ ∇u⋅∇v, σ u v and divergence "
+                                          "cell terms")
             coef = _cell_coef(term.coef)
             k = coef.per_cell(ncells) if coef is not None else None
             k = np.ones(ncells) if k is None else k
-            if isinstance(term, DiffusionTerm):
-                kappa = k if kappa is None else kappa + k
-            else:
+            if isinstance(term, MassTerm):
                 sigma = k if sigma is None else sigma + k
+            else:
+                kappa = k if kappa is None else kappa + k
             degs.add(meas.degree)
-        if len(degs) != 1:
-            raise NotImplementedError("the terms of a block must share one Measure")
+        if len(degs) != 1 or len(kinds) != 1:
+            raise NotImplementedError("the terms of a block must share one Measure and kind")
+        form = kinds.pop()
         self._setup(degs.pop())
         dvals = None
         U = self.U if trial is None else trial
@@ -274,9 +287,8 @@ class RectangularPairAssembler:
                 dvals = dv
         self.ctx.coef_upload(kappa, None, dvals)
         self.ctx.mass_upload(sigma)
-        self.ctx.assemble(_lib.GH_FORM_DIFFUSION, self.precision, _lib.GH_PATH_GENERAL)
+        self.ctx.assemble(form, self.precision, _lib.GH_PATH_GENERAL)
         nz, b = self.ctx.values_get()
-        Vs, Us = _underlying(self.V), _underlying(self.U)
         A = SparseMatrixCSC(Vs.nfree, Us.nfree, self.colptr.copy(), self.rowval.copy(), nz)
         return A, b
 

@@ -42,7 +42,9 @@ extern "C" {
 /* ---- weak forms ---- */
 #define GH_FORM_DIFFUSION 1 /* a(u,v)=∫ κ ∇v⋅∇u dΩ,  l(v)=∫ f v dΩ, κ and f constant per cell
                              * (or f at the quadrature points, gh_source_qp_upload); scalar
-                             * Q1, P1, P2, and Q2 on quads/hexes */
+                             * Q1, P1, P2, and Q2 on quads/hexes; on vector Q1/Q2 spaces of
+                             * the general n-cube path the vector Laplacian ∫ κ ∇v⊙∇u with
+                             * l(v)=∫ v⋅f, f the constant body force of gh_elasticity_params */
 #define GH_FORM_ELASTICITY 2 /* a(u,v)=∫ ε(v) : σ(ε(u)) dΩ, σ = λ tr(ε) I + 2 μ ε with λ, μ from a
                               * per-cell Young's modulus (the κ array) and ν; l(v)=∫ v⋅f dΩ with f
                               * constant (gh_elasticity_params); vector Q1 on quads/hexes and
@@ -55,6 +57,12 @@ extern "C" {
                               * the Jacobian to nzval; scalar Q1 on quads/hexes.  Lattice
                               * meshes take the owner-computes column route unless path is
                               * GH_PATH_GENERAL (the coloured per-cell scatter) */
+#define GH_FORM_DIVERGENCE 4 /* rectangular pairs (gh_trial_space_upload) between a scalar
+                              * and a vector space (one component per dimension):
+                              * scalar test ψ, vector trial u: ∫ κ ψ (∇⋅u) dΩ; vector test v,
+                              * scalar trial φ: ∫ κ (∇⋅v) φ dΩ (the q (∇⋅u) and -(∇⋅v) p
+                              * blocks of test/GridapTests/StokesTaylorHoodTests.jl:54, with
+                              * κ = 1 and -1); κ per cell from gh_coef_upload (NULL = 1) */
 
 /* ---- element precision ---- */
 #define GH_PREC_FP64 0      /* element arithmetic and accumulation in FP64           */

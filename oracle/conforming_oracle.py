@@ -617,6 +617,29 @@ def mixed_cells(wdet, g_test, g_trial, shp_test, shp_trial, kappa=None, sigma=No
     return out
 
 
+def vector_laplacian_cells(wdet, g, kappa=None):
+    """[C, D·nn, D·nn] component-blocked (node + nn·comp) cell matrices of the vector
+    Laplacian ∫ κ ∇v⊙∇u: δ_ij times the scalar diffusion matrix."""
+    K = ncube_diffusion_cells(wdet, g, kappa)
+    D = g.shape[3]
+    return np.einsum("ij,cab->ciajb", np.eye(D), K).reshape(K.shape[0], D * K.shape[1], -1)
+
+
+def divergence_cells(wdet, shp_scalar, g_vector, scalar_test=True, kappa=None):
+    """Cell matrices [C, nt, ns] of the divergence coupling between a scalar basis ψ
+    (shp_scalar [nq, ns]) and the scalar basis of a vector space (physical gradients
+    g_vector [C, nq, nn, D]; local dofs node + nn·comp), test/GridapTests/
+    StokesTaylorHoodTests.jl:54 with κ per cell:
+      scalar_test: entry [c, (b, j), a] = κ Σ_q w|det J| ψ_a ∂_j φ_b   (q (∇⋅u));
+      otherwise:   entry [c, b, (a, i)] = κ Σ_q w|det J| ∂_i φ_a ψ_b   ((∇⋅v) p)."""
+    shp = np.asarray(shp_scalar, dtype=np.float64)
+    C, nq, nn, D = g_vector.shape
+    B = np.einsum("cq,qa,cqbj->cjba", wdet, shp, g_vector).reshape(C, D * nn, shp.shape[1])
+    if kappa is not None:
+        B = np.asarray(kappa, dtype=np.float64)[:, None, None] * B
+    return B if scalar_test else np.ascontiguousarray(B.transpose(0, 2, 1))
+
+
 def q1_map_points(X, n1):
     """Physical coordinates [C, npts, D] of reference points under each cell's Q1 map
     (n1: [npts, nv] Q1 shape values at the points)."""

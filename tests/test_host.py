@@ -157,3 +157,30 @@ def test_tet_degree4_rule_equals_the_moment_equation_root():
     # orbit order of the table: the first S31 orbit's fourth point is (s,s,s) mapped
     s = q.coords[3]
     assert s[0] == s[1] == s[2] and np.all(q.coords[0][:2] == s[:2]) and q.coords[0][2] != s[2]
+
+
+def test_taylor_hood_form_lowering():
+    """StokesTaylorHoodTests.jl's a((u,p),(v,q)) = ∫ ∇v⊙∇u - (∇⋅v) p + q (∇⋅u) splits into
+    the vector Laplacian on (v, u), the (∇⋅v) p coupling with coefficient -1 on (v, p)
+    and the q (∇⋅u) coupling on (q, u); l = ∫ v⋅f + q g into a body force on v and a
+    source on q."""
+    model = G.CartesianDiscreteModel((0, 2, 0, 2), (3, 3))
+    V = G.TestFESpace(model, G.ReferenceFE(G.lagrangian, G.VectorValue[2, G.Float64], 2),
+                      dirichlet_tags="boundary")
+    Q = G.TestFESpace(model, G.ReferenceFE(G.lagrangian, G.Float64, 1), dirichlet_tags="tag_1")
+    Y = G.MultiFieldFESpace([V, Q])
+    X = G.MultiFieldFESpace([G.TrialFESpace(V), G.TrialFESpace(Q)])
+    dΩ = G.Measure(G.Triangulation(model), 2)
+    u, v = G.get_trial_fe_basis(X), G.get_fe_basis(Y)
+    c = G.integrate(G.inner(G.gradient(v[0]), G.gradient(u[0])) - G.divergence(v[0]) * u[1] +
+                    v[1] * G.divergence(u[0]), dΩ)
+    got = [(type(t).__name__, t.fields, getattr(t, "part", None),
+            None if t.coef is None else t.coef.values) for t, _ in c.terms]
+    assert got == [("DiffusionTerm", (0, 0), None, None),
+                   ("DivergenceTerm", (0, 1), "divv_p", -1.0),
+                   ("DivergenceTerm", (1, 0), "q_divu", None)]
+    l = G.integrate(G.dot(v[0], G.VectorValue(-5.0, 5.0)) + v[1] * (lambda x: x[..., 0]), dΩ)
+    assert [(type(t).__name__, t.fields) for t, _ in l.terms] == \
+        [("BodyForceTerm", (0,)), ("SourceTerm", (1,))]
+    with pytest.raises(NotImplementedError):
+        G.divergence(2.0)

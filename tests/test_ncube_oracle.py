@@ -137,3 +137,41 @@ def test_mixed_cells_pins(oracle, D):
     np.testing.assert_allclose(B, np.einsum("ab,cal->cbl", N, K2), rtol=0,
                                atol=1e-12 * np.abs(K2).max())
     assert B.shape == (len(cn), 2 ** D, 3 ** D)
+
+
+@pytest.mark.parametrize("D", [2, 3])
+def test_divergence_and_vector_laplacian_pins(oracle, D):
+    """divergence_cells and vector_laplacian_cells (the Taylor–Hood blocks):
+    * q (∇⋅u) applied to the Q2 interpolant of a linear vector field u = A x + c equals
+      tr(A) ∫ ψ_a (∇⋅u is the constant tr A) — a transposed gradient or a wrong
+      component offset fails it;
+    * the (∇⋅v) p block is the transpose of the q (∇⋅u) block of the same two bases;
+    * the vector Laplacian is δ_ij times the scalar diffusion matrix, and constant
+      vector fields lie in its kernel."""
+    part = (3, 2, 2)[:D]
+    m, xyz, cn, X = cells_of(oracle, D, part, True)
+    qx, qw = oracle.tensor_quadrature(D, 4)
+    s1, g1 = oracle.qk_tabulate(D, 1, qx)
+    s2, g2 = oracle.qk_tabulate(D, 2, qx)
+    wdet, gp2 = CO.physical_gradients(X, qw, g1, g2)
+    C, nn = len(cn), s2.shape[1]
+    kappa = np.exp(np.random.default_rng(D).normal(size=C))
+    B = CO.divergence_cells(wdet, s1, gp2, scalar_test=True, kappa=kappa)   # [C, D nn, n1]
+    A = np.random.default_rng(D + 1).normal(size=(D, D))
+    c0 = np.arange(1.0, D + 1)
+    n1, _ = oracle.qk_tabulate(D, 1, oracle.qk_nodes(D, 2))
+    P = CO.q1_map_points(X, n1)                                  # Q2 nodes [C, nn, D]
+    U = np.einsum("ij,cbj->cib", A, P) + c0[None, :, None]       # [C, D, nn]
+    Bu = np.einsum("cka,ck->ca", B, U.reshape(C, D * nn))
+    ref = kappa[:, None] * np.trace(A) * np.einsum("cq,qa->ca", wdet, s1)
+    np.testing.assert_allclose(Bu, ref, rtol=0, atol=1e-12 * np.abs(ref).max())
+    Bt = CO.divergence_cells(wdet, s1, gp2, scalar_test=False, kappa=kappa)
+    np.testing.assert_array_equal(Bt, B.transpose(0, 2, 1))
+    L = CO.vector_laplacian_cells(wdet, gp2)
+    K = CO.ncube_diffusion_cells(wdet, gp2)
+    for i in range(D):
+        for j in range(D):
+            blk = L[:, i * nn:(i + 1) * nn, j * nn:(j + 1) * nn]
+            np.testing.assert_array_equal(blk, K if i == j else 0 * K)
+    e = np.tile(np.repeat(np.arange(1.0, D + 1), nn), (C, 1))
+    assert np.abs(np.einsum("cab,cb->ca", L, e)).max() <= 1e-12 * np.abs(L).max()
