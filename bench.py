@@ -133,6 +133,7 @@ def main():
     ap.add_argument("--no-boundary", action="store_true", help="skip the boundary-terms leg")
     ap.add_argument("--no-multifield", action="store_true", help="skip the two-field leg")
     ap.add_argument("--mf-n", type=int, default=48)
+    ap.add_argument("--no-stokes", action="store_true", help="skip the Taylor-Hood Stokes leg")
     ap.add_argument("--seed", type=int, default=2024)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -308,6 +309,8 @@ def main():
         out["boundary_terms_assembly"] = boundary_leg(args.n, ev)
     if not args.no_multifield:
         out["multifield_assembly"] = multifield_leg(args.mf_n)
+    if not args.no_stokes:
+        out["stokes_taylor_hood_assembly"] = stokes_leg(ev)
     print(json.dumps(out), flush=True)
 
 
@@ -938,6 +941,59 @@ def multifield_leg(n, reps=3):
             "free_dofs": 2 * V.num_free_dofs(), "nnz": nnz, "first_assembly_s": first_s,
             "wall_s_per_assembly": walls[k], "host_compose_s": comps[k],
             "value": nnz / walls[k], "unit": "NNZ/s"}
+
+
+def stokes_leg(ev, n=256, reps=2, steps=5):
+    """Taylor–Hood Stokes (StokesTaylorHoodTests.jl's form; vector Q2 velocity, Q1
+    pressure) on (0,2)² with n x n cells, velocity Dirichlet data on the boundary, the
+    pressure fixed at one vertex: assemble_matrix_and_vector end to end (wall clock),
+    and each block's device numeric phase (HIP events on its context's stream): the
+    vector Laplacian, q (∇⋅u) and (∇⋅v) p."""
+    import gridap_hip as G
+    from gridap_hip import _lib
+    model = G.CartesianDiscreteModel((0, 2, 0, 2), (n, n))
+    ue = lambda x: np.stack([x[..., 0] ** 2 + 2 * x[..., 1] ** 2, -x[..., 0] ** 2], axis=-1)
+    pe = lambda x: x[..., 0] + 3 * x[..., 1]
+    V = G.TestFESpace(model, G.ReferenceFE(G.lagrangian, G.VectorValue[2, G.Float64], 2),
+                      dirichlet_tags="boundary")
+    Q = G.TestFESpace(model, G.ReferenceFE(G.lagrangian, G.Float64, 1), dirichlet_tags="tag_1")
+    U, P = G.TrialFESpace(V, ue), G.TrialFESpace(Q, pe)
+    Y, X = G.MultiFieldFESpace([V, Q]), G.MultiFieldFESpace([U, P])
+    dΩ = G.Measure(G.Triangulation(model), 2)
+    g = lambda x: 2.0 * x[..., 0]
+    a = lambda u, v: G.integrate(G.inner(G.gradient(v[0]), G.gradient(u[0])) -
+                                 G.divergence(v[0]) * u[1] + v[1] * G.divergence(u[0]), dΩ)
+    l = lambda v: G.integrate(G.dot(v[0], G.VectorValue(-5.0, 5.0)) + v[1] * g, dΩ)
+    assem = G.SparseMatrixAssembler(X, Y)
+    t = time.time()
+    A, b = G.assemble_matrix_and_vector(a, l, X, Y, assem)
+    first_s = time.time() - t
+    walls = []
+    for _ in range(reps):
+        t = time.time()
+        A, b = G.assemble_matrix_and_vector(a, l, X, Y, assem)
+        walls.append(time.time() - t)
+    assert np.all(np.isfinite(A.nzval)) and np.all(np.isfinite(b))
+    blocks = {}
+    for name, (i, j), form in (("vector_laplacian", (0, 0), _lib.GH_FORM_DIFFUSION),
+                               ("q_div_u", (1, 0), _lib.GH_FORM_DIVERGENCE),
+                               ("div_v_p", (0, 1), _lib.GH_FORM_DIVERGENCE)):
+        pa = assem._pair(i, j)
+        ctx = pa.ctx
+        path = _lib.GH_PATH_GENERAL if form == _lib.GH_FORM_DIVERGENCE else pa.path
+        # the pair's matrix assembly as it last ran: its coefficients are on the device
+        if form == _lib.GH_FORM_DIFFUSION:
+            ctx.coef_upload(None, np.zeros(model.num_cells()), None)
+            ctx.elasticity_params(0.3, np.zeros(3))
+        run = lambda: ctx.assemble(form, _lib.GH_PREC_FP64, path)
+        ms = timed_ms(ev, ctx.stream(), run, steps)
+        blocks[name] = {"ms": ms, "nnz": int(ctx.nnz), "kernel": ctx.pattern_info()["last_kernel"],
+                        "value": ctx.nnz / (ms * 1e-3), "unit": "NNZ/s"}
+    return {"workload": "Taylor-Hood Stokes (vector Q2 / Q1) on (0,2)^2, %d x %d cells, "
+                        "velocity Dirichlet boundary, pressure fixed at one vertex" % (n, n),
+            "free_dofs": int(V.nfree + Q.nfree), "nnz": int(A.colptr[-1] - 1),
+            "first_assembly_s": first_s, "wall_s_per_assembly": min(walls),
+            "blocks_fp64_device": blocks}
 
 
 def fsai_leg(k):

@@ -112,68 +112,105 @@ __global__ __launch_bounds__(kBlock) void rect_elem_kernel(const RectArgs A, T *
   const double sig = A.sigma ? A.sigma[cell] : 0.0;
   T *Ec = E + cell * ns * nt;
   double *Evc = Ev + cell * ns;
-  for (int k = 0; k < ns * nt; ++k) Ec[k] = 0;
-  double X[8][D];
-  if (A.gtab)
-    for (int v = 0; v < A.nv; ++v)
-      for (int d = 0; d < D; ++d) X[v][d] = A.xyz[(int64_t)A.cell_nodes[cell * A.nv + v] * D + d];
-  for (int q = 0; q < A.nq; ++q) {
-    const double *tq = A.ttab + (size_t)q * stride;
-    double Ji[D][D], det = 1.0;
-    if (A.gtab) {
-      // J_de = ∂x_d/∂ξ_e of the Q1 map; ∇φ = J⁻ᵀ∇̂φ
-      double J[D * D], Jinv[D * D];
-      for (int k = 0; k < D * D; ++k) J[k] = 0.0;
-      const double *gq = A.gtab + (size_t)q * A.nv * D;
-      for (int v = 0; v < A.nv; ++v)
-        for (int d = 0; d < D; ++d)
-          for (int e = 0; e < D; ++e) J[d * D + e] += X[v][d] * gq[v * D + e];
-      det = det_inv<D>(J, Jinv);
-      for (int d = 0; d < D; ++d)
-        for (int e = 0; e < D; ++e) Ji[d][e] = Jinv[d * D + e];
-    } else {
-      for (int d = 0; d < D; ++d) {
-        det *= A.h[d];
-        for (int e = 0; e < D; ++e) Ji[d][e] = d == e ? 1.0 / A.h[d] : 0.0;
-      }
+  if (!A.gtab) {
+    // Cartesian cells: J = diag(h) at every point, so each entry is summed over the
+    // points in a register and stored once
+    double ih[D], det = 1.0;
+    for (int d = 0; d < D; ++d) {
+      ih[d] = 1.0 / A.h[d];
+      det *= A.h[d];
     }
-    const double w = tq[0] * fabs(det);
-    const double *psi = tq + 1, *gpsi = psi + nns, *phi = gpsi + nns * D, *gphi = phi + ntn;
-    if (A.div && nt > ntn) {  // scalar test, vector trial: ψ_a ∂_j φ_b
-      for (int b = 0; b < ntn; ++b)
-        for (int j = 0; j < D; ++j) {
-          double g = 0.0;
-          for (int e = 0; e < D; ++e) g += Ji[e][j] * gphi[b * D + e];
-          const double wg = w * kap * g;
-          for (int a = 0; a < nns; ++a) Ec[(b + ntn * j) * ns + a] += (T)(wg * psi[a]);
-        }
-      continue;
-    }
-    if (A.div) {  // vector test, scalar trial: ∂_i ψ_a φ_b
-      for (int a = 0; a < nns; ++a)
-        for (int i = 0; i < D; ++i) {
-          double g = 0.0;
-          for (int e = 0; e < D; ++e) g += Ji[e][i] * gpsi[a * D + e];
-          const double wg = w * kap * g;
-          for (int b = 0; b < ntn; ++b) Ec[b * ns + a + nns * i] += (T)(wg * phi[b]);
-        }
-      continue;
-    }
-    for (int lj = 0; lj < nt; ++lj) {
-      double gj[D];
-      for (int d = 0; d < D; ++d) {
-        double s = 0.0;
-        for (int e = 0; e < D; ++e) s += Ji[e][d] * gphi[lj * D + e];
-        gj[d] = s;
-      }
+    det = fabs(det);
+    for (int lj = 0; lj < nt; ++lj)
       for (int li = 0; li < ns; ++li) {
-        double dot = 0.0;
+        double s = 0.0;
+        for (int q = 0; q < A.nq; ++q) {
+          const double *tq = A.ttab + (size_t)q * stride;
+          const double *psi = tq + 1, *gpsi = psi + nns, *phi = gpsi + nns * D,
+                       *gphi = phi + ntn;
+          const double w = tq[0] * det;
+          double v;
+          if (A.div && nt > ntn) {  // (b, j) x a: ψ_a ∂_j φ_b
+            const int b = lj % ntn, j = lj / ntn;
+            v = kap * psi[li] * (gphi[b * D + j] * ih[j]);
+          } else if (A.div) {       // b x (a, i): ∂_i ψ_a φ_b
+            const int a = li % nns, i = li / nns;
+            v = kap * (gpsi[a * D + i] * ih[i]) * phi[lj];
+          } else {
+            double dot = 0.0;
+            for (int d = 0; d < D; ++d)
+              dot += (gpsi[li * D + d] * ih[d]) * (gphi[lj * D + d] * ih[d]);
+            v = kap * dot + sig * (psi[li] * phi[lj]);
+          }
+          s += w * v;
+        }
+        Ec[lj * ns + li] = (T)s;
+      }
+  } else {
+    // mapped cells: the Jacobian of the Q1 map at each point
+    for (int k = 0; k < ns * nt; ++k) Ec[k] = 0;
+    double X[8][D];
+    if (A.gtab)
+      for (int v = 0; v < A.nv; ++v)
+        for (int d = 0; d < D; ++d) X[v][d] = A.xyz[(int64_t)A.cell_nodes[cell * A.nv + v] * D + d];
+    for (int q = 0; q < A.nq; ++q) {
+      const double *tq = A.ttab + (size_t)q * stride;
+      double Ji[D][D], det = 1.0;
+      if (A.gtab) {
+        // J_de = ∂x_d/∂ξ_e of the Q1 map; ∇φ = J⁻ᵀ∇̂φ
+        double J[D * D], Jinv[D * D];
+        for (int k = 0; k < D * D; ++k) J[k] = 0.0;
+        const double *gq = A.gtab + (size_t)q * A.nv * D;
+        for (int v = 0; v < A.nv; ++v)
+          for (int d = 0; d < D; ++d)
+            for (int e = 0; e < D; ++e) J[d * D + e] += X[v][d] * gq[v * D + e];
+        det = det_inv<D>(J, Jinv);
+        for (int d = 0; d < D; ++d)
+          for (int e = 0; e < D; ++e) Ji[d][e] = Jinv[d * D + e];
+      } else {
+        for (int d = 0; d < D; ++d) {
+          det *= A.h[d];
+          for (int e = 0; e < D; ++e) Ji[d][e] = d == e ? 1.0 / A.h[d] : 0.0;
+        }
+      }
+      const double w = tq[0] * fabs(det);
+      const double *psi = tq + 1, *gpsi = psi + nns, *phi = gpsi + nns * D, *gphi = phi + ntn;
+      if (A.div && nt > ntn) {  // scalar test, vector trial: ψ_a ∂_j φ_b
+        for (int b = 0; b < ntn; ++b)
+          for (int j = 0; j < D; ++j) {
+            double g = 0.0;
+            for (int e = 0; e < D; ++e) g += Ji[e][j] * gphi[b * D + e];
+            const double wg = w * kap * g;
+            for (int a = 0; a < nns; ++a) Ec[(b + ntn * j) * ns + a] += (T)(wg * psi[a]);
+          }
+        continue;
+      }
+      if (A.div) {  // vector test, scalar trial: ∂_i ψ_a φ_b
+        for (int a = 0; a < nns; ++a)
+          for (int i = 0; i < D; ++i) {
+            double g = 0.0;
+            for (int e = 0; e < D; ++e) g += Ji[e][i] * gpsi[a * D + e];
+            const double wg = w * kap * g;
+            for (int b = 0; b < ntn; ++b) Ec[b * ns + a + nns * i] += (T)(wg * phi[b]);
+          }
+        continue;
+      }
+      for (int lj = 0; lj < nt; ++lj) {
+        double gj[D];
         for (int d = 0; d < D; ++d) {
           double s = 0.0;
-          for (int e = 0; e < D; ++e) s += Ji[e][d] * gpsi[li * D + e];
-          dot += s * gj[d];
+          for (int e = 0; e < D; ++e) s += Ji[e][d] * gphi[lj * D + e];
+          gj[d] = s;
         }
-        Ec[lj * ns + li] += (T)(w * (kap * dot + sig * (psi[li] * phi[lj])));
+        for (int li = 0; li < ns; ++li) {
+          double dot = 0.0;
+          for (int d = 0; d < D; ++d) {
+            double s = 0.0;
+            for (int e = 0; e < D; ++e) s += Ji[e][d] * gpsi[li * D + e];
+            dot += s * gj[d];
+          }
+          Ec[lj * ns + li] += (T)(w * (kap * dot + sig * (psi[li] * phi[lj])));
+        }
       }
     }
   }
