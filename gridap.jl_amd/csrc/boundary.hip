@@ -122,6 +122,38 @@ __global__ __launch_bounds__(128) void boundary_face_kernel(const FaceArgs A) {
   for (int a = 0; a < NL; ++a) A.r[a * A.nfaces + f] = r[a];
 }
 
+// Vector spaces (NC components sharing the NL scalar shape functions, local dofs
+// node + NL·component): the loads ∫_F φ_a g_i of every face, g_i at the face points
+// (gq[(f*nqf + q)*NC + i]); no face matrix, no lifting.
+template <int NL, int NC>
+__global__ __launch_bounds__(128) void boundary_face_load_kernel(const FaceArgs A) {
+  constexpr int D = NC;
+  const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= A.nfaces) return;
+  const int side = A.face_side[f];
+  const int axis = D - 1 - side / 2;
+  double jf = 1.0;
+#pragma unroll
+  for (int k = 0; k < D; ++k)
+    if (k != axis) jf *= A.h[k];
+  double r[NL * NC];
+#pragma unroll
+  for (int k = 0; k < NL * NC; ++k) r[k] = 0.0;
+  const int stride = 1 + NL + NL * D;
+  for (int q = 0; q < A.nqf; ++q) {
+    const double *tq = A.tab + ((size_t)side * A.nqf + q) * stride;
+    const double dS = tq[0] * jf;
+#pragma unroll
+    for (int i = 0; i < NC; ++i) {
+      const double g = A.gq ? A.gq[(f * A.nqf + q) * NC + i] : 0.0;
+#pragma unroll
+      for (int a = 0; a < NL; ++a) r[a + NL * i] += dS * (tq[1 + a] * g);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < NL * NC; ++k) A.r[k * A.nfaces + f] = r[k];
+}
+
 struct SumArgs {
   int64_t ndst, nown;
   const int32_t *dst_ptr, *dst_ci, *dst_pos;  // contributions: indices into K
@@ -209,10 +241,12 @@ void boundary_lists(Ctx &c) {
   }
   std::vector<int32_t> ofa(ef.size());
   for (size_t k = 0; k < ef.size(); ++k) ofa[k] = (int32_t)(ea[k] * F + ef[k]);
-  // destinations of each owner column j: rows i of the faces' cells
+  // destinations of each owner column j: rows i of the faces' cells (none when the
+  // faces carry loads only)
+  const bool matrix = c.have_balpha || c.have_bbeta;
   std::vector<int32_t> dptr(1, 0), dci, drow, dcol;
   std::vector<std::pair<int32_t, int32_t>> tmp;  // (row, contribution)
-  for (size_t o = 0; o < own.size(); ++o) {
+  for (size_t o = 0; matrix && o < own.size(); ++o) {
     tmp.clear();
     for (int32_t k = optr[o]; k < optr[o + 1]; ++k) {
       const int64_t f = ef[k];
@@ -255,7 +289,7 @@ void boundary_lists(Ctx &c) {
                        c.rowval.get(), c.bdst_pos.get());
     GH_CHECK_HIP(hipGetLastError());
   }
-  c.bface_K.resize(std::max<int64_t>((int64_t)nt * F, 1));
+  c.bface_K.resize(matrix ? std::max<int64_t>((int64_t)nt * F, 1) : 1);
   c.bface_r.resize(std::max<int64_t>((int64_t)nl * F, 1));
   c.bown_n = (int64_t)own.size();
   c.bdst_n = nd;  // built: kept until the faces, the space or the pattern change
@@ -265,12 +299,15 @@ void boundary_lists(Ctx &c) {
 
 void assemble_boundary(Ctx &c) {
   if (c.nbfaces == 0) return;
-  const bool q1 = c.order == 1 && c.nloc == (1 << c.dim);
-  const bool q2quad = c.order == 2 && c.dim == 2 && c.nloc == 9;
-  GH_REQUIRE(c.cartesian && c.reffe_kind == GH_REFFE_QK && c.ncomp == 1 && (q1 || q2quad),
+  const int nn = c.nloc / c.ncomp;  // scalar shape functions
+  const bool q1 = c.order == 1 && nn == (1 << c.dim);
+  const bool q2quad = c.order == 2 && c.dim == 2 && nn == 9;
+  const bool vec = c.ncomp > 1;
+  GH_REQUIRE(c.cartesian && c.reffe_kind == GH_REFFE_QK && (c.ncomp == 1 || c.ncomp == c.dim) &&
+                 (q1 || q2quad),
              GH_ERR_UNSUPPORTED,
-             "boundary terms are provided for scalar Q1, and Q2 on quadrilaterals, on "
-             "Cartesian meshes");
+             "boundary terms are provided for Q1, and Q2 on quadrilaterals, on Cartesian "
+             "meshes (vector spaces: loads only)");
   GH_REQUIRE(c.bface_tab_ok, GH_ERR_STATE, "no face tabulation: call gh_face_tabulation");
   if (c.bdst_n < 0) boundary_lists(c);
   FaceArgs A{};
@@ -289,7 +326,13 @@ void assemble_boundary(Ctx &c) {
   A.K = c.bface_K.get();
   A.r = c.bface_r.get();
   const unsigned fgrid = (unsigned)((c.nbfaces + 127) / 128);
-  if (q2quad)
+  if (vec && q2quad)
+    hipLaunchKernelGGL((boundary_face_load_kernel<9, 2>), dim3(fgrid), dim3(128), 0, c.stream, A);
+  else if (vec && c.dim == 2)
+    hipLaunchKernelGGL((boundary_face_load_kernel<4, 2>), dim3(fgrid), dim3(128), 0, c.stream, A);
+  else if (vec)
+    hipLaunchKernelGGL((boundary_face_load_kernel<8, 3>), dim3(fgrid), dim3(128), 0, c.stream, A);
+  else if (q2quad)
     hipLaunchKernelGGL((boundary_face_kernel<2, 9>), dim3(fgrid), dim3(128), 0, c.stream, A);
   else if (c.dim == 2)
     hipLaunchKernelGGL((boundary_face_kernel<2, 4>), dim3(fgrid), dim3(128), 0, c.stream, A);

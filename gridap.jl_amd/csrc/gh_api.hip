@@ -419,7 +419,8 @@ int gh_face_tabulation(gh_ctx *ctx, int nsides, int nqf, const double *qw, const
     GH_REQUIRE(nsides == 2 * c.dim, GH_ERR_ARG, "one tabulation per facet of the n-cube");
     GH_REQUIRE(nqf >= 1 && nqf <= 64, GH_ERR_ARG, "face quadrature size out of range");
     GH_REQUIRE(qw && shp && grad, GH_ERR_ARG, "NULL face tabulation array");
-    const int NL = c.nloc, D = c.dim, stride = 1 + NL + NL * D;
+    // the scalar shape functions (a vector space's components share them)
+    const int NL = c.nloc / c.ncomp, D = c.dim, stride = 1 + NL + NL * D;
     std::vector<double> tab((size_t)nsides * nqf * stride);
     for (int s = 0; s < nsides; ++s)
       for (int q = 0; q < nqf; ++q) {
@@ -452,6 +453,8 @@ int gh_boundary_upload(gh_ctx *ctx, int64_t nfaces, const int32_t *face_cells,
     GH_REQUIRE(face_cells && face_sides, GH_ERR_ARG, "NULL face arrays");
     GH_REQUIRE(c.bface_tab_ok && c.nqf > 0, GH_ERR_STATE,
                "upload the face tabulation (gh_face_tabulation) first");
+    GH_REQUIRE(c.ncomp == 1 || (!alpha && !beta && !hq), GH_ERR_UNSUPPORTED,
+               "vector spaces take boundary loads ∫ v⋅g only (gq with ncomp values per point)");
     std::vector<int32_t> fc(nfaces), fs(nfaces);
     for (int64_t f = 0; f < nfaces; ++f) {
       GH_REQUIRE(face_cells[f] >= 1 && face_cells[f] <= c.ncells, GH_ERR_RANGE,
@@ -475,7 +478,7 @@ int gh_boundary_upload(gh_ctx *ctx, int64_t nfaces, const int32_t *face_cells,
     };
     up(c.balpha, c.have_balpha, alpha, nfaces);
     up(c.bbeta, c.have_bbeta, beta, nfaces);
-    up(c.bg, c.have_bg, gq, (size_t)nfaces * c.nqf);
+    up(c.bg, c.have_bg, gq, (size_t)nfaces * c.nqf * c.ncomp);
     up(c.bh, c.have_bh, hq, (size_t)nfaces * c.nqf);
     c.h_face_cell.swap(fc);
     c.nbfaces = nfaces;
@@ -612,6 +615,7 @@ int gh_assemble(gh_ctx *ctx, int form_kind, int precision, int path) {
       } else {
         gh::assemble_elasticity(c, precision, path);
       }
+      gh::assemble_boundary(c);  // vector spaces: boundary loads
     } else {
       GH_REQUIRE(!gh::hexq2_eligible(c), GH_ERR_UNSUPPORTED,
                  "vector Q2 spaces assemble the elasticity form");

@@ -236,8 +236,9 @@ class SparseMatrixAssembler:
             extra["fq"] = np.ascontiguousarray(fq)
             fsrc = np.zeros(ncells)
         if mface or vface:
-            if form != _lib.GH_FORM_DIFFUSION:
-                raise NotImplementedError("boundary terms are provided with the diffusion form")
+            if form != _lib.GH_FORM_DIFFUSION and not (self.space.ncomp > 1 and not mface):
+                raise NotImplementedError("boundary terms are provided with the diffusion "
+                                          "form (vector spaces: loads with either form)")
             extra["boundary"] = lower_boundary(self.space.model, self.space.reffe, mface, vface)
         return form, kappa, fsrc, degs.pop(), extra
 

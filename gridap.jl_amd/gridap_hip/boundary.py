@@ -11,7 +11,8 @@ arrays, functions of x, FE functions uh and n⋅∇uh) at the face points.
 import numpy as np
 
 from .celldata import (CellField, CoefProduct, FEValue, MassTerm, NitscheTerm,
-                       NormalGradValue, NormalLoadTerm, PointField, SourceTerm)
+                       NormalGradValue, NormalLoadTerm, NormalTensorField, PointField,
+                       SourceTerm, VectorLoadTerm)
 from .reference_fes import facet_points, facet_quadrature
 
 
@@ -42,6 +43,7 @@ class FaceSet:
         h = np.asarray(model.sizes, dtype=np.float64)
         axis = D - 1 - self.sides // 2
         sgn = np.where(self.sides & 1, 1.0, -1.0)
+        self.axis, self.sgn = axis, sgn                      # outward normal sgn e_axis
         # n⋅∇φ_a at the face points: ±∂φ̂_a/∂ξ_axis / h_axis
         self.dn = (self.grad[self.sides, :, :, :][np.arange(len(axis)), :, :, axis] *
                    (sgn / h[axis])[:, None, None])          # [f, q, a]
@@ -97,12 +99,50 @@ def eval_coef(coef, fs, pos, ncells):
     raise NotImplementedError("unsupported boundary coefficient %r" % (coef,))
 
 
+def _vector_loads(terms, fs, ncells, D):
+    """g [nfaces, nqf, D] of the vector boundary loads (v⋅g): v⋅(n⋅T(x)) and (n⋅v) c."""
+    F, nq = len(fs.cells), fs.nqf
+    g = np.zeros((F, nq, D))
+    for t, m in terms:
+        if not isinstance(t, VectorLoadTerm):
+            raise NotImplementedError("vector spaces take the boundary loads v⋅(n⋅∇u) and "
+                                      "(n⋅v) p only")
+        pos = fs.positions(m.trian)
+        n = np.zeros((len(pos), D))
+        n[np.arange(len(pos)), fs.axis[pos]] = fs.sgn[pos]
+        if t.kind == "vec":
+            if not isinstance(t.coef, NormalTensorField):
+                raise NotImplementedError("v⋅g on a boundary: g = n⋅T(x)")
+            X = fs.x[pos].reshape(-1, D)
+            T = np.asarray(t.coef.func(X), dtype=np.float64).reshape(len(pos), nq, D, D)
+            val = np.einsum("fi,fqij->fqj", n, T)
+        else:
+            c = eval_coef(t.coef, fs, pos, ncells)                        # [f, q]
+            val = c[:, :, None] * n[:, None, :]
+        np.add.at(g, pos, val)
+    return g
+
+
 def lower_boundary(model, reffe, mat_terms, vec_terms):
     """(term, measure) lists over boundary measures -> FaceSet and per-face data
-    (alpha, beta or None; gq, hq [nfaces, nqf] or None)."""
+    (alpha, beta or None; gq, hq [nfaces, nqf] or None; for a vector-valued space gq is
+    [nfaces, nqf, ncomp] and only loads are taken)."""
     terms = list(mat_terms) + list(vec_terms)
     if not terms:
         return None
+    if getattr(reffe, "ncomp", 1) > 1:
+        if mat_terms:
+            raise NotImplementedError("vector spaces take boundary loads only")
+        degs = {m.degree for _, m in terms}
+        if len(degs) > 1:
+            raise NotImplementedError("boundary terms must share one quadrature degree")
+        trians = []
+        for _, m in terms:
+            if all(m.trian is not t for t in trians):
+                trians.append(m.trian)
+        fs = FaceSet(model, reffe, trians, degs.pop())
+        g = _vector_loads(vec_terms, fs, model.num_cells(), model.D)
+        return dict(faces=fs, alpha=None, beta=None, gq=np.ascontiguousarray(g), hq=None)
     degs = {m.degree for _, m in terms}
     if len(degs) > 1:
         raise NotImplementedError("boundary terms must share one quadrature degree")

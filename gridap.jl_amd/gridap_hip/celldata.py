@@ -241,6 +241,31 @@ class DivergenceTerm(_Expr):
         self.part, self.coef = part, coef
 
 
+class _NormalComp(_Expr):
+    """n⋅v of the test basis of a vector-valued space on a boundary."""
+
+    def __init__(self, basis):
+        self.basis = basis
+
+
+class NormalTensorField:
+    """n⋅T(x) of a tensor-valued function T (T[..., i, j], Gridap's ∇u convention
+    ∇u[i, j] = ∂_i u_j) at the boundary quadrature points: the vector Σ_i n_i T_ij."""
+
+    def __init__(self, func):
+        self.func = func
+
+
+class VectorLoadTerm(_Expr):
+    """A boundary load on a vector-valued test basis: v⋅g with g = coef (a
+    NormalTensorField or a vector function of x) for kind "vec", or (n⋅v) coef with a
+    scalar coef for kind "normal" (the terms v⋅(n⋅∇u) and -(n⋅v) p of
+    test/GridapTests/StokesTaylorHoodTests.jl:56)."""
+
+    def __init__(self, kind, coef=None):
+        self.kind, self.coef = kind, coef
+
+
 class MassTerm(_Expr):
     """coef * v u (test times trial)."""
 
@@ -352,6 +377,10 @@ def _product(x, y):
             return _Div(x.basis, _coef_product(x.coef, _as_coef(y)))
     if isinstance(x, _Sum) and isinstance(y, FEBasis):               # (Σ terms) * basis
         return _Sum([_scale(y, t) for t in x.terms])
+    if isinstance(x, _NormalComp) and cy:                            # (n⋅v) * p
+        return _f(VectorLoadTerm("normal", _as_coef(y)), (x.basis.field,))
+    if isinstance(x, VectorLoadTerm) and cy and x.kind == "normal":
+        return _f(VectorLoadTerm("normal", _coef_product(x.coef, _as_coef(y))), x.fields)
     if isinstance(x, _NormalGrad) and not x.basis.trial:            # (n⋅∇v) * y
         if isinstance(y, FEBasis) and y.trial:
             return NitscheTerm("dnv_u")
@@ -465,7 +494,16 @@ def inner(a, b):
             return _NormalGrad(b.basis)
         if isinstance(b, _GradState):
             return NormalGradValue(b.uh)
-        raise NotImplementedError("n⋅x is provided for gradients only")
+        if isinstance(b, FEBasis) and not b.trial:
+            return _NormalComp(b)
+        if callable(b) and not isinstance(b, (_Expr, FEBasis)):
+            return NormalTensorField(b)
+        raise NotImplementedError("n⋅x is provided for gradients, test bases and tensor "
+                                  "functions only")
+    if isinstance(a, NormalTensorField):
+        a, b = b, a
+    if isinstance(a, FEBasis) and not a.trial and isinstance(b, NormalTensorField):
+        return _f(VectorLoadTerm("vec", b), (a.field,))
     if isinstance(a, _Grad) and isinstance(b, _Grad) and a.basis.trial != b.basis.trial:
         v, u = (b, a) if a.basis.trial else (a, b)
         return _f(DiffusionTerm(), (v.basis.field, u.basis.field))
@@ -525,7 +563,7 @@ class DomainContribution:
 
 _CELL_TERMS = (DiffusionTerm, MassTerm, SourceTerm, ElasticityTerm, BodyForceTerm,
                PLaplacianResidualTerm, PLaplacianJacobianTerm, DivergenceTerm)
-_FACE_TERMS = (MassTerm, NitscheTerm, SourceTerm, NormalLoadTerm)
+_FACE_TERMS = (MassTerm, NitscheTerm, SourceTerm, NormalLoadTerm, VectorLoadTerm)
 
 
 def integrate(expr, measure):

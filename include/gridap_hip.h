@@ -179,6 +179,10 @@ int gh_face_tabulation(gh_ctx *ctx, int nsides, int nqf, const double *qw, const
  * gq[f*nqf + q], hq[f*nqf + q] (NULL = 0), followed by the Dirichlet lifting of the face
  * matrix.  Neumann: g = n⋅∇u; Nitsche (test/GridapTests/PoissonTests.jl:101-108):
  * α = γ/h, β = -1, g = (γ/h) u_D, h = -u_D.  nfaces = 0 removes the boundary terms.
+ * Vector spaces (one component per dimension) take loads only: b += ∫_F φ_a g_i with
+ * gq[(f*nqf + q)*ncomp + i] (α, β, h NULL), e.g. v⋅(n⋅∇u) - (n⋅v) p of
+ * test/GridapTests/StokesTaylorHoodTests.jl:56; they are added after the cell terms
+ * of GH_FORM_DIFFUSION or GH_FORM_ELASTICITY.
  * The first gh_assemble after an upload (or a new pattern) builds the face lists: for
  * each nonzero the faces touch, its face contributions (faces ascending). */
 int gh_boundary_upload(gh_ctx *ctx, int64_t nfaces, const int32_t *face_cells,

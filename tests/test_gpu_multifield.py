@@ -342,3 +342,67 @@ def test_stokes_taylor_hood_matches_oracle_and_solution(oracle, part):
     A32, b32, _ = _stokes(part, precision="fp32")
     assert np.linalg.norm(A32.nzval - nz) <= 1e-6 * np.linalg.norm(nz)
     assert relmax(b32, br) <= 1e-6
+
+
+def test_stokes_taylor_hood_driver_as_written(oracle):
+    """test/GridapTests/StokesTaylorHoodTests.jl as written: (0,2)², 3×3 cells,
+    "dirichlet" = entities [1,2,5] for the velocity, "neumann" = [6,7,8] carrying
+    ∫ v⋅(n⋅∇u) - (n⋅v) p dΓ, no pressure constraint.  The composite system matches the
+    oracle (pattern bit-exact, 1e-12) and its solve reproduces u and p to 1e-9."""
+    import scipy.sparse.linalg as spla
+    announce("Stokes Taylor-Hood driver with Neumann terms")
+    model = G.CartesianDiscreteModel((0, 2, 0, 2), (3, 3))
+    labels = model.get_face_labeling()
+    labels.add_tag_from_tags("dirichlet", [1, 2, 5])
+    labels.add_tag_from_tags("neumann", [6, 7, 8])
+    ue = lambda x: np.stack([x[..., 0] ** 2 + 2 * x[..., 1] ** 2, -x[..., 0] ** 2], axis=-1)
+    pe = lambda x: x[..., 0] + 3 * x[..., 1]
+    # ∇u[i, j] = ∂_i u_j
+    gu = lambda x: np.stack([np.stack([2 * x[..., 0], -2 * x[..., 0]], -1),
+                             np.stack([4 * x[..., 1], 0 * x[..., 1]], -1)], -2)
+    V = G.TestFESpace(model, G.ReferenceFE(G.lagrangian, G.VectorValue[2, G.Float64], 2),
+                      labels=labels, dirichlet_tags="dirichlet")
+    Q = G.TestFESpace(model, G.ReferenceFE(G.lagrangian, G.Float64, 1), labels=labels)
+    U, P = G.TrialFESpace(V, ue), G.TrialFESpace(Q)
+    Y, X = G.MultiFieldFESpace([V, Q]), G.MultiFieldFESpace([U, P])
+    Ω = G.Triangulation(model)
+    Γ = G.BoundaryTriangulation(model, labels, tags="neumann")
+    nΓ = G.get_normal_vector(Γ)
+    dΩ, dΓ = G.Measure(Ω, 2), G.Measure(Γ, 2)
+    g = lambda x: 2.0 * x[..., 0]
+    a = lambda u, v: G.integrate(G.inner(G.gradient(v[0]), G.gradient(u[0])) -
+                                 G.divergence(v[0]) * u[1] + v[1] * G.divergence(u[0]), dΩ)
+    l = lambda v: (G.integrate(G.dot(v[0], G.VectorValue(-5.0, 5.0)) + v[1] * g, dΩ) +
+                   G.integrate(G.dot(v[0], G.dot(nΓ, gu)) - G.dot(nΓ, v[0]) * pe, dΓ))
+    A, b = G.assemble_matrix_and_vector(a, l, X, Y, G.SparseMatrixAssembler(X, Y))
+    # oracle: the cell blocks and loads of _stokes_oracle plus the Neumann face loads
+    import conforming_oracle as CO
+    Xc = model.node_coordinates()[model.cell_node_ids() - 1]
+    qx, qw = oracle.tensor_quadrature(2, 2)
+    s1, g1 = oracle.qk_tabulate(2, 1, qx)
+    s2, g2 = oracle.qk_tabulate(2, 2, qx)
+    wdet, gp2 = CO.physical_gradients(Xc, qw, g1, g2)
+    C = Xc.shape[0]
+    mats = {(0, 0): CO.vector_laplacian_cells(wdet, gp2),
+            (1, 0): CO.divergence_cells(wdet, s1, gp2, scalar_test=True),
+            (0, 1): -CO.divergence_cells(wdet, s1, gp2, scalar_test=False)}
+    xq = CO.q1_map_points(Xc, s1)
+    v0 = CO.ncube_loads(wdet, s2, np.tile([-5.0, 5.0], (C, 1)))
+    mo = oracle.cartesian_model((0, 2, 0, 2), (3, 3))
+    faces = oracle.boundary_faces(mo, [6, 7, 8])
+    for i in range(2):
+        gi = lambda x, n, i=i: (n @ gu(np.asarray(x)))[i] - n[i] * pe(np.asarray(x))
+        _, fv = oracle.boundary_face_arrays(mo, faces, 2, 0.0, 0.0, gi, None, order=2)
+        for k, fc in enumerate(faces):
+            v0[fc[0], 9 * i:9 * (i + 1)] += fv[k]
+    vecs = {0: v0, 1: np.einsum("cq,cq,qa->ca", wdet, 2.0 * xq[..., 0], s1)}
+    cds = [np.asarray(V.cell_dof_ids), np.asarray(Q.cell_dof_ids)]
+    cp, rv, nz, br = MF.assemble(cds, [V.nfree, Q.nfree], [V.ndirichlet, Q.ndirichlet], mats,
+                                 vecs, dvals=[U.dirichlet_values(), P.dirichlet_values()])
+    np.testing.assert_array_equal(A.colptr, cp)
+    np.testing.assert_array_equal(A.rowval, rv)
+    assert relmax(A.nzval, nz) <= 1e-12 and relmax(b, br) <= 1e-12
+    x = spla.spsolve(A.to_scipy().tocsc(), b)
+    uI, pI = G.interpolate(ue, U).free_values, G.interpolate(pe, P).free_values
+    assert np.max(np.abs(x[:V.nfree] - uI)) <= 1e-9 * np.max(np.abs(uI))
+    assert np.max(np.abs(x[V.nfree:] - pI)) <= 1e-9 * np.max(np.abs(pI))
