@@ -123,33 +123,73 @@ __global__ __launch_bounds__(128) void boundary_face_kernel(const FaceArgs A) {
 }
 
 // Vector spaces (NC components sharing the NL scalar shape functions, local dofs
-// node + NL·component): the loads ∫_F φ_a g_i of every face, g_i at the face points
-// (gq[(f*nqf + q)*NC + i]); no face matrix, no lifting.
+// node + NL·component), component-wise terms (v⊙u, v⊙(n⋅∇u), (n⋅∇v)⊙u): the scalar face
+// matrix (the same for every component) and the loads ∫_F φ_a g_i + ∂nφ_a h_i minus the
+// lifting of the component's Dirichlet dofs; g_i, h_i at the face points
+// ([(f*nqf + q)*NC + i]).
 template <int NL, int NC>
-__global__ __launch_bounds__(128) void boundary_face_load_kernel(const FaceArgs A) {
+__global__ __launch_bounds__(128) void boundary_face_vec_kernel(const FaceArgs A) {
   constexpr int D = NC;
+  constexpr int NT = NL * (NL + 1) / 2;
   const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (f >= A.nfaces) return;
+  const int64_t cell = A.face_cell[f];
   const int side = A.face_side[f];
   const int axis = D - 1 - side / 2;
+  const double sgn = (side & 1) ? 1.0 : -1.0;
   double jf = 1.0;
 #pragma unroll
   for (int k = 0; k < D; ++k)
     if (k != axis) jf *= A.h[k];
-  double r[NL * NC];
+  const double inv_h = 1.0 / A.h[axis];
+  const double al = A.alpha ? A.alpha[f] : 0.0;
+  const double be = A.beta ? A.beta[f] : 0.0;
+  double K[NT], r[NL * NC];
+#pragma unroll
+  for (int t = 0; t < NT; ++t) K[t] = 0.0;
 #pragma unroll
   for (int k = 0; k < NL * NC; ++k) r[k] = 0.0;
   const int stride = 1 + NL + NL * D;
   for (int q = 0; q < A.nqf; ++q) {
     const double *tq = A.tab + ((size_t)side * A.nqf + q) * stride;
     const double dS = tq[0] * jf;
+    double phi[NL], dn[NL];
+#pragma unroll
+    for (int a = 0; a < NL; ++a) {
+      phi[a] = tq[1 + a];
+      dn[a] = sgn * tq[1 + NL + a * D + axis] * inv_h;
+    }
 #pragma unroll
     for (int i = 0; i < NC; ++i) {
       const double g = A.gq ? A.gq[(f * A.nqf + q) * NC + i] : 0.0;
+      const double hh = A.hq ? A.hq[(f * A.nqf + q) * NC + i] : 0.0;
 #pragma unroll
-      for (int a = 0; a < NL; ++a) r[a + NL * i] += dS * (tq[1 + a] * g);
+      for (int a = 0; a < NL; ++a) r[a + NL * i] += dS * (phi[a] * g + dn[a] * hh);
     }
+    int t = 0;
+#pragma unroll
+    for (int a = 0; a < NL; ++a)
+#pragma unroll
+      for (int b = a; b < NL; ++b, ++t)
+        K[t] += dS * (al * (phi[a] * phi[b]) + be * (phi[a] * dn[b] + dn[a] * phi[b]));
   }
+  if (A.dvals) {
+#pragma unroll
+    for (int i = 0; i < NC; ++i)
+#pragma unroll
+      for (int a = 0; a < NL; ++a) {
+        double s = 0.0;
+#pragma unroll
+        for (int b = 0; b < NL; ++b) {
+          const int32_t d = A.cell_dofs[cell * NL * NC + b + NL * i];
+          const int lo = a < b ? a : b, hi = a < b ? b : a;
+          if (d < 0) s += K[lo * NL - lo * (lo - 1) / 2 + (hi - lo)] * A.dvals[-d - 1];
+        }
+        r[a + NL * i] -= s;
+      }
+  }
+#pragma unroll
+  for (int t = 0; t < NT; ++t) A.K[t * A.nfaces + f] = K[t];
 #pragma unroll
   for (int k = 0; k < NL * NC; ++k) A.r[k * A.nfaces + f] = r[k];
 }
@@ -208,7 +248,8 @@ void upload(DBuf<T> &d, const std::vector<T> &h) {
 void boundary_lists(Ctx &c) {
   const int64_t F = c.nbfaces;
   const int nl = c.nloc;
-  const int nt = nl * (nl + 1) / 2;
+  const int nn = nl / c.ncomp;        // the face matrix acts on each component alike
+  const int nt = nn * (nn + 1) / 2;
   GH_REQUIRE((int64_t)nt * F < INT32_MAX, GH_ERR_UNSUPPORTED,
              "too many boundary faces for the 32-bit face lists");
   const std::vector<int32_t> &cd = c.h_cell_dofs;
@@ -252,11 +293,12 @@ void boundary_lists(Ctx &c) {
       const int64_t f = ef[k];
       const int a = ea[k];
       const int64_t cell = c.h_face_cell[f];
-      for (int b = 0; b < nl; ++b) {
-        const int32_t i = cd[cell * nl + b];
+      const int an = a % nn, comp = a / nn;
+      for (int bn = 0; bn < nn; ++bn) {  // rows of the same component
+        const int32_t i = cd[cell * nl + bn + nn * comp];
         if (i <= 0) continue;
-        const int lo = std::min(a, b), hi = std::max(a, b);
-        const int t = lo * nl - lo * (lo - 1) / 2 + (hi - lo);
+        const int lo = std::min(an, bn), hi = std::max(an, bn);
+        const int t = lo * nn - lo * (lo - 1) / 2 + (hi - lo);
         tmp.emplace_back(i, (int32_t)(t * F + f));
       }
     }
@@ -289,7 +331,7 @@ void boundary_lists(Ctx &c) {
                        c.rowval.get(), c.bdst_pos.get());
     GH_CHECK_HIP(hipGetLastError());
   }
-  c.bface_K.resize(matrix ? std::max<int64_t>((int64_t)nt * F, 1) : 1);
+  c.bface_K.resize(std::max<int64_t>((int64_t)nt * F, 1));  // the face kernels always write it
   c.bface_r.resize(std::max<int64_t>((int64_t)nl * F, 1));
   c.bown_n = (int64_t)own.size();
   c.bdst_n = nd;  // built: kept until the faces, the space or the pattern change
@@ -307,7 +349,7 @@ void assemble_boundary(Ctx &c) {
                  (q1 || q2quad),
              GH_ERR_UNSUPPORTED,
              "boundary terms are provided for Q1, and Q2 on quadrilaterals, on Cartesian "
-             "meshes (vector spaces: loads only)");
+             "meshes (vector spaces: component-wise terms)");
   GH_REQUIRE(c.bface_tab_ok, GH_ERR_STATE, "no face tabulation: call gh_face_tabulation");
   if (c.bdst_n < 0) boundary_lists(c);
   FaceArgs A{};
@@ -327,11 +369,11 @@ void assemble_boundary(Ctx &c) {
   A.r = c.bface_r.get();
   const unsigned fgrid = (unsigned)((c.nbfaces + 127) / 128);
   if (vec && q2quad)
-    hipLaunchKernelGGL((boundary_face_load_kernel<9, 2>), dim3(fgrid), dim3(128), 0, c.stream, A);
+    hipLaunchKernelGGL((boundary_face_vec_kernel<9, 2>), dim3(fgrid), dim3(128), 0, c.stream, A);
   else if (vec && c.dim == 2)
-    hipLaunchKernelGGL((boundary_face_load_kernel<4, 2>), dim3(fgrid), dim3(128), 0, c.stream, A);
+    hipLaunchKernelGGL((boundary_face_vec_kernel<4, 2>), dim3(fgrid), dim3(128), 0, c.stream, A);
   else if (vec)
-    hipLaunchKernelGGL((boundary_face_load_kernel<8, 3>), dim3(fgrid), dim3(128), 0, c.stream, A);
+    hipLaunchKernelGGL((boundary_face_vec_kernel<8, 3>), dim3(fgrid), dim3(128), 0, c.stream, A);
   else if (q2quad)
     hipLaunchKernelGGL((boundary_face_kernel<2, 9>), dim3(fgrid), dim3(128), 0, c.stream, A);
   else if (c.dim == 2)

@@ -453,8 +453,6 @@ int gh_boundary_upload(gh_ctx *ctx, int64_t nfaces, const int32_t *face_cells,
     GH_REQUIRE(face_cells && face_sides, GH_ERR_ARG, "NULL face arrays");
     GH_REQUIRE(c.bface_tab_ok && c.nqf > 0, GH_ERR_STATE,
                "upload the face tabulation (gh_face_tabulation) first");
-    GH_REQUIRE(c.ncomp == 1 || (!alpha && !beta && !hq), GH_ERR_UNSUPPORTED,
-               "vector spaces take boundary loads ∫ v⋅g only (gq with ncomp values per point)");
     std::vector<int32_t> fc(nfaces), fs(nfaces);
     for (int64_t f = 0; f < nfaces; ++f) {
       GH_REQUIRE(face_cells[f] >= 1 && face_cells[f] <= c.ncells, GH_ERR_RANGE,
@@ -479,7 +477,7 @@ int gh_boundary_upload(gh_ctx *ctx, int64_t nfaces, const int32_t *face_cells,
     up(c.balpha, c.have_balpha, alpha, nfaces);
     up(c.bbeta, c.have_bbeta, beta, nfaces);
     up(c.bg, c.have_bg, gq, (size_t)nfaces * c.nqf * c.ncomp);
-    up(c.bh, c.have_bh, hq, (size_t)nfaces * c.nqf);
+    up(c.bh, c.have_bh, hq, (size_t)nfaces * c.nqf * c.ncomp);
     c.h_face_cell.swap(fc);
     c.nbfaces = nfaces;
   });
@@ -574,11 +572,13 @@ int gh_source_qp_upload(gh_ctx *ctx, const double *fq) {
       c.fq.release();
       return;
     }
-    GH_REQUIRE(c.reffe_kind == GH_REFFE_QK && c.ncomp == 1 &&
-                   (c.order == 1 || gh::hexgen_eligible(c)),
+    GH_REQUIRE(c.reffe_kind == GH_REFFE_QK &&
+                   ((c.ncomp == 1 && (c.order == 1 || gh::hexgen_eligible(c))) ||
+                    (c.ncomp == c.dim && gh::hexgen_eligible(c))),
                GH_ERR_UNSUPPORTED,
-               "sources at quadrature points are provided for scalar Q1 and Q2 spaces");
-    const size_t n = (size_t)c.ncells * c.nq;
+               "sources at quadrature points are provided for scalar Q1 and Q2 spaces and for "
+               "vector spaces of the general n-cube path (fq[(cell*nq + q)*ncomp + i])");
+    const size_t n = (size_t)c.ncells * c.nq * c.ncomp;
     c.fq.resize(n);
     GH_CHECK_HIP(hipMemcpy(c.fq.get(), fq, sizeof(double) * n, hipMemcpyHostToDevice));
   });

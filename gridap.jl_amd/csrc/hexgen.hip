@@ -197,7 +197,7 @@ __global__ __launch_bounds__(kHgThreads) void hexgen_element_kernel(HgArgs A) {
     double v;
     if (A.fq) {  // Σ_q w|J| f_q φ_a
       for (int q = 0; q < nq; ++q)
-        m += wq[q] * A.fq[cell * nq + q] * A.tab[(size_t)q * stride + 1 + a];
+        m += wq[q] * A.fq[(cell * nq + q) * A.nc + i] * A.tab[(size_t)q * stride + 1 + a];
       v = m;
     } else {
       for (int q = 0; q < nq; ++q) m += wq[q] * A.tab[(size_t)q * stride + 1 + a];
@@ -354,9 +354,8 @@ void hexgen_assemble(Ctx &c, int precision, bool elasticity) {
   GH_REQUIRE(c.elemlist, GH_ERR_STATE, "no element lists for this space");
   GH_REQUIRE(!c.have_sigma || (!elasticity && c.sigma.n == (size_t)c.ncells), GH_ERR_STATE,
              "the cell mass term belongs to the diffusion form of this mesh");
-  GH_REQUIRE(!c.have_fq || c.ncomp == 1, GH_ERR_UNSUPPORTED,
-             "sources at quadrature points load scalar spaces");
-  GH_REQUIRE(!c.have_fq || (!elasticity && c.fq.n == (size_t)c.ncells * c.nq), GH_ERR_STATE,
+  GH_REQUIRE(!c.have_fq || (!elasticity && c.fq.n == (size_t)c.ncells * c.nq * c.ncomp),
+             GH_ERR_STATE,
              "the quadrature-point source belongs to another mesh or quadrature");
   const int D = c.dim, nloc = c.nloc, nn = nloc / c.ncomp;
   GH_REQUIRE(c.nq >= 1 && c.nq <= kHgMaxNq && (int)c.qw.size() == c.nq &&

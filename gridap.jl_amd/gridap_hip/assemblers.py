@@ -232,6 +232,8 @@ class SparseMatrixAssembler:
             deg = next(iter(degs))
             fq = sum(_eval_cell_points(cf, self.space, deg) for cf, _ in point_sources)
             if fsrc is not None:
+                if fq.ndim != 2:
+                    raise NotImplementedError("per-cell and point sources on a vector space")
                 fq = fq + fsrc[:, None]
             extra["fq"] = np.ascontiguousarray(fq)
             fsrc = np.zeros(ncells)
@@ -310,32 +312,44 @@ class SparseMatrixAssembler:
 
 
 def _eval_cell_points(coef, V, degree):
-    """A coefficient at the quadrature points of every cell, [ncells, nq]: functions of
-    x at the physical points (CartesianMap, or the isoparametric map of the nodes), FE
-    functions by their cell values and the basis, per-cell values broadcast."""
+    """A coefficient at the quadrature points of every cell, [ncells, nq] (scalar
+    spaces) or [ncells, nq, ncomp] (vector spaces): functions of x at the physical points
+    (CartesianMap, or the isoparametric map of the nodes), FE functions by their cell
+    values and the basis, per-cell values broadcast."""
     model = V.model
-    if V.simplex or V.order not in (1, 2) or V.ncomp != 1:
-        raise NotImplementedError("sources f(x) are provided for scalar Q1 and Q2 spaces "
-                                  "on n-cube meshes")
+    if V.simplex or V.order not in (1, 2) or V.ncomp not in (1, model.D):
+        raise NotImplementedError("sources f(x) are provided for Q1 and Q2 spaces (scalar, or "
+                                  "one component per dimension) on n-cube meshes")
+    out = _cell_points(coef, V, degree)                       # [c, q, k], k = 1 or ncomp
+    if V.ncomp == 1:
+        if out.shape[2] != 1:
+            raise ValueError("a vector-valued source on a scalar space")
+        return out[:, :, 0]
+    return np.ascontiguousarray(np.broadcast_to(out, out.shape[:2] + (V.ncomp,)))
+
+
+def _cell_points(coef, V, degree):
+    model = V.model
     D = model.D
     q = Quadrature(D, degree)
-    shp, _ = V.reffe.tabulate(D, q.coords)                    # [q, a]: the FE basis
+    shp, _ = V.reffe.tabulate(D, q.coords)                    # [q, a]: the scalar basis
     # the cell map: the Q1 basis of the cell's vertices (get_cell_map)
     gshp = shp if V.order == 1 else ReferenceFE(lagrangian, Float64, 1).tabulate(D, q.coords)[0]
     C, nq = model.num_cells(), len(q.weights)
     if isinstance(coef, CoefProduct):
-        out = np.ones((C, nq))
+        out = np.ones((C, nq, 1))
         for c in coef.factors:
-            out = out * _eval_cell_points(c, V, degree)
+            out = out * _cell_points(c, V, degree)
         return out
     if isinstance(coef, CellField):
         pc = coef.per_cell(C)
-        return np.ones((C, nq)) * (1.0 if pc is None else pc[:, None])
+        return np.ones((C, nq, 1)) * (1.0 if pc is None else pc[:, None, None])
     if isinstance(coef, PointField):
         X = model.node_coordinates()[model.cell_node_ids() - 1]   # [c, v, D]
         x = np.einsum("qv,cvd->cqd", gshp, X)
         val = np.asarray(coef.func(x.reshape(-1, D)), dtype=np.float64)
-        return np.broadcast_to(val.reshape(-1), (C * nq,)).reshape(C, nq).copy()
+        k = val.size // (C * nq)
+        return np.broadcast_to(val.reshape(-1), (C * nq * k,)).reshape(C, nq, k).copy()
     if isinstance(coef, FEValue):
         uh = coef.uh
         cd = np.asarray(V.cell_dof_ids, dtype=np.int64)
@@ -343,7 +357,9 @@ def _eval_cell_points(coef, V, degree):
         dv = np.asarray(uh.dirichlet_values, dtype=np.float64)
         uc = np.where(cd > 0, fv[np.maximum(cd, 1) - 1],
                       dv[np.maximum(-cd, 1) - 1] if dv.size else 0.0)
-        return uc @ shp.T
+        nc = V.ncomp
+        uc = uc.reshape(C, nc, -1)                             # component-blocked local dofs
+        return np.einsum("cka,qa->cqk", uc, shp)
     raise NotImplementedError("unsupported source coefficient %r" % (coef,))
 
 

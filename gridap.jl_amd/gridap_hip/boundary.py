@@ -99,15 +99,52 @@ def eval_coef(coef, fs, pos, ncells):
     raise NotImplementedError("unsupported boundary coefficient %r" % (coef,))
 
 
+def _cell_values_vec(uh, nc):
+    """[C, nc, nn] cell values of a (vector) FE function, components blocked."""
+    uc = _cell_values(uh)
+    return uc.reshape(uc.shape[0], nc, -1)
+
+
+def eval_coef_vec(coef, fs, pos, ncells, nc):
+    """coef at the face points, [len(pos), nqf, k] with k = 1 (scalar) or nc."""
+    nq = fs.nqf
+    if isinstance(coef, FEValue):
+        uc = _cell_values_vec(coef.uh, nc)[fs.cells[pos]]                  # [f, k, a]
+        return np.einsum("fqa,fka->fqk", fs.shp[fs.sides[pos]], uc)
+    if isinstance(coef, NormalGradValue):
+        uc = _cell_values_vec(coef.uh, nc)[fs.cells[pos]]
+        return np.einsum("fqa,fka->fqk", fs.dn[pos], uc)
+    if isinstance(coef, PointField):
+        X = fs.x[pos].reshape(-1, fs.D)
+        val = np.asarray(coef.func(X), dtype=np.float64)
+        k = val.size // (len(pos) * nq)
+        return val.reshape(len(pos), nq, k)
+    if isinstance(coef, CoefProduct):
+        out = np.ones((len(pos), nq, 1))
+        for c in coef.factors:
+            out = out * eval_coef_vec(c, fs, pos, ncells, nc)
+        return out
+    return eval_coef(coef, fs, pos, ncells)[:, :, None]
+
+
 def _vector_loads(terms, fs, ncells, D):
     """g [nfaces, nqf, D] of the vector boundary loads (v⋅g): v⋅(n⋅T(x)) and (n⋅v) c."""
     F, nq = len(fs.cells), fs.nqf
     g = np.zeros((F, nq, D))
+    h = np.zeros((F, nq, D))
     for t, m in terms:
-        if not isinstance(t, VectorLoadTerm):
-            raise NotImplementedError("vector spaces take the boundary loads v⋅(n⋅∇u) and "
-                                      "(n⋅v) p only")
         pos = fs.positions(m.trian)
+        if isinstance(t, SourceTerm):             # v⊙c: component-wise g
+            np.add.at(g, pos, np.broadcast_to(eval_coef_vec(t.coef, fs, pos, ncells, D),
+                                              (len(pos), nq, D)))
+            continue
+        if isinstance(t, NormalLoadTerm):         # (n⋅∇v)⊙c: component-wise h
+            np.add.at(h, pos, np.broadcast_to(eval_coef_vec(t.coef, fs, pos, ncells, D),
+                                              (len(pos), nq, D)))
+            continue
+        if not isinstance(t, VectorLoadTerm):
+            raise NotImplementedError("vector boundary loads: v⊙g, (n⋅∇v)⊙h, v⋅(n⋅∇u) and "
+                                      "(n⋅v) p")
         n = np.zeros((len(pos), D))
         n[np.arange(len(pos)), fs.axis[pos]] = fs.sgn[pos]
         if t.kind == "vec":
@@ -120,7 +157,7 @@ def _vector_loads(terms, fs, ncells, D):
             c = eval_coef(t.coef, fs, pos, ncells)                        # [f, q]
             val = c[:, :, None] * n[:, None, :]
         np.add.at(g, pos, val)
-    return g
+    return g, h
 
 
 def lower_boundary(model, reffe, mat_terms, vec_terms):
@@ -130,19 +167,7 @@ def lower_boundary(model, reffe, mat_terms, vec_terms):
     terms = list(mat_terms) + list(vec_terms)
     if not terms:
         return None
-    if getattr(reffe, "ncomp", 1) > 1:
-        if mat_terms:
-            raise NotImplementedError("vector spaces take boundary loads only")
-        degs = {m.degree for _, m in terms}
-        if len(degs) > 1:
-            raise NotImplementedError("boundary terms must share one quadrature degree")
-        trians = []
-        for _, m in terms:
-            if all(m.trian is not t for t in trians):
-                trians.append(m.trian)
-        fs = FaceSet(model, reffe, trians, degs.pop())
-        g = _vector_loads(vec_terms, fs, model.num_cells(), model.D)
-        return dict(faces=fs, alpha=None, beta=None, gq=np.ascontiguousarray(g), hq=None)
+    vector = getattr(reffe, "ncomp", 1) > 1
     degs = {m.degree for _, m in terms}
     if len(degs) > 1:
         raise NotImplementedError("boundary terms must share one quadrature degree")
@@ -172,6 +197,14 @@ def lower_boundary(model, reffe, mat_terms, vec_terms):
     if not np.array_equal(beta["v_dnu"], beta["dnv_u"]):
         raise NotImplementedError("the terms v (n⋅∇u) and (n⋅∇v) u must carry the same "
                                   "coefficient (symmetric Nitsche form)")
+    if vector:
+        # component-wise terms: the scalar face matrix per component, loads per component
+        g, h = _vector_loads(vec_terms, fs, ncells, model.D)
+        use = bool(vec_terms)
+        return dict(faces=fs, alpha=alpha if have["alpha"] else None,
+                    beta=beta["v_dnu"] if have["beta"] else None,
+                    gq=np.ascontiguousarray(g) if use else None,
+                    hq=np.ascontiguousarray(h) if use and np.any(h) else None)
     for t, m in vec_terms:
         pos = fs.positions(m.trian)
         val = eval_coef(t.coef, fs, pos, ncells)

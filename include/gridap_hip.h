@@ -179,10 +179,11 @@ int gh_face_tabulation(gh_ctx *ctx, int nsides, int nqf, const double *qw, const
  * gq[f*nqf + q], hq[f*nqf + q] (NULL = 0), followed by the Dirichlet lifting of the face
  * matrix.  Neumann: g = n⋅∇u; Nitsche (test/GridapTests/PoissonTests.jl:101-108):
  * α = γ/h, β = -1, g = (γ/h) u_D, h = -u_D.  nfaces = 0 removes the boundary terms.
- * Vector spaces (one component per dimension) take loads only: b += ∫_F φ_a g_i with
- * gq[(f*nqf + q)*ncomp + i] (α, β, h NULL), e.g. v⋅(n⋅∇u) - (n⋅v) p of
- * test/GridapTests/StokesTaylorHoodTests.jl:56; they are added after the cell terms
- * of GH_FORM_DIFFUSION or GH_FORM_ELASTICITY.
+ * Vector spaces (one component per dimension) take the terms component by component
+ * (v⊙u, v⊙(n⋅∇u), (n⋅∇v)⊙u: the scalar face matrix on every component) with g and h
+ * per component, gq[(f*nqf + q)*ncomp + i] and hq alike (PoissonTests.jl's vector case;
+ * the loads v⋅(n⋅∇u) - (n⋅v) p of test/GridapTests/StokesTaylorHoodTests.jl:56); they
+ * are added after the cell terms of GH_FORM_DIFFUSION or GH_FORM_ELASTICITY.
  * The first gh_assemble after an upload (or a new pattern) builds the face lists: for
  * each nonzero the faces touch, its face contributions (faces ascending). */
 int gh_boundary_upload(gh_ctx *ctx, int64_t nfaces, const int32_t *face_cells,
@@ -215,8 +216,8 @@ int gh_mass_upload(gh_ctx *ctx, const double *sigma);
  * rule of the uploaded tabulation), in place of the per-cell f of gh_coef_upload
  * (NULL: back to the per-cell f).  The x-dependent load of
  * test/GridapTests/PoissonTests.jl:101-108 (l(v) = ∫ v f dΩ with f a function).
- * Scalar Q1 (the assembly then runs the per-cell kernel) and scalar Q2 (the general
- * n-cube element kernel). */
+ * Scalar Q1 and Q2, and vector spaces of the general n-cube element kernel with
+ * fq[(cell*nq + q)*ncomp + i] (l(v) = ∫ v⋅f dΩ). */
 int gh_source_qp_upload(gh_ctx *ctx, const double *fq);
 
 /* Numeric phase on device-resident inputs: nzval and b in HBM, the matrix part of

@@ -694,3 +694,51 @@ def poisson_qk_with_boundary(oracle, domain, partition, order, dirichlet_entitie
     b = oracle.assemble_vector(cd_all, v_all, nfree)
     return dict(model=model, cell_dofs=cd, nfree=nfree, ndiri=ndiri, colptr=colptr,
                 rowval=rowval, nzval=nzval, b=b, dirichlet_vals=dvals)
+
+
+def poisson_qk_vector_with_boundary(oracle, domain, partition, order, cell_dofs, nfree, ndiri,
+                                    dvals, face_terms, degree, face_degree, f):
+    """The vector-valued case of test/GridapTests/PoissonTests.jl (VectorValue{2}): every
+    term acts component by component (∇v⊙∇u, v⊙u, v⊙(n⋅∇u), (n⋅∇v)⊙u), so each cell and
+    face matrix is δ_ij times the scalar one and each load is the scalar load of its
+    component.  cell_dofs [C, ncomp·nn] (components blocked), the Dirichlet values and
+    counts are inputs (the numbering is pinned elsewhere); face_terms: [(entity ids,
+    alpha, beta, g(x, n) -> vector, h(x, n) -> vector)]; f(x) -> vector.  Assembled as
+    poisson_qk_with_boundary assembles the scalar case."""
+    D = len(partition)
+    model = oracle.cartesian_model(domain, partition)
+    X = model["xyz"][model["cell_nodes"] - 1]
+    C = X.shape[0]
+    qx, qw = oracle.tensor_quadrature(D, degree)
+    shp, grad = oracle.qk_tabulate(D, order, qx)
+    n1q, g1 = oracle.qk_tabulate(D, 1, qx)
+    wdet, gp = physical_gradients(X, qw, g1, grad)
+    mats = vector_laplacian_cells(wdet, gp)
+    xq = q1_map_points(X, n1q)                                      # [C, nq, D]
+    fq = np.asarray(f(xq.reshape(-1, D)), dtype=np.float64).reshape(C, len(qw), D)
+    vecs = np.concatenate([np.einsum("cq,cq,qa->ca", wdet, fq[..., i], shp) for i in range(D)],
+                          axis=1)
+    cd = np.asarray(cell_dofs)
+    all_cd, all_m, all_v = [cd], [mats], [vecs]
+    nn = shp.shape[1]
+    for tents, al, be, gf, hf in face_terms:
+        faces = oracle.boundary_faces(model, tents)
+        fvs = []
+        for i in range(D):
+            gi = None if gf is None else (lambda x, n, i=i: gf(x, n)[i])
+            hi = None if hf is None else (lambda x, n, i=i: hf(x, n)[i])
+            fm, fv = oracle.boundary_face_arrays(model, faces, face_degree, al, be, gi, hi,
+                                                 order=order)
+            fvs.append(fv)
+        blk = np.einsum("ij,fab->fiajb", np.eye(D), fm).reshape(len(faces), D * nn, D * nn)
+        all_cd.append(cd[[fc[0] for fc in faces]])
+        all_m.append(blk)
+        all_v.append(np.concatenate(fvs, axis=1))
+    cd_all = np.ascontiguousarray(np.concatenate(all_cd), dtype=np.int32)
+    m_all = np.ascontiguousarray(np.concatenate(all_m))
+    v_all = np.ascontiguousarray(np.concatenate(all_v))
+    if ndiri:
+        v_all = oracle.attach_dirichlet(cd_all, m_all, np.asarray(dvals, dtype=np.float64), v_all)
+    colptr, rowval, nzval = oracle.assemble_csc(cd_all, m_all, nfree, nfree)
+    b = oracle.assemble_vector(cd_all, v_all, nfree)
+    return dict(colptr=colptr, rowval=rowval, nzval=nzval, b=b)

@@ -253,3 +253,57 @@ def test_3d_faces_folded_into_lattice_match_face_kernel(oracle, mass, precision)
             g=U1, kappa=kappa)
         np.testing.assert_array_equal(A.rowval, ref["rowval"])
         assert relmax(A.nzval, ref["nzval"]) <= 1e-12 and relmax(b, ref["b"]) <= 1e-12
+
+
+UV_EX = lambda x: np.stack([x[..., 0] ** 2 + x[..., 1], 4 * x[..., 0] - x[..., 1] ** 2], axis=-1)
+# ∇u[i, j] = ∂_i u_j
+GUV_EX = lambda x: np.array([[2 * x[0], 4.0], [1.0, -2 * x[1]]])
+
+
+@pytest.mark.parametrize("part", [(4, 4), (9, 7)])
+def test_poisson_driver_vector_case(oracle, part):
+    """PoissonTests.jl's vector case (VectorValue{2}, order 2): u = (x1² + x2, 4x1 - x2²),
+    f = -Δu, Neumann and Nitsche terms acting component by component.  Against the
+    oracle (pattern bit-exact, 1e-12) and the device PCG reproduces u (it lies in the
+    space) to 1e-8 relative."""
+    announce("PoissonTests vector case %s" % (part,))
+    gam = 10.0
+    model = G.CartesianDiscreteModel((0, 1, 0, 1), part)
+    labels = model.get_face_labeling()
+    labels.add_tag_from_tags("dirichlet", [1, 2, 5])
+    labels.add_tag_from_tags("neumann", [7, 8])
+    labels.add_tag_from_tags("nitsche", 6)
+    h = 1.0 / part[0]
+    Ω = G.Triangulation(model)
+    Γn = G.BoundaryTriangulation(model, labels, tags="neumann")
+    Γd = G.BoundaryTriangulation(model, labels, tags="nitsche")
+    dΩ, dΓn, dΓd = G.Measure(Ω, 2), G.Measure(Γn, 2), G.Measure(Γd, 2)
+    nn, nd = G.get_normal_vector(Γn), G.get_normal_vector(Γd)
+    V = G.TestFESpace(model, G.ReferenceFE(G.lagrangian, G.VectorValue[2, G.Float64], 2),
+                      dirichlet_tags="dirichlet", labels=labels)
+    U = G.TrialFESpace(V, UV_EX)
+    uh = G.interpolate(UV_EX, U)
+    f = lambda x: np.stack([np.full(x.shape[:-1], -2.0), np.full(x.shape[:-1], 2.0)], axis=-1)
+    a = lambda u, v: (G.integrate(G.inner(G.gradient(v), G.gradient(u)), dΩ) +
+                      G.integrate((gam / h) * v * u - v * G.dot(nd, G.gradient(u)) -
+                                  G.dot(nd, G.gradient(v)) * u, dΓd))
+    l = lambda v: (G.integrate(G.inner(v, f), dΩ) +
+                   G.integrate(v * G.dot(nn, G.gradient(uh)), dΓn) +
+                   G.integrate((gam / h) * v * uh - G.dot(nd, G.gradient(v)) * uh, dΓd))
+    op = G.AffineFEOperator(a, l, U, V)
+    import conforming_oracle as CO
+    U1 = lambda x: UV_EX(np.asarray(x))
+    ref = CO.poisson_qk_vector_with_boundary(
+        oracle, (0, 1, 0, 1), part, 2, np.asarray(V.cell_dof_ids), V.nfree, V.ndirichlet,
+        U.dirichlet_values(),
+        [([7, 8], 0.0, 0.0, lambda x, n: n @ GUV_EX(x), None),
+         ([6], gam / h, -1.0, lambda x, n: gam / h * U1(x), lambda x, n: -U1(x))],
+        2, 2, f)
+    A, b = op.get_matrix(), op.get_vector()
+    np.testing.assert_array_equal(A.colptr, ref["colptr"])
+    np.testing.assert_array_equal(A.rowval, ref["rowval"])
+    assert relmax(A.nzval, ref["nzval"]) <= 1e-12
+    assert relmax(b, ref["b"]) <= 1e-12
+    u = G.solve(op, G.PCGSolver(rtol=1e-13))
+    uI = G.interpolate(UV_EX, U).free_values
+    assert np.max(np.abs(u.free_values - uI)) <= 1e-8 * np.max(np.abs(uI))
