@@ -513,23 +513,45 @@ __device__ __forceinline__ I cart_node_i(int D, const int64_t *np, I cell, const
   return node;
 }
 
-__global__ void lattice_scatter32(Lattice L, int32_t total, int nloc,
-                                  const int32_t *__restrict__ cell_dofs,
-                                  int32_t *__restrict__ node_dof) {
-  const int32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= total) return;
-  const int32_t c = t / nloc;
-  node_dof[cart_node_i<int32_t>(L.D, L.np, c, L.nc, t - c * nloc)] = cell_dofs[t];
+// One thread per cell: the cell's first node from one index split, its 2^D vertices
+// by adding the lattice strides (no divisions per vertex); the cell's dof ids are one
+// contiguous run.
+__device__ __forceinline__ int32_t cell_base_node(const Lattice &L, int32_t c, int32_t *st) {
+  int32_t node = 0, stride = 1, r = c;
+  for (int d = 0; d < L.D; ++d) {
+    const int32_t ci = r % (int32_t)L.nc[d];
+    r /= (int32_t)L.nc[d];
+    node += ci * stride;
+    st[d] = stride;
+    stride *= (int32_t)L.np[d];
+  }
+  return node;
 }
 
-__global__ void lattice_verify32(Lattice L, int32_t total, int nloc,
+__global__ void lattice_scatter32(Lattice L, int32_t ncells, int nloc,
+                                  const int32_t *__restrict__ cell_dofs,
+                                  int32_t *__restrict__ node_dof) {
+  const int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncells) return;
+  int32_t st[3] = {0, 0, 0};
+  const int32_t base = cell_base_node(L, c, st);
+  for (int a = 0; a < nloc; ++a)
+    node_dof[base + (a & 1) * st[0] + ((a >> 1) & 1) * st[1] + ((a >> 2) & 1) * st[2]] =
+        cell_dofs[c * nloc + a];
+}
+
+__global__ void lattice_verify32(Lattice L, int32_t ncells, int nloc,
                                  const int32_t *__restrict__ cell_dofs,
                                  const int32_t *__restrict__ node_dof, int32_t *bad) {
-  const int32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= total) return;
-  const int32_t c = t / nloc;
-  if (node_dof[cart_node_i<int32_t>(L.D, L.np, c, L.nc, t - c * nloc)] != cell_dofs[t])
-    atomicOr(bad, 1);
+  const int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncells) return;
+  int32_t st[3] = {0, 0, 0};
+  const int32_t base = cell_base_node(L, c, st);
+  bool ok = true;
+  for (int a = 0; a < nloc; ++a)
+    ok = ok && node_dof[base + (a & 1) * st[0] + ((a >> 1) & 1) * st[1] +
+                        ((a >> 2) & 1) * st[2]] == cell_dofs[c * nloc + a];
+  if (!ok) atomicOr(bad, 1);
 }
 
 // lattice_masks with 32-bit indices and the stencil unrolled (D known at compile time)
@@ -684,11 +706,11 @@ bool symbolic_lattice32(Ctx &c) {
   c.colptr.resize(c.nfree + 1);
   int64_t *h = c.h_small.get(16);
   hipLaunchKernelGGL(lattice_init, dim3(1), dim3(64), 0, s, c.ws_flag.get(), c.ws_box.get());
-  const int32_t t = (int32_t)(c.ncells * c.nloc);
-  hipLaunchKernelGGL(lattice_scatter32, dim3(grid_for(t)), dim3(kBlock), 0, s, L, t, c.nloc,
-                     c.cell_dofs.get(), c.node_dof.get());
-  hipLaunchKernelGGL(lattice_verify32, dim3(grid_for(t)), dim3(kBlock), 0, s, L, t, c.nloc,
-                     c.cell_dofs.get(), c.node_dof.get(), c.ws_flag.get());
+  const int32_t nc32 = (int32_t)c.ncells;
+  hipLaunchKernelGGL(lattice_scatter32, dim3(grid_for(nc32)), dim3(kBlock), 0, s, L, nc32,
+                     c.nloc, c.cell_dofs.get(), c.node_dof.get());
+  hipLaunchKernelGGL(lattice_verify32, dim3(grid_for(nc32)), dim3(kBlock), 0, s, L, nc32,
+                     c.nloc, c.cell_dofs.get(), c.node_dof.get(), c.ws_flag.get());
   hipLaunchKernelGGL(free_flags, dim3(grid_for(nn)), dim3(kBlock), 0, s, c.nnodes,
                      c.node_dof.get(), c.ws_f.get(), c.ws_flag.get());
   GH_CHECK_HIP(hipGetLastError());
