@@ -205,15 +205,39 @@ struct SumArgs {
 // Threads [0, ndst): one nonzero each; [ndst, ndst + nown): one load row each.
 __global__ __launch_bounds__(256) void boundary_sum_kernel(const SumArgs A) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  // batches of 4: the index loads, then the value loads, then the ordered sum
+  constexpr int U = 4;
   if (t < A.ndst) {
     const int32_t pos = A.dst_pos[t];
+    const int32_t e = A.dst_ptr[t + 1];
     double s = 0.0;
-    for (int32_t k = A.dst_ptr[t]; k < A.dst_ptr[t + 1]; ++k) s += A.K[A.dst_ci[k]];
+    for (int32_t k0 = A.dst_ptr[t]; k0 < e; k0 += U) {
+      int32_t id[U];
+      double v[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) id[u] = k0 + u < e ? A.dst_ci[k0 + u] : -1;
+#pragma unroll
+      for (int u = 0; u < U; ++u) v[u] = id[u] >= 0 ? A.K[id[u]] : 0.0;
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (id[u] >= 0) s += v[u];
+    }
     if (pos >= 0) A.nzval[pos] += s;
   } else if (t < A.ndst + A.nown) {
     const int64_t o = t - A.ndst;
+    const int32_t e = A.own_ptr[o + 1];
     double s = 0.0;
-    for (int32_t k = A.own_ptr[o]; k < A.own_ptr[o + 1]; ++k) s += A.r[A.own_fa[k]];
+    for (int32_t k0 = A.own_ptr[o]; k0 < e; k0 += U) {
+      int32_t id[U];
+      double v[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) id[u] = k0 + u < e ? A.own_fa[k0 + u] : -1;
+#pragma unroll
+      for (int u = 0; u < U; ++u) v[u] = id[u] >= 0 ? A.r[id[u]] : 0.0;
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (id[u] >= 0) s += v[u];
+    }
     A.b[A.own_dof[o] - 1] += s;
   }
 }

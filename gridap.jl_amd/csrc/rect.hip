@@ -231,8 +231,21 @@ __global__ void rect_sum_kernel(int64_t nnz, const int32_t *__restrict__ cstart,
                                 double *__restrict__ nzval) {
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= nnz) return;
+  // batches of 4: the index loads, then the value loads, then the ordered sum
+  constexpr int U = 4;
+  const int32_t e = cstart[k + 1];
   double s = 0.0;
-  for (int32_t m = cstart[k]; m < cstart[k + 1]; ++m) s += (double)E[perm[m]];
+  for (int32_t m0 = cstart[k]; m0 < e; m0 += U) {
+    int32_t id[U];
+    T v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) id[u] = m0 + u < e ? perm[m0 + u] : -1;
+#pragma unroll
+    for (int u = 0; u < U; ++u) v[u] = id[u] >= 0 ? E[id[u]] : (T)0;
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (id[u] >= 0) s += (double)v[u];
+  }
   nzval[k] = s;
 }
 
