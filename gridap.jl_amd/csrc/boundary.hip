@@ -2,7 +2,8 @@
 // integrated over a BoundaryTriangulation (src/Geometry/BoundaryTriangulations.jl:
 // 122-183) of a Cartesian mesh, as in the Neumann and Nitsche drivers of
 // test/GridapTests/PoissonTests.jl:101-108 (scalar Q1, and the driver's own order 2 on
-// quadrilaterals).
+// quadrilaterals; vector spaces component by component, as in the driver's vector case
+// and the Neumann loads of test/GridapTests/StokesTaylorHoodTests.jl:56).
 //
 // Per boundary face F (a facet `side` of cell c, outward unit normal n) with the face
 // quadrature (reference weights w_q on the facet, the cell's reference tabulation at
@@ -25,6 +26,8 @@
 //    into b.  No two threads write one value, so the additions are race-free and
 //    deterministic, and the assembled matrix keeps the symmetry of the cell terms
 //    bitwise.
+// Vector spaces (one component per dimension) use one scalar face matrix for every
+// component: the lists pair rows and columns of the same component only.
 #include <algorithm>
 #include <climits>
 #include <cmath>
