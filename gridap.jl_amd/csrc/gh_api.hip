@@ -617,9 +617,17 @@ int gh_assemble(gh_ctx *ctx, int form_kind, int precision, int path) {
       }
       gh::assemble_boundary(c);  // vector spaces: boundary loads
     } else {
-      GH_REQUIRE(!gh::hexq2_eligible(c), GH_ERR_UNSUPPORTED,
-                 "vector Q2 spaces assemble the elasticity form");
-      if (gh::hexgen_eligible(c)) {
+      if (gh::hexq2_eligible(c)) {
+        // vector Q2 on Cartesian hexahedra: the vector Laplacian on the elasticity lattice
+        GH_REQUIRE(c.q2lattice && path != GH_PATH_GENERAL, GH_ERR_UNSUPPORTED,
+                   "the vector Laplacian on Cartesian vector-Q2 hexahedra takes the lattice "
+                   "path");
+        GH_REQUIRE((int)c.shp.size() == c.nq * 27 && (int)c.grad.size() == c.nq * 27 * 3,
+                   GH_ERR_STATE, "the vector Laplacian needs the scalar Q2 tabulation");
+        GH_REQUIRE(!c.have_sigma && !c.have_fq, GH_ERR_UNSUPPORTED,
+                   "the vector-Q2 lattice takes κ per cell and a constant body force");
+        gh::q2lattice_assemble(c, precision == GH_PREC_FP32_ELEM, true);
+      } else if (gh::hexgen_eligible(c)) {
         GH_REQUIRE(path != GH_PATH_STRUCTURED, GH_ERR_UNSUPPORTED,
                    "the lattice path assembles scalar Q1 spaces");
         gh::hexgen_assemble(c, precision, false);

@@ -346,7 +346,8 @@ bool hexq2_eligible(const Ctx &c);
 void assemble_elasticity(Ctx &c, int precision, int path);
 // hexq2_lattice.hip
 void q2lattice_prepare(Ctx &c);                 // symbolic phase: node map, pattern check
-void q2lattice_assemble(Ctx &c, bool f32);      // numeric phase: nzval and b
+// numeric phase: nzval and b; vlap: the vector Laplacian ∫ κ ∇v⊙∇u instead of elasticity
+void q2lattice_assemble(Ctx &c, bool f32, bool vlap = false);
 // solve.hip
 void spmv_device(Ctx &c, const double *x, double *y);
 void pcg_device(Ctx &c, int precond, double rtol, int maxit, gh_solve_stats *st);

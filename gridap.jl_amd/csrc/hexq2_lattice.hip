@@ -749,10 +749,11 @@ void q2lattice_prepare(Ctx &c) {
 
 // The numeric phase on the lattice: K̂ and m̂ from the uploaded tabulation (FP64, once
 // per tabulation, cell sizes and ν), then one launch writes nzval and b.
-void q2lattice_assemble(Ctx &c, bool f32) {
+void q2lattice_assemble(Ctx &c, bool f32, bool vlap) {
   GH_REQUIRE(c.q2lattice, GH_ERR_STATE, "no vector-Q2 lattice for this space");
   c.last_kernel = GH_KERNEL_Q2_LATTICE;
-  const double key[4] = {c.nu, c.sizes[0], c.sizes[1], c.sizes[2]};
+  // the form is part of the key: ν lies in (-1, 0.5), -2 marks the vector Laplacian
+  const double key[4] = {vlap ? -2.0 : c.nu, c.sizes[0], c.sizes[1], c.sizes[2]};
   const bool fresh = !(c.q2_ref_ready && std::equal(key, key + 4, c.q2_ref_key));
   if (fresh) {
     const int nq = c.nq, NN = 27;
@@ -781,8 +782,9 @@ void q2lattice_assemble(Ctx &c, bool f32) {
       for (int a = 0; a < NN; ++a)
         for (int j = 0; j < 3; ++j)
           for (int b = 0; b < NN; ++b) {
-            double v = lam * Pij(i, j, a, b) + mu * Pij(j, i, a, b);
-            if (i == j) v += mu * (Pij(0, 0, a, b) + Pij(1, 1, a, b) + Pij(2, 2, a, b));
+            double v = vlap ? 0.0 : lam * Pij(i, j, a, b) + mu * Pij(j, i, a, b);
+            if (i == j)  // vector Laplacian: δ_ij ∇φ_a⋅∇φ_b
+              v += (vlap ? 1.0 : mu) * (Pij(0, 0, a, b) + Pij(1, 1, a, b) + Pij(2, 2, a, b));
             K[(size_t)(a + 27 * i) * 81 + (b + 27 * j)] = v;  // [row][col]
           }
     // exactly symmetric; stored by column ([col][row]) for the kernel

@@ -408,18 +408,21 @@ def test_stokes_taylor_hood_driver_as_written(oracle):
     assert np.max(np.abs(x[V.nfree:] - pI)) <= 1e-9 * np.max(np.abs(pI))
 
 
-def test_stokes_taylor_hood_3d_affine_mapped(oracle):
-    """Taylor–Hood Stokes on a 3-D mesh under a sheared affine map (the mapped paths:
-    the vector Laplacian of the general n-cube element kernel and the divergence
-    couplings with the Q1 geometry map), u = (x1² + 2 x2², -x1², x3), p = x1 + 3 x2 + x3,
+@pytest.mark.parametrize("mapped", [True, False])
+def test_stokes_taylor_hood_3d(oracle, mapped):
+    """Taylor–Hood Stokes on a 3-D mesh, Cartesian (the vector Laplacian on the
+    vector-Q2 lattice kernel) or under a sheared affine map (the vector Laplacian of the
+    general n-cube element kernel and the divergence couplings with the Q1 geometry map), u = (x1² + 2 x2², -x1², x3), p = x1 + 3 x2 + x3,
     velocity Dirichlet data on the boundary, pressure fixed at one vertex: against the
     oracle (pattern bit-exact, 1e-12) and the direct solve reproduces u and p to 1e-9
     (both lie in the spaces under an affine map)."""
     import scipy.sparse.linalg as spla
     import conforming_oracle as CO
-    announce("Stokes Taylor-Hood 3-D affine-mapped 3x3x2")
+    announce("Stokes Taylor-Hood 3-D 3x3x2 mapped=%s" % mapped)
     M = np.array([[1.0, 0.2, 0.1], [0.0, 0.9, 0.15], [0.05, 0.0, 1.1]])
-    model = G.CartesianDiscreteModel((0, 1, 0, 1, 0, 1), (3, 3, 2), map=lambda x: x @ M.T)
+    # Cartesian: the vector Laplacian takes the vector-Q2 lattice kernel
+    model = G.CartesianDiscreteModel((0, 1, 0, 1, 0, 1), (3, 3, 2),
+                                     map=(lambda x: x @ M.T) if mapped else None)
     ue = lambda x: np.stack([x[..., 0] ** 2 + 2 * x[..., 1] ** 2, -x[..., 0] ** 2, x[..., 2]],
                             axis=-1)
     pe = lambda x: x[..., 0] + 3 * x[..., 1] + x[..., 2]
