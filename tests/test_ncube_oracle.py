@@ -175,3 +175,73 @@ def test_divergence_and_vector_laplacian_pins(oracle, D):
             np.testing.assert_array_equal(blk, K if i == j else 0 * K)
     e = np.tile(np.repeat(np.arange(1.0, D + 1), nn), (C, 1))
     assert np.abs(np.einsum("cab,cb->ca", L, e)).max() <= 1e-12 * np.abs(L).max()
+
+
+def _solve(cp, rv, nz, b, n):
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+    return spla.spsolve(sp.csc_matrix((nz, np.asarray(rv) - 1, np.asarray(cp) - 1),
+                                      shape=(n, n)), b)
+
+
+def test_vector_poisson_oracle_reproduces_u(oracle):
+    """poisson_qk_vector_with_boundary (PoissonTests.jl's vector case): its system's solve
+    is the Q2 interpolant of u = (x1² + x2, 4x1 - x2²) (u lies in the space), with the
+    Neumann and Nitsche terms as the driver writes them — a dropped face term, a wrong
+    normal sign or a component mix-up breaks it."""
+    import gridap_hip as G
+    uv = lambda x: np.stack([x[..., 0] ** 2 + x[..., 1], 4 * x[..., 0] - x[..., 1] ** 2], -1)
+    guv = lambda x: np.array([[2 * x[0], 4.0], [1.0, -2 * x[1]]])
+    f = lambda x: np.stack([np.full(x.shape[:-1], -2.0), np.full(x.shape[:-1], 2.0)], -1)
+    part, gam = (5, 4), 10.0
+    h = 1.0 / part[0]
+    model = G.CartesianDiscreteModel((0, 1, 0, 1), part)
+    labels = model.get_face_labeling()
+    labels.add_tag_from_tags("dirichlet", [1, 2, 5])
+    V = G.TestFESpace(model, G.ReferenceFE(G.lagrangian, G.VectorValue[2, G.Float64], 2),
+                      dirichlet_tags="dirichlet", labels=labels)
+    U = G.TrialFESpace(V, uv)
+    u1 = lambda x: uv(np.asarray(x))
+    ref = CO.poisson_qk_vector_with_boundary(
+        oracle, (0, 1, 0, 1), part, 2, np.asarray(V.cell_dof_ids), V.nfree, V.ndirichlet,
+        U.dirichlet_values(),
+        [([7, 8], 0.0, 0.0, lambda x, n: n @ guv(x), None),
+         ([6], gam / h, -1.0, lambda x, n: gam / h * u1(x), lambda x, n: -u1(x))], 2, 2, f)
+    x = _solve(ref["colptr"], ref["rowval"], ref["nzval"], ref["b"], V.nfree)
+    uI = G.interpolate(uv, U).free_values
+    assert np.abs(x - uI).max() <= 1e-12 * np.abs(uI).max()
+
+
+def test_stokes_oracle_reproduces_u_and_p(oracle):
+    """The Taylor–Hood blocks (vector_laplacian_cells, divergence_cells) assembled by the
+    multi-field oracle for StokesTaylorHoodTests.jl's u and p (velocity Dirichlet data
+    on the boundary, pressure fixed at a vertex): the solve is the interpolant of u and
+    p — a sign flip in either coupling or a transposed block breaks it."""
+    import gridap_hip as G
+    import multifield_oracle as MF
+    model = G.CartesianDiscreteModel((0, 2, 0, 2), (3, 3))
+    ue = lambda x: np.stack([x[..., 0] ** 2 + 2 * x[..., 1] ** 2, -x[..., 0] ** 2], axis=-1)
+    pe = lambda x: x[..., 0] + 3 * x[..., 1]
+    V = G.TestFESpace(model, G.ReferenceFE(G.lagrangian, G.VectorValue[2, G.Float64], 2),
+                      dirichlet_tags="boundary")
+    Q = G.TestFESpace(model, G.ReferenceFE(G.lagrangian, G.Float64, 1), dirichlet_tags="tag_1")
+    U, P = G.TrialFESpace(V, ue), G.TrialFESpace(Q, pe)
+    X = model.node_coordinates()[model.cell_node_ids() - 1]
+    qx, qw = oracle.tensor_quadrature(2, 2)
+    s1, g1 = oracle.qk_tabulate(2, 1, qx)
+    s2, g2 = oracle.qk_tabulate(2, 2, qx)
+    wdet, gp2 = CO.physical_gradients(X, qw, g1, g2)
+    C = X.shape[0]
+    mats = {(0, 0): CO.vector_laplacian_cells(wdet, gp2),
+            (1, 0): CO.divergence_cells(wdet, s1, gp2, scalar_test=True),
+            (0, 1): -CO.divergence_cells(wdet, s1, gp2, scalar_test=False)}
+    xq = CO.q1_map_points(X, s1)
+    vecs = {0: CO.ncube_loads(wdet, s2, np.tile([-5.0, 5.0], (C, 1))),
+            1: np.einsum("cq,cq,qa->ca", wdet, 2.0 * xq[..., 0], s1)}
+    cds = [np.asarray(V.cell_dof_ids), np.asarray(Q.cell_dof_ids)]
+    cp, rv, nz, b = MF.assemble(cds, [V.nfree, Q.nfree], [V.ndirichlet, Q.ndirichlet], mats,
+                                vecs, dvals=[U.dirichlet_values(), P.dirichlet_values()])
+    x = _solve(cp, rv, nz, b, V.nfree + Q.nfree)
+    uI, pI = G.interpolate(ue, U).free_values, G.interpolate(pe, P).free_values
+    assert np.abs(x[:V.nfree] - uI).max() <= 1e-11 * np.abs(uI).max()
+    assert np.abs(x[V.nfree:] - pI).max() <= 1e-10 * np.abs(pI).max()
