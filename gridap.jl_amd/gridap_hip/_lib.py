@@ -286,9 +286,18 @@ class Context:
     def assemble(self, form=GH_FORM_DIFFUSION, precision=GH_PREC_FP64, path=GH_PATH_AUTO):
         self.check(lib().gh_assemble(self.h, form, precision, path))
 
-    def values_get(self, nz=None, b=None):
+    def values_get(self, nz=None, b=None, want_nz=True):
         """nzval and b of the last assembly; into nz / b (contiguous float64 arrays of
-        nnz / nfree entries, written in place) when given, else into new arrays."""
+        nnz / nfree entries, written in place) when given, else into new arrays.
+        want_nz=False copies b only and returns (None, b)."""
+        if not want_nz:
+            if b is not None and (not isinstance(b, np.ndarray) or b.dtype != np.float64 or
+                                  not b.flags.c_contiguous or b.shape != (self.nfree,)):
+                raise ValueError("b must be a contiguous float64 array of %d entries"
+                                 % self.nfree)
+            bo = b if b is not None and self.nfree > 0 else np.zeros(max(self.nfree, 1))
+            self.check(lib().gh_values_get(self.h, None, ptr(bo)))
+            return None, (b if b is not None else bo[:self.nfree])
         for name, a, n in (("nz", nz, self.nnz), ("b", b, self.nfree)):
             if a is not None and (not isinstance(a, np.ndarray) or a.dtype != np.float64 or
                                   not a.flags.c_contiguous or a.shape != (n,)):

@@ -292,6 +292,11 @@ class SparseMatrixAssembler:
             # Gridap's add_entry! fails on an entry missing from the stored pattern
             raise ValueError("A does not have this assembler's sparsity pattern")
         self.ctx.assemble(form, self.precision, self.path)
+        if matcontrib is None and A is None:
+            # a vector-only assembly: the matrix values stay on the device
+            _, bv = self.ctx.values_get(None, b, want_nz=False)
+            self.device_matrix = None
+            return None, bv
         nz, bv = self.ctx.values_get(None if A is None else A.nzval, b)
         if A is None:
             n = self.space.nfree
