@@ -187,7 +187,10 @@ def compose_blocks(sub, row_fields, col_fields, nrows, ncols, plan=None):
     nz = np.empty(len(rv), dtype=np.float64)
     for key, dst in scatter:
         nz[dst] = sub[key].nzval
-    return SparseMatrixCSC(m, n, cp.copy(), rv.copy(), nz)
+    # the pattern arrays are the plan's, shared read-only by every matrix composed on it
+    cp.flags.writeable = False
+    rv.flags.writeable = False
+    return SparseMatrixCSC(m, n, cp, rv, nz)
 
 
 class RectangularPairAssembler:
