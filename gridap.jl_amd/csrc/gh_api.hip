@@ -690,6 +690,73 @@ int gh_values_get(gh_ctx *ctx, double *nzval, double *b) {
   });
 }
 
+int gh_compose_block(gh_ctx *ctx, gh_ctx *src, const int64_t *dst, int64_t n, int64_t nnz) {
+  return guarded(ctx, [&](gh::Ctx &c) {
+    GH_REQUIRE(src && src != ctx, GH_ERR_ARG, "a composite block is another context");
+    GH_REQUIRE(src->c.device == c.device, GH_ERR_ARG, "a composite block is on another device");
+    GH_REQUIRE(src->c.have_pattern, GH_ERR_STATE, "the block has no pattern: call gh_symbolic");
+    GH_REQUIRE(n == src->c.nnz, GH_ERR_ARG, "n is not the block's number of stored values");
+    GH_REQUIRE(n == 0 || dst, GH_ERR_ARG, "NULL dst");
+    GH_REQUIRE(nnz >= 0, GH_ERR_ARG, "negative nnz");
+    GH_REQUIRE(nnz <= INT32_MAX, GH_ERR_UNSUPPORTED, "composite above 2^31 - 1 stored values");
+    GH_REQUIRE(c.comp_nnz < 0 || c.comp_nnz == nnz, GH_ERR_ARG,
+               "nnz differs from the composite's registered blocks");
+    // marked on a copy, kept only when the whole block is valid
+    std::vector<bool> taken = c.comp_nnz < 0 ? std::vector<bool>((size_t)nnz, false)
+                                             : c.comp_taken;
+    std::vector<int32_t> d((size_t)n);
+    for (int64_t k = 0; k < n; ++k) {
+      GH_REQUIRE(dst[k] >= 0 && dst[k] < nnz, GH_ERR_ARG, "composite position out of range");
+      GH_REQUIRE(!taken[(size_t)dst[k]], GH_ERR_ARG,
+                 "composite position written by two stored values");
+      taken[(size_t)dst[k]] = true;
+      d[(size_t)k] = (int32_t)dst[k];
+    }
+    auto blk = std::make_unique<gh::Ctx::ComposeBlock>();
+    blk->src = &src->c;
+    blk->n = n;
+    blk->dst.resize((size_t)n);
+    if (n)
+      GH_CHECK_HIP(hipMemcpyAsync(blk->dst.get(), d.data(), sizeof(int32_t) * n,
+                                  hipMemcpyHostToDevice, c.stream));
+    if (c.comp_nnz < 0) {
+      // positions no block covers read as stored zeros
+      c.comp_nz.resize((size_t)nnz);
+      if (nnz)
+        GH_CHECK_HIP(hipMemsetAsync(c.comp_nz.get(), 0, sizeof(double) * nnz, c.stream));
+    }
+    GH_CHECK_HIP(hipStreamSynchronize(c.stream));
+    c.comp.push_back(std::move(blk));
+    c.comp_taken.swap(taken);
+    c.comp_nnz = nnz;
+  });
+}
+
+int gh_compose_move(gh_ctx *ctx, int block) {
+  return guarded(ctx, [&](gh::Ctx &c) { gh::compose_move(c, block); });
+}
+
+int gh_compose_values_get(gh_ctx *ctx, double *nzval) {
+  return guarded(ctx, [&](gh::Ctx &c) {
+    GH_REQUIRE(c.comp_nnz >= 0, GH_ERR_STATE, "no composite blocks: call gh_compose_block");
+    GH_REQUIRE(nzval || c.comp_nnz == 0, GH_ERR_ARG, "NULL nzval");
+    for (const auto &blk : c.comp) GH_CHECK_HIP(hipStreamSynchronize(blk->src->stream));
+    if (c.comp_nnz)
+      GH_CHECK_HIP(hipMemcpyAsync(nzval, c.comp_nz.get(), sizeof(double) * c.comp_nnz,
+                                  hipMemcpyDeviceToHost, c.stream));
+    GH_CHECK_HIP(hipStreamSynchronize(c.stream));
+  });
+}
+
+int gh_compose_clear(gh_ctx *ctx) {
+  return guarded(ctx, [&](gh::Ctx &c) {
+    c.comp.clear();
+    c.comp_nz.release();
+    c.comp_taken.clear();
+    c.comp_nnz = -1;
+  });
+}
+
 int gh_values_set(gh_ctx *ctx, const double *nzval, const double *b) {
   return guarded(ctx, [&](gh::Ctx &c) {
     GH_REQUIRE(c.have_pattern, GH_ERR_STATE, "no pattern: call gh_symbolic first");

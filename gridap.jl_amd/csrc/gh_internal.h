@@ -5,6 +5,7 @@
 #include <cstdint>
 #include <stdexcept>
 #include <string>
+#include <memory>
 #include <vector>
 
 #include "gridap_hip.h"
@@ -168,6 +169,18 @@ struct Ctx {
   int tnq = 0;                      // points of the trial tabulation
   std::vector<double> tshp, tgrad;  // trial values and reference gradients there
 
+  // ---- composite matrix of a multi-field form (gh_compose_block): the stored values of
+  //      other contexts' matrices, each moved to its composite positions ----
+  struct ComposeBlock {
+    Ctx *src = nullptr;        // the context whose nzval is the block
+    int64_t n = 0;             // its stored values
+    DBuf<int32_t> dst;         // composite position of each, 0-based
+  };
+  std::vector<std::unique_ptr<ComposeBlock>> comp;
+  int64_t comp_nnz = -1;     // composite stored values (-1: none registered)
+  DBuf<double> comp_nz;
+  std::vector<bool> comp_taken;  // composite positions some block writes
+
   // ---- reference tabulation ----
   int nq = 0;
   std::vector<double> qw, shp, grad;
@@ -329,6 +342,9 @@ void hexgen_assemble(Ctx &c, int precision, bool elasticity);
 void hexgen_rhs(Ctx &c);
 void rect_symbolic(Ctx &c);                  // pattern of a rectangular field pair
 void rect_assemble(Ctx &c, int form, int precision);  // its values and lifting loads
+// one composite block's current nzval moved to its positions in c.comp_nz, enqueued on
+// the block's stream
+void compose_move(Ctx &c, int block);
 void elemlist_symbolic(Ctx &c);         // dof -> cell adjacency, Dirichlet masks
 void elemlist_sum(Ctx &c, bool f32);     // nzval from the element entries
 void assemble_simplex(Ctx &c, int precision);

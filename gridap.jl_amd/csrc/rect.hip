@@ -259,7 +259,32 @@ __global__ void rect_rhs_kernel(int64_t nfree, int ns, const int32_t *__restrict
   b[i] = s;
 }
 
+// out[dst[k]] = v[k]: one block's stored values to their composite positions.  A block
+// column's values are one consecutive run of composite positions, so both sides of the
+// move stay coalesced.
+__global__ void compose_move_kernel(int64_t n, const double *__restrict__ v,
+                                    const int32_t *__restrict__ dst, double *__restrict__ out) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride)
+    out[dst[k]] = v[k];
+}
+
 }  // namespace
+
+void compose_move(Ctx &c, int block) {
+  GH_REQUIRE(block >= 0 && block < (int)c.comp.size(), GH_ERR_ARG, "no such composite block");
+  const auto &blk = c.comp[(size_t)block];
+  Ctx &s = *blk->src;
+  GH_REQUIRE(s.have_values, GH_ERR_STATE, "the composite block holds no assembled values");
+  GH_REQUIRE(s.have_pattern && s.nnz == blk->n, GH_ERR_STATE,
+             "the composite block's pattern changed since it was registered");
+  if (blk->n == 0) return;
+  // on the block's stream: after the assembly that wrote its values, before the next
+  const int64_t blocks = std::min<int64_t>((blk->n + 255) / 256, 8192);
+  compose_move_kernel<<<(unsigned)blocks, 256, 0, s.stream>>>(blk->n, s.nzval.get(),
+                                                             blk->dst.get(), c.comp_nz.get());
+  GH_CHECK_HIP(hipGetLastError());
+}
 
 void rect_symbolic(Ctx &c) {
   hipStream_t s = c.stream;

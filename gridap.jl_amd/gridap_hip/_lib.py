@@ -83,6 +83,10 @@ SIGNATURES = {
     "gh_source_qp_upload": (_i32, [_vp, _vp]),
     "gh_values_get": (_i32, [_vp, _vp, _vp]),
     "gh_values_set": (_i32, [_vp, _vp, _vp]),
+    "gh_compose_block": (_i32, [_vp, _vp, _vp, _i64, _i64]),
+    "gh_compose_move": (_i32, [_vp, _i32]),
+    "gh_compose_values_get": (_i32, [_vp, _vp]),
+    "gh_compose_clear": (_i32, [_vp]),
     "gh_fefunction_upload": (_i32, [_vp, _vp, _vp]),
     "gh_plaplacian_params": (_i32, [_vp, _dbl]),
     "gh_pattern_info_get": (_i32, [_vp, C.POINTER(PatternInfo)]),
@@ -306,6 +310,38 @@ class Context:
         bo = b if b is not None and self.nfree > 0 else np.zeros(max(self.nfree, 1))
         self.check(lib().gh_values_get(self.h, ptr(nzo), ptr(bo)))
         return (nz if nz is not None else nzo[:self.nnz]), (b if b is not None else bo[:self.nfree])
+
+    def compose_block(self, src, dst, nnz):
+        """Register src's matrix (a Context with its pattern built) as a block of this
+        context's composite of nnz stored values: its k-th stored value goes to
+        composite position dst[k] (0-based)."""
+        d = np.ascontiguousarray(dst, dtype=np.int64)
+        if d.shape != (src.nnz,):
+            raise ValueError("dst must have one entry per stored value of the block")
+        self.check(lib().gh_compose_block(self.h, src.h, ptr(d) if d.size else None,
+                                          d.size, int(nnz)))
+        self._comp_nnz = int(nnz)
+
+    def compose_move(self, block):
+        """Move block's current assembled values to their composite positions (on the
+        device, ordered with the block context's own work)."""
+        self.check(lib().gh_compose_move(self.h, int(block)))
+
+    def compose_values_get(self, nz=None):
+        """The composite's values, once every move enqueued so far has finished."""
+        n = getattr(self, "_comp_nnz", None)
+        if n is None:
+            raise RuntimeError("no composite blocks registered")
+        if nz is not None and (not isinstance(nz, np.ndarray) or nz.dtype != np.float64 or
+                               not nz.flags.c_contiguous or nz.shape != (n,)):
+            raise ValueError("nz must be a contiguous float64 array of %d entries" % n)
+        out = nz if nz is not None and n > 0 else np.empty(max(n, 1))
+        self.check(lib().gh_compose_values_get(self.h, ptr(out)))
+        return nz if nz is not None else out[:n]
+
+    def compose_clear(self):
+        self.check(lib().gh_compose_clear(self.h))
+        self._comp_nnz = None
 
     def fefunction_upload(self, free_vals, dirichlet_vals=None):
         """The FE function uh of a nonlinear form (free and Dirichlet values)."""

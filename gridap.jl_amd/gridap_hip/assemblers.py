@@ -259,12 +259,14 @@ class SparseMatrixAssembler:
                                  bnd["gq"], bnd["hq"])
         self._has_boundary = True
 
-    def assemble(self, matcontrib, veccontrib, lift, A=None, b=None, trial=None):
+    def assemble(self, matcontrib, veccontrib, lift, A=None, b=None, trial=None,
+                 copy_values=True):
         """Device numeric phase; returns (SparseMatrixCSC, b).  With A (a matrix on this
         assembler's pattern) and/or b given, their values are overwritten in place and
         they are returned (fillstored! then the numeric loop,
         src/FESpaces/SparseMatrixAssemblers.jl:60-63).  trial: the space whose Dirichlet
-        values are lifted (default: the assembler's U)."""
+        values are lifted (default: the assembler's U).  copy_values=False leaves the
+        matrix values on the device and returns (None, b)."""
         form, kappa, fsrc, degree, extra = self._lower(matcontrib, veccontrib)
         self._setup(degree)
         dvals = None
@@ -292,7 +294,7 @@ class SparseMatrixAssembler:
             # Gridap's add_entry! fails on an entry missing from the stored pattern
             raise ValueError("A does not have this assembler's sparsity pattern")
         self.ctx.assemble(form, self.precision, self.path)
-        if matcontrib is None and A is None:
+        if (matcontrib is None or not copy_values) and A is None:
             # a vector-only assembly: the matrix values stay on the device
             _, bv = self.ctx.values_get(None, b, want_nz=False)
             self.device_matrix = None

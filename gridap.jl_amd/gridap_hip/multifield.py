@@ -247,7 +247,7 @@ class RectangularPairAssembler:
                 self.ctx.geometry_tabulation(ggrad)
             self._degree = degree
 
-    def assemble(self, matcontrib, veccontrib, lift, trial=None):
+    def assemble(self, matcontrib, veccontrib, lift, trial=None, copy_values=True):
         from . import _lib
         from .assemblers import SparseMatrixCSC
         from .celldata import DiffusionTerm, DivergenceTerm, MassTerm, _cell_coef
@@ -291,6 +291,8 @@ class RectangularPairAssembler:
         self.ctx.coef_upload(kappa, None, dvals)
         self.ctx.mass_upload(sigma)
         self.ctx.assemble(form, self.precision, _lib.GH_PATH_GENERAL)
+        if not copy_values:
+            return self.ctx.values_get(None, None, want_nz=False)
         nz, b = self.ctx.values_get()
         A = SparseMatrixCSC(Vs.nfree, Us.nfree, self.colptr.copy(), self.rowval.copy(), nz)
         return A, b
@@ -312,13 +314,42 @@ class MultiFieldSparseMatrixAssembler:
         self._assems = []   # (test space, single-field assembler)
         self._rects = []    # ((test space, trial space), rectangular pair assembler)
         self._plans = {}    # compose_plan per (row fields, column fields, touched pairs)
+        # per plan key: (context holding the composite on the device, {pair: block})
+        self._composers = {}
+
+    @staticmethod
+    def _key(pairs, rg, cg):
+        return (tuple(rg), tuple(cg), tuple(sorted(k for k in pairs if k[0] in rg and k[1] in cg)))
 
     def _compose(self, sub, rg, cg, nrows, ncols):
         # the pair patterns are the fixed patterns of the single-field assemblers
-        key = (tuple(rg), tuple(cg), tuple(sorted(k for k in sub if k[0] in rg and k[1] in cg)))
+        key = self._key(sub, rg, cg)
         if key not in self._plans:
             self._plans[key] = compose_plan(sub, rg, cg, nrows, ncols)
         return compose_blocks(sub, rg, cg, nrows, ncols, plan=self._plans[key])
+
+    def _register(self, key):
+        """Compose the plan of key on the device from now on: one context holds the
+        composite values and every touched pair's assembled values move there."""
+        from ._lib import Context
+        _, _, _, rv, scatter = self._plans[key]
+        if not scatter or key in self._composers:
+            return
+        ctx = Context(self.device)
+        blocks = {}
+        for idx, (pair, dst) in enumerate(scatter):
+            ctx.compose_block(self._pair(*pair).ctx, dst, len(rv))
+            blocks[pair] = idx
+        self._composers[key] = (ctx, blocks)
+
+    def _compose_device(self, key):
+        from .assemblers import SparseMatrixCSC
+        m, n, cp, rv, _ = self._plans[key]
+        nz = self._composers[key][0].compose_values_get() if key in self._composers \
+            else np.empty(0)
+        cp.flags.writeable = False
+        rv.flags.writeable = False
+        return SparseMatrixCSC(m, n, cp, rv, nz)
 
     def get_rows(self):
         return np.arange(1, self.V.num_free_dofs() + 1)
@@ -364,12 +395,25 @@ class MultiFieldSparseMatrixAssembler:
             raise NotImplementedError("in-place multi-field assembly is not provided")
         mats = self._split(matcontrib, 2)
         vecs = self._split(veccontrib, 1)
-        nf = len(self.V)
         sub = {}
         rhs = [np.zeros(V.nfree) for V in self.V.spaces]
+        groups, cgroups = self.V.groups(), self.U.groups()
+        keys = {(tuple(rg), tuple(cg)): self._key(mats, rg, cg) for rg in groups for cg in cgroups}
+        # composed on the device once every touched plan has been composed on the host
+        # (the first assembly of a form on this assembler builds the pair patterns)
+        ondev = matcontrib is not None and all(
+            k in self._composers or (k in self._plans and not self._plans[k][4])
+            for k in keys.values())
         for (i, j) in sorted(mats):
-            Aij, lij = self._pair(i, j).assemble(mats[(i, j)], None, lift=lift,
-                                                 trial=self.U.spaces[j])
+            pa = self._pair(i, j)
+            Aij, lij = pa.assemble(mats[(i, j)], None, lift=lift, trial=self.U.spaces[j],
+                                   copy_values=not ondev)
+            if ondev:
+                # move the pair's values now: a later pair may share its assembler
+                for k in keys.values():
+                    if (i, j) in k[2]:
+                        ctx, blocks = self._composers[k]
+                        ctx.compose_move(blocks[(i, j)])
             sub[(i, j)] = Aij
             if lift:
                 rhs[i] = rhs[i] + lij
@@ -378,11 +422,14 @@ class MultiFieldSparseMatrixAssembler:
             rhs[i] = rhs[i] + bi
         nrows = [V.nfree for V in self.V.spaces]
         ncols = [U.nfree for U in self.U.spaces]
-        groups = self.V.groups()
         mat = vec = None
         if matcontrib is not None:
-            blocks = [[self._compose(sub, rg, cg, nrows, ncols) for cg in self.U.groups()]
+            blocks = [[self._compose_device(keys[(tuple(rg), tuple(cg))]) if ondev else
+                       self._compose(sub, rg, cg, nrows, ncols) for cg in cgroups]
                       for rg in groups]
+            if not ondev:
+                for k in keys.values():
+                    self._register(k)
             mat = blocks[0][0] if not isinstance(self.V.style, BlockMultiFieldStyle) \
                 else BlockSparseMatrix(blocks)
         if veccontrib is not None or lift:

@@ -246,6 +246,25 @@ int gh_plaplacian_params(gh_ctx *ctx, double p);
 /* Copy the assembled nzval[nnz] and b[nfree] to the host (either may be NULL). */
 int gh_values_get(gh_ctx *ctx, double *nzval, double *b);
 
+/* Composite matrix of a multi-field form on the device: the matrix that
+ * src/MultiField/BlockSparseMatrixAssemblers.jl:19-36 (and the consecutive multi-field
+ * layout of src/MultiField/MultiFieldFESpaces.jl) builds from the field-pair blocks.
+ * ctx holds the composite's nnz values; gh_compose_block registers another context
+ * src (same device, pattern built) whose k-th stored value goes to composite position
+ * dst[k] (0-based, k < n = src's nnz).  Positions must be distinct across all blocks;
+ * those no block covers read as stored zeros.  nnz <= 2^31 - 1.  src must stay alive
+ * while it is registered.
+ * Blocks are numbered 0, 1, ... in registration order.  gh_compose_move(block) moves
+ * the block's current assembled values to their positions on the device, ordered
+ * after the work its context has enqueued and before what it enqueues next (so one
+ * context may serve several blocks, reassembled between their moves).
+ * gh_compose_values_get waits for the moves and copies the nnz composite values to the
+ * host.  gh_compose_clear drops the blocks. */
+int gh_compose_block(gh_ctx *ctx, gh_ctx *src, const int64_t *dst, int64_t n, int64_t nnz);
+int gh_compose_move(gh_ctx *ctx, int block);
+int gh_compose_values_get(gh_ctx *ctx, double *nzval);
+int gh_compose_clear(gh_ctx *ctx);
+
 /* Load a matrix with this context's pattern (nzval[nnz], the stored values of a
  * SparseMatrixCSC whose colptr/rowval are gh_pattern_get's) and/or a right-hand side
  * b[nfree] into the device, in place of the assembled ones; either may be NULL.  The

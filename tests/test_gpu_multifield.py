@@ -278,7 +278,7 @@ def test_mixed_order_fields_match_oracle(oracle, part, mapped):
     np.testing.assert_array_equal(np.concatenate(bb.blocks), b)
 
 
-def _stokes(part, precision="fp64"):
+def _stokes(part, precision="fp64", calls=1):
     """test/GridapTests/StokesTaylorHoodTests.jl's Taylor–Hood form on (0,2)² with
     u = (x1² + 2 x2², -x1²) and p = x1 + 3 x2 (f = -Δu + ∇p = (-5, 5), g = ∇⋅u = 2 x1),
     velocity Dirichlet data on the whole boundary and the pressure fixed at the vertex
@@ -296,9 +296,10 @@ def _stokes(part, precision="fp64"):
     a = lambda u, v: G.integrate(G.inner(G.gradient(v[0]), G.gradient(u[0])) -
                                  G.divergence(v[0]) * u[1] + v[1] * G.divergence(u[0]), dΩ)
     l = lambda v: G.integrate(G.dot(v[0], G.VectorValue(-5.0, 5.0)) + v[1] * g, dΩ)
-    A, b = G.assemble_matrix_and_vector(a, l, X, Y,
-                                        G.SparseMatrixAssembler(X, Y, precision=precision))
-    return A, b, (model, V, Q, U, P, ue, pe)
+    assem = G.SparseMatrixAssembler(X, Y, precision=precision)
+    runs = [G.assemble_matrix_and_vector(a, l, X, Y, assem) for _ in range(calls)]
+    A, b = runs[0]
+    return A, b, (model, V, Q, U, P, ue, pe) + ((runs, assem) if calls > 1 else ())
 
 
 def _stokes_oracle(oracle, model, V, Q, U, P):
@@ -460,3 +461,74 @@ def test_stokes_taylor_hood_3d(oracle, mapped):
     uI, pI = G.interpolate(ue, U).free_values, G.interpolate(pe, P).free_values
     assert np.max(np.abs(x[:V.nfree] - uI)) <= 1e-9 * np.max(np.abs(uI))
     assert np.max(np.abs(x[V.nfree:] - pI)) <= 1e-9 * np.max(np.abs(pI))
+
+
+def biform3b(u, v, dΩ):
+    (u1, u2, u3), (v1, v2, v3) = u, v
+    return G.integrate(G.inner(G.gradient(u1), G.gradient(v1)) + G.inner(u2, v2) +
+                       G.inner(u1, v2) - G.inner(u3, v2) + G.inner(u2, v3), dΩ)
+
+
+def _matrices(A):
+    return [B for row in A.blocks for B in row] if isinstance(A, G.BlockSparseMatrix) else [A]
+
+
+@pytest.mark.parametrize("blocked", [False, True])
+def test_device_composition_equals_host(oracle, blocked):
+    """The first assembly of a form on a multi-field assembler composes the field-pair
+    values on the host and registers the composition on the device; later assemblies
+    move each pair's values into the composite on the device, right after the pair's
+    own assembly (the five pairs of the three-field form share one assembler, so each
+    move precedes the next pair's reassembly).  Bitwise equal to the host composition;
+    a changed form on the same pairs (first call of its own, already on the device)
+    equals a fresh assembler's host composition bitwise and the oracle within 1e-12."""
+    announce("device composition, three fields, blocked=%s" % blocked)
+    V, U, dΩ = spaces(3)
+    if blocked:
+        st = G.BlockMultiFieldStyle(2, (1, 2))
+        X, Y = G.MultiFieldFESpace([U] * 3, style=st), G.MultiFieldFESpace([V] * 3, style=st)
+    else:
+        X, Y = G.MultiFieldFESpace([U] * 3), G.MultiFieldFESpace([V] * 3)
+    a = lambda u, v: biform3(u, v, dΩ)
+    ab = lambda u, v: biform3b(u, v, dΩ)
+    l = lambda v: liform3(v, dΩ)
+    assem = G.SparseMatrixAssembler(X, Y)
+    A1, b1 = G.assemble_matrix_and_vector(a, l, X, Y, assem)
+    assert assem._composers, "the first assembly registers the device composition"
+    A2, b2 = G.assemble_matrix_and_vector(a, l, X, Y, assem)
+    for B1, B2 in zip(_matrices(A1), _matrices(A2)):
+        np.testing.assert_array_equal(B2.colptr, B1.colptr)
+        np.testing.assert_array_equal(B2.rowval, B1.rowval)
+        np.testing.assert_array_equal(B2.nzval, B1.nzval)
+    Ab, _ = G.assemble_matrix_and_vector(ab, l, X, Y, assem)
+    Ar, _ = G.assemble_matrix_and_vector(ab, l, X, Y, G.SparseMatrixAssembler(X, Y))
+    for B1, B2 in zip(_matrices(Ar), _matrices(Ab)):
+        np.testing.assert_array_equal(B2.nzval, B1.nzval)
+    if not blocked:
+        pairs = dict(PAIRS3)
+        pairs[(1, 0)] = ("mass", 1.0)
+        pairs[(2, 1)] = ("mass", 1.0)
+        _, (cp, rv, nz, _) = oracle_system(3, pairs, VECS3)
+        np.testing.assert_array_equal(Ab.rowval, rv)
+        assert relmax(Ab.nzval, nz) <= 1e-12
+    # the matrix-free call in between leaves the composition in place
+    G.assemble_vector(l, assem, Y)
+    A3, _ = G.assemble_matrix_and_vector(a, l, X, Y, assem)
+    for B1, B3 in zip(_matrices(A1), _matrices(A3)):
+        np.testing.assert_array_equal(B3.nzval, B1.nzval)
+
+
+def test_stokes_device_composition_equals_host(oracle):
+    """Taylor–Hood Stokes assembled three times on one assembler: the first call composes
+    on the host, the later ones move the vector Laplacian and both rectangular coupling
+    blocks into the composite on the device; matrices and vectors bitwise equal."""
+    announce("Stokes device composition (8, 6)")
+    _, _, env = _stokes((8, 6), calls=3)
+    runs, assem = env[-2], env[-1]
+    assert assem._composers
+    A1, b1 = runs[0]
+    for A, b in runs[1:]:
+        np.testing.assert_array_equal(A.colptr, A1.colptr)
+        np.testing.assert_array_equal(A.rowval, A1.rowval)
+        np.testing.assert_array_equal(A.nzval, A1.nzval)
+        np.testing.assert_array_equal(b, b1)
