@@ -342,11 +342,18 @@ class MultiFieldSparseMatrixAssembler:
             blocks[pair] = idx
         self._composers[key] = (ctx, blocks)
 
-    def _compose_device(self, key):
+    def _compose_device(self, key, T=None):
+        """The composite of key from the device, into T's stored values when given."""
         from .assemblers import SparseMatrixCSC
         m, n, cp, rv, _ = self._plans[key]
-        nz = self._composers[key][0].compose_values_get() if key in self._composers \
-            else np.empty(0)
+        if T is not None:
+            self._check_target(T, m, n, cp, rv)
+        if key in self._composers:
+            nz = self._composers[key][0].compose_values_get(None if T is None else T.nzval)
+        else:
+            nz = np.empty(0)
+        if T is not None:
+            return T
         cp.flags.writeable = False
         rv.flags.writeable = False
         return SparseMatrixCSC(m, n, cp, rv, nz)
@@ -390,9 +397,36 @@ class MultiFieldSparseMatrixAssembler:
             out.setdefault(tuple(f), []).append((term, meas))
         return {k: DomainContribution(v) for k, v in out.items()}
 
+    def _targets(self, A, nr, nc):
+        """The blocks of an in-place matrix A, checked against this assembler's layout."""
+        if isinstance(self.V.style, BlockMultiFieldStyle):
+            if not (isinstance(A, BlockSparseMatrix) and len(A.blocks) == nr and
+                    all(len(row) == nc for row in A.blocks)):
+                raise ValueError("A must be a BlockSparseMatrix of %d x %d blocks" % (nr, nc))
+            return A.blocks
+        from .assemblers import SparseMatrixCSC
+        if not isinstance(A, SparseMatrixCSC):
+            raise ValueError("A must be a SparseMatrixCSC")
+        return [[A]]
+
+    @staticmethod
+    def _check_target(T, m, n, cp, rv):
+        # Gridap's add_entry! fails on an entry missing from the stored pattern
+        if not (T.m == m and T.n == n and (T.colptr is cp or np.array_equal(T.colptr, cp)) and
+                (T.rowval is rv or np.array_equal(T.rowval, rv))):
+            raise ValueError("A does not have this assembler's sparsity pattern")
+        if not (isinstance(T.nzval, np.ndarray) and T.nzval.dtype == np.float64 and
+                T.nzval.flags.c_contiguous and T.nzval.flags.writeable):
+            raise ValueError("A's stored values must be a writeable contiguous float64 array")
+
     def assemble(self, matcontrib, veccontrib, lift, A=None, b=None):
-        if A is not None or b is not None:
-            raise NotImplementedError("in-place multi-field assembly is not provided")
+        """Returns (matrix, vector) in the layout of the spaces' style.  With A and/or b
+        given (a matrix of this assembler's layout from an earlier assembly, a vector of
+        num_free_dofs entries or a BlockVector), their values are overwritten in place
+        and they are returned (assemble_matrix_and_vector!, src/FESpaces/Assemblers.jl:
+        330-346)."""
+        if A is not None and matcontrib is None:
+            raise ValueError("an in-place matrix needs a bilinear form")
         mats = self._split(matcontrib, 2)
         vecs = self._split(veccontrib, 1)
         sub = {}
@@ -423,17 +457,39 @@ class MultiFieldSparseMatrixAssembler:
         nrows = [V.nfree for V in self.V.spaces]
         ncols = [U.nfree for U in self.U.spaces]
         mat = vec = None
+        blocked = isinstance(self.V.style, BlockMultiFieldStyle)
         if matcontrib is not None:
-            blocks = [[self._compose_device(keys[(tuple(rg), tuple(cg))]) if ondev else
-                       self._compose(sub, rg, cg, nrows, ncols) for cg in cgroups]
-                      for rg in groups]
+            targets = None if A is None else self._targets(A, len(groups), len(cgroups))
+            blocks = []
+            for r, rg in enumerate(groups):
+                row = []
+                for c, cg in enumerate(cgroups):
+                    T = None if targets is None else targets[r][c]
+                    if ondev:
+                        B = self._compose_device(keys[(tuple(rg), tuple(cg))], T)
+                    else:
+                        B = self._compose(sub, rg, cg, nrows, ncols)
+                        if T is not None:
+                            self._check_target(T, B.m, B.n, B.colptr, B.rowval)
+                            T.nzval[...] = B.nzval
+                            B = T
+                    row.append(B)
+                blocks.append(row)
             if not ondev:
                 for k in keys.values():
                     self._register(k)
-            mat = blocks[0][0] if not isinstance(self.V.style, BlockMultiFieldStyle) \
-                else BlockSparseMatrix(blocks)
+            mat = A if A is not None else (BlockSparseMatrix(blocks) if blocked else blocks[0][0])
         if veccontrib is not None or lift:
             parts = [np.concatenate([rhs[i] for i in g]) if g else np.zeros(0) for g in groups]
-            vec = parts[0] if not isinstance(self.V.style, BlockMultiFieldStyle) \
-                else BlockVector(parts)
+            if b is not None:
+                outs = b.blocks if blocked and isinstance(b, BlockVector) else [b]
+                if len(outs) != len(parts) or any(
+                        not isinstance(o, np.ndarray) or o.shape != p.shape for o, p in
+                        zip(outs, parts)):
+                    raise ValueError("b does not have this assembler's layout")
+                for o, p in zip(outs, parts):
+                    o[...] = p
+                vec = b
+            else:
+                vec = BlockVector(parts) if blocked else parts[0]
         return mat, vec

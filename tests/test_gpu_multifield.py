@@ -532,3 +532,41 @@ def test_stokes_device_composition_equals_host(oracle):
         np.testing.assert_array_equal(A.rowval, A1.rowval)
         np.testing.assert_array_equal(A.nzval, A1.nzval)
         np.testing.assert_array_equal(b, b1)
+
+
+@pytest.mark.parametrize("blocked", [False, True])
+def test_multifield_inplace_equals_fresh(blocked):
+    """assemble_matrix_and_vector! on a multi-field assembler: A's stored values and b
+    are overwritten in place (the device composition copies straight into A's values),
+    bitwise equal to a fresh assembler's result; the first in-place call of a fresh
+    assembler (host composition) too; a matrix of another layout is rejected."""
+    announce("multi-field in place, three fields, blocked=%s" % blocked)
+    V, U, dΩ = spaces(3)
+    if blocked:
+        st = G.BlockMultiFieldStyle(2, (1, 2))
+        X, Y = G.MultiFieldFESpace([U] * 3, style=st), G.MultiFieldFESpace([V] * 3, style=st)
+    else:
+        X, Y = G.MultiFieldFESpace([U] * 3), G.MultiFieldFESpace([V] * 3)
+    a = lambda u, v: biform3(u, v, dΩ)
+    ab = lambda u, v: biform3b(u, v, dΩ)
+    l = lambda v: liform3(v, dΩ)
+    arr = lambda x: x.to_array() if isinstance(x, G.BlockVector) else x
+    assem = G.SparseMatrixAssembler(X, Y)
+    A, b = G.assemble_matrix_and_vector(a, l, X, Y, assem)
+    Ar, br = G.assemble_matrix_and_vector(ab, l, X, Y, G.SparseMatrixAssembler(X, Y))
+    nzs = [B.nzval for B in _matrices(A)]
+    A2, b2 = G.assemble_matrix_and_vector_inplace(ab, l, A, b, assem)
+    assert A2 is A and b2 is b
+    for B, Bref, nz in zip(_matrices(A), _matrices(Ar), nzs):
+        assert B.nzval is nz
+        np.testing.assert_array_equal(B.nzval, Bref.nzval)
+    np.testing.assert_array_equal(arr(b), arr(br))
+    A1, _ = G.assemble_matrix_and_vector(a, l, X, Y, G.SparseMatrixAssembler(X, Y))
+    assert G.assemble_matrix_inplace(a, A, G.SparseMatrixAssembler(X, Y)) is A
+    for B, Bref in zip(_matrices(A), _matrices(A1)):
+        np.testing.assert_array_equal(B.nzval, Bref.nzval)
+    X2, Y2 = G.MultiFieldFESpace([U] * 2), G.MultiFieldFESpace([V] * 2)
+    Aother = G.assemble_matrix(lambda u, v: biform2(u, v, dΩ), G.SparseMatrixAssembler(X2, Y2),
+                               X2, Y2)
+    with pytest.raises(ValueError):
+        G.assemble_matrix_inplace(a, Aother, assem)
